@@ -1,0 +1,305 @@
+/*
+ * rheon_gpu.h -- C ABI of rheon_amd: Rheon's fixed-box MAC projection step on
+ * one AMD Instinct MI355X (gfx950).
+ *
+ * The boundary mirrors the reference's pressure-provider seam
+ * (src/solver_provider.rs:43-76, PressureRequest / PressureProvider /
+ * ProviderRejection), its host qualification + fallback workspace
+ * (src/solver_provider.rs:94-327, ProviderWorkspace), the pressure workspace
+ * (src/pressure.rs:243-740, PressureWorkspace::solve_rhs/solve_velocity) and
+ * the per-step operator calls of the carrier simulation
+ * (src/simulation.rs:926-1277: advect_velocity, build_rhs, solve,
+ * correct_candidate_in_place, actual_divergence_max, gates).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Arrays are x-fastest flat arrays exactly
+ *    as in the reference (cell i + nx*(j + ny*k); face arrays of axis a have
+ *    counts n + e_a, src/geometry.rs:250-258).
+ *  - Functions taking `d_` pointers expect device memory from rh_device_alloc
+ *    (or any hipMalloc'd memory on the current device); all others take host
+ *    memory.
+ *  - Every entry point is synchronous with respect to the caller (results are
+ *    complete on return), never throws, and reports failures as rh_status.
+ *    rh_last_error() returns a thread-local message for the last failure.
+ *  - Cancellation callbacks are polled only on the calling thread, between
+ *    device batches (the reference polls once per iteration,
+ *    src/pressure.rs:652-656; see rh_pressure_settings.poll_interval).
+ *  - Not thread-safe per object: one thread at a time may use a given
+ *    solver, provider, workspace or scene.
+ */
+#ifndef RHEON_GPU_H
+#define RHEON_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RH_ABI_VERSION 1
+
+/* Status codes.  0-6 mirror ProviderRejection (src/solver_provider.rs:54-62);
+ * 7-19 mirror PressureError / OperatorError / SimulationError variants
+ * (src/pressure.rs:51-77, src/operator.rs:6-17, src/simulation.rs gates). */
+typedef enum rh_status {
+  RH_OK = 0,
+  RH_UNSUPPORTED = 1,          /* ProviderRejection::Unsupported */
+  RH_EXECUTION = 2,            /* ProviderRejection::Execution */
+  RH_NONFINITE_OUTPUT = 3,     /* ProviderRejection::NonFiniteOutput */
+  RH_NONZERO_GAUGE = 4,        /* ProviderRejection::NonZeroGauge */
+  RH_QUALITY_GATE = 5,         /* ProviderRejection::QualityGate */
+  RH_CANCELLED = 6,            /* ProviderRejection::Cancelled / PressureError::Cancelled */
+  RH_INVALID_ARGUMENT = 7,     /* null pointer, length mismatch, bad enum */
+  RH_INVALID_SETTINGS = 8,     /* PressureError::InvalidSettings */
+  RH_INCOMPATIBLE_RHS = 9,     /* PressureError::IncompatibleRhs */
+  RH_BREAKDOWN = 10,           /* PressureError::Breakdown */
+  RH_ITERATION_LIMIT = 11,     /* PressureError::IterationLimit */
+  RH_NONFINITE_INPUT = 12,     /* OperatorError::NonFiniteInput / SamplingError::NonFiniteField */
+  RH_NONZERO_WALL_VELOCITY = 13, /* OperatorError::NonZeroWallVelocity */
+  RH_ARITHMETIC_FAILURE = 14,  /* OperatorError/SamplingError::ArithmeticFailure */
+  RH_INVALID_TIME_STEP = 15,   /* OperatorError::InvalidTimeStep */
+  RH_INVALID_DENSITY = 16,     /* OperatorError::InvalidDensity */
+  RH_DIVERGENCE_LIMIT = 18,    /* SimulationError::DivergenceLimit */
+  RH_COURANT_LIMIT = 19,       /* SimulationError::CourantLimit */
+  RH_GEOMETRY = 20,            /* GeometryError::* */
+  RH_BUFFER_LIMIT = 21,        /* PressureError::BufferLimit / GeometryError::BufferLimit */
+  RH_DEVICE_ERROR = 22         /* HIP runtime failure (message in rh_last_error) */
+} rh_status;
+
+/* GridGeometry::new inputs (src/geometry.rs:114-118). */
+typedef struct rh_grid {
+  uint64_t counts[3];
+  double spacing[3];
+  double origin[3];
+} rh_grid;
+
+/* Storage precision of face and pressure arrays. */
+typedef enum rh_precision { RH_F64 = 0, RH_F32 = 1 } rh_precision;
+
+/* Pressure methods.  JACOBI_PCG is the reference's "jacobi-pcg-v1"
+ * (src/pressure.rs:6-29), iterate-for-iterate.  RBSGS_PCG is PCG with a
+ * symmetric red-black Gauss-Seidel preconditioner (replaces the
+ * lexicographic "sgs-pcg-v1"; different preconditioner, same operator,
+ * gauge and stopping rules).  MG_PCG and MIXED_MG are the multigrid solvers. */
+typedef enum rh_method {
+  RH_JACOBI_PCG = 0,
+  RH_RBSGS_PCG = 1,
+  RH_MG_PCG = 2,
+  RH_MIXED_MG = 3
+} rh_method;
+
+/* PressureSettings (src/pressure.rs:33-39) + device polling cadence. */
+typedef struct rh_pressure_settings {
+  double relative_residual;
+  double absolute_residual;
+  double divergence_limit;
+  int64_t max_iterations;
+  int32_t poll_interval;   /* iterations between host polls; 0 = default (16) */
+  int32_t reserved;
+} rh_pressure_settings;
+
+/* PressureReport (src/pressure.rs:43-49) + error payload. */
+typedef struct rh_pressure_report {
+  int64_t iterations;
+  double true_residual_l2;
+  double true_residual_max;
+  double predicted_divergence_max;
+  int64_t error_iteration;   /* Breakdown/IterationLimit/Cancelled iteration */
+  double error_value;        /* IncompatibleRhs sum, IterationLimit residual_max */
+  double threshold;          /* max(absolute, relative*||b||) actually used */
+  double seconds;            /* device wall time of the solve */
+} rh_pressure_report;
+
+typedef int (*rh_cancel_fn)(void* ctx); /* nonzero = cancel */
+
+/* ---------------------------------------------------------------- basics */
+int32_t rh_abi_version(void);
+const char* rh_last_error(void);
+/* Number of visible devices and the gfx arch name of device 0 ("gfx950"). */
+rh_status rh_device_info(int32_t* count, char* arch, size_t arch_len);
+rh_status rh_device_alloc(size_t bytes, void** out);
+rh_status rh_device_free(void* ptr);
+rh_status rh_copy_to_device(void* dst, const void* src, size_t bytes);
+rh_status rh_copy_to_host(void* dst, const void* src, size_t bytes);
+rh_status rh_device_synchronize(void);
+/* GridGeometry::new validation (src/geometry.rs:114-191); cells may be NULL. */
+rh_status rh_grid_validate(const rh_grid* grid, uint64_t* cells);
+/* BufferPlan::for_grid (src/geometry.rs:284-315): reference f32 payload. */
+rh_status rh_buffer_plan(const rh_grid* grid, uint64_t* velocity_pair, uint64_t* tracer_pair,
+                         uint64_t* pressure_workspace, uint64_t* total);
+
+/* ------------------------------------------------ operator kernels (device) */
+/* PressureOperator::apply_full, box path (src/operator.rs:378-456). f64. */
+rh_status rh_apply_full(const rh_grid* grid, double density, const double* d_x, double* d_y);
+/* advect_velocity (src/advection.rs:28-68): faces in `precision` storage. */
+rh_status rh_advect_velocity(const rh_grid* grid, rh_precision precision, const void* d_u,
+                             const void* d_v, const void* d_w, double dt, void* d_out_u,
+                             void* d_out_v, void* d_out_w);
+/* advect_tracer (src/advection.rs:72-102), f32 storage. */
+rh_status rh_advect_tracer(const rh_grid* grid, const float* d_old, const float* d_u,
+                           const float* d_v, const float* d_w, double dt, float* d_out);
+/* build_rhs / build_rhs_f64 (src/operator.rs:460-509, :672-708), validating
+ * finiteness and zero wall faces (:850-875). Output f64. */
+rh_status rh_build_rhs(const rh_grid* grid, rh_precision precision, const void* d_u,
+                       const void* d_v, const void* d_w, double dt, double* d_rhs);
+/* correct_candidate_in_place / correct_candidate_f64 (src/operator.rs:627-668,
+ * :746-781). Pressure f64. */
+rh_status rh_correct_velocity(const rh_grid* grid, rh_precision precision, double density,
+                              const double* d_pressure, double dt, void* d_u, void* d_v,
+                              void* d_w);
+/* actual_divergence_max (src/pressure.rs:330-342 with src/operator.rs:513-568). */
+rh_status rh_divergence_max(const rh_grid* grid, rh_precision precision, const void* d_u,
+                            const void* d_v, const void* d_w, double* out_max);
+/* velocity_rate (src/simulation.rs:1415-1429): sum_d max|u_d| / h_d. */
+rh_status rh_velocity_rate(const rh_grid* grid, rh_precision precision, const void* d_u,
+                           const void* d_v, const void* d_w, double* out_rate);
+/* One preconditioner application z = M^-1 r on the gauge-eliminated system
+ * (precondition_jacobi src/pressure.rs:534-575; RBSGS: red-black symmetric
+ * Gauss-Seidel).  r[0] is zeroed as in the reference; returns dot(r, z). */
+rh_status rh_precondition(const rh_grid* grid, double density, rh_method method, double* d_r,
+                          double* d_z, double* out_rz);
+
+/* -------------------------------------------- pressure workspace (device) */
+/* PressureWorkspace on the device (src/pressure.rs:243-314): the solver
+ * owns its Krylov arrays and reduction scratch; construction allocates,
+ * solves do not. */
+typedef struct rh_solver rh_solver;
+typedef struct rh_solver_config {
+  rh_method method;
+  rh_precision precision;    /* Krylov vector storage (F64 = reference arithmetic) */
+  int32_t mg_levels;         /* multigrid levels (0 = automatic) */
+  int32_t mg_pre_sweeps;     /* default 2 */
+  int32_t mg_post_sweeps;    /* default 2 */
+  double mg_omega;           /* weighted-Jacobi smoother weight (0 = default) */
+  uint64_t memory_limit;     /* bytes; 0 = unlimited (PressureWorkspace limit) */
+} rh_solver_config;
+
+rh_status rh_solver_create(const rh_grid* grid, const rh_solver_config* config,
+                           rh_solver** out);
+void rh_solver_destroy(rh_solver* solver);
+uint64_t rh_solver_allocated_bytes(const rh_solver* solver);
+/* solve_rhs (src/pressure.rs:365-393 -> solve_loaded :577-740).  d_rhs: N f64
+ * (full, ungauged RHS).  d_pressure: N f64, receives the last iterate (also
+ * on ITERATION_LIMIT), gauge cell 0 exactly 0. */
+rh_status rh_solver_solve_rhs(rh_solver* solver, double density, const double* d_rhs,
+                              double dt, const rh_pressure_settings* settings,
+                              double* d_pressure, rh_pressure_report* report,
+                              rh_cancel_fn cancel, void* cancel_ctx);
+
+/* ------------------------------------------------- provider (host memory) */
+/* PressureRequest (src/solver_provider.rs:47-52). rhs: N f64, host. */
+typedef struct rh_pressure_request {
+  const rh_grid* grid;
+  double density;
+  double dt;
+  const double* rhs;
+  size_t rhs_len;
+} rh_pressure_request;
+
+/* SolverMetadata + TensorMetadata (src/solver_provider.rs:17-33). */
+typedef struct rh_solver_metadata {
+  char id[48];
+  char version[16];
+  char stage[16];
+  uint64_t shape_xyz[3];
+  char layout[32];
+  char scalar[8];
+  char units_in[16];
+  char units_out[16];
+  char domain[48];
+} rh_solver_metadata;
+
+typedef struct rh_provider_config {
+  rh_solver_config solver;
+  rh_pressure_settings settings;  /* the provider's own stopping rule */
+} rh_provider_config;
+
+/* impl PressureProvider for the GPU solver (trait src/solver_provider.rs:67-76). */
+typedef struct rh_provider rh_provider;
+rh_status rh_provider_create(const rh_provider_config* config, rh_provider** out);
+void rh_provider_destroy(rh_provider* provider);
+/* metadata(): shape_xyz taken from `grid` (the trait's metadata is per provider;
+ * the shape is that of the request it will serve). */
+rh_status rh_provider_metadata(const rh_provider* provider, const rh_grid* grid,
+                               rh_solver_metadata* out);
+/* supports(): RH_OK or RH_UNSUPPORTED with a static reason string. */
+rh_status rh_provider_supports(const rh_provider* provider, const rh_pressure_request* request,
+                               const char** reason);
+/* predict(): writes all N cells of `pressure` (host), p[0] == 0.  Returns
+ * RH_OK, RH_CANCELLED, RH_UNSUPPORTED or RH_EXECUTION (+ reason). */
+rh_status rh_provider_predict(rh_provider* provider, const rh_pressure_request* request,
+                              double* pressure, size_t pressure_len, rh_cancel_fn cancel,
+                              void* cancel_ctx, const char** reason);
+
+/* ProviderReport (src/solver_provider.rs:78-92); durations in seconds. */
+typedef struct rh_provider_report {
+  char selected[48];
+  int32_t attempted;                 /* a preferred provider was offered */
+  rh_status rejection;               /* RH_OK if none */
+  int32_t has_candidate;
+  rh_pressure_report candidate;
+  rh_pressure_report accepted;
+  double total, inference, qualification, fallback;
+} rh_provider_report;
+
+/* ProviderWorkspace (src/solver_provider.rs:94-327): input checks, NaN
+ * pre-fill, predict, residual qualification (on the device), and the native
+ * fallback -- the device "jacobi-pcg-v1" -- when the candidate is rejected. */
+typedef struct rh_provider_workspace rh_provider_workspace;
+rh_status rh_provider_workspace_create(const rh_grid* grid, uint64_t limit,
+                                       rh_method fallback, rh_provider_workspace** out);
+void rh_provider_workspace_destroy(rh_provider_workspace* ws);
+rh_status rh_provider_workspace_solve(rh_provider_workspace* ws,
+                                      const rh_pressure_request* request,
+                                      const rh_pressure_settings* settings,
+                                      rh_provider* preferred /* may be NULL */,
+                                      rh_cancel_fn cancel, void* cancel_ctx,
+                                      rh_provider_report* report);
+/* pressure(): the last accepted pressure (host copy, N f64). */
+rh_status rh_provider_workspace_pressure(const rh_provider_workspace* ws, double* out,
+                                         size_t len);
+
+/* --------------------------------------------- device-resident scene step */
+/* A fixed-box carrier with accepted and candidate faces on the device
+ * (src/simulation.rs accepted/candidate sets, commit swap :805-811). */
+typedef struct rh_scene rh_scene;
+typedef struct rh_scene_config {
+  double density;
+  rh_precision precision;           /* face storage */
+  rh_solver_config solver;
+  rh_pressure_settings pressure;
+  double actual_divergence_limit;   /* SimulationConfig (src/simulation.rs:11-18) */
+  double max_courant;               /* 0 disables the Courant gate */
+} rh_scene_config;
+
+/* StepReport (src/simulation.rs:44-53) subset + timings. */
+typedef struct rh_step_report {
+  double dt;
+  rh_pressure_report pressure;
+  double actual_divergence_max;
+  double courant;
+  double seconds;          /* device time of the whole step */
+  double advect_seconds, rhs_seconds, solve_seconds, correct_seconds;
+} rh_step_report;
+
+rh_status rh_scene_create(const rh_grid* grid, const rh_scene_config* config, rh_scene** out);
+void rh_scene_destroy(rh_scene* scene);
+uint64_t rh_scene_allocated_bytes(const rh_scene* scene);
+/* Upload/download the accepted faces (host arrays in the scene precision). */
+rh_status rh_scene_set_velocity(rh_scene* scene, const void* u, const void* v, const void* w);
+rh_status rh_scene_get_velocity(const rh_scene* scene, void* u, void* v, void* w);
+/* Pressure of the last accepted step (N f64, host). */
+rh_status rh_scene_get_pressure(const rh_scene* scene, double* out, size_t len);
+/* Device pointers of the accepted faces (for zero-copy producers). */
+rh_status rh_scene_device_velocity(rh_scene* scene, void** d_u, void** d_v, void** d_w);
+/* One projection step with an explicit dt: advect -> build_rhs -> solve ->
+ * correct -> divergence gate -> Courant gate -> commit.  On any failure the
+ * accepted faces are unchanged (transactional, src/simulation.rs:391-392). */
+rh_status rh_scene_step(rh_scene* scene, double dt, rh_step_report* report,
+                        rh_cancel_fn cancel, void* cancel_ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RHEON_GPU_H */

@@ -1,0 +1,358 @@
+"""CPU oracle for Rheon's fixed-box MAC projection step.
+
+TEST INFRASTRUCTURE ONLY.  This package is the checker for the HIP path in
+``rheon_amd``: only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import it.  The product never calls
+it and has no CPU fallback through it.
+
+It wraps ``oracle/liboracle.so`` (built from ``rheon_oracle.c`` by
+``oracle/Makefile``), a C restatement of the reference routines
+(src/geometry.rs, src/sampling.rs, src/advection.rs, src/operator.rs,
+src/pressure.rs; line citations beside each C function).  Arrays are numpy,
+x-fastest, exactly the reference's flat layouts.
+
+Parity status of each function (how it is pinned) is listed in the test
+module tests/test_oracle.py; every function here has at least one check
+against a reference known-answer test, a reference-published golden run, or
+a closed form.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+OK = 0
+CANCELLED = 6
+INVALID_ARGUMENT = 7
+INVALID_SETTINGS = 8
+INCOMPATIBLE_RHS = 9
+BREAKDOWN = 10
+ITERATION_LIMIT = 11
+NONFINITE_INPUT = 12
+NONZERO_WALL_VELOCITY = 13
+ARITHMETIC_FAILURE = 14
+INVALID_TIME_STEP = 15
+INVALID_DENSITY = 16
+NONFINITE_POSITION = 17
+GEOMETRY = 20
+
+JACOBI, SGS, RBSGS = 0, 1, 2
+METHODS = {"jacobi-pcg-v1": JACOBI, "sgs-pcg-v1": SGS, "rbsgs-pcg": RBSGS}
+
+
+class _Grid(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64 * 3),
+        ("fn", (C.c_int64 * 3) * 3),
+        ("cell_len", C.c_int64),
+        ("face_len", C.c_int64 * 3),
+        ("h", C.c_double * 3),
+        ("origin", C.c_double * 3),
+        ("upper", C.c_double * 3),
+        ("volume", C.c_double),
+    ]
+
+
+class _Settings(C.Structure):
+    _fields_ = [
+        ("relative_residual", C.c_double),
+        ("absolute_residual", C.c_double),
+        ("divergence_limit", C.c_double),
+        ("max_iterations", C.c_int64),
+    ]
+
+
+class _Report(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("true_residual_l2", C.c_double),
+        ("true_residual_max", C.c_double),
+        ("predicted_divergence_max", C.c_double),
+        ("error_iteration", C.c_int64),
+        ("error_value", C.c_double),
+    ]
+
+
+def build() -> None:
+    """Compile liboracle.so with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = C.c_void_p
+_G = C.POINTER(_Grid)
+
+
+def _declare(L):
+    L.or_grid_init.argtypes = [_G, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.or_buffer_plan.argtypes = [_G, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.or_buffer_plan.restype = C.c_int64
+    for suf in ("f32", "f64"):
+        getattr(L, f"or_sample_{suf}").argtypes = [_G, C.c_int, _P, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        getattr(L, f"or_backtrace_{suf}").argtypes = [_G, _P, _P, _P, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]
+        getattr(L, f"or_advect_velocity_{suf}").argtypes = [_G, _P, _P, _P, C.c_double, _P, _P, _P, C.c_int]
+        getattr(L, f"or_build_rhs_{suf}").argtypes = [_G, _P, _P, _P, C.c_double, _P]
+        getattr(L, f"or_divergence_{suf}").argtypes = [_G, _P, _P, _P, _P]
+        getattr(L, f"or_correct_{suf}").argtypes = [_G, C.c_double, _P, C.c_double, _P, _P, _P]
+        getattr(L, f"or_actual_divergence_max_{suf}").argtypes = [_G, _P, _P, _P, C.POINTER(C.c_double)]
+        getattr(L, f"or_velocity_rate_{suf}").argtypes = [_G, _P, _P, _P]
+        getattr(L, f"or_velocity_rate_{suf}").restype = C.c_double
+    L.or_advect_tracer_f32.argtypes = [_G, _P, _P, _P, _P, C.c_double, _P, C.c_int]
+    L.or_apply_full.argtypes = [_G, C.c_double, _P, _P, C.c_int]
+    L.or_diagonal.argtypes = [_G, C.c_double, C.POINTER(C.c_int64)]
+    L.or_diagonal.restype = C.c_double
+    L.or_compensated_sum.argtypes = [_P, C.c_int64]
+    L.or_compensated_sum.restype = C.c_double
+    L.or_dot.argtypes = [_P, _P, C.c_int64]
+    L.or_dot.restype = C.c_double
+    for name in ("or_precondition_sgs", "or_precondition_rbsgs"):
+        getattr(L, name).argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double)]
+    L.or_precondition_jacobi.argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double), C.c_int]
+    L.or_solve_rhs.argtypes = [_G, C.c_double, _P, C.c_double, C.POINTER(_Settings), C.c_int, C.c_int, _P, C.POINTER(_Report)]
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        super().__init__(f"oracle status {status} {what}")
+        self.status = status
+
+
+def _ptr(a: np.ndarray) -> int:
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+class Grid:
+    """GridGeometry (src/geometry.rs:84-267)."""
+
+    def __init__(self, counts, spacing, origin=(0.0, 0.0, 0.0)):
+        self.s = _Grid()
+        st = lib().or_grid_init(C.byref(self.s), (C.c_uint64 * 3)(*[int(c) for c in counts]), _d3(spacing), _d3(origin))
+        if st != OK:
+            raise OracleError(st, "GridGeometry::new")
+        self.counts = tuple(int(c) for c in counts)
+        self.spacing = tuple(float(h) for h in spacing)
+        self.origin = tuple(float(o) for o in origin)
+
+    @property
+    def cell_len(self) -> int:
+        return int(self.s.cell_len)
+
+    def face_counts(self, axis: int):
+        return tuple(int(self.s.fn[axis][d]) for d in range(3))
+
+    def face_len(self, axis: int) -> int:
+        return int(self.s.face_len[axis])
+
+    @property
+    def upper(self):
+        return tuple(self.s.upper)
+
+    @property
+    def volume(self) -> float:
+        return float(self.s.volume)
+
+    def buffer_plan(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        total = lib().or_buffer_plan(C.byref(self.s), C.byref(a), C.byref(b), C.byref(c))
+        return {"velocity_pair_bytes": a.value, "tracer_pair_bytes": b.value,
+                "pressure_workspace_bytes": c.value, "total_bytes": total}
+
+
+def _suffix(dtype) -> str:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return "f32"
+    if dt == np.float64:
+        return "f64"
+    raise TypeError(f"unsupported storage {dt}")
+
+
+def sample(grid: Grid, component: int, values: np.ndarray, pos) -> float:
+    out = C.c_double()
+    st = getattr(lib(), f"or_sample_{_suffix(values.dtype)}")(C.byref(grid.s), component, _ptr(values), _d3(pos), C.byref(out))
+    if st != OK:
+        raise OracleError(st, "sample")
+    return out.value
+
+
+def backtrace(grid: Grid, vel, pos, dt: float):
+    out = (C.c_double * 3)()
+    u, v, w = vel
+    st = getattr(lib(), f"or_backtrace_{_suffix(u.dtype)}")(C.byref(grid.s), _ptr(u), _ptr(v), _ptr(w), _d3(pos), dt, out)
+    if st != OK:
+        raise OracleError(st, "backtrace")
+    return [out[0], out[1], out[2]]
+
+
+def advect_velocity(grid: Grid, vel, dt: float, threads: int = 1):
+    u, v, w = vel
+    outs = [np.empty_like(x) for x in vel]
+    st = getattr(lib(), f"or_advect_velocity_{_suffix(u.dtype)}")(
+        C.byref(grid.s), _ptr(u), _ptr(v), _ptr(w), dt, _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]), threads)
+    if st != OK:
+        raise OracleError(st, "advect_velocity")
+    return outs
+
+
+def advect_tracer(grid: Grid, old: np.ndarray, vel, dt: float, threads: int = 1):
+    u, v, w = vel
+    out = np.empty_like(old)
+    st = lib().or_advect_tracer_f32(C.byref(grid.s), _ptr(old), _ptr(u), _ptr(v), _ptr(w), dt, _ptr(out), threads)
+    if st != OK:
+        raise OracleError(st, "advect_tracer")
+    return out
+
+
+def build_rhs(grid: Grid, vel, dt: float) -> np.ndarray:
+    u, v, w = vel
+    out = np.empty(grid.cell_len, np.float64)
+    st = getattr(lib(), f"or_build_rhs_{_suffix(u.dtype)}")(C.byref(grid.s), _ptr(u), _ptr(v), _ptr(w), dt, _ptr(out))
+    if st != OK:
+        raise OracleError(st, "build_rhs")
+    return out
+
+
+def apply_full(grid: Grid, density: float, x: np.ndarray, threads: int = 1) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    st = lib().or_apply_full(C.byref(grid.s), density, _ptr(x), _ptr(y), threads)
+    if st != OK:
+        raise OracleError(st, "apply_full")
+    return y
+
+
+def diagonal(grid: Grid, density: float, p) -> float:
+    return lib().or_diagonal(C.byref(grid.s), density, (C.c_int64 * 3)(*p))
+
+
+def divergence(grid: Grid, vel) -> np.ndarray:
+    u, v, w = vel
+    out = np.empty(grid.cell_len, np.float64)
+    st = getattr(lib(), f"or_divergence_{_suffix(u.dtype)}")(C.byref(grid.s), _ptr(u), _ptr(v), _ptr(w), _ptr(out))
+    if st != OK:
+        raise OracleError(st, "divergence")
+    return out
+
+
+def actual_divergence_max(grid: Grid, vel) -> float:
+    u, v, w = vel
+    out = C.c_double()
+    st = getattr(lib(), f"or_actual_divergence_max_{_suffix(u.dtype)}")(C.byref(grid.s), _ptr(u), _ptr(v), _ptr(w), C.byref(out))
+    if st != OK:
+        raise OracleError(st, "actual_divergence_max")
+    return out.value
+
+
+def correct(grid: Grid, density: float, p: np.ndarray, dt: float, vel):
+    """Return corrected copies of vel (correct_candidate_in_place / _f64)."""
+    outs = [np.array(x, copy=True) for x in vel]
+    st = getattr(lib(), f"or_correct_{_suffix(outs[0].dtype)}")(
+        C.byref(grid.s), density, _ptr(p), dt, _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]))
+    if st != OK:
+        raise OracleError(st, "correct")
+    return outs
+
+
+def velocity_rate(grid: Grid, vel) -> float:
+    u, v, w = vel
+    return getattr(lib(), f"or_velocity_rate_{_suffix(u.dtype)}")(C.byref(grid.s), _ptr(u), _ptr(v), _ptr(w))
+
+
+def compensated_sum(x: np.ndarray) -> float:
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_compensated_sum(_ptr(x), x.size)
+
+
+def dot(a: np.ndarray, b: np.ndarray) -> float:
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return lib().or_dot(_ptr(a), _ptr(b), a.size)
+
+
+def precondition(grid: Grid, density: float, r: np.ndarray, method: str = "jacobi-pcg-v1", threads: int = 1):
+    """Return (r with gauge row zeroed, z, rz)."""
+    r = np.array(r, np.float64, copy=True)
+    z = np.zeros_like(r)
+    rz = C.c_double()
+    m = METHODS[method]
+    if m == JACOBI:
+        st = lib().or_precondition_jacobi(C.byref(grid.s), density, _ptr(r), _ptr(z), C.byref(rz), threads)
+    elif m == SGS:
+        st = lib().or_precondition_sgs(C.byref(grid.s), density, _ptr(r), _ptr(z), C.byref(rz))
+    else:
+        st = lib().or_precondition_rbsgs(C.byref(grid.s), density, _ptr(r), _ptr(z), C.byref(rz))
+    if st != OK:
+        raise OracleError(st, "precondition")
+    return r, z, rz.value
+
+
+@dataclass
+class Settings:
+    """PressureSettings (src/pressure.rs:33-39)."""
+    relative_residual: float = 1e-9
+    absolute_residual: float = 1e-12
+    divergence_limit: float = 1e-7
+    max_iterations: int = 2000
+
+
+@dataclass
+class SolveResult:
+    status: int
+    pressure: np.ndarray
+    iterations: int
+    true_residual_l2: float
+    true_residual_max: float
+    predicted_divergence_max: float
+    error_iteration: int
+    error_value: float
+
+    @property
+    def ok(self) -> bool:
+        return self.status == OK
+
+
+def solve_rhs(grid: Grid, density: float, rhs: np.ndarray, dt: float, settings: Settings,
+              method: str = "jacobi-pcg-v1", threads: int = 1) -> SolveResult:
+    """PressureWorkspace::solve_rhs (src/pressure.rs:365-393 -> :577-740).
+
+    Returns the status instead of raising; ``pressure`` holds the last iterate
+    (also on IterationLimit, as the reference's workspace does)."""
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    p = np.zeros(grid.cell_len, np.float64)
+    s = _Settings(settings.relative_residual, settings.absolute_residual, settings.divergence_limit,
+                  int(settings.max_iterations))
+    rep = _Report()
+    st = lib().or_solve_rhs(C.byref(grid.s), density, _ptr(rhs), dt, C.byref(s), METHODS[method], threads,
+                            _ptr(p), C.byref(rep))
+    return SolveResult(st, p, rep.iterations, rep.true_residual_l2, rep.true_residual_max,
+                       rep.predicted_divergence_max, rep.error_iteration, rep.error_value)
+
+
+def solve_velocity(grid: Grid, density: float, vel, dt: float, settings: Settings,
+                   method: str = "jacobi-pcg-v1", threads: int = 1) -> SolveResult:
+    """PressureWorkspace::solve_velocity (src/pressure.rs:344-361)."""
+    return solve_rhs(grid, density, build_rhs(grid, vel, dt), dt, settings, method, threads)

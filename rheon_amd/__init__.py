@@ -1,0 +1,528 @@
+"""rheon_amd: Rheon's fixed-box MAC projection step on AMD Instinct MI355X.
+
+The compute path is hand-written HIP for gfx950 in ``rheon_amd/csrc``,
+exposed through the C ABI of ``include/rheon_gpu.h`` and bound here with
+ctypes.  This module mirrors the reference's host-side interface for the
+path (src/solver_provider.rs PressureProvider / ProviderWorkspace,
+src/pressure.rs PressureWorkspace / PressureSettings / PressureReport,
+src/simulation.rs step sequence) so callers and tests read like the
+reference's own.  There is no CPU fallback: every compute call goes to the
+device library and raises if it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from .scenes import GridSpec
+
+# rh_status values (include/rheon_gpu.h)
+OK = 0
+UNSUPPORTED = 1
+EXECUTION = 2
+NONFINITE_OUTPUT = 3
+NONZERO_GAUGE = 4
+QUALITY_GATE = 5
+CANCELLED = 6
+INVALID_ARGUMENT = 7
+INVALID_SETTINGS = 8
+INCOMPATIBLE_RHS = 9
+BREAKDOWN = 10
+ITERATION_LIMIT = 11
+NONFINITE_INPUT = 12
+NONZERO_WALL_VELOCITY = 13
+ARITHMETIC_FAILURE = 14
+INVALID_TIME_STEP = 15
+INVALID_DENSITY = 16
+DIVERGENCE_LIMIT = 18
+COURANT_LIMIT = 19
+GEOMETRY = 20
+BUFFER_LIMIT = 21
+DEVICE_ERROR = 22
+
+STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) and k.isupper()}
+
+F64, F32 = 0, 1
+JACOBI_PCG, RBSGS_PCG, MG_PCG, MIXED_MG = 0, 1, 2, 3
+METHOD_IDS = {"jacobi-pcg-v1": JACOBI_PCG, "rbsgs-pcg": RBSGS_PCG, "mg-pcg": MG_PCG,
+              "mixed-mg-dc": MIXED_MG}
+
+
+class RheonError(RuntimeError):
+    def __init__(self, status: int, where: str = ""):
+        msg = N.lib().rh_last_error().decode(errors="replace") if status == DEVICE_ERROR else ""
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)} {msg}".strip())
+        self.status = status
+
+
+def _check(status: int, where: str) -> None:
+    if status != OK:
+        raise RheonError(status, where)
+
+
+def lib():
+    return N.lib()
+
+
+def precision_of(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float64:
+        return F64
+    if dt == np.float32:
+        return F32
+    raise TypeError(f"unsupported storage dtype {dt}")
+
+
+def dtype_of(precision: int):
+    return np.float64 if precision == F64 else np.float32
+
+
+def make_grid(spec: GridSpec) -> N.Grid:
+    g = N.Grid()
+    for d in range(3):
+        g.counts[d] = int(spec.counts[d])
+        g.spacing[d] = float(spec.spacing[d])
+        g.origin[d] = float(spec.origin[d])
+    return g
+
+
+def device_info():
+    n = C.c_int32()
+    arch = C.create_string_buffer(64)
+    _check(lib().rh_device_info(C.byref(n), arch, 64), "rh_device_info")
+    return n.value, arch.value.decode()
+
+
+class DeviceArray:
+    """Device memory owned through rh_device_alloc/rh_device_free."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        self.nbytes = self.count * self.dtype.itemsize
+        ptr = C.c_void_p()
+        _check(lib().rh_device_alloc(self.nbytes, C.byref(ptr)), "rh_device_alloc")
+        self.ptr = ptr.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.count
+        _check(lib().rh_copy_to_device(self.ptr, a.ctypes.data, self.nbytes), "rh_copy_to_device")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.count, self.dtype)
+        _check(lib().rh_copy_to_host(out.ctypes.data, self.ptr, self.nbytes), "rh_copy_to_host")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().rh_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _faces(vel, dtype=None):
+    arrs = [x if isinstance(x, DeviceArray) else DeviceArray.from_host(np.ascontiguousarray(x, dtype or x.dtype))
+            for x in vel]
+    return arrs
+
+
+# ----------------------------------------------------------- operator calls
+def apply_full(spec: GridSpec, density: float, x: np.ndarray) -> np.ndarray:
+    """PressureOperator::apply_full (src/operator.rs:378-456) on the device."""
+    g = make_grid(spec)
+    dx = DeviceArray.from_host(np.ascontiguousarray(x, np.float64))
+    dy = DeviceArray(dx.count, np.float64)
+    _check(lib().rh_apply_full(C.byref(g), density, dx.ptr, dy.ptr), "rh_apply_full")
+    return dy.download()
+
+
+def advect_velocity(spec: GridSpec, vel, dt: float):
+    """advect_velocity (src/advection.rs:28-68); storage = dtype of vel."""
+    g = make_grid(spec)
+    prec = precision_of(vel[0].dtype)
+    src = _faces(vel)
+    dst = [DeviceArray(a.count, a.dtype) for a in src]
+    _check(lib().rh_advect_velocity(C.byref(g), prec, src[0].ptr, src[1].ptr, src[2].ptr, dt,
+                                    dst[0].ptr, dst[1].ptr, dst[2].ptr), "rh_advect_velocity")
+    return [d.download() for d in dst]
+
+
+def advect_tracer(spec: GridSpec, old: np.ndarray, vel, dt: float) -> np.ndarray:
+    """advect_tracer (src/advection.rs:72-102), f32."""
+    g = make_grid(spec)
+    src = _faces(vel, np.float32)
+    dold = DeviceArray.from_host(np.ascontiguousarray(old, np.float32))
+    out = DeviceArray(dold.count, np.float32)
+    _check(lib().rh_advect_tracer(C.byref(g), dold.ptr, src[0].ptr, src[1].ptr, src[2].ptr, dt, out.ptr),
+           "rh_advect_tracer")
+    return out.download()
+
+
+def build_rhs(spec: GridSpec, vel, dt: float) -> np.ndarray:
+    """PressureOperator::build_rhs (src/operator.rs:460-509)."""
+    g = make_grid(spec)
+    prec = precision_of(vel[0].dtype)
+    src = _faces(vel)
+    out = DeviceArray(spec.cell_len, np.float64)
+    _check(lib().rh_build_rhs(C.byref(g), prec, src[0].ptr, src[1].ptr, src[2].ptr, dt, out.ptr), "rh_build_rhs")
+    return out.download()
+
+
+def correct_velocity(spec: GridSpec, density: float, pressure: np.ndarray, dt: float, vel):
+    """correct_candidate_in_place (src/operator.rs:627-668); returns corrected copies."""
+    g = make_grid(spec)
+    prec = precision_of(vel[0].dtype)
+    faces = _faces(vel)
+    dp = DeviceArray.from_host(np.ascontiguousarray(pressure, np.float64))
+    _check(lib().rh_correct_velocity(C.byref(g), prec, density, dp.ptr, dt, faces[0].ptr, faces[1].ptr,
+                                     faces[2].ptr), "rh_correct_velocity")
+    return [f.download() for f in faces]
+
+
+def divergence_max(spec: GridSpec, vel) -> float:
+    """PressureWorkspace::actual_divergence_max (src/pressure.rs:330-342)."""
+    g = make_grid(spec)
+    prec = precision_of(vel[0].dtype)
+    faces = _faces(vel)
+    out = C.c_double()
+    _check(lib().rh_divergence_max(C.byref(g), prec, faces[0].ptr, faces[1].ptr, faces[2].ptr, C.byref(out)),
+           "rh_divergence_max")
+    return out.value
+
+
+def velocity_rate(spec: GridSpec, vel) -> float:
+    """velocity_rate (src/simulation.rs:1415-1429)."""
+    g = make_grid(spec)
+    prec = precision_of(vel[0].dtype)
+    faces = _faces(vel)
+    out = C.c_double()
+    _check(lib().rh_velocity_rate(C.byref(g), prec, faces[0].ptr, faces[1].ptr, faces[2].ptr, C.byref(out)),
+           "rh_velocity_rate")
+    return out.value
+
+
+def precondition(spec: GridSpec, density: float, r: np.ndarray, method: str = "jacobi-pcg-v1"):
+    """One preconditioner application; returns (r with r[0]=0, z, rz)."""
+    g = make_grid(spec)
+    dr = DeviceArray.from_host(np.ascontiguousarray(r, np.float64))
+    dz = DeviceArray(dr.count, np.float64)
+    rz = C.c_double()
+    _check(lib().rh_precondition(C.byref(g), density, METHOD_IDS[method], dr.ptr, dz.ptr, C.byref(rz)),
+           "rh_precondition")
+    return dr.download(), dz.download(), rz.value
+
+
+# ----------------------------------------------------------- pressure solve
+@dataclass
+class PressureSettings:
+    """PressureSettings (src/pressure.rs:33-39)."""
+    relative_residual: float = 1e-9
+    absolute_residual: float = 1e-12
+    divergence_limit: float = 1e-7
+    max_iterations: int = 2000
+    poll_interval: int = 0
+
+    def native(self) -> N.PressureSettings:
+        return N.PressureSettings(self.relative_residual, self.absolute_residual, self.divergence_limit,
+                                  int(self.max_iterations), int(self.poll_interval), 0)
+
+
+@dataclass
+class PressureReport:
+    """PressureReport (src/pressure.rs:43-49) + error payload and timing."""
+    status: int
+    iterations: int
+    true_residual_l2: float
+    true_residual_max: float
+    predicted_divergence_max: float
+    error_iteration: int = 0
+    error_value: float = 0.0
+    threshold: float = 0.0
+    seconds: float = 0.0
+
+    @classmethod
+    def from_native(cls, status: int, r: N.PressureReport) -> "PressureReport":
+        return cls(status, r.iterations, r.true_residual_l2, r.true_residual_max, r.predicted_divergence_max,
+                   r.error_iteration, r.error_value, r.threshold, r.seconds)
+
+    @property
+    def ok(self) -> bool:
+        return self.status == OK
+
+
+def _cancel_cb(cancelled):
+    if cancelled is None:
+        return N.CANCEL_FN(0), None
+    cb = N.CANCEL_FN(lambda _ctx: 1 if cancelled() else 0)
+    return cb, cb
+
+
+class PressureWorkspace:
+    """PressureWorkspace on the device (src/pressure.rs:243-740)."""
+
+    def __init__(self, spec: GridSpec, method: str = "jacobi-pcg-v1", precision: int = F64,
+                 memory_limit: int = 0, mg_levels: int = 0):
+        self.spec = spec
+        self.grid = make_grid(spec)
+        cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, 0, 0, 0.0, memory_limit)
+        h = C.c_void_p()
+        _check(lib().rh_solver_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_solver_create")
+        self.handle = h.value
+        self.pressure = DeviceArray(spec.cell_len, np.float64)
+
+    def allocated_bytes(self) -> int:
+        return int(lib().rh_solver_allocated_bytes(self.handle))
+
+    def solve_rhs(self, density: float, rhs, dt: float, settings: PressureSettings, cancelled=None) -> PressureReport:
+        """PressureWorkspace::solve_rhs (src/pressure.rs:365-393); returns the
+        report with its status (pressure() holds the last iterate)."""
+        drhs = rhs if isinstance(rhs, DeviceArray) else DeviceArray.from_host(np.ascontiguousarray(rhs, np.float64))
+        rep = N.PressureReport()
+        st = settings.native()
+        cb, _keep = _cancel_cb(cancelled)
+        status = lib().rh_solver_solve_rhs(self.handle, density, drhs.ptr, dt, C.byref(st), self.pressure.ptr,
+                                           C.byref(rep), cb, None)
+        return PressureReport.from_native(status, rep)
+
+    def solve_velocity(self, density: float, vel, dt: float, settings: PressureSettings, cancelled=None):
+        """PressureWorkspace::solve_velocity (src/pressure.rs:344-361)."""
+        return self.solve_rhs(density, build_rhs(self.spec, vel, dt), dt, settings, cancelled)
+
+    def get_pressure(self) -> np.ndarray:
+        return self.pressure.download()
+
+    def close(self):
+        if self.handle:
+            lib().rh_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ provider seam
+@dataclass
+class PressureRequest:
+    """PressureRequest (src/solver_provider.rs:47-52)."""
+    spec: GridSpec
+    density: float
+    dt: float
+    rhs: np.ndarray
+
+    def native(self):
+        g = make_grid(self.spec)
+        rhs = np.ascontiguousarray(self.rhs, np.float64)
+        req = N.PressureRequest(C.pointer(g), self.density, self.dt, rhs.ctypes.data, rhs.size)
+        return req, (g, rhs)
+
+
+class ProviderRejection(Exception):
+    """ProviderRejection (src/solver_provider.rs:54-62)."""
+
+    def __init__(self, status: int, reason: str | None = None):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {reason}")
+        self.status = status
+        self.reason = reason
+
+
+class GpuPressureProvider:
+    """impl PressureProvider (src/solver_provider.rs:67-76) on the device."""
+
+    def __init__(self, method: str = "jacobi-pcg-v1", precision: int = F64,
+                 settings: PressureSettings | None = None):
+        settings = settings or PressureSettings()
+        cfg = N.ProviderConfig(N.SolverConfig(METHOD_IDS[method], precision, 0, 0, 0, 0.0, 0), settings.native())
+        h = C.c_void_p()
+        _check(lib().rh_provider_create(C.byref(cfg), C.byref(h)), "rh_provider_create")
+        self.handle = h.value
+
+    def metadata(self, spec: GridSpec) -> dict:
+        m = N.SolverMetadata()
+        g = make_grid(spec)
+        _check(lib().rh_provider_metadata(self.handle, C.byref(g), C.byref(m)), "rh_provider_metadata")
+        return {k: (getattr(m, k).decode() if isinstance(getattr(m, k), bytes) else list(getattr(m, k)))
+                for k, _ in N.SolverMetadata._fields_}
+
+    def supports(self, request: PressureRequest) -> None:
+        req, _keep = request.native()
+        reason = C.c_char_p()
+        st = lib().rh_provider_supports(self.handle, C.byref(req), C.byref(reason))
+        if st != OK:
+            raise ProviderRejection(st, reason.value.decode() if reason.value else None)
+
+    def predict(self, request: PressureRequest, pressure: np.ndarray, cancelled=None) -> None:
+        req, _keep = request.native()
+        reason = C.c_char_p()
+        cb, _k2 = _cancel_cb(cancelled)
+        assert pressure.dtype == np.float64 and pressure.flags["C_CONTIGUOUS"]
+        st = lib().rh_provider_predict(self.handle, C.byref(req), pressure.ctypes.data, pressure.size, cb, None,
+                                       C.byref(reason))
+        if st != OK:
+            raise ProviderRejection(st, reason.value.decode() if reason.value else None)
+
+    def close(self):
+        if self.handle:
+            lib().rh_provider_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class ProviderReport:
+    """ProviderReport (src/solver_provider.rs:78-92)."""
+    selected: str
+    attempted: bool
+    rejection: int
+    candidate: PressureReport | None
+    accepted: PressureReport
+    total: float
+    inference: float
+    qualification: float
+    fallback: float
+
+
+class ProviderWorkspace:
+    """ProviderWorkspace (src/solver_provider.rs:94-327): qualification on the
+    device; native fallback = device jacobi-pcg-v1."""
+
+    def __init__(self, spec: GridSpec, limit: int = 0, fallback: str = "jacobi-pcg-v1"):
+        self.spec = spec
+        g = make_grid(spec)
+        h = C.c_void_p()
+        _check(lib().rh_provider_workspace_create(C.byref(g), limit, METHOD_IDS[fallback], C.byref(h)),
+               "rh_provider_workspace_create")
+        self.handle = h.value
+
+    def solve(self, request: PressureRequest, settings: PressureSettings, preferred: GpuPressureProvider | None,
+              cancelled=None) -> ProviderReport:
+        req, _keep = request.native()
+        st = settings.native()
+        rep = N.ProviderReport()
+        cb, _k2 = _cancel_cb(cancelled)
+        status = lib().rh_provider_workspace_solve(self.handle, C.byref(req), C.byref(st),
+                                                   preferred.handle if preferred else None, cb, None, C.byref(rep))
+        if status != OK:
+            raise RheonError(status, "ProviderWorkspace::solve")
+        cand = PressureReport.from_native(OK, rep.candidate) if rep.has_candidate else None
+        return ProviderReport(rep.selected.decode(), bool(rep.attempted), rep.rejection, cand,
+                              PressureReport.from_native(OK, rep.accepted), rep.total, rep.inference,
+                              rep.qualification, rep.fallback)
+
+    def pressure(self) -> np.ndarray:
+        out = np.empty(self.spec.cell_len, np.float64)
+        _check(lib().rh_provider_workspace_pressure(self.handle, out.ctypes.data, out.size),
+               "rh_provider_workspace_pressure")
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().rh_provider_workspace_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ scene
+@dataclass
+class StepReport:
+    status: int
+    dt: float
+    pressure: PressureReport
+    actual_divergence_max: float
+    courant: float
+    seconds: float
+    stage_seconds: dict = field(default_factory=dict)
+
+
+class Scene:
+    """Device-resident fixed-box carrier: accepted and candidate faces stay in
+    HBM; step() runs advect -> RHS -> solve -> correct -> gates -> commit
+    (src/simulation.rs:988-1277, commit :805-811)."""
+
+    def __init__(self, spec: GridSpec, density: float = 1.0, precision: int = F32, method: str = "jacobi-pcg-v1",
+                 solver_precision: int | None = None, settings: PressureSettings | None = None,
+                 actual_divergence_limit: float = 1e-5, max_courant: float = 0.0, mg_levels: int = 0):
+        self.spec = spec
+        self.precision = precision
+        self.grid = make_grid(spec)
+        settings = settings or PressureSettings()
+        sp = precision if solver_precision is None else solver_precision
+        cfg = N.SceneConfig(density, precision, N.SolverConfig(METHOD_IDS[method], sp, mg_levels, 0, 0, 0.0, 0),
+                            settings.native(), actual_divergence_limit, max_courant)
+        h = C.c_void_p()
+        _check(lib().rh_scene_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_scene_create")
+        self.handle = h.value
+
+    def allocated_bytes(self) -> int:
+        return int(lib().rh_scene_allocated_bytes(self.handle))
+
+    def set_velocity(self, vel) -> None:
+        dt = dtype_of(self.precision)
+        arrs = [np.ascontiguousarray(v, dt) for v in vel]
+        for a in range(3):
+            assert arrs[a].size == self.spec.face_len(a)
+        _check(lib().rh_scene_set_velocity(self.handle, arrs[0].ctypes.data, arrs[1].ctypes.data,
+                                           arrs[2].ctypes.data), "rh_scene_set_velocity")
+
+    def get_velocity(self):
+        dt = dtype_of(self.precision)
+        out = [np.empty(self.spec.face_len(a), dt) for a in range(3)]
+        _check(lib().rh_scene_get_velocity(self.handle, out[0].ctypes.data, out[1].ctypes.data,
+                                           out[2].ctypes.data), "rh_scene_get_velocity")
+        return out
+
+    def get_pressure(self) -> np.ndarray:
+        out = np.empty(self.spec.cell_len, np.float64)
+        _check(lib().rh_scene_get_pressure(self.handle, out.ctypes.data, out.size), "rh_scene_get_pressure")
+        return out
+
+    def step(self, dt: float, cancelled=None) -> StepReport:
+        rep = N.StepReport()
+        cb, _keep = _cancel_cb(cancelled)
+        status = lib().rh_scene_step(self.handle, dt, C.byref(rep), cb, None)
+        return StepReport(status, rep.dt, PressureReport.from_native(status, rep.pressure),
+                          rep.actual_divergence_max, rep.courant, rep.seconds,
+                          {"advect": rep.advect_seconds, "rhs": rep.rhs_seconds, "solve": rep.solve_seconds,
+                           "correct": rep.correct_seconds})
+
+    def close(self):
+        if self.handle:
+            lib().rh_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]

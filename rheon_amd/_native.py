@@ -1,0 +1,207 @@
+"""ctypes binding of include/rheon_gpu.h (librheon_amd.so).
+
+The library is loaded lazily on first use and never replaced by anything
+else: if it is missing or fails to load, every call raises NativeMissing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librheon_amd.so")
+
+
+class NativeMissing(RuntimeError):
+    pass
+
+
+# ----------------------------------------------------------------- structs
+class Grid(C.Structure):
+    _fields_ = [("counts", C.c_uint64 * 3), ("spacing", C.c_double * 3), ("origin", C.c_double * 3)]
+
+
+class PressureSettings(C.Structure):
+    _fields_ = [
+        ("relative_residual", C.c_double),
+        ("absolute_residual", C.c_double),
+        ("divergence_limit", C.c_double),
+        ("max_iterations", C.c_int64),
+        ("poll_interval", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class PressureReport(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("true_residual_l2", C.c_double),
+        ("true_residual_max", C.c_double),
+        ("predicted_divergence_max", C.c_double),
+        ("error_iteration", C.c_int64),
+        ("error_value", C.c_double),
+        ("threshold", C.c_double),
+        ("seconds", C.c_double),
+    ]
+
+
+class SolverConfig(C.Structure):
+    _fields_ = [
+        ("method", C.c_int),
+        ("precision", C.c_int),
+        ("mg_levels", C.c_int32),
+        ("mg_pre_sweeps", C.c_int32),
+        ("mg_post_sweeps", C.c_int32),
+        ("mg_omega", C.c_double),
+        ("memory_limit", C.c_uint64),
+    ]
+
+
+class PressureRequest(C.Structure):
+    _fields_ = [
+        ("grid", C.POINTER(Grid)),
+        ("density", C.c_double),
+        ("dt", C.c_double),
+        ("rhs", C.c_void_p),
+        ("rhs_len", C.c_size_t),
+    ]
+
+
+class SolverMetadata(C.Structure):
+    _fields_ = [
+        ("id", C.c_char * 48),
+        ("version", C.c_char * 16),
+        ("stage", C.c_char * 16),
+        ("shape_xyz", C.c_uint64 * 3),
+        ("layout", C.c_char * 32),
+        ("scalar", C.c_char * 8),
+        ("units_in", C.c_char * 16),
+        ("units_out", C.c_char * 16),
+        ("domain", C.c_char * 48),
+    ]
+
+
+class ProviderConfig(C.Structure):
+    _fields_ = [("solver", SolverConfig), ("settings", PressureSettings)]
+
+
+class ProviderReport(C.Structure):
+    _fields_ = [
+        ("selected", C.c_char * 48),
+        ("attempted", C.c_int32),
+        ("rejection", C.c_int),
+        ("has_candidate", C.c_int32),
+        ("candidate", PressureReport),
+        ("accepted", PressureReport),
+        ("total", C.c_double),
+        ("inference", C.c_double),
+        ("qualification", C.c_double),
+        ("fallback", C.c_double),
+    ]
+
+
+class SceneConfig(C.Structure):
+    _fields_ = [
+        ("density", C.c_double),
+        ("precision", C.c_int),
+        ("solver", SolverConfig),
+        ("pressure", PressureSettings),
+        ("actual_divergence_limit", C.c_double),
+        ("max_courant", C.c_double),
+    ]
+
+
+class StepReport(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double),
+        ("pressure", PressureReport),
+        ("actual_divergence_max", C.c_double),
+        ("courant", C.c_double),
+        ("seconds", C.c_double),
+        ("advect_seconds", C.c_double),
+        ("rhs_seconds", C.c_double),
+        ("solve_seconds", C.c_double),
+        ("correct_seconds", C.c_double),
+    ]
+
+
+CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
+
+_P = C.c_void_p
+_G = C.POINTER(Grid)
+_S = C.c_int
+
+# name -> (restype, argtypes); every function declared in include/rheon_gpu.h
+SIGNATURES = {
+    "rh_abi_version": (C.c_int32, []),
+    "rh_last_error": (C.c_char_p, []),
+    "rh_device_info": (_S, [C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]),
+    "rh_device_alloc": (_S, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "rh_device_free": (_S, [_P]),
+    "rh_copy_to_device": (_S, [_P, _P, C.c_size_t]),
+    "rh_copy_to_host": (_S, [_P, _P, C.c_size_t]),
+    "rh_device_synchronize": (_S, []),
+    "rh_grid_validate": (_S, [_G, C.POINTER(C.c_uint64)]),
+    "rh_buffer_plan": (_S, [_G] + [C.POINTER(C.c_uint64)] * 4),
+    "rh_apply_full": (_S, [_G, C.c_double, _P, _P]),
+    "rh_advect_velocity": (_S, [_G, C.c_int, _P, _P, _P, C.c_double, _P, _P, _P]),
+    "rh_advect_tracer": (_S, [_G, _P, _P, _P, _P, C.c_double, _P]),
+    "rh_build_rhs": (_S, [_G, C.c_int, _P, _P, _P, C.c_double, _P]),
+    "rh_correct_velocity": (_S, [_G, C.c_int, C.c_double, _P, C.c_double, _P, _P, _P]),
+    "rh_divergence_max": (_S, [_G, C.c_int, _P, _P, _P, C.POINTER(C.c_double)]),
+    "rh_velocity_rate": (_S, [_G, C.c_int, _P, _P, _P, C.POINTER(C.c_double)]),
+    "rh_precondition": (_S, [_G, C.c_double, C.c_int, _P, _P, C.POINTER(C.c_double)]),
+    "rh_solver_create": (_S, [_G, C.POINTER(SolverConfig), C.POINTER(C.c_void_p)]),
+    "rh_solver_destroy": (None, [_P]),
+    "rh_solver_allocated_bytes": (C.c_uint64, [_P]),
+    "rh_solver_solve_rhs": (_S, [_P, C.c_double, _P, C.c_double, C.POINTER(PressureSettings), _P,
+                                 C.POINTER(PressureReport), CANCEL_FN, _P]),
+    "rh_provider_create": (_S, [C.POINTER(ProviderConfig), C.POINTER(C.c_void_p)]),
+    "rh_provider_destroy": (None, [_P]),
+    "rh_provider_metadata": (_S, [_P, _G, C.POINTER(SolverMetadata)]),
+    "rh_provider_supports": (_S, [_P, C.POINTER(PressureRequest), C.POINTER(C.c_char_p)]),
+    "rh_provider_predict": (_S, [_P, C.POINTER(PressureRequest), _P, C.c_size_t, CANCEL_FN, _P,
+                                 C.POINTER(C.c_char_p)]),
+    "rh_provider_workspace_create": (_S, [_G, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    "rh_provider_workspace_destroy": (None, [_P]),
+    "rh_provider_workspace_solve": (_S, [_P, C.POINTER(PressureRequest), C.POINTER(PressureSettings), _P,
+                                         CANCEL_FN, _P, C.POINTER(ProviderReport)]),
+    "rh_provider_workspace_pressure": (_S, [_P, _P, C.c_size_t]),
+    "rh_scene_create": (_S, [_G, C.POINTER(SceneConfig), C.POINTER(C.c_void_p)]),
+    "rh_scene_destroy": (None, [_P]),
+    "rh_scene_allocated_bytes": (C.c_uint64, [_P]),
+    "rh_scene_set_velocity": (_S, [_P, _P, _P, _P]),
+    "rh_scene_get_velocity": (_S, [_P, _P, _P, _P]),
+    "rh_scene_get_pressure": (_S, [_P, _P, C.c_size_t]),
+    "rh_scene_device_velocity": (_S, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    "rh_scene_step": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load librheon_amd.so (built by rheon_amd/csrc/Makefile)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeMissing(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+        try:
+            L = C.CDLL(LIB_PATH)
+        except OSError as e:  # pragma: no cover - depends on the host
+            raise NativeMissing(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/rheon_gpu.h."""
+    import re
+    path = os.path.join(os.path.dirname(_HERE), "include", "rheon_gpu.h")
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rh_[a-z0-9_]+)\s*\(", text)) - {"rh_cancel_fn"})

@@ -1,0 +1,807 @@
+// extern "C" entry points of include/rheon_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "runtime.hpp"
+
+using rh::DevGrid;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
+rh_status grid_ok(const rh_grid* g) {
+  if (!g) return RH_INVALID_ARGUMENT;
+  rh_status s = rh::validate_grid(g, nullptr);
+  if (s) return s;
+  return rh::check_indexable(g);
+}
+
+size_t cells_of(const rh_grid& g) { return (size_t)g.counts[0] * g.counts[1] * g.counts[2]; }
+size_t face_len(const rh_grid& g, int a) {
+  size_t f = 1;
+  for (int d = 0; d < 3; ++d) f *= g.counts[d] + (d == a ? 1 : 0);
+  return f;
+}
+size_t prec_bytes(rh_precision p) { return p == RH_F32 ? 4 : 8; }
+
+bool same_grid(const rh_grid& a, const rh_grid& b) {
+  for (int d = 0; d < 3; ++d)
+    if (a.counts[d] != b.counts[d] || a.spacing[d] != b.spacing[d] || a.origin[d] != b.origin[d])
+      return false;
+  return true;
+}
+
+void copy_str(char* dst, size_t n, const char* src) {
+  std::strncpy(dst, src, n - 1);
+  dst[n - 1] = '\0';
+}
+
+const char* method_id(rh_method m, rh_precision p) {
+  switch (m) {
+    case RH_JACOBI_PCG: return p == RH_F64 ? "jacobi-pcg-v1" : "jacobi-pcg-f32";
+    case RH_RBSGS_PCG: return p == RH_F64 ? "rbsgs-pcg" : "rbsgs-pcg-f32";
+    case RH_MG_PCG: return p == RH_F64 ? "mg-pcg" : "mg-pcg-f32";
+    case RH_MIXED_MG: return "mixed-mg-dc";
+  }
+  return "unknown";
+}
+
+template <typename F>
+rh_status dispatch_precision(rh_precision p, F f) {
+  if (p == RH_F64) return f((double*)nullptr);
+  if (p == RH_F32) return f((float*)nullptr);
+  return RH_INVALID_ARGUMENT;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- basics
+
+extern "C" int32_t rh_abi_version(void) { return RH_ABI_VERSION; }
+extern "C" const char* rh_last_error(void) { return rh::last_error(); }
+
+extern "C" rh_status rh_device_info(int32_t* count, char* arch, size_t arch_len) {
+  int n = 0;
+  RH_HIP(hipGetDeviceCount(&n));
+  if (count) *count = n;
+  if (arch && arch_len) {
+    hipDeviceProp_t prop;
+    RH_HIP(hipGetDeviceProperties(&prop, 0));
+    copy_str(arch, arch_len, prop.gcnArchName);
+  }
+  return RH_OK;
+}
+
+extern "C" rh_status rh_device_alloc(size_t bytes, void** out) {
+  if (!out) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  RH_HIP(hipMalloc(out, bytes ? bytes : 1));
+  return RH_OK;
+}
+extern "C" rh_status rh_device_free(void* ptr) {
+  if (ptr) RH_HIP(hipFree(ptr));
+  return RH_OK;
+}
+extern "C" rh_status rh_copy_to_device(void* dst, const void* src, size_t bytes) {
+  RH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return RH_OK;
+}
+extern "C" rh_status rh_copy_to_host(void* dst, const void* src, size_t bytes) {
+  RH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return RH_OK;
+}
+extern "C" rh_status rh_device_synchronize(void) {
+  RH_HIP(hipDeviceSynchronize());
+  return RH_OK;
+}
+
+extern "C" rh_status rh_grid_validate(const rh_grid* grid, uint64_t* cells) {
+  size_t n = 0;
+  rh_status s = rh::validate_grid(grid, &n);
+  if (!s && cells) *cells = n;
+  return s;
+}
+
+// BufferPlan::for_grid (src/geometry.rs:284-315).
+extern "C" rh_status rh_buffer_plan(const rh_grid* grid, uint64_t* velocity_pair,
+                                    uint64_t* tracer_pair, uint64_t* pressure_workspace,
+                                    uint64_t* total) {
+  size_t n = 0;
+  rh_status s = rh::validate_grid(grid, &n);
+  if (s) return s;
+  const uint64_t faces = face_len(*grid, 0) + face_len(*grid, 1) + face_len(*grid, 2);
+  const uint64_t v = faces * 8, t = (uint64_t)n * 8, p = (uint64_t)n * 48;
+  if (velocity_pair) *velocity_pair = v;
+  if (tracer_pair) *tracer_pair = t;
+  if (pressure_workspace) *pressure_workspace = p;
+  if (total) *total = v + t + p;
+  return RH_OK;
+}
+
+// ------------------------------------------------------- operator kernels
+
+extern "C" rh_status rh_apply_full(const rh_grid* grid, double density, const double* d_x,
+                                   double* d_y) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_x || !d_y) return RH_INVALID_ARGUMENT;
+  if ((s = rh::check_density(*grid, density))) return s;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, density);
+  if ((s = sc.clear_flags())) return s;
+  rh::launch_apply_full(g, d_x, d_y, sc.flags, sc.stream);
+  RH_HIP(hipGetLastError());
+  unsigned f = 0;
+  if ((s = sc.read_flags(&f))) return s;
+  return rh::flags_to_status(f);
+}
+
+extern "C" rh_status rh_advect_velocity(const rh_grid* grid, rh_precision precision,
+                                        const void* d_u, const void* d_v, const void* d_w,
+                                        double dt, void* d_out_u, void* d_out_v, void* d_out_w) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_u || !d_v || !d_w || !d_out_u || !d_out_v || !d_out_w) return RH_INVALID_ARGUMENT;
+  if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, 1.0);
+  if ((s = sc.clear_flags())) return s;
+  s = dispatch_precision(precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_validate<T>(g, d_u, d_v, d_w, 0, sc.flags, sc.stream);
+    rh::launch_advect<T>(g, d_u, d_v, d_w, dt, d_out_u, d_out_v, d_out_w, sc.flags, sc.stream);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipGetLastError());
+  unsigned f = 0;
+  if ((s = sc.read_flags(&f))) return s;
+  return rh::flags_to_status(f);
+}
+
+extern "C" rh_status rh_advect_tracer(const rh_grid* grid, const float* d_old, const float* d_u,
+                                      const float* d_v, const float* d_w, double dt,
+                                      float* d_out) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_old || !d_u || !d_v || !d_w || !d_out) return RH_INVALID_ARGUMENT;
+  if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, 1.0);
+  if ((s = sc.clear_flags())) return s;
+  rh::launch_validate<float>(g, d_u, d_v, d_w, 0, sc.flags, sc.stream);
+  rh::launch_advect_tracer(g, d_old, d_u, d_v, d_w, dt, d_out, sc.flags, sc.stream);
+  RH_HIP(hipGetLastError());
+  unsigned f = 0;
+  if ((s = sc.read_flags(&f))) return s;
+  return rh::flags_to_status(f);
+}
+
+extern "C" rh_status rh_build_rhs(const rh_grid* grid, rh_precision precision, const void* d_u,
+                                  const void* d_v, const void* d_w, double dt, double* d_rhs) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_u || !d_v || !d_w || !d_rhs) return RH_INVALID_ARGUMENT;
+  if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, 1.0);
+  if ((s = sc.clear_flags())) return s;
+  s = dispatch_precision(precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_validate<T>(g, d_u, d_v, d_w, 1, sc.flags, sc.stream);
+    rh::launch_rhs<T>(g, d_u, d_v, d_w, dt, d_rhs, sc.flags, sc.stream);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipGetLastError());
+  unsigned f = 0;
+  if ((s = sc.read_flags(&f))) return s;
+  return rh::flags_to_status(f);
+}
+
+extern "C" rh_status rh_correct_velocity(const rh_grid* grid, rh_precision precision,
+                                         double density, const double* d_pressure, double dt,
+                                         void* d_u, void* d_v, void* d_w) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_pressure || !d_u || !d_v || !d_w) return RH_INVALID_ARGUMENT;
+  if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
+  if ((s = rh::check_density(*grid, density))) return s;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, density);
+  if ((s = sc.clear_flags())) return s;
+  s = dispatch_precision(precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_validate<T>(g, d_u, d_v, d_w, 1, sc.flags, sc.stream);
+    return RH_OK;
+  });
+  if (s) return s;
+  unsigned f = 0;
+  if ((s = sc.read_flags(&f))) return s;
+  if (f) return rh::flags_to_status(f);
+  s = dispatch_precision(precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_correct<T>(g, d_pressure, dt, d_u, d_v, d_w, sc.flags, sc.stream);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipGetLastError());
+  if ((s = sc.read_flags(&f))) return s;
+  return rh::flags_to_status(f);
+}
+
+extern "C" rh_status rh_divergence_max(const rh_grid* grid, rh_precision precision,
+                                       const void* d_u, const void* d_v, const void* d_w,
+                                       double* out_max) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_u || !d_v || !d_w || !out_max) return RH_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, 1.0);
+  if ((s = sc.clear_flags())) return s;
+  s = dispatch_precision(precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_validate<T>(g, d_u, d_v, d_w, 1, sc.flags, sc.stream);
+    rh::launch_divergence_max<T>(g, d_u, d_v, d_w, sc.reduce(0), sc.stream);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipGetLastError());
+  unsigned f = 0;
+  if ((s = sc.read_flags(&f))) return s;
+  if (f) return rh::flags_to_status(f);
+  return sc.read_scalars(out_max, 1, 0);
+}
+
+extern "C" rh_status rh_velocity_rate(const rh_grid* grid, rh_precision precision,
+                                      const void* d_u, const void* d_v, const void* d_w,
+                                      double* out_rate) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_u || !d_v || !d_w || !out_rate) return RH_INVALID_ARGUMENT;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, 1.0);
+  s = dispatch_precision(precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_velocity_rate<T>(g, d_u, d_v, d_w, sc.reduce(0), sc.stream);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipGetLastError());
+  return sc.read_scalars(out_rate, 1, 0);
+}
+
+extern "C" rh_status rh_precondition(const rh_grid* grid, double density, rh_method method,
+                                     double* d_r, double* d_z, double* out_rz) {
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (!d_r || !d_z) return RH_INVALID_ARGUMENT;
+  if (method != RH_JACOBI_PCG && method != RH_RBSGS_PCG) return RH_UNSUPPORTED;
+  if ((s = rh::check_density(*grid, density))) return s;
+  std::unique_lock<std::mutex> lock;
+  rh::Scratch& sc = rh::global_scratch(lock);
+  if ((s = sc.init())) return s;
+  const DevGrid g = rh::make_dev_grid(*grid, density);
+  RH_HIP(hipMemsetAsync(sc.ctl, 0, sizeof(rh::PcgControl), sc.stream));
+  rh::launch_precondition(g, method, d_r, d_z, sc.ctl, sc.pdd, sc.pmx, sc.stream);
+  RH_HIP(hipGetLastError());
+  rh::PcgControl c;
+  RH_HIP(hipMemcpyAsync(sc.host, sc.ctl, sizeof(c), hipMemcpyDeviceToHost, sc.stream));
+  RH_HIP(hipStreamSynchronize(sc.stream));
+  std::memcpy(&c, sc.host, sizeof(c));
+  if (out_rz) *out_rz = c.rz[1];
+  return RH_OK;
+}
+
+// ------------------------------------------------------- device solver
+
+struct rh_solver {
+  rh::PcgSolver impl;
+};
+
+extern "C" rh_status rh_solver_create(const rh_grid* grid, const rh_solver_config* config,
+                                      rh_solver** out) {
+  if (!out || !config) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  auto* solver = new (std::nothrow) rh_solver();
+  if (!solver) return RH_DEVICE_ERROR;
+  if ((s = solver->impl.create(*grid, *config))) {
+    delete solver;
+    return s;
+  }
+  *out = solver;
+  return RH_OK;
+}
+
+extern "C" void rh_solver_destroy(rh_solver* solver) { delete solver; }
+
+extern "C" uint64_t rh_solver_allocated_bytes(const rh_solver* solver) {
+  return solver ? solver->impl.allocated_bytes() : 0;
+}
+
+extern "C" rh_status rh_solver_solve_rhs(rh_solver* solver, double density, const double* d_rhs,
+                                         double dt, const rh_pressure_settings* settings,
+                                         double* d_pressure, rh_pressure_report* report,
+                                         rh_cancel_fn cancel, void* cancel_ctx) {
+  if (!solver || !settings) return RH_INVALID_ARGUMENT;
+  return solver->impl.solve(density, d_rhs, dt, *settings, d_pressure, report, cancel, cancel_ctx);
+}
+
+// ------------------------------------------------------------- provider
+
+struct rh_provider {
+  rh_provider_config config{};
+  std::unique_ptr<rh_solver> solver;
+  rh_grid grid{};
+  double* d_rhs = nullptr;
+  double* d_p = nullptr;
+  size_t cap = 0;
+  ~rh_provider() {
+    if (d_rhs) (void)hipFree(d_rhs);
+    if (d_p) (void)hipFree(d_p);
+  }
+};
+
+extern "C" rh_status rh_provider_create(const rh_provider_config* config, rh_provider** out) {
+  if (!config || !out) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto* p = new (std::nothrow) rh_provider();
+  if (!p) return RH_DEVICE_ERROR;
+  p->config = *config;
+  *out = p;
+  return RH_OK;
+}
+
+extern "C" void rh_provider_destroy(rh_provider* provider) { delete provider; }
+
+extern "C" rh_status rh_provider_metadata(const rh_provider* provider, const rh_grid* grid,
+                                          rh_solver_metadata* out) {
+  if (!provider || !out) return RH_INVALID_ARGUMENT;
+  std::memset(out, 0, sizeof(*out));
+  copy_str(out->id, sizeof(out->id),
+           method_id(provider->config.solver.method, provider->config.solver.precision));
+  copy_str(out->version, sizeof(out->version), "1");
+  copy_str(out->stage, sizeof(out->stage), "pressure");
+  if (grid)
+    for (int d = 0; d < 3; ++d) out->shape_xyz[d] = grid->counts[d];
+  copy_str(out->layout, sizeof(out->layout), "x-fastest-cell-centered");
+  copy_str(out->scalar, sizeof(out->scalar), "f64");
+  copy_str(out->units_in, sizeof(out->units_in), "m^3/s^2");
+  copy_str(out->units_out, sizeof(out->units_out), "Pa");
+  copy_str(out->domain, sizeof(out->domain), "sealed-constant-density-mac-box");
+  return RH_OK;
+}
+
+static rh_status provider_supports(const rh_provider* provider, const rh_pressure_request* req,
+                                   const char** reason) {
+  auto why = [&](const char* r) {
+    if (reason) *reason = r;
+    return RH_UNSUPPORTED;
+  };
+  if (reason) *reason = nullptr;
+  if (!provider || !req || !req->grid) return RH_INVALID_ARGUMENT;
+  if (rh::validate_grid(req->grid, nullptr)) return why("invalid grid geometry");
+  if (rh::check_indexable(req->grid)) return why("grid exceeds 32-bit device indexing");
+  if (rh::check_density(*req->grid, req->density)) return why("density must be finite and positive");
+  if (!std::isfinite(req->dt) || req->dt <= 0.0) return why("time step must be finite and positive");
+  if (req->rhs_len != cells_of(*req->grid)) return why("rhs length does not match the grid");
+  const rh_method m = provider->config.solver.method;
+  if (m != RH_JACOBI_PCG && m != RH_RBSGS_PCG) return why("method not available in this build");
+  return RH_OK;
+}
+
+extern "C" rh_status rh_provider_supports(const rh_provider* provider,
+                                          const rh_pressure_request* request, const char** reason) {
+  return provider_supports(provider, request, reason);
+}
+
+extern "C" rh_status rh_provider_predict(rh_provider* provider, const rh_pressure_request* req,
+                                         double* pressure, size_t pressure_len, rh_cancel_fn cancel,
+                                         void* cancel_ctx, const char** reason) {
+  if (reason) *reason = nullptr;
+  rh_status s = provider_supports(provider, req, reason);
+  if (s) return s;
+  if (!pressure || pressure_len != req->rhs_len || !req->rhs) return RH_INVALID_ARGUMENT;
+  const size_t n = req->rhs_len;
+  auto exec = [&](const char* r) {
+    if (reason) *reason = r;
+    return RH_EXECUTION;
+  };
+  if (!provider->solver || !same_grid(provider->grid, *req->grid)) {
+    provider->solver.reset();
+    rh_solver* raw = nullptr;
+    if (rh_solver_create(req->grid, &provider->config.solver, &raw)) return exec("device solver creation failed");
+    provider->solver.reset(raw);
+    provider->grid = *req->grid;
+  }
+  if (provider->cap < n) {
+    if (provider->d_rhs) (void)hipFree(provider->d_rhs);
+    if (provider->d_p) (void)hipFree(provider->d_p);
+    provider->d_rhs = provider->d_p = nullptr;
+    provider->cap = 0;
+    if (hipMalloc(&provider->d_rhs, n * 8) != hipSuccess || hipMalloc(&provider->d_p, n * 8) != hipSuccess)
+      return exec("device allocation failed");
+    provider->cap = n;
+  }
+  if (hipMemcpy(provider->d_rhs, req->rhs, n * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return exec("host-to-device copy failed");
+  rh_pressure_report rep;
+  s = rh_solver_solve_rhs(provider->solver.get(), req->density, provider->d_rhs, req->dt,
+                          &provider->config.settings, provider->d_p, &rep, cancel, cancel_ctx);
+  if (s == RH_CANCELLED) return RH_CANCELLED;
+  if (s != RH_OK) return exec("device pressure solve did not converge");
+  if (hipMemcpy(pressure, provider->d_p, n * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return exec("device-to-host copy failed");
+  return RH_OK;
+}
+
+// ---------------------------------------------------- provider workspace
+
+struct rh_provider_workspace {
+  rh_grid grid{};
+  rh_method fallback = RH_JACOBI_PCG;
+  std::unique_ptr<rh_solver> solver;  // native fallback on the device
+  double* d_rhs = nullptr;
+  double* d_p = nullptr;
+  std::vector<double> pressure;  // last result (host)
+  ~rh_provider_workspace() {
+    if (d_rhs) (void)hipFree(d_rhs);
+    if (d_p) (void)hipFree(d_p);
+  }
+};
+
+extern "C" rh_status rh_provider_workspace_create(const rh_grid* grid, uint64_t limit,
+                                                  rh_method fallback,
+                                                  rh_provider_workspace** out) {
+  if (!out) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if (fallback != RH_JACOBI_PCG && fallback != RH_RBSGS_PCG) return RH_UNSUPPORTED;
+  const size_t n = cells_of(*grid);
+  // ProviderWorkspace::new (src/solver_provider.rs:105-144): 16 N extra plus
+  // the 48 N reference workspace must fit the limit.
+  if (limit && (uint64_t)n * 64 > limit) return RH_BUFFER_LIMIT;
+  auto ws = std::unique_ptr<rh_provider_workspace>(new (std::nothrow) rh_provider_workspace());
+  if (!ws) return RH_DEVICE_ERROR;
+  ws->grid = *grid;
+  ws->fallback = fallback;
+  rh_solver_config cfg{};
+  cfg.method = fallback;
+  cfg.precision = RH_F64;
+  rh_solver* raw = nullptr;
+  if ((s = rh_solver_create(grid, &cfg, &raw))) return s;
+  ws->solver.reset(raw);
+  RH_HIP(hipMalloc(&ws->d_rhs, n * 8));
+  RH_HIP(hipMalloc(&ws->d_p, n * 8));
+  try {
+    ws->pressure.assign(n, 0.0);
+  } catch (...) {
+    return RH_BUFFER_LIMIT;
+  }
+  *out = ws.release();
+  return RH_OK;
+}
+
+extern "C" void rh_provider_workspace_destroy(rh_provider_workspace* ws) { delete ws; }
+
+extern "C" rh_status rh_provider_workspace_pressure(const rh_provider_workspace* ws, double* out,
+                                                    size_t len) {
+  if (!ws || !out || len != ws->pressure.size()) return RH_INVALID_ARGUMENT;
+  std::memcpy(out, ws->pressure.data(), len * 8);
+  return RH_OK;
+}
+
+// ProviderWorkspace::solve_observed (src/solver_provider.rs:160-327).
+extern "C" rh_status rh_provider_workspace_solve(rh_provider_workspace* ws,
+                                                 const rh_pressure_request* req,
+                                                 const rh_pressure_settings* settings,
+                                                 rh_provider* preferred, rh_cancel_fn cancel,
+                                                 void* cancel_ctx, rh_provider_report* report) {
+  const auto start = Clock::now();
+  if (!ws || !req || !req->grid || !settings || !report) return RH_INVALID_ARGUMENT;
+  std::memset(report, 0, sizeof(*report));
+  auto cancelled = [&]() { return cancel && cancel(cancel_ctx); };
+  if (!same_grid(*req->grid, ws->grid)) return RH_INVALID_ARGUMENT;  // LengthMismatch
+  rh_status s = rh::check_density(ws->grid, req->density);
+  if (s) return s;
+  const size_t n = ws->pressure.size();
+  if (!req->rhs || req->rhs_len != n) return RH_INVALID_ARGUMENT;
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(req->rhs[i])) return RH_NONFINITE_INPUT;
+  if (!std::isfinite(req->dt) || req->dt <= 0.0 || !std::isfinite(settings->relative_residual) ||
+      settings->relative_residual < 0.0 || !std::isfinite(settings->absolute_residual) ||
+      settings->absolute_residual < 0.0 || !std::isfinite(settings->divergence_limit) ||
+      settings->divergence_limit < 0.0)
+    return RH_INVALID_SETTINGS;
+  rh::Scratch& sc = ws->solver->impl.scratch();
+  const DevGrid g = rh::make_dev_grid(ws->grid, req->density);
+  // Compatibility sums and threshold on the device (:186-206).
+  RH_HIP(hipMemcpyAsync(ws->d_rhs, req->rhs, n * 8, hipMemcpyHostToDevice, sc.stream));
+  rh::PcgControl h{};
+  h.rel = settings->relative_residual;
+  h.abs_tol = settings->absolute_residual;
+  std::memcpy(sc.host, &h, sizeof(h));
+  RH_HIP(hipMemcpyAsync(sc.ctl, sc.host, sizeof(h), hipMemcpyHostToDevice, sc.stream));
+  rh::launch_rhs_sums(g, ws->d_rhs, sc.ctl, sc.pdd, sc.pmx, sc.stream);
+  RH_HIP(hipMemcpyAsync(sc.host, sc.ctl, sizeof(h), hipMemcpyDeviceToHost, sc.stream));
+  RH_HIP(hipStreamSynchronize(sc.stream));
+  std::memcpy(&h, sc.host, sizeof(h));
+  if (h.done) {
+    report->accepted.error_value = h.error_value;
+    return (rh_status)h.status;
+  }
+  const double threshold = h.threshold;
+  if (cancelled()) return RH_CANCELLED;
+  copy_str(report->selected, sizeof(report->selected), method_id(ws->fallback, RH_F64));
+  if (preferred) {
+    report->attempted = 1;
+    rh_solver_metadata meta;
+    rh_provider_metadata(preferred, &ws->grid, &meta);
+    const char* reason = nullptr;
+    rh_status attempt = rh_provider_supports(preferred, req, &reason);
+    if (attempt == RH_OK) {
+      std::fill(ws->pressure.begin(), ws->pressure.end(), std::numeric_limits<double>::quiet_NaN());
+      const auto t0 = Clock::now();
+      attempt = rh_provider_predict(preferred, req, ws->pressure.data(), n, cancel, cancel_ctx, &reason);
+      report->inference = seconds_since(t0);
+    }
+    if (cancelled() || attempt == RH_CANCELLED) return RH_CANCELLED;
+    const auto t1 = Clock::now();
+    rh_status qualified = attempt;
+    if (qualified == RH_OK) {
+      for (size_t i = 0; i < n && qualified == RH_OK; ++i)
+        if (!std::isfinite(ws->pressure[i])) qualified = RH_NONFINITE_OUTPUT;
+      if (qualified == RH_OK && ws->pressure[0] != 0.0) qualified = RH_NONZERO_GAUGE;
+    }
+    if (qualified == RH_OK) {
+      RH_HIP(hipMemcpyAsync(ws->d_p, ws->pressure.data(), n * 8, hipMemcpyHostToDevice, sc.stream));
+      rh::launch_residual_report(g, ws->d_p, ws->d_rhs, sc.reduce(0), sc.stream);
+      RH_HIP(hipGetLastError());
+      double out[3];
+      if ((s = sc.read_scalars(out, 3, 0))) return s;
+      if (out[2] != 0.0) {
+        qualified = RH_NONFINITE_OUTPUT;
+      } else {
+        const double l2 = std::sqrt(out[0]);
+        const double maximum = out[1];
+        const double divergence = maximum * req->dt / g.volume;
+        report->has_candidate = 1;
+        report->candidate.iterations = 0;
+        report->candidate.true_residual_l2 = l2;
+        report->candidate.true_residual_max = maximum;
+        report->candidate.predicted_divergence_max = divergence;
+        report->candidate.threshold = threshold;
+        if (!std::isfinite(l2) || !std::isfinite(divergence) || l2 > threshold ||
+            divergence > settings->divergence_limit)
+          qualified = RH_QUALITY_GATE;
+      }
+    }
+    report->qualification = seconds_since(t1);
+    if (cancelled()) return RH_CANCELLED;
+    if (qualified == RH_OK) {
+      copy_str(report->selected, sizeof(report->selected), meta.id);
+      report->accepted = report->candidate;
+      report->total = seconds_since(start);
+      return RH_OK;
+    }
+    report->rejection = qualified;
+  }
+  // Native fallback: the device jacobi-pcg-v1 (:310-318).
+  const auto t2 = Clock::now();
+  s = rh_solver_solve_rhs(ws->solver.get(), req->density, ws->d_rhs, req->dt, settings, ws->d_p,
+                          &report->accepted, cancel, cancel_ctx);
+  if (s != RH_OK) return s;
+  RH_HIP(hipMemcpy(ws->pressure.data(), ws->d_p, n * 8, hipMemcpyDeviceToHost));
+  report->fallback = seconds_since(t2);
+  report->total = seconds_since(start);
+  return RH_OK;
+}
+
+// ----------------------------------------------------------------- scene
+
+struct rh_scene {
+  rh_grid grid{};
+  rh_scene_config config{};
+  size_t n = 0, flen[3] = {0, 0, 0};
+  void* acc[3] = {nullptr, nullptr, nullptr};
+  void* cand[3] = {nullptr, nullptr, nullptr};
+  double* rhs = nullptr;
+  double* p = nullptr;
+  void* arena = nullptr;
+  uint64_t bytes = 0;
+  std::unique_ptr<rh_solver> solver;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  ~rh_scene() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (arena) (void)hipFree(arena);
+  }
+};
+
+extern "C" rh_status rh_scene_create(const rh_grid* grid, const rh_scene_config* config,
+                                     rh_scene** out) {
+  if (!config || !out) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  rh_status s = grid_ok(grid);
+  if (s) return s;
+  if ((s = rh::check_density(*grid, config->density))) return s;
+  if (config->precision != RH_F64 && config->precision != RH_F32) return RH_INVALID_ARGUMENT;
+  auto sc = std::unique_ptr<rh_scene>(new (std::nothrow) rh_scene());
+  if (!sc) return RH_DEVICE_ERROR;
+  sc->grid = *grid;
+  sc->config = *config;
+  sc->n = cells_of(*grid);
+  const size_t fb = prec_bytes(config->precision);
+  size_t total = 0;
+  auto round = [](size_t b) { return (b + 255) / 256 * 256; };
+  for (int a = 0; a < 3; ++a) {
+    sc->flen[a] = face_len(*grid, a);
+    total += 2 * round(sc->flen[a] * fb);
+  }
+  total += 2 * round(sc->n * 8);
+  RH_HIP(hipMalloc(&sc->arena, total));
+  RH_HIP(hipMemset(sc->arena, 0, total));
+  char* p = (char*)sc->arena;
+  for (int a = 0; a < 3; ++a) {
+    sc->acc[a] = p;
+    p += round(sc->flen[a] * fb);
+    sc->cand[a] = p;
+    p += round(sc->flen[a] * fb);
+  }
+  sc->rhs = (double*)p;
+  p += round(sc->n * 8);
+  sc->p = (double*)p;
+  rh_solver* raw = nullptr;
+  if ((s = rh_solver_create(grid, &config->solver, &raw))) return s;
+  sc->solver.reset(raw);
+  for (auto& e : sc->ev) RH_HIP(hipEventCreate(&e));
+  sc->bytes = total + raw->impl.allocated_bytes();
+  if (config->solver.memory_limit && sc->bytes > config->solver.memory_limit) return RH_BUFFER_LIMIT;
+  *out = sc.release();
+  return RH_OK;
+}
+
+extern "C" void rh_scene_destroy(rh_scene* scene) { delete scene; }
+
+extern "C" uint64_t rh_scene_allocated_bytes(const rh_scene* scene) {
+  return scene ? scene->bytes : 0;
+}
+
+extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const void* v,
+                                           const void* w) {
+  if (!sc || !u || !v || !w) return RH_INVALID_ARGUMENT;
+  const void* src[3] = {u, v, w};
+  const size_t fb = prec_bytes(sc->config.precision);
+  for (int a = 0; a < 3; ++a)
+    RH_HIP(hipMemcpy(sc->acc[a], src[a], sc->flen[a] * fb, hipMemcpyHostToDevice));
+  return RH_OK;
+}
+
+extern "C" rh_status rh_scene_get_velocity(const rh_scene* sc, void* u, void* v, void* w) {
+  if (!sc || !u || !v || !w) return RH_INVALID_ARGUMENT;
+  void* dst[3] = {u, v, w};
+  const size_t fb = prec_bytes(sc->config.precision);
+  for (int a = 0; a < 3; ++a)
+    RH_HIP(hipMemcpy(dst[a], sc->acc[a], sc->flen[a] * fb, hipMemcpyDeviceToHost));
+  return RH_OK;
+}
+
+extern "C" rh_status rh_scene_get_pressure(const rh_scene* sc, double* out, size_t len) {
+  if (!sc || !out || len != sc->n) return RH_INVALID_ARGUMENT;
+  RH_HIP(hipMemcpy(out, sc->p, len * 8, hipMemcpyDeviceToHost));
+  return RH_OK;
+}
+
+extern "C" rh_status rh_scene_device_velocity(rh_scene* sc, void** d_u, void** d_v, void** d_w) {
+  if (!sc || !d_u || !d_v || !d_w) return RH_INVALID_ARGUMENT;
+  *d_u = sc->acc[0];
+  *d_v = sc->acc[1];
+  *d_w = sc->acc[2];
+  return RH_OK;
+}
+
+// Simulation::prepare_impl carrier sequence (src/simulation.rs:988-1277):
+// advect -> build_rhs -> pressure solve -> correct_candidate_in_place ->
+// actual divergence gate -> Courant gate -> commit swap (:805-811).
+extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* report,
+                                   rh_cancel_fn cancel, void* cancel_ctx) {
+  if (!sc) return RH_INVALID_ARGUMENT;
+  rh_step_report local;
+  rh_step_report* rep = report ? report : &local;
+  std::memset(rep, 0, sizeof(*rep));
+  rep->dt = dt;
+  if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
+  if (cancel && cancel(cancel_ctx)) return RH_CANCELLED;
+  rh::Scratch& scr = sc->solver->impl.scratch();
+  hipStream_t st = scr.stream;
+  const DevGrid g = rh::make_dev_grid(sc->grid, sc->config.density);
+  const rh_precision prec = sc->config.precision;
+  rh_status s = scr.clear_flags();
+  if (s) return s;
+  RH_HIP(hipEventRecord(sc->ev[0], st));
+  s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_validate<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], 0, scr.flags, st);
+    rh::launch_advect<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], dt, sc->cand[0], sc->cand[1],
+                         sc->cand[2], scr.flags, st);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipEventRecord(sc->ev[1], st));
+  s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_validate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], 1, scr.flags, st);
+    rh::launch_rhs<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], dt, sc->rhs, scr.flags, st);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipEventRecord(sc->ev[2], st));
+  RH_HIP(hipGetLastError());
+  unsigned f = 0;
+  if ((s = scr.read_flags(&f))) return s;
+  if (f) return rh::flags_to_status(f);
+  s = rh_solver_solve_rhs(sc->solver.get(), sc->config.density, sc->rhs, dt, &sc->config.pressure,
+                          sc->p, &rep->pressure, cancel, cancel_ctx);
+  if (s) return s;
+  RH_HIP(hipEventRecord(sc->ev[3], st));
+  s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_correct<T>(g, sc->p, dt, sc->cand[0], sc->cand[1], sc->cand[2], scr.flags, st);
+    rh::launch_divergence_max<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(0), st);
+    rh::launch_velocity_rate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(1), st);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipEventRecord(sc->ev[4], st));
+  RH_HIP(hipGetLastError());
+  if ((s = scr.read_flags(&f))) return s;
+  if (f) return rh::flags_to_status(f);
+  double sv[2];
+  if ((s = scr.read_scalars(sv, 2, 0))) return s;
+  float ms[4] = {0, 0, 0, 0}, total = 0;
+  for (int i = 0; i < 4; ++i) RH_HIP(hipEventElapsedTime(&ms[i], sc->ev[i], sc->ev[i + 1]));
+  RH_HIP(hipEventElapsedTime(&total, sc->ev[0], sc->ev[4]));
+  rep->advect_seconds = ms[0] * 1e-3;
+  rep->rhs_seconds = ms[1] * 1e-3;
+  rep->solve_seconds = ms[2] * 1e-3;
+  rep->correct_seconds = ms[3] * 1e-3;
+  rep->seconds = total * 1e-3;
+  rep->actual_divergence_max = sv[0];
+  rep->courant = dt * sv[1];
+  if (sv[0] > sc->config.actual_divergence_limit) return RH_DIVERGENCE_LIMIT;
+  if (sc->config.max_courant > 0.0 && (!std::isfinite(rep->courant) || rep->courant > sc->config.max_courant))
+    return RH_COURANT_LIMIT;
+  for (int a = 0; a < 3; ++a) std::swap(sc->acc[a], sc->cand[a]);
+  return RH_OK;
+}

@@ -1,0 +1,214 @@
+// Shared device/host definitions for the rheon_amd HIP kernels (gfx950).
+//
+// Layout: every cell array is x-fastest, index i + nx*(j + ny*k); face array
+// of axis a has counts n + e_a and the same x-fastest order
+// (src/geometry.rs:250-258).  Kernels walk "rows" (fixed j,k) with 64 lanes
+// along x, so every wave load is one contiguous, coalesced x segment.
+//
+// All code is built with -ffp-contract=off: each multiply and add rounds
+// separately, as in the reference, so the f64 paths are bit-comparable.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/rheon_gpu.h"
+
+namespace rh {
+
+constexpr int kLanesX = 64;        // lanes along x (one wavefront)
+constexpr int kRowsPerBlock = 4;   // rows (j,k) per block: 4 waves = 256 threads
+constexpr int kBlock = kLanesX * kRowsPerBlock;
+constexpr int kMaxBlocks = 2048;   // persistent-style grid cap (8 blocks per CU)
+
+// Validated box geometry and operator constants, passed by value.
+struct DevGrid {
+  int nx, ny, nz;
+  int pad;
+  double h[3];
+  double origin[3];
+  double upper[3];
+  double volume;
+  double area[3];  // face areas (src/operator.rs:63)
+  double acc[3];   // 1/(rho h_d)   (src/operator.rs:64)
+  double w[3];     // area*acc      (src/operator.rs:65)
+  double diag[64]; // cached boundary diagonals by neighbour mask (src/operator.rs:81-91)
+};
+
+// Double-double accumulator.  Summing rounded products with error-free
+// TwoSum gives (almost always) the correctly rounded sum, which is also what
+// the reference's serial Neumaier loop returns (src/pressure.rs:135-161).
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ void dd_acc(DD& x, double v) {
+  double s, e;
+  two_sum(x.hi, v, s, e);
+  x.hi = s;
+  x.lo += e;
+}
+__device__ __forceinline__ DD dd_add(DD x, DD y) {
+  double s, e;
+  two_sum(x.hi, y.hi, s, e);
+  e += x.lo + y.lo;
+  double h = s + e;
+  return DD{h, e - (h - s)};
+}
+__device__ __forceinline__ double dd_value(DD x) { return x.hi + x.lo; }
+
+__device__ __forceinline__ DD shfl_xor_dd(DD v, int m) {
+  return DD{__shfl_xor(v.hi, m, 64), __shfl_xor(v.lo, m, 64)};
+}
+
+// Reduce NV double-double values across the block (fixed order: butterfly in
+// the wave, then waves 0..3 in order).  Result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_reduce_dd(DD (&v)[NV]) {
+  __shared__ DD lds[NV][kRowsPerBlock];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = dd_add(v[q], shfl_xor_dd(v[q], m));
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) lds[q][wave] = v[q];
+  __syncthreads();
+  if (lane == 0 && wave == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      DD a = lds[q][0];
+      for (int w = 1; w < kRowsPerBlock; ++w) a = dd_add(a, lds[q][w]);
+      v[q] = a;
+    }
+  }
+  __syncthreads();
+}
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_max(double (&v)[NV]) {
+  __shared__ double lds[NV][kRowsPerBlock];
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = fmax(v[q], __shfl_xor(v[q], m, 64));
+  const int lane = threadIdx.x, wave = threadIdx.y;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) lds[q][wave] = v[q];
+  __syncthreads();
+  if (lane == 0 && wave == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double a = lds[q][0];
+      for (int w = 1; w < kRowsPerBlock; ++w) a = fmax(a, lds[q][w]);
+      v[q] = a;
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int flat_tid() { return threadIdx.y * kLanesX + threadIdx.x; }
+
+// "Last block done" grid reduction: each block publishes its partials, and
+// the block that arrives last combines all partials in block-index order
+// (deterministic for a given launch shape) and runs `fin(sums, maxima)` in
+// thread 0.  `counter` must be zero on entry; the last block resets it.
+// ND double-double sums and NM maxima (both >= 1; pass dummies if unused).
+template <int ND, int NM, typename Fin>
+__device__ __forceinline__ void grid_finish(DD (&dd)[ND], double (&mx)[NM], DD* __restrict__ pdd,
+                                            double* __restrict__ pmx, unsigned* counter, Fin fin) {
+  __shared__ bool last;
+  block_reduce_dd<ND>(dd);
+  block_reduce_max<NM>(mx);
+  if (flat_tid() == 0) {
+#pragma unroll
+    for (int q = 0; q < ND; ++q) pdd[q * kMaxBlocks + blockIdx.x] = dd[q];
+#pragma unroll
+    for (int q = 0; q < NM; ++q) pmx[q * kMaxBlocks + blockIdx.x] = mx[q];
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    last = prev == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const int t = flat_tid();
+  DD a[ND];
+  double m[NM];
+#pragma unroll
+  for (int q = 0; q < ND; ++q) a[q] = DD{0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < NM; ++q) m[q] = 0.0;
+  for (int b = t; b < (int)gridDim.x; b += kBlock) {
+#pragma unroll
+    for (int q = 0; q < ND; ++q) {
+      const volatile DD* src = pdd + q * kMaxBlocks + b;
+      a[q] = dd_add(a[q], DD{src->hi, src->lo});
+    }
+#pragma unroll
+    for (int q = 0; q < NM; ++q)
+      m[q] = fmax(m[q], *(const volatile double*)(pmx + q * kMaxBlocks + b));
+  }
+  block_reduce_dd<ND>(a);
+  block_reduce_max<NM>(m);
+  if (t == 0) {
+    *counter = 0u;
+    double sums[ND];
+#pragma unroll
+    for (int q = 0; q < ND; ++q) sums[q] = dd_value(a[q]);
+    fin(sums, m);
+  }
+}
+
+// Iterate the cells of rows [row_begin, row_end) of an (nx, ny*nz) lattice.
+// Each wave owns one row at a time; lanes stride along x.
+struct RowRange {
+  int begin, end;
+};
+
+// Contiguous row-group partition of a dim grid; block ids are remapped so the
+// 8 XCDs (round-robin dispatch) each own a contiguous slab of rows, which
+// keeps the j/k neighbours of a stencil inside one XCD's L2.
+__device__ __forceinline__ RowRange block_rows(int rows) {
+  const int nb = gridDim.x;
+  const int b = blockIdx.x;
+  int e = b;
+  if ((nb & 7) == 0) e = (b & 7) * (nb >> 3) + (b >> 3);
+  const int groups = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+  const int per = (groups + nb - 1) / nb;
+  int g0 = e * per, g1 = g0 + per;
+  if (g1 > groups) g1 = groups;
+  RowRange r;
+  r.begin = g0 * kRowsPerBlock;
+  r.end = g1 * kRowsPerBlock;
+  if (r.end > rows) r.end = rows;
+  if (r.begin > r.end) r.begin = r.end;
+  return r;
+}
+
+// Host helpers ---------------------------------------------------------------
+int grid_blocks(int rows);
+DevGrid make_dev_grid(const rh_grid& g, double density);
+rh_status validate_grid(const rh_grid* g, size_t* cells_out);
+
+}  // namespace rh
+
+#define RH_HIP(call)                                  \
+  do {                                                \
+    hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) {                           \
+      rh::set_last_error(hipGetErrorString(e_), #call); \
+      return RH_DEVICE_ERROR;                         \
+    }                                                 \
+  } while (0)
+
+namespace rh {
+void set_last_error(const char* what, const char* where);
+}

@@ -1,0 +1,396 @@
+// Face-field kernels of the projection step: RK2 semi-Lagrangian advection,
+// RHS (integrated divergence), pressure-gradient correction, divergence
+// maximum and velocity rate.  Each restates one reference routine with the
+// same operation order (cited per kernel); built with -ffp-contract=off.
+#include <cfloat>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace rh {
+
+// ---------------------------------------------------------------- sampling
+// ScalarSampler::sample (src/sampling.rs:74-122).  comp 0..2 = face lattice
+// of that axis, 3 = cell centres.  f64 arithmetic on T samples.
+template <typename T>
+__device__ __forceinline__ unsigned sample(const DevGrid& g, int comp, const T* __restrict__ vals,
+                                           const double pos[3], double& out) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+  int lo[3], hi[3], dims[3];
+  double t[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dims[d] = n[d] + (comp == d ? 1 : 0);
+    const double off = (comp == d) ? 0.0 : 0.5;
+    double x = pos[d];
+    if (x < g.origin[d]) x = g.origin[d];
+    if (x > g.upper[d]) x = g.upper[d];
+    double q = (x - g.origin[d]) / g.h[d] - off;
+    const double top = (double)(dims[d] - 1);
+    if (q < 0.0) q = 0.0;
+    if (q > top) q = top;
+    if (!isfinite(q)) return kFlagArithmetic;  // NaN survives the clamps
+    lo[d] = (int)floor(q);
+    hi[d] = lo[d] + 1 < dims[d] - 1 ? lo[d] + 1 : dims[d] - 1;
+    t[d] = q - (double)lo[d];
+  }
+  double result = 0.0, minimum = INFINITY, maximum = -INFINITY;
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int px = x ? hi[0] : lo[0];
+        const int py = y ? hi[1] : lo[1];
+        const int pz = z ? hi[2] : lo[2];
+        double weight = (x ? t[0] : 1.0 - t[0]);
+        weight = weight * (y ? t[1] : 1.0 - t[1]);
+        weight = weight * (z ? t[2] : 1.0 - t[2]);
+        const double value = (double)vals[px + dims[0] * (py + dims[1] * pz)];
+        minimum = fmin(minimum, value);
+        maximum = fmax(maximum, value);
+        result += weight * value;
+      }
+  if (!isfinite(result)) return kFlagArithmetic;
+  if (result < minimum) result = minimum;
+  if (result > maximum) result = maximum;
+  out = result;
+  return 0;
+}
+
+// VelocitySampler::backtrace / departure (src/sampling.rs:151-175).
+template <typename T>
+__device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restrict__ u,
+                                              const T* __restrict__ v, const T* __restrict__ w,
+                                              const double pos[3], double dt, double out[3]) {
+  double vel[3], mid[3];
+  unsigned f = sample<T>(g, 0, u, pos, vel[0]) | sample<T>(g, 1, v, pos, vel[1]) |
+               sample<T>(g, 2, w, pos, vel[2]);
+  if (f) return f;
+  const double half = 0.5 * dt;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double value = pos[d] - half * vel[d];
+    if (!isfinite(value)) return kFlagArithmetic;
+    if (value < g.origin[d]) value = g.origin[d];
+    if (value > g.upper[d]) value = g.upper[d];
+    mid[d] = value;
+  }
+  f = sample<T>(g, 0, u, mid, vel[0]) | sample<T>(g, 1, v, mid, vel[1]) |
+      sample<T>(g, 2, w, mid, vel[2]);
+  if (f) return f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double value = pos[d] - dt * vel[d];
+    if (!isfinite(value)) return kFlagArithmetic;
+    if (value < g.origin[d]) value = g.origin[d];
+    if (value > g.upper[d]) value = g.upper[d];
+    out[d] = value;
+  }
+  return 0;
+}
+
+// advect_velocity (src/advection.rs:28-68).  blockIdx.y = component.  Wall
+// normal faces are exactly zero; every other face samples its own component
+// at the RK2 departure of its own face position (src/geometry.rs:240-248).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                  const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
+                  T* __restrict__ ow, unsigned* __restrict__ flags) {
+  const int a = blockIdx.y;
+  const T* src = a == 0 ? u : (a == 1 ? v : w);
+  T* dst = a == 0 ? ou : (a == 1 ? ov : ow);
+  const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
+  const RowRange rr = block_rows(fy * fz);
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / fy, j = row - k * fy;
+    for (int i = threadIdx.x; i < fx; i += kLanesX) {
+      const int p[3] = {i, j, k};
+      const int idx = i + fx * row;
+      const int top = (a == 0 ? fx : (a == 1 ? fy : fz)) - 1;
+      if (p[a] == 0 || p[a] == top) {
+        dst[idx] = (T)0.0;
+        continue;
+      }
+      double pos[3], dep[3], value = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
+      unsigned f = backtrace<T>(g, u, v, w, pos, dt, dep);
+      if (!f) f = sample<T>(g, a, src, dep, value);
+      if (f) {
+        bad |= f;
+        continue;
+      }
+      dst[idx] = (T)value;
+    }
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
+// advect_tracer (src/advection.rs:72-102), every cell evaluated (the
+// reference's support skipping is exact: it writes the same zeros).
+__global__ void __launch_bounds__(kBlock)
+k_advect_tracer(DevGrid g, const float* __restrict__ old, const float* __restrict__ u,
+                const float* __restrict__ v, const float* __restrict__ w, double dt,
+                float* __restrict__ out, unsigned* __restrict__ flags) {
+  const RowRange rr = block_rows(g.ny * g.nz);
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    for (int i = threadIdx.x; i < g.nx; i += kLanesX) {
+      const int p[3] = {i, j, k};
+      double pos[3], dep[3], value = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pos[d] = g.origin[d] + ((double)p[d] + 0.5) * g.h[d];
+      unsigned f = backtrace<float>(g, u, v, w, pos, dt, dep);
+      if (!f) f = sample<float>(g, 3, old, dep, value);
+      if (f) {
+        bad |= f;
+        continue;
+      }
+      out[i + g.nx * row] = (float)value;
+    }
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
+// Finite check of every face and (optionally) exact zero on wall-normal faces
+// (validate_velocity, src/operator.rs:850-875; SamplingError::NonFiniteField,
+// src/sampling.rs:62-64).  blockIdx.y = component.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_validate_faces(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                 const T* __restrict__ w, int check_walls, unsigned* __restrict__ flags) {
+  const int a = blockIdx.y;
+  const T* src = a == 0 ? u : (a == 1 ? v : w);
+  const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
+  const RowRange rr = block_rows(fy * fz);
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / fy, j = row - k * fy;
+    for (int i = threadIdx.x; i < fx; i += kLanesX) {
+      const double value = (double)src[i + fx * row];
+      if (!isfinite(value)) bad |= kFlagNonfiniteInput;
+      if (check_walls) {
+        const int pa = a == 0 ? i : (a == 1 ? j : k);
+        const int top = a == 0 ? g.nx : (a == 1 ? g.ny : g.nz);
+        if ((pa == 0 || pa == top) && value != 0.0) bad |= kFlagNonzeroWall;
+      }
+    }
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
+// build_rhs (src/operator.rs:460-509): b = (sum_d A_d (u_lo - u_hi)) / dt,
+// axis order X, Y, Z, f64 from T faces.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_build_rhs(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+            const T* __restrict__ w, double dt, double* __restrict__ out,
+            unsigned* __restrict__ flags) {
+  const int nx = g.nx, ny = g.ny;
+  const RowRange rr = block_rows(g.ny * g.nz);
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / ny, j = row - k * ny;
+    for (int i = threadIdx.x; i < nx; i += kLanesX) {
+      const int xu = i + (nx + 1) * row;
+      const int yv = i + nx * (j + (ny + 1) * k);
+      const int zw = i + nx * row;
+      const double ulo = (double)u[xu], uhi = (double)u[xu + 1];
+      const double vlo = (double)v[yv], vhi = (double)v[yv + nx];
+      const double wlo = (double)w[zw], whi = (double)w[zw + nx * ny];
+      double flux = 0.0;
+      flux += g.area[0] * (ulo - uhi);
+      flux += g.area[1] * (vlo - vhi);
+      flux += g.area[2] * (wlo - whi);
+      const double value = flux / dt;
+      if (!isfinite(value)) bad |= kFlagArithmetic;
+      out[i + nx * row] = value;
+    }
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
+// correct_candidate_in_place (src/operator.rs:627-668) for f32 storage:
+// u = f32(f64(u) - dt*acc_d*(p_R - p_L)), evaluated left to right;
+// correct_candidate_f64 (:746-781) for f64: u - dt*(acc_d*(p_R - p_L)).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_correct(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u,
+          T* __restrict__ v, T* __restrict__ w, unsigned* __restrict__ flags) {
+  const int a = blockIdx.y;
+  T* dst = a == 0 ? u : (a == 1 ? v : w);
+  const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
+  const int stride = a == 0 ? 1 : (a == 1 ? g.nx : g.nx * g.ny);
+  const RowRange rr = block_rows(fy * fz);
+  const double acc = g.acc[a];
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / fy, j = row - k * fy;
+    const int pa_row = a == 1 ? j : k;
+    const int top_row = a == 1 ? g.ny : g.nz;
+    if (a != 0 && (pa_row == 0 || pa_row == top_row)) continue;
+    for (int i = threadIdx.x; i < fx; i += kLanesX) {
+      if (a == 0 && (i == 0 || i == g.nx)) continue;
+      const int index = i + fx * row;
+      const int right = i + g.nx * (j + g.ny * k);
+      const double difference = p[right] - p[right - stride];
+      double value;
+      if (sizeof(T) == 4) {
+        value = (double)dst[index] - dt * acc * difference;
+        if (!isfinite(value) || fabs(value) > (double)FLT_MAX) {
+          bad |= kFlagArithmetic;
+          continue;
+        }
+      } else {
+        const double gradient = acc * difference;
+        value = (double)dst[index] - dt * gradient;
+        if (!isfinite(value)) {
+          bad |= kFlagArithmetic;
+          continue;
+        }
+      }
+      dst[index] = (T)value;
+    }
+  }
+  if (bad) atomicOr(flags, bad);
+}
+
+// divergence (src/operator.rs:513-568) and its all-cell maximum
+// (actual_divergence_max, src/pressure.rs:330-342):
+// div = (u_hi-u_lo)/h0 + (v_hi-v_lo)/h1 + (w_hi-w_lo)/h2.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_divergence_max(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                 const T* __restrict__ w, Reduce red) {
+  const int nx = g.nx, ny = g.ny;
+  const RowRange rr = block_rows(g.ny * g.nz);
+  double m[1] = {0.0};
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / ny, j = row - k * ny;
+    for (int i = threadIdx.x; i < nx; i += kLanesX) {
+      const int xu = i + (nx + 1) * row;
+      const int yv = i + nx * (j + (ny + 1) * k);
+      const int zw = i + nx * row;
+      const double value = ((double)u[xu + 1] - (double)u[xu]) / g.h[0] +
+                           ((double)v[yv + nx] - (double)v[yv]) / g.h[1] +
+                           ((double)w[zw + nx * ny] - (double)w[zw]) / g.h[2];
+      if (!isfinite(value)) bad |= kFlagArithmetic;
+      m[0] = fmax(m[0], fabs(value));
+    }
+  }
+  if (bad) atomicOr(red.flags, bad);
+  DD dd[1] = {DD{0.0, 0.0}};
+  double* out = red.out;
+  grid_finish<1, 1>(dd, m, red.pdd, red.pmx, red.counter,
+                    [=](const double*, const double* mx) { out[0] = mx[0]; });
+}
+
+// velocity_rate (src/simulation.rs:1415-1429): sum_d max|u_d| / h_d.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_velocity_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                const T* __restrict__ w, Reduce red) {
+  double m[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < 3; ++a) {
+    const T* src = a == 0 ? u : (a == 1 ? v : w);
+    const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
+    const RowRange rr = block_rows(fy * fz);
+    for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock)
+      for (int i = threadIdx.x; i < fx; i += kLanesX)
+        m[a] = fmax(m[a], fabs((double)src[i + fx * row]));
+  }
+  DD dd[1] = {DD{0.0, 0.0}};
+  double* out = red.out;
+  const double h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
+  grid_finish<1, 3>(dd, m, red.pdd, red.pmx, red.counter, [=](const double*, const double* mx) {
+    double rate = 0.0;
+    rate += mx[0] / h0;
+    rate += mx[1] / h1;
+    rate += mx[2] / h2;
+    out[0] = rate;
+    out[1] = mx[0];
+    out[2] = mx[1];
+    out[3] = mx[2];
+  });
+}
+
+// ------------------------------------------------------------ host launchers
+template <typename T>
+void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
+                   void* ou, void* ov, void* ow, unsigned* flags, hipStream_t s) {
+  const int rows = (g.ny + 1) * (g.nz + 1);
+  dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
+  k_advect_velocity<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, dt,
+                                              (T*)ou, (T*)ov, (T*)ow, flags);
+}
+
+template <typename T>
+void launch_validate(const DevGrid& g, const void* u, const void* v, const void* w,
+                     int check_walls, unsigned* flags, hipStream_t s) {
+  const int rows = (g.ny + 1) * (g.nz + 1);
+  dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
+  k_validate_faces<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w,
+                                             check_walls, flags);
+}
+
+template <typename T>
+void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
+                double* out, unsigned* flags, hipStream_t s) {
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  k_build_rhs<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
+}
+
+template <typename T>
+void launch_correct(const DevGrid& g, const double* p, double dt, void* u, void* v, void* w,
+                    unsigned* flags, hipStream_t s) {
+  const int rows = (g.ny + 1) * (g.nz + 1);
+  dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
+  k_correct<T><<<grid, block, 0, s>>>(g, p, dt, (T*)u, (T*)v, (T*)w, flags);
+}
+
+template <typename T>
+void launch_divergence_max(const DevGrid& g, const void* u, const void* v, const void* w,
+                           const Reduce& red, hipStream_t s) {
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  k_divergence_max<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, red);
+}
+
+template <typename T>
+void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const void* w,
+                          const Reduce& red, hipStream_t s) {
+  const int rows = (g.ny + 1) * (g.nz + 1);
+  dim3 grid(grid_blocks(rows)), block(kLanesX, kRowsPerBlock);
+  k_velocity_rate<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, red);
+}
+
+void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,
+                          const float* w, double dt, float* out, unsigned* flags, hipStream_t s) {
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  k_advect_tracer<<<grid, block, 0, s>>>(g, old, u, v, w, dt, out, flags);
+}
+
+#define RH_INSTANTIATE(T)                                                                     \
+  template void launch_advect<T>(const DevGrid&, const void*, const void*, const void*,        \
+                                 double, void*, void*, void*, unsigned*, hipStream_t);         \
+  template void launch_validate<T>(const DevGrid&, const void*, const void*, const void*, int, \
+                                   unsigned*, hipStream_t);                                    \
+  template void launch_rhs<T>(const DevGrid&, const void*, const void*, const void*, double,   \
+                              double*, unsigned*, hipStream_t);                                \
+  template void launch_correct<T>(const DevGrid&, const double*, double, void*, void*, void*,  \
+                                  unsigned*, hipStream_t);                                     \
+  template void launch_divergence_max<T>(const DevGrid&, const void*, const void*, const void*, \
+                                         const Reduce&, hipStream_t);                          \
+  template void launch_velocity_rate<T>(const DevGrid&, const void*, const void*, const void*,  \
+                                        const Reduce&, hipStream_t);
+RH_INSTANTIATE(float)
+RH_INSTANTIATE(double)
+#undef RH_INSTANTIATE
+
+}  // namespace rh

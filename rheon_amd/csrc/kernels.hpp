@@ -1,0 +1,99 @@
+// Kernel launch interface shared by the host code of rheon_amd.
+#pragma once
+#include "common.hpp"
+
+namespace rh {
+
+// Error bits collected by kernels with atomicOr into a device flag word.
+enum : unsigned {
+  kFlagNonfiniteInput = 1u,
+  kFlagNonzeroWall = 2u,
+  kFlagArithmetic = 4u,
+};
+
+// Grid-reduction scratch: per-block partials, an arrival counter and the
+// output scalars of one reduction.
+struct Reduce {
+  DD* pdd;            // 4 * kMaxBlocks
+  double* pmx;        // 4 * kMaxBlocks
+  unsigned* counter;  // zero on entry
+  double* out;        // result scalars
+  unsigned* flags;    // error bits
+};
+
+// ------------------------------------------------------------ field kernels
+template <typename T>
+void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
+                   void* ou, void* ov, void* ow, unsigned* flags, hipStream_t s);
+template <typename T>
+void launch_validate(const DevGrid& g, const void* u, const void* v, const void* w,
+                     int check_walls, unsigned* flags, hipStream_t s);
+template <typename T>
+void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
+                double* out, unsigned* flags, hipStream_t s);
+template <typename T>
+void launch_correct(const DevGrid& g, const double* p, double dt, void* u, void* v, void* w,
+                    unsigned* flags, hipStream_t s);
+template <typename T>
+void launch_divergence_max(const DevGrid& g, const void* u, const void* v, const void* w,
+                           const Reduce& red, hipStream_t s);
+template <typename T>
+void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const void* w,
+                          const Reduce& red, hipStream_t s);
+void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,
+                          const float* w, double dt, float* out, unsigned* flags, hipStream_t s);
+
+// -------------------------------------------------------------- PCG kernels
+// Device-side solver control block: all scalar decisions of solve_loaded
+// (src/pressure.rs:577-740) are taken on the device, so the host only
+// enqueues iterations and polls `done` every few iterations.
+struct PcgControl {
+  double rz[2];        // rz of iteration it is rz[it & 1]
+  double alpha;
+  double threshold;
+  double rel, abs_tol, div_limit;
+  double dt, volume;
+  double rr;           // last recursive r.r
+  double true_l2, true_max, pred_div;
+  double sum, sum_abs; // compatibility sums
+  double error_value;
+  int max_iterations;
+  int iteration;       // last completed iteration
+  int done;            // nonzero once a terminal status is set
+  int status;          // rh_status
+  int error_iteration;
+  int candidate;       // this iteration needs a true report
+  int replaced;        // this iteration replaced r by the true residual
+  int pad;
+  unsigned arrive[8];  // grid-reduction counters, one per kernel slot
+};
+
+struct PcgArrays {
+  void* p;       // T
+  void* r;       // T
+  void* d;       // T
+  void* q;       // T (also holds the true residual)
+  void* z;       // T (RB-SGS only)
+  const double* b;
+  double* p64;   // f64 output pressure when T = float (may alias p when T = double)
+};
+
+template <typename T>
+void pcg_prologue(const DevGrid& g, const PcgArrays& a, PcgControl* ctl, DD* pdd, double* pmx,
+                  hipStream_t s);
+template <typename T>
+void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl* ctl, DD* pdd,
+                   double* pmx, int it, hipStream_t s);
+template <typename T>
+void pcg_export_pressure(const DevGrid& g, const PcgArrays& a, hipStream_t s);
+
+void launch_apply_full(const DevGrid& g, const double* x, double* y, unsigned* flags,
+                       hipStream_t s);
+void launch_precondition(const DevGrid& g, int method, double* r, double* z, PcgControl* ctl,
+                         DD* pdd, double* pmx, hipStream_t s);
+void launch_residual_report(const DevGrid& g, const double* p, const double* b, const Reduce& red,
+                            hipStream_t s);
+void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
+                     hipStream_t s);
+
+}  // namespace rh

@@ -1,0 +1,484 @@
+// Preconditioned conjugate gradient on the device: the reference's
+// "jacobi-pcg-v1" (src/pressure.rs:577-740) with every scalar decision taken
+// on the device, plus a symmetric red-black Gauss-Seidel preconditioner.
+//
+// One iteration `it` is a fixed sequence of kernels; each begins by reading
+// the control block and returns at once when the solve has finished or the
+// kernel's branch is not taken, so the host can enqueue many iterations
+// ahead and poll `done` only every few iterations:
+//   apply_dir   q = K d, q[0] = 0, curvature = d.q, alpha        (pressure.rs:657-668)
+//   update      p += alpha d, r -= alpha q (i >= 1), r.r, z, r.z (:669-681, :534-575)
+//   [sgs black, sgs red]                                        (RB-SGS only)
+//   true        if candidate: product = b - K p, l2, max, gates  (:686-709, :432-456)
+//   replace     if replaced: r = product, z, r.z                 (:705-707, :716)
+//   [sgs black, sgs red]                                        (RB-SGS replace path)
+//   direction   d = z + beta d (or d = z after a replacement)    (:716-733)
+// Dot products are double-double sums of the rounded products (see DD in
+// common.hpp), so they equal the reference's Neumaier sums except in
+// vanishingly rare midpoint cases.  Vector storage T is double (reference
+// arithmetic) or float (halved traffic; scalars and arithmetic stay f64).
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace rh {
+
+#define FOR_CELLS(g)                                                                 \
+  {                                                                                  \
+    const RowRange rr_ = block_rows((g).ny * (g).nz);                                \
+    for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
+      const int k = row / (g).ny, j = row - k * (g).ny;                              \
+      for (int i = threadIdx.x; i < (g).nx; i += kLanesX) {                          \
+        const int c = i + (g).nx * row;
+#define END_CELLS \
+  }               \
+  }               \
+  }
+
+template <typename T>
+__device__ __forceinline__ double ld(const T* __restrict__ x, int i) {
+  return (double)x[i];
+}
+
+// apply_full box fast path (src/operator.rs:400-416): axis 0..2, lower then upper.
+template <typename T>
+__device__ __forceinline__ double apply_at(const DevGrid& g, const T* __restrict__ x, int c, int i,
+                                           int j, int k) {
+  const double xc = ld(x, c);
+  const int sy = g.nx, sz = g.nx * g.ny;
+  double v = 0.0;
+  if (i > 0) v += g.w[0] * (xc - ld(x, c - 1));
+  if (i + 1 < g.nx) v += g.w[0] * (xc - ld(x, c + 1));
+  if (j > 0) v += g.w[1] * (xc - ld(x, c - sy));
+  if (j + 1 < g.ny) v += g.w[1] * (xc - ld(x, c + sy));
+  if (k > 0) v += g.w[2] * (xc - ld(x, c - sz));
+  if (k + 1 < g.nz) v += g.w[2] * (xc - ld(x, c + sz));
+  return v;
+}
+
+// diagonal(): cached sum of existing neighbour weights (src/operator.rs:813-822).
+__device__ __forceinline__ double diag_at(const DevGrid& g, int i, int j, int k) {
+  const int mask = (i > 0) | ((i + 1 < g.nx) << 1) | ((j > 0) << 2) | ((j + 1 < g.ny) << 3) |
+                   ((k > 0) << 4) | ((k + 1 < g.nz) << 5);
+  return g.diag[mask];
+}
+
+// Weighted neighbour sum sum_nbr w_d * x_nbr, axis 0..2, lower then upper.
+template <typename T>
+__device__ __forceinline__ double nbr_sum(const DevGrid& g, const T* __restrict__ x, double init,
+                                          int c, int i, int j, int k) {
+  const int sy = g.nx, sz = g.nx * g.ny;
+  double v = init;
+  if (i > 0) v += g.w[0] * ld(x, c - 1);
+  if (i + 1 < g.nx) v += g.w[0] * ld(x, c + 1);
+  if (j > 0) v += g.w[1] * ld(x, c - sy);
+  if (j + 1 < g.ny) v += g.w[1] * ld(x, c + sy);
+  if (k > 0) v += g.w[2] * ld(x, c - sz);
+  if (k + 1 < g.nz) v += g.w[2] * ld(x, c + sz);
+  return v;
+}
+
+__device__ __forceinline__ void fail(PcgControl* ctl, int status, int it) {
+  ctl->done = 1;
+  ctl->status = status;
+  ctl->error_iteration = it;
+}
+
+// ---------------------------------------------------------------- kernels
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill_zero(T* __restrict__ x, int n) {
+  for (int i = blockIdx.x * kBlock + flat_tid(); i < n; i += gridDim.x * kBlock) x[i] = (T)0.0;
+}
+
+// Compatibility sums and threshold (src/pressure.rs:585-633): sum(b),
+// sum(|b|), b.b; also validate_cells (finite, src/operator.rs:840-848).
+__global__ void __launch_bounds__(kBlock)
+k_rhs_sums(DevGrid g, const double* __restrict__ b, PcgControl* ctl, DD* pdd, double* pmx) {
+  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  const double v = b[c];
+  if (!isfinite(v)) bad[0] = 1.0;
+  dd_acc(s[0], v);
+  dd_acc(s[1], fabs(v));
+  dd_acc(s[2], v * v);
+  END_CELLS
+  grid_finish<3, 1>(s, bad, pdd, pmx, &ctl->arrive[0], [=](const double* sum, const double* mx) {
+    if (mx[0] != 0.0) return fail(ctl, RH_NONFINITE_INPUT, 0);
+    const double total = sum[0], total_abs = sum[1];
+    ctl->sum = total;
+    ctl->sum_abs = total_abs;
+    const double budget = (64.0 * 2.220446049250313e-16) * total_abs;
+    if (!isfinite(total) || !isfinite(total_abs)) return fail(ctl, RH_BREAKDOWN, 0);
+    if (fabs(total) > budget) {
+      ctl->error_value = total;
+      return fail(ctl, RH_INCOMPATIBLE_RHS, 0);
+    }
+    const double rhs_l2 = sqrt(sum[2]);
+    const double threshold = fmax(ctl->abs_tol, ctl->rel * rhs_l2);
+    if (!isfinite(rhs_l2) || !isfinite(threshold)) return fail(ctl, RH_BREAKDOWN, 0);
+    ctl->threshold = threshold;
+  });
+}
+
+// q = K d, q[0] = 0 (gauge), curvature = d.q, alpha = rz/curvature.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_apply_dir(DevGrid g, const T* __restrict__ d, T* __restrict__ q, PcgControl* ctl, DD* pdd,
+            double* pmx, int it) {
+  if (ctl->done) return;
+  DD s[1] = {{0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  double v = apply_at<T>(g, d, c, i, j, k);
+  if (!isfinite(v)) bad[0] = 1.0;
+  if (c == 0) v = 0.0;
+  const T qs = (T)v;
+  q[c] = qs;
+  dd_acc(s[0], ld(d, c) * (double)qs);
+  END_CELLS
+  grid_finish<1, 1>(s, bad, pdd, pmx, &ctl->arrive[1], [=](const double* sum, const double* mx) {
+    if (mx[0] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
+    const double curvature = sum[0];
+    const double rz = ctl->rz[it & 1];
+    if (!isfinite(curvature) || curvature <= 0.0 || !isfinite(rz) || rz <= 0.0)
+      return fail(ctl, RH_BREAKDOWN, it);
+    const double alpha = rz / curvature;
+    if (!isfinite(alpha)) return fail(ctl, RH_BREAKDOWN, it);
+    ctl->alpha = alpha;
+  });
+}
+
+// p += alpha d, r -= alpha q for i >= 1; recursive r.r; candidate flag.
+// JACOBI: z = r/D on the fly and r.z (speculative next rz).
+// RB-SGS: first (red) half-sweep y_red = r/D stored in z.
+template <typename T, bool SGS>
+__global__ void __launch_bounds__(kBlock)
+k_update(DevGrid g, T* __restrict__ p, T* __restrict__ r, const T* __restrict__ d,
+         const T* __restrict__ q, T* __restrict__ z, PcgControl* ctl, DD* pdd, double* pmx,
+         int it) {
+  if (ctl->done) return;
+  const double alpha = ctl->alpha;
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  FOR_CELLS(g)
+  if (c == 0) {
+    if (SGS) z[0] = (T)0.0;
+    continue;
+  }
+  const T pn = (T)(ld(p, c) + alpha * ld(d, c));
+  const T rn = (T)(ld(r, c) - alpha * ld(q, c));
+  p[c] = pn;
+  r[c] = rn;
+  const double rv = (double)rn;
+  dd_acc(s[0], rv * rv);
+  if (SGS) {
+    if (((i + j + k) & 1) == 0) z[c] = (T)(rv / diag_at(g, i, j, k));
+  } else {
+    const double zv = rv / diag_at(g, i, j, k);
+    dd_acc(s[1], rv * zv);
+  }
+  END_CELLS
+  grid_finish<2, 1>(s, none, pdd, pmx, &ctl->arrive[2], [=](const double* sum, const double*) {
+    ctl->rr = sum[0];
+    const double recursive_l2 = sqrt(sum[0]);
+    ctl->candidate = (recursive_l2 <= ctl->threshold) || it == ctl->max_iterations;
+    ctl->replaced = 0;
+    if (!SGS) ctl->rz[(it + 1) & 1] = sum[1];
+  });
+}
+
+// true_report (src/pressure.rs:432-456): product = b - K p over every cell
+// (gauge row included), l2 (dot), max, predicted divergence = max*dt/V;
+// then the acceptance / replacement / iteration-limit decisions of
+// solve_loaded (:638-647 at it = 0, :686-715 otherwise).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_true_residual(DevGrid g, const T* __restrict__ p, const double* __restrict__ b,
+                T* __restrict__ out, PcgControl* ctl, DD* pdd, double* pmx, int it) {
+  if (ctl->done) return;
+  if (it > 0 && !ctl->candidate) return;
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  FOR_CELLS(g)
+  const double v = apply_at<T>(g, p, c, i, j, k);
+  if (!isfinite(v)) m[1] = 1.0;
+  const double prod = b[c] - v;
+  out[c] = (T)prod;
+  dd_acc(s[0], prod * prod);
+  m[0] = fmax(m[0], fabs(prod));
+  END_CELLS
+  grid_finish<1, 2>(s, m, pdd, pmx, &ctl->arrive[3], [=](const double* sum, const double* mx) {
+    if (mx[1] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
+    const double squared = sum[0], maximum = mx[0];
+    const double divergence = maximum * ctl->dt / ctl->volume;
+    if (!isfinite(squared) || !isfinite(divergence)) return fail(ctl, RH_BREAKDOWN, it);
+    const double l2 = sqrt(squared);
+    ctl->true_l2 = l2;
+    ctl->true_max = maximum;
+    ctl->pred_div = divergence;
+    const bool accepted = l2 <= ctl->threshold && divergence <= ctl->div_limit;
+    if (accepted) {
+      ctl->iteration = it;
+      ctl->done = 1;
+      ctl->status = RH_OK;
+      return;
+    }
+    if (it > 0 && l2 > ctl->threshold) ctl->replaced = 1;
+    if (it == ctl->max_iterations) {
+      ctl->error_value = maximum;
+      ctl->iteration = it;
+      return fail(ctl, RH_ITERATION_LIMIT, it);
+    }
+  });
+}
+
+// Replacement r = product (r[0] = 0) and the first preconditioner stage.
+// it == 0 is the initial precondition before the loop (src/pressure.rs:648-650).
+// JACOBI: r.z with z = r/D (next rz).  RB-SGS: y_red = r/D into z.
+template <typename T, bool SGS>
+__global__ void __launch_bounds__(kBlock)
+k_replace(DevGrid g, T* r, const T* prod, T* __restrict__ z,
+          PcgControl* ctl, DD* pdd, double* pmx, int it) {
+  if (ctl->done) return;
+  if (it > 0 && !ctl->replaced) return;
+  DD s[1] = {{0.0, 0.0}};
+  double none[1] = {0.0};
+  FOR_CELLS(g)
+  if (c == 0) {
+    r[0] = (T)0.0;
+    if (SGS) z[0] = (T)0.0;
+    continue;
+  }
+  const T rn = prod[c];
+  r[c] = rn;
+  const double rv = (double)rn;
+  if (SGS) {
+    if (((i + j + k) & 1) == 0) z[c] = (T)(rv / diag_at(g, i, j, k));
+  } else {
+    const double zv = rv / diag_at(g, i, j, k);
+    dd_acc(s[0], rv * zv);
+  }
+  END_CELLS
+  if (SGS) return;
+  grid_finish<1, 1>(s, none, pdd, pmx, &ctl->arrive[4], [=](const double* sum, const double*) {
+    ctl->rz[(it + 1) & 1] = sum[0];
+  });
+}
+
+// Which path a red-black stage belongs to: the speculative one right after
+// `update` (0) or the replacement / initial one after `replace` (1).
+__device__ __forceinline__ bool sgs_active(const PcgControl* ctl, int it, int mode) {
+  if (ctl->done) return false;
+  if (mode == 0) return it > 0;
+  return it == 0 || ctl->replaced;
+}
+
+// Black forward half-sweep: y_b = (r_b + sum_red w y) / D (neighbours are red).
+// The black reverse step z_b = y_b is the identity (no later unknowns).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sgs_black(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgControl* ctl, int it,
+            int mode) {
+  if (!sgs_active(ctl, it, mode)) return;
+  FOR_CELLS(g)
+  if (((i + j + k) & 1) == 0) continue;
+  const double v = nbr_sum<T>(g, z, ld(r, c), c, i, j, k);
+  z[c] = (T)(v / diag_at(g, i, j, k));
+  END_CELLS
+}
+
+// Red reverse half-sweep z_r = y_r + (sum_black w z)/D, then r.z over all
+// cells (black z are final).  Gauge cell 0 (red) stays 0.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, PcgControl* ctl, DD* pdd,
+          double* pmx, int it, int mode) {
+  if (!sgs_active(ctl, it, mode)) return;
+  DD s[1] = {{0.0, 0.0}};
+  double none[1] = {0.0};
+  FOR_CELLS(g)
+  double zv;
+  if (((i + j + k) & 1) == 0 && c != 0) {
+    const double corr = nbr_sum<T>(g, z, 0.0, c, i, j, k);
+    const T zn = (T)(ld(z, c) + corr / diag_at(g, i, j, k));
+    z[c] = zn;
+    zv = (double)zn;
+  } else {
+    zv = ld(z, c);
+  }
+  dd_acc(s[0], ld(r, c) * zv);
+  END_CELLS
+  grid_finish<1, 1>(s, none, pdd, pmx, &ctl->arrive[5], [=](const double* sum, const double*) {
+    ctl->rz[(it + 1) & 1] = sum[0];
+  });
+}
+
+// d = z + beta d (i >= 1), or d = z after a replacement / at it = 0;
+// next_rz breakdown check (src/pressure.rs:716-719).
+template <typename T, bool SGS>
+__global__ void __launch_bounds__(kBlock)
+k_direction(DevGrid g, const T* __restrict__ r, const T* __restrict__ z, T* __restrict__ d,
+            PcgControl* ctl, int it) {
+  if (ctl->done) return;
+  const double next_rz = ctl->rz[(it + 1) & 1];
+  if (it > 0 && (!isfinite(next_rz) || next_rz <= 0.0)) {
+    if (blockIdx.x == 0 && flat_tid() == 0) fail(ctl, RH_BREAKDOWN, it);
+    return;
+  }
+  const bool copy = it == 0 || ctl->replaced;
+  const double beta = copy ? 0.0 : next_rz / ctl->rz[it & 1];
+  FOR_CELLS(g)
+  if (c == 0) {
+    d[0] = (T)0.0;
+    continue;
+  }
+  const double zv = SGS ? ld(z, c) : ld(r, c) / diag_at(g, i, j, k);
+  d[c] = copy ? (T)zv : (T)(zv + beta * ld(d, c));
+  END_CELLS
+  if (blockIdx.x == 0 && flat_tid() == 0) ctl->iteration = it;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_to_f64(const T* __restrict__ x, double* __restrict__ y,
+                                                   int n) {
+  for (int i = blockIdx.x * kBlock + flat_tid(); i < n; i += gridDim.x * kBlock) y[i] = (double)x[i];
+}
+
+// Stateless operator kernels for the C ABI -----------------------------------
+
+__global__ void __launch_bounds__(kBlock)
+k_apply_full(DevGrid g, const double* __restrict__ x, double* __restrict__ y, unsigned* flags) {
+  unsigned bad = 0;
+  FOR_CELLS(g)
+  const double v = apply_at<double>(g, x, c, i, j, k);
+  if (!isfinite(v)) bad |= kFlagArithmetic;
+  y[c] = v;
+  END_CELLS
+  if (bad) atomicOr(flags, bad);
+}
+
+// ------------------------------------------------------------ host launchers
+
+static inline dim3 cells_grid(const DevGrid& g) { return dim3(grid_blocks(g.ny * g.nz)); }
+static inline dim3 block_shape() { return dim3(kLanesX, kRowsPerBlock); }
+
+template <typename T>
+void pcg_prologue(const DevGrid& g, const PcgArrays& a, PcgControl* ctl, DD* pdd, double* pmx,
+                  hipStream_t s) {
+  const int n = g.nx * g.ny * g.nz;
+  const int fill_blocks = n / kBlock + 1 < kMaxBlocks ? n / kBlock + 1 : kMaxBlocks;
+  k_rhs_sums<<<cells_grid(g), block_shape(), 0, s>>>(g, a.b, ctl, pdd, pmx);
+  k_fill_zero<T><<<fill_blocks, block_shape(), 0, s>>>((T*)a.p, n);
+  k_true_residual<T><<<cells_grid(g), block_shape(), 0, s>>>(g, (const T*)a.p, a.b, (T*)a.q, ctl,
+                                                             pdd, pmx, 0);
+}
+
+template <typename T>
+void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl* ctl, DD* pdd,
+                   double* pmx, int it, hipStream_t s) {
+  const dim3 grid = cells_grid(g), block = block_shape();
+  T* p = (T*)a.p;
+  T* r = (T*)a.r;
+  T* d = (T*)a.d;
+  T* q = (T*)a.q;
+  T* z = (T*)a.z;
+  const bool sgs = method == RH_RBSGS_PCG;
+  if (it > 0) {
+    k_apply_dir<T><<<grid, block, 0, s>>>(g, d, q, ctl, pdd, pmx, it);
+    if (sgs) {
+      k_update<T, true><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
+      k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 0);
+      k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 0);
+    } else {
+      k_update<T, false><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
+    }
+    k_true_residual<T><<<grid, block, 0, s>>>(g, p, a.b, q, ctl, pdd, pmx, it);
+  }
+  if (sgs) {
+    k_replace<T, true><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
+    k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 1);
+    k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 1);
+    k_direction<T, true><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+  } else {
+    k_replace<T, false><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
+    k_direction<T, false><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+  }
+}
+
+template <typename T>
+void pcg_export_pressure(const DevGrid& g, const PcgArrays& a, hipStream_t s) {
+  if (sizeof(T) == sizeof(double) && (void*)a.p64 == a.p) return;
+  const int n = g.nx * g.ny * g.nz;
+  const int blocks = n / kBlock + 1 < kMaxBlocks ? n / kBlock + 1 : kMaxBlocks;
+  k_to_f64<T><<<blocks, block_shape(), 0, s>>>((const T*)a.p, a.p64, n);
+}
+
+template void pcg_prologue<double>(const DevGrid&, const PcgArrays&, PcgControl*, DD*, double*,
+                                   hipStream_t);
+template void pcg_prologue<float>(const DevGrid&, const PcgArrays&, PcgControl*, DD*, double*,
+                                  hipStream_t);
+template void pcg_iteration<double>(const DevGrid&, int, const PcgArrays&, PcgControl*, DD*,
+                                    double*, int, hipStream_t);
+template void pcg_iteration<float>(const DevGrid&, int, const PcgArrays&, PcgControl*, DD*,
+                                   double*, int, hipStream_t);
+template void pcg_export_pressure<double>(const DevGrid&, const PcgArrays&, hipStream_t);
+template void pcg_export_pressure<float>(const DevGrid&, const PcgArrays&, hipStream_t);
+
+void launch_apply_full(const DevGrid& g, const double* x, double* y, unsigned* flags,
+                       hipStream_t s) {
+  k_apply_full<<<cells_grid(g), block_shape(), 0, s>>>(g, x, y, flags);
+}
+
+// Stateless z = M^-1 r (gauge row zeroed); rz = r.z lands in ctl->rz[1].
+// Reuses the it = 0 path of the PCG kernels with prod aliasing r; `ctl`
+// must be zeroed by the caller.
+void launch_precondition(const DevGrid& g, int method, double* r, double* z, PcgControl* ctl,
+                         DD* pdd, double* pmx, hipStream_t s) {
+  const dim3 grid = cells_grid(g), block = block_shape();
+  if (method == RH_RBSGS_PCG) {
+    k_replace<double, true><<<grid, block, 0, s>>>(g, r, r, z, ctl, pdd, pmx, 0);
+    k_sgs_black<double><<<grid, block, 0, s>>>(g, r, z, ctl, 0, 1);
+    k_sgs_red<double><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, 0, 1);
+  } else {
+    k_replace<double, false><<<grid, block, 0, s>>>(g, r, r, z, ctl, pdd, pmx, 0);
+    // z = r/D through the direction kernel (d := z, copy branch at it = 0)
+    k_direction<double, false><<<grid, block, 0, s>>>(g, r, z, z, ctl, 0);
+  }
+}
+
+// Residual report of a supplied pressure (ProviderWorkspace qualification,
+// src/solver_provider.rs:246-254): out[0] = sum (b - K p)^2, out[1] = max|b - K p|.
+__global__ void __launch_bounds__(kBlock)
+k_residual_report(DevGrid g, const double* __restrict__ p, const double* __restrict__ b,
+                  Reduce red) {
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  FOR_CELLS(g)
+  const double v = apply_at<double>(g, p, c, i, j, k);
+  if (!isfinite(v)) m[1] = 1.0;
+  const double prod = b[c] - v;
+  dd_acc(s[0], prod * prod);
+  m[0] = fmax(m[0], fabs(prod));
+  END_CELLS
+  double* out = red.out;
+  grid_finish<1, 2>(s, m, red.pdd, red.pmx, red.counter, [=](const double* sum, const double* mx) {
+    out[0] = sum[0];
+    out[1] = mx[0];
+    out[2] = mx[1];
+  });
+}
+
+void launch_residual_report(const DevGrid& g, const double* p, const double* b, const Reduce& red,
+                            hipStream_t s) {
+  k_residual_report<<<cells_grid(g), block_shape(), 0, s>>>(g, p, b, red);
+}
+
+// Compatibility sums of a RHS into ctl (sum, sum_abs, threshold) with the
+// settings already stored in ctl (rel, abs_tol).
+void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
+                     hipStream_t s) {
+  k_rhs_sums<<<cells_grid(g), block_shape(), 0, s>>>(g, b, ctl, pdd, pmx);
+}
+
+}  // namespace rh

@@ -1,0 +1,479 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+Each test restates one of the reference's own known-answer tests (cited),
+a reference-published golden run, or a closed form / brute-force identity
+chosen so a plausible slip (dropped term, wrong sign, transposed stride,
+wrong summation order) fails it.  CPU only.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from rheon_amd.scenes import GridSpec, lcg_faces
+
+
+def faces_from(g: O.Grid, axis: int, f, dtype=np.float32):
+    """Sample f at face positions in x-fastest order (tests/sampling_contract.rs:2-13)."""
+    fx, fy, fz = g.face_counts(axis)
+    out = np.empty(fx * fy * fz, dtype)
+    n = 0
+    for z in range(fz):
+        for y in range(fy):
+            for x in range(fx):
+                p = [g.origin[d] + ((x, y, z)[d] + (0.0 if d == axis else 0.5)) * g.spacing[d] for d in range(3)]
+                out[n] = f(p)
+                n += 1
+    return out
+
+
+# --------------------------------------------------------------- geometry
+def test_geometry_shapes_and_byte_plan():
+    # tests/geometry_contract.rs:4-16, :60-73
+    g = O.Grid((3, 2, 4), (0.2, 0.3, 0.4), (-1.0, 2.0, 0.0))
+    assert g.cell_len == 24
+    assert g.face_counts(0) == (4, 2, 4)
+    assert g.face_counts(1) == (3, 3, 4)
+    assert g.face_counts(2) == (3, 2, 5)
+    g = O.Grid((64, 64, 64), (1 / 64,) * 3)
+    plan = g.buffer_plan()
+    assert plan == {"velocity_pair_bytes": 6_389_760, "tracer_pair_bytes": 2_097_152,
+                    "pressure_workspace_bytes": 12_582_912, "total_bytes": 21_069_824}
+
+
+@pytest.mark.parametrize("counts,spacing,origin", [
+    ((0, 1, 1), (1.0,) * 3, (0.0,) * 3),
+    ((1, 1, 1), (0.0, 1.0, 1.0), (0.0,) * 3),
+    ((1, 1, 1), (-1.0, 1.0, 1.0), (0.0,) * 3),
+    ((1, 1, 1), (math.nan, 1.0, 1.0), (0.0,) * 3),
+    ((1, 1, 1), (math.inf, 1.0, 1.0), (0.0,) * 3),
+    ((1, 1, 1), (1.0,) * 3, (math.nan, 0.0, 0.0)),
+    ((1, 1, 1), (2.2250738585072014e-308,) * 3, (0.0,) * 3),
+    ((2, 2, 2), (1.0,) * 3, (1e30, 0.0, 0.0)),
+])
+def test_geometry_rejections(counts, spacing, origin):
+    # tests/geometry_contract.rs:38-57
+    with pytest.raises(O.OracleError):
+        O.Grid(counts, spacing, origin)
+
+
+# --------------------------------------------------------------- operator
+def test_three_cell_hand_calculation():
+    # tests/operator_contract.rs:4-26
+    g = O.Grid((3, 1, 1), (1.0,) * 3)
+    x = np.array([0.0, 2.0, -1.0, 0.0], np.float32)
+    y = np.zeros(6, np.float32)
+    z = np.zeros(6, np.float32)
+    rhs = O.build_rhs(g, [x, y, z], 0.5)
+    assert rhs.tolist() == [-4.0, 6.0, -2.0]
+    pressure = np.array([0.0, 4.0, 2.0])
+    assert O.apply_full(g, 1.0, pressure).tolist() == rhs.tolist()
+    ox, oy, oz = O.correct(g, 1.0, pressure, 0.5, [x, y, z])
+    assert ox.tolist() == [0.0] * 4
+    assert O.divergence(g, [ox, oy, oz]).tolist() == [0.0] * 3
+
+
+def test_rectangular_dense_operator():
+    # tests/operator_contract.rs:29-57
+    g = O.Grid((3, 2, 1), (0.5, 2.0, 3.0))
+    edges = [(0, 1, 6.0), (1, 2, 6.0), (3, 4, 6.0), (4, 5, 6.0), (0, 3, 0.375), (1, 4, 0.375), (2, 5, 0.375)]
+    p = np.array([0.7, -0.3, 1.4, 2.0, -1.0, 0.9])
+    expected = np.zeros(6)
+    for a, b, w in edges:
+        v = w * (p[a] - p[b])
+        expected[a] += v
+        expected[b] -= v
+    actual = O.apply_full(g, 2.0, p)
+    assert np.max(np.abs(actual - expected)) < 1e-12
+    assert O.apply_full(g, 2.0, np.full(6, 3.0)).tolist() == [0.0] * 6
+
+
+def test_integrated_rhs_matches_physical_divergence():
+    # tests/operator_contract.rs:60-75
+    g = O.Grid((2, 1, 1), (0.25, 0.5, 2.0))
+    x = np.array([0.0, 0.75, 0.0], np.float32)
+    y = np.zeros(4, np.float32)
+    z = np.zeros(4, np.float32)
+    rhs = O.build_rhs(g, [x, y, z], 0.1)
+    div = O.divergence(g, [x, y, z])
+    assert div.tolist() == [3.0, -3.0]
+    assert rhs.tolist() == [-7.5, 7.5]
+    assert np.all(np.abs(div + 0.1 * rhs / g.volume) < 1e-12)
+
+
+def test_single_cell_walls_and_rejections():
+    # tests/operator_contract.rs:78-111
+    g = O.Grid((1, 1, 1), (1.0,) * 3)
+    zero = np.zeros(2, np.float32)
+    assert O.build_rhs(g, [zero, zero, zero], 1.0).tolist() == [0.0]
+    assert O.apply_full(g, 1.0, np.array([5.0])).tolist() == [0.0]
+    with pytest.raises(O.OracleError) as e:
+        O.build_rhs(g, [np.array([0.0, 1.0], np.float32), zero, zero], 1.0)
+    assert e.value.status == O.NONZERO_WALL_VELOCITY
+    g = O.Grid((2, 2, 2), (1.0,) * 3)
+    with pytest.raises(O.OracleError) as e:
+        O.apply_full(g, 1.0, np.full(8, np.inf))
+    assert e.value.status == O.NONFINITE_INPUT
+
+
+def test_cached_diagonal_matches_original_order():
+    # src/operator.rs:898-924: diagonal bits equal the lower/upper X,Y,Z sum
+    for counts in [(5, 4, 3), (1, 7, 2), (3, 1, 1), (1, 1, 1)]:
+        h = (0.13, 0.27, 0.41)
+        g = O.Grid(counts, h, (-1.0, 2.0, 0.3))
+        rho = 1.37
+        area = [h[1] * h[2], h[0] * h[2], h[0] * h[1]]
+        w = [area[d] * (1.0 / (rho * h[d])) for d in range(3)]
+        for z in range(counts[2]):
+            for y in range(counts[1]):
+                for x in range(counts[0]):
+                    p = (x, y, z)
+                    orig = 0.0
+                    for d in range(3):
+                        if p[d] > 0:
+                            orig += w[d]
+                        if p[d] + 1 < counts[d]:
+                            orig += w[d]
+                    assert O.diagonal(g, rho, p) == orig
+
+
+def test_operator_against_dense_assembly():
+    # brute force: K from the independent edge list, K.1 = 0, K symmetric PSD
+    g = O.Grid((4, 3, 2), (0.3, 0.2, 0.5))
+    n = g.cell_len
+    K = np.stack([O.apply_full(g, 1.7, np.eye(n)[i]) for i in range(n)], axis=1)
+    assert np.allclose(K, K.T, atol=1e-12)
+    assert np.allclose(K @ np.ones(n), 0.0, atol=1e-12)
+    assert np.min(np.linalg.eigvalsh(K)) > -1e-12
+    # off-diagonal entries equal -area/(rho*h) for the six neighbours
+    h = (0.3, 0.2, 0.5)
+    area = [h[1] * h[2], h[0] * h[2], h[0] * h[1]]
+    for a, stride in enumerate((1, 4, 12)):
+        assert math.isclose(K[0, stride], -area[a] / (1.7 * h[a]), rel_tol=1e-14)
+
+
+# --------------------------------------------------------------- sampling
+def test_affine_fields_are_exact():
+    # tests/sampling_contract.rs:14-26
+    g = O.Grid((4, 5, 6), (0.5, 0.25, 0.125), (-1.0, -0.5, 0.0))
+    f = lambda p: p[0] + 2.0 * p[1] - 3.0 * p[2] + 2.0
+    for axis in range(3):
+        values = faces_from(g, axis, f)
+        p = [-0.2, 0.2, 0.4]
+        assert abs(O.sample(g, axis, values, p) - f(p)) < 1e-12
+
+
+def test_convex_bounds_and_clamping():
+    # tests/sampling_contract.rs:27-38
+    g = O.Grid((2, 1, 1), (1.0,) * 3)
+    v = np.array([-2.0, 8.0], np.float32)
+    assert O.sample(g, 3, v, [1.0, 0.5, 0.5]) == 3.0
+    assert O.sample(g, 3, v, [-1.7976931348623157e308] * 3) == -2.0
+    assert O.sample(g, 3, v, [1.7976931348623157e308] * 3) == 8.0
+    with pytest.raises(O.OracleError) as e:
+        O.sample(g, 3, v, [math.nan] * 3)
+    assert e.value.status == O.NONFINITE_POSITION
+
+
+def test_backtrace_translation_and_rotation():
+    # tests/sampling_contract.rs:39-64
+    g = O.Grid((8, 8, 8), (1.0,) * 3)
+    u = faces_from(g, 0, lambda p: 1.0)
+    v = faces_from(g, 1, lambda p: -2.0)
+    w = faces_from(g, 2, lambda p: 0.5)
+    assert O.backtrace(g, [u, v, w], [4.0] * 3, 0.5) == [3.5, 5.0, 3.75]
+    assert O.backtrace(g, [u, v, w], [0.1, 7.9, 0.1], 0.5) == [0.0, 8.0, 0.0]
+    with pytest.raises(O.OracleError):
+        O.backtrace(g, [u, v, w], [4.0] * 3, 0.0)
+    g = O.Grid((8, 8, 1), (1.0,) * 3, (-4.0, -4.0, 0.0))
+    u = faces_from(g, 0, lambda p: -p[1])
+    v = faces_from(g, 1, lambda p: p[0])
+    w = faces_from(g, 2, lambda p: 0.0)
+    dt = 0.1
+    a = O.backtrace(g, [u, v, w], [1.0, 0.0, 0.5], dt)
+    assert abs(a[0] - (1.0 - dt * dt / 2.0)) < 1e-14
+    assert abs(a[1] + dt) < 1e-14
+
+
+def test_tracer_pulse_and_rest_state():
+    # tests/advection_contract.rs:68-96
+    g = O.Grid((8, 1, 1), (1.0,) * 3)
+    u = np.ones(g.face_len(0), np.float32)
+    v = np.zeros(g.face_len(1), np.float32)
+    w = np.zeros(g.face_len(2), np.float32)
+    old = np.array([0, 0, 1, 0, 0, 0, 0, 0], np.float32)
+    assert O.advect_tracer(g, old, [u, v, w], 1.0).tolist() == [0, 0, 0, 1, 0, 0, 0, 0]
+    g = O.Grid((1, 1, 1), (0.5, 0.25, 0.125))
+    zero = np.zeros(2, np.float32)
+    outs = O.advect_velocity(g, [zero, zero, zero], 0.1)
+    assert all(o.tolist() == [0.0, 0.0] for o in outs)
+    assert O.advect_tracer(g, np.array([0.37], np.float32), outs, 0.1).tolist() == [np.float32(0.37)]
+
+
+def test_advection_matches_per_face_backtrace():
+    # tests/advection_contract.rs:6-41 pattern: dense per-element RK2 bits,
+    # here for velocity faces on an anisotropic offset grid and three dt.
+    g = O.Grid((9, 7, 5), (0.07, 0.11, 0.19), (-2.0, 4.0, 0.3))
+    vel = []
+    for d in range(3):
+        x = np.arange(g.face_len(d))
+        f = (0.8 * np.sin(x * 0.017 + d)).astype(np.float32)
+        fc = g.face_counts(d)
+        f3 = f.reshape(fc[2], fc[1], fc[0])
+        idx = [slice(None)] * 3
+        idx[2 - d] = 0
+        f3[tuple(idx)] = 0
+        idx[2 - d] = -1
+        f3[tuple(idx)] = 0
+        vel.append(f3.ravel())
+    for dt in (0.01, 0.5, 3.0):
+        outs = O.advect_velocity(g, vel, dt)
+        for d in range(3):
+            fc = g.face_counts(d)
+            for k in range(fc[2]):
+                for j in range(fc[1]):
+                    for i in range(fc[0]):
+                        p = (i, j, k)
+                        idx = i + fc[0] * (j + fc[1] * k)
+                        if p[d] == 0 or p[d] == fc[d] - 1:
+                            assert outs[d][idx] == 0.0
+                            continue
+                        pos = [g.origin[a] + (p[a] + (0.0 if a == d else 0.5)) * g.spacing[a] for a in range(3)]
+                        dep = O.backtrace(g, vel, pos, dt)
+                        assert outs[d][idx] == np.float32(O.sample(g, d, vel[d], dep))
+
+
+# ------------------------------------------------------------- reductions
+def test_compensated_sum_recovers_cancellation():
+    x = np.array([1e16, 1.0, -1e16, 1.0])
+    assert O.compensated_sum(x) == 2.0
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(10000)
+    exact = math.fsum(a)
+    assert O.compensated_sum(a) == exact
+
+
+# ------------------------------------------------------------- pressure
+def settings(**kw):
+    s = O.Settings(1e-11, 1e-12, 1e-9, 1000)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def test_hand_chain_and_zero_rhs():
+    # tests/pressure_contract.rs:12-37
+    g = O.Grid((3, 1, 1), (1.0,) * 3)
+    r = O.solve_rhs(g, 1.0, np.array([-4.0, 6.0, -2.0]), 0.5, settings())
+    assert r.ok
+    assert abs(r.pressure[1] - 4.0) < 1e-10 and abs(r.pressure[2] - 2.0) < 1e-10
+    assert r.pressure[0] == 0.0 and r.true_residual_max < 1e-10
+    g = O.Grid((1, 1, 1), (1.0,) * 3)
+    r = O.solve_rhs(g, 1.0, np.array([0.0]), 1.0, settings())
+    assert r.ok and r.iterations == 0 and r.true_residual_max == 0.0
+
+
+def test_incompatible_and_gauge_row_stopping_and_limits():
+    # tests/pressure_contract.rs:39-71, :159-192
+    g = O.Grid((2, 1, 1), (1.0,) * 3)
+    assert O.solve_rhs(g, 1.0, np.array([1.0, 0.0]), 1.0, settings()).status == O.INCOMPATIBLE_RHS
+    g = O.Grid((4, 1, 1), (1.0,) * 3)
+    r = O.solve_rhs(g, 1.0, np.array([-0.003, 0.001, 0.001, 0.001]), 1.0,
+                    O.Settings(1.0, 1.0, 0.0015, 0))
+    assert r.status == O.ITERATION_LIMIT and r.error_iteration == 0
+    g = O.Grid((3, 1, 1), (1.0,) * 3)
+    r = O.solve_rhs(g, 1.0, np.array([-4.0, 6.0, -2.0]), 0.5, settings(max_iterations=1))
+    assert r.status == O.ITERATION_LIMIT and r.error_iteration == 1
+    g = O.Grid((2, 2, 2), (1.0,) * 3)
+    assert O.solve_rhs(g, 1.0, np.zeros(8), 1.0, settings(relative_residual=math.nan)).status == O.INVALID_SETTINGS
+
+
+def test_manufactured_pressure_recovered_modulo_gauge():
+    # tests/pressure_contract.rs:73-88
+    g = O.Grid((5, 4, 3), (0.2, 0.25, 0.3))
+    exact = np.sin(np.arange(g.cell_len) * 0.7)
+    rhs = O.apply_full(g, 1.3, exact)
+    r = O.solve_rhs(g, 1.3, rhs, 0.02, settings())
+    assert r.ok
+    assert np.max(np.abs(r.pressure - (exact - exact[0]))) < 1e-8
+    assert r.predicted_divergence_max <= 1e-9
+
+
+def test_tighter_divergence_gate_keeps_cg_progress():
+    # tests/pressure_contract.rs:90-157
+    counts, spacing, rho = (5, 4, 3), (0.2, 0.25, 0.3), 1.3
+    g = O.Grid(counts, spacing)
+    exact = np.array([((i * 17) % 31) / 31.0 for i in range(g.cell_len)])
+
+    def product(p):
+        strides = [1, counts[0], counts[0] * counts[1]]
+        vol = spacing[0] * spacing[1] * spacing[2]
+        out = np.zeros(g.cell_len)
+        for row in range(g.cell_len):
+            q = [row % counts[0], (row // counts[0]) % counts[1], row // strides[2]]
+            v = 0.0
+            for d in range(3):
+                w = vol / (rho * spacing[d] * spacing[d])
+                if q[d] > 0:
+                    v += w * (p[row] - p[row - strides[d]])
+                if q[d] + 1 < counts[d]:
+                    v += w * (p[row] - p[row + strides[d]])
+            out[row] = v
+        return out
+
+    rhs = product(exact)
+    lim = O.Settings(1e-3, 1e-12, 1e-9, 120)
+    r = O.solve_rhs(g, rho, rhs, 0.02, lim)
+    assert r.ok
+    res = rhs - product(r.pressure)
+    assert np.sqrt(np.sum(res * res)) <= max(1e-12, 1e-3 * np.sqrt(np.sum(rhs * rhs)))
+    assert np.max(np.abs(res)) * 0.02 / g.volume <= 1e-9
+    assert r.pressure[0] == 0.0
+    assert np.max(np.abs(r.pressure - exact)) < 1e-8
+
+
+def test_random_projection_lcg_field():
+    # tests/pressure_contract.rs:194-235 (LCG seed 0x12345678)
+    spec = GridSpec((8, 7, 6), (0.2, 0.3, 0.4))
+    g = O.Grid(spec.counts, spec.spacing)
+    fields = lcg_faces(spec, 0x1234_5678)
+    r = O.solve_velocity(g, 1.0, fields, 0.02, settings())
+    assert r.ok and r.predicted_divergence_max < 1e-9
+    out = O.correct(g, 1.0, r.pressure, 0.02, fields)
+    assert O.actual_divergence_max(g, out) < 1e-5
+    before = sum(float(np.sum(f.astype(np.float64) ** 2)) for f in fields)
+    after = sum(float(np.sum(f.astype(np.float64) ** 2)) for f in out)
+    assert after <= before + 1e-6
+
+
+def test_circulation_needs_zero_iterations():
+    # tests/pressure_contract.rs:237-262
+    g = O.Grid((2, 2, 1), (1.0,) * 3)
+    x = np.array([0, 1, 0, 0, -1, 0], np.float32)
+    y = np.array([0, 0, -1, 1, 0, 0], np.float32)
+    z = np.zeros(8, np.float32)
+    r = O.solve_velocity(g, 1.0, [x, y, z], 0.1, settings())
+    assert r.ok and r.iterations == 0
+    ox, oy, oz = O.correct(g, 1.0, r.pressure, 0.1, [x, y, z])
+    assert ox.tolist() == x.tolist() and oy.tolist() == y.tolist()
+
+
+def test_sgs_known_answer():
+    # src/pressure.rs:824-842
+    g = O.Grid((2, 2, 1), (1.0,) * 3)
+    r, z, rz = O.precondition(g, 1.0, np.array([-12.0, 2.0, 4.0, 6.0]), "sgs-pcg-v1")
+    assert z.tolist() == [0.0, 3.25, 4.25, 4.5]
+    assert rz == 50.5
+    assert r[0] == 0.0
+
+
+def _dense_K(g, rho):
+    n = g.cell_len
+    return np.stack([O.apply_full(g, rho, np.eye(n)[i]) for i in range(n)], axis=1)
+
+
+def test_sgs_and_rbsgs_match_dense_triangular_solves():
+    # M = (D+L) D^-1 (D+U) on the gauge-eliminated system, solved densely.
+    g = O.Grid((4, 3, 3), (0.3, 0.2, 0.5))
+    rho = 1.1
+    K = _dense_K(g, rho)[1:, 1:]
+    rng = np.random.default_rng(7)
+    r = rng.standard_normal(g.cell_len)
+    for method, order in (("sgs-pcg-v1", np.arange(1, g.cell_len)), ("rbsgs-pcg", None)):
+        idx = np.arange(1, g.cell_len)
+        if order is None:
+            i, j, k = idx % 4, (idx // 4) % 3, idx // 12
+            colour = (i + j + k) & 1
+            order = np.concatenate([idx[colour == 0], idx[colour == 1]])
+        perm = order - 1
+        A = K[np.ix_(perm, perm)]
+        D = np.diag(np.diag(A))
+        L = np.tril(A, -1)
+        M = (D + L) @ np.linalg.inv(D) @ (D + L).T
+        zr = np.linalg.solve(M, r[order])
+        expected = np.zeros(g.cell_len)
+        expected[order] = zr
+        rr, z, rz = O.precondition(g, rho, r, method)
+        assert np.max(np.abs(z - expected)) < 1e-10 * np.max(np.abs(expected))
+        assert z[0] == 0.0
+        assert math.isclose(rz, float(rr @ z), rel_tol=1e-13)
+
+
+def test_jacobi_pcg_matches_dense_least_squares():
+    g = O.Grid((5, 4, 3), (0.2, 0.25, 0.3))
+    rho = 1.3
+    K = _dense_K(g, rho)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(g.cell_len)
+    b -= b.mean()
+    ref = np.zeros(g.cell_len)
+    ref[1:] = np.linalg.lstsq(K[1:, 1:], b[1:], rcond=None)[0]
+    for method in ("jacobi-pcg-v1", "sgs-pcg-v1", "rbsgs-pcg"):
+        r = O.solve_rhs(g, rho, b, 0.02, settings(divergence_limit=1.0))
+        r = O.solve_rhs(g, rho, b, 0.02, settings(divergence_limit=1.0), method=method)
+        assert r.ok, method
+        assert np.max(np.abs(r.pressure - ref)) < 1e-8, method
+
+
+def test_thread_count_bit_parity():
+    # tests/parallel_contract.rs:70-97: every bit equal across pool sizes
+    spec = GridSpec.unit_box(32)
+    g = O.Grid(spec.counts, spec.spacing)
+    fields = lcg_faces(spec, 42)
+    s = settings(relative_residual=1e-9, divergence_limit=1e-7)
+    r1 = O.solve_velocity(g, 1.0, fields, 0.01, s, threads=1)
+    r4 = O.solve_velocity(g, 1.0, fields, 0.01, s, threads=4)
+    assert r1.ok and r4.ok and r1.iterations == r4.iterations
+    assert r1.pressure.tobytes() == r4.pressure.tobytes()
+    a1 = O.advect_velocity(g, fields, 0.01, threads=1)
+    a4 = O.advect_velocity(g, fields, 0.01, threads=4)
+    assert all(x.tobytes() == y.tobytes() for x, y in zip(a1, a4))
+
+
+# --------------------------------------------------------------- goldens
+def steady_scene(n):
+    """examples/accepted_pressure_lab.rs:47-112 'steady' step 0: unit box n^3,
+    rho 1.25, from rest, acceleration (0.125, -0.25, 0.0625), dt 1/64
+    (src/forces.rs:144-205 gives f32(0 + dt*a) on interior faces)."""
+    spec = GridSpec.unit_box(n)
+    g = O.Grid(spec.counts, spec.spacing)
+    dt = 1.0 / 64
+    acc = (0.125, -0.25, 0.0625)
+    vel = []
+    for d in range(3):
+        m = spec.interior_mask(d).ravel()
+        f = np.zeros(m.size, np.float32)
+        f[m] = np.float32(0.0 + dt * acc[d])
+        vel.append(f)
+    return spec, g, vel, dt, acc
+
+
+GOLDEN = {
+    # docs/plans/accepted-pressure-warm-start/pilot/steady-cold.csv:2
+    32: dict(iterations=248, l2=1.18841115931624628e-10, rmax=3.64179933225964453e-12,
+             div=1.86460126505583190e-9, speed=3.47726951399618400e-11, perr=3.69079031226071663e-10),
+    # docs/plans/accepted-pressure-warm-start/interrupted/64-64-64-w1-steady-jacobi-r4-base.csv:2
+    64: dict(iterations=504, l2=5.85703360090484362e-11, rmax=8.33838025550101038e-13,
+             div=3.41540054571431995e-9, speed=3.08400423254528988e-11, perr=4.20643117560182134e-10),
+}
+
+
+@pytest.mark.parametrize("n", [32, 64])
+def test_steady_scene_golden_bits(n):
+    spec, g, vel, dt, acc = steady_scene(n)
+    r = O.solve_velocity(g, 1.25, vel, dt, O.Settings(1e-8, 1e-12, 1e-7, 3000), threads=8)
+    gold = GOLDEN[n]
+    assert r.ok
+    assert r.iterations == gold["iterations"]
+    assert r.true_residual_l2 == gold["l2"]
+    assert r.true_residual_max == gold["rmax"]
+    out = O.correct(g, 1.25, r.pressure, dt, vel)
+    assert O.actual_divergence_max(g, out) == gold["div"]
+    assert max(float(np.max(np.abs(o))) for o in out) == np.float32(gold["speed"])
+    # hydrostatic closed form p = rho a.x relative to the first centre
+    # (examples/accepted_pressure_lab.rs:146-163)
+    i = np.arange(g.cell_len)
+    xyz = [i % n, (i // n) % n, i // (n * n)]
+    exact = sum(1.25 * acc[d] * xyz[d] * spec.spacing[d] for d in range(3))
+    assert float(np.max(np.abs(r.pressure - exact))) == gold["perr"]
