@@ -129,6 +129,14 @@ rh_status rh_grid_validate(const rh_grid* grid, uint64_t* cells);
 rh_status rh_buffer_plan(const rh_grid* grid, uint64_t* velocity_pair, uint64_t* tracer_pair,
                          uint64_t* pressure_workspace, uint64_t* total);
 
+/* Kernel timing with HIP events recorded on each kernel's own stream
+ * (diagnostics; no reference counterpart).  enable(1) clears the totals;
+ * query() returns the launches and summed seconds of one kernel, by name
+ * ("pcg_update", "mg_smooth", ...).  Launches enqueued after a solve has
+ * converged (which return at once) are not counted. */
+rh_status rh_profile_enable(int32_t on);
+rh_status rh_profile_query(const char* kernel, int64_t* launches, double* seconds);
+
 /* ------------------------------------------------ operator kernels (device) */
 /* PressureOperator::apply_full, box path (src/operator.rs:378-456). f64. */
 rh_status rh_apply_full(const rh_grid* grid, double density, const double* d_x, double* d_y);

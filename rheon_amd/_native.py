@@ -143,6 +143,8 @@ SIGNATURES = {
     "rh_device_synchronize": (_S, []),
     "rh_grid_validate": (_S, [_G, C.POINTER(C.c_uint64)]),
     "rh_buffer_plan": (_S, [_G] + [C.POINTER(C.c_uint64)] * 4),
+    "rh_profile_enable": (_S, [C.c_int32]),
+    "rh_profile_query": (_S, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     "rh_apply_full": (_S, [_G, C.c_double, _P, _P]),
     "rh_advect_velocity": (_S, [_G, C.c_int, _P, _P, _P, C.c_double, _P, _P, _P]),
     "rh_advect_tracer": (_S, [_G, _P, _P, _P, _P, C.c_double, _P]),

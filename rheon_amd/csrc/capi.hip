@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "profiler.hpp"
 
 using rh::DevGrid;
 
@@ -753,6 +754,7 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
     rh::launch_validate<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], 0, scr.flags, st);
+    rh::prof::Scope ps(rh::prof::kAdvect, st, -1);
     rh::launch_advect<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], dt, sc->cand[0], sc->cand[1],
                          sc->cand[2], scr.flags, st);
     return RH_OK;
@@ -762,6 +764,7 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
     rh::launch_validate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], 1, scr.flags, st);
+    rh::prof::Scope ps(rh::prof::kRhs, st, -1);
     rh::launch_rhs<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], dt, sc->rhs, scr.flags, st);
     return RH_OK;
   });
@@ -777,8 +780,14 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   RH_HIP(hipEventRecord(sc->ev[3], st));
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
-    rh::launch_correct<T>(g, sc->p, dt, sc->cand[0], sc->cand[1], sc->cand[2], scr.flags, st);
-    rh::launch_divergence_max<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(0), st);
+    {
+      rh::prof::Scope ps(rh::prof::kCorrect, st, -1);
+      rh::launch_correct<T>(g, sc->p, dt, sc->cand[0], sc->cand[1], sc->cand[2], scr.flags, st);
+    }
+    {
+      rh::prof::Scope ps(rh::prof::kDivMax, st, -1);
+      rh::launch_divergence_max<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(0), st);
+    }
     rh::launch_velocity_rate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(1), st);
     return RH_OK;
   });
@@ -789,6 +798,7 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   if (f) return rh::flags_to_status(f);
   double sv[2];
   if ((s = scr.read_scalars(sv, 2, 0))) return s;
+  if (rh::prof::enabled()) rh::prof::settle(INT32_MAX);
   float ms[4] = {0, 0, 0, 0}, total = 0;
   for (int i = 0; i < 4; ++i) RH_HIP(hipEventElapsedTime(&ms[i], sc->ev[i], sc->ev[i + 1]));
   RH_HIP(hipEventElapsedTime(&total, sc->ev[0], sc->ev[4]));

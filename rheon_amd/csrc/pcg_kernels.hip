@@ -19,6 +19,7 @@
 // arithmetic) or float (halved traffic; scalars and arithmetic stay f64).
 #include "common.hpp"
 #include "kernels.hpp"
+#include "profiler.hpp"
 
 namespace rh {
 
@@ -385,24 +386,60 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
   T* z = (T*)a.z;
   const bool sgs = method == RH_RBSGS_PCG;
   if (it > 0) {
-    k_apply_dir<T><<<grid, block, 0, s>>>(g, d, q, ctl, pdd, pmx, it);
-    if (sgs) {
-      k_update<T, true><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
-      k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 0);
-      k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 0);
-    } else {
-      k_update<T, false><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
+    {
+      prof::Scope ps(prof::kApplyDir, s, it);
+      k_apply_dir<T><<<grid, block, 0, s>>>(g, d, q, ctl, pdd, pmx, it);
     }
-    k_true_residual<T><<<grid, block, 0, s>>>(g, p, a.b, q, ctl, pdd, pmx, it);
+    if (sgs) {
+      {
+        prof::Scope ps(prof::kUpdate, s, it);
+        k_update<T, true><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
+      }
+      {
+        prof::Scope ps(prof::kSgsBlack, s, it);
+        k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 0);
+      }
+      {
+        prof::Scope ps(prof::kSgsRed, s, it);
+        k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 0);
+      }
+    } else {
+      {
+        prof::Scope ps(prof::kUpdate, s, it);
+        k_update<T, false><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
+      }
+    }
+    {
+      prof::Scope ps(prof::kTrueResidual, s, it);
+      k_true_residual<T><<<grid, block, 0, s>>>(g, p, a.b, q, ctl, pdd, pmx, it);
+    }
   }
   if (sgs) {
-    k_replace<T, true><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
-    k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 1);
-    k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 1);
-    k_direction<T, true><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+    {
+      prof::Scope ps(prof::kReplace, s, it);
+      k_replace<T, true><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
+    }
+    {
+      prof::Scope ps(prof::kSgsBlack, s, it);
+      k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 1);
+    }
+    {
+      prof::Scope ps(prof::kSgsRed, s, it);
+      k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 1);
+    }
+    {
+      prof::Scope ps(prof::kDirection, s, it);
+      k_direction<T, true><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+    }
   } else {
-    k_replace<T, false><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
-    k_direction<T, false><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+    {
+      prof::Scope ps(prof::kReplace, s, it);
+      k_replace<T, false><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
+    }
+    {
+      prof::Scope ps(prof::kDirection, s, it);
+      k_direction<T, false><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+    }
   }
 }
 

@@ -1,0 +1,120 @@
+// Event-based kernel timer (see profiler.hpp).
+#include "profiler.hpp"
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+
+namespace rh {
+namespace prof {
+
+namespace {
+
+struct Pair {
+  hipEvent_t a = nullptr, b = nullptr;
+  int slot = -1;
+  int tag = -1;
+};
+
+struct State {
+  std::mutex mu;
+  bool on = false;
+  std::vector<Pair> pool;    // reusable event pairs
+  std::vector<int> open;     // pool index of the open pair per slot
+  size_t used = 0;
+  int64_t launches[kNumSlots] = {};
+  double seconds[kNumSlots] = {};
+};
+
+State& st() {
+  static State s;
+  return s;
+}
+
+const char* kNames[kNumSlots] = {
+    "pcg_apply_dir", "pcg_update",      "pcg_true_residual", "pcg_replace",   "pcg_direction",
+    "pcg_sgs_black", "pcg_sgs_red",     "advect_velocity",   "build_rhs",     "correct",
+    "divergence_max", "mg_smooth",      "mg_residual",       "mg_restrict",   "mg_prolong",
+    "mg_coarse",      "outer_residual", "outer_update",      "batch_step"};
+
+}  // namespace
+
+bool enabled() { return st().on; }
+
+const char* name(int slot) { return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?"; }
+
+void start(int slot, hipStream_t s, int tag) {
+  State& S = st();
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (S.open.empty()) S.open.assign(kNumSlots, -1);
+  if (S.used == S.pool.size()) {
+    Pair p;
+    if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) return;
+    S.pool.push_back(p);
+  }
+  Pair& p = S.pool[S.used];
+  p.slot = slot;
+  p.tag = tag;
+  (void)hipEventRecord(p.a, s);
+  S.open[slot] = (int)S.used;
+  ++S.used;
+}
+
+void stop(int slot, hipStream_t s) {
+  State& S = st();
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (S.open.empty() || S.open[slot] < 0) return;
+  (void)hipEventRecord(S.pool[S.open[slot]].b, s);
+  S.open[slot] = -1;
+}
+
+void settle(int keep_max_tag) {
+  State& S = st();
+  std::lock_guard<std::mutex> lk(S.mu);
+  for (size_t i = 0; i < S.used; ++i) {
+    Pair& p = S.pool[i];
+    if (p.tag >= 0 && p.tag > keep_max_tag) continue;
+    if (hipEventSynchronize(p.b) != hipSuccess) continue;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) continue;
+    S.launches[p.slot] += 1;
+    S.seconds[p.slot] += ms * 1e-3;
+  }
+  S.used = 0;
+}
+
+void enable(bool on) {
+  State& S = st();
+  std::lock_guard<std::mutex> lk(S.mu);
+  S.on = on;
+  S.used = 0;
+  std::memset(S.launches, 0, sizeof(S.launches));
+  std::memset(S.seconds, 0, sizeof(S.seconds));
+}
+
+bool query(const char* kernel, int64_t* launches, double* seconds) {
+  State& S = st();
+  std::lock_guard<std::mutex> lk(S.mu);
+  for (int i = 0; i < kNumSlots; ++i)
+    if (std::strcmp(kernel, kNames[i]) == 0) {
+      *launches = S.launches[i];
+      *seconds = S.seconds[i];
+      return true;
+    }
+  return false;
+}
+
+}  // namespace prof
+}  // namespace rh
+
+extern "C" rh_status rh_profile_enable(int32_t on) {
+  rh::prof::enable(on != 0);
+  return RH_OK;
+}
+
+extern "C" rh_status rh_profile_query(const char* kernel, int64_t* launches, double* seconds) {
+  if (!kernel || !launches || !seconds) return RH_INVALID_ARGUMENT;
+  return rh::prof::query(kernel, launches, seconds) ? RH_OK : RH_INVALID_ARGUMENT;
+}

@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "profiler.hpp"
 
 namespace rh {
 
@@ -299,6 +300,7 @@ rh_status PcgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressur
   RH_HIP(hipMemcpyAsync(slot[0], scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
   RH_HIP(hipStreamSynchronize(s));
   const PcgControl& c = *slot[0];
+  if (prof::enabled()) prof::settle(c.iteration);
   float ms = 0.0f;
   RH_HIP(hipEventElapsedTime(&ms, scratch_.ev0, scratch_.ev1));
   if (report) {
