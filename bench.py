@@ -29,12 +29,31 @@ from rheon_amd.scenes import GridSpec, courant_dt, fourier_modes  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
+def mg_cycle_bytes(spec: GridSpec, s: int, pre: int = 2, post: int = 2) -> float:
+    """Compulsory bytes of one defect-correction cycle (outer f64 residual +
+    V(pre, post) in storage of s bytes): per finest cell
+      outer residual  read p, b (f64) + write b0, x0           16 + 2s
+      extra pre-sweeps  (pre-1) x (read x, b + write x)         3s each
+      restrict  read x, b + write coarse b, x (1/8 cells)       2s + 2s/8
+      prolong   read x + coarse x (1/8) + write x               2s + s/8
+      post-sweeps (post-1) x 3s, last one adds to p (f64 r/w)   2s + 16
+    and coarse levels l >= 1 the same minus the f64 terms, on N/8^l cells."""
+    n = spec.cell_len
+    finest = (16 + 2 * s) + (pre - 1) * 3 * s + (2 * s + 2 * s / 8) + (2 * s + s / 8) + (post - 1) * 3 * s + (2 * s + 16)
+    coarse_per_cell = (pre - 1) * 3 * s + (2 * s + 2 * s / 8) + (2 * s + s / 8) + post * 3 * s
+    coarse = sum(coarse_per_cell * n / 8 ** l for l in range(1, 8))
+    return finest * n + coarse
+
+
 def algorithmic_bytes(spec: GridSpec, face_bytes: int, method: str, solver_bytes: int, iterations: int) -> dict:
     """Compulsory HBM bytes of one step (per-kernel minimal-fusion model)."""
     n = spec.cell_len
     F = spec.face_total
     sv = solver_bytes
-    per_iter = {"jacobi-pcg-v1": 11 * sv * n, "rbsgs-pcg": 17 * sv * n}.get(method, 0)
+    if method in ("mixed-mg-dc", "mg-f64-dc"):
+        per_iter = mg_cycle_bytes(spec, sv)
+    else:
+        per_iter = {"jacobi-pcg-v1": 11 * sv * n, "rbsgs-pcg": 17 * sv * n}.get(method, 0)
     return {
         "advect": 2 * face_bytes * F,
         "rhs": face_bytes * F + 8 * n,
@@ -52,6 +71,11 @@ KERNEL_BYTES_PER_CELL = {
     "pcg_sgs_black": lambda s: 3 * s,      # read r, z; write z (interleaved colours)
     "pcg_sgs_red": lambda s: 3 * s,        # read r, z; write z
     "pcg_true_residual": lambda s: 8 + 8 + s,  # read p (s), b (8), write product (s)
+    "mg_outer_residual": lambda s: 16 + 2 * s,  # read p, b (f64); write b0, x0
+    "mg_smooth": lambda s: 3 * s,              # read x, b; write x
+    "mg_smooth_update": lambda s: 2 * s + 16,  # read x, b; read+write p (f64)
+    "mg_restrict": lambda s: 2 * s + 2 * s / 8,  # read x, b; write coarse b, x
+    "mg_prolong": lambda s: 2 * s + s / 8,       # read x, coarse x; write x
 }
 
 
@@ -98,7 +122,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--method", default="rbsgs-pcg", choices=["jacobi-pcg-v1", "rbsgs-pcg"])
+    ap.add_argument("--method", default="mixed-mg-dc",
+                    choices=["jacobi-pcg-v1", "rbsgs-pcg", "mixed-mg-dc", "mg-f64-dc"])
     ap.add_argument("--rel", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--json-out", default=None)
@@ -111,8 +136,11 @@ def main() -> int:
     spec = GridSpec.unit_box(args.n)
     vel = fourier_modes(spec, seed=2026, modes=32, kmax=16, amplitude=1.0, dtype=np.float32)
     dt = courant_dt(spec, vel, 0.5)
-    settings = R.PressureSettings(args.rel, 1e-14, 1e-4, 20000, poll_interval=32)
-    scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method, solver_precision=R.F64,
+    mg = args.method in ("mixed-mg-dc", "mg-f64-dc")
+    s_vec = 4 if args.method == "mixed-mg-dc" else 8  # bytes per Krylov / V-cycle value
+    settings = R.PressureSettings(args.rel, 1e-14, 1e-4, 200 if mg else 20000, poll_interval=2 if mg else 32)
+    scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method,
+                    solver_precision=R.F32 if args.method == "mixed-mg-dc" else R.F64,
                     settings=settings, actual_divergence_limit=1e-3, max_courant=0.0)
     scene.set_velocity(vel)
     N.lib().rh_device_synchronize()
@@ -144,13 +172,13 @@ def main() -> int:
     # per-kernel live timings (HIP events on the solver stream)
     import ctypes as C
     kernels = {}
-    for name in list(KERNEL_BYTES_PER_CELL) + ["advect_velocity", "build_rhs", "correct", "divergence_max"]:
+    for name in list(KERNEL_BYTES_PER_CELL) + ["advect_velocity", "build_rhs", "correct", "divergence_max",
+                                               "mg_coarse_levels"]:
         n_l, sec = C.c_int64(), C.c_double()
         if lib.rh_profile_query(name.encode(), C.byref(n_l), C.byref(sec)) == 0 and n_l.value:
             kernels[name] = (n_l.value, sec.value)
     lib.rh_profile_enable(0)
 
-    s_vec = 8  # f64 Krylov vectors
     dom = max((k for k in kernels if k in KERNEL_BYTES_PER_CELL), key=lambda k: kernels[k][1])
     launches, secs = kernels[dom]
     avg = secs / launches
@@ -173,7 +201,8 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 faces, f64 pressure",
+        "dtype": ("f32 faces + f32 V-cycles, f64 outer residual/pressure" if args.method == "mixed-mg-dc"
+                  else "f32 faces, f64 pressure"),
         "data": "synthetic: 32 seeded Fourier modes |k| in [1,16], amplitude ~|k|^-5/3, max|u| = 1",
         "config": {
             "workload": f"projection step {args.n}^3, {args.method} to rel {args.rel:g}",

@@ -48,7 +48,7 @@ STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) a
 F64, F32 = 0, 1
 JACOBI_PCG, RBSGS_PCG, MG_PCG, MIXED_MG = 0, 1, 2, 3
 METHOD_IDS = {"jacobi-pcg-v1": JACOBI_PCG, "rbsgs-pcg": RBSGS_PCG, "mg-pcg": MG_PCG,
-              "mixed-mg-dc": MIXED_MG}
+              "mixed-mg-dc": MIXED_MG, "mg-f64-dc": MIXED_MG}
 
 
 class RheonError(RuntimeError):

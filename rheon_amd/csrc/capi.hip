@@ -56,7 +56,7 @@ const char* method_id(rh_method m, rh_precision p) {
     case RH_JACOBI_PCG: return p == RH_F64 ? "jacobi-pcg-v1" : "jacobi-pcg-f32";
     case RH_RBSGS_PCG: return p == RH_F64 ? "rbsgs-pcg" : "rbsgs-pcg-f32";
     case RH_MG_PCG: return p == RH_F64 ? "mg-pcg" : "mg-pcg-f32";
-    case RH_MIXED_MG: return "mixed-mg-dc";
+    case RH_MIXED_MG: return p == RH_F32 ? "mixed-mg-dc" : "mg-f64-dc";
   }
   return "unknown";
 }
@@ -325,7 +325,7 @@ extern "C" rh_status rh_precondition(const rh_grid* grid, double density, rh_met
 // ------------------------------------------------------- device solver
 
 struct rh_solver {
-  rh::PcgSolver impl;
+  std::unique_ptr<rh::Solver> impl;
 };
 
 extern "C" rh_status rh_solver_create(const rh_grid* grid, const rh_solver_config* config,
@@ -336,7 +336,12 @@ extern "C" rh_status rh_solver_create(const rh_grid* grid, const rh_solver_confi
   if (s) return s;
   auto* solver = new (std::nothrow) rh_solver();
   if (!solver) return RH_DEVICE_ERROR;
-  if ((s = solver->impl.create(*grid, *config))) {
+  solver->impl.reset(rh::make_solver(config->method));
+  if (!solver->impl) {
+    delete solver;
+    return RH_UNSUPPORTED;
+  }
+  if ((s = solver->impl->create(*grid, *config))) {
     delete solver;
     return s;
   }
@@ -347,7 +352,7 @@ extern "C" rh_status rh_solver_create(const rh_grid* grid, const rh_solver_confi
 extern "C" void rh_solver_destroy(rh_solver* solver) { delete solver; }
 
 extern "C" uint64_t rh_solver_allocated_bytes(const rh_solver* solver) {
-  return solver ? solver->impl.allocated_bytes() : 0;
+  return solver ? solver->impl->allocated_bytes() : 0;
 }
 
 extern "C" rh_status rh_solver_solve_rhs(rh_solver* solver, double density, const double* d_rhs,
@@ -355,7 +360,7 @@ extern "C" rh_status rh_solver_solve_rhs(rh_solver* solver, double density, cons
                                          double* d_pressure, rh_pressure_report* report,
                                          rh_cancel_fn cancel, void* cancel_ctx) {
   if (!solver || !settings) return RH_INVALID_ARGUMENT;
-  return solver->impl.solve(density, d_rhs, dt, *settings, d_pressure, report, cancel, cancel_ctx);
+  return solver->impl->solve(density, d_rhs, dt, *settings, d_pressure, report, cancel, cancel_ctx);
 }
 
 // ------------------------------------------------------------- provider
@@ -417,7 +422,8 @@ static rh_status provider_supports(const rh_provider* provider, const rh_pressur
   if (!std::isfinite(req->dt) || req->dt <= 0.0) return why("time step must be finite and positive");
   if (req->rhs_len != cells_of(*req->grid)) return why("rhs length does not match the grid");
   const rh_method m = provider->config.solver.method;
-  if (m != RH_JACOBI_PCG && m != RH_RBSGS_PCG) return why("method not available in this build");
+  if (m != RH_JACOBI_PCG && m != RH_RBSGS_PCG && m != RH_MIXED_MG)
+    return why("method not available in this build");
   return RH_OK;
 }
 
@@ -545,7 +551,7 @@ extern "C" rh_status rh_provider_workspace_solve(rh_provider_workspace* ws,
       settings->absolute_residual < 0.0 || !std::isfinite(settings->divergence_limit) ||
       settings->divergence_limit < 0.0)
     return RH_INVALID_SETTINGS;
-  rh::Scratch& sc = ws->solver->impl.scratch();
+  rh::Scratch& sc = ws->solver->impl->scratch();
   const DevGrid g = rh::make_dev_grid(ws->grid, req->density);
   // Compatibility sums and threshold on the device (:186-206).
   RH_HIP(hipMemcpyAsync(ws->d_rhs, req->rhs, n * 8, hipMemcpyHostToDevice, sc.stream));
@@ -687,7 +693,7 @@ extern "C" rh_status rh_scene_create(const rh_grid* grid, const rh_scene_config*
   if ((s = rh_solver_create(grid, &config->solver, &raw))) return s;
   sc->solver.reset(raw);
   for (auto& e : sc->ev) RH_HIP(hipEventCreate(&e));
-  sc->bytes = total + raw->impl.allocated_bytes();
+  sc->bytes = total + raw->impl->allocated_bytes();
   if (config->solver.memory_limit && sc->bytes > config->solver.memory_limit) return RH_BUFFER_LIMIT;
   *out = sc.release();
   return RH_OK;
@@ -744,7 +750,7 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   rep->dt = dt;
   if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
   if (cancel && cancel(cancel_ctx)) return RH_CANCELLED;
-  rh::Scratch& scr = sc->solver->impl.scratch();
+  rh::Scratch& scr = sc->solver->impl->scratch();
   hipStream_t st = scr.stream;
   const DevGrid g = rh::make_dev_grid(sc->grid, sc->config.density);
   const rh_precision prec = sc->config.precision;

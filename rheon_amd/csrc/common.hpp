@@ -116,15 +116,17 @@ __device__ __forceinline__ void block_reduce_max(double (&v)[NV]) {
 
 __device__ __forceinline__ int flat_tid() { return threadIdx.y * kLanesX + threadIdx.x; }
 
-// "Last block done" grid reduction: each block publishes its partials, and
-// the block that arrives last combines all partials in block-index order
-// (deterministic for a given launch shape) and runs `fin(sums, maxima)` in
-// thread 0.  `counter` must be zero on entry; the last block resets it.
+// Grid reductions are two kernels: every block of the producing kernel
+// publishes its partials with plain stores (block_publish), and a one-block
+// finisher kernel (k_finish) combines them in block-index order -- a fixed
+// order, so results are deterministic for a launch shape -- and runs the
+// decision logic `fin(sums, maxima)` in thread 0.  (A single-kernel
+// "last block" scheme needs a device-scope release fence per block, which on
+// gfx950 writes back and invalidates the whole L2 and destroys stencil reuse.)
 // ND double-double sums and NM maxima (both >= 1; pass dummies if unused).
-template <int ND, int NM, typename Fin>
-__device__ __forceinline__ void grid_finish(DD (&dd)[ND], double (&mx)[NM], DD* __restrict__ pdd,
-                                            double* __restrict__ pmx, unsigned* counter, Fin fin) {
-  __shared__ bool last;
+template <int ND, int NM>
+__device__ __forceinline__ void block_publish(DD (&dd)[ND], double (&mx)[NM], DD* __restrict__ pdd,
+                                              double* __restrict__ pmx) {
   block_reduce_dd<ND>(dd);
   block_reduce_max<NM>(mx);
   if (flat_tid() == 0) {
@@ -132,13 +134,16 @@ __device__ __forceinline__ void grid_finish(DD (&dd)[ND], double (&mx)[NM], DD* 
     for (int q = 0; q < ND; ++q) pdd[q * kMaxBlocks + blockIdx.x] = dd[q];
 #pragma unroll
     for (int q = 0; q < NM; ++q) pmx[q * kMaxBlocks + blockIdx.x] = mx[q];
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    last = prev == gridDim.x - 1;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+// `Fin` provides `bool active() const` (false when the producer returned
+// early without publishing) and `void operator()(const double* sums,
+// const double* maxima) const`, both __device__.
+template <int ND, int NM, typename Fin>
+__global__ void __launch_bounds__(kBlock)
+k_finish(const DD* __restrict__ pdd, const double* __restrict__ pmx, int nb, Fin fin) {
+  if (!fin.active()) return;
   const int t = flat_tid();
   DD a[ND];
   double m[NM];
@@ -146,26 +151,52 @@ __device__ __forceinline__ void grid_finish(DD (&dd)[ND], double (&mx)[NM], DD* 
   for (int q = 0; q < ND; ++q) a[q] = DD{0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = 0.0;
-  for (int b = t; b < (int)gridDim.x; b += kBlock) {
+  for (int b = t; b < nb; b += kBlock) {
 #pragma unroll
-    for (int q = 0; q < ND; ++q) {
-      const volatile DD* src = pdd + q * kMaxBlocks + b;
-      a[q] = dd_add(a[q], DD{src->hi, src->lo});
-    }
+    for (int q = 0; q < ND; ++q) a[q] = dd_add(a[q], pdd[q * kMaxBlocks + b]);
 #pragma unroll
-    for (int q = 0; q < NM; ++q)
-      m[q] = fmax(m[q], *(const volatile double*)(pmx + q * kMaxBlocks + b));
+    for (int q = 0; q < NM; ++q) m[q] = fmax(m[q], pmx[q * kMaxBlocks + b]);
   }
   block_reduce_dd<ND>(a);
   block_reduce_max<NM>(m);
   if (t == 0) {
-    *counter = 0u;
     double sums[ND];
 #pragma unroll
     for (int q = 0; q < ND; ++q) sums[q] = dd_value(a[q]);
     fin(sums, m);
   }
 }
+
+template <int ND, int NM, typename Fin>
+inline void launch_finish(const DD* pdd, const double* pmx, int nb, const Fin& fin, hipStream_t s) {
+  k_finish<ND, NM, Fin><<<1, dim3(kLanesX, kRowsPerBlock), 0, s>>>(pdd, pmx, nb, fin);
+}
+
+// Writes sums/maxima to an output array (stateless reductions).
+struct FinStore {
+  double* out;
+  int mode;  // 0: out[0] = max0; 1: velocity rate; 2: out[0..2] = sum0, max0, max1
+  double h0, h1, h2;
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double* sums, const double* mx) const {
+    if (mode == 0) {
+      out[0] = mx[0];
+    } else if (mode == 1) {
+      double rate = 0.0;
+      rate += mx[0] / h0;
+      rate += mx[1] / h1;
+      rate += mx[2] / h2;
+      out[0] = rate;
+      out[1] = mx[0];
+      out[2] = mx[1];
+      out[3] = mx[2];
+    } else {
+      out[0] = sums[0];
+      out[1] = mx[0];
+      out[2] = mx[1];
+    }
+  }
+};
 
 // Iterate the cells of rows [row_begin, row_end) of an (nx, ny*nz) lattice.
 // Each wave owns one row at a time; lanes stride along x.

@@ -287,9 +287,7 @@ k_divergence_max(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
   }
   if (bad) atomicOr(red.flags, bad);
   DD dd[1] = {DD{0.0, 0.0}};
-  double* out = red.out;
-  grid_finish<1, 1>(dd, m, red.pdd, red.pmx, red.counter,
-                    [=](const double*, const double* mx) { out[0] = mx[0]; });
+  block_publish<1, 1>(dd, m, red.pdd, red.pmx);
 }
 
 // velocity_rate (src/simulation.rs:1415-1429): sum_d max|u_d| / h_d.
@@ -307,18 +305,7 @@ k_velocity_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
         m[a] = fmax(m[a], fabs((double)src[i + fx * row]));
   }
   DD dd[1] = {DD{0.0, 0.0}};
-  double* out = red.out;
-  const double h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
-  grid_finish<1, 3>(dd, m, red.pdd, red.pmx, red.counter, [=](const double*, const double* mx) {
-    double rate = 0.0;
-    rate += mx[0] / h0;
-    rate += mx[1] / h1;
-    rate += mx[2] / h2;
-    out[0] = rate;
-    out[1] = mx[0];
-    out[2] = mx[1];
-    out[3] = mx[2];
-  });
+  block_publish<1, 3>(dd, m, red.pdd, red.pmx);
 }
 
 // ------------------------------------------------------------ host launchers
@@ -360,6 +347,7 @@ void launch_divergence_max(const DevGrid& g, const void* u, const void* v, const
                            const Reduce& red, hipStream_t s) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
   k_divergence_max<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, red);
+  launch_finish<1, 1>(red.pdd, red.pmx, grid.x, FinStore{red.out, 0, 0.0, 0.0, 0.0}, s);
 }
 
 template <typename T>
@@ -368,6 +356,7 @@ void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const 
   const int rows = (g.ny + 1) * (g.nz + 1);
   dim3 grid(grid_blocks(rows)), block(kLanesX, kRowsPerBlock);
   k_velocity_rate<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, red);
+  launch_finish<1, 3>(red.pdd, red.pmx, grid.x, FinStore{red.out, 1, g.h[0], g.h[1], g.h[2]}, s);
 }
 
 void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,

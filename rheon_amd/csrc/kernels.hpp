@@ -16,7 +16,6 @@ enum : unsigned {
 struct Reduce {
   DD* pdd;            // 4 * kMaxBlocks
   double* pmx;        // 4 * kMaxBlocks
-  unsigned* counter;  // zero on entry
   double* out;        // result scalars
   unsigned* flags;    // error bits
 };
@@ -65,7 +64,8 @@ struct PcgControl {
   int candidate;       // this iteration needs a true report
   int replaced;        // this iteration replaced r by the true residual
   int pad;
-  unsigned arrive[8];  // grid-reduction counters, one per kernel slot
+  double sum_r;        // sum of r (multigrid gauge shift)
+  double spare[7];
 };
 
 struct PcgArrays {

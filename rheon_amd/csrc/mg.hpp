@@ -1,0 +1,35 @@
+// Multigrid hierarchy shared by the host solver and the MG kernels.
+#pragma once
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace rh {
+
+constexpr int kMgMaxLevels = 12;
+constexpr int kMgCoarsestMax = 4096;  // cells solved by one workgroup in LDS
+
+// One level's shape and rediscretised box-operator constants.
+struct MgLevel {
+  int nx, ny, nz, pad;
+  double w[3];
+  double diag[64];
+};
+
+struct MgHierarchy {
+  int levels = 0;
+  int pre = 2, post = 2;
+  double omega = 6.0 / 7.0;
+  MgLevel lv[kMgMaxLevels];
+  void* x[kMgMaxLevels] = {};  // V-cycle iterate (S)
+  void* y[kMgMaxLevels] = {};  // Jacobi ping-pong partner (S)
+  void* b[kMgMaxLevels] = {};  // level right-hand side (S)
+};
+
+template <typename S>
+void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
+                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s);
+template <typename S>
+void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s);
+void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s);
+
+}  // namespace rh

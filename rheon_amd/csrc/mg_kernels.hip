@@ -1,0 +1,385 @@
+// Geometric multigrid for the fixed-box pressure Poisson problem.
+//
+// Cell-centred hierarchy: level l+1 halves every axis of level l (all axes
+// even), spacing doubles, and the operator is the box operator rediscretised
+// on the coarse spacing (weights A_d/(rho h_d), the integrated operator of
+// src/operator.rs:58-97 on that grid).  One V(nu1, nu2) cycle:
+//   pre-smooth  x = omega b/D (first sweep from zero), then nu1-1 weighted-
+//               Jacobi sweeps x <- x + omega (b - K x)/D
+//   restrict    b_c = sum over the 8 children of the fine residual
+//               (integrated residuals add), x_c = omega b_c/D_c
+//   recurse / coarsest solve
+//   prolong     x += trilinear interpolation of x_c (cell-centred weights
+//               27/64, 9/64, 3/64, 1/64; Neumann reflection at walls)
+//   post-smooth nu2 sweeps; at the finest level the last sweep also adds
+//               the correction to the f64 pressure.
+// The coarsest level (<= 4096 cells) is solved by one workgroup with f64 CG
+// on the mean-free right-hand side.  The outer loop (FP64 defect correction)
+// computes the true residual b - K p every cycle with the reference's
+// acceptance rule (src/pressure.rs:432-456, :686-715).
+// Storage type S of the V-cycle is float (mixed precision) or double.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mg.hpp"
+#include "profiler.hpp"
+#include "stencil.hpp"
+
+namespace rh {
+
+namespace {
+
+__device__ __forceinline__ void mg_fail(PcgControl* ctl, int status, int it) {
+  ctl->done = 1;
+  ctl->status = status;
+  ctl->error_iteration = it;
+}
+
+// Level-local stencil in the V-cycle precision S (arithmetic in S).
+template <typename S>
+__device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, int c, int i, int j,
+                                     int k) {
+  const S xc = x[c];
+  const int sy = L.nx, sz = L.nx * L.ny;
+  const S w0 = (S)L.w[0], w1 = (S)L.w[1], w2 = (S)L.w[2];
+  S v = (S)0;
+  if (i > 0) v += w0 * (xc - x[c - 1]);
+  if (i + 1 < L.nx) v += w0 * (xc - x[c + 1]);
+  if (j > 0) v += w1 * (xc - x[c - sy]);
+  if (j + 1 < L.ny) v += w1 * (xc - x[c + sy]);
+  if (k > 0) v += w2 * (xc - x[c - sz]);
+  if (k + 1 < L.nz) v += w2 * (xc - x[c + sz]);
+  return v;
+}
+
+__device__ __forceinline__ double lv_diag(const MgLevel& L, int i, int j, int k) {
+  const int mask = (i > 0) | ((i + 1 < L.nx) << 1) | ((j > 0) << 2) | ((j + 1 < L.ny) << 3) |
+                   ((k > 0) << 4) | ((k + 1 < L.nz) << 5);
+  return L.diag[mask];
+}
+
+#define FOR_LEVEL_CELLS(L)                                                           \
+  {                                                                                  \
+    const RowRange rr_ = block_rows((L).ny * (L).nz);                                \
+    for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
+      const int k = row / (L).ny, j = row - k * (L).ny;                              \
+      for (int i = threadIdx.x; i < (L).nx; i += kLanesX) {                          \
+        const int c = i + (L).nx * row;
+
+}  // namespace
+
+// ------------------------------------------------------------------ kernels
+
+// Outer FP64 residual r = b - K p (every cell), written as the finest V-cycle
+// right-hand side b0 = S(r) with the first pre-smoothing sweep x0 = omega
+// b0/D fused in; partial sum r^2, max|r|, nonfinite flag.
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
+                    S* __restrict__ b0, S* __restrict__ x0, double omega, const PcgControl* ctl,
+                    DD* pdd, double* pmx) {
+  if (ctl->done) return;
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  FOR_CELLS(g)
+  const double v = apply_at<double>(g, p, c, i, j, k);
+  if (!isfinite(v)) m[1] = 1.0;
+  const double r = b[c] - v;
+  dd_acc(s[0], r * r);
+  m[0] = fmax(m[0], fabs(r));
+  const S rs = (S)r;
+  b0[c] = rs;
+  x0[c] = (S)(omega * (double)rs / lv_diag(L0, i, j, k));
+  END_CELLS
+  block_publish<1, 2>(s, m, pdd, pmx);
+}
+
+// One weighted-Jacobi sweep x_out = x_in + omega (b - K x_in)/D.  With
+// `p` non-null (finest level, last post-sweep) also p += x_out.
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __restrict__ xout,
+            double omega, double* __restrict__ p, const PcgControl* ctl) {
+  if (ctl->done) return;
+  const S om = (S)omega;
+  FOR_LEVEL_CELLS(L)
+  const S ax = s_apply<S>(L, xin, c, i, j, k);
+  const S xn = xin[c] + om * (b[c] - ax) / (S)lv_diag(L, i, j, k);
+  if (p) {
+    p[c] = p[c] + (double)xn;
+  } else {
+    xout[c] = xn;
+  }
+  END_CELLS
+}
+
+// Restriction: coarse cell (I,J,K) sums the residual b - K x of its 8
+// children; writes b_c and the first coarse pre-smoothing sweep x_c.
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restrict__ b,
+              S* __restrict__ bc, S* __restrict__ xc, double omega, const PcgControl* ctl) {
+  if (ctl->done) return;
+  FOR_LEVEL_CELLS(Cl)
+  S sum = (S)0;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const int fi = 2 * i + dx, fj = 2 * j + dy, fk = 2 * k + dz;
+        const int fc = fi + F.nx * (fj + F.ny * fk);
+        sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
+      }
+  bc[c] = sum;
+  xc[c] = (S)(omega * (double)sum / lv_diag(Cl, i, j, k));
+  END_CELLS
+}
+
+// Trilinear prolongation of the coarse correction, added to the fine x.
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_prolong(MgLevel F, MgLevel Cl, const S* __restrict__ xc, const S* __restrict__ xin,
+             S* __restrict__ xout, const PcgControl* ctl) {
+  if (ctl->done) return;
+  FOR_LEVEL_CELLS(F)
+  const int I = i >> 1, J = j >> 1, K = k >> 1;
+  int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
+  // Neumann reflection: a missing neighbour mirrors the parent.
+  if (In < 0 || In >= Cl.nx) In = I;
+  if (Jn < 0 || Jn >= Cl.ny) Jn = J;
+  if (Kn < 0 || Kn >= Cl.nz) Kn = K;
+  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
+  const S a = (S)0.75, bq = (S)0.25;
+  const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
+  const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
+  const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
+  const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
+  const S v0 = a * v00 + bq * v10;
+  const S v1 = a * v01 + bq * v11;
+  xout[c] = xin[c] + (a * v0 + bq * v1);
+  END_CELLS
+}
+
+// Coarsest level: one workgroup solves K x = b - mean(b) with f64 CG in LDS
+// (n <= kMgCoarsestMax), x mean-free; tolerance relative 1e-12 or n+2
+// iterations.
+constexpr int kCoarseThreads = 1024;
+
+template <typename S>
+__global__ void __launch_bounds__(kCoarseThreads)
+k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const PcgControl* ctl) {
+  if (ctl->done) return;
+  __shared__ double xs[kMgCoarsestMax], rs[kMgCoarsestMax], ds[kMgCoarsestMax], qs[kMgCoarsestMax];
+  __shared__ double red[kCoarseThreads / 64];
+  __shared__ double bcast;
+  const int n = L.nx * L.ny * L.nz;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  auto block_sum = [&](double v) -> double {
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    if (t == 0) {
+      double a = 0.0;
+      for (int w = 0; w < kCoarseThreads / 64; ++w) a += red[w];
+      bcast = a;
+    }
+    __syncthreads();
+    const double out = bcast;
+    __syncthreads();
+    return out;
+  };
+  double local = 0.0;
+  for (int c = t; c < n; c += kCoarseThreads) local += (double)b[c];
+  const double mean = block_sum(local) / (double)n;
+  double rr_local = 0.0;
+  for (int c = t; c < n; c += kCoarseThreads) {
+    const double r = (double)b[c] - mean;
+    xs[c] = 0.0;
+    rs[c] = r;
+    ds[c] = r;
+    rr_local += r * r;
+  }
+  double rr = block_sum(rr_local);
+  const double rr0 = rr;
+  const int sy = L.nx, sz = L.nx * L.ny;
+  for (int it = 0; it < n + 2 && rr > 1e-24 * rr0 && rr > 0.0; ++it) {
+    double dq_local = 0.0;
+    for (int c = t; c < n; c += kCoarseThreads) {
+      const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / sz;
+      const double xc = ds[c];
+      double v = 0.0;
+      if (i > 0) v += L.w[0] * (xc - ds[c - 1]);
+      if (i + 1 < L.nx) v += L.w[0] * (xc - ds[c + 1]);
+      if (j > 0) v += L.w[1] * (xc - ds[c - sy]);
+      if (j + 1 < L.ny) v += L.w[1] * (xc - ds[c + sy]);
+      if (k > 0) v += L.w[2] * (xc - ds[c - sz]);
+      if (k + 1 < L.nz) v += L.w[2] * (xc - ds[c + sz]);
+      qs[c] = v;
+      dq_local += xc * v;
+    }
+    const double dq = block_sum(dq_local);
+    if (!(dq > 0.0)) break;
+    const double alpha = rr / dq;
+    double rn_local = 0.0;
+    for (int c = t; c < n; c += kCoarseThreads) {
+      xs[c] += alpha * ds[c];
+      rs[c] -= alpha * qs[c];
+      rn_local += rs[c] * rs[c];
+    }
+    const double rn = block_sum(rn_local);
+    const double beta = rn / rr;
+    rr = rn;
+    for (int c = t; c < n; c += kCoarseThreads) ds[c] = rs[c] + beta * ds[c];
+    __syncthreads();
+  }
+  double xm_local = 0.0;
+  for (int c = t; c < n; c += kCoarseThreads) xm_local += xs[c];
+  const double xmean = block_sum(xm_local) / (double)n;
+  for (int c = t; c < n; c += kCoarseThreads) xout[c] = (S)(xs[c] - xmean);
+}
+
+// p <- p - p0 with p0 = p[0] captured in ctl->spare[0] beforehand.
+__global__ void k_capture_gauge(const double* __restrict__ p, PcgControl* ctl) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[0];
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl) {
+  const double p0 = ctl->spare[0];
+  FOR_CELLS(g)
+  p[c] = p[c] - p0;
+  END_CELLS
+}
+
+// Acceptance of the outer defect-correction residual; every cycle is a
+// "candidate" (the true residual is computed each cycle).
+struct FinMgTrue {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done; }
+  __device__ void operator()(const double* sum, const double* mx) const {
+    if (mx[1] != 0.0) return mg_fail(ctl, RH_ARITHMETIC_FAILURE, it);
+    const double squared = sum[0], maximum = mx[0];
+    const double divergence = maximum * ctl->dt / ctl->volume;
+    if (!isfinite(squared) || !isfinite(divergence)) return mg_fail(ctl, RH_BREAKDOWN, it);
+    const double l2 = sqrt(squared);
+    ctl->true_l2 = l2;
+    ctl->true_max = maximum;
+    ctl->pred_div = divergence;
+    ctl->iteration = it;
+    if (l2 <= ctl->threshold && divergence <= ctl->div_limit) {
+      ctl->done = 1;
+      ctl->status = RH_OK;
+      return;
+    }
+    if (it >= ctl->max_iterations) {
+      ctl->error_value = maximum;
+      return mg_fail(ctl, RH_ITERATION_LIMIT, it);
+    }
+  }
+};
+
+// ------------------------------------------------------------ launchers
+
+static inline dim3 level_grid(const MgLevel& L) { return dim3(grid_blocks(L.ny * L.nz)); }
+
+template <typename S>
+void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
+                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s) {
+  const dim3 grid = cells_grid(g);
+  {
+    prof::Scope ps(prof::kOuterResidual, s, it);
+    k_mg_outer_residual<S><<<grid, block_shape(), 0, s>>>(g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
+                                                          h.omega, ctl, pdd, pmx);
+  }
+  launch_finish<1, 2>(pdd, pmx, grid.x, FinMgTrue{ctl, it}, s);
+}
+
+// Profiler slot of a V-cycle kernel: finest-level kernels have their own
+// slots (tagged with the iteration); all coarser work shares one slot.
+static inline int mg_slot(int l, int finest_slot) { return l == 0 ? finest_slot : prof::kMgCoarse; }
+
+template <typename S>
+static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p,
+                   const PcgControl* ctl, int it, hipStream_t s) {
+  prof::Scope ps(mg_slot(l, p ? prof::kMgSmoothUpdate : prof::kMgSmooth), s, l == 0 ? it : -2);
+  k_mg_smooth<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], xin, (const S*)h.b[l], xout,
+                                                               h.omega, p, ctl);
+}
+
+// V-cycle from level l; on entry x[l] holds the first pre-smoothing sweep.
+template <typename S>
+static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl, int it,
+                   hipStream_t s) {
+  S* x = (S*)h.x[l];
+  S* y = (S*)h.y[l];
+  const S* b = (const S*)h.b[l];
+  if (l == h.levels - 1) {
+    prof::Scope ps(prof::kMgCoarse, s, -2);
+    k_mg_coarsest<S><<<1, kCoarseThreads, 0, s>>>(h.lv[l], b, x, (PcgControl*)ctl);
+    return;
+  }
+  // pre-smoothing: sweeps 2..nu1 (ping-pong x <-> y), result in `cur`
+  S* cur = x;
+  S* nxt = y;
+  for (int sweep = 1; sweep < h.pre; ++sweep) {
+    smooth<S>(h, l, cur, nxt, nullptr, ctl, it, s);
+    S* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  {
+    prof::Scope ps(mg_slot(l, prof::kMgRestrict), s, l == 0 ? it : -2);
+    k_mg_restrict<S><<<level_grid(h.lv[l + 1]), block_shape(), 0, s>>>(
+        h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.omega, ctl);
+  }
+  vcycle<S>(h, l + 1, p, ctl, it, s);
+  {
+    prof::Scope ps(mg_slot(l, prof::kMgProlong), s, l == 0 ? it : -2);
+    k_mg_prolong<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], h.lv[l + 1],
+                                                                   (const S*)h.x[l + 1], cur, nxt, ctl);
+  }
+  {
+    S* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  for (int sweep = 0; sweep < h.post; ++sweep) {
+    const bool last = sweep + 1 == h.post;
+    if (last && l == 0) {
+      smooth<S>(h, l, cur, nxt, p, ctl, it, s);  // p += x (finest, last sweep)
+    } else {
+      smooth<S>(h, l, cur, nxt, nullptr, ctl, it, s);
+      S* t = cur;
+      cur = nxt;
+      nxt = t;
+    }
+  }
+  if (l > 0 && cur != x) {
+    // keep the level result in x[l] for the parent's prolongation
+    // (only reached when the sweep count leaves it in y)
+    hipMemcpyAsync(x, cur, sizeof(S) * (size_t)h.lv[l].nx * h.lv[l].ny * h.lv[l].nz,
+                   hipMemcpyDeviceToDevice, s);
+  }
+}
+
+template <typename S>
+void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s) {
+  vcycle<S>(h, 0, p, ctl, it, s);
+}
+
+void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
+  k_capture_gauge<<<1, 64, 0, s>>>(p, ctl);
+  k_regauge<<<cells_grid(g), block_shape(), 0, s>>>(g, p, ctl);
+}
+
+template void mg_outer_residual<float>(const DevGrid&, const MgHierarchy&, const double*,
+                                       const double*, PcgControl*, DD*, double*, int, hipStream_t);
+template void mg_outer_residual<double>(const DevGrid&, const MgHierarchy&, const double*,
+                                        const double*, PcgControl*, DD*, double*, int, hipStream_t);
+template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t);
+template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t);
+
+}  // namespace rh

@@ -5,7 +5,10 @@
 // One iteration `it` is a fixed sequence of kernels; each begins by reading
 // the control block and returns at once when the solve has finished or the
 // kernel's branch is not taken, so the host can enqueue many iterations
-// ahead and poll `done` only every few iterations:
+// ahead and poll `done` only every few iterations.  A kernel with a dot
+// product publishes per-block partials and is followed by a one-block
+// finisher (k_finish) that combines them in fixed order and takes the
+// decision:
 //   apply_dir   q = K d, q[0] = 0, curvature = d.q, alpha        (pressure.rs:657-668)
 //   update      p += alpha d, r -= alpha q (i >= 1), r.r, z, r.z (:669-681, :534-575)
 //   [sgs black, sgs red]                                        (RB-SGS only)
@@ -20,63 +23,9 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "profiler.hpp"
+#include "stencil.hpp"
 
 namespace rh {
-
-#define FOR_CELLS(g)                                                                 \
-  {                                                                                  \
-    const RowRange rr_ = block_rows((g).ny * (g).nz);                                \
-    for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
-      const int k = row / (g).ny, j = row - k * (g).ny;                              \
-      for (int i = threadIdx.x; i < (g).nx; i += kLanesX) {                          \
-        const int c = i + (g).nx * row;
-#define END_CELLS \
-  }               \
-  }               \
-  }
-
-template <typename T>
-__device__ __forceinline__ double ld(const T* __restrict__ x, int i) {
-  return (double)x[i];
-}
-
-// apply_full box fast path (src/operator.rs:400-416): axis 0..2, lower then upper.
-template <typename T>
-__device__ __forceinline__ double apply_at(const DevGrid& g, const T* __restrict__ x, int c, int i,
-                                           int j, int k) {
-  const double xc = ld(x, c);
-  const int sy = g.nx, sz = g.nx * g.ny;
-  double v = 0.0;
-  if (i > 0) v += g.w[0] * (xc - ld(x, c - 1));
-  if (i + 1 < g.nx) v += g.w[0] * (xc - ld(x, c + 1));
-  if (j > 0) v += g.w[1] * (xc - ld(x, c - sy));
-  if (j + 1 < g.ny) v += g.w[1] * (xc - ld(x, c + sy));
-  if (k > 0) v += g.w[2] * (xc - ld(x, c - sz));
-  if (k + 1 < g.nz) v += g.w[2] * (xc - ld(x, c + sz));
-  return v;
-}
-
-// diagonal(): cached sum of existing neighbour weights (src/operator.rs:813-822).
-__device__ __forceinline__ double diag_at(const DevGrid& g, int i, int j, int k) {
-  const int mask = (i > 0) | ((i + 1 < g.nx) << 1) | ((j > 0) << 2) | ((j + 1 < g.ny) << 3) |
-                   ((k > 0) << 4) | ((k + 1 < g.nz) << 5);
-  return g.diag[mask];
-}
-
-// Weighted neighbour sum sum_nbr w_d * x_nbr, axis 0..2, lower then upper.
-template <typename T>
-__device__ __forceinline__ double nbr_sum(const DevGrid& g, const T* __restrict__ x, double init,
-                                          int c, int i, int j, int k) {
-  const int sy = g.nx, sz = g.nx * g.ny;
-  double v = init;
-  if (i > 0) v += g.w[0] * ld(x, c - 1);
-  if (i + 1 < g.nx) v += g.w[0] * ld(x, c + 1);
-  if (j > 0) v += g.w[1] * ld(x, c - sy);
-  if (j + 1 < g.ny) v += g.w[1] * ld(x, c + sy);
-  if (k > 0) v += g.w[2] * ld(x, c - sz);
-  if (k + 1 < g.nz) v += g.w[2] * ld(x, c + sz);
-  return v;
-}
 
 __device__ __forceinline__ void fail(PcgControl* ctl, int status, int it) {
   ctl->done = 1;
@@ -84,27 +33,21 @@ __device__ __forceinline__ void fail(PcgControl* ctl, int status, int it) {
   ctl->error_iteration = it;
 }
 
-// ---------------------------------------------------------------- kernels
-
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_fill_zero(T* __restrict__ x, int n) {
-  for (int i = blockIdx.x * kBlock + flat_tid(); i < n; i += gridDim.x * kBlock) x[i] = (T)0.0;
+// Which path a red-black stage belongs to: the speculative one right after
+// `update` (0) or the replacement / initial one after `replace` (1).
+__device__ __forceinline__ bool sgs_active(const PcgControl* ctl, int it, int mode) {
+  if (ctl->done) return false;
+  if (mode == 0) return it > 0;
+  return it == 0 || ctl->replaced;
 }
 
-// Compatibility sums and threshold (src/pressure.rs:585-633): sum(b),
-// sum(|b|), b.b; also validate_cells (finite, src/operator.rs:840-848).
-__global__ void __launch_bounds__(kBlock)
-k_rhs_sums(DevGrid g, const double* __restrict__ b, PcgControl* ctl, DD* pdd, double* pmx) {
-  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-  double bad[1] = {0.0};
-  FOR_CELLS(g)
-  const double v = b[c];
-  if (!isfinite(v)) bad[0] = 1.0;
-  dd_acc(s[0], v);
-  dd_acc(s[1], fabs(v));
-  dd_acc(s[2], v * v);
-  END_CELLS
-  grid_finish<3, 1>(s, bad, pdd, pmx, &ctl->arrive[0], [=](const double* sum, const double* mx) {
+// -------------------------------------------------------------- finishers
+
+// Compatibility sums and threshold (src/pressure.rs:603-633).
+struct FinRhsSums {
+  PcgControl* ctl;
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double* sum, const double* mx) const {
     if (mx[0] != 0.0) return fail(ctl, RH_NONFINITE_INPUT, 0);
     const double total = sum[0], total_abs = sum[1];
     ctl->sum = total;
@@ -119,26 +62,15 @@ k_rhs_sums(DevGrid g, const double* __restrict__ b, PcgControl* ctl, DD* pdd, do
     const double threshold = fmax(ctl->abs_tol, ctl->rel * rhs_l2);
     if (!isfinite(rhs_l2) || !isfinite(threshold)) return fail(ctl, RH_BREAKDOWN, 0);
     ctl->threshold = threshold;
-  });
-}
+  }
+};
 
-// q = K d, q[0] = 0 (gauge), curvature = d.q, alpha = rz/curvature.
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_apply_dir(DevGrid g, const T* __restrict__ d, T* __restrict__ q, PcgControl* ctl, DD* pdd,
-            double* pmx, int it) {
-  if (ctl->done) return;
-  DD s[1] = {{0.0, 0.0}};
-  double bad[1] = {0.0};
-  FOR_CELLS(g)
-  double v = apply_at<T>(g, d, c, i, j, k);
-  if (!isfinite(v)) bad[0] = 1.0;
-  if (c == 0) v = 0.0;
-  const T qs = (T)v;
-  q[c] = qs;
-  dd_acc(s[0], ld(d, c) * (double)qs);
-  END_CELLS
-  grid_finish<1, 1>(s, bad, pdd, pmx, &ctl->arrive[1], [=](const double* sum, const double* mx) {
+// curvature checks and alpha (src/pressure.rs:661-668).
+struct FinApplyDir {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done; }
+  __device__ void operator()(const double* sum, const double* mx) const {
     if (mx[0] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
     const double curvature = sum[0];
     const double rz = ctl->rz[it & 1];
@@ -147,69 +79,32 @@ k_apply_dir(DevGrid g, const T* __restrict__ d, T* __restrict__ q, PcgControl* c
     const double alpha = rz / curvature;
     if (!isfinite(alpha)) return fail(ctl, RH_BREAKDOWN, it);
     ctl->alpha = alpha;
-  });
-}
+  }
+};
 
-// p += alpha d, r -= alpha q for i >= 1; recursive r.r; candidate flag.
-// JACOBI: z = r/D on the fly and r.z (speculative next rz).
-// RB-SGS: first (red) half-sweep y_red = r/D stored in z.
-template <typename T, bool SGS>
-__global__ void __launch_bounds__(kBlock)
-k_update(DevGrid g, T* __restrict__ p, T* __restrict__ r, const T* __restrict__ d,
-         const T* __restrict__ q, T* __restrict__ z, PcgControl* ctl, DD* pdd, double* pmx,
-         int it) {
-  if (ctl->done) return;
-  const double alpha = ctl->alpha;
-  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  double none[1] = {0.0};
-  FOR_CELLS(g)
-  if (c == 0) {
-    if (SGS) z[0] = (T)0.0;
-    continue;
-  }
-  const T pn = (T)(ld(p, c) + alpha * ld(d, c));
-  const T rn = (T)(ld(r, c) - alpha * ld(q, c));
-  p[c] = pn;
-  r[c] = rn;
-  const double rv = (double)rn;
-  dd_acc(s[0], rv * rv);
-  if (SGS) {
-    if (((i + j + k) & 1) == 0) z[c] = (T)(rv / diag_at(g, i, j, k));
-  } else {
-    const double zv = rv / diag_at(g, i, j, k);
-    dd_acc(s[1], rv * zv);
-  }
-  END_CELLS
-  grid_finish<2, 1>(s, none, pdd, pmx, &ctl->arrive[2], [=](const double* sum, const double*) {
+// recursive residual and candidate test (src/pressure.rs:680-686).
+struct FinUpdate {
+  PcgControl* ctl;
+  int it;
+  int jacobi;
+  __device__ bool active() const { return !ctl->done; }
+  __device__ void operator()(const double* sum, const double*) const {
     ctl->rr = sum[0];
     const double recursive_l2 = sqrt(sum[0]);
     ctl->candidate = (recursive_l2 <= ctl->threshold) || it == ctl->max_iterations;
     ctl->replaced = 0;
-    if (!SGS) ctl->rz[(it + 1) & 1] = sum[1];
-  });
-}
+    if (jacobi) ctl->rz[(it + 1) & 1] = sum[1];
+    ctl->sum_r = sum[2];
+  }
+};
 
-// true_report (src/pressure.rs:432-456): product = b - K p over every cell
-// (gauge row included), l2 (dot), max, predicted divergence = max*dt/V;
-// then the acceptance / replacement / iteration-limit decisions of
-// solve_loaded (:638-647 at it = 0, :686-715 otherwise).
-template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_true_residual(DevGrid g, const T* __restrict__ p, const double* __restrict__ b,
-                T* __restrict__ out, PcgControl* ctl, DD* pdd, double* pmx, int it) {
-  if (ctl->done) return;
-  if (it > 0 && !ctl->candidate) return;
-  DD s[1] = {{0.0, 0.0}};
-  double m[2] = {0.0, 0.0};
-  FOR_CELLS(g)
-  const double v = apply_at<T>(g, p, c, i, j, k);
-  if (!isfinite(v)) m[1] = 1.0;
-  const double prod = b[c] - v;
-  out[c] = (T)prod;
-  dd_acc(s[0], prod * prod);
-  m[0] = fmax(m[0], fabs(prod));
-  END_CELLS
-  grid_finish<1, 2>(s, m, pdd, pmx, &ctl->arrive[3], [=](const double* sum, const double* mx) {
+// true_report and the decisions of solve_loaded (src/pressure.rs:432-456,
+// :638-647 at it = 0, :686-715 otherwise).
+struct FinTrue {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done && (it == 0 || ctl->candidate); }
+  __device__ void operator()(const double* sum, const double* mx) const {
     if (mx[1] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
     const double squared = sum[0], maximum = mx[0];
     const double divergence = maximum * ctl->dt / ctl->volume;
@@ -231,16 +126,128 @@ k_true_residual(DevGrid g, const T* __restrict__ p, const double* __restrict__ b
       ctl->iteration = it;
       return fail(ctl, RH_ITERATION_LIMIT, it);
     }
-  });
+  }
+};
+
+// next rz after the (initial or replacement) Jacobi preconditioning.
+struct FinReplace {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done && (it == 0 || ctl->replaced); }
+  __device__ void operator()(const double* sum, const double*) const { ctl->rz[(it + 1) & 1] = sum[0]; }
+};
+
+// next rz after a red-black preconditioner application.
+struct FinSgsRed {
+  PcgControl* ctl;
+  int it;
+  int mode;
+  __device__ bool active() const { return sgs_active(ctl, it, mode); }
+  __device__ void operator()(const double* sum, const double*) const { ctl->rz[(it + 1) & 1] = sum[0]; }
+};
+
+// ---------------------------------------------------------------- kernels
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_fill_zero(T* __restrict__ x, int n) {
+  for (int i = blockIdx.x * kBlock + flat_tid(); i < n; i += gridDim.x * kBlock) x[i] = (T)0.0;
+}
+
+// sum(b), sum(|b|), b.b and validate_cells (finite, src/operator.rs:840-848).
+__global__ void __launch_bounds__(kBlock)
+k_rhs_sums(DevGrid g, const double* __restrict__ b, DD* pdd, double* pmx) {
+  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  const double v = b[c];
+  if (!isfinite(v)) bad[0] = 1.0;
+  dd_acc(s[0], v);
+  dd_acc(s[1], fabs(v));
+  dd_acc(s[2], v * v);
+  END_CELLS
+  block_publish<3, 1>(s, bad, pdd, pmx);
+}
+
+// q = K d, q[0] = 0 (gauge), partial d.q.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_apply_dir(DevGrid g, const T* __restrict__ d, T* __restrict__ q, const PcgControl* ctl, DD* pdd,
+            double* pmx) {
+  if (ctl->done) return;
+  DD s[1] = {{0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  double v = apply_at<T>(g, d, c, i, j, k);
+  if (!isfinite(v)) bad[0] = 1.0;
+  if (c == 0) v = 0.0;
+  const T qs = (T)v;
+  q[c] = qs;
+  dd_acc(s[0], ld(d, c) * (double)qs);
+  END_CELLS
+  block_publish<1, 1>(s, bad, pdd, pmx);
+}
+
+// p += alpha d, r -= alpha q for i >= 1; partial r.r and sum(r).
+// JACOBI: z = r/D on the fly and partial r.z (speculative next rz).
+// RB-SGS: first (red) half-sweep y_red = r/D stored in z.
+template <typename T, bool SGS>
+__global__ void __launch_bounds__(kBlock)
+k_update(DevGrid g, T* __restrict__ p, T* __restrict__ r, const T* __restrict__ d,
+         const T* __restrict__ q, T* __restrict__ z, const PcgControl* ctl, DD* pdd, double* pmx) {
+  if (ctl->done) return;
+  const double alpha = ctl->alpha;
+  DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  FOR_CELLS(g)
+  if (c == 0) {
+    if (SGS) z[0] = (T)0.0;
+    continue;
+  }
+  const T pn = (T)(ld(p, c) + alpha * ld(d, c));
+  const T rn = (T)(ld(r, c) - alpha * ld(q, c));
+  p[c] = pn;
+  r[c] = rn;
+  const double rv = (double)rn;
+  dd_acc(s[0], rv * rv);
+  dd_acc(s[2], rv);
+  if (SGS) {
+    if (((i + j + k) & 1) == 0) z[c] = (T)(rv / diag_at(g, i, j, k));
+  } else {
+    const double zv = rv / diag_at(g, i, j, k);
+    dd_acc(s[1], rv * zv);
+  }
+  END_CELLS
+  block_publish<3, 1>(s, none, pdd, pmx);
+}
+
+// product = b - K p over every cell (gauge row included): partial sum of
+// squares, max|.|, nonfinite flag.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_true_residual(DevGrid g, const T* __restrict__ p, const double* __restrict__ b,
+                T* __restrict__ out, const PcgControl* ctl, DD* pdd, double* pmx, int it) {
+  if (ctl->done) return;
+  if (it > 0 && !ctl->candidate) return;
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  FOR_CELLS(g)
+  const double v = apply_at<T>(g, p, c, i, j, k);
+  if (!isfinite(v)) m[1] = 1.0;
+  const double prod = b[c] - v;
+  out[c] = (T)prod;
+  dd_acc(s[0], prod * prod);
+  m[0] = fmax(m[0], fabs(prod));
+  END_CELLS
+  block_publish<1, 2>(s, m, pdd, pmx);
 }
 
 // Replacement r = product (r[0] = 0) and the first preconditioner stage.
 // it == 0 is the initial precondition before the loop (src/pressure.rs:648-650).
-// JACOBI: r.z with z = r/D (next rz).  RB-SGS: y_red = r/D into z.
+// JACOBI: partial r.z with z = r/D.  RB-SGS: y_red = r/D into z.
 template <typename T, bool SGS>
 __global__ void __launch_bounds__(kBlock)
-k_replace(DevGrid g, T* r, const T* prod, T* __restrict__ z,
-          PcgControl* ctl, DD* pdd, double* pmx, int it) {
+k_replace(DevGrid g, T* r, const T* prod, T* __restrict__ z, const PcgControl* ctl, DD* pdd,
+          double* pmx, int it) {
   if (ctl->done) return;
   if (it > 0 && !ctl->replaced) return;
   DD s[1] = {{0.0, 0.0}};
@@ -261,18 +268,7 @@ k_replace(DevGrid g, T* r, const T* prod, T* __restrict__ z,
     dd_acc(s[0], rv * zv);
   }
   END_CELLS
-  if (SGS) return;
-  grid_finish<1, 1>(s, none, pdd, pmx, &ctl->arrive[4], [=](const double* sum, const double*) {
-    ctl->rz[(it + 1) & 1] = sum[0];
-  });
-}
-
-// Which path a red-black stage belongs to: the speculative one right after
-// `update` (0) or the replacement / initial one after `replace` (1).
-__device__ __forceinline__ bool sgs_active(const PcgControl* ctl, int it, int mode) {
-  if (ctl->done) return false;
-  if (mode == 0) return it > 0;
-  return it == 0 || ctl->replaced;
+  if (!SGS) block_publish<1, 1>(s, none, pdd, pmx);
 }
 
 // Black forward half-sweep: y_b = (r_b + sum_red w y) / D (neighbours are red).
@@ -289,11 +285,11 @@ k_sgs_black(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgCont
   END_CELLS
 }
 
-// Red reverse half-sweep z_r = y_r + (sum_black w z)/D, then r.z over all
-// cells (black z are final).  Gauge cell 0 (red) stays 0.
+// Red reverse half-sweep z_r = y_r + (sum_black w z)/D, then partial r.z
+// over all cells (black z are final).  Gauge cell 0 (red) stays 0.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, PcgControl* ctl, DD* pdd,
+k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgControl* ctl, DD* pdd,
           double* pmx, int it, int mode) {
   if (!sgs_active(ctl, it, mode)) return;
   DD s[1] = {{0.0, 0.0}};
@@ -310,9 +306,7 @@ k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, PcgControl* ctl
   }
   dd_acc(s[0], ld(r, c) * zv);
   END_CELLS
-  grid_finish<1, 1>(s, none, pdd, pmx, &ctl->arrive[5], [=](const double* sum, const double*) {
-    ctl->rz[(it + 1) & 1] = sum[0];
-  });
+  block_publish<1, 1>(s, none, pdd, pmx);
 }
 
 // d = z + beta d (i >= 1), or d = z after a replacement / at it = 0;
@@ -359,26 +353,43 @@ k_apply_full(DevGrid g, const double* __restrict__ x, double* __restrict__ y, un
   if (bad) atomicOr(flags, bad);
 }
 
-// ------------------------------------------------------------ host launchers
+// Residual report of a supplied pressure (ProviderWorkspace qualification,
+// src/solver_provider.rs:246-254): sum (b - K p)^2, max|b - K p|, nonfinite.
+__global__ void __launch_bounds__(kBlock)
+k_residual_report(DevGrid g, const double* __restrict__ p, const double* __restrict__ b, DD* pdd,
+                  double* pmx) {
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  FOR_CELLS(g)
+  const double v = apply_at<double>(g, p, c, i, j, k);
+  if (!isfinite(v)) m[1] = 1.0;
+  const double prod = b[c] - v;
+  dd_acc(s[0], prod * prod);
+  m[0] = fmax(m[0], fabs(prod));
+  END_CELLS
+  block_publish<1, 2>(s, m, pdd, pmx);
+}
 
-static inline dim3 cells_grid(const DevGrid& g) { return dim3(grid_blocks(g.ny * g.nz)); }
-static inline dim3 block_shape() { return dim3(kLanesX, kRowsPerBlock); }
+// ------------------------------------------------------------ host launchers
 
 template <typename T>
 void pcg_prologue(const DevGrid& g, const PcgArrays& a, PcgControl* ctl, DD* pdd, double* pmx,
                   hipStream_t s) {
   const int n = g.nx * g.ny * g.nz;
   const int fill_blocks = n / kBlock + 1 < kMaxBlocks ? n / kBlock + 1 : kMaxBlocks;
-  k_rhs_sums<<<cells_grid(g), block_shape(), 0, s>>>(g, a.b, ctl, pdd, pmx);
+  const dim3 grid = cells_grid(g);
+  k_rhs_sums<<<grid, block_shape(), 0, s>>>(g, a.b, pdd, pmx);
+  launch_finish<3, 1>(pdd, pmx, grid.x, FinRhsSums{ctl}, s);
   k_fill_zero<T><<<fill_blocks, block_shape(), 0, s>>>((T*)a.p, n);
-  k_true_residual<T><<<cells_grid(g), block_shape(), 0, s>>>(g, (const T*)a.p, a.b, (T*)a.q, ctl,
-                                                             pdd, pmx, 0);
+  k_true_residual<T><<<grid, block_shape(), 0, s>>>(g, (const T*)a.p, a.b, (T*)a.q, ctl, pdd, pmx, 0);
+  launch_finish<1, 2>(pdd, pmx, grid.x, FinTrue{ctl, 0}, s);
 }
 
 template <typename T>
 void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl* ctl, DD* pdd,
                    double* pmx, int it, hipStream_t s) {
   const dim3 grid = cells_grid(g), block = block_shape();
+  const int nb = grid.x;
   T* p = (T*)a.p;
   T* r = (T*)a.r;
   T* d = (T*)a.d;
@@ -388,13 +399,18 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
   if (it > 0) {
     {
       prof::Scope ps(prof::kApplyDir, s, it);
-      k_apply_dir<T><<<grid, block, 0, s>>>(g, d, q, ctl, pdd, pmx, it);
+      k_apply_dir<T><<<grid, block, 0, s>>>(g, d, q, ctl, pdd, pmx);
     }
+    launch_finish<1, 1>(pdd, pmx, nb, FinApplyDir{ctl, it}, s);
+    {
+      prof::Scope ps(prof::kUpdate, s, it);
+      if (sgs)
+        k_update<T, true><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx);
+      else
+        k_update<T, false><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx);
+    }
+    launch_finish<3, 1>(pdd, pmx, nb, FinUpdate{ctl, it, sgs ? 0 : 1}, s);
     if (sgs) {
-      {
-        prof::Scope ps(prof::kUpdate, s, it);
-        k_update<T, true><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
-      }
       {
         prof::Scope ps(prof::kSgsBlack, s, it);
         k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 0);
@@ -403,16 +419,13 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
         prof::Scope ps(prof::kSgsRed, s, it);
         k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 0);
       }
-    } else {
-      {
-        prof::Scope ps(prof::kUpdate, s, it);
-        k_update<T, false><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx, it);
-      }
+      launch_finish<1, 1>(pdd, pmx, nb, FinSgsRed{ctl, it, 0}, s);
     }
     {
       prof::Scope ps(prof::kTrueResidual, s, it);
       k_true_residual<T><<<grid, block, 0, s>>>(g, p, a.b, q, ctl, pdd, pmx, it);
     }
+    launch_finish<1, 2>(pdd, pmx, nb, FinTrue{ctl, it}, s);
   }
   if (sgs) {
     {
@@ -427,6 +440,7 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
       prof::Scope ps(prof::kSgsRed, s, it);
       k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 1);
     }
+    launch_finish<1, 1>(pdd, pmx, nb, FinSgsRed{ctl, it, 1}, s);
     {
       prof::Scope ps(prof::kDirection, s, it);
       k_direction<T, true><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
@@ -436,6 +450,7 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
       prof::Scope ps(prof::kReplace, s, it);
       k_replace<T, false><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
     }
+    launch_finish<1, 1>(pdd, pmx, nb, FinReplace{ctl, it}, s);
     {
       prof::Scope ps(prof::kDirection, s, it);
       k_direction<T, false><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
@@ -477,45 +492,27 @@ void launch_precondition(const DevGrid& g, int method, double* r, double* z, Pcg
     k_replace<double, true><<<grid, block, 0, s>>>(g, r, r, z, ctl, pdd, pmx, 0);
     k_sgs_black<double><<<grid, block, 0, s>>>(g, r, z, ctl, 0, 1);
     k_sgs_red<double><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, 0, 1);
+    launch_finish<1, 1>(pdd, pmx, grid.x, FinSgsRed{ctl, 0, 1}, s);
   } else {
     k_replace<double, false><<<grid, block, 0, s>>>(g, r, r, z, ctl, pdd, pmx, 0);
+    launch_finish<1, 1>(pdd, pmx, grid.x, FinReplace{ctl, 0}, s);
     // z = r/D through the direction kernel (d := z, copy branch at it = 0)
     k_direction<double, false><<<grid, block, 0, s>>>(g, r, z, z, ctl, 0);
   }
 }
 
-// Residual report of a supplied pressure (ProviderWorkspace qualification,
-// src/solver_provider.rs:246-254): out[0] = sum (b - K p)^2, out[1] = max|b - K p|.
-__global__ void __launch_bounds__(kBlock)
-k_residual_report(DevGrid g, const double* __restrict__ p, const double* __restrict__ b,
-                  Reduce red) {
-  DD s[1] = {{0.0, 0.0}};
-  double m[2] = {0.0, 0.0};
-  FOR_CELLS(g)
-  const double v = apply_at<double>(g, p, c, i, j, k);
-  if (!isfinite(v)) m[1] = 1.0;
-  const double prod = b[c] - v;
-  dd_acc(s[0], prod * prod);
-  m[0] = fmax(m[0], fabs(prod));
-  END_CELLS
-  double* out = red.out;
-  grid_finish<1, 2>(s, m, red.pdd, red.pmx, red.counter, [=](const double* sum, const double* mx) {
-    out[0] = sum[0];
-    out[1] = mx[0];
-    out[2] = mx[1];
-  });
-}
-
 void launch_residual_report(const DevGrid& g, const double* p, const double* b, const Reduce& red,
                             hipStream_t s) {
-  k_residual_report<<<cells_grid(g), block_shape(), 0, s>>>(g, p, b, red);
+  const dim3 grid = cells_grid(g);
+  k_residual_report<<<grid, block_shape(), 0, s>>>(g, p, b, red.pdd, red.pmx);
+  launch_finish<1, 2>(red.pdd, red.pmx, grid.x, FinStore{red.out, 2, 0.0, 0.0, 0.0}, s);
 }
 
-// Compatibility sums of a RHS into ctl (sum, sum_abs, threshold) with the
-// settings already stored in ctl (rel, abs_tol).
 void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
                      hipStream_t s) {
-  k_rhs_sums<<<cells_grid(g), block_shape(), 0, s>>>(g, b, ctl, pdd, pmx);
+  const dim3 grid = cells_grid(g);
+  k_rhs_sums<<<grid, block_shape(), 0, s>>>(g, b, pdd, pmx);
+  launch_finish<3, 1>(pdd, pmx, grid.x, FinRhsSums{ctl}, s);
 }
 
 }  // namespace rh

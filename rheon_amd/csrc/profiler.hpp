@@ -28,6 +28,7 @@ enum Slot {
   kOuterResidual,
   kOuterUpdate,
   kBatch,
+  kMgSmoothUpdate,
   kNumSlots
 };
 

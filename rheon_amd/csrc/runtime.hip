@@ -164,7 +164,6 @@ Reduce Scratch::reduce(int out_slot) {
   Reduce r;
   r.pdd = pdd;
   r.pmx = pmx;
-  r.counter = counters + 8;
   r.out = scal + out_slot;
   r.flags = flags;
   return r;
@@ -202,144 +201,6 @@ Scratch& global_scratch(std::unique_lock<std::mutex>& lock) {
   static Scratch s;
   lock = std::unique_lock<std::mutex>(mu);
   return s;
-}
-
-// --------------------------------------------------------------- PcgSolver
-
-rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config) {
-  grid_ = grid;
-  config_ = config;
-  if (config.method != RH_JACOBI_PCG && config.method != RH_RBSGS_PCG) return RH_UNSUPPORTED;
-  if (config.precision != RH_F64 && config.precision != RH_F32) return RH_INVALID_ARGUMENT;
-  rh_status st = scratch_.init();
-  if (st) return st;
-  n_ = (size_t)grid.counts[0] * grid.counts[1] * grid.counts[2];
-  const size_t s = config.precision == RH_F64 ? 8 : 4;
-  // r, d, q (+ z for RB-SGS) (+ p when vectors are f32; f64 p is the output).
-  int vectors = 3 + (config.method == RH_RBSGS_PCG ? 1 : 0) + (config.precision == RH_F32 ? 1 : 0);
-  bytes_ = s * n_ * vectors;
-  if (config.memory_limit && bytes_ > config.memory_limit) return RH_BUFFER_LIMIT;
-  RH_HIP(hipMalloc(&arena_, bytes_ + 256 * vectors));
-  char* p = (char*)arena_;
-  auto take = [&]() {
-    void* r = p;
-    p += ((s * n_ + 255) / 256) * 256;
-    return r;
-  };
-  r_ = take();
-  d_ = take();
-  q_ = take();
-  z_ = config.method == RH_RBSGS_PCG ? take() : nullptr;
-  p32_ = config.precision == RH_F32 ? take() : nullptr;
-  return RH_OK;
-}
-
-PcgSolver::~PcgSolver() {
-  if (arena_) (void)hipFree(arena_);
-  scratch_.release();
-}
-
-uint64_t PcgSolver::allocated_bytes() const { return bytes_ + scratch_.bytes_; }
-
-template <typename T>
-rh_status PcgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
-                         double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
-                         void* ctx) {
-  hipStream_t s = scratch_.stream;
-  PcgArrays a;
-  a.p = sizeof(T) == 8 ? (void*)d_pressure : p32_;
-  a.r = r_;
-  a.d = d_;
-  a.q = q_;
-  a.z = z_;
-  a.b = d_rhs;
-  a.p64 = d_pressure;
-  PcgControl h;
-  std::memset(&h, 0, sizeof(h));
-  h.rel = st.relative_residual;
-  h.abs_tol = st.absolute_residual;
-  h.div_limit = st.divergence_limit;
-  h.dt = dt_;
-  h.volume = g.volume;
-  h.max_iterations = (int)(st.max_iterations > INT32_MAX - 2 ? INT32_MAX - 2 : st.max_iterations);
-  std::memcpy(scratch_.host, &h, sizeof(h));
-  RH_HIP(hipMemcpyAsync(scratch_.ctl, scratch_.host, sizeof(h), hipMemcpyHostToDevice, s));
-  RH_HIP(hipEventRecord(scratch_.ev0, s));
-  pcg_prologue<T>(g, a, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
-  pcg_iteration<T>(g, config_.method, a, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s);
-  RH_HIP(hipGetLastError());
-  const int poll = st.poll_interval > 0 ? st.poll_interval : 16;
-  PcgControl* slot[2] = {(PcgControl*)scratch_.host + 1, (PcgControl*)scratch_.host + 2};
-  bool cancelled = false;
-  int it = 1;
-  int pending = -1;  // slot index with an outstanding readback
-  // Enqueue batch k+1 before waiting for batch k's readback, so the device
-  // never idles on the host's poll.
-  for (int batch = 0;; ++batch) {
-    const int end = std::min(h.max_iterations, it + poll - 1);
-    for (; it <= end; ++it)
-      pcg_iteration<T>(g, config_.method, a, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s);
-    const int cur = batch & 1;
-    RH_HIP(hipMemcpyAsync(slot[cur], scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
-    RH_HIP(hipEventRecord(scratch_.poll_ev[cur], s));
-    bool finished = it > h.max_iterations;
-    if (pending >= 0) {
-      RH_HIP(hipEventSynchronize(scratch_.poll_ev[pending]));
-      if (slot[pending]->done) finished = true;
-      else if (cancel && cancel(ctx)) {
-        cancelled = true;
-        finished = true;
-      }
-    }
-    pending = cur;
-    if (finished) break;
-  }
-  RH_HIP(hipGetLastError());
-  pcg_export_pressure<T>(g, a, s);
-  RH_HIP(hipEventRecord(scratch_.ev1, s));
-  RH_HIP(hipMemcpyAsync(slot[0], scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
-  RH_HIP(hipStreamSynchronize(s));
-  const PcgControl& c = *slot[0];
-  if (prof::enabled()) prof::settle(c.iteration);
-  float ms = 0.0f;
-  RH_HIP(hipEventElapsedTime(&ms, scratch_.ev0, scratch_.ev1));
-  if (report) {
-    report->iterations = c.iteration;
-    report->true_residual_l2 = c.true_l2;
-    report->true_residual_max = c.true_max;
-    report->predicted_divergence_max = c.pred_div;
-    report->error_iteration = c.error_iteration;
-    report->error_value = c.error_value;
-    report->threshold = c.threshold;
-    report->seconds = ms * 1e-3;
-  }
-  if (c.done) return (rh_status)c.status;
-  if (cancelled) {
-    if (report) report->error_iteration = c.iteration;
-    return RH_CANCELLED;
-  }
-  set_last_error("solver ended without a terminal status", "PcgSolver::run");
-  return RH_EXECUTION;
-}
-
-rh_status PcgSolver::solve(double density, const double* d_rhs, double dt,
-                           const rh_pressure_settings& st, double* d_pressure,
-                           rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) {
-  if (report) std::memset(report, 0, sizeof(*report));
-  if (!d_rhs || !d_pressure) return RH_INVALID_ARGUMENT;
-  rh_status s = check_density(grid_, density);
-  if (s) return s;
-  // solve_loaded settings check (src/pressure.rs:588-599).
-  if (!std::isfinite(dt) || dt <= 0.0 || !std::isfinite(st.relative_residual) ||
-      st.relative_residual < 0.0 || !std::isfinite(st.absolute_residual) ||
-      st.absolute_residual < 0.0 || !std::isfinite(st.divergence_limit) ||
-      st.divergence_limit < 0.0 || st.max_iterations < 0)
-    return RH_INVALID_SETTINGS;
-  if (cancel && cancel(ctx)) return RH_CANCELLED;
-  dt_ = dt;
-  const DevGrid g = make_dev_grid(grid_, density);
-  if (config_.precision == RH_F64) return run<double>(g, d_rhs, st, d_pressure, report, cancel, ctx);
-  return run<float>(g, d_rhs, st, d_pressure, report, cancel, ctx);
 }
 
 }  // namespace rh

@@ -2,10 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "mg.hpp"
 
 namespace rh {
 
@@ -15,7 +17,7 @@ rh_status check_density(const rh_grid& g, double density);
 rh_status flags_to_status(unsigned f);
 
 // Per-object device scratch: a stream, reduction partials, counters, error
-// flags, result scalars, the PCG control block and a pinned readback page.
+// flags, result scalars, the solver control block and a pinned readback page.
 struct Scratch {
   hipStream_t stream = nullptr;
   void* base = nullptr;
@@ -40,29 +42,77 @@ struct Scratch {
 
 Scratch& global_scratch(std::unique_lock<std::mutex>& lock);
 
-// PressureWorkspace on the device (src/pressure.rs:243-740).
-class PcgSolver {
+// A device pressure solver behind PressureWorkspace::solve_rhs
+// (src/pressure.rs:365-393): the caller's RHS and pressure are device arrays.
+class Solver {
  public:
-  rh_status create(const rh_grid& grid, const rh_solver_config& config);
-  ~PcgSolver();
-  rh_status solve(double density, const double* d_rhs, double dt, const rh_pressure_settings& st,
-                  double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
-  uint64_t allocated_bytes() const;
+  virtual ~Solver() = default;
+  virtual rh_status create(const rh_grid& grid, const rh_solver_config& config) = 0;
+  virtual rh_status solve(double density, const double* d_rhs, double dt,
+                          const rh_pressure_settings& st, double* d_pressure,
+                          rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) = 0;
+  virtual uint64_t allocated_bytes() const = 0;
   Scratch& scratch() { return scratch_; }
   const rh_grid& grid() const { return grid_; }
+
+ protected:
+  rh_status check_and_begin(double density, const double* d_rhs, double dt,
+                            const rh_pressure_settings& st, double* d_pressure,
+                            rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
+  // Enqueues `step(it)` for it = 1.. in batches of `poll`, reading the
+  // control block back one batch behind; returns when the device reports a
+  // terminal status, the iteration budget is spent or `cancel` fires.
+  template <typename Step>
+  rh_status drive(int max_iterations, int poll, Step step, bool* cancelled, rh_cancel_fn cancel,
+                  void* ctx);
+  rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out);
+  rh_grid grid_{};
+  rh_solver_config config_{};
+  Scratch scratch_;
+  double dt_ = 0.0;
+};
+
+Solver* make_solver(rh_method method);
+
+// jacobi-pcg-v1 and RB-SGS PCG (src/pressure.rs:577-740).
+class PcgSolver : public Solver {
+ public:
+  rh_status create(const rh_grid& grid, const rh_solver_config& config) override;
+  ~PcgSolver() override;
+  rh_status solve(double density, const double* d_rhs, double dt, const rh_pressure_settings& st,
+                  double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
+                  void* ctx) override;
+  uint64_t allocated_bytes() const override;
 
  private:
   template <typename T>
   rh_status run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
                 double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
-  rh_grid grid_{};
-  rh_solver_config config_{};
-  Scratch scratch_;
   void* arena_ = nullptr;
   void *r_ = nullptr, *d_ = nullptr, *q_ = nullptr, *z_ = nullptr, *p32_ = nullptr;
   size_t n_ = 0;
   uint64_t bytes_ = 0;
-  double dt_ = 0.0;
+};
+
+// Multigrid V-cycles inside an f64 defect-correction loop.
+class MgSolver : public Solver {
+ public:
+  rh_status create(const rh_grid& grid, const rh_solver_config& config) override;
+  ~MgSolver() override;
+  rh_status solve(double density, const double* d_rhs, double dt, const rh_pressure_settings& st,
+                  double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
+                  void* ctx) override;
+  uint64_t allocated_bytes() const override;
+  int levels() const { return h_.levels; }
+
+ private:
+  template <typename S>
+  rh_status run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
+                double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
+  void set_density(double density);
+  MgHierarchy h_;
+  void* arena_ = nullptr;
+  uint64_t bytes_ = 0;
 };
 
 }  // namespace rh

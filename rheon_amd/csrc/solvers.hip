@@ -1,0 +1,342 @@
+// Device pressure solvers (PressureWorkspace on the GPU, src/pressure.rs:243-740):
+// the PCG family and the multigrid defect-correction solver.  Both keep every
+// scalar decision on the device; the host enqueues iterations in batches and
+// reads the control block back one batch behind, so the device never waits
+// for the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mg.hpp"
+#include "profiler.hpp"
+#include "runtime.hpp"
+
+namespace rh {
+
+Solver* make_solver(rh_method method) {
+  switch (method) {
+    case RH_JACOBI_PCG:
+    case RH_RBSGS_PCG: return new PcgSolver();
+    case RH_MIXED_MG: return new MgSolver();
+    default: return nullptr;
+  }
+}
+
+// solve_loaded settings check (src/pressure.rs:588-599) and cancellation
+// before any work (:600-602).
+rh_status Solver::check_and_begin(double density, const double* d_rhs, double dt,
+                                  const rh_pressure_settings& st, double* d_pressure,
+                                  rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) {
+  if (report) std::memset(report, 0, sizeof(*report));
+  if (!d_rhs || !d_pressure) return RH_INVALID_ARGUMENT;
+  rh_status s = check_density(grid_, density);
+  if (s) return s;
+  if (!std::isfinite(dt) || dt <= 0.0 || !std::isfinite(st.relative_residual) ||
+      st.relative_residual < 0.0 || !std::isfinite(st.absolute_residual) ||
+      st.absolute_residual < 0.0 || !std::isfinite(st.divergence_limit) ||
+      st.divergence_limit < 0.0 || st.max_iterations < 0)
+    return RH_INVALID_SETTINGS;
+  if (cancel && cancel(ctx)) return RH_CANCELLED;
+  dt_ = dt;
+  return RH_OK;
+}
+
+template <typename Step>
+rh_status Solver::drive(int max_iterations, int poll, Step step, bool* cancelled,
+                        rh_cancel_fn cancel, void* ctx) {
+  hipStream_t s = scratch_.stream;
+  PcgControl* slot[2] = {(PcgControl*)scratch_.host + 1, (PcgControl*)scratch_.host + 2};
+  *cancelled = false;
+  int it = 1;
+  int pending = -1;
+  for (int batch = 0;; ++batch) {
+    const int end = std::min(max_iterations, it + poll - 1);
+    for (; it <= end; ++it) step(it);
+    const int cur = batch & 1;
+    RH_HIP(hipMemcpyAsync(slot[cur], scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
+    RH_HIP(hipEventRecord(scratch_.poll_ev[cur], s));
+    bool finished = it > max_iterations;
+    if (pending >= 0) {
+      RH_HIP(hipEventSynchronize(scratch_.poll_ev[pending]));
+      if (slot[pending]->done) {
+        finished = true;
+      } else if (cancel && cancel(ctx)) {
+        *cancelled = true;
+        finished = true;
+      }
+    }
+    pending = cur;
+    if (finished) break;
+  }
+  RH_HIP(hipGetLastError());
+  return RH_OK;
+}
+
+rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out) {
+  hipStream_t s = scratch_.stream;
+  PcgControl* c = (PcgControl*)scratch_.host + 1;
+  RH_HIP(hipEventRecord(scratch_.ev1, s));
+  RH_HIP(hipMemcpyAsync(c, scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
+  RH_HIP(hipStreamSynchronize(s));
+  if (prof::enabled()) prof::settle(c->iteration);
+  float ms = 0.0f;
+  RH_HIP(hipEventElapsedTime(&ms, scratch_.ev0, scratch_.ev1));
+  if (report) {
+    report->iterations = c->iteration;
+    report->true_residual_l2 = c->true_l2;
+    report->true_residual_max = c->true_max;
+    report->predicted_divergence_max = c->pred_div;
+    report->error_iteration = c->error_iteration;
+    report->error_value = c->error_value;
+    report->threshold = c->threshold;
+    report->seconds = ms * 1e-3;
+  }
+  if (out) *out = *c;
+  if (c->done) return (rh_status)c->status;
+  if (cancelled) {
+    if (report) report->error_iteration = c->iteration;
+    return RH_CANCELLED;
+  }
+  set_last_error("solver ended without a terminal status", "Solver::finish_report");
+  return RH_EXECUTION;
+}
+
+static void init_control(PcgControl& h, const rh_pressure_settings& st, double dt, double volume) {
+  std::memset(&h, 0, sizeof(h));
+  h.rel = st.relative_residual;
+  h.abs_tol = st.absolute_residual;
+  h.div_limit = st.divergence_limit;
+  h.dt = dt;
+  h.volume = volume;
+  h.max_iterations = (int)(st.max_iterations > INT32_MAX - 2 ? INT32_MAX - 2 : st.max_iterations);
+}
+
+// ------------------------------------------------------------------- PCG
+
+rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config) {
+  grid_ = grid;
+  config_ = config;
+  if (config.method != RH_JACOBI_PCG && config.method != RH_RBSGS_PCG) return RH_UNSUPPORTED;
+  if (config.precision != RH_F64 && config.precision != RH_F32) return RH_INVALID_ARGUMENT;
+  rh_status st = scratch_.init();
+  if (st) return st;
+  n_ = (size_t)grid.counts[0] * grid.counts[1] * grid.counts[2];
+  const size_t s = config.precision == RH_F64 ? 8 : 4;
+  // r, d, q (+ z for RB-SGS) (+ p when vectors are f32; f64 p is the output).
+  int vectors = 3 + (config.method == RH_RBSGS_PCG ? 1 : 0) + (config.precision == RH_F32 ? 1 : 0);
+  bytes_ = s * n_ * vectors;
+  if (config.memory_limit && bytes_ > config.memory_limit) return RH_BUFFER_LIMIT;
+  RH_HIP(hipMalloc(&arena_, bytes_ + 256 * vectors));
+  char* p = (char*)arena_;
+  auto take = [&]() {
+    void* r = p;
+    p += ((s * n_ + 255) / 256) * 256;
+    return r;
+  };
+  r_ = take();
+  d_ = take();
+  q_ = take();
+  z_ = config.method == RH_RBSGS_PCG ? take() : nullptr;
+  p32_ = config.precision == RH_F32 ? take() : nullptr;
+  return RH_OK;
+}
+
+PcgSolver::~PcgSolver() {
+  if (arena_) (void)hipFree(arena_);
+  scratch_.release();
+}
+
+uint64_t PcgSolver::allocated_bytes() const { return bytes_ + scratch_.bytes_; }
+
+template <typename T>
+rh_status PcgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
+                         double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
+                         void* ctx) {
+  hipStream_t s = scratch_.stream;
+  PcgArrays a;
+  a.p = sizeof(T) == 8 ? (void*)d_pressure : p32_;
+  a.r = r_;
+  a.d = d_;
+  a.q = q_;
+  a.z = z_;
+  a.b = d_rhs;
+  a.p64 = d_pressure;
+  PcgControl h;
+  init_control(h, st, dt_, g.volume);
+  std::memcpy(scratch_.host, &h, sizeof(h));
+  RH_HIP(hipMemcpyAsync(scratch_.ctl, scratch_.host, sizeof(h), hipMemcpyHostToDevice, s));
+  RH_HIP(hipEventRecord(scratch_.ev0, s));
+  pcg_prologue<T>(g, a, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
+  pcg_iteration<T>(g, config_.method, a, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s);
+  RH_HIP(hipGetLastError());
+  bool cancelled = false;
+  const int poll = st.poll_interval > 0 ? st.poll_interval : 16;
+  rh_status rs = drive(
+      h.max_iterations, poll,
+      [&](int it) {
+        pcg_iteration<T>(g, config_.method, a, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s);
+      },
+      &cancelled, cancel, ctx);
+  if (rs) return rs;
+  pcg_export_pressure<T>(g, a, s);
+  return finish_report(report, cancelled, nullptr);
+}
+
+rh_status PcgSolver::solve(double density, const double* d_rhs, double dt,
+                           const rh_pressure_settings& st, double* d_pressure,
+                           rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) {
+  rh_status s = check_and_begin(density, d_rhs, dt, st, d_pressure, report, cancel, ctx);
+  if (s) return s;
+  const DevGrid g = make_dev_grid(grid_, density);
+  if (config_.precision == RH_F64) return run<double>(g, d_rhs, st, d_pressure, report, cancel, ctx);
+  return run<float>(g, d_rhs, st, d_pressure, report, cancel, ctx);
+}
+
+// -------------------------------------------------------------- multigrid
+
+rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) {
+  grid_ = grid;
+  config_ = config;
+  if (config.method != RH_MIXED_MG) return RH_UNSUPPORTED;
+  if (config.precision != RH_F64 && config.precision != RH_F32) return RH_INVALID_ARGUMENT;
+  rh_status st = scratch_.init();
+  if (st) return st;
+  h_.pre = config.mg_pre_sweeps > 0 ? config.mg_pre_sweeps : 2;
+  h_.post = config.mg_post_sweeps > 0 ? config.mg_post_sweeps : 2;
+  h_.omega = config.mg_omega > 0.0 ? config.mg_omega : 6.0 / 7.0;
+  // Coarsen while every axis is even and the level is above the one-
+  // workgroup coarsest size (or up to the requested level count).
+  int n[3] = {(int)grid.counts[0], (int)grid.counts[1], (int)grid.counts[2]};
+  int levels = 1;
+  h_.lv[0].nx = n[0];
+  h_.lv[0].ny = n[1];
+  h_.lv[0].nz = n[2];
+  const int want = config.mg_levels > 0 ? std::min(config.mg_levels, kMgMaxLevels) : kMgMaxLevels;
+  while (levels < want) {
+    const int* m = &h_.lv[levels - 1].nx;
+    const long cells = (long)m[0] * m[1] * m[2];
+    const bool even = (m[0] % 2 == 0) && (m[1] % 2 == 0) && (m[2] % 2 == 0);
+    const int mn = std::min(m[0], std::min(m[1], m[2]));
+    if (!even || mn < 4) break;
+    if (config.mg_levels <= 0 && cells <= 512) break;
+    h_.lv[levels].nx = m[0] / 2;
+    h_.lv[levels].ny = m[1] / 2;
+    h_.lv[levels].nz = m[2] / 2;
+    ++levels;
+  }
+  h_.levels = levels;
+  const MgLevel& last = h_.lv[levels - 1];
+  if ((long)last.nx * last.ny * last.nz > kMgCoarsestMax) return RH_UNSUPPORTED;
+  const size_t s = config.precision == RH_F64 ? 8 : 4;
+  size_t total = 0;
+  for (int l = 0; l < levels; ++l) {
+    const size_t cells = (size_t)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz;
+    total += 3 * ((cells * s + 255) / 256 * 256);
+  }
+  bytes_ = total;
+  if (config.memory_limit && bytes_ > config.memory_limit) return RH_BUFFER_LIMIT;
+  RH_HIP(hipMalloc(&arena_, total));
+  RH_HIP(hipMemset(arena_, 0, total));
+  char* p = (char*)arena_;
+  for (int l = 0; l < levels; ++l) {
+    const size_t cells = (size_t)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz;
+    const size_t sz = (cells * s + 255) / 256 * 256;
+    h_.x[l] = p;
+    p += sz;
+    h_.y[l] = p;
+    p += sz;
+    h_.b[l] = p;
+    p += sz;
+  }
+  return RH_OK;
+}
+
+MgSolver::~MgSolver() {
+  if (arena_) (void)hipFree(arena_);
+  scratch_.release();
+}
+
+uint64_t MgSolver::allocated_bytes() const { return bytes_ + scratch_.bytes_; }
+
+// Rediscretised coefficients per level: spacing doubles each level.
+void MgSolver::set_density(double density) {
+  double h[3] = {grid_.spacing[0], grid_.spacing[1], grid_.spacing[2]};
+  for (int l = 0; l < h_.levels; ++l) {
+    MgLevel& L = h_.lv[l];
+    const double area[3] = {h[1] * h[2], h[0] * h[2], h[0] * h[1]};
+    for (int a = 0; a < 3; ++a) L.w[a] = area[a] * (1.0 / (density * h[a]));
+    for (int mask = 0; mask < 64; ++mask) {
+      double v = 0.0;
+      for (int side = 0; side < 6; ++side)
+        if (mask & (1 << side)) v += L.w[side / 2];
+      L.diag[mask] = v;
+    }
+    for (int a = 0; a < 3; ++a) h[a] *= 2.0;
+  }
+}
+
+template <typename S>
+rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
+                        double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
+                        void* ctx) {
+  hipStream_t s = scratch_.stream;
+  PcgControl h;
+  init_control(h, st, dt_, g.volume);
+  std::memcpy(scratch_.host, &h, sizeof(h));
+  RH_HIP(hipMemcpyAsync(scratch_.ctl, scratch_.host, sizeof(h), hipMemcpyHostToDevice, s));
+  RH_HIP(hipEventRecord(scratch_.ev0, s));
+  // compatibility + threshold, p = 0, initial true residual (it = 0)
+  launch_rhs_sums(g, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
+  RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
+  mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s);
+  RH_HIP(hipGetLastError());
+  bool cancelled = false;
+  const int poll = st.poll_interval > 0 ? st.poll_interval : 2;
+  rh_status rs = drive(
+      h.max_iterations, poll,
+      [&](int it) {
+        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s);
+        mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s);
+      },
+      &cancelled, cancel, ctx);
+  if (rs) return rs;
+  // Fix the gauge p[0] = 0 (the reference's eliminated gauge cell) and
+  // report the true residual of the returned pressure.
+  PcgControl c;
+  rs = finish_report(report, cancelled, &c);
+  if (rs == RH_ITERATION_LIMIT) mg_regauge(g, d_pressure, scratch_.ctl, s);
+  if (rs != RH_OK) return rs;
+  mg_regauge(g, d_pressure, scratch_.ctl, s);
+  launch_residual_report(g, d_pressure, d_rhs, scratch_.reduce(0), s);
+  double out[3];
+  rs = scratch_.read_scalars(out, 3, 0);
+  if (rs) return rs;
+  if (report) {
+    report->true_residual_l2 = std::sqrt(out[0]);
+    report->true_residual_max = out[1];
+    report->predicted_divergence_max = out[1] * dt_ / g.volume;
+    report->seconds += 0.0;
+  }
+  if (!(std::sqrt(out[0]) <= c.threshold) || !(out[1] * dt_ / g.volume <= c.div_limit)) {
+    set_last_error("gauge shift moved the residual past the threshold", "MgSolver::run");
+    return RH_QUALITY_GATE;
+  }
+  return RH_OK;
+}
+
+rh_status MgSolver::solve(double density, const double* d_rhs, double dt,
+                          const rh_pressure_settings& st, double* d_pressure,
+                          rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) {
+  rh_status s = check_and_begin(density, d_rhs, dt, st, d_pressure, report, cancel, ctx);
+  if (s) return s;
+  const DevGrid g = make_dev_grid(grid_, density);
+  set_density(density);
+  if (config_.precision == RH_F64) return run<double>(g, d_rhs, st, d_pressure, report, cancel, ctx);
+  return run<float>(g, d_rhs, st, d_pressure, report, cancel, ctx);
+}
+
+}  // namespace rh

@@ -1,0 +1,105 @@
+"""Multigrid solver: the CPU statement (oracle/mg.py) is pinned to the
+reference's converged JPCG answer and to dense least squares; the device MG
+is then checked against both."""
+import numpy as np
+import pytest
+
+import oracle as O
+import oracle.mg as M
+import rheon_amd as R
+from rheon_amd.scenes import GridSpec, fourier_modes
+
+from helpers import config1, ogrid, rel_err
+
+
+def test_mg_statement_operator_matches_reference_apply():
+    spec = GridSpec((8, 6, 4), (0.3, 0.2, 0.5))
+    L = M.Level(spec.counts, spec.spacing, 1.7)
+    x = np.random.default_rng(1).standard_normal(spec.cell_len)
+    got = L.apply(x.reshape(4, 6, 8)).ravel()
+    want = O.apply_full(ogrid(spec), 1.7, x)
+    assert np.max(np.abs(got - want)) < 1e-13 * np.max(np.abs(want))
+
+
+def test_mg_statement_levels_and_transfers():
+    lv = M.build_levels((64, 32, 16), (1 / 64, 1 / 32, 1 / 16), 1.0)
+    assert [L.n for L in lv] == [(64, 32, 16), (32, 16, 8), (16, 8, 4)]
+    lv = M.build_levels((128,) * 3, (1 / 128,) * 3, 1.0, levels=4)
+    assert [L.n for L in lv][-1] == (16, 16, 16)
+    # prolongation reproduces constants and linear fields away from walls
+    xc = np.ones((4, 4, 4))
+    assert np.allclose(M.prolong(xc, (8, 8, 8)), 1.0)
+    # restriction preserves the total (integrated residuals add)
+    r = np.random.default_rng(2).standard_normal((8, 8, 8))
+    assert abs(M.restrict(r).sum() - r.sum()) < 1e-12
+
+
+def test_mg_statement_converges_to_reference_pressure():
+    spec, vel, dt = config1(32, seed=3)
+    g = ogrid(spec)
+    rhs = O.build_rhs(g, vel, dt)
+    ref = O.solve_rhs(g, 1.0, rhs, dt, O.Settings(1e-12, 1e-14, 1.0, 5000), threads=4)
+    st, p, cycles, l2, rmax = M.solve(spec.counts, spec.spacing, 1.0, rhs, dt, 1e-12, 1e-14, 1.0, 60)
+    assert st == "ok" and cycles < 25
+    assert p[0] == 0.0
+    assert rel_err(p, ref.pressure) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [R.F32, R.F64])
+def test_device_mg_reaches_reference_pressure(precision):
+    spec, vel, dt = config1(64, seed=4, dtype=np.float32)
+    g = ogrid(spec)
+    rhs = O.build_rhs(g, vel, dt)
+    ref = O.solve_rhs(g, 1.0, rhs, dt, O.Settings(1e-12, 1e-14, 1.0, 5000), threads=8)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", precision)
+    rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-10, 1e-14, 1.0, 100))
+    assert rep.ok, R.STATUS_NAMES.get(rep.status)
+    assert rep.iterations < 25
+    p = ws.get_pressure()
+    assert p[0] == 0.0
+    assert rel_err(p, ref.pressure) < 1e-8
+    # the reported residual is that of the returned (gauged) pressure
+    res = rhs - O.apply_full(g, 1.0, p)
+    assert abs(np.sqrt(np.sum(res * res)) - rep.true_residual_l2) <= 1e-6 * rep.true_residual_l2
+
+
+@pytest.mark.gpu
+def test_device_mg_cycles_track_the_cpu_statement():
+    spec, vel, dt = config1(32, seed=5)
+    rhs = O.build_rhs(ogrid(spec), vel, dt)
+    hist = []
+    M.solve(spec.counts, spec.spacing, 1.0, rhs, dt, 0.0, 0.0, 0.0, 6, history=hist)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F64)
+    for k in range(1, 6):
+        rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(0.0, 0.0, 0.0, k))
+        assert rep.status == R.ITERATION_LIMIT
+        # residual after k cycles agrees with the statement (coarsest solved
+        # by CG on the device vs least squares on the CPU)
+        assert abs(rep.true_residual_l2 - hist[k]) <= 1e-6 * hist[k], (k, rep.true_residual_l2, hist[k])
+
+
+@pytest.mark.gpu
+def test_device_mg_four_levels_config2_shape():
+    """128^3 FP32 V-cycles, 4 levels (128 -> 16), to rel 1e-5."""
+    spec, vel, dt = config1(128, seed=6, dtype=np.float32)
+    rhs = R.build_rhs(spec, vel, dt)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32, mg_levels=4)
+    rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-5, 1e-14, 1.0, 100))
+    assert rep.ok and rep.iterations <= 12
+    out = R.correct_velocity(spec, 1.0, ws.get_pressure(), dt, vel)
+    assert R.divergence_max(spec, out) < 1e-1
+
+
+@pytest.mark.gpu
+def test_device_mg_fourier_projection_divergence():
+    spec = GridSpec.unit_box(64)
+    vel = fourier_modes(spec, 9, dtype=np.float32)
+    dt = 0.5 / 64
+    rhs = R.build_rhs(spec, vel, dt)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32)
+    rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-10, 1e-14, 1.0, 100))
+    assert rep.ok
+    out = R.correct_velocity(spec, 1.0, ws.get_pressure(), dt, vel)
+    # f32 face rounding floor ~ |u| eps32 / h
+    assert R.divergence_max(spec, out) < 64 * 64 * 6e-8
