@@ -5,7 +5,7 @@ numpy module states the algorithm rheon_amd implements so the device V-cycle
 can be checked cycle by cycle, and its convergence studied, on the CPU:
 
 * levels: every axis halves while all axes are even and the level has more
-  than 512 cells (or a requested level count); spacing doubles; the operator
+  than 64 cells (or a requested level count); spacing doubles; the operator
   is the box operator of src/operator.rs:58-97 rediscretised on each level
   (weights A_d/(rho h_d)), Neumann walls, applied as in :400-416.
 * V(nu1, nu2) with weighted Jacobi (omega = 6/7); first pre-sweep from zero
@@ -59,7 +59,7 @@ class Level:
         return y
 
 
-def build_levels(counts, spacing, density=1.0, levels=0, coarsest_cells=512):
+def build_levels(counts, spacing, density=1.0, levels=0, coarsest_cells=64):
     lv = [Level(counts, spacing, density)]
     h = list(spacing)
     n = list(counts)

@@ -13,12 +13,18 @@ struct MgLevel {
   int nx, ny, nz, pad;
   double w[3];
   double diag[64];
+  double odiag[64];  // omega / diag (weighted-Jacobi scale)
 };
+
+constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
 
 struct MgHierarchy {
   int levels = 0;
   int pre = 2, post = 2;
   double omega = 6.0 / 7.0;
+  // Coarsest level <= kMgDenseMax cells: (K + c 11^T)^-1, column-major n x n
+  // f64 on the device; x = M^-1 (b - mean b) is the mean-free solution.
+  const double* coarse_inv = nullptr;
   MgLevel lv[kMgMaxLevels];
   void* x[kMgMaxLevels] = {};  // V-cycle iterate (S)
   void* y[kMgMaxLevels] = {};  // Jacobi ping-pong partner (S)

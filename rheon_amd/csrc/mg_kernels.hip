@@ -35,26 +35,29 @@ __device__ __forceinline__ void mg_fail(PcgControl* ctl, int status, int it) {
 }
 
 // Level-local stencil in the V-cycle precision S (arithmetic in S).
+// Branch-free: all six neighbours are loaded at once from clamped
+// addresses (a wall neighbour is the cell itself and contributes 0).
 template <typename S>
 __device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, int c, int i, int j,
                                      int k) {
+  const Nbr n = nbr_offsets(L.nx, L.ny, L.nz, i, j, k);
   const S xc = x[c];
-  const int sy = L.nx, sz = L.nx * L.ny;
+  const S a = x[c + n.xm], b = x[c + n.xp], cy = x[c + n.ym], d = x[c + n.yp], e = x[c + n.zm],
+          f = x[c + n.zp];
   const S w0 = (S)L.w[0], w1 = (S)L.w[1], w2 = (S)L.w[2];
-  S v = (S)0;
-  if (i > 0) v += w0 * (xc - x[c - 1]);
-  if (i + 1 < L.nx) v += w0 * (xc - x[c + 1]);
-  if (j > 0) v += w1 * (xc - x[c - sy]);
-  if (j + 1 < L.ny) v += w1 * (xc - x[c + sy]);
-  if (k > 0) v += w2 * (xc - x[c - sz]);
-  if (k + 1 < L.nz) v += w2 * (xc - x[c + sz]);
-  return v;
+  return w0 * ((xc - a) + (xc - b)) + w1 * ((xc - cy) + (xc - d)) + w2 * ((xc - e) + (xc - f));
 }
 
+__device__ __forceinline__ int lv_mask(const MgLevel& L, int i, int j, int k) {
+  return (i > 0) | ((i + 1 < L.nx) << 1) | ((j > 0) << 2) | ((j + 1 < L.ny) << 3) |
+         ((k > 0) << 4) | ((k + 1 < L.nz) << 5);
+}
 __device__ __forceinline__ double lv_diag(const MgLevel& L, int i, int j, int k) {
-  const int mask = (i > 0) | ((i + 1 < L.nx) << 1) | ((j > 0) << 2) | ((j + 1 < L.ny) << 3) |
-                   ((k > 0) << 4) | ((k + 1 < L.nz) << 5);
-  return L.diag[mask];
+  return L.diag[lv_mask(L, i, j, k)];
+}
+// omega / D (weighted-Jacobi scale) of the cell.
+__device__ __forceinline__ double lv_odiag(const MgLevel& L, int i, int j, int k) {
+  return L.odiag[lv_mask(L, i, j, k)];
 }
 
 #define FOR_LEVEL_CELLS(L)                                                           \
@@ -88,7 +91,7 @@ k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const d
   m[0] = fmax(m[0], fabs(r));
   const S rs = (S)r;
   b0[c] = rs;
-  x0[c] = (S)(omega * (double)rs / lv_diag(L0, i, j, k));
+  x0[c] = (S)lv_odiag(L0, i, j, k) * rs;
   END_CELLS
   block_publish<1, 2>(s, m, pdd, pmx);
 }
@@ -100,10 +103,9 @@ __global__ void __launch_bounds__(kBlock)
 k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __restrict__ xout,
             double omega, double* __restrict__ p, const PcgControl* ctl) {
   if (ctl->done) return;
-  const S om = (S)omega;
   FOR_LEVEL_CELLS(L)
   const S ax = s_apply<S>(L, xin, c, i, j, k);
-  const S xn = xin[c] + om * (b[c] - ax) / (S)lv_diag(L, i, j, k);
+  const S xn = xin[c] + (S)lv_odiag(L, i, j, k) * (b[c] - ax);
   if (p) {
     p[c] = p[c] + (double)xn;
   } else {
@@ -132,7 +134,7 @@ k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restric
         sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
       }
   bc[c] = sum;
-  xc[c] = (S)(omega * (double)sum / lv_diag(Cl, i, j, k));
+  xc[c] = (S)lv_odiag(Cl, i, j, k) * sum;
   END_CELLS
 }
 
@@ -240,6 +242,36 @@ k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const Pc
   for (int c = t; c < n; c += kCoarseThreads) xout[c] = (S)(xs[c] - xmean);
 }
 
+// Coarsest level with a dense inverse: x = M^-1 (b - mean b), one workgroup.
+constexpr int kDenseThreads = 512;
+
+template <typename S>
+__global__ void __launch_bounds__(kDenseThreads)
+k_mg_coarsest_dense(MgLevel L, const double* __restrict__ minv, const S* __restrict__ b,
+                    S* __restrict__ xout, const PcgControl* ctl) {
+  if (ctl->done) return;
+  __shared__ double bs[kMgDenseMax];
+  __shared__ double red[kDenseThreads / 64];
+  const int n = L.nx * L.ny * L.nz;
+  const int t = threadIdx.x;
+  double v = 0.0;
+  for (int c = t; c < n; c += kDenseThreads) {
+    bs[c] = (double)b[c];
+    v += bs[c];
+  }
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double total = 0.0;
+  for (int w = 0; w < kDenseThreads / 64; ++w) total += red[w];
+  const double mean = total / (double)n;
+  for (int c = t; c < n; c += kDenseThreads) {
+    double acc = 0.0;
+    for (int j2 = 0; j2 < n; ++j2) acc += minv[(size_t)j2 * n + c] * (bs[j2] - mean);
+    xout[c] = (S)acc;
+  }
+}
+
 // p <- p - p0 with p0 = p[0] captured in ctl->spare[0] beforehand.
 __global__ void k_capture_gauge(const double* __restrict__ p, PcgControl* ctl) {
   if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[0];
@@ -318,7 +350,10 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   const S* b = (const S*)h.b[l];
   if (l == h.levels - 1) {
     prof::Scope ps(prof::kMgCoarse, s, -2);
-    k_mg_coarsest<S><<<1, kCoarseThreads, 0, s>>>(h.lv[l], b, x, (PcgControl*)ctl);
+    if (h.coarse_inv)
+      k_mg_coarsest_dense<S><<<1, kDenseThreads, 0, s>>>(h.lv[l], h.coarse_inv, b, x, ctl);
+    else
+      k_mg_coarsest<S><<<1, kCoarseThreads, 0, s>>>(h.lv[l], b, x, (PcgControl*)ctl);
     return;
   }
   // pre-smoothing: sweeps 2..nu1 (ping-pong x <-> y), result in `cur`

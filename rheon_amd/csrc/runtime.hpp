@@ -109,9 +109,11 @@ class MgSolver : public Solver {
   template <typename S>
   rh_status run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
                 double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
-  void set_density(double density);
+  rh_status set_density(double density);
   MgHierarchy h_;
   void* arena_ = nullptr;
+  double* d_inv_ = nullptr;   // dense coarsest inverse (device)
+  double inv_density_ = 0.0;  // density the inverse was built for
   uint64_t bytes_ = 0;
 };
 

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -222,7 +223,7 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
     const bool even = (m[0] % 2 == 0) && (m[1] % 2 == 0) && (m[2] % 2 == 0);
     const int mn = std::min(m[0], std::min(m[1], m[2]));
     if (!even || mn < 4) break;
-    if (config.mg_levels <= 0 && cells <= 512) break;
+    if (config.mg_levels <= 0 && cells <= 64) break;
     h_.lv[levels].nx = m[0] / 2;
     h_.lv[levels].ny = m[1] / 2;
     h_.lv[levels].nz = m[2] / 2;
@@ -257,13 +258,68 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
 
 MgSolver::~MgSolver() {
   if (arena_) (void)hipFree(arena_);
+  if (d_inv_) (void)hipFree(d_inv_);
   scratch_.release();
 }
 
 uint64_t MgSolver::allocated_bytes() const { return bytes_ + scratch_.bytes_; }
 
-// Rediscretised coefficients per level: spacing doubles each level.
-void MgSolver::set_density(double density) {
+// Dense inverse of M = K + c 11^T (SPD for the Neumann K; c = mean
+// diagonal / n) by Cholesky on the host; for mean-free b, x = M^-1 b is
+// the mean-free solution of K x = b.
+static bool dense_coarse_inverse(const MgLevel& L, std::vector<double>& inv) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
+  std::vector<double> A((size_t)n * n, 0.0);
+  double dsum = 0.0;
+  for (int c = 0; c < n; ++c) {
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const int nb[6] = {i > 0 ? c - 1 : -1, i + 1 < nx ? c + 1 : -1, j > 0 ? c - nx : -1,
+                       j + 1 < ny ? c + nx : -1, k > 0 ? c - nx * ny : -1,
+                       k + 1 < nz ? c + nx * ny : -1};
+    for (int s = 0; s < 6; ++s)
+      if (nb[s] >= 0) {
+        A[(size_t)c * n + c] += L.w[s / 2];
+        A[(size_t)c * n + nb[s]] -= L.w[s / 2];
+      }
+    dsum += A[(size_t)c * n + c];
+  }
+  const double shift = dsum / n / n;
+  for (auto& a : A) a += shift;
+  // Cholesky A = L L^T (lower, in place)
+  for (int j = 0; j < n; ++j) {
+    double d = A[(size_t)j * n + j];
+    for (int k = 0; k < j; ++k) d -= A[(size_t)j * n + k] * A[(size_t)j * n + k];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    A[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; ++i) {
+      double v = A[(size_t)i * n + j];
+      for (int k = 0; k < j; ++k) v -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+      A[(size_t)i * n + j] = v / d;
+    }
+  }
+  // inverse column by column: solve L L^T x = e_c
+  inv.assign((size_t)n * n, 0.0);
+  std::vector<double> y(n);
+  for (int col = 0; col < n; ++col) {
+    for (int i = 0; i < n; ++i) {
+      double v = i == col ? 1.0 : 0.0;
+      for (int k = 0; k < i; ++k) v -= A[(size_t)i * n + k] * y[k];
+      y[i] = v / A[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+      double v = y[i];
+      for (int k = i + 1; k < n; ++k) v -= A[(size_t)k * n + i] * y[k];
+      y[i] = v / A[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)col * n + i] = y[i];
+  }
+  return true;
+}
+
+// Rediscretised coefficients per level (spacing doubles each level) and the
+// dense coarsest inverse for this density.
+rh_status MgSolver::set_density(double density) {
   double h[3] = {grid_.spacing[0], grid_.spacing[1], grid_.spacing[2]};
   for (int l = 0; l < h_.levels; ++l) {
     MgLevel& L = h_.lv[l];
@@ -274,9 +330,30 @@ void MgSolver::set_density(double density) {
       for (int side = 0; side < 6; ++side)
         if (mask & (1 << side)) v += L.w[side / 2];
       L.diag[mask] = v;
+      L.odiag[mask] = v > 0.0 ? h_.omega / v : 0.0;
     }
     for (int a = 0; a < 3; ++a) h[a] *= 2.0;
   }
+  const MgLevel& last = h_.lv[h_.levels - 1];
+  const int n = last.nx * last.ny * last.nz;
+  if (n > kMgDenseMax) {
+    h_.coarse_inv = nullptr;
+    return RH_OK;
+  }
+  if (d_inv_ && inv_density_ == density) {
+    h_.coarse_inv = d_inv_;
+    return RH_OK;
+  }
+  std::vector<double> inv;
+  if (!dense_coarse_inverse(last, inv)) {
+    h_.coarse_inv = nullptr;
+    return RH_OK;
+  }
+  if (!d_inv_) RH_HIP(hipMalloc(&d_inv_, sizeof(double) * (size_t)kMgDenseMax * kMgDenseMax));
+  RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+  inv_density_ = density;
+  h_.coarse_inv = d_inv_;
+  return RH_OK;
 }
 
 template <typename S>
@@ -334,7 +411,7 @@ rh_status MgSolver::solve(double density, const double* d_rhs, double dt,
   rh_status s = check_and_begin(density, d_rhs, dt, st, d_pressure, report, cancel, ctx);
   if (s) return s;
   const DevGrid g = make_dev_grid(grid_, density);
-  set_density(density);
+  if ((s = set_density(density))) return s;
   if (config_.precision == RH_F64) return run<double>(g, d_rhs, st, d_pressure, report, cancel, ctx);
   return run<float>(g, d_rhs, st, d_pressure, report, cancel, ctx);
 }

@@ -25,19 +25,42 @@ __device__ __forceinline__ double ld(const T* __restrict__ x, int i) {
   return (double)x[i];
 }
 
+// Neighbour offsets of cell (i,j,k), 0 where the neighbour is a wall.
+// Branch-free stencils load every neighbour at once (a clamped address is
+// the cell itself): a missing neighbour then contributes w*(x_c - x_c) =
+// +0.0 to the running sum, which leaves every partial sum bit-identical to
+// the reference's skipped term (the sum starts at +0.0 and never becomes
+// -0.0), so the operation order of src/operator.rs:400-416 is kept.
+struct Nbr {
+  int xm, xp, ym, yp, zm, zp;
+};
+__device__ __forceinline__ Nbr nbr_offsets(int nx, int ny, int nz, int i, int j, int k) {
+  const int sy = nx, sz = nx * ny;
+  Nbr n;
+  n.xm = i > 0 ? -1 : 0;
+  n.xp = i + 1 < nx ? 1 : 0;
+  n.ym = j > 0 ? -sy : 0;
+  n.yp = j + 1 < ny ? sy : 0;
+  n.zm = k > 0 ? -sz : 0;
+  n.zp = k + 1 < nz ? sz : 0;
+  return n;
+}
+
 // apply_full box fast path (src/operator.rs:400-416): axis 0..2, lower then upper.
 template <typename T>
 __device__ __forceinline__ double apply_at(const DevGrid& g, const T* __restrict__ x, int c, int i,
                                            int j, int k) {
+  const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
   const double xc = ld(x, c);
-  const int sy = g.nx, sz = g.nx * g.ny;
+  const double a = ld(x, c + n.xm), b = ld(x, c + n.xp), cy = ld(x, c + n.ym),
+               d = ld(x, c + n.yp), e = ld(x, c + n.zm), f = ld(x, c + n.zp);
   double v = 0.0;
-  if (i > 0) v += g.w[0] * (xc - ld(x, c - 1));
-  if (i + 1 < g.nx) v += g.w[0] * (xc - ld(x, c + 1));
-  if (j > 0) v += g.w[1] * (xc - ld(x, c - sy));
-  if (j + 1 < g.ny) v += g.w[1] * (xc - ld(x, c + sy));
-  if (k > 0) v += g.w[2] * (xc - ld(x, c - sz));
-  if (k + 1 < g.nz) v += g.w[2] * (xc - ld(x, c + sz));
+  v += g.w[0] * (xc - a);
+  v += g.w[0] * (xc - b);
+  v += g.w[1] * (xc - cy);
+  v += g.w[1] * (xc - d);
+  v += g.w[2] * (xc - e);
+  v += g.w[2] * (xc - f);
   return v;
 }
 
@@ -49,17 +72,28 @@ __device__ __forceinline__ double diag_at(const DevGrid& g, int i, int j, int k)
 }
 
 // Weighted neighbour sum sum_nbr w_d * x_nbr, axis 0..2, lower then upper.
+// A missing neighbour contributes w * (+0.0) = +0.0 (same argument as above;
+// the red-black kernels' `init` is never -0.0 unless the reference's is).
 template <typename T>
 __device__ __forceinline__ double nbr_sum(const DevGrid& g, const T* __restrict__ x, double init,
                                           int c, int i, int j, int k) {
-  const int sy = g.nx, sz = g.nx * g.ny;
+  const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
+  // unconditional loads (clamped addresses), then select
+  const double la = ld(x, c + n.xm), lb = ld(x, c + n.xp), lc = ld(x, c + n.ym),
+               ldd = ld(x, c + n.yp), le = ld(x, c + n.zm), lf = ld(x, c + n.zp);
+  const double a = n.xm ? la : 0.0;
+  const double b = n.xp ? lb : 0.0;
+  const double cy = n.ym ? lc : 0.0;
+  const double d = n.yp ? ldd : 0.0;
+  const double e = n.zm ? le : 0.0;
+  const double f = n.zp ? lf : 0.0;
   double v = init;
-  if (i > 0) v += g.w[0] * ld(x, c - 1);
-  if (i + 1 < g.nx) v += g.w[0] * ld(x, c + 1);
-  if (j > 0) v += g.w[1] * ld(x, c - sy);
-  if (j + 1 < g.ny) v += g.w[1] * ld(x, c + sy);
-  if (k > 0) v += g.w[2] * ld(x, c - sz);
-  if (k + 1 < g.nz) v += g.w[2] * ld(x, c + sz);
+  v += g.w[0] * a;
+  v += g.w[0] * b;
+  v += g.w[1] * cy;
+  v += g.w[1] * d;
+  v += g.w[2] * e;
+  v += g.w[2] * f;
   return v;
 }
 

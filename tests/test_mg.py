@@ -23,7 +23,7 @@ def test_mg_statement_operator_matches_reference_apply():
 
 def test_mg_statement_levels_and_transfers():
     lv = M.build_levels((64, 32, 16), (1 / 64, 1 / 32, 1 / 16), 1.0)
-    assert [L.n for L in lv] == [(64, 32, 16), (32, 16, 8), (16, 8, 4)]
+    assert [L.n for L in lv] == [(64, 32, 16), (32, 16, 8), (16, 8, 4), (8, 4, 2)]
     lv = M.build_levels((128,) * 3, (1 / 128,) * 3, 1.0, levels=4)
     assert [L.n for L in lv][-1] == (16, 16, 16)
     # prolongation reproduces constants and linear fields away from walls
