@@ -33,6 +33,9 @@ struct DevGrid {
   double acc[3];   // 1/(rho h_d)   (src/operator.rs:64)
   double w[3];     // area*acc      (src/operator.rs:65)
   double diag[64]; // cached boundary diagonals by neighbour mask (src/operator.rs:81-91)
+  double inv_h[3]; // 1/h, used only where h is a power of two (exact)
+  int pow2[3];     // h is a power of two: x/h == x*inv_h bit for bit
+  int pad2;
 };
 
 // Double-double accumulator.  Summing rounded products with error-free

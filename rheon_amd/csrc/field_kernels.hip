@@ -59,14 +59,82 @@ __device__ __forceinline__ unsigned sample(const DevGrid& g, int comp, const T* 
   return 0;
 }
 
+// (x - origin) / h of a world position clamped to the box, per axis: the
+// part of ScalarSampler::sample (src/sampling.rs:87-88) shared by the three
+// velocity components at one position.  When h is a power of two the
+// reciprocal is exact and the product equals the quotient bit for bit.
+__device__ __forceinline__ void base_coords(const DevGrid& g, const double pos[3], double out[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double x = pos[d];
+    if (x < g.origin[d]) x = g.origin[d];
+    if (x > g.upper[d]) x = g.upper[d];
+    out[d] = g.pow2[d] ? (x - g.origin[d]) * g.inv_h[d] : (x - g.origin[d]) / g.h[d];
+  }
+}
+
+// ScalarSampler::sample from precomputed base coordinates: identical
+// arithmetic to sample() above, the division hoisted out.
+template <typename T>
+__device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const T* __restrict__ vals,
+                                              const double base[3], double& out) {
+  const int n[3] = {g.nx, g.ny, g.nz};
+  int lo[3], hi[3], dims[3];
+  double t[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dims[d] = n[d] + (comp == d ? 1 : 0);
+    const double off = (comp == d) ? 0.0 : 0.5;
+    double q = base[d] - off;
+    const double top = (double)(dims[d] - 1);
+    if (q < 0.0) q = 0.0;
+    if (q > top) q = top;
+    if (!isfinite(q)) return kFlagArithmetic;
+    lo[d] = (int)floor(q);
+    hi[d] = lo[d] + 1 < dims[d] - 1 ? lo[d] + 1 : dims[d] - 1;
+    t[d] = q - (double)lo[d];
+  }
+  const int sy = dims[0], sz = dims[0] * dims[1];
+  double vals8[8];
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+        vals8[x + 2 * y + 4 * z] =
+            (double)vals[(x ? hi[0] : lo[0]) + sy * (y ? hi[1] : lo[1]) + sz * (z ? hi[2] : lo[2])];
+  double result = 0.0, minimum = INFINITY, maximum = -INFINITY;
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        double weight = (x ? t[0] : 1.0 - t[0]);
+        weight = weight * (y ? t[1] : 1.0 - t[1]);
+        weight = weight * (z ? t[2] : 1.0 - t[2]);
+        const double value = vals8[x + 2 * y + 4 * z];
+        minimum = fmin(minimum, value);
+        maximum = fmax(maximum, value);
+        result += weight * value;
+      }
+  if (!isfinite(result)) return kFlagArithmetic;
+  if (result < minimum) result = minimum;
+  if (result > maximum) result = maximum;
+  out = result;
+  return 0;
+}
+
 // VelocitySampler::backtrace / departure (src/sampling.rs:151-175).
 template <typename T>
 __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restrict__ u,
                                               const T* __restrict__ v, const T* __restrict__ w,
                                               const double pos[3], double dt, double out[3]) {
-  double vel[3], mid[3];
-  unsigned f = sample<T>(g, 0, u, pos, vel[0]) | sample<T>(g, 1, v, pos, vel[1]) |
-               sample<T>(g, 2, w, pos, vel[2]);
+  double vel[3], mid[3], base[3];
+  base_coords(g, pos, base);
+  unsigned f = sample_at<T>(g, 0, u, base, vel[0]) | sample_at<T>(g, 1, v, base, vel[1]) |
+               sample_at<T>(g, 2, w, base, vel[2]);
   if (f) return f;
   const double half = 0.5 * dt;
 #pragma unroll
@@ -77,8 +145,9 @@ __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restr
     if (value > g.upper[d]) value = g.upper[d];
     mid[d] = value;
   }
-  f = sample<T>(g, 0, u, mid, vel[0]) | sample<T>(g, 1, v, mid, vel[1]) |
-      sample<T>(g, 2, w, mid, vel[2]);
+  base_coords(g, mid, base);
+  f = sample_at<T>(g, 0, u, base, vel[0]) | sample_at<T>(g, 1, v, base, vel[1]) |
+      sample_at<T>(g, 2, w, base, vel[2]);
   if (f) return f;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -120,7 +189,11 @@ k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
       for (int d = 0; d < 3; ++d)
         pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
       unsigned f = backtrace<T>(g, u, v, w, pos, dt, dep);
-      if (!f) f = sample<T>(g, a, src, dep, value);
+      if (!f) {
+        double base[3];
+        base_coords(g, dep, base);
+        f = sample_at<T>(g, a, src, base, value);
+      }
       if (f) {
         bad |= f;
         continue;
@@ -147,7 +220,11 @@ k_advect_tracer(DevGrid g, const float* __restrict__ old, const float* __restric
 #pragma unroll
       for (int d = 0; d < 3; ++d) pos[d] = g.origin[d] + ((double)p[d] + 0.5) * g.h[d];
       unsigned f = backtrace<float>(g, u, v, w, pos, dt, dep);
-      if (!f) f = sample<float>(g, 3, old, dep, value);
+      if (!f) {
+        double base[3];
+        base_coords(g, dep, base);
+        f = sample_at<float>(g, 3, old, base, value);
+      }
       if (f) {
         bad |= f;
         continue;

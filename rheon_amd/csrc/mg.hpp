@@ -17,6 +17,7 @@ struct MgLevel {
 };
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
+constexpr int kMgTailCells = 32768;  // levels up to this size run in the one-workgroup tail
 
 struct MgHierarchy {
   int levels = 0;
@@ -25,6 +26,10 @@ struct MgHierarchy {
   // Coarsest level <= kMgDenseMax cells: (K + c 11^T)^-1, column-major n x n
   // f64 on the device; x = M^-1 (b - mean b) is the mean-free solution.
   const double* coarse_inv = nullptr;
+  // Levels >= tail_level (> 0) run in one single-workgroup kernel that reads
+  // this struct from device memory at d_self; 0 = no tail.
+  int tail_level = 0;
+  const MgHierarchy* d_self = nullptr;
   MgLevel lv[kMgMaxLevels];
   void* x[kMgMaxLevels] = {};  // V-cycle iterate (S)
   void* y[kMgMaxLevels] = {};  // Jacobi ping-pong partner (S)

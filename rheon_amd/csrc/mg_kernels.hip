@@ -272,6 +272,137 @@ k_mg_coarsest_dense(MgLevel L, const double* __restrict__ minv, const S* __restr
   }
 }
 
+// ------------------------------------------------------------- V-cycle tail
+// Levels l0..L-1 (each <= kMgTailCells cells) run inside one workgroup of
+// kTailThreads threads: the level arrays stay in L2, and each phase is
+// separated by a workgroup barrier instead of a kernel launch.
+constexpr int kTailThreads = 1024;
+
+template <typename S>
+__device__ __forceinline__ void tail_smooth(const MgLevel& L, const S* __restrict__ xin,
+                                            const S* __restrict__ b, S* __restrict__ xout) {
+  const int n = L.nx * L.ny * L.nz;
+  for (int c = threadIdx.x; c < n; c += kTailThreads) {
+    const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
+    const S ax = s_apply<S>(L, xin, c, i, j, k);
+    xout[c] = xin[c] + (S)lv_odiag(L, i, j, k) * (b[c] - ax);
+  }
+  __syncthreads();
+}
+
+template <typename S>
+__device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& Cl, const S* __restrict__ x,
+                                              const S* __restrict__ b, S* __restrict__ bc,
+                                              S* __restrict__ xc) {
+  const int n = Cl.nx * Cl.ny * Cl.nz;
+  for (int c = threadIdx.x; c < n; c += kTailThreads) {
+    const int i = c % Cl.nx, j = (c / Cl.nx) % Cl.ny, k = c / (Cl.nx * Cl.ny);
+    S sum = (S)0;
+    for (int dz = 0; dz < 2; ++dz)
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx) {
+          const int fi = 2 * i + dx, fj = 2 * j + dy, fk = 2 * k + dz;
+          const int fc = fi + F.nx * (fj + F.ny * fk);
+          sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
+        }
+    bc[c] = sum;
+    xc[c] = (S)lv_odiag(Cl, i, j, k) * sum;
+  }
+  __syncthreads();
+}
+
+template <typename S>
+__device__ __forceinline__ void tail_prolong(const MgLevel& F, const MgLevel& Cl, const S* __restrict__ xc,
+                                             const S* __restrict__ xin, S* __restrict__ xout) {
+  const int n = F.nx * F.ny * F.nz;
+  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
+  const S a = (S)0.75, bq = (S)0.25;
+  for (int c = threadIdx.x; c < n; c += kTailThreads) {
+    const int i = c % F.nx, j = (c / F.nx) % F.ny, k = c / (F.nx * F.ny);
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
+    if (In < 0 || In >= Cl.nx) In = I;
+    if (Jn < 0 || Jn >= Cl.ny) Jn = J;
+    if (Kn < 0 || Kn >= Cl.nz) Kn = K;
+    const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
+    const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
+    const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
+    const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
+    xout[c] = xin[c] + (a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11));
+  }
+  __syncthreads();
+}
+
+template <typename S>
+__device__ __forceinline__ void tail_copy(int n, const S* __restrict__ src, S* __restrict__ dst) {
+  for (int c = threadIdx.x; c < n; c += kTailThreads) dst[c] = src[c];
+  __syncthreads();
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kTailThreads)
+k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
+  if (ctl->done) return;
+  __shared__ double bs[kMgDenseMax];
+  __shared__ double red[kTailThreads / 64];
+  const int levels = H->levels;
+  S* cur[kMgMaxLevels];
+  S* nxt[kMgMaxLevels];
+  for (int l = l0; l < levels - 1; ++l) {
+    const MgLevel& L = H->lv[l];
+    cur[l] = (S*)H->x[l];
+    nxt[l] = (S*)H->y[l];
+    for (int sweep = 1; sweep < H->pre; ++sweep) {
+      tail_smooth<S>(L, cur[l], (const S*)H->b[l], nxt[l]);
+      S* t = cur[l];
+      cur[l] = nxt[l];
+      nxt[l] = t;
+    }
+    tail_restrict<S>(L, H->lv[l + 1], cur[l], (const S*)H->b[l], (S*)H->b[l + 1], (S*)H->x[l + 1]);
+  }
+  // coarsest: dense inverse (callers guarantee n <= kMgDenseMax)
+  {
+    const MgLevel& L = H->lv[levels - 1];
+    const int n = L.nx * L.ny * L.nz;
+    const S* b = (const S*)H->b[levels - 1];
+    S* x = (S*)H->x[levels - 1];
+    const int t = threadIdx.x;
+    double v = 0.0;
+    for (int c = t; c < n; c += kTailThreads) {
+      bs[c] = (double)b[c];
+      v += bs[c];
+    }
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    if ((t & 63) == 0) red[t >> 6] = v;
+    __syncthreads();
+    double total = 0.0;
+    for (int w = 0; w < kTailThreads / 64; ++w) total += red[w];
+    const double mean = total / (double)n;
+    for (int c = t; c < n; c += kTailThreads) {
+      double acc = 0.0;
+      for (int j2 = 0; j2 < n; ++j2) acc += H->coarse_inv[(size_t)j2 * n + c] * (bs[j2] - mean);
+      x[c] = (S)acc;
+    }
+    __syncthreads();
+  }
+  for (int l = levels - 2; l >= l0; --l) {
+    const MgLevel& L = H->lv[l];
+    tail_prolong<S>(L, H->lv[l + 1], (const S*)H->x[l + 1], cur[l], nxt[l]);
+    {
+      S* t = cur[l];
+      cur[l] = nxt[l];
+      nxt[l] = t;
+    }
+    for (int sweep = 0; sweep < H->post; ++sweep) {
+      tail_smooth<S>(L, cur[l], (const S*)H->b[l], nxt[l]);
+      S* t = cur[l];
+      cur[l] = nxt[l];
+      nxt[l] = t;
+    }
+    if (cur[l] != (S*)H->x[l]) tail_copy<S>(L.nx * L.ny * L.nz, cur[l], (S*)H->x[l]);
+  }
+}
+
 // p <- p - p0 with p0 = p[0] captured in ctl->spare[0] beforehand.
 __global__ void k_capture_gauge(const double* __restrict__ p, PcgControl* ctl) {
   if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[0];
@@ -348,6 +479,11 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   S* x = (S*)h.x[l];
   S* y = (S*)h.y[l];
   const S* b = (const S*)h.b[l];
+  if (h.tail_level > 0 && l == h.tail_level) {
+    prof::Scope ps(prof::kMgCoarse, s, -2);
+    k_mg_tail<S><<<1, kTailThreads, 0, s>>>(h.d_self, l, ctl);
+    return;
+  }
   if (l == h.levels - 1) {
     prof::Scope ps(prof::kMgCoarse, s, -2);
     if (h.coarse_inv)

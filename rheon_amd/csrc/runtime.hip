@@ -91,6 +91,13 @@ DevGrid make_dev_grid(const rh_grid& g, double density) {
     d.upper[a] = g.origin[a] + (double)g.counts[a] * g.spacing[a];
   }
   d.volume = g.spacing[0] * g.spacing[1] * g.spacing[2];
+  for (int a = 0; a < 3; ++a) {
+    int e = 0;
+    const double m = std::frexp(g.spacing[a], &e);
+    const double inv = 1.0 / g.spacing[a];
+    d.inv_h[a] = inv;
+    d.pow2[a] = (m == 0.5 && std::isnormal(inv) && g.spacing[a] * inv == 1.0) ? 1 : 0;
+  }
   const double* h = g.spacing;
   d.area[0] = h[1] * h[2];
   d.area[1] = h[0] * h[2];

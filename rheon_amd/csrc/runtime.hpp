@@ -113,6 +113,7 @@ class MgSolver : public Solver {
   MgHierarchy h_;
   void* arena_ = nullptr;
   double* d_inv_ = nullptr;   // dense coarsest inverse (device)
+  MgHierarchy* d_h_ = nullptr;  // device copy of h_ read by the tail kernel
   double inv_density_ = 0.0;  // density the inverse was built for
   uint64_t bytes_ = 0;
 };

@@ -259,6 +259,7 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
 MgSolver::~MgSolver() {
   if (arena_) (void)hipFree(arena_);
   if (d_inv_) (void)hipFree(d_inv_);
+  if (d_h_) (void)hipFree(d_h_);
   scratch_.release();
 }
 
@@ -334,25 +335,33 @@ rh_status MgSolver::set_density(double density) {
     }
     for (int a = 0; a < 3; ++a) h[a] *= 2.0;
   }
+  if (d_inv_ && d_h_ && inv_density_ == density) {
+    h_.coarse_inv = d_inv_;
+    h_.d_self = d_h_;
+    return RH_OK;
+  }
   const MgLevel& last = h_.lv[h_.levels - 1];
   const int n = last.nx * last.ny * last.nz;
-  if (n > kMgDenseMax) {
-    h_.coarse_inv = nullptr;
-    return RH_OK;
-  }
-  if (d_inv_ && inv_density_ == density) {
-    h_.coarse_inv = d_inv_;
-    return RH_OK;
-  }
+  h_.coarse_inv = nullptr;
   std::vector<double> inv;
-  if (!dense_coarse_inverse(last, inv)) {
-    h_.coarse_inv = nullptr;
-    return RH_OK;
+  if (n <= kMgDenseMax && dense_coarse_inverse(last, inv)) {
+    if (!d_inv_) RH_HIP(hipMalloc(&d_inv_, sizeof(double) * (size_t)kMgDenseMax * kMgDenseMax));
+    RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+    h_.coarse_inv = d_inv_;
   }
-  if (!d_inv_) RH_HIP(hipMalloc(&d_inv_, sizeof(double) * (size_t)kMgDenseMax * kMgDenseMax));
-  RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
+  // Tail: the first level below the finest that fits kMgTailCells (the tail
+  // needs the dense coarsest inverse).
+  h_.tail_level = 0;
+  if (h_.coarse_inv)
+    for (int l = 1; l < h_.levels; ++l)
+      if ((long)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz <= kMgTailCells) {
+        h_.tail_level = l;
+        break;
+      }
+  if (!d_h_) RH_HIP(hipMalloc(&d_h_, sizeof(MgHierarchy)));
+  h_.d_self = d_h_;
+  RH_HIP(hipMemcpy(d_h_, &h_, sizeof(MgHierarchy), hipMemcpyHostToDevice));
   inv_density_ = density;
-  h_.coarse_inv = d_inv_;
   return RH_OK;
 }
 

@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 import oracle.mg as M
 import rheon_amd as R
-from rheon_amd.scenes import GridSpec, fourier_modes
+from rheon_amd.scenes import GridSpec, fourier_modes, taylor_green_noise
 
 from helpers import config1, ogrid, rel_err
 
@@ -62,6 +62,23 @@ def test_device_mg_reaches_reference_pressure(precision):
     # the reported residual is that of the returned (gauged) pressure
     res = rhs - O.apply_full(g, 1.0, p)
     assert abs(np.sqrt(np.sum(res * res)) - rep.true_residual_l2) <= 1e-6 * rep.true_residual_l2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("counts", [(40, 24, 16), (60, 60, 60), (33, 17, 9), (8, 8, 8)])
+def test_device_mg_uneven_hierarchies(counts):
+    """Dense coarsest + one-workgroup tail (40x24x16), CG coarsest (60^3 ->
+    15^3), no coarsening at all (odd axes), tiny grids."""
+    spec = GridSpec.unit_box(*counts)
+    vel = taylor_green_noise(spec, 7, 0.1, np.float32)
+    dt = 0.25 * min(spec.spacing)
+    g = ogrid(spec)
+    rhs = O.build_rhs(g, vel, dt)
+    ref = O.solve_rhs(g, 1.0, rhs, dt, O.Settings(1e-12, 1e-14, 1.0, 20000), threads=8)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32)
+    rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-10, 1e-14, 1.0, 400))
+    assert rep.ok, R.STATUS_NAMES.get(rep.status)
+    assert rel_err(ws.get_pressure(), ref.pressure) < 1e-7
 
 
 @pytest.mark.gpu
