@@ -17,7 +17,7 @@ struct MgLevel {
 };
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
-constexpr int kMgTailCells = 32768;  // levels up to this size run in the one-workgroup tail
+constexpr int kMgTailLds = 56 * 1024;  // LDS bytes for the tail levels' x, y, b arrays
 
 struct MgHierarchy {
   int levels = 0;
@@ -29,6 +29,8 @@ struct MgHierarchy {
   // Levels >= tail_level (> 0) run in one single-workgroup kernel that reads
   // this struct from device memory at d_self; 0 = no tail.
   int tail_level = 0;
+  int tail_lds = 0;                  // dynamic LDS bytes of the tail launch
+  int tail_off[kMgMaxLevels][3] = {};  // LDS byte offsets of x, y, b per tail level
   const MgHierarchy* d_self = nullptr;
   MgLevel lv[kMgMaxLevels];
   void* x[kMgMaxLevels] = {};  // V-cycle iterate (S)

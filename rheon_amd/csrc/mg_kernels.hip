@@ -343,29 +343,50 @@ template <typename S>
 __global__ void __launch_bounds__(kTailThreads)
 k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
   if (ctl->done) return;
+  // Level arrays of the tail live in LDS (offsets in H->tail_off); level l0
+  // is loaded from and written back to global memory.
+  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double bs[kMgDenseMax];
   __shared__ double red[kTailThreads / 64];
   const int levels = H->levels;
+  auto X = [&](int l) { return (S*)(smem + H->tail_off[l][0]); };
+  auto Y = [&](int l) { return (S*)(smem + H->tail_off[l][1]); };
+  auto B = [&](int l) { return (S*)(smem + H->tail_off[l][2]); };
+  {
+    const MgLevel& L = H->lv[l0];
+    const int n = L.nx * L.ny * L.nz;
+    const S* gb = (const S*)H->b[l0];
+    const S* gx = (const S*)H->x[l0];
+    for (int c = threadIdx.x; c < n; c += kTailThreads) {
+      B(l0)[c] = gb[c];
+      X(l0)[c] = gx[c];
+    }
+    __syncthreads();
+  }
+  // Every tail level's iterate starts in X(l); when the tail is only the
+  // coarsest level (l0 == levels - 1) no loop below touches cur[l0].
   S* cur[kMgMaxLevels];
   S* nxt[kMgMaxLevels];
+  for (int l = l0; l < levels; ++l) {
+    cur[l] = X(l);
+    nxt[l] = Y(l);
+  }
   for (int l = l0; l < levels - 1; ++l) {
     const MgLevel& L = H->lv[l];
-    cur[l] = (S*)H->x[l];
-    nxt[l] = (S*)H->y[l];
     for (int sweep = 1; sweep < H->pre; ++sweep) {
-      tail_smooth<S>(L, cur[l], (const S*)H->b[l], nxt[l]);
+      tail_smooth<S>(L, cur[l], B(l), nxt[l]);
       S* t = cur[l];
       cur[l] = nxt[l];
       nxt[l] = t;
     }
-    tail_restrict<S>(L, H->lv[l + 1], cur[l], (const S*)H->b[l], (S*)H->b[l + 1], (S*)H->x[l + 1]);
+    tail_restrict<S>(L, H->lv[l + 1], cur[l], B(l), B(l + 1), X(l + 1));
   }
   // coarsest: dense inverse (callers guarantee n <= kMgDenseMax)
   {
     const MgLevel& L = H->lv[levels - 1];
     const int n = L.nx * L.ny * L.nz;
-    const S* b = (const S*)H->b[levels - 1];
-    S* x = (S*)H->x[levels - 1];
+    const S* b = B(levels - 1);
+    S* x = X(levels - 1);
     const int t = threadIdx.x;
     double v = 0.0;
     for (int c = t; c < n; c += kTailThreads) {
@@ -387,20 +408,38 @@ k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
   }
   for (int l = levels - 2; l >= l0; --l) {
     const MgLevel& L = H->lv[l];
-    tail_prolong<S>(L, H->lv[l + 1], (const S*)H->x[l + 1], cur[l], nxt[l]);
+    tail_prolong<S>(L, H->lv[l + 1], X(l + 1), cur[l], nxt[l]);
     {
       S* t = cur[l];
       cur[l] = nxt[l];
       nxt[l] = t;
     }
     for (int sweep = 0; sweep < H->post; ++sweep) {
-      tail_smooth<S>(L, cur[l], (const S*)H->b[l], nxt[l]);
+      tail_smooth<S>(L, cur[l], B(l), nxt[l]);
       S* t = cur[l];
       cur[l] = nxt[l];
       nxt[l] = t;
     }
-    if (cur[l] != (S*)H->x[l]) tail_copy<S>(L.nx * L.ny * L.nz, cur[l], (S*)H->x[l]);
+    if (l > l0 && cur[l] != X(l)) tail_copy<S>(L.nx * L.ny * L.nz, cur[l], X(l));
   }
+  // level l0's result back to global x[l0] for the parent's prolongation
+  {
+    const MgLevel& L = H->lv[l0];
+    const int n = L.nx * L.ny * L.nz;
+    S* gx = (S*)H->x[l0];
+    for (int c = threadIdx.x; c < n; c += kTailThreads) gx[c] = cur[l0][c];
+  }
+}
+
+// p += x: a one-level hierarchy (no axis can be halved) solves the finest
+// level directly, so the correction is added here.
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_add(DevGrid g, const S* __restrict__ x, double* __restrict__ p, const PcgControl* ctl) {
+  if (ctl->done) return;
+  FOR_CELLS(g)
+  p[c] = p[c] + (double)x[c];
+  END_CELLS
 }
 
 // p <- p - p0 with p0 = p[0] captured in ctl->spare[0] beforehand.
@@ -481,7 +520,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   const S* b = (const S*)h.b[l];
   if (h.tail_level > 0 && l == h.tail_level) {
     prof::Scope ps(prof::kMgCoarse, s, -2);
-    k_mg_tail<S><<<1, kTailThreads, 0, s>>>(h.d_self, l, ctl);
+    k_mg_tail<S><<<1, kTailThreads, h.tail_lds, s>>>(h.d_self, l, ctl);
     return;
   }
   if (l == h.levels - 1) {
@@ -490,6 +529,13 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
       k_mg_coarsest_dense<S><<<1, kDenseThreads, 0, s>>>(h.lv[l], h.coarse_inv, b, x, ctl);
     else
       k_mg_coarsest<S><<<1, kCoarseThreads, 0, s>>>(h.lv[l], b, x, (PcgControl*)ctl);
+    if (l == 0) {
+      DevGrid g{};
+      g.nx = h.lv[0].nx;
+      g.ny = h.lv[0].ny;
+      g.nz = h.lv[0].nz;
+      k_mg_add<S><<<cells_grid(g), block_shape(), 0, s>>>(g, x, p, ctl);
+    }
     return;
   }
   // pre-smoothing: sweeps 2..nu1 (ping-pong x <-> y), result in `cur`

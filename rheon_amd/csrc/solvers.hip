@@ -352,12 +352,27 @@ rh_status MgSolver::set_density(double density) {
   // Tail: the first level below the finest that fits kMgTailCells (the tail
   // needs the dense coarsest inverse).
   h_.tail_level = 0;
-  if (h_.coarse_inv)
-    for (int l = 1; l < h_.levels; ++l)
-      if ((long)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz <= kMgTailCells) {
-        h_.tail_level = l;
-        break;
+  h_.tail_lds = 0;
+  if (h_.coarse_inv) {
+    const size_t s = config_.precision == RH_F64 ? 8 : 4;
+    for (int l = 1; l < h_.levels; ++l) {
+      size_t bytes = 0;
+      for (int m = l; m < h_.levels; ++m)
+        bytes += 3 * ((s * h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz + 15) / 16 * 16);
+      if (bytes > (size_t)kMgTailLds) continue;
+      size_t off = 0;
+      for (int m = l; m < h_.levels; ++m) {
+        const size_t a = (s * h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz + 15) / 16 * 16;
+        for (int q = 0; q < 3; ++q) {
+          h_.tail_off[m][q] = (int)off;
+          off += a;
+        }
       }
+      h_.tail_level = l;
+      h_.tail_lds = (int)bytes;
+      break;
+    }
+  }
   if (!d_h_) RH_HIP(hipMalloc(&d_h_, sizeof(MgHierarchy)));
   h_.d_self = d_h_;
   RH_HIP(hipMemcpy(d_h_, &h_, sizeof(MgHierarchy), hipMemcpyHostToDevice));
