@@ -195,6 +195,15 @@ rh_status rh_solver_solve_rhs(rh_solver* solver, double density, const double* d
                               double* d_pressure, rh_pressure_report* report,
                               rh_cancel_fn cancel, void* cancel_ctx);
 
+/* Solid obstacles for this solver: solid = one flag per cell (nonzero =
+ * solid, host, len N) or NULL for the sealed box.  The discretisation of
+ * StaticObstaclePressure (src/obstacle_pressure.rs:224-309) with open areas in
+ * {0, A} and fluid volumes in {0, V}: a face is open iff both cells are fluid,
+ * solid cells are not unknowns, and the gauge is the lowest-index fluid cell
+ * (p there is exactly 0; p of solid cells is returned as 0).  The fluid must
+ * be one connected region (UNSUPPORTED otherwise). */
+rh_status rh_solver_set_solid(rh_solver* solver, const uint8_t* solid, size_t len);
+
 /* ------------------------------------------------- provider (host memory) */
 /* PressureRequest (src/solver_provider.rs:47-52). rhs: N f64, host. */
 typedef struct rh_pressure_request {
@@ -297,6 +306,10 @@ uint64_t rh_scene_allocated_bytes(const rh_scene* scene);
 /* Upload/download the accepted faces (host arrays in the scene precision). */
 rh_status rh_scene_set_velocity(rh_scene* scene, const void* u, const void* v, const void* w);
 rh_status rh_scene_get_velocity(const rh_scene* scene, void* u, void* v, void* w);
+/* Solid obstacles of the scene (see rh_solver_set_solid).  Obstacle faces
+ * of the accepted velocity are set to 0 now and after every set_velocity,
+ * advection writes 0 on them and the correction leaves them at 0. */
+rh_status rh_scene_set_solid(rh_scene* scene, const uint8_t* solid, size_t len);
 /* Pressure of the last accepted step (N f64, host). */
 rh_status rh_scene_get_pressure(const rh_scene* scene, double* out, size_t len);
 /* Device pointers of the accepted faces (for zero-copy producers). */

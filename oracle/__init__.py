@@ -128,6 +128,13 @@ def _declare(L):
         getattr(L, name).argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double)]
     L.or_precondition_jacobi.argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double), C.c_int]
     L.or_solve_rhs.argtypes = [_G, C.c_double, _P, C.c_double, C.POINTER(_Settings), C.c_int, C.c_int, _P, C.POINTER(_Report)]
+    L.or_obstacle_mask.argtypes = [_G, _P, _P, C.POINTER(C.c_int64)]
+    L.or_apply_masked.argtypes = [_G, C.c_double, _P, _P, _P]
+    L.or_precondition_masked.argtypes = [_G, C.c_double, _P, C.c_int64, C.c_int, _P, _P, C.POINTER(C.c_double)]
+    L.or_solve_rhs_masked.argtypes = [_G, C.c_double, _P, C.c_int64, _P, C.c_double, C.POINTER(_Settings), C.c_int,
+                                      C.c_int, _P, C.POINTER(_Report)]
+    L.or_correct_f32_masked.argtypes = [_G, C.c_double, _P, _P, C.c_double, _P, _P, _P]
+    L.or_actual_divergence_max_f32_masked.argtypes = [_G, _P, _P, _P, _P, C.POINTER(C.c_double)]
 
 
 class OracleError(RuntimeError):
@@ -356,3 +363,101 @@ def solve_velocity(grid: Grid, density: float, vel, dt: float, settings: Setting
                    method: str = "jacobi-pcg-v1", threads: int = 1) -> SolveResult:
     """PressureWorkspace::solve_velocity (src/pressure.rs:344-361)."""
     return solve_rhs(grid, density, build_rhs(grid, vel, dt), dt, settings, method, threads)
+
+
+# --------------------------------------------------------------------------
+# Solid obstacles (src/obstacle_pressure.rs:224-309 with all-or-nothing face
+# apertures): a per-cell open-face mask and the lowest-index fluid cell as the
+# gauge.
+# --------------------------------------------------------------------------
+
+@dataclass
+class Obstacle:
+    mask: np.ndarray  # uint8 per cell: bits 0..5 open faces x-,x+,y-,y+,z-,z+; bit 6 fluid
+    gauge: int
+
+
+def obstacle_mask(grid: Grid, solid: np.ndarray) -> Obstacle:
+    solid = np.ascontiguousarray(solid, np.uint8).reshape(-1)
+    assert solid.size == grid.cell_len
+    mask = np.zeros(grid.cell_len, np.uint8)
+    gauge = C.c_int64()
+    st = lib().or_obstacle_mask(C.byref(grid.s), _ptr(solid), _ptr(mask), C.byref(gauge))
+    if st != OK:
+        raise OracleError(st, "obstacle_mask")
+    return Obstacle(mask, gauge.value)
+
+
+def apply_masked(grid: Grid, density: float, ob: Obstacle, x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    st = lib().or_apply_masked(C.byref(grid.s), density, _ptr(ob.mask), _ptr(x), _ptr(y))
+    if st != OK:
+        raise OracleError(st, "apply_masked")
+    return y
+
+
+def precondition_masked(grid: Grid, density: float, ob: Obstacle, r: np.ndarray, method: str = "jacobi-pcg-v1"):
+    r = np.array(r, np.float64, copy=True)
+    z = np.zeros_like(r)
+    rz = C.c_double()
+    st = lib().or_precondition_masked(C.byref(grid.s), density, _ptr(ob.mask), ob.gauge, METHODS[method],
+                                      _ptr(r), _ptr(z), C.byref(rz))
+    if st != OK:
+        raise OracleError(st, "precondition_masked")
+    return r, z, rz.value
+
+
+def solve_rhs_masked(grid: Grid, density: float, ob: Obstacle, rhs: np.ndarray, dt: float, settings: Settings,
+                     method: str = "jacobi-pcg-v1", threads: int = 1) -> SolveResult:
+    rhs = np.ascontiguousarray(rhs, np.float64)
+    p = np.zeros(grid.cell_len, np.float64)
+    s = _Settings(settings.relative_residual, settings.absolute_residual, settings.divergence_limit,
+                  int(settings.max_iterations))
+    rep = _Report()
+    st = lib().or_solve_rhs_masked(C.byref(grid.s), density, _ptr(ob.mask), ob.gauge, _ptr(rhs), dt, C.byref(s),
+                                   METHODS[method], threads, _ptr(p), C.byref(rep))
+    return SolveResult(st, p, rep.iterations, rep.true_residual_l2, rep.true_residual_max,
+                       rep.predicted_divergence_max, rep.error_iteration, rep.error_value)
+
+
+def correct_masked(grid: Grid, density: float, ob: Obstacle, p: np.ndarray, dt: float, vel):
+    outs = [np.array(x, np.float32, copy=True) for x in vel]
+    st = lib().or_correct_f32_masked(C.byref(grid.s), density, _ptr(ob.mask), _ptr(p), dt,
+                                     _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]))
+    if st != OK:
+        raise OracleError(st, "correct_masked")
+    return outs
+
+
+def actual_divergence_max_masked(grid: Grid, ob: Obstacle, vel) -> float:
+    u, v, w = vel
+    out = C.c_double()
+    st = lib().or_actual_divergence_max_f32_masked(C.byref(grid.s), _ptr(ob.mask), _ptr(u), _ptr(v), _ptr(w),
+                                                   C.byref(out))
+    if st != OK:
+        raise OracleError(st, "actual_divergence_max_masked")
+    return out.value
+
+
+def zero_blocked_faces(grid: Grid, ob: Obstacle, vel):
+    """Faces with a solid cell on either side are closed (velocity 0); box
+    walls are not touched (they carry the wall condition)."""
+    nx, ny, nz = (int(v) for v in grid.s.n)
+    fluid = (ob.mask & 64).astype(bool).reshape(nz, ny, nx)
+    outs = []
+    for axis, comp in enumerate(vel):
+        a = np.array(comp, copy=True).reshape(nz + (axis == 2), ny + (axis == 1), nx + (axis == 0))
+        sl_lo = [slice(None)] * 3
+        sl_hi = [slice(None)] * 3
+        ax = 2 - axis
+        sl_lo[ax] = slice(0, -1)
+        sl_hi[ax] = slice(1, None)
+        inner = [slice(None)] * 3
+        inner[ax] = slice(1, -1)
+        left = fluid[tuple(sl_lo)]
+        right = fluid[tuple(sl_hi)]
+        open_ = left & right
+        a[tuple(inner)] = np.where(open_, a[tuple(inner)], 0)
+        outs.append(a.reshape(-1))
+    return outs

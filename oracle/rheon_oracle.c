@@ -274,11 +274,17 @@ int or_advect_tracer_f32(const or_grid* g, const float* old, const float* u, con
 /* ------------------------------------------------------------------------ */
 
 /* PressureOperator::new coefficients (src/operator.rs:58-97). */
+/* Optional obstacle: per-cell open-face mask (bits x-,x+,y-,y+,z-,z+ and
+ * 64 = fluid) and the eliminated gauge cell (0 in the plain box). */
 typedef struct op_coeff {
   double area[3], acc[3], w[3], diag[64];
+  const uint8_t* mask;
+  int64_t gauge;
 } op_coeff;
 
 static int op_init(const or_grid* g, double density, op_coeff* c) {
+  c->mask = NULL;
+  c->gauge = 0;
   if (!isfinite(density) || density <= 0.0) return OR_INVALID_DENSITY;
   const double* h = g->h;
   c->area[0] = h[1] * h[2];
@@ -307,6 +313,12 @@ static inline int neighbor_mask(const or_grid* g, const int64_t p[3]) {
     mask |= (p[d] + 1 < g->n[d]) << (2 * d + 1);
   }
   return mask;
+}
+
+/* Open-face bits of a row: the neighbour bits in the plain box, the mask
+ * with an obstacle (a face is open when both cells are fluid). */
+static inline int row_mask(const or_grid* g, const op_coeff* c, int64_t row, const int64_t p[3]) {
+  return c->mask ? (c->mask[row] & 63) : neighbor_mask(g, p);
 }
 
 double or_diagonal(const or_grid* g, double density, const int64_t p[3]) {
@@ -383,10 +395,11 @@ static int apply_range(const or_grid* g, const op_coeff* c, const double* x, dou
   const int64_t stride[3] = {1, nx, nx * ny};
   for (int64_t row = begin; row < end; ++row) {
     int64_t p[3] = {row % nx, (row / nx) % ny, row / (nx * ny)};
+    const int m = row_mask(g, c, row, p);
     double value = 0.0;
     for (int d = 0; d < 3; ++d) {
-      if (p[d] > 0) value += c->w[d] * (x[row] - x[row - stride[d]]);
-      if (p[d] + 1 < g->n[d]) value += c->w[d] * (x[row] - x[row + stride[d]]);
+      if (m & (1 << (2 * d))) value += c->w[d] * (x[row] - x[row - stride[d]]);
+      if (m & (2 << (2 * d))) value += c->w[d] * (x[row] - x[row + stride[d]]);
     }
     if (!isfinite(value)) return OR_ARITHMETIC_FAILURE;
     y[row] = value;
@@ -592,8 +605,9 @@ double or_dot(const double* a, const double* b, int64_t n) {
 
 static int jacobi_c(const or_grid* g, const op_coeff* c, double* r, double* z, double* rz,
                     int threads) {
-  r[0] = 0.0;
-  z[0] = 0.0;
+  const int64_t gauge = c->gauge;
+  r[gauge] = 0.0;
+  z[gauge] = 0.0;
   int64_t n = g->cell_len;
   const int64_t nx = g->n[0], ny = g->n[1];
   int status = OR_OK;
@@ -603,9 +617,13 @@ static int jacobi_c(const or_grid* g, const op_coeff* c, double* r, double* z, d
   for (int64_t ch = 0; ch < chunks; ++ch) {
     int64_t e = (ch + 1) * CHUNK_VALUES < n ? (ch + 1) * CHUNK_VALUES : n;
     for (int64_t row = ch * CHUNK_VALUES; row < e; ++row) {
-      if (row == 0) continue;
+      if (row == gauge) continue;
+      if (c->mask && !(c->mask[row] & 64)) {  // solid cell: not an unknown
+        z[row] = 0.0;
+        continue;
+      }
       int64_t p[3] = {row % nx, (row / nx) % ny, row / (nx * ny)};
-      double diagonal = c->diag[neighbor_mask(g, p)];
+      double diagonal = c->diag[row_mask(g, c, row, p)];
       if (!isfinite(diagonal) || diagonal <= 0.0) {
 #pragma omp atomic write
         status = OR_BREAKDOWN;
@@ -670,30 +688,37 @@ int or_precondition_sgs(const or_grid* g, double density, double* r, double* z, 
  *   red reverse:   z = y + (sum_nbr w*z) / D  (neighbours are black)
  * Neighbour order: axis 0..2, lower then upper. */
 static int rbsgs_c(const or_grid* g, const op_coeff* c, double* r, double* z, double* rz) {
-  r[0] = 0.0;
-  z[0] = 0.0;
+  const int64_t gauge = c->gauge;
+  r[gauge] = 0.0;
+  z[gauge] = 0.0;
   const int64_t nx = g->n[0], ny = g->n[1], n = g->cell_len;
   const int64_t stride[3] = {1, nx, nx * ny};
   for (int pass = 0; pass < 3; ++pass) {
     int colour = pass == 1 ? 1 : 0;
-    for (int64_t row = 1; row < n; ++row) {
+    for (int64_t row = 0; row < n; ++row) {
+      if (row == gauge) continue;
       int64_t p[3] = {row % nx, (row / nx) % ny, row / (nx * ny)};
       if (((p[0] + p[1] + p[2]) & 1) != colour) continue;
-      double diagonal = c->diag[neighbor_mask(g, p)];
+      if (c->mask && !(c->mask[row] & 64)) {  // solid cell: not an unknown
+        z[row] = 0.0;
+        continue;
+      }
+      const int m = row_mask(g, c, row, p);
+      double diagonal = c->diag[m];
       if (pass == 0) {
         z[row] = r[row] / diagonal;
       } else if (pass == 1) {
         double value = r[row];
         for (int d = 0; d < 3; ++d) {
-          if (p[d] > 0) value += c->w[d] * z[row - stride[d]];
-          if (p[d] + 1 < g->n[d]) value += c->w[d] * z[row + stride[d]];
+          if (m & (1 << (2 * d))) value += c->w[d] * z[row - stride[d]];
+          if (m & (2 << (2 * d))) value += c->w[d] * z[row + stride[d]];
         }
         z[row] = value / diagonal;
       } else {
         double correction = 0.0;
         for (int d = 0; d < 3; ++d) {
-          if (p[d] > 0) correction += c->w[d] * z[row - stride[d]];
-          if (p[d] + 1 < g->n[d]) correction += c->w[d] * z[row + stride[d]];
+          if (m & (1 << (2 * d))) correction += c->w[d] * z[row - stride[d]];
+          if (m & (2 << (2 * d))) correction += c->w[d] * z[row + stride[d]];
         }
         z[row] = z[row] + correction / diagonal;
       }
@@ -757,12 +782,8 @@ static int precondition(const or_grid* g, const op_coeff* c, int method, double*
   }
 }
 
-int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
-                 const or_settings* st, int method, int threads, double* p, or_report* rep) {
-  memset(rep, 0, sizeof(*rep));
-  op_coeff c;
-  int s = op_init(g, density, &c);
-  if (s) return s;
+static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, double dt,
+                   const or_settings* st, int method, int threads, double* p, or_report* rep) {
   const int64_t n = g->cell_len;
   if (!finite_cells(rhs, n)) return OR_NONFINITE_INPUT;
   if (!isfinite(dt) || dt <= 0.0 || !isfinite(st->relative_residual) ||
@@ -798,7 +819,7 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
   or_report report;
   memset(&report, 0, sizeof(report));
 #define ACCEPTED(R) ((R).true_residual_l2 <= threshold && (R).predicted_divergence_max <= st->divergence_limit)
-  if ((status = true_report(g, &c, rhs, p, w.q, dt, 0, threads, &report))) goto done;
+  if ((status = true_report(g, c, rhs, p, w.q, dt, 0, threads, &report))) goto done;
   *rep = report;
   if (ACCEPTED(report)) goto done;
   if (st->max_iterations == 0) {
@@ -809,13 +830,15 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
   }
   memcpy(w.r, w.q, sizeof(double) * n);
   double rz;
-  if ((status = precondition(g, &c, method, w.r, w.z, &rz, threads))) goto done;
+  if ((status = precondition(g, c, method, w.r, w.z, &rz, threads))) goto done;
   memcpy(w.d, w.z, sizeof(double) * n);
+  /* element-wise updates skip the gauge row (src/pressure.rs:669-679) */
+  const int64_t gauge = c->gauge;
   int par = use_parallel(n - 1, threads);
-  int64_t chunks = (n - 1 + CHUNK_VALUES - 1) / CHUNK_VALUES;
+  int64_t chunks = (n + CHUNK_VALUES - 1) / CHUNK_VALUES;
   for (int64_t it = 1; it <= st->max_iterations; ++it) {
-    if ((status = apply_full_c(g, &c, w.d, w.q, threads))) goto done;
-    w.q[0] = 0.0;
+    if ((status = apply_full_c(g, c, w.d, w.q, threads))) goto done;
+    w.q[gauge] = 0.0;
     double curvature = or_dot(w.d, w.q, n);
     if (!isfinite(curvature) || curvature <= 0.0 || !isfinite(rz) || rz <= 0.0) {
       rep->error_iteration = it;
@@ -830,9 +853,10 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
     }
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads) if (par)
     for (int64_t ch = 0; ch < chunks; ++ch) {
-      int64_t b = 1 + ch * CHUNK_VALUES;
+      int64_t b = ch * CHUNK_VALUES;
       int64_t e = b + CHUNK_VALUES < n ? b + CHUNK_VALUES : n;
       for (int64_t i = b; i < e; ++i) {
+        if (i == gauge) continue;
         p[i] += alpha * w.d[i];
         w.r[i] -= alpha * w.q[i];
       }
@@ -841,7 +865,7 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
     int candidate = recursive_l2 <= threshold;
     int replaced = 0;
     if (candidate || it == st->max_iterations) {
-      if ((status = true_report(g, &c, rhs, p, w.q, dt, it, threads, &report))) {
+      if ((status = true_report(g, c, rhs, p, w.q, dt, it, threads, &report))) {
         rep->error_iteration = it;
         goto done;
       }
@@ -859,7 +883,7 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
       goto done;
     }
     double next_rz;
-    if ((status = precondition(g, &c, method, w.r, w.z, &next_rz, threads))) goto done;
+    if ((status = precondition(g, c, method, w.r, w.z, &next_rz, threads))) goto done;
     if (!isfinite(next_rz) || next_rz <= 0.0) {
       rep->error_iteration = it;
       status = OR_BREAKDOWN;
@@ -871,9 +895,10 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
       double beta = next_rz / rz;
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads) if (par)
       for (int64_t ch = 0; ch < chunks; ++ch) {
-        int64_t b = 1 + ch * CHUNK_VALUES;
+        int64_t b = ch * CHUNK_VALUES;
         int64_t e = b + CHUNK_VALUES < n ? b + CHUNK_VALUES : n;
-        for (int64_t i = b; i < e; ++i) w.d[i] = w.z[i] + beta * w.d[i];
+        for (int64_t i = b; i < e; ++i)
+          if (i != gauge) w.d[i] = w.z[i] + beta * w.d[i];
       }
     }
     rz = next_rz;
@@ -882,4 +907,122 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
 done:
   free(w.r), free(w.d), free(w.q), free(w.z);
   return status;
+}
+
+int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
+                 const or_settings* st, int method, int threads, double* p, or_report* rep) {
+  memset(rep, 0, sizeof(*rep));
+  op_coeff c;
+  int s = op_init(g, density, &c);
+  if (s) return s;
+  return solve_c(g, &c, rhs, dt, st, method, threads, p, rep);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solid obstacles (builder-defined sphere scenes).  Discretisation of      */
+/* src/obstacle_pressure.rs:224-284 with open areas in {0, A} and fluid      */
+/* volumes in {0, V}: a face is open iff both cells are fluid; weights      */
+/* A/(rho h) on open faces; solid cells are not unknowns; one gauge at the  */
+/* lowest-index fluid cell (the largest-volume rule of :299-309 when all    */
+/* fluid volumes tie).  Summation order is the box operator's.              */
+/* ------------------------------------------------------------------------ */
+
+int or_obstacle_mask(const or_grid* g, const uint8_t* solid, uint8_t* mask, int64_t* gauge) {
+  const int64_t nx = g->n[0], ny = g->n[1], n = g->cell_len;
+  const int64_t stride[3] = {1, nx, nx * ny};
+  *gauge = -1;
+  for (int64_t row = 0; row < n; ++row) {
+    int64_t p[3] = {row % nx, (row / nx) % ny, row / (nx * ny)};
+    uint8_t m = 0;
+    if (!solid[row]) {
+      m = 64;
+      if (*gauge < 0) *gauge = row;
+      for (int d = 0; d < 3; ++d) {
+        if (p[d] > 0 && !solid[row - stride[d]]) m |= (uint8_t)(1 << (2 * d));
+        if (p[d] + 1 < g->n[d] && !solid[row + stride[d]]) m |= (uint8_t)(2 << (2 * d));
+      }
+    }
+    mask[row] = m;
+  }
+  return *gauge < 0 ? OR_INVALID_ARGUMENT : OR_OK;
+}
+
+int or_apply_masked(const or_grid* g, double density, const uint8_t* mask, const double* x,
+                    double* y) {
+  op_coeff c;
+  int s = op_init(g, density, &c);
+  if (s) return s;
+  c.mask = mask;
+  return apply_full_c(g, &c, x, y, 1);
+}
+
+int or_precondition_masked(const or_grid* g, double density, const uint8_t* mask, int64_t gauge,
+                           int method, double* r, double* z, double* rz) {
+  op_coeff c;
+  int s = op_init(g, density, &c);
+  if (s) return s;
+  c.mask = mask;
+  c.gauge = gauge;
+  if (method == OR_SGS) return OR_INVALID_ARGUMENT;
+  return precondition(g, &c, method, r, z, rz, 1);
+}
+
+int or_solve_rhs_masked(const or_grid* g, double density, const uint8_t* mask, int64_t gauge,
+                        const double* rhs, double dt, const or_settings* st, int method,
+                        int threads, double* p, or_report* rep) {
+  memset(rep, 0, sizeof(*rep));
+  op_coeff c;
+  int s = op_init(g, density, &c);
+  if (s) return s;
+  if (method == OR_SGS) return OR_INVALID_ARGUMENT;
+  c.mask = mask;
+  c.gauge = gauge;
+  return solve_c(g, &c, rhs, dt, st, method, threads, p, rep);
+}
+
+/* Corrected f32 faces with closed faces left at zero; max |div| over fluid
+ * cells only (src/obstacle_pressure.rs:566-578 keeps solid faces at 0). */
+int or_correct_f32_masked(const or_grid* g, double density, const uint8_t* mask,
+                          const double* p, double dt, float* u, float* v, float* w) {
+  if (!isfinite(dt) || dt <= 0.0) return OR_INVALID_TIME_STEP;
+  op_coeff c;
+  int s = op_init(g, density, &c);
+  if (s) return s;
+  float* vel[3] = {u, v, w};
+  const int64_t stride[3] = {1, g->n[0], g->n[0] * g->n[1]};
+  for (int d = 0; d < 3; ++d) {
+    const int64_t* dims = g->fn[d];
+    for (int64_t k = 0; k < dims[2]; ++k)
+      for (int64_t j = 0; j < dims[1]; ++j)
+        for (int64_t i = 0; i < dims[0]; ++i) {
+          int64_t q[3] = {i, j, k};
+          int64_t index = face_index(g, d, i, j, k);
+          if (q[d] == 0 || q[d] == g->n[d]) continue;
+          int64_t right = i + g->n[0] * (j + g->n[1] * k);
+          if (!(mask[right] & (1 << (2 * d)))) {
+            vel[d][index] = 0.0f;
+            continue;
+          }
+          int64_t left = right - stride[d];
+          double difference = p[right] - p[left];
+          double value = (double)vel[d][index] - dt * c.acc[d] * difference;
+          if (!isfinite(value) || fabs(value) > (double)FLT_MAX) return OR_ARITHMETIC_FAILURE;
+          vel[d][index] = (float)value;
+        }
+  }
+  return OR_OK;
+}
+
+int or_actual_divergence_max_f32_masked(const or_grid* g, const uint8_t* mask, const float* u,
+                                        const float* v, const float* w, double* out) {
+  double* div = malloc(sizeof(double) * g->cell_len);
+  if (!div) return OR_INVALID_ARGUMENT;
+  int s = or_divergence_f32(g, u, v, w, div);
+  double m = 0.0;
+  if (!s)
+    for (int64_t i = 0; i < g->cell_len; ++i)
+      if (mask[i] & 64) m = maxd(m, fabs(div[i]));
+  free(div);
+  *out = m;
+  return s;
 }

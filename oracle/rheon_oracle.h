@@ -147,6 +147,22 @@ int or_actual_divergence_max_f32(const or_grid* g, const float* u, const float* 
                                  const float* w, double* out);
 int or_actual_divergence_max_f64(const or_grid* g, const double* u, const double* v,
                                  const double* w, double* out);
+/* Solid obstacles: per-cell open-face mask from a solid-cell flag array
+ * (bits 0..5 = faces x-,x+,y-,y+,z-,z+ open, bit 6 = fluid) and the gauge
+ * (lowest-index fluid cell).  Discretisation of src/obstacle_pressure.rs:224-309
+ * for all-or-nothing face apertures; summation order of the box operator. */
+int or_obstacle_mask(const or_grid* g, const uint8_t* solid, uint8_t* mask, int64_t* gauge);
+int or_apply_masked(const or_grid* g, double density, const uint8_t* mask, const double* x,
+                    double* y);
+int or_precondition_masked(const or_grid* g, double density, const uint8_t* mask, int64_t gauge,
+                           int method, double* r, double* z, double* rz);
+int or_solve_rhs_masked(const or_grid* g, double density, const uint8_t* mask, int64_t gauge,
+                        const double* rhs, double dt, const or_settings* s, int method,
+                        int threads, double* p, or_report* rep);
+int or_correct_f32_masked(const or_grid* g, double density, const uint8_t* mask,
+                          const double* p, double dt, float* u, float* v, float* w);
+int or_actual_divergence_max_f32_masked(const or_grid* g, const uint8_t* mask, const float* u,
+                                        const float* v, const float* w, double* out);
 /* velocity_rate (src/simulation.rs:1415-1429). */
 double or_velocity_rate_f32(const or_grid* g, const float* u, const float* v, const float* w);
 double or_velocity_rate_f64(const or_grid* g, const double* u, const double* v,

@@ -289,6 +289,16 @@ class PressureWorkspace:
     def allocated_bytes(self) -> int:
         return int(lib().rh_solver_allocated_bytes(self.handle))
 
+    def set_solid(self, solid) -> None:
+        """Solid obstacle cells (one flag per cell, nonzero = solid) or None
+        for the sealed box (StaticObstaclePressure's discretisation,
+        src/obstacle_pressure.rs:224-309, with all-or-nothing apertures)."""
+        if solid is None:
+            _check(lib().rh_solver_set_solid(self.handle, None, 0), "rh_solver_set_solid")
+            return
+        a = np.ascontiguousarray(solid, np.uint8).reshape(-1)
+        _check(lib().rh_solver_set_solid(self.handle, a.ctypes.data, a.size), "rh_solver_set_solid")
+
     def solve_rhs(self, density: float, rhs, dt: float, settings: PressureSettings, cancelled=None) -> PressureReport:
         """PressureWorkspace::solve_rhs (src/pressure.rs:365-393); returns the
         report with its status (pressure() holds the last iterate)."""
@@ -483,6 +493,15 @@ class Scene:
 
     def allocated_bytes(self) -> int:
         return int(lib().rh_scene_allocated_bytes(self.handle))
+
+    def set_solid(self, solid) -> None:
+        """Solid obstacle cells (see PressureWorkspace.set_solid); obstacle
+        faces of the velocity are closed from now on."""
+        if solid is None:
+            _check(lib().rh_scene_set_solid(self.handle, None, 0), "rh_scene_set_solid")
+            return
+        a = np.ascontiguousarray(solid, np.uint8).reshape(-1)
+        _check(lib().rh_scene_set_solid(self.handle, a.ctypes.data, a.size), "rh_scene_set_solid")
 
     def set_velocity(self, vel) -> None:
         dt = dtype_of(self.precision)

@@ -156,6 +156,7 @@ SIGNATURES = {
     "rh_solver_create": (_S, [_G, C.POINTER(SolverConfig), C.POINTER(C.c_void_p)]),
     "rh_solver_destroy": (None, [_P]),
     "rh_solver_allocated_bytes": (C.c_uint64, [_P]),
+    "rh_solver_set_solid": (_S, [_P, _P, C.c_size_t]),
     "rh_solver_solve_rhs": (_S, [_P, C.c_double, _P, C.c_double, C.POINTER(PressureSettings), _P,
                                  C.POINTER(PressureReport), CANCEL_FN, _P]),
     "rh_provider_create": (_S, [C.POINTER(ProviderConfig), C.POINTER(C.c_void_p)]),
@@ -173,6 +174,7 @@ SIGNATURES = {
     "rh_scene_destroy": (None, [_P]),
     "rh_scene_allocated_bytes": (C.c_uint64, [_P]),
     "rh_scene_set_velocity": (_S, [_P, _P, _P, _P]),
+    "rh_scene_set_solid": (_S, [_P, _P, C.c_size_t]),
     "rh_scene_get_velocity": (_S, [_P, _P, _P, _P]),
     "rh_scene_get_pressure": (_S, [_P, _P, C.c_size_t]),
     "rh_scene_device_velocity": (_S, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),

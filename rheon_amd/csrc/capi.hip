@@ -363,6 +363,12 @@ extern "C" rh_status rh_solver_solve_rhs(rh_solver* solver, double density, cons
   return solver->impl->solve(density, d_rhs, dt, *settings, d_pressure, report, cancel, cancel_ctx);
 }
 
+extern "C" rh_status rh_solver_set_solid(rh_solver* solver, const uint8_t* solid, size_t len) {
+  if (!solver) return RH_INVALID_ARGUMENT;
+  if (solid && len != cells_of(solver->impl->grid())) return RH_INVALID_ARGUMENT;
+  return solver->impl->set_solid(solid);
+}
+
 // ------------------------------------------------------------- provider
 
 struct rh_provider {
@@ -705,6 +711,22 @@ extern "C" uint64_t rh_scene_allocated_bytes(const rh_scene* scene) {
   return scene ? scene->bytes : 0;
 }
 
+// Obstacle faces of the accepted velocity are closed (exactly 0).
+static rh_status zero_blocked(rh_scene* sc) {
+  const DevGrid g = sc->solver->impl->dev_grid(sc->config.density);
+  if (!g.mask) return RH_OK;
+  hipStream_t st = sc->solver->impl->scratch().stream;
+  rh_status s = dispatch_precision(sc->config.precision, [&](auto* tag) -> rh_status {
+    using T = std::remove_pointer_t<decltype(tag)>;
+    rh::launch_zero_blocked<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], st);
+    return RH_OK;
+  });
+  if (s) return s;
+  RH_HIP(hipGetLastError());
+  RH_HIP(hipStreamSynchronize(st));
+  return RH_OK;
+}
+
 extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const void* v,
                                            const void* w) {
   if (!sc || !u || !v || !w) return RH_INVALID_ARGUMENT;
@@ -712,7 +734,15 @@ extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const vo
   const size_t fb = prec_bytes(sc->config.precision);
   for (int a = 0; a < 3; ++a)
     RH_HIP(hipMemcpy(sc->acc[a], src[a], sc->flen[a] * fb, hipMemcpyHostToDevice));
-  return RH_OK;
+  return zero_blocked(sc);
+}
+
+extern "C" rh_status rh_scene_set_solid(rh_scene* sc, const uint8_t* solid, size_t len) {
+  if (!sc) return RH_INVALID_ARGUMENT;
+  if (solid && len != sc->n) return RH_INVALID_ARGUMENT;
+  rh_status s = sc->solver->impl->set_solid(solid);
+  if (s) return s;
+  return zero_blocked(sc);
 }
 
 extern "C" rh_status rh_scene_get_velocity(const rh_scene* sc, void* u, void* v, void* w) {
@@ -752,7 +782,7 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   if (cancel && cancel(cancel_ctx)) return RH_CANCELLED;
   rh::Scratch& scr = sc->solver->impl->scratch();
   hipStream_t st = scr.stream;
-  const DevGrid g = rh::make_dev_grid(sc->grid, sc->config.density);
+  const DevGrid g = sc->solver->impl->dev_grid(sc->config.density);
   const rh_precision prec = sc->config.precision;
   rh_status s = scr.clear_flags();
   if (s) return s;

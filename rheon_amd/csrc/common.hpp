@@ -35,8 +35,22 @@ struct DevGrid {
   double diag[64]; // cached boundary diagonals by neighbour mask (src/operator.rs:81-91)
   double inv_h[3]; // 1/h, used only where h is a power of two (exact)
   int pow2[3];     // h is a power of two: x/h == x*inv_h bit for bit
-  int pad2;
+  int gauge;       // eliminated gauge cell (0 in the sealed box)
+  // Optional per-cell open-face mask for solid obstacles: bits 0..5 = faces
+  // x-, x+, y-, y+, z-, z+ open (the neighbour bits of the cached diagonal
+  // table), bit 6 = fluid cell.  nullptr = the plain box.
+  const unsigned char* mask;
 };
+
+constexpr unsigned char kFluidBit = 64;
+
+// Interior face of axis a with face lattice index (i, j, k): its upper cell
+// is (i, j, k), and the face is open when both cells are fluid (the lower-
+// face bit of the upper cell).  Always open in the plain box.
+__device__ __forceinline__ bool face_open(const DevGrid& g, int a, int i, int j, int k) {
+  if (!g.mask) return true;
+  return (g.mask[i + g.nx * (j + g.ny * k)] >> (2 * a)) & 1;
+}
 
 // Double-double accumulator.  Summing rounded products with error-free
 // TwoSum gives (almost always) the correctly rounded sum, which is also what

@@ -180,7 +180,7 @@ k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
       const int p[3] = {i, j, k};
       const int idx = i + fx * row;
       const int top = (a == 0 ? fx : (a == 1 ? fy : fz)) - 1;
-      if (p[a] == 0 || p[a] == top) {
+      if (p[a] == 0 || p[a] == top || !face_open(g, a, i, j, k)) {
         dst[idx] = (T)0.0;
         continue;
       }
@@ -315,6 +315,10 @@ k_correct(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u,
     for (int i = threadIdx.x; i < fx; i += kLanesX) {
       if (a == 0 && (i == 0 || i == g.nx)) continue;
       const int index = i + fx * row;
+      if (!face_open(g, a, i, j, k)) {  // closed obstacle face stays 0
+        dst[index] = (T)0.0;
+        continue;
+      }
       const int right = i + g.nx * (j + g.ny * k);
       const double difference = p[right] - p[right - stride];
       double value;
@@ -442,7 +446,35 @@ void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, co
   k_advect_tracer<<<grid, block, 0, s>>>(g, old, u, v, w, dt, out, flags);
 }
 
+// Obstacle faces (both cells not fluid on either side) set to 0; walls and
+// open faces untouched (src/obstacle_pressure.rs:566-578 keeps them at 0).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_zero_blocked(DevGrid g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w) {
+  const int a = blockIdx.y;
+  T* dst = a == 0 ? u : (a == 1 ? v : w);
+  const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
+  const RowRange rr = block_rows(fy * fz);
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / fy, j = row - k * fy;
+    for (int i = threadIdx.x; i < fx; i += kLanesX) {
+      const int p[3] = {i, j, k};
+      const int top = (a == 0 ? fx : (a == 1 ? fy : fz)) - 1;
+      if (p[a] == 0 || p[a] == top) continue;
+      if (!face_open(g, a, i, j, k)) dst[i + fx * row] = (T)0.0;
+    }
+  }
+}
+
+template <typename T>
+void launch_zero_blocked(const DevGrid& g, void* u, void* v, void* w, hipStream_t s) {
+  if (!g.mask) return;
+  dim3 grid(grid_blocks((g.ny + 1) * (g.nz + 1)), 3), block(kLanesX, kRowsPerBlock);
+  k_zero_blocked<T><<<grid, block, 0, s>>>(g, (T*)u, (T*)v, (T*)w);
+}
+
 #define RH_INSTANTIATE(T)                                                                     \
+  template void launch_zero_blocked<T>(const DevGrid&, void*, void*, void*, hipStream_t);      \
   template void launch_advect<T>(const DevGrid&, const void*, const void*, const void*,        \
                                  double, void*, void*, void*, unsigned*, hipStream_t);         \
   template void launch_validate<T>(const DevGrid&, const void*, const void*, const void*, int, \

@@ -9,12 +9,28 @@ constexpr int kMgMaxLevels = 12;
 constexpr int kMgCoarsestMax = 4096;  // cells solved by one workgroup in LDS
 
 // One level's shape and rediscretised box-operator constants.
+//
+// Solid obstacles: level 0 reads the per-cell open-face mask of the fine
+// operator (bits 0..5 = faces x-, x+, y-, y+, z-, z+ open, bit 6 = fluid).
+// Levels >= 1 read `frac`: per cell the open fraction of its three lower
+// faces (10 bits each: x in bits 0..9, y in 10..19, z in 20..29; fraction =
+// value * fscale) and bit 30 = the cell holds fluid.  A coarse face's open
+// fraction is the mean of its four sub-faces' (the coarse weight is half the
+// sum of the sub-face weights, the area-weighted rediscretisation), so the
+// coarse operator keeps a constant null space per fluid region.  Upper faces
+// are the lower faces of the upper neighbours; wall faces are 0.
 struct MgLevel {
   int nx, ny, nz, pad;
   double w[3];
   double diag[64];
-  double odiag[64];  // omega / diag (weighted-Jacobi scale)
+  double odiag[64];  // omega / diag (weighted-Jacobi scale); 0 for diag 0
+  const unsigned char* mask = nullptr;  // level 0 with obstacles
+  const unsigned* frac = nullptr;       // levels >= 1 with obstacles
+  double wf[3] = {0.0, 0.0, 0.0};       // w[d] * fscale
+  double omega = 0.0;                   // smoother weight (for frac levels)
 };
+
+constexpr unsigned kFracFluid = 1u << 30;
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
 constexpr int kMgTailLds = 56 * 1024;  // LDS bytes for the tail levels' x, y, b arrays
@@ -23,8 +39,10 @@ struct MgHierarchy {
   int levels = 0;
   int pre = 2, post = 2;
   double omega = 6.0 / 7.0;
-  // Coarsest level <= kMgDenseMax cells: (K + c 11^T)^-1, column-major n x n
-  // f64 on the device; x = M^-1 (b - mean b) is the mean-free solution.
+  // Coarsest level <= kMgDenseMax cells: G = M^-1 P with M = K + c 1_a 1_a^T
+  // over the active (fluid) cells a and P the projection removing the mean
+  // over a; column-major n x n f64 (rows and columns of inactive cells are
+  // 0).  x = G b is the solution of K x = b - mean_a(b) with mean_a(x) = 0.
   const double* coarse_inv = nullptr;
   // Levels >= tail_level (> 0) run in one single-workgroup kernel that reads
   // this struct from device memory at d_self; 0 = no tail.

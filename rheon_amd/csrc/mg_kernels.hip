@@ -34,13 +34,57 @@ __device__ __forceinline__ void mg_fail(PcgControl* ctl, int status, int it) {
   ctl->error_iteration = it;
 }
 
+// Active (fluid-holding) cell of a level; every cell of the plain box.
+__device__ __forceinline__ bool lv_active(const MgLevel& L, int c) {
+  if (L.frac) return (L.frac[c] & kFracFluid) != 0;
+  if (L.mask) return (L.mask[c] & kFluidBit) != 0;
+  return true;
+}
+
+// Open fractions (in units of fscale) of the six faces of cell c on a level
+// with `frac`; upper faces are the neighbours' lower faces, walls are 0.
+struct Frac6 {
+  int xm, xp, ym, yp, zm, zp;
+};
+__device__ __forceinline__ Frac6 lv_frac6(const MgLevel& L, int c, int i, int j, int k) {
+  const int sy = L.nx, sz = L.nx * L.ny;
+  const bool hx = i + 1 < L.nx, hy = j + 1 < L.ny, hz = k + 1 < L.nz;
+  const unsigned q = L.frac[c];
+  const unsigned qx = L.frac[hx ? c + 1 : c];
+  const unsigned qy = L.frac[hy ? c + sy : c];
+  const unsigned qz = L.frac[hz ? c + sz : c];
+  Frac6 f;
+  f.xm = (int)(q & 1023u);
+  f.ym = (int)((q >> 10) & 1023u);
+  f.zm = (int)((q >> 20) & 1023u);
+  f.xp = hx ? (int)(qx & 1023u) : 0;
+  f.yp = hy ? (int)((qy >> 10) & 1023u) : 0;
+  f.zp = hz ? (int)((qz >> 20) & 1023u) : 0;
+  return f;
+}
+
+__device__ __forceinline__ int lv_mask(const MgLevel& L, int c, int i, int j, int k) {
+  return L.mask ? (L.mask[c] & 63) : box_mask6(L.nx, L.ny, L.nz, i, j, k);
+}
+
 // Level-local stencil in the V-cycle precision S (arithmetic in S).
 // Branch-free: all six neighbours are loaded at once from clamped
-// addresses (a wall neighbour is the cell itself and contributes 0).
+// addresses (a wall or closed-face neighbour is the cell itself and
+// contributes 0).
 template <typename S>
 __device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, int c, int i, int j,
                                      int k) {
-  const Nbr n = nbr_offsets(L.nx, L.ny, L.nz, i, j, k);
+  if (L.frac) {
+    const Frac6 f = lv_frac6(L, c, i, j, k);
+    const Nbr n = nbr_offsets(L.nx, L.ny, L.nz, i, j, k);
+    const S xc = x[c];
+    const S a = x[c + n.xm], b = x[c + n.xp], cy = x[c + n.ym], d = x[c + n.yp],
+            e = x[c + n.zm], ff = x[c + n.zp];
+    return (S)L.wf[0] * ((S)f.xm * (xc - a) + (S)f.xp * (xc - b)) +
+           (S)L.wf[1] * ((S)f.ym * (xc - cy) + (S)f.yp * (xc - d)) +
+           (S)L.wf[2] * ((S)f.zm * (xc - e) + (S)f.zp * (xc - ff));
+  }
+  const Nbr n = nbr_from_mask(lv_mask(L, c, i, j, k), L.nx, L.nx * L.ny);
   const S xc = x[c];
   const S a = x[c + n.xm], b = x[c + n.xp], cy = x[c + n.ym], d = x[c + n.yp], e = x[c + n.zm],
           f = x[c + n.zp];
@@ -48,16 +92,54 @@ __device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, 
   return w0 * ((xc - a) + (xc - b)) + w1 * ((xc - cy) + (xc - d)) + w2 * ((xc - e) + (xc - f));
 }
 
-__device__ __forceinline__ int lv_mask(const MgLevel& L, int i, int j, int k) {
-  return (i > 0) | ((i + 1 < L.nx) << 1) | ((j > 0) << 2) | ((j + 1 < L.ny) << 3) |
-         ((k > 0) << 4) | ((k + 1 < L.nz) << 5);
+// omega / D (weighted-Jacobi scale) of the cell; 0 where D = 0 (solid).
+__device__ __forceinline__ double lv_odiag(const MgLevel& L, int c, int i, int j, int k) {
+  if (L.frac) {
+    const Frac6 f = lv_frac6(L, c, i, j, k);
+    const double d = L.wf[0] * (double)(f.xm + f.xp) + L.wf[1] * (double)(f.ym + f.yp) +
+                     L.wf[2] * (double)(f.zm + f.zp);
+    return d > 0.0 ? L.omega / d : 0.0;
+  }
+  return L.odiag[lv_mask(L, c, i, j, k)];
 }
-__device__ __forceinline__ double lv_diag(const MgLevel& L, int i, int j, int k) {
-  return L.diag[lv_mask(L, i, j, k)];
-}
-// omega / D (weighted-Jacobi scale) of the cell.
-__device__ __forceinline__ double lv_odiag(const MgLevel& L, int i, int j, int k) {
-  return L.odiag[lv_mask(L, i, j, k)];
+
+// Trilinear prolongation value of fine cell (i, j, k) from the coarse
+// iterate (cell-centred weights 27/64, 9/64, 3/64, 1/64; a missing
+// neighbour at a wall mirrors the parent).  With obstacles the weights of
+// inactive coarse cells are dropped and the rest renormalised.
+template <typename S>
+__device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restrict__ xc, int i, int j,
+                                           int k) {
+  const int I = i >> 1, J = j >> 1, K = k >> 1;
+  int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
+  if (In < 0 || In >= Cl.nx) In = I;
+  if (Jn < 0 || Jn >= Cl.ny) Jn = J;
+  if (Kn < 0 || Kn >= Cl.nz) Kn = K;
+  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
+  const S a = (S)0.75, bq = (S)0.25;
+  if (!Cl.frac) {
+    const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
+    const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
+    const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
+    const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
+    return a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11);
+  }
+  const int ci[2] = {I, In}, cj[2] = {J, Jn}, ck[2] = {K, Kn};
+  const S wt[2] = {a, bq};
+  S num = (S)0, den = (S)0;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const int idx = ci[dx] + sy * cj[dy] + sz * ck[dz];
+        const S w = wt[dx] * wt[dy] * wt[dz];
+        const bool act = (Cl.frac[idx] & kFracFluid) != 0;
+        num += act ? w * xc[idx] : (S)0;
+        den += act ? w : (S)0;
+      }
+  return den > (S)0 ? num / den : (S)0;
 }
 
 #define FOR_LEVEL_CELLS(L)                                                           \
@@ -91,7 +173,7 @@ k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const d
   m[0] = fmax(m[0], fabs(r));
   const S rs = (S)r;
   b0[c] = rs;
-  x0[c] = (S)lv_odiag(L0, i, j, k) * rs;
+  x0[c] = (S)lv_odiag(L0, c, i, j, k) * rs;
   END_CELLS
   block_publish<1, 2>(s, m, pdd, pmx);
 }
@@ -105,7 +187,7 @@ k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __
   if (ctl->done) return;
   FOR_LEVEL_CELLS(L)
   const S ax = s_apply<S>(L, xin, c, i, j, k);
-  const S xn = xin[c] + (S)lv_odiag(L, i, j, k) * (b[c] - ax);
+  const S xn = xin[c] + (S)lv_odiag(L, c, i, j, k) * (b[c] - ax);
   if (p) {
     p[c] = p[c] + (double)xn;
   } else {
@@ -134,7 +216,7 @@ k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restric
         sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
       }
   bc[c] = sum;
-  xc[c] = (S)lv_odiag(Cl, i, j, k) * sum;
+  xc[c] = (S)lv_odiag(Cl, c, i, j, k) * sum;
   END_CELLS
 }
 
@@ -145,27 +227,13 @@ k_mg_prolong(MgLevel F, MgLevel Cl, const S* __restrict__ xc, const S* __restric
              S* __restrict__ xout, const PcgControl* ctl) {
   if (ctl->done) return;
   FOR_LEVEL_CELLS(F)
-  const int I = i >> 1, J = j >> 1, K = k >> 1;
-  int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
-  // Neumann reflection: a missing neighbour mirrors the parent.
-  if (In < 0 || In >= Cl.nx) In = I;
-  if (Jn < 0 || Jn >= Cl.ny) Jn = J;
-  if (Kn < 0 || Kn >= Cl.nz) Kn = K;
-  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
-  const S a = (S)0.75, bq = (S)0.25;
-  const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
-  const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
-  const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
-  const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
-  const S v0 = a * v00 + bq * v10;
-  const S v1 = a * v01 + bq * v11;
-  xout[c] = xin[c] + (a * v0 + bq * v1);
+  xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k);
   END_CELLS
 }
 
-// Coarsest level: one workgroup solves K x = b - mean(b) with f64 CG in LDS
-// (n <= kMgCoarsestMax), x mean-free; tolerance relative 1e-12 or n+2
-// iterations.
+// Coarsest level: one workgroup solves K x = b - mean_a(b) with f64 CG in
+// LDS (n <= kMgCoarsestMax) over the active cells a, x mean-free over a and
+// 0 elsewhere; tolerance relative 1e-12 or n+2 iterations.
 constexpr int kCoarseThreads = 1024;
 
 template <typename S>
@@ -192,12 +260,17 @@ k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const Pc
     __syncthreads();
     return out;
   };
-  double local = 0.0;
-  for (int c = t; c < n; c += kCoarseThreads) local += (double)b[c];
-  const double mean = block_sum(local) / (double)n;
+  double local = 0.0, active = 0.0;
+  for (int c = t; c < n; c += kCoarseThreads)
+    if (lv_active(L, c)) {
+      local += (double)b[c];
+      active += 1.0;
+    }
+  const double na = block_sum(active);
+  const double mean = na > 0.0 ? block_sum(local) / na : 0.0;
   double rr_local = 0.0;
   for (int c = t; c < n; c += kCoarseThreads) {
-    const double r = (double)b[c] - mean;
+    const double r = lv_active(L, c) ? (double)b[c] - mean : 0.0;
     xs[c] = 0.0;
     rs[c] = r;
     ds[c] = r;
@@ -205,21 +278,13 @@ k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const Pc
   }
   double rr = block_sum(rr_local);
   const double rr0 = rr;
-  const int sy = L.nx, sz = L.nx * L.ny;
   for (int it = 0; it < n + 2 && rr > 1e-24 * rr0 && rr > 0.0; ++it) {
     double dq_local = 0.0;
     for (int c = t; c < n; c += kCoarseThreads) {
-      const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / sz;
-      const double xc = ds[c];
-      double v = 0.0;
-      if (i > 0) v += L.w[0] * (xc - ds[c - 1]);
-      if (i + 1 < L.nx) v += L.w[0] * (xc - ds[c + 1]);
-      if (j > 0) v += L.w[1] * (xc - ds[c - sy]);
-      if (j + 1 < L.ny) v += L.w[1] * (xc - ds[c + sy]);
-      if (k > 0) v += L.w[2] * (xc - ds[c - sz]);
-      if (k + 1 < L.nz) v += L.w[2] * (xc - ds[c + sz]);
+      const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
+      const double v = s_apply<double>(L, ds, c, i, j, k);
       qs[c] = v;
-      dq_local += xc * v;
+      dq_local += ds[c] * v;
     }
     const double dq = block_sum(dq_local);
     if (!(dq > 0.0)) break;
@@ -237,13 +302,28 @@ k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const Pc
     __syncthreads();
   }
   double xm_local = 0.0;
-  for (int c = t; c < n; c += kCoarseThreads) xm_local += xs[c];
-  const double xmean = block_sum(xm_local) / (double)n;
-  for (int c = t; c < n; c += kCoarseThreads) xout[c] = (S)(xs[c] - xmean);
+  for (int c = t; c < n; c += kCoarseThreads)
+    if (lv_active(L, c)) xm_local += xs[c];
+  const double xmean = na > 0.0 ? block_sum(xm_local) / na : 0.0;
+  for (int c = t; c < n; c += kCoarseThreads)
+    xout[c] = lv_active(L, c) ? (S)(xs[c] - xmean) : (S)0;
 }
 
-// Coarsest level with a dense inverse: x = M^-1 (b - mean b), one workgroup.
+// Coarsest level with the dense projected inverse: x = G b, one workgroup.
 constexpr int kDenseThreads = 512;
+
+template <int kThreads, typename S>
+__device__ __forceinline__ void dense_solve(int n, const double* __restrict__ G, const S* __restrict__ b,
+                                            S* __restrict__ x, double* bs) {
+  for (int c = threadIdx.x; c < n; c += kThreads) bs[c] = (double)b[c];
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += kThreads) {
+    double acc = 0.0;
+    for (int j2 = 0; j2 < n; ++j2) acc += G[(size_t)j2 * n + c] * bs[j2];
+    x[c] = (S)acc;
+  }
+  __syncthreads();
+}
 
 template <typename S>
 __global__ void __launch_bounds__(kDenseThreads)
@@ -251,25 +331,7 @@ k_mg_coarsest_dense(MgLevel L, const double* __restrict__ minv, const S* __restr
                     S* __restrict__ xout, const PcgControl* ctl) {
   if (ctl->done) return;
   __shared__ double bs[kMgDenseMax];
-  __shared__ double red[kDenseThreads / 64];
-  const int n = L.nx * L.ny * L.nz;
-  const int t = threadIdx.x;
-  double v = 0.0;
-  for (int c = t; c < n; c += kDenseThreads) {
-    bs[c] = (double)b[c];
-    v += bs[c];
-  }
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  if ((t & 63) == 0) red[t >> 6] = v;
-  __syncthreads();
-  double total = 0.0;
-  for (int w = 0; w < kDenseThreads / 64; ++w) total += red[w];
-  const double mean = total / (double)n;
-  for (int c = t; c < n; c += kDenseThreads) {
-    double acc = 0.0;
-    for (int j2 = 0; j2 < n; ++j2) acc += minv[(size_t)j2 * n + c] * (bs[j2] - mean);
-    xout[c] = (S)acc;
-  }
+  dense_solve<kDenseThreads, S>(L.nx * L.ny * L.nz, minv, b, xout, bs);
 }
 
 // ------------------------------------------------------------- V-cycle tail
@@ -285,7 +347,7 @@ __device__ __forceinline__ void tail_smooth(const MgLevel& L, const S* __restric
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
     const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
     const S ax = s_apply<S>(L, xin, c, i, j, k);
-    xout[c] = xin[c] + (S)lv_odiag(L, i, j, k) * (b[c] - ax);
+    xout[c] = xin[c] + (S)lv_odiag(L, c, i, j, k) * (b[c] - ax);
   }
   __syncthreads();
 }
@@ -306,7 +368,7 @@ __device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& C
           sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
         }
     bc[c] = sum;
-    xc[c] = (S)lv_odiag(Cl, i, j, k) * sum;
+    xc[c] = (S)lv_odiag(Cl, c, i, j, k) * sum;
   }
   __syncthreads();
 }
@@ -315,20 +377,9 @@ template <typename S>
 __device__ __forceinline__ void tail_prolong(const MgLevel& F, const MgLevel& Cl, const S* __restrict__ xc,
                                              const S* __restrict__ xin, S* __restrict__ xout) {
   const int n = F.nx * F.ny * F.nz;
-  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
-  const S a = (S)0.75, bq = (S)0.25;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
     const int i = c % F.nx, j = (c / F.nx) % F.ny, k = c / (F.nx * F.ny);
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
-    int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
-    if (In < 0 || In >= Cl.nx) In = I;
-    if (Jn < 0 || Jn >= Cl.ny) Jn = J;
-    if (Kn < 0 || Kn >= Cl.nz) Kn = K;
-    const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
-    const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
-    const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
-    const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
-    xout[c] = xin[c] + (a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11));
+    xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k);
   }
   __syncthreads();
 }
@@ -347,7 +398,6 @@ k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
   // is loaded from and written back to global memory.
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double bs[kMgDenseMax];
-  __shared__ double red[kTailThreads / 64];
   const int levels = H->levels;
   auto X = [&](int l) { return (S*)(smem + H->tail_off[l][0]); };
   auto Y = [&](int l) { return (S*)(smem + H->tail_off[l][1]); };
@@ -381,30 +431,10 @@ k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
     }
     tail_restrict<S>(L, H->lv[l + 1], cur[l], B(l), B(l + 1), X(l + 1));
   }
-  // coarsest: dense inverse (callers guarantee n <= kMgDenseMax)
+  // coarsest: dense projected inverse (callers guarantee n <= kMgDenseMax)
   {
     const MgLevel& L = H->lv[levels - 1];
-    const int n = L.nx * L.ny * L.nz;
-    const S* b = B(levels - 1);
-    S* x = X(levels - 1);
-    const int t = threadIdx.x;
-    double v = 0.0;
-    for (int c = t; c < n; c += kTailThreads) {
-      bs[c] = (double)b[c];
-      v += bs[c];
-    }
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    if ((t & 63) == 0) red[t >> 6] = v;
-    __syncthreads();
-    double total = 0.0;
-    for (int w = 0; w < kTailThreads / 64; ++w) total += red[w];
-    const double mean = total / (double)n;
-    for (int c = t; c < n; c += kTailThreads) {
-      double acc = 0.0;
-      for (int j2 = 0; j2 < n; ++j2) acc += H->coarse_inv[(size_t)j2 * n + c] * (bs[j2] - mean);
-      x[c] = (S)acc;
-    }
-    __syncthreads();
+    dense_solve<kTailThreads, S>(L.nx * L.ny * L.nz, H->coarse_inv, B(levels - 1), X(levels - 1), bs);
   }
   for (int l = levels - 2; l >= l0; --l) {
     const MgLevel& L = H->lv[l];
@@ -442,16 +472,17 @@ k_mg_add(DevGrid g, const S* __restrict__ x, double* __restrict__ p, const PcgCo
   END_CELLS
 }
 
-// p <- p - p0 with p0 = p[0] captured in ctl->spare[0] beforehand.
-__global__ void k_capture_gauge(const double* __restrict__ p, PcgControl* ctl) {
-  if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[0];
+// p <- p - p0 with p0 = p[gauge] captured in ctl->spare[0] beforehand;
+// solid cells (not unknowns) are set to 0.
+__global__ void k_capture_gauge(const double* __restrict__ p, int gauge, PcgControl* ctl) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[gauge];
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl) {
   const double p0 = ctl->spare[0];
   FOR_CELLS(g)
-  p[c] = p[c] - p0;
+  p[c] = is_fluid(g, c) ? p[c] - p0 : 0.0;
   END_CELLS
 }
 
@@ -588,7 +619,7 @@ void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hi
 }
 
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
-  k_capture_gauge<<<1, 64, 0, s>>>(p, ctl);
+  k_capture_gauge<<<1, 64, 0, s>>>(p, g.gauge, ctl);
   k_regauge<<<cells_grid(g), block_shape(), 0, s>>>(g, p, ctl);
 }
 

@@ -27,6 +27,10 @@
 
 namespace rh {
 
+// z = r / D; a solid cell (D = 0, r = 0) gets z = 0.  For every fluid cell
+// this is the reference's division (src/pressure.rs:567).
+__device__ __forceinline__ double jdiv(double r, double d) { return d > 0.0 ? r / d : 0.0; }
+
 __device__ __forceinline__ void fail(PcgControl* ctl, int status, int it) {
   ctl->done = 1;
   ctl->status = status;
@@ -179,7 +183,7 @@ k_apply_dir(DevGrid g, const T* __restrict__ d, T* __restrict__ q, const PcgCont
   FOR_CELLS(g)
   double v = apply_at<T>(g, d, c, i, j, k);
   if (!isfinite(v)) bad[0] = 1.0;
-  if (c == 0) v = 0.0;
+  if (c == g.gauge) v = 0.0;
   const T qs = (T)v;
   q[c] = qs;
   dd_acc(s[0], ld(d, c) * (double)qs);
@@ -199,8 +203,8 @@ k_update(DevGrid g, T* __restrict__ p, T* __restrict__ r, const T* __restrict__ 
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   double none[1] = {0.0};
   FOR_CELLS(g)
-  if (c == 0) {
-    if (SGS) z[0] = (T)0.0;
+  if (c == g.gauge) {
+    if (SGS) z[c] = (T)0.0;
     continue;
   }
   const T pn = (T)(ld(p, c) + alpha * ld(d, c));
@@ -211,9 +215,9 @@ k_update(DevGrid g, T* __restrict__ p, T* __restrict__ r, const T* __restrict__ 
   dd_acc(s[0], rv * rv);
   dd_acc(s[2], rv);
   if (SGS) {
-    if (((i + j + k) & 1) == 0) z[c] = (T)(rv / diag_at(g, i, j, k));
+    if (((i + j + k) & 1) == 0) z[c] = (T)jdiv(rv, diag_at(g, c, i, j, k));
   } else {
-    const double zv = rv / diag_at(g, i, j, k);
+    const double zv = jdiv(rv, diag_at(g, c, i, j, k));
     dd_acc(s[1], rv * zv);
   }
   END_CELLS
@@ -253,18 +257,18 @@ k_replace(DevGrid g, T* r, const T* prod, T* __restrict__ z, const PcgControl* c
   DD s[1] = {{0.0, 0.0}};
   double none[1] = {0.0};
   FOR_CELLS(g)
-  if (c == 0) {
-    r[0] = (T)0.0;
-    if (SGS) z[0] = (T)0.0;
+  if (c == g.gauge) {
+    r[c] = (T)0.0;
+    if (SGS) z[c] = (T)0.0;
     continue;
   }
   const T rn = prod[c];
   r[c] = rn;
   const double rv = (double)rn;
   if (SGS) {
-    if (((i + j + k) & 1) == 0) z[c] = (T)(rv / diag_at(g, i, j, k));
+    if (((i + j + k) & 1) == 0) z[c] = (T)jdiv(rv, diag_at(g, c, i, j, k));
   } else {
-    const double zv = rv / diag_at(g, i, j, k);
+    const double zv = jdiv(rv, diag_at(g, c, i, j, k));
     dd_acc(s[0], rv * zv);
   }
   END_CELLS
@@ -279,9 +283,9 @@ k_sgs_black(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgCont
             int mode) {
   if (!sgs_active(ctl, it, mode)) return;
   FOR_CELLS(g)
-  if (((i + j + k) & 1) == 0) continue;
+  if (((i + j + k) & 1) == 0 || c == g.gauge) continue;
   const double v = nbr_sum<T>(g, z, ld(r, c), c, i, j, k);
-  z[c] = (T)(v / diag_at(g, i, j, k));
+  z[c] = (T)jdiv(v, diag_at(g, c, i, j, k));
   END_CELLS
 }
 
@@ -296,9 +300,9 @@ k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgContro
   double none[1] = {0.0};
   FOR_CELLS(g)
   double zv;
-  if (((i + j + k) & 1) == 0 && c != 0) {
+  if (((i + j + k) & 1) == 0 && c != g.gauge) {
     const double corr = nbr_sum<T>(g, z, 0.0, c, i, j, k);
-    const T zn = (T)(ld(z, c) + corr / diag_at(g, i, j, k));
+    const T zn = (T)(ld(z, c) + jdiv(corr, diag_at(g, c, i, j, k)));
     z[c] = zn;
     zv = (double)zn;
   } else {
@@ -324,11 +328,11 @@ k_direction(DevGrid g, const T* __restrict__ r, const T* __restrict__ z, T* __re
   const bool copy = it == 0 || ctl->replaced;
   const double beta = copy ? 0.0 : next_rz / ctl->rz[it & 1];
   FOR_CELLS(g)
-  if (c == 0) {
-    d[0] = (T)0.0;
+  if (c == g.gauge) {
+    d[c] = (T)0.0;
     continue;
   }
-  const double zv = SGS ? ld(z, c) : ld(r, c) / diag_at(g, i, j, k);
+  const double zv = SGS ? ld(z, c) : jdiv(ld(r, c), diag_at(g, c, i, j, k));
   d[c] = copy ? (T)zv : (T)(zv + beta * ld(d, c));
   END_CELLS
   if (blockIdx.x == 0 && flat_tid() == 0) ctl->iteration = it;

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -122,6 +123,66 @@ rh_status check_density(const rh_grid& g, double density) {
     if (!std::isfinite(d.area[a]) || d.area[a] <= 0.0 || !std::isfinite(d.acc[a]) ||
         d.acc[a] <= 0.0 || !std::isfinite(d.w[a]) || d.w[a] <= 0.0)
       return RH_INVALID_ARGUMENT;  // OperatorError::InvalidCoefficient
+  return RH_OK;
+}
+
+// Solid obstacles (src/obstacle_pressure.rs:224-309 with all-or-nothing
+// apertures): open-face mask per cell from a solid flag per cell, and the
+// gauge at the lowest-index fluid cell.  The fluid must be one connected
+// region (the reference keeps one gauge per component; this solver keeps
+// one).
+rh_status build_obstacle_mask(const rh_grid& g, const uint8_t* solid, std::vector<uint8_t>& mask,
+                              int* gauge) {
+  const int64_t nx = (int64_t)g.counts[0], ny = (int64_t)g.counts[1], nz = (int64_t)g.counts[2];
+  const int64_t n = nx * ny * nz;
+  mask.assign((size_t)n, 0);
+  int64_t first = -1, fluid = 0;
+  for (int64_t k = 0; k < nz; ++k)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const int64_t c = i + nx * (j + ny * k);
+        if (solid[c]) continue;
+        ++fluid;
+        if (first < 0) first = c;
+        uint8_t m = kFluidBit;
+        if (i > 0 && !solid[c - 1]) m |= 1;
+        if (i + 1 < nx && !solid[c + 1]) m |= 2;
+        if (j > 0 && !solid[c - nx]) m |= 4;
+        if (j + 1 < ny && !solid[c + nx]) m |= 8;
+        if (k > 0 && !solid[c - nx * ny]) m |= 16;
+        if (k + 1 < nz && !solid[c + nx * ny]) m |= 32;
+        mask[(size_t)c] = m;
+      }
+  if (first < 0) {
+    set_last_error("every cell is solid", "build_obstacle_mask");
+    return RH_INVALID_ARGUMENT;
+  }
+  // connectivity: flood fill over open faces from the gauge
+  std::vector<uint8_t> seen((size_t)n, 0);
+  std::vector<int64_t> stack;
+  stack.push_back(first);
+  seen[(size_t)first] = 1;
+  int64_t reached = 0;
+  const int64_t off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
+  while (!stack.empty()) {
+    const int64_t c = stack.back();
+    stack.pop_back();
+    ++reached;
+    const uint8_t m = mask[(size_t)c];
+    for (int side = 0; side < 6; ++side)
+      if ((m >> side) & 1) {
+        const int64_t d = c + off[side];
+        if (!seen[(size_t)d]) {
+          seen[(size_t)d] = 1;
+          stack.push_back(d);
+        }
+      }
+  }
+  if (reached != fluid) {
+    set_last_error("the fluid cells form more than one connected region", "build_obstacle_mask");
+    return RH_UNSUPPORTED;
+  }
+  *gauge = (int)first;
   return RH_OK;
 }
 

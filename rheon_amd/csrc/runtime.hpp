@@ -4,6 +4,7 @@
 
 #include <memory>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -15,6 +16,8 @@ const char* last_error();
 rh_status check_indexable(const rh_grid* g);
 rh_status check_density(const rh_grid& g, double density);
 rh_status flags_to_status(unsigned f);
+rh_status build_obstacle_mask(const rh_grid& g, const uint8_t* solid, std::vector<uint8_t>& mask,
+                              int* gauge);
 
 // Per-object device scratch: a stream, reduction partials, counters, error
 // flags, result scalars, the solver control block and a pinned readback page.
@@ -46,7 +49,7 @@ Scratch& global_scratch(std::unique_lock<std::mutex>& lock);
 // (src/pressure.rs:365-393): the caller's RHS and pressure are device arrays.
 class Solver {
  public:
-  virtual ~Solver() = default;
+  virtual ~Solver() { release_mask(); }
   virtual rh_status create(const rh_grid& grid, const rh_solver_config& config) = 0;
   virtual rh_status solve(double density, const double* d_rhs, double dt,
                           const rh_pressure_settings& st, double* d_pressure,
@@ -54,6 +57,14 @@ class Solver {
   virtual uint64_t allocated_bytes() const = 0;
   Scratch& scratch() { return scratch_; }
   const rh_grid& grid() const { return grid_; }
+  // Solid obstacles: `solid` holds one flag per cell (nonzero = solid) on
+  // the host, or is null for the plain box.  Builds the open-face mask,
+  // checks that the fluid is connected and keeps the mask on the device.
+  virtual rh_status set_solid(const uint8_t* solid);
+  const unsigned char* device_mask() const { return d_mask_; }
+  int gauge() const { return gauge_; }
+  // Operator constants with this solver's obstacle mask and gauge.
+  DevGrid dev_grid(double density) const;
 
  protected:
   rh_status check_and_begin(double density, const double* d_rhs, double dt,
@@ -66,10 +77,14 @@ class Solver {
   rh_status drive(int max_iterations, int poll, Step step, bool* cancelled, rh_cancel_fn cancel,
                   void* ctx);
   rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out);
+  void release_mask();
   rh_grid grid_{};
   rh_solver_config config_{};
   Scratch scratch_;
   double dt_ = 0.0;
+  std::vector<uint8_t> hmask_;      // host copy of the obstacle mask (empty = box)
+  unsigned char* d_mask_ = nullptr;  // device obstacle mask
+  int gauge_ = 0;
 };
 
 Solver* make_solver(rh_method method);
@@ -104,6 +119,7 @@ class MgSolver : public Solver {
                   void* ctx) override;
   uint64_t allocated_bytes() const override;
   int levels() const { return h_.levels; }
+  rh_status set_solid(const uint8_t* solid) override;
 
  private:
   template <typename S>
@@ -111,6 +127,10 @@ class MgSolver : public Solver {
                 double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
   rh_status set_density(double density);
   MgHierarchy h_;
+  unsigned* d_frac_ = nullptr;                // coarse-level face fractions (device)
+  std::vector<std::vector<unsigned>> hfrac_;  // host copies per level (index = level)
+  double fscale_[kMgMaxLevels] = {};
+  bool dirty_ = true;                         // hierarchy must be re-uploaded
   void* arena_ = nullptr;
   double* d_inv_ = nullptr;   // dense coarsest inverse (device)
   MgHierarchy* d_h_ = nullptr;  // device copy of h_ read by the tail kernel

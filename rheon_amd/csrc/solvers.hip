@@ -106,6 +106,38 @@ rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgC
   return RH_EXECUTION;
 }
 
+// ------------------------------------------------------------ obstacles
+
+void Solver::release_mask() {
+  if (d_mask_) (void)hipFree(d_mask_);
+  d_mask_ = nullptr;
+  hmask_.clear();
+  gauge_ = 0;
+}
+
+rh_status Solver::set_solid(const uint8_t* solid) {
+  if (!solid) {
+    release_mask();
+    return RH_OK;
+  }
+  std::vector<uint8_t> mask;
+  int gauge = 0;
+  rh_status st = build_obstacle_mask(grid_, solid, mask, &gauge);
+  if (st) return st;
+  if (!d_mask_) RH_HIP(hipMalloc(&d_mask_, mask.size()));
+  RH_HIP(hipMemcpy(d_mask_, mask.data(), mask.size(), hipMemcpyHostToDevice));
+  hmask_.swap(mask);
+  gauge_ = gauge;
+  return RH_OK;
+}
+
+DevGrid Solver::dev_grid(double density) const {
+  DevGrid g = make_dev_grid(grid_, density);
+  g.mask = d_mask_;
+  g.gauge = gauge_;
+  return g;
+}
+
 static void init_control(PcgControl& h, const rh_pressure_settings& st, double dt, double volume) {
   std::memset(&h, 0, sizeof(h));
   h.rel = st.relative_residual;
@@ -192,7 +224,7 @@ rh_status PcgSolver::solve(double density, const double* d_rhs, double dt,
                            rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) {
   rh_status s = check_and_begin(density, d_rhs, dt, st, d_pressure, report, cancel, ctx);
   if (s) return s;
-  const DevGrid g = make_dev_grid(grid_, density);
+  const DevGrid g = dev_grid(density);
   if (config_.precision == RH_F64) return run<double>(g, d_rhs, st, d_pressure, report, cancel, ctx);
   return run<float>(g, d_rhs, st, d_pressure, report, cancel, ctx);
 }
@@ -258,6 +290,7 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
 
 MgSolver::~MgSolver() {
   if (arena_) (void)hipFree(arena_);
+  if (d_frac_) (void)hipFree(d_frac_);
   if (d_inv_) (void)hipFree(d_inv_);
   if (d_h_) (void)hipFree(d_h_);
   scratch_.release();
@@ -265,27 +298,66 @@ MgSolver::~MgSolver() {
 
 uint64_t MgSolver::allocated_bytes() const { return bytes_ + scratch_.bytes_; }
 
-// Dense inverse of M = K + c 11^T (SPD for the Neumann K; c = mean
-// diagonal / n) by Cholesky on the host; for mean-free b, x = M^-1 b is
-// the mean-free solution of K x = b.
-static bool dense_coarse_inverse(const MgLevel& L, std::vector<double>& inv) {
-  const int nx = L.nx, ny = L.ny, nz = L.nz, n = nx * ny * nz;
+// Host view of one level's face weights: the box, the level-0 obstacle mask
+// or the coarse face fractions (same rules as the device stencil).
+struct LevelFaces {
+  const MgLevel& L;
+  const std::vector<uint8_t>* mask;   // level 0 with obstacles
+  const std::vector<unsigned>* frac;  // levels >= 1 with obstacles
+  double fscale;
+  bool active(int c) const {
+    if (frac) return ((*frac)[c] & kFracFluid) != 0;
+    if (mask) return ((*mask)[c] & kFluidBit) != 0;
+    return true;
+  }
+  // weight of face `side` (0..5 = x-, x+, y-, y+, z-, z+) of cell c; 0 if closed
+  double weight(int c, int side) const {
+    const int nx = L.nx, ny = L.ny, nz = L.nz;
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const int a = side / 2;
+    const bool upper = side & 1;
+    const int pos[3] = {i, j, k}, n[3] = {nx, ny, nz}, stride[3] = {1, nx, nx * ny};
+    if (upper ? pos[a] + 1 >= n[a] : pos[a] == 0) return 0.0;
+    if (frac) {
+      const unsigned q = (*frac)[upper ? c + stride[a] : c];
+      return L.w[a] * fscale * (double)((q >> (10 * a)) & 1023u);
+    }
+    if (mask) return ((*mask)[c] >> side) & 1 ? L.w[a] : 0.0;
+    return L.w[a];
+  }
+};
+
+// G = M^-1 P over the active cells a: M = K + c 1_a 1_a^T (SPD on a
+// connected region; c = mean active diagonal / n_a) by Cholesky on the host,
+// P = I - 1_a 1_a^T / n_a.  For any b, x = G b solves K x = P b with
+// mean_a(x) = 0.  Inactive rows and columns of G are 0.
+static bool dense_coarse_inverse(const LevelFaces& F, std::vector<double>& G) {
+  const MgLevel& L = F.L;
+  const int nx = L.nx, ny = L.ny, n = L.nx * L.ny * L.nz;
   std::vector<double> A((size_t)n * n, 0.0);
   double dsum = 0.0;
+  int na = 0;
+  const int off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
   for (int c = 0; c < n; ++c) {
-    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
-    const int nb[6] = {i > 0 ? c - 1 : -1, i + 1 < nx ? c + 1 : -1, j > 0 ? c - nx : -1,
-                       j + 1 < ny ? c + nx : -1, k > 0 ? c - nx * ny : -1,
-                       k + 1 < nz ? c + nx * ny : -1};
-    for (int s = 0; s < 6; ++s)
-      if (nb[s] >= 0) {
-        A[(size_t)c * n + c] += L.w[s / 2];
-        A[(size_t)c * n + nb[s]] -= L.w[s / 2];
-      }
+    if (!F.active(c)) {
+      A[(size_t)c * n + c] = 1.0;
+      continue;
+    }
+    ++na;
+    for (int s = 0; s < 6; ++s) {
+      const double w = F.weight(c, s);
+      if (w == 0.0) continue;
+      A[(size_t)c * n + c] += w;
+      A[(size_t)c * n + c + off[s]] -= w;
+    }
     dsum += A[(size_t)c * n + c];
   }
-  const double shift = dsum / n / n;
-  for (auto& a : A) a += shift;
+  if (na == 0) return false;
+  const double shift = dsum > 0.0 ? dsum / na / na : 1.0;
+  for (int r = 0; r < n; ++r)
+    if (F.active(r))
+      for (int c = 0; c < n; ++c)
+        if (F.active(c)) A[(size_t)r * n + c] += shift;
   // Cholesky A = L L^T (lower, in place)
   for (int j = 0; j < n; ++j) {
     double d = A[(size_t)j * n + j];
@@ -299,8 +371,8 @@ static bool dense_coarse_inverse(const MgLevel& L, std::vector<double>& inv) {
       A[(size_t)i * n + j] = v / d;
     }
   }
-  // inverse column by column: solve L L^T x = e_c
-  inv.assign((size_t)n * n, 0.0);
+  // M^-1 column by column: solve L L^T x = e_c (column-major result)
+  std::vector<double> inv((size_t)n * n, 0.0);
   std::vector<double> y(n);
   for (int col = 0; col < n; ++col) {
     for (int i = 0; i < n; ++i) {
@@ -315,7 +387,100 @@ static bool dense_coarse_inverse(const MgLevel& L, std::vector<double>& inv) {
     }
     for (int i = 0; i < n; ++i) inv[(size_t)col * n + i] = y[i];
   }
+  // G = M^-1 P: subtract the mean of the active columns from each active column
+  std::vector<double> mean_col(n, 0.0);
+  for (int col = 0; col < n; ++col)
+    if (F.active(col))
+      for (int i = 0; i < n; ++i) mean_col[i] += inv[(size_t)col * n + i];
+  for (int i = 0; i < n; ++i) mean_col[i] /= na;
+  G.assign((size_t)n * n, 0.0);
+  for (int col = 0; col < n; ++col) {
+    if (!F.active(col)) continue;
+    for (int i = 0; i < n; ++i)
+      if (F.active(i)) G[(size_t)col * n + i] = inv[(size_t)col * n + i] - mean_col[i];
+  }
   return true;
+}
+
+// Obstacle hierarchy: level-0 mask and the coarse face fractions (open
+// finest sub-faces / 4^l; stored exactly as counts while 4^l <= 256, else
+// rounded to 1/1023 steps with an open face never rounded to 0).
+rh_status MgSolver::set_solid(const uint8_t* solid) {
+  rh_status st = Solver::set_solid(solid);
+  if (st) return st;
+  dirty_ = true;
+  hfrac_.assign(h_.levels, {});
+  for (int l = 0; l < h_.levels; ++l) {
+    h_.lv[l].mask = nullptr;
+    h_.lv[l].frac = nullptr;
+    fscale_[l] = 0.0;
+  }
+  if (!d_mask_) return RH_OK;
+  h_.lv[0].mask = d_mask_;
+  if (h_.levels == 1) return RH_OK;
+  // open sub-face counts of each level's three lower faces, fluid flags
+  std::vector<uint32_t> cnt[3];
+  std::vector<uint8_t> fluid;
+  {
+    const size_t n = hmask_.size();
+    for (int a = 0; a < 3; ++a) {
+      cnt[a].resize(n);
+      for (size_t c = 0; c < n; ++c) cnt[a][c] = (hmask_[c] >> (2 * a)) & 1u;
+    }
+    fluid.resize(n);
+    for (size_t c = 0; c < n; ++c) fluid[c] = (hmask_[c] & kFluidBit) ? 1 : 0;
+  }
+  size_t total = 0;
+  for (int l = 1; l < h_.levels; ++l)
+    total += ((size_t)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz + 63) / 64 * 64;
+  if (d_frac_) (void)hipFree(d_frac_);
+  d_frac_ = nullptr;
+  RH_HIP(hipMalloc(&d_frac_, sizeof(unsigned) * total));
+  size_t offset = 0;
+  for (int l = 1; l < h_.levels; ++l) {
+    const MgLevel& F = h_.lv[l - 1];
+    const MgLevel& C = h_.lv[l];
+    const size_t nc = (size_t)C.nx * C.ny * C.nz;
+    std::vector<uint32_t> cc[3];
+    std::vector<uint8_t> cf(nc, 0);
+    for (int a = 0; a < 3; ++a) cc[a].assign(nc, 0);
+    for (int K = 0; K < C.nz; ++K)
+      for (int J = 0; J < C.ny; ++J)
+        for (int I = 0; I < C.nx; ++I) {
+          const size_t c = I + (size_t)C.nx * (J + (size_t)C.ny * K);
+          for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+              for (int dx = 0; dx < 2; ++dx) {
+                const size_t f =
+                    (2 * I + dx) + (size_t)F.nx * ((2 * J + dy) + (size_t)F.ny * (2 * K + dz));
+                cf[c] |= fluid[f];
+                // the child's lower face lies on the coarse lower face of axis a
+                if (dx == 0) cc[0][c] += cnt[0][f];
+                if (dy == 0) cc[1][c] += cnt[1][f];
+                if (dz == 0) cc[2][c] += cnt[2][f];
+              }
+        }
+    const double sub = std::ldexp(1.0, 2 * l);  // finest sub-faces per coarse face
+    const bool exact = sub <= 256.0;
+    fscale_[l] = exact ? 1.0 / sub : 1.0 / 1023.0;
+    std::vector<unsigned> q(nc);
+    for (size_t c = 0; c < nc; ++c) {
+      unsigned word = cf[c] ? kFracFluid : 0u;
+      for (int a = 0; a < 3; ++a) {
+        unsigned v = cc[a][c];
+        if (!exact && v) v = std::max(1u, (unsigned)std::lround(v * 1023.0 / sub));
+        word |= (v & 1023u) << (10 * a);
+      }
+      q[c] = word;
+    }
+    RH_HIP(hipMemcpy(d_frac_ + offset, q.data(), sizeof(unsigned) * nc, hipMemcpyHostToDevice));
+    h_.lv[l].frac = d_frac_ + offset;
+    offset += (nc + 63) / 64 * 64;
+    hfrac_[l].swap(q);
+    for (int a = 0; a < 3; ++a) cnt[a].swap(cc[a]);
+    fluid.swap(cf);
+  }
+  return RH_OK;
 }
 
 // Rediscretised coefficients per level (spacing doubles each level) and the
@@ -326,6 +491,8 @@ rh_status MgSolver::set_density(double density) {
     MgLevel& L = h_.lv[l];
     const double area[3] = {h[1] * h[2], h[0] * h[2], h[0] * h[1]};
     for (int a = 0; a < 3; ++a) L.w[a] = area[a] * (1.0 / (density * h[a]));
+    for (int a = 0; a < 3; ++a) L.wf[a] = L.w[a] * (l > 0 ? fscale_[l] : 1.0);
+    L.omega = h_.omega;
     for (int mask = 0; mask < 64; ++mask) {
       double v = 0.0;
       for (int side = 0; side < 6; ++side)
@@ -335,7 +502,7 @@ rh_status MgSolver::set_density(double density) {
     }
     for (int a = 0; a < 3; ++a) h[a] *= 2.0;
   }
-  if (d_inv_ && d_h_ && inv_density_ == density) {
+  if (!dirty_ && d_inv_ && d_h_ && inv_density_ == density) {
     h_.coarse_inv = d_inv_;
     h_.d_self = d_h_;
     return RH_OK;
@@ -344,7 +511,10 @@ rh_status MgSolver::set_density(double density) {
   const int n = last.nx * last.ny * last.nz;
   h_.coarse_inv = nullptr;
   std::vector<double> inv;
-  if (n <= kMgDenseMax && dense_coarse_inverse(last, inv)) {
+  const int cl = h_.levels - 1;
+  const LevelFaces faces{last, (cl == 0 && !hmask_.empty()) ? &hmask_ : nullptr,
+                         (cl > 0 && !hmask_.empty()) ? &hfrac_[cl] : nullptr, fscale_[cl]};
+  if (n <= kMgDenseMax && dense_coarse_inverse(faces, inv)) {
     if (!d_inv_) RH_HIP(hipMalloc(&d_inv_, sizeof(double) * (size_t)kMgDenseMax * kMgDenseMax));
     RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
     h_.coarse_inv = d_inv_;
@@ -377,6 +547,7 @@ rh_status MgSolver::set_density(double density) {
   h_.d_self = d_h_;
   RH_HIP(hipMemcpy(d_h_, &h_, sizeof(MgHierarchy), hipMemcpyHostToDevice));
   inv_density_ = density;
+  dirty_ = false;
   return RH_OK;
 }
 
@@ -434,7 +605,7 @@ rh_status MgSolver::solve(double density, const double* d_rhs, double dt,
                           rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) {
   rh_status s = check_and_begin(density, d_rhs, dt, st, d_pressure, report, cancel, ctx);
   if (s) return s;
-  const DevGrid g = make_dev_grid(grid_, density);
+  const DevGrid g = dev_grid(density);
   if ((s = set_density(density))) return s;
   if (config_.precision == RH_F64) return run<double>(g, d_rhs, st, d_pressure, report, cancel, ctx);
   return run<float>(g, d_rhs, st, d_pressure, report, cancel, ctx);

@@ -1,5 +1,6 @@
 // Cell-lattice iteration and the 7-point stencil helpers shared by the PCG
-// and multigrid kernels (box fast path of the reference operator).
+// and multigrid kernels (box operator of the reference, optionally with
+// solid obstacles through DevGrid::mask).
 #pragma once
 #include "common.hpp"
 
@@ -25,32 +26,48 @@ __device__ __forceinline__ double ld(const T* __restrict__ x, int i) {
   return (double)x[i];
 }
 
-// Neighbour offsets of cell (i,j,k), 0 where the neighbour is a wall.
-// Branch-free stencils load every neighbour at once (a clamped address is
-// the cell itself): a missing neighbour then contributes w*(x_c - x_c) =
-// +0.0 to the running sum, which leaves every partial sum bit-identical to
-// the reference's skipped term (the sum starts at +0.0 and never becomes
-// -0.0), so the operation order of src/operator.rs:400-416 is kept.
+// Open-face bits of a cell in the order of the cached diagonal table
+// (src/operator.rs:816-820): x-, x+, y-, y+, z-, z+.  In the plain box a face
+// is open when the neighbour exists; with obstacles it comes from the mask.
+__device__ __forceinline__ int box_mask6(int nx, int ny, int nz, int i, int j, int k) {
+  return (i > 0) | ((i + 1 < nx) << 1) | ((j > 0) << 2) | ((j + 1 < ny) << 3) | ((k > 0) << 4) |
+         ((k + 1 < nz) << 5);
+}
+__device__ __forceinline__ int cell_mask6(const DevGrid& g, int c, int i, int j, int k) {
+  return g.mask ? (g.mask[c] & 63) : box_mask6(g.nx, g.ny, g.nz, i, j, k);
+}
+__device__ __forceinline__ bool is_fluid(const DevGrid& g, int c) {
+  return !g.mask || (g.mask[c] & kFluidBit);
+}
+
+// Neighbour offsets across open faces, 0 across closed ones.  Branch-free
+// stencils load every neighbour at once (a clamped address is the cell
+// itself): a closed face then contributes w*(x_c - x_c) = +0.0 to the
+// running sum, which leaves every partial sum bit-identical to the
+// reference's skipped term (the sum starts at +0.0 and never becomes -0.0),
+// so the operation order of src/operator.rs:400-416 is kept.
 struct Nbr {
   int xm, xp, ym, yp, zm, zp;
 };
-__device__ __forceinline__ Nbr nbr_offsets(int nx, int ny, int nz, int i, int j, int k) {
-  const int sy = nx, sz = nx * ny;
+__device__ __forceinline__ Nbr nbr_from_mask(int m, int sy, int sz) {
   Nbr n;
-  n.xm = i > 0 ? -1 : 0;
-  n.xp = i + 1 < nx ? 1 : 0;
-  n.ym = j > 0 ? -sy : 0;
-  n.yp = j + 1 < ny ? sy : 0;
-  n.zm = k > 0 ? -sz : 0;
-  n.zp = k + 1 < nz ? sz : 0;
+  n.xm = (m & 1) ? -1 : 0;
+  n.xp = (m & 2) ? 1 : 0;
+  n.ym = (m & 4) ? -sy : 0;
+  n.yp = (m & 8) ? sy : 0;
+  n.zm = (m & 16) ? -sz : 0;
+  n.zp = (m & 32) ? sz : 0;
   return n;
 }
+__device__ __forceinline__ Nbr nbr_offsets(int nx, int ny, int nz, int i, int j, int k) {
+  return nbr_from_mask(box_mask6(nx, ny, nz, i, j, k), nx, nx * ny);
+}
 
-// apply_full box fast path (src/operator.rs:400-416): axis 0..2, lower then upper.
+// apply_full (src/operator.rs:400-416): axis 0..2, lower then upper.
 template <typename T>
 __device__ __forceinline__ double apply_at(const DevGrid& g, const T* __restrict__ x, int c, int i,
                                            int j, int k) {
-  const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
+  const Nbr n = nbr_from_mask(cell_mask6(g, c, i, j, k), g.nx, g.nx * g.ny);
   const double xc = ld(x, c);
   const double a = ld(x, c + n.xm), b = ld(x, c + n.xp), cy = ld(x, c + n.ym),
                d = ld(x, c + n.yp), e = ld(x, c + n.zm), f = ld(x, c + n.zp);
@@ -64,20 +81,19 @@ __device__ __forceinline__ double apply_at(const DevGrid& g, const T* __restrict
   return v;
 }
 
-// diagonal(): cached sum of existing neighbour weights (src/operator.rs:813-822).
-__device__ __forceinline__ double diag_at(const DevGrid& g, int i, int j, int k) {
-  const int mask = (i > 0) | ((i + 1 < g.nx) << 1) | ((j > 0) << 2) | ((j + 1 < g.ny) << 3) |
-                   ((k > 0) << 4) | ((k + 1 < g.nz) << 5);
-  return g.diag[mask];
+// diagonal(): cached sum of open-face weights (src/operator.rs:813-822); 0
+// for a solid cell.
+__device__ __forceinline__ double diag_at(const DevGrid& g, int c, int i, int j, int k) {
+  return g.diag[cell_mask6(g, c, i, j, k)];
 }
 
 // Weighted neighbour sum sum_nbr w_d * x_nbr, axis 0..2, lower then upper.
-// A missing neighbour contributes w * (+0.0) = +0.0 (same argument as above;
-// the red-black kernels' `init` is never -0.0 unless the reference's is).
+// A closed face contributes w * (+0.0) = +0.0 (same argument as above; the
+// red-black kernels' `init` is never -0.0 unless the reference's is).
 template <typename T>
 __device__ __forceinline__ double nbr_sum(const DevGrid& g, const T* __restrict__ x, double init,
                                           int c, int i, int j, int k) {
-  const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
+  const Nbr n = nbr_from_mask(cell_mask6(g, c, i, j, k), g.nx, g.nx * g.ny);
   // unconditional loads (clamped addresses), then select
   const double la = ld(x, c + n.xm), lb = ld(x, c + n.xp), lc = ld(x, c + n.ym),
                ldd = ld(x, c + n.yp), le = ld(x, c + n.zm), lf = ld(x, c + n.zp);

@@ -205,6 +205,47 @@ def fourier_modes(spec: GridSpec, seed: int, modes: int = 32, kmax: int = 16, am
     return out
 
 
+def spheres_solid(spec: GridSpec, spheres) -> np.ndarray:
+    """Solid flag per cell (uint8, x-fastest): a cell is solid when its centre
+    lies strictly inside one of ``spheres`` = [(cx, cy, cz, r), ...].  Faces
+    with a solid cell on either side are closed."""
+    axes = [spec.origin[d] + (np.arange(spec.counts[d], dtype=np.float64) + 0.5) * spec.spacing[d]
+            for d in range(3)]
+    x = axes[0][None, None, :]
+    y = axes[1][None, :, None]
+    z = axes[2][:, None, None]
+    solid = np.zeros((spec.counts[2], spec.counts[1], spec.counts[0]), bool)
+    for cx, cy, cz, r in spheres:
+        solid |= (x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 < r * r
+    return np.ascontiguousarray(solid.astype(np.uint8).ravel())
+
+
+def centred_sphere(spec: GridSpec, radius: float = 0.2) -> np.ndarray:
+    """One sphere of ``radius`` at the box centre."""
+    c = [spec.origin[d] + 0.5 * spec.counts[d] * spec.spacing[d] for d in range(3)]
+    return spheres_solid(spec, [(c[0], c[1], c[2], radius)])
+
+
+def close_obstacle_faces(spec: GridSpec, solid: np.ndarray, fields):
+    """Copies of ``fields`` with every face next to a solid cell set to 0."""
+    nx, ny, nz = spec.counts
+    s = np.asarray(solid).reshape(nz, ny, nx).astype(bool)
+    out = []
+    for a, f in enumerate(fields):
+        g = np.array(f, copy=True).reshape(spec.face_counts(a)[::-1])
+        ax = 2 - a
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        inner = [slice(None)] * 3
+        lo[ax] = slice(0, -1)
+        hi[ax] = slice(1, None)
+        inner[ax] = slice(1, -1)
+        blocked = s[tuple(lo)] | s[tuple(hi)]
+        g[tuple(inner)][blocked] = 0
+        out.append(np.ascontiguousarray(g.ravel()))
+    return out
+
+
 def courant_dt(spec: GridSpec, fields, courant: float = 0.5) -> float:
     """dt = courant * h / max|u| (the explicit step of the workload recipes)."""
     return courant * min(spec.spacing) / max_abs(fields)
