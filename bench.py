@@ -24,12 +24,12 @@ sys.path.insert(0, ROOT)
 
 import rheon_amd as R  # noqa: E402
 from rheon_amd import _native as N  # noqa: E402
-from rheon_amd.scenes import GridSpec, courant_dt, fourier_modes  # noqa: E402
+from rheon_amd.scenes import GridSpec, centred_sphere, courant_dt, fourier_modes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def mg_cycle_bytes(spec: GridSpec, s: int, pre: int = 2, post: int = 2) -> float:
+def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 2, post: int = 2) -> float:
     """Compulsory bytes of one defect-correction cycle (outer f64 residual +
     V(pre, post) in storage of s bytes): per finest cell
       outer residual  read p, b (f64) + write b0, x0           16 + 2s
@@ -37,23 +37,28 @@ def mg_cycle_bytes(spec: GridSpec, s: int, pre: int = 2, post: int = 2) -> float
       restrict  read x, b + write coarse b, x (1/8 cells)       2s + 2s/8
       prolong   read x + coarse x (1/8) + write x               2s + s/8
       post-sweeps (post-1) x 3s, last one adds to p (f64 r/w)   2s + 16
-    and coarse levels l >= 1 the same minus the f64 terms, on N/8^l cells."""
+    and coarse levels l >= 1 the same minus the f64 terms, on N/8^l cells.
+    With obstacles (m = 1) every finest stencil kernel also reads the 1-byte
+    mask and every coarse stencil kernel the 4-byte face-fraction word."""
     n = spec.cell_len
     finest = (16 + 2 * s) + (pre - 1) * 3 * s + (2 * s + 2 * s / 8) + (2 * s + s / 8) + (post - 1) * 3 * s + (2 * s + 16)
+    finest += m * (1 + (pre - 1) + 1 + post)  # obstacle mask reads
     coarse_per_cell = (pre - 1) * 3 * s + (2 * s + 2 * s / 8) + (2 * s + s / 8) + post * 3 * s
+    coarse_per_cell += 4 * m * (pre + 1 + post)  # face-fraction words
     coarse = sum(coarse_per_cell * n / 8 ** l for l in range(1, 8))
     return finest * n + coarse
 
 
-def algorithmic_bytes(spec: GridSpec, face_bytes: int, method: str, solver_bytes: int, iterations: int) -> dict:
+def algorithmic_bytes(spec: GridSpec, face_bytes: int, method: str, solver_bytes: int, iterations: int,
+                      m: int = 0) -> dict:
     """Compulsory HBM bytes of one step (per-kernel minimal-fusion model)."""
     n = spec.cell_len
     F = spec.face_total
     sv = solver_bytes
     if method in ("mixed-mg-dc", "mg-f64-dc"):
-        per_iter = mg_cycle_bytes(spec, sv)
+        per_iter = mg_cycle_bytes(spec, sv, m)
     else:
-        per_iter = {"jacobi-pcg-v1": 11 * sv * n, "rbsgs-pcg": 17 * sv * n}.get(method, 0)
+        per_iter = {"jacobi-pcg-v1": 11 * sv * n + m * n, "rbsgs-pcg": 17 * sv * n + 3 * m * n}.get(method, 0)
     return {
         "advect": 2 * face_bytes * F,
         "rhs": face_bytes * F + 8 * n,
@@ -63,44 +68,58 @@ def algorithmic_bytes(spec: GridSpec, face_bytes: int, method: str, solver_bytes
     }
 
 
-# Algorithmic bytes per launch of each profiled kernel, per cell (s = vector bytes).
+# Algorithmic bytes per launch of each profiled kernel, per cell (s = vector
+# bytes, m = obstacle mask bytes per cell: 1 with obstacles, else 0; the
+# first coarse level's face-fraction word is 4 bytes per coarse cell).
 KERNEL_BYTES_PER_CELL = {
-    "pcg_apply_dir": lambda s: 2 * s,      # read d, write q
-    "pcg_update": lambda s: 6 * s,         # read p, d, r, q; write p, r
-    "pcg_direction": lambda s: 3 * s,      # read r (or z), d; write d
-    "pcg_sgs_black": lambda s: 3 * s,      # read r, z; write z (interleaved colours)
-    "pcg_sgs_red": lambda s: 3 * s,        # read r, z; write z
-    "pcg_true_residual": lambda s: 8 + 8 + s,  # read p (s), b (8), write product (s)
-    "mg_outer_residual": lambda s: 16 + 2 * s,  # read p, b (f64); write b0, x0
-    "mg_smooth": lambda s: 3 * s,              # read x, b; write x
-    "mg_smooth_update": lambda s: 2 * s + 16,  # read x, b; read+write p (f64)
-    "mg_restrict": lambda s: 2 * s + 2 * s / 8,  # read x, b; write coarse b, x
-    "mg_prolong": lambda s: 2 * s + s / 8,       # read x, coarse x; write x
+    "pcg_apply_dir": lambda s, m: 2 * s + m,      # read d (+mask), write q
+    "pcg_update": lambda s, m: 6 * s,             # read p, d, r, q; write p, r
+    "pcg_direction": lambda s, m: 3 * s,          # read r (or z), d; write d
+    "pcg_sgs_black": lambda s, m: 3 * s + m,      # read r, z; write z (interleaved colours)
+    "pcg_sgs_red": lambda s, m: 3 * s + m,        # read r, z; write z
+    "pcg_true_residual": lambda s, m: 8 + 8 + s + m,  # read p (s), b (8), write product (s)
+    "mg_outer_residual": lambda s, m: 16 + 2 * s + m,  # read p, b (f64) (+mask); write b0, x0
+    "mg_smooth": lambda s, m: 3 * s + m,               # read x, b (+mask); write x
+    "mg_smooth_update": lambda s, m: 2 * s + 16 + m,   # read x, b (+mask); read+write p (f64)
+    "mg_restrict": lambda s, m: 2 * s + 2 * s / 8 + m + 4 * m / 8,  # read x, b; write coarse b, x
+    "mg_prolong": lambda s, m: 2 * s + s / 8 + 4 * m / 8,           # read x, coarse x; write x
 }
 
 
-def cpu_baseline(spec: GridSpec, vel, dt: float, iterations_needed: int, budget_s: float = 20.0) -> dict:
+def cpu_baseline(spec: GridSpec, vel, dt: float, iterations_needed: int, solid=None) -> dict:
     """The oracle (C restatement of the reference, its parallel.rs
     decomposition) on this host's cores, on a bounded sample: one advection,
     the RHS, a few JPCG iterations and the correction, extrapolated to the
     JPCG iteration count the reference method needs for this step."""
     import oracle as O
+    from rheon_amd.scenes import close_obstacle_faces
 
     threads = min(os.cpu_count() or 1, 16)
     g = O.Grid(spec.counts, spec.spacing, spec.origin)
+    ob = O.obstacle_mask(g, solid) if solid is not None else None
     t0 = time.perf_counter()
     adv = O.advect_velocity(g, vel, dt, threads=threads)
+    if solid is not None:
+        adv = close_obstacle_faces(spec, solid, adv)
     t_adv = time.perf_counter() - t0
     t0 = time.perf_counter()
     rhs = O.build_rhs(g, adv, dt)
     t_rhs = time.perf_counter() - t0
     k = 4
+    st = O.Settings(0.0, 0.0, 0.0, k)
     t0 = time.perf_counter()
-    r = O.solve_rhs(g, 1.0, rhs, dt, O.Settings(0.0, 0.0, 0.0, k), threads=threads)
+    if ob is None:
+        r = O.solve_rhs(g, 1.0, rhs, dt, st, threads=threads)
+    else:
+        r = O.solve_rhs_masked(g, 1.0, ob, rhs, dt, st, threads=threads)
     t_iter = (time.perf_counter() - t0) / k
     t0 = time.perf_counter()
-    out = O.correct(g, 1.0, r.pressure, dt, adv)
-    O.actual_divergence_max(g, out)
+    if ob is None:
+        out = O.correct(g, 1.0, r.pressure, dt, adv)
+        O.actual_divergence_max(g, out)
+    else:
+        out = O.correct_masked(g, 1.0, ob, r.pressure, dt, adv)
+        O.actual_divergence_max_masked(g, ob, out)
     t_corr = time.perf_counter() - t0
     step = t_adv + t_rhs + t_corr + t_iter * iterations_needed
     return {
@@ -116,6 +135,74 @@ def cpu_baseline(spec: GridSpec, vel, dt: float, iterations_needed: int, budget_
     }
 
 
+def jacobi_settings(st: R.PressureSettings) -> R.PressureSettings:
+    return R.PressureSettings(st.relative_residual, st.absolute_residual, st.divergence_limit, 20000,
+                              poll_interval=32)
+
+
+def step_rhs(spec: GridSpec, scene: R.Scene, dt: float, solid) -> np.ndarray:
+    """RHS of the next step from the scene's accepted faces (device advection,
+    obstacle faces closed, device RHS)."""
+    from rheon_amd.scenes import close_obstacle_faces
+
+    adv = R.advect_velocity(spec, scene.get_velocity(), dt)
+    if solid is not None:
+        adv = close_obstacle_faces(spec, solid, adv)
+    return R.build_rhs(spec, adv, dt)
+
+
+def compare_methods(spec: GridSpec, scene: R.Scene, dt: float, solid, settings: R.PressureSettings) -> dict:
+    """Time-to-tolerance of one pressure solve of the same RHS by each method
+    (device time), and the pressure error of each against the f64
+    jacobi-pcg-v1 solution (bit-identical to the oracle's on the sizes the
+    parity tests cover)."""
+    rhs = step_rhs(spec, scene, dt, solid)
+    out, ref = {}, None
+    cases = [("jacobi-pcg-v1", R.F64), ("jacobi-pcg-v1", R.F32), ("rbsgs-pcg", R.F64), ("mg-f64-dc", R.F64),
+             ("mixed-mg-dc", R.F32)]
+    for method, prec in cases:
+        ws = R.PressureWorkspace(spec, method, prec)
+        if solid is not None:
+            ws.set_solid(solid)
+        st = settings if method in ("mg-f64-dc", "mixed-mg-dc") else jacobi_settings(settings)
+        if prec == R.F32 and method == "jacobi-pcg-v1":
+            # f32 Krylov vectors stall far above 1e-10: capped, reported as reached
+            st = R.PressureSettings(st.relative_residual, st.absolute_residual, st.divergence_limit, 3000,
+                                    poll_interval=32)
+        ws.solve_rhs(1.0, rhs, dt, st)  # warm (allocations, coarse inverse)
+        rep = ws.solve_rhs(1.0, rhs, dt, st)
+        p = ws.get_pressure()
+        key = f"{method}/{'f64' if prec == R.F64 else 'f32'}"
+        entry = {"status": R.STATUS_NAMES.get(rep.status), "iterations": rep.iterations,
+                 "seconds": rep.seconds, "true_residual_l2": rep.true_residual_l2,
+                 "predicted_div_max": rep.predicted_divergence_max}
+        if ref is None:
+            ref = p
+        else:
+            entry["pressure_rel_err_vs_jpcg_f64"] = float(np.max(np.abs(p - ref)) / np.max(np.abs(ref)))
+        out[key] = entry
+        ws.close()
+    base = out["jacobi-pcg-v1/f64"]["seconds"]
+    for k, v in out.items():
+        v["speedup_vs_jpcg_f64"] = base / v["seconds"] if v["seconds"] > 0 else None
+    return out
+
+
+def copy_probe(mib: int) -> int:
+    import ctypes as C
+    lib = N.lib()
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    R._check(lib.rh_device_alloc(nbytes, C.byref(a)), "rh_device_alloc")
+    R._check(lib.rh_device_alloc(nbytes, C.byref(b)), "rh_device_alloc")
+    R._check(lib.rh_copy_device(b, a, nbytes), "rh_copy_device")
+    lib.rh_device_free(a)
+    lib.rh_device_free(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "kernel": "k_copy_bytes", "read_bytes": nbytes,
+                      "write_bytes": nbytes}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,9 +212,17 @@ def main() -> int:
     ap.add_argument("--method", default="mixed-mg-dc",
                     choices=["jacobi-pcg-v1", "rbsgs-pcg", "mixed-mg-dc", "mg-f64-dc"])
     ap.add_argument("--rel", type=float, default=1e-10)
+    ap.add_argument("--sphere", type=float, default=0.2, help="radius of the centred solid sphere (0 = none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--compare", action="store_true",
+                    help="also time one solve of the step's RHS by every method and report pressure errors")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once with the plain copy kernel (the known-size "
+                         "reference for hardware traffic counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py measures one GPU: a projection scene lives on one device and many scenes shard "
               "across GPUs by whole scenes (no collectives); run with --gpus 1.", file=sys.stderr)
@@ -142,6 +237,9 @@ def main() -> int:
     scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method,
                     solver_precision=R.F32 if args.method == "mixed-mg-dc" else R.F64,
                     settings=settings, actual_divergence_limit=1e-3, max_courant=0.0)
+    solid = centred_sphere(spec, args.sphere) if args.sphere > 0 else None
+    if solid is not None:
+        scene.set_solid(solid)
     scene.set_velocity(vel)
     N.lib().rh_device_synchronize()
 
@@ -151,7 +249,32 @@ def main() -> int:
             print(f"warmup step failed: {R.STATUS_NAMES.get(rep.status)}", file=sys.stderr)
             return 1
 
+    import ctypes as C
     lib = N.lib()
+
+    def kernel_table():
+        out = {}
+        for name in list(KERNEL_BYTES_PER_CELL) + ["advect_velocity", "build_rhs", "correct", "divergence_max",
+                                                   "mg_coarse_levels"]:
+            n_l, sec = C.c_int64(), C.c_double()
+            if lib.rh_profile_query(name.encode(), C.byref(n_l), C.byref(sec)) == 0 and n_l.value:
+                out[name] = (n_l.value, sec.value)
+        return out
+
+    # Survey step (untimed): every kernel bracketed by HIP events, to find
+    # the dominant kernel.  Event pairs cost the stream a few microseconds
+    # each, so the timed steps bracket only the dominant kernel.
+    lib.rh_profile_focus(None)
+    lib.rh_profile_enable(1)
+    rep = scene.step(dt)
+    if rep.status != R.OK:
+        print(f"survey step failed: {R.STATUS_NAMES.get(rep.status)}", file=sys.stderr)
+        return 1
+    kernels = kernel_table()
+    lib.rh_profile_enable(0)
+    dom = max((k for k in kernels if k in KERNEL_BYTES_PER_CELL), key=lambda k: kernels[k][1])
+
+    lib.rh_profile_focus(dom.encode())
     lib.rh_profile_enable(1)
     lib.rh_device_synchronize()
     lat, iters, divs = [], [], []
@@ -167,31 +290,22 @@ def main() -> int:
         divs.append(rep.actual_divergence_max)
     lib.rh_device_synchronize()
     elapsed = time.perf_counter() - t0
-    lib.rh_profile_enable(0) if False else None
-
-    # per-kernel live timings (HIP events on the solver stream)
-    import ctypes as C
-    kernels = {}
-    for name in list(KERNEL_BYTES_PER_CELL) + ["advect_velocity", "build_rhs", "correct", "divergence_max",
-                                               "mg_coarse_levels"]:
-        n_l, sec = C.c_int64(), C.c_double()
-        if lib.rh_profile_query(name.encode(), C.byref(n_l), C.byref(sec)) == 0 and n_l.value:
-            kernels[name] = (n_l.value, sec.value)
+    launches, secs = kernel_table()[dom]
     lib.rh_profile_enable(0)
+    lib.rh_profile_focus(None)
 
-    dom = max((k for k in kernels if k in KERNEL_BYTES_PER_CELL), key=lambda k: kernels[k][1])
-    launches, secs = kernels[dom]
     avg = secs / launches
-    per_launch = KERNEL_BYTES_PER_CELL[dom](s_vec) * spec.cell_len
+    m_bytes = 1 if solid is not None else 0
+    per_launch = KERNEL_BYTES_PER_CELL[dom](s_vec, m_bytes) * spec.cell_len
     achieved = per_launch / avg / 1e9
 
     mean_iters = float(np.mean(iters))
-    ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)))
+    ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)), m_bytes)
     step_bytes = sum(ab.values())
     ms = elapsed / args.steps * 1e3
 
     result = {
-        "metric": "projection steps/s at 256^3 to fixed rel residual + max|div u| gate (HBM GB/s, p50/p99)",
+        "metric": "projection steps/s + HBM GB/s at 256^3 to fixed max|div u|; p50/p99 latency",
         "value": args.steps / elapsed,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -203,20 +317,23 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": ("f32 faces + f32 V-cycles, f64 outer residual/pressure" if args.method == "mixed-mg-dc"
                   else "f32 faces, f64 pressure"),
-        "data": "synthetic: 32 seeded Fourier modes |k| in [1,16], amplitude ~|k|^-5/3, max|u| = 1",
+        "data": ("synthetic: 32 seeded Fourier modes |k| in [1,16], amplitude ~|k|^-5/3, max|u| = 1"
+                 + (f", solid sphere r={args.sphere:g} at the box centre" if solid is not None else "")),
         "config": {
             "workload": f"projection step {args.n}^3, {args.method} to rel {args.rel:g}",
             "grid": list(spec.counts),
             "method": args.method,
             "relative_residual": args.rel,
-            "obstacle": "none",
+            "obstacle": (f"sphere r={args.sphere:g} at (0.5,0.5,0.5), {int(solid.sum())} solid cells"
+                         if solid is not None else "none"),
             "iterations_mean": mean_iters,
             "max_div_u": max(divs),
             "latency_ms_p50": float(np.percentile(np.array(lat) * 1e3, 50)),
             "latency_ms_p99": float(np.percentile(np.array(lat) * 1e3, 99)),
             "algorithmic_GBps": step_bytes / (elapsed / args.steps) / 1e9,
             "cell_updates_per_s": spec.cell_len * args.steps / elapsed,
-            "kernels": {k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e6} for k, v in kernels.items()},
+            "kernels_survey_step": {k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e6}
+                                    for k, v in kernels.items()},
         },
         "roofline": {
             "bound": "hbm",
@@ -228,6 +345,7 @@ def main() -> int:
             "traffic": None,
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_us": avg * 1e6,
+            "timed_launches": launches,
         },
         "cpu_baseline": None,
     }
@@ -237,10 +355,14 @@ def main() -> int:
             need = int(round(mean_iters))
         else:
             ws = R.PressureWorkspace(spec, "jacobi-pcg-v1", R.F64)
-            rhs = R.build_rhs(spec, R.advect_velocity(spec, scene.get_velocity(), dt), dt)
-            need = ws.solve_rhs(1.0, rhs, dt, settings).iterations
+            if solid is not None:
+                ws.set_solid(solid)
+            rhs = step_rhs(spec, scene, dt, solid)
+            need = ws.solve_rhs(1.0, rhs, dt, jacobi_settings(settings)).iterations
             ws.close()
-        result["cpu_baseline"] = cpu_baseline(spec, scene.get_velocity(), dt, need)
+        result["cpu_baseline"] = cpu_baseline(spec, scene.get_velocity(), dt, need, solid)
+    if args.compare:
+        result["config"]["compare"] = compare_methods(spec, scene, dt, solid, settings)
     line = json.dumps(result)
     print(line, flush=True)
     if args.json_out:

@@ -123,6 +123,9 @@ rh_status rh_device_free(void* ptr);
 rh_status rh_copy_to_device(void* dst, const void* src, size_t bytes);
 rh_status rh_copy_to_host(void* dst, const void* src, size_t bytes);
 rh_status rh_device_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel (also the known-size
+ * reference for hardware traffic counters); bytes a multiple of 16. */
+rh_status rh_copy_device(void* dst, const void* src, size_t bytes);
 /* GridGeometry::new validation (src/geometry.rs:114-191); cells may be NULL. */
 rh_status rh_grid_validate(const rh_grid* grid, uint64_t* cells);
 /* BufferPlan::for_grid (src/geometry.rs:284-315): reference f32 payload. */
@@ -136,6 +139,9 @@ rh_status rh_buffer_plan(const rh_grid* grid, uint64_t* velocity_pair, uint64_t*
  * converged (which return at once) are not counted. */
 rh_status rh_profile_enable(int32_t on);
 rh_status rh_profile_query(const char* kernel, int64_t* launches, double* seconds);
+/* Time only `kernel`'s launches (NULL = every kernel); each timed launch
+ * costs the stream an event pair. */
+rh_status rh_profile_focus(const char* kernel);
 
 /* ------------------------------------------------ operator kernels (device) */
 /* PressureOperator::apply_full, box path (src/operator.rs:378-456). f64. */
@@ -181,7 +187,12 @@ typedef struct rh_solver_config {
   int32_t mg_post_sweeps;    /* default 2 */
   double mg_omega;           /* weighted-Jacobi smoother weight (0 = default) */
   uint64_t memory_limit;     /* bytes; 0 = unlimited (PressureWorkspace limit) */
+  uint32_t kernel_variant;   /* 0 = default kernels; RH_VARIANT_* bits select alternatives */
+  uint32_t reserved;
 } rh_solver_config;
+
+/* Kernel variants (same results; for tuning and for cross-checking). */
+#define RH_VARIANT_SCALAR_MG 1u /* one cell per lane in every multigrid kernel */
 
 rh_status rh_solver_create(const rh_grid* grid, const rh_solver_config* config,
                            rh_solver** out);

@@ -46,6 +46,7 @@ DEVICE_ERROR = 22
 STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) and k.isupper()}
 
 F64, F32 = 0, 1
+VARIANT_SCALAR_MG = 1
 JACOBI_PCG, RBSGS_PCG, MG_PCG, MIXED_MG = 0, 1, 2, 3
 METHOD_IDS = {"jacobi-pcg-v1": JACOBI_PCG, "rbsgs-pcg": RBSGS_PCG, "mg-pcg": MG_PCG,
               "mixed-mg-dc": MIXED_MG, "mg-f64-dc": MIXED_MG}
@@ -277,10 +278,10 @@ class PressureWorkspace:
     """PressureWorkspace on the device (src/pressure.rs:243-740)."""
 
     def __init__(self, spec: GridSpec, method: str = "jacobi-pcg-v1", precision: int = F64,
-                 memory_limit: int = 0, mg_levels: int = 0):
+                 memory_limit: int = 0, mg_levels: int = 0, kernel_variant: int = 0):
         self.spec = spec
         self.grid = make_grid(spec)
-        cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, 0, 0, 0.0, memory_limit)
+        cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, 0, 0, 0.0, memory_limit, kernel_variant, 0)
         h = C.c_void_p()
         _check(lib().rh_solver_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_solver_create")
         self.handle = h.value
@@ -479,13 +480,15 @@ class Scene:
 
     def __init__(self, spec: GridSpec, density: float = 1.0, precision: int = F32, method: str = "jacobi-pcg-v1",
                  solver_precision: int | None = None, settings: PressureSettings | None = None,
-                 actual_divergence_limit: float = 1e-5, max_courant: float = 0.0, mg_levels: int = 0):
+                 actual_divergence_limit: float = 1e-5, max_courant: float = 0.0, mg_levels: int = 0,
+                 kernel_variant: int = 0):
         self.spec = spec
         self.precision = precision
         self.grid = make_grid(spec)
         settings = settings or PressureSettings()
         sp = precision if solver_precision is None else solver_precision
-        cfg = N.SceneConfig(density, precision, N.SolverConfig(METHOD_IDS[method], sp, mg_levels, 0, 0, 0.0, 0),
+        cfg = N.SceneConfig(density, precision,
+                            N.SolverConfig(METHOD_IDS[method], sp, mg_levels, 0, 0, 0.0, 0, kernel_variant, 0),
                             settings.native(), actual_divergence_limit, max_courant)
         h = C.c_void_p()
         _check(lib().rh_scene_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_scene_create")

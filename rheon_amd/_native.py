@@ -54,6 +54,8 @@ class SolverConfig(C.Structure):
         ("mg_post_sweeps", C.c_int32),
         ("mg_omega", C.c_double),
         ("memory_limit", C.c_uint64),
+        ("kernel_variant", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -141,10 +143,12 @@ SIGNATURES = {
     "rh_copy_to_device": (_S, [_P, _P, C.c_size_t]),
     "rh_copy_to_host": (_S, [_P, _P, C.c_size_t]),
     "rh_device_synchronize": (_S, []),
+    "rh_copy_device": (_S, [_P, _P, C.c_size_t]),
     "rh_grid_validate": (_S, [_G, C.POINTER(C.c_uint64)]),
     "rh_buffer_plan": (_S, [_G] + [C.POINTER(C.c_uint64)] * 4),
     "rh_profile_enable": (_S, [C.c_int32]),
     "rh_profile_query": (_S, [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
+    "rh_profile_focus": (_S, [C.c_char_p]),
     "rh_apply_full": (_S, [_G, C.c_double, _P, _P]),
     "rh_advect_velocity": (_S, [_G, C.c_int, _P, _P, _P, C.c_double, _P, _P, _P]),
     "rh_advect_tracer": (_S, [_G, _P, _P, _P, _P, C.c_double, _P]),

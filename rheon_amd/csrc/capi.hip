@@ -110,6 +110,24 @@ extern "C" rh_status rh_device_synchronize(void) {
   return RH_OK;
 }
 
+// Streaming copy: 16-byte vector loads and stores, grid-stride.
+__global__ void __launch_bounds__(256) k_copy_bytes(uint4* __restrict__ dst,
+                                                    const uint4* __restrict__ src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+extern "C" rh_status rh_copy_device(void* dst, const void* src, size_t bytes) {
+  if (!dst || !src || bytes % 16) return RH_INVALID_ARGUMENT;
+  const size_t n = bytes / 16;
+  if (n == 0) return RH_OK;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+  k_copy_bytes<<<(unsigned)blocks, 256>>>((uint4*)dst, (const uint4*)src, n);
+  RH_HIP(hipGetLastError());
+  RH_HIP(hipDeviceSynchronize());
+  return RH_OK;
+}
+
 extern "C" rh_status rh_grid_validate(const rh_grid* grid, uint64_t* cells) {
   size_t n = 0;
   rh_status s = rh::validate_grid(grid, &n);

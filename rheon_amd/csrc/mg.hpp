@@ -14,7 +14,8 @@ constexpr int kMgCoarsestMax = 4096;  // cells solved by one workgroup in LDS
 // operator (bits 0..5 = faces x-, x+, y-, y+, z-, z+ open, bit 6 = fluid).
 // Levels >= 1 read `frac`: per cell the open fraction of its three lower
 // faces (10 bits each: x in bits 0..9, y in 10..19, z in 20..29; fraction =
-// value * fscale) and bit 30 = the cell holds fluid.  A coarse face's open
+// value * fscale), bit 30 = the cell holds fluid and bit 31 = the cell and
+// all 26 neighbours hold fluid.  A coarse face's open
 // fraction is the mean of its four sub-faces' (the coarse weight is half the
 // sum of the sub-face weights, the area-weighted rediscretisation), so the
 // coarse operator keeps a constant null space per fluid region.  Upper faces
@@ -31,6 +32,9 @@ struct MgLevel {
 };
 
 constexpr unsigned kFracFluid = 1u << 30;
+// Bit 31: the cell and its 26 neighbours (clamped to the level) are active,
+// so prolongation around it needs no renormalisation.
+constexpr unsigned kFracInterior = 1u << 31;
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
 constexpr int kMgTailLds = 56 * 1024;  // LDS bytes for the tail levels' x, y, b arrays
@@ -46,6 +50,7 @@ struct MgHierarchy {
   const double* coarse_inv = nullptr;
   // Levels >= tail_level (> 0) run in one single-workgroup kernel that reads
   // this struct from device memory at d_self; 0 = no tail.
+  bool quad = true;  // use the four-cells-per-lane kernels where a level allows
   int tail_level = 0;
   int tail_lds = 0;                  // dynamic LDS bytes of the tail launch
   int tail_off[kMgMaxLevels][3] = {};  // LDS byte offsets of x, y, b per tail level
@@ -55,6 +60,20 @@ struct MgHierarchy {
   void* y[kMgMaxLevels] = {};  // Jacobi ping-pong partner (S)
   void* b[kMgMaxLevels] = {};  // level right-hand side (S)
 };
+
+// Four-cells-per-lane FP32 kernels (levels with nx % 4 == 0 and no face
+// fractions); identical values to the scalar kernels.
+bool mg_quad_level(const MgLevel& L);
+bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, const double* b);
+void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
+                                float* b0, float* x0, const PcgControl* ctl, DD* pdd, double* pmx,
+                                hipStream_t s);
+void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
+                        const PcgControl* ctl, hipStream_t s);
+void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
+                          float* xc, const PcgControl* ctl, hipStream_t s);
+void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,
+                         float* xout, const PcgControl* ctl, hipStream_t s);
 
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,

@@ -23,6 +23,7 @@
 #include "mg.hpp"
 #include "profiler.hpp"
 #include "stencil.hpp"
+#include "mg_stencil.hpp"
 
 namespace rh {
 
@@ -33,122 +34,6 @@ __device__ __forceinline__ void mg_fail(PcgControl* ctl, int status, int it) {
   ctl->status = status;
   ctl->error_iteration = it;
 }
-
-// Active (fluid-holding) cell of a level; every cell of the plain box.
-__device__ __forceinline__ bool lv_active(const MgLevel& L, int c) {
-  if (L.frac) return (L.frac[c] & kFracFluid) != 0;
-  if (L.mask) return (L.mask[c] & kFluidBit) != 0;
-  return true;
-}
-
-// Open fractions (in units of fscale) of the six faces of cell c on a level
-// with `frac`; upper faces are the neighbours' lower faces, walls are 0.
-struct Frac6 {
-  int xm, xp, ym, yp, zm, zp;
-};
-__device__ __forceinline__ Frac6 lv_frac6(const MgLevel& L, int c, int i, int j, int k) {
-  const int sy = L.nx, sz = L.nx * L.ny;
-  const bool hx = i + 1 < L.nx, hy = j + 1 < L.ny, hz = k + 1 < L.nz;
-  const unsigned q = L.frac[c];
-  const unsigned qx = L.frac[hx ? c + 1 : c];
-  const unsigned qy = L.frac[hy ? c + sy : c];
-  const unsigned qz = L.frac[hz ? c + sz : c];
-  Frac6 f;
-  f.xm = (int)(q & 1023u);
-  f.ym = (int)((q >> 10) & 1023u);
-  f.zm = (int)((q >> 20) & 1023u);
-  f.xp = hx ? (int)(qx & 1023u) : 0;
-  f.yp = hy ? (int)((qy >> 10) & 1023u) : 0;
-  f.zp = hz ? (int)((qz >> 20) & 1023u) : 0;
-  return f;
-}
-
-__device__ __forceinline__ int lv_mask(const MgLevel& L, int c, int i, int j, int k) {
-  return L.mask ? (L.mask[c] & 63) : box_mask6(L.nx, L.ny, L.nz, i, j, k);
-}
-
-// Level-local stencil in the V-cycle precision S (arithmetic in S).
-// Branch-free: all six neighbours are loaded at once from clamped
-// addresses (a wall or closed-face neighbour is the cell itself and
-// contributes 0).
-template <typename S>
-__device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, int c, int i, int j,
-                                     int k) {
-  if (L.frac) {
-    const Frac6 f = lv_frac6(L, c, i, j, k);
-    const Nbr n = nbr_offsets(L.nx, L.ny, L.nz, i, j, k);
-    const S xc = x[c];
-    const S a = x[c + n.xm], b = x[c + n.xp], cy = x[c + n.ym], d = x[c + n.yp],
-            e = x[c + n.zm], ff = x[c + n.zp];
-    return (S)L.wf[0] * ((S)f.xm * (xc - a) + (S)f.xp * (xc - b)) +
-           (S)L.wf[1] * ((S)f.ym * (xc - cy) + (S)f.yp * (xc - d)) +
-           (S)L.wf[2] * ((S)f.zm * (xc - e) + (S)f.zp * (xc - ff));
-  }
-  const Nbr n = nbr_from_mask(lv_mask(L, c, i, j, k), L.nx, L.nx * L.ny);
-  const S xc = x[c];
-  const S a = x[c + n.xm], b = x[c + n.xp], cy = x[c + n.ym], d = x[c + n.yp], e = x[c + n.zm],
-          f = x[c + n.zp];
-  const S w0 = (S)L.w[0], w1 = (S)L.w[1], w2 = (S)L.w[2];
-  return w0 * ((xc - a) + (xc - b)) + w1 * ((xc - cy) + (xc - d)) + w2 * ((xc - e) + (xc - f));
-}
-
-// omega / D (weighted-Jacobi scale) of the cell; 0 where D = 0 (solid).
-__device__ __forceinline__ double lv_odiag(const MgLevel& L, int c, int i, int j, int k) {
-  if (L.frac) {
-    const Frac6 f = lv_frac6(L, c, i, j, k);
-    const double d = L.wf[0] * (double)(f.xm + f.xp) + L.wf[1] * (double)(f.ym + f.yp) +
-                     L.wf[2] * (double)(f.zm + f.zp);
-    return d > 0.0 ? L.omega / d : 0.0;
-  }
-  return L.odiag[lv_mask(L, c, i, j, k)];
-}
-
-// Trilinear prolongation value of fine cell (i, j, k) from the coarse
-// iterate (cell-centred weights 27/64, 9/64, 3/64, 1/64; a missing
-// neighbour at a wall mirrors the parent).  With obstacles the weights of
-// inactive coarse cells are dropped and the rest renormalised.
-template <typename S>
-__device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restrict__ xc, int i, int j,
-                                           int k) {
-  const int I = i >> 1, J = j >> 1, K = k >> 1;
-  int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
-  if (In < 0 || In >= Cl.nx) In = I;
-  if (Jn < 0 || Jn >= Cl.ny) Jn = J;
-  if (Kn < 0 || Kn >= Cl.nz) Kn = K;
-  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
-  const S a = (S)0.75, bq = (S)0.25;
-  if (!Cl.frac) {
-    const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
-    const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
-    const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
-    const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
-    return a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11);
-  }
-  const int ci[2] = {I, In}, cj[2] = {J, Jn}, ck[2] = {K, Kn};
-  const S wt[2] = {a, bq};
-  S num = (S)0, den = (S)0;
-#pragma unroll
-  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-      for (int dx = 0; dx < 2; ++dx) {
-        const int idx = ci[dx] + sy * cj[dy] + sz * ck[dz];
-        const S w = wt[dx] * wt[dy] * wt[dz];
-        const bool act = (Cl.frac[idx] & kFracFluid) != 0;
-        num += act ? w * xc[idx] : (S)0;
-        den += act ? w : (S)0;
-      }
-  return den > (S)0 ? num / den : (S)0;
-}
-
-#define FOR_LEVEL_CELLS(L)                                                           \
-  {                                                                                  \
-    const RowRange rr_ = block_rows((L).ny * (L).nz);                                \
-    for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
-      const int k = row / (L).ny, j = row - k * (L).ny;                              \
-      for (int i = threadIdx.x; i < (L).nx; i += kLanesX) {                          \
-        const int c = i + (L).nx * row;
 
 }  // namespace
 
@@ -524,8 +409,11 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
   const dim3 grid = cells_grid(g);
   {
     prof::Scope ps(prof::kOuterResidual, s, it);
-    k_mg_outer_residual<S><<<grid, block_shape(), 0, s>>>(g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
-                                                          h.omega, ctl, pdd, pmx);
+    if (sizeof(S) == 4 && h.quad && mg_quad_outer_ok(g, h.lv[0], p, b))
+      launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], (float*)h.x[0], ctl, pdd, pmx, s);
+    else
+      k_mg_outer_residual<S><<<grid, block_shape(), 0, s>>>(g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
+                                                            h.omega, ctl, pdd, pmx);
   }
   launch_finish<1, 2>(pdd, pmx, grid.x, FinMgTrue{ctl, it}, s);
 }
@@ -538,8 +426,11 @@ template <typename S>
 static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p,
                    const PcgControl* ctl, int it, hipStream_t s) {
   prof::Scope ps(mg_slot(l, p ? prof::kMgSmoothUpdate : prof::kMgSmooth), s, l == 0 ? it : -2);
-  k_mg_smooth<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], xin, (const S*)h.b[l], xout,
-                                                               h.omega, p, ctl);
+  if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]) && ((uintptr_t)p & 15u) == 0)
+    launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, ctl, s);
+  else
+    k_mg_smooth<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], xin, (const S*)h.b[l], xout,
+                                                                 h.omega, p, ctl);
 }
 
 // V-cycle from level l; on entry x[l] holds the first pre-smoothing sweep.
@@ -580,14 +471,22 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   }
   {
     prof::Scope ps(mg_slot(l, prof::kMgRestrict), s, l == 0 ? it : -2);
-    k_mg_restrict<S><<<level_grid(h.lv[l + 1]), block_shape(), 0, s>>>(
-        h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.omega, ctl);
+    if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]))
+      launch_mg_restrict_q(h.lv[l], h.lv[l + 1], (const float*)cur, (const float*)b, (float*)h.b[l + 1],
+                           (float*)h.x[l + 1], ctl, s);
+    else
+      k_mg_restrict<S><<<level_grid(h.lv[l + 1]), block_shape(), 0, s>>>(
+          h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.omega, ctl);
   }
   vcycle<S>(h, l + 1, p, ctl, it, s);
   {
     prof::Scope ps(mg_slot(l, prof::kMgProlong), s, l == 0 ? it : -2);
-    k_mg_prolong<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], h.lv[l + 1],
-                                                                   (const S*)h.x[l + 1], cur, nxt, ctl);
+    if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]))
+      launch_mg_prolong_q(h.lv[l], h.lv[l + 1], (const float*)h.x[l + 1], (const float*)cur, (float*)nxt,
+                          ctl, s);
+    else
+      k_mg_prolong<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], h.lv[l + 1],
+                                                                     (const S*)h.x[l + 1], cur, nxt, ctl);
   }
   {
     S* t = cur;
@@ -608,7 +507,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   if (l > 0 && cur != x) {
     // keep the level result in x[l] for the parent's prolongation
     // (only reached when the sweep count leaves it in y)
-    hipMemcpyAsync(x, cur, sizeof(S) * (size_t)h.lv[l].nx * h.lv[l].ny * h.lv[l].nz,
+    (void)hipMemcpyAsync(x, cur, sizeof(S) * (size_t)h.lv[l].nx * h.lv[l].ny * h.lv[l].nz,
                    hipMemcpyDeviceToDevice, s);
   }
 }

@@ -1,0 +1,322 @@
+// Multigrid kernels with four consecutive x-cells per lane (levels whose nx
+// is a multiple of 4 and whose operator is the box or the level-0 obstacle
+// mask; FP32 V-cycle storage).  Each lane loads its cells and their y/z
+// neighbours as 16-byte vectors and the two x-edge neighbours as scalars,
+// so a stencil over four cells costs 7 vector loads instead of 28 scalar
+// ones.  Arithmetic and operation order per cell are those of the scalar
+// kernels (s_apply, apply_at, prolong_value), so both give identical values.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mg.hpp"
+#include "mg_stencil.hpp"
+#include "profiler.hpp"
+#include "stencil.hpp"
+
+namespace rh {
+
+namespace {
+
+// Chunk loop over this block's rows: chunk t covers cells 4q..4q+3 of one
+// row; consecutive threads take consecutive chunks (coalesced).
+#define FOR_QUADS(NX, NY)                                                   \
+  {                                                                         \
+    const int cpr_ = (NX) >> 2;                                             \
+    const RowRange rr_ = block_rows((NY) * nz_);                            \
+    const int t_end_ = rr_.end * cpr_;                                      \
+    for (int t_ = rr_.begin * cpr_ + flat_tid(); t_ < t_end_; t_ += kBlock) { \
+      const int row = t_ / cpr_, q = t_ - row * cpr_;                       \
+      const int k = row / (NY), j = row - k * (NY);                         \
+      const int i0 = q << 2;                                                \
+      const int c = i0 + (NX) * row;
+#define END_QUADS \
+  }               \
+  }
+
+__device__ __forceinline__ void load4(const float* __restrict__ p, int c, float v[4]) {
+  const float4 t = *reinterpret_cast<const float4*>(p + c);
+  v[0] = t.x;
+  v[1] = t.y;
+  v[2] = t.z;
+  v[3] = t.w;
+}
+__device__ __forceinline__ void load4(const double* __restrict__ p, int c, double v[4]) {
+  const double2 a = *reinterpret_cast<const double2*>(p + c);
+  const double2 b = *reinterpret_cast<const double2*>(p + c + 2);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+__device__ __forceinline__ void store4(float* __restrict__ p, int c, const float v[4]) {
+  *reinterpret_cast<float4*>(p + c) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void store4(double* __restrict__ p, int c, const double v[4]) {
+  *reinterpret_cast<double2*>(p + c) = make_double2(v[0], v[1]);
+  *reinterpret_cast<double2*>(p + c + 2) = make_double2(v[2], v[3]);
+}
+
+// Four cells and their six neighbours; a missing wall neighbour is the cell
+// itself (box clamping), a closed obstacle face is resolved by `close`.
+template <typename T>
+struct Quad {
+  T c[4], xm[4], xp[4], ym[4], yp[4], zm[4], zp[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_quad(const T* __restrict__ x, int nx, int ny, int nz, int i0,
+                                          int j, int k, int c, Quad<T>& Q) {
+  const int sy = nx, sz = nx * ny;
+  load4(x, c, Q.c);
+  load4(x, j > 0 ? c - sy : c, Q.ym);
+  load4(x, j + 1 < ny ? c + sy : c, Q.yp);
+  load4(x, k > 0 ? c - sz : c, Q.zm);
+  load4(x, k + 1 < nz ? c + sz : c, Q.zp);
+  const T left = x[i0 > 0 ? c - 1 : c];
+  const T right = x[i0 + 4 < nx ? c + 4 : c + 3];
+  Q.xm[0] = left;
+  Q.xm[1] = Q.c[0];
+  Q.xm[2] = Q.c[1];
+  Q.xm[3] = Q.c[2];
+  Q.xp[0] = Q.c[1];
+  Q.xp[1] = Q.c[2];
+  Q.xp[2] = Q.c[3];
+  Q.xp[3] = right;
+}
+
+// Open-face bits of cell i0+u: obstacle mask byte or the box rule.
+__device__ __forceinline__ int quad_mask(const unsigned char* mask, unsigned m4, int nx, int ny, int nz,
+                                         int i, int j, int k, int u) {
+  return mask ? (int)((m4 >> (8 * u)) & 63u) : box_mask6(nx, ny, nz, i, j, k);
+}
+
+// Closed faces take the cell's own value (contribution exactly 0).
+template <typename T>
+__device__ __forceinline__ void close_faces(Quad<T>& Q, int u, int m) {
+  const T xc = Q.c[u];
+  Q.xm[u] = (m & 1) ? Q.xm[u] : xc;
+  Q.xp[u] = (m & 2) ? Q.xp[u] : xc;
+  Q.ym[u] = (m & 4) ? Q.ym[u] : xc;
+  Q.yp[u] = (m & 8) ? Q.yp[u] : xc;
+  Q.zm[u] = (m & 16) ? Q.zm[u] : xc;
+  Q.zp[u] = (m & 32) ? Q.zp[u] : xc;
+}
+
+// s_apply in S = float for cell u of the quad (same expression).
+__device__ __forceinline__ float quad_apply_f(const Quad<float>& Q, int u, float w0, float w1, float w2) {
+  const float xc = Q.c[u];
+  return w0 * ((xc - Q.xm[u]) + (xc - Q.xp[u])) + w1 * ((xc - Q.ym[u]) + (xc - Q.yp[u])) +
+         w2 * ((xc - Q.zm[u]) + (xc - Q.zp[u]));
+}
+
+// apply_at in f64 for cell u (the reference operator's order).
+__device__ __forceinline__ double quad_apply_d(const Quad<double>& Q, int u, const DevGrid& g) {
+  const double xc = Q.c[u];
+  double v = 0.0;
+  v += g.w[0] * (xc - Q.xm[u]);
+  v += g.w[0] * (xc - Q.xp[u]);
+  v += g.w[1] * (xc - Q.ym[u]);
+  v += g.w[1] * (xc - Q.yp[u]);
+  v += g.w[2] * (xc - Q.zm[u]);
+  v += g.w[2] * (xc - Q.zp[u]);
+  return v;
+}
+
+}  // namespace
+
+// Outer FP64 residual r = b - K p with the first pre-smoothing sweep fused
+// (as k_mg_outer_residual).
+__global__ void __launch_bounds__(kBlock)
+k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
+                      float* __restrict__ b0, float* __restrict__ x0, const PcgControl* ctl, DD* pdd,
+                      double* pmx) {
+  if (ctl->done) return;
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  const int nz_ = g.nz;
+  FOR_QUADS(g.nx, g.ny)
+  Quad<double> Q;
+  load_quad(p, g.nx, g.ny, g.nz, i0, j, k, c, Q);
+  double bb[4];
+  load4(b, c, bb);
+  const unsigned m4 = g.mask ? *reinterpret_cast<const unsigned*>(g.mask + c) : 0u;
+  float rb[4], rx[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int mk = quad_mask(g.mask, m4, g.nx, g.ny, g.nz, i0 + u, j, k, u);
+    if (g.mask) close_faces(Q, u, mk);
+    const double v = quad_apply_d(Q, u, g);
+    if (!isfinite(v)) m[1] = 1.0;
+    const double r = bb[u] - v;
+    dd_acc(s[0], r * r);
+    m[0] = fmax(m[0], fabs(r));
+    rb[u] = (float)r;
+    rx[u] = (float)L0.odiag[mk] * rb[u];
+  }
+  store4(b0, c, rb);
+  store4(x0, c, rx);
+  END_QUADS
+  block_publish<1, 2>(s, m, pdd, pmx);
+}
+
+// One weighted-Jacobi sweep (as k_mg_smooth); with `p` non-null p += x_out.
+__global__ void __launch_bounds__(kBlock)
+k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
+              float* __restrict__ xout, double* __restrict__ p, const PcgControl* ctl) {
+  if (ctl->done) return;
+  const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
+  const int nz_ = L.nz;
+  FOR_QUADS(L.nx, L.ny)
+  Quad<float> Q;
+  load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);
+  float bb[4];
+  load4(b, c, bb);
+  const unsigned m4 = L.mask ? *reinterpret_cast<const unsigned*>(L.mask + c) : 0u;
+  float xn[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int mk = quad_mask(L.mask, m4, L.nx, L.ny, L.nz, i0 + u, j, k, u);
+    if (L.mask) close_faces(Q, u, mk);
+    const float ax = quad_apply_f(Q, u, w0, w1, w2);
+    xn[u] = Q.c[u] + (float)L.odiag[mk] * (bb[u] - ax);
+  }
+  if (p) {
+    double pv[4];
+    load4(p, c, pv);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) pv[u] = pv[u] + (double)xn[u];
+    store4(p, c, pv);
+  } else {
+    store4(xout, c, xn);
+  }
+  END_QUADS
+}
+
+// Restriction (as k_mg_restrict): each lane owns two coarse cells
+// (2 I0, 2 I0 + 1) of coarse row (J, K), i.e. one four-cell chunk in each of
+// the four fine rows (2J + dy, 2K + dz).  Sums keep the (dz, dy, dx) order.
+__global__ void __launch_bounds__(kBlock)
+k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float* __restrict__ b,
+                float* __restrict__ bc, float* __restrict__ xc, const PcgControl* ctl) {
+  if (ctl->done) return;
+  const float w0 = (float)F.w[0], w1 = (float)F.w[1], w2 = (float)F.w[2];
+  const int nz_ = Cl.nz;
+  // coarse pairs: Cl.nx = F.nx / 2 is even, two coarse cells per lane
+  FOR_QUADS(2 * Cl.nx, Cl.ny)
+  (void)c;
+  const int J = j, K = k, I0 = i0 >> 1;
+  float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy) {
+      const int fj = 2 * J + dy, fk = 2 * K + dz;
+      const int fc = i0 + F.nx * (fj + F.ny * fk);
+      Quad<float> Q;
+      load_quad(x, F.nx, F.ny, F.nz, i0, fj, fk, fc, Q);
+      float bb[4];
+      load4(b, fc, bb);
+      const unsigned m4 = F.mask ? *reinterpret_cast<const unsigned*>(F.mask + fc) : 0u;
+      float r[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (F.mask) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
+        r[u] = bb[u] - quad_apply_f(Q, u, w0, w1, w2);
+      }
+      s0 += r[0];
+      s0 += r[1];
+      s1 += r[2];
+      s1 += r[3];
+    }
+  const int cc = I0 + Cl.nx * (J + Cl.ny * K);
+  *reinterpret_cast<float2*>(bc + cc) = make_float2(s0, s1);
+  const float o0 = (float)lv_odiag(Cl, cc, I0, J, K), o1 = (float)lv_odiag(Cl, cc + 1, I0 + 1, J, K);
+  *reinterpret_cast<float2*>(xc + cc) = make_float2(o0 * s0, o1 * s1);
+  END_QUADS
+}
+
+// Prolongation (as k_mg_prolong) for four fine cells: parents I0 = i0/2 and
+// I0 + 1 and their x-neighbours I0 - 1 and I0 + 2 (mirrored at walls).
+__global__ void __launch_bounds__(kBlock)
+k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float* __restrict__ xin,
+               float* __restrict__ xout, const PcgControl* ctl) {
+  if (ctl->done) return;
+  const int nz_ = F.nz;
+  const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
+  const float a = 0.75f, bq = 0.25f;
+  FOR_QUADS(F.nx, F.ny)
+  const int I0 = i0 >> 1, J = j >> 1, K = k >> 1;
+  int Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
+  if (Jn < 0 || Jn >= Cl.ny) Jn = J;
+  if (Kn < 0 || Kn >= Cl.nz) Kn = K;
+  const int Im = I0 > 0 ? I0 - 1 : I0;              // x-neighbour of the even child
+  const int Ip = I0 + 2 < Cl.nx ? I0 + 2 : I0 + 1;  // x-neighbour of the last odd child
+  float xi[4];
+  load4(xin, c, xi);
+  float out[4];
+  bool fast = true;
+  if (Cl.frac) {
+    const unsigned q0 = Cl.frac[I0 + sy * J + sz * K], q1 = Cl.frac[I0 + 1 + sy * J + sz * K];
+    fast = (q0 & q1 & kFracInterior) != 0;
+  }
+  if (fast) {
+    // coarse rows (J,K), (Jn,K), (J,Kn), (Jn,Kn): cells Im, I0, I0+1, Ip
+    float v[4][4];
+    const int rows[4] = {sy * J + sz * K, sy * Jn + sz * K, sy * J + sz * Kn, sy * Jn + sz * Kn};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float2 mid = *reinterpret_cast<const float2*>(xc + I0 + rows[r]);
+      v[r][0] = xc[Im + rows[r]];
+      v[r][1] = mid.x;
+      v[r][2] = mid.y;
+      v[r][3] = xc[Ip + rows[r]];
+    }
+    // fine cell u: parent index P = 1 + (u >> 1), neighbour N = u even ? P-1 : P+1
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int P = 1 + (u >> 1), N = (u & 1) ? P + 1 : P - 1;
+      const float v00 = a * v[0][P] + bq * v[0][N];
+      const float v10 = a * v[1][P] + bq * v[1][N];
+      const float v01 = a * v[2][P] + bq * v[2][N];
+      const float v11 = a * v[3][P] + bq * v[3][N];
+      out[u] = xi[u] + (a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11));
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) out[u] = xi[u] + prolong_value<float>(Cl, xc, i0 + u, j, k);
+  }
+  store4(xout, c, out);
+  END_QUADS
+}
+
+// ------------------------------------------------------------- launchers
+
+bool mg_quad_level(const MgLevel& L) { return L.nx % 4 == 0 && !L.frac; }
+
+static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, const double* b) {
+  return mg_quad_level(L0) && g.nx == L0.nx && aligned16(p) && aligned16(b);
+}
+
+void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
+                                float* b0, float* x0, const PcgControl* ctl, DD* pdd, double* pmx,
+                                hipStream_t s) {
+  k_mg_outer_residual_q<<<cells_grid(g), block_shape(), 0, s>>>(g, L0, p, b, b0, x0, ctl, pdd, pmx);
+}
+
+void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
+                        const PcgControl* ctl, hipStream_t s) {
+  k_mg_smooth_q<<<dim3(grid_blocks(L.ny * L.nz)), block_shape(), 0, s>>>(L, xin, b, xout, p, ctl);
+}
+
+void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
+                          float* xc, const PcgControl* ctl, hipStream_t s) {
+  k_mg_restrict_q<<<dim3(grid_blocks(Cl.ny * Cl.nz)), block_shape(), 0, s>>>(F, Cl, x, b, bc, xc, ctl);
+}
+
+void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,
+                         float* xout, const PcgControl* ctl, hipStream_t s) {
+  k_mg_prolong_q<<<dim3(grid_blocks(F.ny * F.nz)), block_shape(), 0, s>>>(F, Cl, xc, xin, xout, ctl);
+}
+
+}  // namespace rh

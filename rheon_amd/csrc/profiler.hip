@@ -21,6 +21,7 @@ struct Pair {
 struct State {
   std::mutex mu;
   bool on = false;
+  int focus = -1;            // slot timed alone, or -1 = every slot
   std::vector<Pair> pool;    // reusable event pairs
   std::vector<int> open;     // pool index of the open pair per slot
   size_t used = 0;
@@ -42,6 +43,11 @@ const char* kNames[kNumSlots] = {
 }  // namespace
 
 bool enabled() { return st().on; }
+
+bool enabled_for(int slot) {
+  const State& S = st();
+  return S.on && (S.focus < 0 || S.focus == slot);
+}
 
 const char* name(int slot) { return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?"; }
 
@@ -94,6 +100,21 @@ void enable(bool on) {
   std::memset(S.seconds, 0, sizeof(S.seconds));
 }
 
+bool set_focus(const char* kernel) {
+  State& S = st();
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (!kernel) {
+    S.focus = -1;
+    return true;
+  }
+  for (int i = 0; i < kNumSlots; ++i)
+    if (std::strcmp(kernel, kNames[i]) == 0) {
+      S.focus = i;
+      return true;
+    }
+  return false;
+}
+
 bool query(const char* kernel, int64_t* launches, double* seconds) {
   State& S = st();
   std::lock_guard<std::mutex> lk(S.mu);
@@ -117,4 +138,8 @@ extern "C" rh_status rh_profile_enable(int32_t on) {
 extern "C" rh_status rh_profile_query(const char* kernel, int64_t* launches, double* seconds) {
   if (!kernel || !launches || !seconds) return RH_INVALID_ARGUMENT;
   return rh::prof::query(kernel, launches, seconds) ? RH_OK : RH_INVALID_ARGUMENT;
+}
+
+extern "C" rh_status rh_profile_focus(const char* kernel) {
+  return rh::prof::set_focus(kernel) ? RH_OK : RH_INVALID_ARGUMENT;
 }

@@ -1,5 +1,7 @@
 // Live per-kernel timing with HIP events recorded on the launching stream.
-// Disabled by default (one branch per launch).  Entries carry the solver
+// Disabled by default (one branch per launch).  Every event pair costs the
+// stream a few microseconds, so a timed run can focus on one slot: only
+// that slot's launches are bracketed.  Entries carry the solver
 // iteration they belong to, so launches enqueued past convergence (which
 // return at once) can be discarded before they are counted.
 #pragma once
@@ -33,6 +35,8 @@ enum Slot {
 };
 
 bool enabled();
+// Timing on for this slot (profiling on and no focus, or focus == slot).
+bool enabled_for(int slot);
 // Record a start event for `slot` on stream s; `tag` = solver iteration or -1.
 void start(int slot, hipStream_t s, int tag);
 void stop(int slot, hipStream_t s);
@@ -45,7 +49,7 @@ struct Scope {
   int slot;
   hipStream_t s;
   bool on;
-  Scope(int sl, hipStream_t st, int tag) : slot(sl), s(st), on(enabled()) {
+  Scope(int sl, hipStream_t st, int tag) : slot(sl), s(st), on(enabled_for(sl)) {
     if (on) start(sl, st, tag);
   }
   ~Scope() {

@@ -241,6 +241,7 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
   h_.pre = config.mg_pre_sweeps > 0 ? config.mg_pre_sweeps : 2;
   h_.post = config.mg_post_sweeps > 0 ? config.mg_post_sweeps : 2;
   h_.omega = config.mg_omega > 0.0 ? config.mg_omega : 6.0 / 7.0;
+  h_.quad = (config.kernel_variant & RH_VARIANT_SCALAR_MG) == 0;
   // Coarsen while every axis is even and the level is above the one-
   // workgroup coarsest size (or up to the requested level count).
   int n[3] = {(int)grid.counts[0], (int)grid.counts[1], (int)grid.counts[2]};
@@ -473,6 +474,20 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
       }
       q[c] = word;
     }
+    for (int K = 0; K < C.nz; ++K)
+      for (int J = 0; J < C.ny; ++J)
+        for (int I = 0; I < C.nx; ++I) {
+          bool all = true;
+          for (int dz = -1; dz <= 1 && all; ++dz)
+            for (int dy = -1; dy <= 1 && all; ++dy)
+              for (int dx = -1; dx <= 1 && all; ++dx) {
+                const int a = std::min(std::max(I + dx, 0), C.nx - 1);
+                const int b = std::min(std::max(J + dy, 0), C.ny - 1);
+                const int e = std::min(std::max(K + dz, 0), C.nz - 1);
+                all = cf[a + (size_t)C.nx * (b + (size_t)C.ny * e)] != 0;
+              }
+          if (all) q[I + (size_t)C.nx * (J + (size_t)C.ny * K)] |= kFracInterior;
+        }
     RH_HIP(hipMemcpy(d_frac_ + offset, q.data(), sizeof(unsigned) * nc, hipMemcpyHostToDevice));
     h_.lv[l].frac = d_frac_ + offset;
     offset += (nc + 63) / 64 * 64;

@@ -64,13 +64,25 @@ __device__ __forceinline__ Nbr nbr_offsets(int nx, int ny, int nz, int i, int j,
 }
 
 // apply_full (src/operator.rs:400-416): axis 0..2, lower then upper.
+// The six neighbours are loaded from box-clamped addresses, independent of
+// the obstacle mask (one round of loads in flight); a closed obstacle face
+// then selects x_c itself, so its term is the exact +0.0 of the box wall.
 template <typename T>
 __device__ __forceinline__ double apply_at(const DevGrid& g, const T* __restrict__ x, int c, int i,
                                            int j, int k) {
-  const Nbr n = nbr_from_mask(cell_mask6(g, c, i, j, k), g.nx, g.nx * g.ny);
+  const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
   const double xc = ld(x, c);
-  const double a = ld(x, c + n.xm), b = ld(x, c + n.xp), cy = ld(x, c + n.ym),
-               d = ld(x, c + n.yp), e = ld(x, c + n.zm), f = ld(x, c + n.zp);
+  double a = ld(x, c + n.xm), b = ld(x, c + n.xp), cy = ld(x, c + n.ym), d = ld(x, c + n.yp),
+         e = ld(x, c + n.zm), f = ld(x, c + n.zp);
+  if (g.mask) {
+    const int m = g.mask[c];
+    a = (m & 1) ? a : xc;
+    b = (m & 2) ? b : xc;
+    cy = (m & 4) ? cy : xc;
+    d = (m & 8) ? d : xc;
+    e = (m & 16) ? e : xc;
+    f = (m & 32) ? f : xc;
+  }
   double v = 0.0;
   v += g.w[0] * (xc - a);
   v += g.w[0] * (xc - b);
@@ -93,16 +105,17 @@ __device__ __forceinline__ double diag_at(const DevGrid& g, int c, int i, int j,
 template <typename T>
 __device__ __forceinline__ double nbr_sum(const DevGrid& g, const T* __restrict__ x, double init,
                                           int c, int i, int j, int k) {
-  const Nbr n = nbr_from_mask(cell_mask6(g, c, i, j, k), g.nx, g.nx * g.ny);
-  // unconditional loads (clamped addresses), then select
+  const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
+  // unconditional loads (box-clamped addresses), then select
   const double la = ld(x, c + n.xm), lb = ld(x, c + n.xp), lc = ld(x, c + n.ym),
                ldd = ld(x, c + n.yp), le = ld(x, c + n.zm), lf = ld(x, c + n.zp);
-  const double a = n.xm ? la : 0.0;
-  const double b = n.xp ? lb : 0.0;
-  const double cy = n.ym ? lc : 0.0;
-  const double d = n.yp ? ldd : 0.0;
-  const double e = n.zm ? le : 0.0;
-  const double f = n.zp ? lf : 0.0;
+  const int m = cell_mask6(g, c, i, j, k);
+  const double a = (m & 1) ? la : 0.0;
+  const double b = (m & 2) ? lb : 0.0;
+  const double cy = (m & 4) ? lc : 0.0;
+  const double d = (m & 8) ? ldd : 0.0;
+  const double e = (m & 16) ? le : 0.0;
+  const double f = (m & 32) ? lf : 0.0;
   double v = init;
   v += g.w[0] * a;
   v += g.w[0] * b;

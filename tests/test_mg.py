@@ -7,7 +7,7 @@ import pytest
 import oracle as O
 import oracle.mg as M
 import rheon_amd as R
-from rheon_amd.scenes import GridSpec, fourier_modes, taylor_green_noise
+from rheon_amd.scenes import GridSpec, courant_dt, fourier_modes, taylor_green_noise
 
 from helpers import config1, ogrid, rel_err
 
@@ -120,3 +120,31 @@ def test_device_mg_fourier_projection_divergence():
     out = R.correct_velocity(spec, 1.0, ws.get_pressure(), dt, vel)
     # f32 face rounding floor ~ |u| eps32 / h
     assert R.divergence_max(spec, out) < 64 * 64 * 6e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("counts,sphere", [((64, 32, 48), False), ((64, 32, 48), True), ((40, 24, 16), False),
+                                           ((40, 24, 16), True)])
+def test_device_mg_quad_kernels_match_scalar(counts, sphere):
+    """The four-cells-per-lane FP32 kernels give the scalar kernels' values
+    bit for bit (same per-cell arithmetic)."""
+    import rheon_amd as R
+    from rheon_amd.scenes import centred_sphere, close_obstacle_faces
+    spec = GridSpec((counts), tuple(1.0 / n for n in counts))
+    vel = fourier_modes(spec, 9, modes=8, kmax=6, dtype=np.float32)
+    solid = centred_sphere(spec, 0.25) if sphere else None
+    if solid is not None:
+        vel = close_obstacle_faces(spec, solid, vel)
+    dt = courant_dt(spec, vel)
+    rhs = O.build_rhs(ogrid(spec), vel, dt)
+    out = []
+    for variant in (0, R.VARIANT_SCALAR_MG):
+        ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32, kernel_variant=variant)
+        if solid is not None:
+            ws.set_solid(solid)
+        rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-10, 1e-14, 1e-3, 60))
+        assert rep.ok, R.STATUS_NAMES.get(rep.status)
+        out.append((rep.iterations, ws.get_pressure()))
+        ws.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1].view(np.uint64), out[1][1].view(np.uint64))
