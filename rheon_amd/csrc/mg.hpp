@@ -14,8 +14,7 @@ constexpr int kMgCoarsestMax = 4096;  // cells solved by one workgroup in LDS
 // operator (bits 0..5 = faces x-, x+, y-, y+, z-, z+ open, bit 6 = fluid).
 // Levels >= 1 read `frac`: per cell the open fraction of its three lower
 // faces (10 bits each: x in bits 0..9, y in 10..19, z in 20..29; fraction =
-// value * fscale), bit 30 = the cell holds fluid and bit 31 = the cell and
-// all 26 neighbours hold fluid.  A coarse face's open
+// value * fscale) and bit 30 = the cell holds fluid.  A coarse face's open
 // fraction is the mean of its four sub-faces' (the coarse weight is half the
 // sum of the sub-face weights, the area-weighted rediscretisation), so the
 // coarse operator keeps a constant null space per fluid region.  Upper faces
@@ -27,14 +26,15 @@ struct MgLevel {
   double odiag[64];  // omega / diag (weighted-Jacobi scale); 0 for diag 0
   const unsigned char* mask = nullptr;  // level 0 with obstacles
   const unsigned* frac = nullptr;       // levels >= 1 with obstacles
+  // levels >= 1 with obstacles: bit o (o = x | y << 1 | z << 2 parities of a
+  // fine child) is set when the trilinear stencil of that child holds only
+  // active coarse cells, so prolongation needs no renormalisation there
+  const unsigned char* octant = nullptr;
   double wf[3] = {0.0, 0.0, 0.0};       // w[d] * fscale
   double omega = 0.0;                   // smoother weight (for frac levels)
 };
 
 constexpr unsigned kFracFluid = 1u << 30;
-// Bit 31: the cell and its 26 neighbours (clamped to the level) are active,
-// so prolongation around it needs no renormalisation.
-constexpr unsigned kFracInterior = 1u << 31;
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
 constexpr int kMgTailLds = 56 * 1024;  // LDS bytes for the tail levels' x, y, b arrays

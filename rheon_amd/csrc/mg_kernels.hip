@@ -112,7 +112,7 @@ k_mg_prolong(MgLevel F, MgLevel Cl, const S* __restrict__ xc, const S* __restric
              S* __restrict__ xout, const PcgControl* ctl) {
   if (ctl->done) return;
   FOR_LEVEL_CELLS(F)
-  xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k);
+  xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k, lv_fine_fluid(F, c));
   END_CELLS
 }
 
@@ -264,7 +264,7 @@ __device__ __forceinline__ void tail_prolong(const MgLevel& F, const MgLevel& Cl
   const int n = F.nx * F.ny * F.nz;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
     const int i = c % F.nx, j = (c / F.nx) % F.ny, k = c / (F.nx * F.ny);
-    xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k);
+    xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k, lv_fine_fluid(F, c));
   }
   __syncthreads();
 }

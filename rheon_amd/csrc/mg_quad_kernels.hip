@@ -158,7 +158,9 @@ k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const
   block_publish<1, 2>(s, m, pdd, pmx);
 }
 
-// One weighted-Jacobi sweep (as k_mg_smooth); with `p` non-null p += x_out.
+// One weighted-Jacobi sweep (as k_mg_smooth); kUpdate: p += x_out instead of
+// storing x_out (the finest level's last sweep).
+template <bool kUpdate>
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
               float* __restrict__ xout, double* __restrict__ p, const PcgControl* ctl) {
@@ -170,6 +172,8 @@ k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict_
   load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);
   float bb[4];
   load4(b, c, bb);
+  double pv[4];
+  if (kUpdate) load4(p, c, pv);
   const unsigned m4 = L.mask ? *reinterpret_cast<const unsigned*>(L.mask + c) : 0u;
   float xn[4];
 #pragma unroll
@@ -179,9 +183,7 @@ k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict_
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
     xn[u] = Q.c[u] + (float)L.odiag[mk] * (bb[u] - ax);
   }
-  if (p) {
-    double pv[4];
-    load4(p, c, pv);
+  if (kUpdate) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) pv[u] = pv[u] + (double)xn[u];
     store4(p, c, pv);
@@ -250,39 +252,78 @@ k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float*
   if (Kn < 0 || Kn >= Cl.nz) Kn = K;
   const int Im = I0 > 0 ? I0 - 1 : I0;              // x-neighbour of the even child
   const int Ip = I0 + 2 < Cl.nx ? I0 + 2 : I0 + 1;  // x-neighbour of the last odd child
+  // every load is issued up front (all in bounds); the renormalised
+  // obstacle form replaces the result only next to inactive coarse cells
   float xi[4];
   load4(xin, c, xi);
-  float out[4];
-  bool fast = true;
-  if (Cl.frac) {
-    const unsigned q0 = Cl.frac[I0 + sy * J + sz * K], q1 = Cl.frac[I0 + 1 + sy * J + sz * K];
-    fast = (q0 & q1 & kFracInterior) != 0;
-  }
-  if (fast) {
-    // coarse rows (J,K), (Jn,K), (J,Kn), (Jn,Kn): cells Im, I0, I0+1, Ip
-    float v[4][4];
-    const int rows[4] = {sy * J + sz * K, sy * Jn + sz * K, sy * J + sz * Kn, sy * Jn + sz * Kn};
+  float v[4][4];
+  const int rows[4] = {sy * J + sz * K, sy * Jn + sz * K, sy * J + sz * Kn, sy * Jn + sz * Kn};
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const float2 mid = *reinterpret_cast<const float2*>(xc + I0 + rows[r]);
-      v[r][0] = xc[Im + rows[r]];
-      v[r][1] = mid.x;
-      v[r][2] = mid.y;
-      v[r][3] = xc[Ip + rows[r]];
-    }
-    // fine cell u: parent index P = 1 + (u >> 1), neighbour N = u even ? P-1 : P+1
+  for (int r = 0; r < 4; ++r) {
+    const float2 mid = *reinterpret_cast<const float2*>(xc + I0 + rows[r]);
+    v[r][0] = xc[Im + rows[r]];
+    v[r][1] = mid.x;
+    v[r][2] = mid.y;
+    v[r][3] = xc[Ip + rows[r]];
+  }
+  // cells needing the renormalised obstacle form: fluid fine cells whose
+  // octant stencil reaches an inactive coarse cell
+  unsigned slow = 0;
+  if (Cl.frac) {
+    const unsigned oc = *reinterpret_cast<const unsigned short*>(Cl.octant + I0 + rows[0]);
+    const unsigned m4 = F.mask ? *reinterpret_cast<const unsigned*>(F.mask + c) : 0x40404040u;
+    const int oyz = ((j & 1) << 1) | ((k & 1) << 2);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int P = 1 + (u >> 1), N = (u & 1) ? P + 1 : P - 1;
-      const float v00 = a * v[0][P] + bq * v[0][N];
-      const float v10 = a * v[1][P] + bq * v[1][N];
-      const float v01 = a * v[2][P] + bq * v[2][N];
-      const float v11 = a * v[3][P] + bq * v[3][N];
-      out[u] = xi[u] + (a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11));
+      const unsigned bits = (oc >> (8 * (u >> 1))) & 0xffu;
+      const bool fluid = (m4 >> (8 * u)) & kFluidBit;
+      if (fluid && !((bits >> ((u & 1) | oyz)) & 1u)) slow |= 1u << u;
     }
-  } else {
+  }
+  float out[4];
+  // fine cell u: parent index P = 1 + (u >> 1), neighbour N = u even ? P-1 : P+1
 #pragma unroll
-    for (int u = 0; u < 4; ++u) out[u] = xi[u] + prolong_value<float>(Cl, xc, i0 + u, j, k);
+  for (int u = 0; u < 4; ++u) {
+    const int P = 1 + (u >> 1), N = (u & 1) ? P + 1 : P - 1;
+    const float v00 = a * v[0][P] + bq * v[0][N];
+    const float v10 = a * v[1][P] + bq * v[1][N];
+    const float v01 = a * v[2][P] + bq * v[2][N];
+    const float v11 = a * v[3][P] + bq * v[3][N];
+    out[u] = xi[u] + (a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11));
+  }
+  if (slow) {
+    // renormalised form (prolong_value's order: dz, dy, dx) from the values
+    // already loaded; one batch of activity-flag loads
+    bool act[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint2 mid = *reinterpret_cast<const uint2*>(Cl.frac + I0 + rows[r]);
+      act[r][0] = (Cl.frac[Im + rows[r]] & kFracFluid) != 0;
+      act[r][1] = (mid.x & kFracFluid) != 0;
+      act[r][2] = (mid.y & kFracFluid) != 0;
+      act[r][3] = (Cl.frac[Ip + rows[r]] & kFracFluid) != 0;
+    }
+    const float wt[2] = {a, bq};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!((slow >> u) & 1)) continue;
+      const int P = 1 + (u >> 1), N = (u & 1) ? P + 1 : P - 1;
+      const int xs[2] = {P, N};
+      float num = 0.0f, den = 0.0f;
+#pragma unroll
+      for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+          for (int dx = 0; dx < 2; ++dx) {
+            const int r = dy + 2 * dz;
+            const float w = wt[dx] * wt[dy] * wt[dz];
+            const bool on = act[r][xs[dx]];
+            num += on ? w * v[r][xs[dx]] : 0.0f;
+            den += on ? w : 0.0f;
+          }
+      out[u] = xi[u] + (den > 0.0f ? num / den : 0.0f);
+    }
   }
   store4(xout, c, out);
   END_QUADS
@@ -306,7 +347,10 @@ void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const doubl
 
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
                         const PcgControl* ctl, hipStream_t s) {
-  k_mg_smooth_q<<<dim3(grid_blocks(L.ny * L.nz)), block_shape(), 0, s>>>(L, xin, b, xout, p, ctl);
+  if (p)
+    k_mg_smooth_q<true><<<dim3(grid_blocks(L.ny * L.nz)), block_shape(), 0, s>>>(L, xin, b, xout, p, ctl);
+  else
+    k_mg_smooth_q<false><<<dim3(grid_blocks(L.ny * L.nz)), block_shape(), 0, s>>>(L, xin, b, xout, p, ctl);
 }
 
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,

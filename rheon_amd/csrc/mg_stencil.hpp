@@ -88,13 +88,19 @@ __device__ __forceinline__ double lv_odiag(const MgLevel& L, int c, int i, int j
   return L.odiag[lv_mask(L, c, i, j, k)];
 }
 
+// Fine cell c of level F holds fluid (always in the plain box).
+__device__ __forceinline__ bool lv_fine_fluid(const MgLevel& F, int c) { return lv_active(F, c); }
+
 // Trilinear prolongation value of fine cell (i, j, k) from the coarse
 // iterate (cell-centred weights 27/64, 9/64, 3/64, 1/64; a missing
-// neighbour at a wall mirrors the parent).  With obstacles the weights of
-// inactive coarse cells are dropped and the rest renormalised.
+// neighbour at a wall mirrors the parent).  With obstacles, for a fluid
+// fine cell whose stencil reaches an inactive coarse cell, the weights of
+// inactive cells are dropped and the rest renormalised (a solid fine
+// cell's value is never read by a fluid equation, so it keeps the plain
+// form).
 template <typename S>
 __device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restrict__ xc, int i, int j,
-                                           int k) {
+                                           int k, bool fine_fluid) {
   const int I = i >> 1, J = j >> 1, K = k >> 1;
   int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
   if (In < 0 || In >= Cl.nx) In = I;
@@ -102,20 +108,16 @@ __device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restric
   if (Kn < 0 || Kn >= Cl.nz) Kn = K;
   const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
   const S a = (S)0.75, bq = (S)0.25;
-  if (!Cl.frac) {
-    const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
-    const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
-    const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
-    const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
-    return a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11);
-  }
-  if (Cl.frac[I + sy * J + sz * K] & kFracInterior) {  // no inactive cell around the parent
-    const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
-    const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
-    const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
-    const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
-    return a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11);
-  }
+  // box form with every load issued up front; with obstacles it stands
+  // unless an inactive coarse cell is near the parent
+  const S v00 = a * xc[I + sy * J + sz * K] + bq * xc[In + sy * J + sz * K];
+  const S v10 = a * xc[I + sy * Jn + sz * K] + bq * xc[In + sy * Jn + sz * K];
+  const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
+  const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
+  const S box = a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11);
+  if (!Cl.frac || !fine_fluid) return box;
+  const int o = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
+  if ((Cl.octant[I + sy * J + sz * K] >> o) & 1) return box;
   const int ci[2] = {I, In}, cj[2] = {J, Jn}, ck[2] = {K, Kn};
   const S wt[2] = {a, bq};
   S num = (S)0, den = (S)0;

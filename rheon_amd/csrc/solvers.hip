@@ -414,6 +414,7 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
   for (int l = 0; l < h_.levels; ++l) {
     h_.lv[l].mask = nullptr;
     h_.lv[l].frac = nullptr;
+    h_.lv[l].octant = nullptr;
     fscale_[l] = 0.0;
   }
   if (!d_mask_) return RH_OK;
@@ -432,8 +433,10 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
     for (size_t c = 0; c < n; ++c) fluid[c] = (hmask_[c] & kFluidBit) ? 1 : 0;
   }
   size_t total = 0;
-  for (int l = 1; l < h_.levels; ++l)
-    total += ((size_t)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz + 63) / 64 * 64;
+  for (int l = 1; l < h_.levels; ++l) {
+    const size_t nc = (size_t)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz;
+    total += (nc + 63) / 64 * 64 + (nc + 255) / 256 * 64;  // words + octant bytes
+  }
   if (d_frac_) (void)hipFree(d_frac_);
   d_frac_ = nullptr;
   RH_HIP(hipMalloc(&d_frac_, sizeof(unsigned) * total));
@@ -474,23 +477,36 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
       }
       q[c] = word;
     }
+    // octant bits: fine child parities (x, y, z) -> its trilinear stencil
+    // {P, P + s} per axis (s = -1 for even, +1 for odd, mirrored at walls)
+    std::vector<uint8_t> oct(nc, 0);
     for (int K = 0; K < C.nz; ++K)
       for (int J = 0; J < C.ny; ++J)
         for (int I = 0; I < C.nx; ++I) {
-          bool all = true;
-          for (int dz = -1; dz <= 1 && all; ++dz)
-            for (int dy = -1; dy <= 1 && all; ++dy)
-              for (int dx = -1; dx <= 1 && all; ++dx) {
-                const int a = std::min(std::max(I + dx, 0), C.nx - 1);
-                const int b = std::min(std::max(J + dy, 0), C.ny - 1);
-                const int e = std::min(std::max(K + dz, 0), C.nz - 1);
-                all = cf[a + (size_t)C.nx * (b + (size_t)C.ny * e)] != 0;
-              }
-          if (all) q[I + (size_t)C.nx * (J + (size_t)C.ny * K)] |= kFracInterior;
+          uint8_t bits = 0;
+          for (int o = 0; o < 8; ++o) {
+            const int P[3] = {I, J, K}, n[3] = {C.nx, C.ny, C.nz};
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; ++a) {
+              const int nb = P[a] + (((o >> a) & 1) ? 1 : -1);
+              lo[a] = std::min(P[a], nb < 0 || nb >= n[a] ? P[a] : nb);
+              hi[a] = std::max(P[a], nb < 0 || nb >= n[a] ? P[a] : nb);
+            }
+            bool all = true;
+            for (int e = lo[2]; e <= hi[2] && all; ++e)
+              for (int b = lo[1]; b <= hi[1] && all; ++b)
+                for (int a = lo[0]; a <= hi[0] && all; ++a)
+                  all = cf[a + (size_t)C.nx * (b + (size_t)C.ny * e)] != 0;
+            if (all) bits |= (uint8_t)(1u << o);
+          }
+          oct[I + (size_t)C.nx * (J + (size_t)C.ny * K)] = bits;
         }
     RH_HIP(hipMemcpy(d_frac_ + offset, q.data(), sizeof(unsigned) * nc, hipMemcpyHostToDevice));
     h_.lv[l].frac = d_frac_ + offset;
-    offset += (nc + 63) / 64 * 64;
+    unsigned char* d_oct = (unsigned char*)(d_frac_ + offset + (nc + 63) / 64 * 64);
+    RH_HIP(hipMemcpy(d_oct, oct.data(), nc, hipMemcpyHostToDevice));
+    h_.lv[l].octant = d_oct;
+    offset += (nc + 63) / 64 * 64 + (nc + 255) / 256 * 64;
     hfrac_[l].swap(q);
     for (int a = 0; a < 3; ++a) cnt[a].swap(cc[a]);
     fluid.swap(cf);
