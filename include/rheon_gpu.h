@@ -331,6 +331,37 @@ rh_status rh_scene_device_velocity(rh_scene* scene, void** d_u, void** d_v, void
 rh_status rh_scene_step(rh_scene* scene, double dt, rh_step_report* report,
                         rh_cancel_fn cancel, void* cancel_ctx);
 
+/* ------------------------------------------------------ batched scenes */
+/* Many device-resident scenes advanced together: each stage of the step is
+ * one launch over every scene (no counterpart in the reference, which runs
+ * scenes one at a time or as CPU BatchRun jobs, src/scheduling.rs:128-221;
+ * per scene the step is rh_scene_step's, src/simulation.rs:988-1277).  The
+ * solve enqueues max_cycles V-cycles per step and converged scenes' blocks
+ * return at once, so one step is a fixed launch sequence that can be
+ * captured once in a hipGraph and replayed. */
+typedef struct rh_batch rh_batch;
+typedef struct rh_batch_config {
+  rh_scene_config scene;  /* shared: f32 faces, RH_MIXED_MG with f32 V-cycles */
+  int32_t max_cycles;     /* V-cycles per step (0 = scene.pressure.max_iterations) */
+  int32_t use_graph;      /* nonzero: capture one step in a hipGraph, replay it */
+} rh_batch_config;
+
+rh_status rh_batch_create(const rh_grid* grids, size_t n, const rh_batch_config* config,
+                          rh_batch** out);
+void rh_batch_destroy(rh_batch* batch);
+uint64_t rh_batch_allocated_bytes(const rh_batch* batch);
+/* Borrowed handle of scene i (set_velocity / set_solid / get_* as usual). */
+rh_status rh_batch_scene(rh_batch* batch, size_t i, rh_scene** out);
+/* dt per scene (n entries), kept for every following step. */
+rh_status rh_batch_set_dt(rh_batch* batch, const double* dt, size_t n);
+/* Kernel variant (RH_VARIANT_* bits) of the batched launches. */
+rh_status rh_batch_set_variant(rh_batch* batch, uint32_t kernel_variant);
+/* `steps` steps of every scene; reports/statuses (n entries each, may be
+ * NULL) describe the last step.  A scene whose step fails keeps its
+ * accepted faces (transactional, as rh_scene_step) and its status says why. */
+rh_status rh_batch_step(rh_batch* batch, int32_t steps, rh_step_report* reports,
+                        rh_status* statuses);
+
 #ifdef __cplusplus
 }
 #endif

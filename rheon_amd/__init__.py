@@ -535,10 +535,21 @@ class Scene:
                           {"advect": rep.advect_seconds, "rhs": rep.rhs_seconds, "solve": rep.solve_seconds,
                            "correct": rep.correct_seconds})
 
+    @classmethod
+    def _borrowed(cls, handle: int, spec: GridSpec, precision: int) -> "Scene":
+        """A scene owned by a SceneBatch (not destroyed by this object)."""
+        obj = cls.__new__(cls)
+        obj.spec = spec
+        obj.precision = precision
+        obj.grid = make_grid(spec)
+        obj.handle = handle
+        obj._owned = False
+        return obj
+
     def close(self):
-        if self.handle:
+        if self.handle and getattr(self, "_owned", True):
             lib().rh_scene_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -548,3 +559,70 @@ class Scene:
 
 
 __all__ = [n for n in dir() if not n.startswith("_")]
+
+
+class SceneBatch:
+    """Many device-resident scenes advanced together: one launch per stage
+    over every scene, optionally one hipGraph replay per step (f32 faces,
+    mixed-precision multigrid).  Per scene the step is Scene.step's."""
+
+    def __init__(self, specs, density: float = 1.0, settings: PressureSettings | None = None,
+                 actual_divergence_limit: float = 1e-3, max_courant: float = 0.0, max_cycles: int = 0,
+                 use_graph: bool = True, kernel_variant: int = 0):
+        self.specs = list(specs)
+        settings = settings or PressureSettings(1e-8, 1e-14, 1e-3, 40)
+        cfg = N.BatchConfig(
+            N.SceneConfig(density, F32, N.SolverConfig(MIXED_MG, F32, 0, 0, 0, 0.0, 0, kernel_variant, 0),
+                          settings.native(), actual_divergence_limit, max_courant),
+            int(max_cycles), 1 if use_graph else 0)
+        grids = (N.Grid * len(self.specs))(*[make_grid(sp) for sp in self.specs])
+        h = C.c_void_p()
+        _check(lib().rh_batch_create(grids, len(self.specs), C.byref(cfg), C.byref(h)), "rh_batch_create")
+        self.handle = h.value
+        self._scenes = []
+        for i, sp in enumerate(self.specs):
+            sh = C.c_void_p()
+            _check(lib().rh_batch_scene(self.handle, i, C.byref(sh)), "rh_batch_scene")
+            self._scenes.append(Scene._borrowed(sh.value, sp, F32))
+
+    def __len__(self) -> int:
+        return len(self.specs)
+
+    def scene(self, i: int) -> Scene:
+        return self._scenes[i]
+
+    def allocated_bytes(self) -> int:
+        return int(lib().rh_batch_allocated_bytes(self.handle))
+
+    def set_dt(self, dts) -> None:
+        a = np.ascontiguousarray(dts, np.float64)
+        _check(lib().rh_batch_set_dt(self.handle, a.ctypes.data, a.size), "rh_batch_set_dt")
+
+    def set_variant(self, kernel_variant: int) -> None:
+        _check(lib().rh_batch_set_variant(self.handle, int(kernel_variant)), "rh_batch_set_variant")
+
+    def step(self, steps: int = 1):
+        """Run `steps` steps; returns (statuses, StepReports) of the last step."""
+        n = len(self.specs)
+        reps = (N.StepReport * n)()
+        sts = (C.c_int * n)()
+        _check(lib().rh_batch_step(self.handle, int(steps), reps, sts), "rh_batch_step")
+        out = []
+        for i in range(n):
+            r = reps[i]
+            out.append(StepReport(sts[i], r.dt, PressureReport.from_native(sts[i], r.pressure),
+                                  r.actual_divergence_max, r.courant, r.seconds, {}))
+        return out
+
+    def close(self):
+        for sc in self._scenes:
+            sc.handle = None
+        if self.handle:
+            lib().rh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -113,6 +113,14 @@ class SceneConfig(C.Structure):
     ]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("scene", SceneConfig),
+        ("max_cycles", C.c_int32),
+        ("use_graph", C.c_int32),
+    ]
+
+
 class StepReport(C.Structure):
     _fields_ = [
         ("dt", C.c_double),
@@ -183,6 +191,13 @@ SIGNATURES = {
     "rh_scene_get_pressure": (_S, [_P, _P, C.c_size_t]),
     "rh_scene_device_velocity": (_S, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rh_scene_step": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
+    "rh_batch_create": (_S, [C.POINTER(Grid), C.c_size_t, C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
+    "rh_batch_destroy": (None, [_P]),
+    "rh_batch_allocated_bytes": (C.c_uint64, [_P]),
+    "rh_batch_scene": (_S, [_P, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "rh_batch_set_dt": (_S, [_P, _P, C.c_size_t]),
+    "rh_batch_set_variant": (_S, [_P, C.c_uint32]),
+    "rh_batch_step": (_S, [_P, C.c_int32, C.POINTER(StepReport), C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -1,0 +1,860 @@
+// Batched projection steps over many device-resident scenes.
+//
+// Every stage of the step (validation, advection, RHS, the multigrid solve
+// with its defect-correction loop, gauge fix, quality gate, correction,
+// divergence and Courant gates, commit) is one launch over all scenes: the
+// grid is the concatenation of each scene's own block decomposition and each
+// block runs the one-scene kernel body for its scene (see batch.hpp).  All
+// decisions of the one-scene step (src/simulation.rs:988-1277) are taken on
+// the device per scene, so one step is a fixed launch sequence: the solve
+// enqueues `max_cycles` V-cycles and a converged scene's blocks return at
+// once.  The sequence can be captured once in a hipGraph and replayed per
+// step; the host reads the per-scene reports after the step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "batch.hpp"
+#include "common.hpp"
+#include "handles.hpp"
+#include "kernels.hpp"
+#include "mg.hpp"
+#include "runtime.hpp"
+
+namespace rh {
+
+// Per-scene record of the batched step (device memory).
+struct BatchScene {
+  PcgControl tmpl;   // solver control block at the start of a solve
+  PcgControl* ctl;   // the scene solver's control block
+  unsigned* flags;   // validation / arithmetic flags
+  double* scal;      // [0..2] residual report, [3] max|div u|, [4..7] rate
+  const double* p;
+  double volume, actual_limit, max_courant, dt;
+  int gauge;
+  int status;
+  int accepted;
+  int pad;
+  rh_pressure_report rep;
+  double actual_div, courant;
+};
+
+namespace {
+
+__device__ int status_of_flags(unsigned f) {
+  if (f & kFlagNonfiniteInput) return RH_NONFINITE_INPUT;
+  if (f & kFlagNonzeroWall) return RH_NONZERO_WALL_VELOCITY;
+  if (f & kFlagArithmetic) return RH_ARITHMETIC_FAILURE;
+  return RH_OK;
+}
+
+__global__ void kb_begin_step(BatchScene* S, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  *S[s].flags = 0u;
+  S[s].status = RH_OK;
+  S[s].accepted = 0;
+  S[s].actual_div = 0.0;
+  S[s].courant = 0.0;
+  S[s].ctl->spare[2] = 0.0;
+}
+
+// Control block of the solve; a scene whose fields failed validation is
+// marked done with that status, so every solver stage skips it.
+__global__ void kb_begin_solve(BatchScene* S, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PcgControl c = S[s].tmpl;
+  const unsigned f = *S[s].flags;
+  if (f) {
+    c.done = 1;
+    c.status = status_of_flags(f);
+    S[s].status = c.status;
+  }
+  *S[s].ctl = c;
+}
+
+// After the last cycle: the report, and the gauge shift p - p[gauge] for a
+// converged (or iteration-limited) solve (the one-scene MgSolver::run).
+__global__ void kb_end_solve(BatchScene* S, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  PcgControl* c = S[s].ctl;
+  rh_pressure_report& r = S[s].rep;
+  r.iterations = c->iteration;
+  r.true_residual_l2 = c->true_l2;
+  r.true_residual_max = c->true_max;
+  r.predicted_divergence_max = c->pred_div;
+  r.error_iteration = c->error_iteration;
+  r.error_value = c->error_value;
+  r.threshold = c->threshold;
+  r.seconds = 0.0;
+  int st = c->done ? c->status : RH_EXECUTION;
+  if (!c->done) {
+    c->done = 1;
+    c->status = st;
+  }
+  const bool shift = st == RH_OK || st == RH_ITERATION_LIMIT;
+  c->spare[0] = shift ? S[s].p[S[s].gauge] : 0.0;
+  c->spare[1] = shift ? 1.0 : 0.0;
+  if (S[s].status == RH_OK) S[s].status = st;
+}
+
+// Quality gate on the gauge-shifted pressure (MgSolver::run's final check).
+__global__ void kb_quality(BatchScene* S, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  if (S[s].status != RH_OK) return;
+  const double l2 = sqrt(S[s].scal[0]), mx = S[s].scal[1];
+  const double div = mx * S[s].dt / S[s].volume;
+  S[s].rep.true_residual_l2 = l2;
+  S[s].rep.true_residual_max = mx;
+  S[s].rep.predicted_divergence_max = div;
+  if (S[s].scal[2] != 0.0 || !(l2 <= S[s].rep.threshold) || !(div <= S[s].tmpl.div_limit)) {
+    S[s].status = RH_QUALITY_GATE;
+    S[s].ctl->status = RH_QUALITY_GATE;
+  }
+}
+
+// Actual divergence and Courant gates, then the commit decision
+// (src/simulation.rs:1235-1277).
+__global__ void kb_gate(BatchScene* S, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  if (S[s].status != RH_OK) return;
+  const unsigned f = *S[s].flags;
+  if (f) {
+    S[s].status = status_of_flags(f);
+    return;
+  }
+  const double actual = S[s].scal[3];
+  const double courant = S[s].dt * S[s].scal[4];
+  S[s].actual_div = actual;
+  S[s].courant = courant;
+  if (actual > S[s].actual_limit) {
+    S[s].status = RH_DIVERGENCE_LIMIT;
+    return;
+  }
+  if (S[s].max_courant > 0.0 && (!isfinite(courant) || courant > S[s].max_courant)) {
+    S[s].status = RH_COURANT_LIMIT;
+    return;
+  }
+  S[s].accepted = 1;
+  S[s].ctl->spare[2] = 1.0;
+}
+
+// Commit: accepted faces <- candidate faces (blockIdx.y = component).
+__global__ void __launch_bounds__(kBlock)
+kb_commit(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  const int s = batch_scene(prefix, n, blockIdx.x);
+  const BFaceArgs& a = A[s];
+  if (a.ctl->spare[2] == 0.0) return;
+  const VBlock vb = batch_block(prefix, s);
+  const DevGrid& g = *a.g;
+  const int ax = blockIdx.y;
+  const float* src = ax == 0 ? a.u : (ax == 1 ? a.v : a.w);
+  float* dst = ax == 0 ? a.ou : (ax == 1 ? a.ov : a.ow);
+  const int fx = g.nx + (ax == 0), fy = g.ny + (ax == 1), fz = g.nz + (ax == 2);
+  const RowRange rr = block_rows(fy * fz, vb);
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock)
+    for (int i = threadIdx.x; i < fx; i += kLanesX) dst[i + fx * row] = src[i + fx * row];
+}
+
+// Finishers: one block per scene, skipped when the scene has no producer
+// blocks in this stage or its gate says so.
+template <int ND, int NM, typename Fin>
+struct BFin {
+  const DD* pdd;
+  const double* pmx;
+  int nb;
+  int pad;
+  Fin fin;
+};
+template <int ND, int NM, typename Fin>
+__global__ void __launch_bounds__(kBlock) kb_finish(const BFin<ND, NM, Fin>* __restrict__ A) {
+  const BFin<ND, NM, Fin>& a = A[blockIdx.x];
+  if (a.nb == 0) return;
+  finish_body<ND, NM, Fin>(a.pdd, a.pmx, a.nb, a.fin);
+}
+
+// Runs `f` only while the scene's solve is live.
+template <typename F>
+struct FinNotDone {
+  const PcgControl* ctl;
+  F f;
+  __device__ bool active() const { return !ctl->done && f.active(); }
+  __device__ void operator()(const double* s, const double* m) const { f(s, m); }
+};
+// Runs `f` only when the scene's solve (and gates so far) succeeded.
+template <typename F>
+struct FinStatusOk {
+  const PcgControl* ctl;
+  F f;
+  __device__ bool active() const { return ctl->status == RH_OK && f.active(); }
+  __device__ void operator()(const double* s, const double* m) const { f(s, m); }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+
+namespace {
+
+enum class Op {
+  kBeginStep, kValidateAcc, kAdvect, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kFill,
+  kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothS, kRestrictQ, kRestrictS,
+  kProlongQ, kProlongS, kTail, kCoarseDense, kEndSolve, kRegauge, kReport, kFinReport,
+  kQuality, kCorrect, kDivMax, kFinDiv, kRate, kFinRate, kGate, kCommit
+};
+
+struct Launch {
+  Op op;
+  size_t prefix_off = 0, args_off = 0;  // offsets in the table arena
+  int blocks = 0;
+  int extra = 0;  // dynamic LDS bytes (tail)
+};
+
+// Host staging of the device tables: records, prefixes and finisher
+// records, appended with alignment and copied once.
+struct Tables {
+  std::vector<unsigned char> bytes;
+  size_t add(const void* data, size_t size, size_t align = 16) {
+    size_t off = (bytes.size() + align - 1) / align * align;
+    bytes.resize(off + size);
+    std::memcpy(bytes.data() + off, data, size);
+    return off;
+  }
+};
+
+}  // namespace
+
+class Batch {
+ public:
+  ~Batch();
+  rh_status create(const rh_grid* grids, size_t n, const rh_batch_config& cfg);
+  rh_status step(int steps, rh_step_report* reports, rh_status* statuses);
+  rh_scene* scene(size_t i) { return i < scenes_.size() ? scenes_[i].get() : nullptr; }
+  size_t size() const { return scenes_.size(); }
+  uint64_t bytes() const;
+  rh_status set_dt_all(const double* dt, size_t n);
+  const rh_batch_config& config() const { return cfg_; }
+  rh_status set_variant(uint32_t variant);
+
+ private:
+  rh_status build();
+  void run_program(hipStream_t s);
+  void release_graph();
+  MgSolver* mg(size_t i) { return static_cast<MgSolver*>(scenes_[i]->solver->impl.get()); }
+
+  std::vector<std::unique_ptr<rh_scene>> scenes_;
+  std::vector<double> dt_;
+  std::vector<uint64_t> gens_;
+  rh_batch_config cfg_{};
+  uint32_t variant_ = 0;
+  hipStream_t stream_ = nullptr;
+  DevGrid* d_grids_ = nullptr;
+  BatchScene* d_scenes_ = nullptr;
+  std::vector<BatchScene> h_scenes_;
+  unsigned char* d_tables_ = nullptr;
+  size_t tables_bytes_ = 0;
+  std::vector<Launch> program_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  bool built_ = false;
+  int max_cycles_ = 0;
+};
+
+Batch::~Batch() {
+  release_graph();
+  if (d_tables_) (void)hipFree(d_tables_);
+  if (d_grids_) (void)hipFree(d_grids_);
+  if (d_scenes_) (void)hipFree(d_scenes_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Batch::release_graph() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+uint64_t Batch::bytes() const {
+  uint64_t b = tables_bytes_ + scenes_.size() * (sizeof(DevGrid) + sizeof(BatchScene));
+  for (const auto& sc : scenes_) b += sc->bytes;
+  return b;
+}
+
+rh_status Batch::create(const rh_grid* grids, size_t n, const rh_batch_config& cfg) {
+  cfg_ = cfg;
+  const rh_scene_config& sc = cfg.scene;
+  if (n == 0 || !grids) return RH_INVALID_ARGUMENT;
+  if (sc.precision != RH_F32 || sc.solver.method != RH_MIXED_MG || sc.solver.precision != RH_F32) {
+    set_last_error("batched steps run f32 faces with the mixed-precision multigrid solve", "rh_batch_create");
+    return RH_UNSUPPORTED;
+  }
+  variant_ = sc.solver.kernel_variant;
+  for (size_t i = 0; i < n; ++i) {
+    rh_scene* raw = nullptr;
+    rh_status s = rh_scene_create(&grids[i], &sc, &raw);
+    if (s) return s;
+    scenes_.emplace_back(raw);
+    if (mg(i)->levels() < 2) {
+      set_last_error("a batched scene needs at least two multigrid levels", "rh_batch_create");
+      return RH_UNSUPPORTED;
+    }
+  }
+  dt_.assign(n, 0.0);
+  gens_.assign(n, ~0ull);
+  RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  RH_HIP(hipMalloc(&d_grids_, sizeof(DevGrid) * n));
+  RH_HIP(hipMalloc(&d_scenes_, sizeof(BatchScene) * n));
+  h_scenes_.assign(n, BatchScene{});
+  return RH_OK;
+}
+
+rh_status Batch::set_dt_all(const double* dt, size_t n) {
+  if (!dt || n != scenes_.size()) return RH_INVALID_ARGUMENT;
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(dt[i]) || dt[i] <= 0.0) return RH_INVALID_TIME_STEP;
+    if (dt_[i] != dt[i]) built_ = false;
+    dt_[i] = dt[i];
+  }
+  return RH_OK;
+}
+
+rh_status Batch::set_variant(uint32_t variant) {
+  if (variant == variant_) return RH_OK;
+  variant_ = variant;
+  built_ = false;
+  return RH_OK;
+}
+
+// Builds the launch tables and the step program from the scenes' current
+// device state (hierarchies, masks, buffers).
+rh_status Batch::build() {
+  const size_t n = scenes_.size();
+  const rh_scene_config& sc = cfg_.scene;
+  const double density = sc.density;
+  release_graph();
+  // device-side operator data of every scene, then the DevGrid records
+  std::vector<DevGrid> grids(n);
+  for (size_t i = 0; i < n; ++i) {
+    rh_status s = mg(i)->prepare(density);
+    if (s) return s;
+    grids[i] = mg(i)->dev_grid(density);
+  }
+  RH_HIP(hipMemcpy(d_grids_, grids.data(), sizeof(DevGrid) * n, hipMemcpyHostToDevice));
+  const int pre = std::max(1, sc.solver.mg_pre_sweeps > 0 ? sc.solver.mg_pre_sweeps : 2);
+  const int post = std::max(1, sc.solver.mg_post_sweeps > 0 ? sc.solver.mg_post_sweeps : 2);
+  int64_t cycles = sc.pressure.max_iterations;
+  if (cfg_.max_cycles > 0) cycles = std::min<int64_t>(cycles, cfg_.max_cycles);
+  if (cycles < 1) cycles = 1;
+  max_cycles_ = (int)cycles;
+  // per-scene records
+  for (size_t i = 0; i < n; ++i) {
+    rh_scene* s = scenes_[i].get();
+    Scratch& scr = s->solver->impl->scratch();
+    BatchScene& b = h_scenes_[i];
+    b = BatchScene{};
+    PcgControl& c = b.tmpl;
+    std::memset(&c, 0, sizeof(c));
+    c.rel = sc.pressure.relative_residual;
+    c.abs_tol = sc.pressure.absolute_residual;
+    c.div_limit = sc.pressure.divergence_limit;
+    c.dt = dt_[i];
+    c.volume = grids[i].volume;
+    c.max_iterations = max_cycles_;
+    b.ctl = scr.ctl;
+    b.flags = scr.flags;
+    b.scal = scr.scal;
+    b.p = s->p;
+    b.volume = grids[i].volume;
+    b.actual_limit = sc.actual_divergence_limit;
+    b.max_courant = sc.max_courant;
+    b.dt = dt_[i];
+    b.gauge = grids[i].gauge;
+  }
+  RH_HIP(hipMemcpy(d_scenes_, h_scenes_.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice));
+
+  Tables T;
+  program_.clear();
+  const bool quad_ok = (variant_ & RH_VARIANT_SCALAR_MG) == 0;
+  auto face_blocks = [&](size_t i) {
+    return grid_blocks((int)((grids[i].ny + 1) * (grids[i].nz + 1)));
+  };
+  auto cell_blocks = [&](size_t i) { return grid_blocks(grids[i].ny * grids[i].nz); };
+  // a stage: records for all scenes + blocks per scene (0 = not taking part)
+  auto stage = [&](Op op, const void* recs, size_t rec_size, const std::vector<int>& blocks, int extra = 0) {
+    std::vector<int> prefix(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) prefix[i + 1] = prefix[i] + blocks[i];
+    Launch L;
+    L.op = op;
+    L.blocks = prefix[n];
+    L.extra = extra;
+    if (L.blocks == 0) return;
+    L.prefix_off = T.add(prefix.data(), sizeof(int) * (n + 1));
+    L.args_off = T.add(recs, rec_size * n);
+    program_.push_back(L);
+  };
+  auto per_scene = [&](Op op) {
+    Launch L;
+    L.op = op;
+    L.blocks = (int)((n + 255) / 256);
+    program_.push_back(L);
+  };
+  // --- field records
+  std::vector<BFaceArgs> acc(n), adv(n), cand(n), rhs(n), cor(n), div(n), rate(n), commit(n);
+  std::vector<int> fb(n), cb(n), rb(n);
+  for (size_t i = 0; i < n; ++i) {
+    rh_scene* s = scenes_[i].get();
+    Scratch& scr = s->solver->impl->scratch();
+    BFaceArgs base{};
+    base.g = d_grids_ + i;
+    base.dt = &d_scenes_[i].dt;
+    base.flags = scr.flags;
+    base.pdd = scr.pdd;
+    base.pmx = scr.pmx;
+    base.ctl = scr.ctl;
+    BFaceArgs a = base;
+    a.u = (const float*)s->acc[0];
+    a.v = (const float*)s->acc[1];
+    a.w = (const float*)s->acc[2];
+    a.mode = 0;
+    acc[i] = a;
+    a.ou = (float*)s->cand[0];
+    a.ov = (float*)s->cand[1];
+    a.ow = (float*)s->cand[2];
+    adv[i] = a;
+    BFaceArgs c = base;
+    c.u = (const float*)s->cand[0];
+    c.v = (const float*)s->cand[1];
+    c.w = (const float*)s->cand[2];
+    c.mode = 1;
+    cand[i] = c;
+    c.cells = s->rhs;
+    rhs[i] = c;
+    BFaceArgs k = base;
+    k.ou = (float*)s->cand[0];
+    k.ov = (float*)s->cand[1];
+    k.ow = (float*)s->cand[2];
+    k.cells = s->p;
+    cor[i] = k;
+    div[i] = cand[i];
+    rate[i] = cand[i];
+    BFaceArgs m = base;
+    m.u = (const float*)s->cand[0];
+    m.v = (const float*)s->cand[1];
+    m.w = (const float*)s->cand[2];
+    m.ou = (float*)s->acc[0];
+    m.ov = (float*)s->acc[1];
+    m.ow = (float*)s->acc[2];
+    commit[i] = m;
+    fb[i] = face_blocks(i);
+    cb[i] = cell_blocks(i);
+    rb[i] = face_blocks(i);
+  }
+  // --- cell records
+  std::vector<BCellArgs> cells(n);
+  for (size_t i = 0; i < n; ++i) {
+    rh_scene* s = scenes_[i].get();
+    Scratch& scr = s->solver->impl->scratch();
+    cells[i] = BCellArgs{d_grids_ + i, s->rhs, s->p, scr.pdd, scr.pmx, scr.ctl};
+  }
+  // --- finisher records
+  using FRhs = FinNotDone<FinRhsSums>;
+  using FReport = FinStatusOk<FinStore>;
+  auto fin_table = [&](auto make, int nd, int nm, const std::vector<int>& nb) -> size_t {
+    (void)nd;
+    (void)nm;
+    using R = decltype(make(0, 0));
+    std::vector<R> recs(n);
+    for (size_t i = 0; i < n; ++i) recs[i] = make(i, nb[i]);
+    return T.add(recs.data(), sizeof(R) * n);
+  };
+  auto fin_launch = [&](Op op, size_t off) {
+    Launch L;
+    L.op = op;
+    L.args_off = off;
+    L.blocks = (int)n;
+    program_.push_back(L);
+  };
+
+  // ===== program
+  per_scene(Op::kBeginStep);
+  stage(Op::kValidateAcc, acc.data(), sizeof(BFaceArgs), fb);
+  stage(Op::kAdvect, adv.data(), sizeof(BFaceArgs), fb);
+  stage(Op::kValidateCand, cand.data(), sizeof(BFaceArgs), fb);
+  stage(Op::kRhs, rhs.data(), sizeof(BFaceArgs), cb);
+  per_scene(Op::kBeginSolve);
+  stage(Op::kRhsSums, cells.data(), sizeof(BCellArgs), cb);
+  fin_launch(Op::kFinRhs, fin_table(
+                              [&](size_t i, int nb) {
+                                Scratch& scr = scenes_[i]->solver->impl->scratch();
+                                return BFin<3, 1, FRhs>{scr.pdd, scr.pmx, nb, 0, FRhs{scr.ctl, FinRhsSums{scr.ctl}}};
+                              },
+                              3, 1, cb));
+  stage(Op::kFill, cells.data(), sizeof(BCellArgs), cb);
+
+  // outer residual records (quad / scalar split) and the V-cycle stages
+  std::vector<BOuterArgs> outer(n);
+  std::vector<int> outer_q(n, 0), outer_s(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const MgHierarchy& h = mg(i)->hierarchy();
+    const MgHierarchy* dh = mg(i)->device_hierarchy();
+    Scratch& scr = scenes_[i]->solver->impl->scratch();
+    outer[i] = BOuterArgs{d_grids_ + i, &dh->lv[0], scenes_[i]->p, scenes_[i]->rhs, (float*)h.b[0],
+                          (float*)h.x[0], scr.ctl, scr.pdd, scr.pmx};
+    const bool q = quad_ok && mg_quad_outer_ok(grids[i], h.lv[0], scenes_[i]->p, scenes_[i]->rhs);
+    (q ? outer_q : outer_s)[i] = cb[i];
+  }
+  std::vector<int> outer_nb(n);
+  for (size_t i = 0; i < n; ++i) outer_nb[i] = cb[i];
+  auto outer_stage = [&](int it) {
+    stage(Op::kOuterQ, outer.data(), sizeof(BOuterArgs), outer_q);
+    stage(Op::kOuterS, outer.data(), sizeof(BOuterArgs), outer_s);
+    fin_launch(Op::kFinOuter, fin_table(
+                                  [&](size_t i, int nb) {
+                                    Scratch& scr = scenes_[i]->solver->impl->scratch();
+                                    return BFin<1, 2, FinMgTrue>{scr.pdd, scr.pmx, nb, 0, FinMgTrue{scr.ctl, it}};
+                                  },
+                                  1, 2, outer_nb));
+  };
+
+  // V-cycle stages, built once (same records every cycle)
+  std::vector<Launch> cycle;
+  {
+    const size_t mark = program_.size();
+    int maxL = 0;
+    for (size_t i = 0; i < n; ++i) maxL = std::max(maxL, mg(i)->hierarchy().levels);
+    // bottom level of each scene: its tail level, else its coarsest level
+    std::vector<int> bottom(n);
+    for (size_t i = 0; i < n; ++i) {
+      const MgHierarchy& h = mg(i)->hierarchy();
+      bottom[i] = h.tail_level > 0 ? h.tail_level : h.levels - 1;
+    }
+    // ping-pong state per scene and level: true = current iterate in x
+    std::vector<std::vector<bool>> in_x(n, std::vector<bool>(kMgMaxLevels, true));
+    auto smooth_stage = [&](int l, bool to_p) {
+      std::vector<BSmoothArgs> r(n);
+      std::vector<int> bq(n, 0), bs(n, 0);
+      for (size_t i = 0; i < n; ++i) {
+        if (l >= bottom[i]) continue;
+        const MgHierarchy& h = mg(i)->hierarchy();
+        const MgHierarchy* dh = mg(i)->device_hierarchy();
+        const bool cx = in_x[i][l];
+        r[i] = BSmoothArgs{&dh->lv[l], (const float*)(cx ? h.x[l] : h.y[l]), (const float*)h.b[l],
+                           (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
+                           scenes_[i]->solver->impl->scratch().ctl};
+        const int blocks = grid_blocks(h.lv[l].ny * h.lv[l].nz);
+        const bool q = quad_ok && mg_quad_level(h.lv[l]);
+        (q ? bq : bs)[i] = blocks;
+        if (!to_p) in_x[i][l] = !cx;
+      }
+      stage(to_p ? Op::kSmoothQUpdate : Op::kSmoothQ, r.data(), sizeof(BSmoothArgs), bq);
+      stage(Op::kSmoothS, r.data(), sizeof(BSmoothArgs), bs);  // the scalar body updates p itself
+    };
+    for (int l = 0; l < maxL; ++l) {
+      for (int sw = 1; sw < pre; ++sw) smooth_stage(l, false);
+      // restriction l -> l+1
+      std::vector<BRestrictArgs> r(n);
+      std::vector<int> bq(n, 0), bs(n, 0);
+      for (size_t i = 0; i < n; ++i) {
+        if (l >= bottom[i]) continue;
+        const MgHierarchy& h = mg(i)->hierarchy();
+        const MgHierarchy* dh = mg(i)->device_hierarchy();
+        r[i] = BRestrictArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)(in_x[i][l] ? h.x[l] : h.y[l]),
+                             (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1],
+                             scenes_[i]->solver->impl->scratch().ctl};
+        const int blocks = grid_blocks(h.lv[l + 1].ny * h.lv[l + 1].nz);
+        (quad_ok && mg_quad_level(h.lv[l]) ? bq : bs)[i] = blocks;
+        in_x[i][l + 1] = true;
+      }
+      stage(Op::kRestrictQ, r.data(), sizeof(BRestrictArgs), bq);
+      stage(Op::kRestrictS, r.data(), sizeof(BRestrictArgs), bs);
+      // bottoms at level l + 1
+      std::vector<BTailArgs> t(n);
+      std::vector<BCoarseArgs> cd(n);
+      std::vector<int> bt(n, 0), bc(n, 0);
+      int lds = 0;
+      for (size_t i = 0; i < n; ++i) {
+        if (bottom[i] != l + 1) continue;
+        const MgHierarchy& h = mg(i)->hierarchy();
+        const MgHierarchy* dh = mg(i)->device_hierarchy();
+        const PcgControl* ctl = scenes_[i]->solver->impl->scratch().ctl;
+        if (h.tail_level > 0) {
+          t[i] = BTailArgs{dh, l + 1, ctl};
+          bt[i] = 1;
+          lds = std::max(lds, h.tail_lds);
+        } else if (h.coarse_inv) {
+          cd[i] = BCoarseArgs{&dh->lv[l + 1], h.coarse_inv, (const float*)h.b[l + 1], (float*)h.x[l + 1], ctl};
+          bc[i] = 1;
+        } else {
+          set_last_error("a batched scene's coarsest level has no dense inverse", "rh_batch_step");
+          return RH_UNSUPPORTED;
+        }
+      }
+      stage(Op::kTail, t.data(), sizeof(BTailArgs), bt, lds);
+      stage(Op::kCoarseDense, cd.data(), sizeof(BCoarseArgs), bc);
+    }
+    for (int l = maxL - 1; l >= 0; --l) {
+      std::vector<BProlongArgs> r(n);
+      std::vector<int> bq(n, 0), bs(n, 0);
+      for (size_t i = 0; i < n; ++i) {
+        if (l >= bottom[i]) continue;
+        const MgHierarchy& h = mg(i)->hierarchy();
+        const MgHierarchy* dh = mg(i)->device_hierarchy();
+        const bool cx = in_x[i][l];
+        r[i] = BProlongArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)h.x[l + 1],
+                            (const float*)(cx ? h.x[l] : h.y[l]), (float*)(cx ? h.y[l] : h.x[l]),
+                            scenes_[i]->solver->impl->scratch().ctl};
+        const int blocks = grid_blocks(h.lv[l].ny * h.lv[l].nz);
+        (quad_ok && mg_quad_level(h.lv[l]) ? bq : bs)[i] = blocks;
+        in_x[i][l] = !cx;
+      }
+      stage(Op::kProlongQ, r.data(), sizeof(BProlongArgs), bq);
+      stage(Op::kProlongS, r.data(), sizeof(BProlongArgs), bs);
+      for (int sw = 0; sw < post; ++sw) {
+        const bool last = sw + 1 == post;
+        if (last && l == 0) {
+          smooth_stage(0, true);
+        } else {
+          smooth_stage(l, false);
+        }
+      }
+      // a coarse level's result must end in x[l] for the parent's prolongation
+      for (size_t i = 0; i < n; ++i)
+        if (l > 0 && l < bottom[i] && !in_x[i][l]) {
+          set_last_error("sweep counts leave a level result outside x", "rh_batch_step");
+          return RH_UNSUPPORTED;
+        }
+    }
+    cycle.assign(program_.begin() + mark, program_.end());
+    program_.resize(mark);
+  }
+  outer_stage(0);
+  for (int it = 1; it <= max_cycles_; ++it) {
+    program_.insert(program_.end(), cycle.begin(), cycle.end());
+    outer_stage(it);
+  }
+  per_scene(Op::kEndSolve);
+  stage(Op::kRegauge, cells.data(), sizeof(BCellArgs), cb);
+  stage(Op::kReport, cells.data(), sizeof(BCellArgs), cb);
+  fin_launch(Op::kFinReport, fin_table(
+                                 [&](size_t i, int nb) {
+                                   Scratch& scr = scenes_[i]->solver->impl->scratch();
+                                   return BFin<1, 2, FReport>{scr.pdd, scr.pmx, nb, 0,
+                                                              FReport{scr.ctl, FinStore{scr.scal, 2, 0.0, 0.0, 0.0}}};
+                                 },
+                                 1, 2, cb));
+  per_scene(Op::kQuality);
+  stage(Op::kCorrect, cor.data(), sizeof(BFaceArgs), fb);
+  stage(Op::kDivMax, div.data(), sizeof(BFaceArgs), cb);
+  fin_launch(Op::kFinDiv, fin_table(
+                              [&](size_t i, int nb) {
+                                Scratch& scr = scenes_[i]->solver->impl->scratch();
+                                return BFin<1, 1, FReport>{scr.pdd, scr.pmx, nb, 0,
+                                                           FReport{scr.ctl, FinStore{scr.scal + 3, 0, 0.0, 0.0, 0.0}}};
+                              },
+                              1, 1, cb));
+  stage(Op::kRate, rate.data(), sizeof(BFaceArgs), rb);
+  fin_launch(Op::kFinRate, fin_table(
+                               [&](size_t i, int nb) {
+                                 Scratch& scr = scenes_[i]->solver->impl->scratch();
+                                 return BFin<1, 3, FReport>{
+                                     scr.pdd, scr.pmx, nb, 0,
+                                     FReport{scr.ctl, FinStore{scr.scal + 4, 1, grids[i].h[0], grids[i].h[1], grids[i].h[2]}}};
+                               },
+                               1, 3, rb));
+  per_scene(Op::kGate);
+  stage(Op::kCommit, commit.data(), sizeof(BFaceArgs), fb);
+
+  // upload the tables
+  if (d_tables_) (void)hipFree(d_tables_);
+  d_tables_ = nullptr;
+  tables_bytes_ = T.bytes.size();
+  RH_HIP(hipMalloc(&d_tables_, std::max<size_t>(tables_bytes_, 16)));
+  RH_HIP(hipMemcpy(d_tables_, T.bytes.data(), tables_bytes_, hipMemcpyHostToDevice));
+  for (size_t i = 0; i < n; ++i) gens_[i] = scenes_[i]->solver->impl->generation();
+  if (cfg_.use_graph) {
+    RH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    run_program(stream_);
+    hipError_t e = hipStreamEndCapture(stream_, &graph_);
+    if (e != hipSuccess) {
+      set_last_error(hipGetErrorString(e), "hipStreamEndCapture");
+      return RH_DEVICE_ERROR;
+    }
+    RH_HIP(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+  built_ = true;
+  return RH_OK;
+}
+
+void Batch::run_program(hipStream_t s) {
+  const int n = (int)scenes_.size();
+  for (const Launch& L : program_) {
+    BatchLaunch bl;
+    bl.blocks = L.blocks;
+    bl.n = n;
+    bl.prefix = (const int*)(d_tables_ + L.prefix_off);
+    bl.args = d_tables_ + L.args_off;
+    const int pb = L.blocks;
+    switch (L.op) {
+      case Op::kBeginStep: kb_begin_step<<<pb, 256, 0, s>>>(d_scenes_, n); break;
+      case Op::kBeginSolve: kb_begin_solve<<<pb, 256, 0, s>>>(d_scenes_, n); break;
+      case Op::kEndSolve: kb_end_solve<<<pb, 256, 0, s>>>(d_scenes_, n); break;
+      case Op::kQuality: kb_quality<<<pb, 256, 0, s>>>(d_scenes_, n); break;
+      case Op::kGate: kb_gate<<<pb, 256, 0, s>>>(d_scenes_, n); break;
+      case Op::kValidateAcc:
+      case Op::kValidateCand: launch_b_validate(bl, s); break;
+      case Op::kAdvect: launch_b_advect(bl, s); break;
+      case Op::kRhs: launch_b_rhs(bl, s); break;
+      case Op::kRhsSums: launch_b_rhs_sums(bl, s); break;
+      case Op::kFinRhs:
+        kb_finish<3, 1, FinNotDone<FinRhsSums>><<<n, dim3(kLanesX, kRowsPerBlock), 0, s>>>(
+            (const BFin<3, 1, FinNotDone<FinRhsSums>>*)bl.args);
+        break;
+      case Op::kFill: launch_b_fill_pressure(bl, s); break;
+      case Op::kOuterQ: launch_b_mg_outer_q(bl, s); break;
+      case Op::kOuterS: launch_b_mg_outer(bl, s); break;
+      case Op::kFinOuter:
+        kb_finish<1, 2, FinMgTrue><<<n, dim3(kLanesX, kRowsPerBlock), 0, s>>>(
+            (const BFin<1, 2, FinMgTrue>*)bl.args);
+        break;
+      case Op::kSmoothQ: launch_b_mg_smooth_q(bl, false, s); break;
+      case Op::kSmoothQUpdate: launch_b_mg_smooth_q(bl, true, s); break;
+      case Op::kSmoothS: launch_b_mg_smooth(bl, s); break;
+      case Op::kRestrictQ: launch_b_mg_restrict_q(bl, s); break;
+      case Op::kRestrictS: launch_b_mg_restrict(bl, s); break;
+      case Op::kProlongQ: launch_b_mg_prolong_q(bl, s); break;
+      case Op::kProlongS: launch_b_mg_prolong(bl, s); break;
+      case Op::kTail: launch_b_mg_tail(bl, L.extra, s); break;
+      case Op::kCoarseDense: launch_b_mg_coarsest_dense(bl, s); break;
+      case Op::kRegauge: launch_b_regauge(bl, s); break;
+      case Op::kReport: launch_b_residual_report(bl, s); break;
+      case Op::kFinReport:
+        kb_finish<1, 2, FinStatusOk<FinStore>><<<n, dim3(kLanesX, kRowsPerBlock), 0, s>>>(
+            (const BFin<1, 2, FinStatusOk<FinStore>>*)bl.args);
+        break;
+      case Op::kCorrect: launch_b_correct(bl, s); break;
+      case Op::kDivMax: launch_b_divergence_max(bl, s); break;
+      case Op::kFinDiv:
+        kb_finish<1, 1, FinStatusOk<FinStore>><<<n, dim3(kLanesX, kRowsPerBlock), 0, s>>>(
+            (const BFin<1, 1, FinStatusOk<FinStore>>*)bl.args);
+        break;
+      case Op::kRate: launch_b_velocity_rate(bl, s); break;
+      case Op::kFinRate:
+        kb_finish<1, 3, FinStatusOk<FinStore>><<<n, dim3(kLanesX, kRowsPerBlock), 0, s>>>(
+            (const BFin<1, 3, FinStatusOk<FinStore>>*)bl.args);
+        break;
+      case Op::kCommit:
+        kb_commit<<<dim3(L.blocks, 3), dim3(kLanesX, kRowsPerBlock), 0, s>>>((const BFaceArgs*)bl.args,
+                                                                            bl.prefix, n);
+        break;
+    }
+  }
+}
+
+rh_status Batch::step(int steps, rh_step_report* reports, rh_status* statuses) {
+  const size_t n = scenes_.size();
+  if (steps < 1) return RH_INVALID_ARGUMENT;
+  for (size_t i = 0; i < n; ++i)
+    if (!(dt_[i] > 0.0)) return RH_INVALID_TIME_STEP;
+  for (size_t i = 0; i < n; ++i)
+    if (scenes_[i]->solver->impl->generation() != gens_[i]) built_ = false;
+  if (!built_) {
+    rh_status s = build();
+    if (s) return s;
+  }
+  // every scene's own stream must be idle (uploads went through them)
+  for (size_t i = 0; i < n; ++i) RH_HIP(hipStreamSynchronize(scenes_[i]->solver->impl->scratch().stream));
+  hipEvent_t e0, e1;
+  RH_HIP(hipEventCreate(&e0));
+  RH_HIP(hipEventCreate(&e1));
+  RH_HIP(hipEventRecord(e0, stream_));
+  for (int k = 0; k < steps; ++k) {
+    if (exec_) {
+      RH_HIP(hipGraphLaunch(exec_, stream_));
+    } else {
+      run_program(stream_);
+    }
+  }
+  RH_HIP(hipEventRecord(e1, stream_));
+  RH_HIP(hipGetLastError());
+  RH_HIP(hipMemcpyAsync(h_scenes_.data(), d_scenes_, sizeof(BatchScene) * n, hipMemcpyDeviceToHost,
+                        stream_));
+  RH_HIP(hipStreamSynchronize(stream_));
+  float ms = 0.0f;
+  RH_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (size_t i = 0; i < n; ++i) {
+    const BatchScene& b = h_scenes_[i];
+    if (statuses) statuses[i] = (rh_status)b.status;
+    if (reports) {
+      rh_step_report& r = reports[i];
+      std::memset(&r, 0, sizeof(r));
+      r.dt = b.dt;
+      r.pressure = b.rep;
+      r.actual_divergence_max = b.actual_div;
+      r.courant = b.courant;
+      r.seconds = ms * 1e-3 / steps;
+    }
+  }
+  return RH_OK;
+}
+
+}  // namespace rh
+
+struct rh_batch {
+  rh::Batch impl;
+};
+
+extern "C" rh_status rh_batch_create(const rh_grid* grids, size_t n, const rh_batch_config* config,
+                                     rh_batch** out) {
+  if (!grids || !config || !out || n == 0) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto* b = new (std::nothrow) rh_batch();
+  if (!b) return RH_DEVICE_ERROR;
+  rh_status s = b->impl.create(grids, n, *config);
+  if (s) {
+    delete b;
+    return s;
+  }
+  *out = b;
+  return RH_OK;
+}
+
+extern "C" void rh_batch_destroy(rh_batch* batch) { delete batch; }
+
+extern "C" uint64_t rh_batch_allocated_bytes(const rh_batch* batch) {
+  return batch ? batch->impl.bytes() : 0;
+}
+
+extern "C" rh_status rh_batch_scene(rh_batch* batch, size_t i, rh_scene** out) {
+  if (!batch || !out) return RH_INVALID_ARGUMENT;
+  *out = batch->impl.scene(i);
+  return *out ? RH_OK : RH_INVALID_ARGUMENT;
+}
+
+extern "C" rh_status rh_batch_set_dt(rh_batch* batch, const double* dt, size_t n) {
+  if (!batch) return RH_INVALID_ARGUMENT;
+  return batch->impl.set_dt_all(dt, n);
+}
+
+extern "C" rh_status rh_batch_set_variant(rh_batch* batch, uint32_t kernel_variant) {
+  if (!batch) return RH_INVALID_ARGUMENT;
+  return batch->impl.set_variant(kernel_variant);
+}
+
+extern "C" rh_status rh_batch_step(rh_batch* batch, int32_t steps, rh_step_report* reports,
+                                   rh_status* statuses) {
+  if (!batch) return RH_INVALID_ARGUMENT;
+  return batch->impl.step(steps, reports, statuses);
+}

@@ -13,6 +13,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "runtime.hpp"
+#include "handles.hpp"
 #include "profiler.hpp"
 
 using rh::DevGrid;
@@ -342,9 +343,6 @@ extern "C" rh_status rh_precondition(const rh_grid* grid, double density, rh_met
 
 // ------------------------------------------------------- device solver
 
-struct rh_solver {
-  std::unique_ptr<rh::Solver> impl;
-};
 
 extern "C" rh_status rh_solver_create(const rh_grid* grid, const rh_solver_config* config,
                                       rh_solver** out) {
@@ -661,24 +659,6 @@ extern "C" rh_status rh_provider_workspace_solve(rh_provider_workspace* ws,
 
 // ----------------------------------------------------------------- scene
 
-struct rh_scene {
-  rh_grid grid{};
-  rh_scene_config config{};
-  size_t n = 0, flen[3] = {0, 0, 0};
-  void* acc[3] = {nullptr, nullptr, nullptr};
-  void* cand[3] = {nullptr, nullptr, nullptr};
-  double* rhs = nullptr;
-  double* p = nullptr;
-  void* arena = nullptr;
-  uint64_t bytes = 0;
-  std::unique_ptr<rh_solver> solver;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  ~rh_scene() {
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-    if (arena) (void)hipFree(arena);
-  }
-};
 
 extern "C" rh_status rh_scene_create(const rh_grid* grid, const rh_scene_config* config,
                                      rh_scene** out) {

@@ -141,25 +141,37 @@ __device__ __forceinline__ int flat_tid() { return threadIdx.y * kLanesX + threa
 // "last block" scheme needs a device-scope release fence per block, which on
 // gfx950 writes back and invalidates the whole L2 and destroys stencil reuse.)
 // ND double-double sums and NM maxima (both >= 1; pass dummies if unused).
+// A kernel body runs as block b of nb: the hardware block of a one-scene
+// launch, or a block of one scene's share of a batched launch.
+struct VBlock {
+  int b, nb;
+};
+__device__ __forceinline__ VBlock hw_block() { return VBlock{(int)blockIdx.x, (int)gridDim.x}; }
+
 template <int ND, int NM>
 __device__ __forceinline__ void block_publish(DD (&dd)[ND], double (&mx)[NM], DD* __restrict__ pdd,
-                                              double* __restrict__ pmx) {
+                                              double* __restrict__ pmx, VBlock vb) {
   block_reduce_dd<ND>(dd);
   block_reduce_max<NM>(mx);
   if (flat_tid() == 0) {
 #pragma unroll
-    for (int q = 0; q < ND; ++q) pdd[q * kMaxBlocks + blockIdx.x] = dd[q];
+    for (int q = 0; q < ND; ++q) pdd[q * kMaxBlocks + vb.b] = dd[q];
 #pragma unroll
-    for (int q = 0; q < NM; ++q) pmx[q * kMaxBlocks + blockIdx.x] = mx[q];
+    for (int q = 0; q < NM; ++q) pmx[q * kMaxBlocks + vb.b] = mx[q];
   }
+}
+template <int ND, int NM>
+__device__ __forceinline__ void block_publish(DD (&dd)[ND], double (&mx)[NM], DD* __restrict__ pdd,
+                                              double* __restrict__ pmx) {
+  block_publish<ND, NM>(dd, mx, pdd, pmx, hw_block());
 }
 
 // `Fin` provides `bool active() const` (false when the producer returned
 // early without publishing) and `void operator()(const double* sums,
 // const double* maxima) const`, both __device__.
 template <int ND, int NM, typename Fin>
-__global__ void __launch_bounds__(kBlock)
-k_finish(const DD* __restrict__ pdd, const double* __restrict__ pmx, int nb, Fin fin) {
+__device__ __forceinline__ void finish_body(const DD* __restrict__ pdd, const double* __restrict__ pmx,
+                                            int nb, const Fin& fin) {
   if (!fin.active()) return;
   const int t = flat_tid();
   DD a[ND];
@@ -182,6 +194,12 @@ k_finish(const DD* __restrict__ pdd, const double* __restrict__ pmx, int nb, Fin
     for (int q = 0; q < ND; ++q) sums[q] = dd_value(a[q]);
     fin(sums, m);
   }
+}
+
+template <int ND, int NM, typename Fin>
+__global__ void __launch_bounds__(kBlock)
+k_finish(const DD* __restrict__ pdd, const double* __restrict__ pmx, int nb, Fin fin) {
+  finish_body<ND, NM, Fin>(pdd, pmx, nb, fin);
 }
 
 template <int ND, int NM, typename Fin>
@@ -224,9 +242,9 @@ struct RowRange {
 // Contiguous row-group partition of a dim grid; block ids are remapped so the
 // 8 XCDs (round-robin dispatch) each own a contiguous slab of rows, which
 // keeps the j/k neighbours of a stencil inside one XCD's L2.
-__device__ __forceinline__ RowRange block_rows(int rows) {
-  const int nb = gridDim.x;
-  const int b = blockIdx.x;
+__device__ __forceinline__ RowRange block_rows(int rows, VBlock vb) {
+  const int nb = vb.nb;
+  const int b = vb.b;
   int e = b;
   if ((nb & 7) == 0) e = (b & 7) * (nb >> 3) + (b >> 3);
   const int groups = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
@@ -239,6 +257,22 @@ __device__ __forceinline__ RowRange block_rows(int rows) {
   if (r.end > rows) r.end = rows;
   if (r.begin > r.end) r.begin = r.end;
   return r;
+}
+__device__ __forceinline__ RowRange block_rows(int rows) { return block_rows(rows, hw_block()); }
+
+// Scene of block b in a batched launch: prefix[s] <= b < prefix[s + 1]
+// (prefix has n + 1 entries; scenes with no blocks are skipped).
+__device__ __forceinline__ int batch_scene(const int* __restrict__ prefix, int n, int b) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ VBlock batch_block(const int* __restrict__ prefix, int s) {
+  return VBlock{(int)blockIdx.x - prefix[s], prefix[s + 1] - prefix[s]};
 }
 
 // Host helpers ---------------------------------------------------------------

@@ -6,6 +6,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "batch.hpp"
 
 namespace rh {
 
@@ -164,15 +165,15 @@ __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restr
 // normal faces are exactly zero; every other face samples its own component
 // at the RK2 departure of its own face position (src/geometry.rs:240-248).
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void
+b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
                   T* __restrict__ ow, unsigned* __restrict__ flags) {
   const int a = blockIdx.y;
   const T* src = a == 0 ? u : (a == 1 ? v : w);
   T* dst = a == 0 ? ou : (a == 1 ? ov : ow);
   const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
-  const RowRange rr = block_rows(fy * fz);
+  const RowRange rr = block_rows(fy * fz, vb);
   unsigned bad = 0;
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
     const int k = row / fy, j = row - k * fy;
@@ -202,6 +203,13 @@ k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
     }
   }
   if (bad) atomicOr(flags, bad);
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                  const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
+                  T* __restrict__ ow, unsigned* __restrict__ flags) {
+  b_advect_velocity<T>(hw_block(), g, u, v, w, dt, ou, ov, ow, flags);
 }
 
 // advect_tracer (src/advection.rs:72-102), every cell evaluated (the
@@ -239,13 +247,13 @@ k_advect_tracer(DevGrid g, const float* __restrict__ old, const float* __restric
 // (validate_velocity, src/operator.rs:850-875; SamplingError::NonFiniteField,
 // src/sampling.rs:62-64).  blockIdx.y = component.
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_validate_faces(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void
+b_validate_faces(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                  const T* __restrict__ w, int check_walls, unsigned* __restrict__ flags) {
   const int a = blockIdx.y;
   const T* src = a == 0 ? u : (a == 1 ? v : w);
   const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
-  const RowRange rr = block_rows(fy * fz);
+  const RowRange rr = block_rows(fy * fz, vb);
   unsigned bad = 0;
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
     const int k = row / fy, j = row - k * fy;
@@ -261,16 +269,22 @@ k_validate_faces(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
   }
   if (bad) atomicOr(flags, bad);
 }
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_validate_faces(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                 const T* __restrict__ w, int check_walls, unsigned* __restrict__ flags) {
+  b_validate_faces<T>(hw_block(), g, u, v, w, check_walls, flags);
+}
 
 // build_rhs (src/operator.rs:460-509): b = (sum_d A_d (u_lo - u_hi)) / dt,
 // axis order X, Y, Z, f64 from T faces.
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_build_rhs(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void
+b_build_rhs(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
             const T* __restrict__ w, double dt, double* __restrict__ out,
             unsigned* __restrict__ flags) {
   const int nx = g.nx, ny = g.ny;
-  const RowRange rr = block_rows(g.ny * g.nz);
+  const RowRange rr = block_rows(g.ny * g.nz, vb);
   unsigned bad = 0;
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
     const int k = row / ny, j = row - k * ny;
@@ -292,19 +306,26 @@ k_build_rhs(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
   }
   if (bad) atomicOr(flags, bad);
 }
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_build_rhs(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+            const T* __restrict__ w, double dt, double* __restrict__ out,
+            unsigned* __restrict__ flags) {
+  b_build_rhs<T>(hw_block(), g, u, v, w, dt, out, flags);
+}
 
 // correct_candidate_in_place (src/operator.rs:627-668) for f32 storage:
 // u = f32(f64(u) - dt*acc_d*(p_R - p_L)), evaluated left to right;
 // correct_candidate_f64 (:746-781) for f64: u - dt*(acc_d*(p_R - p_L)).
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_correct(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u,
+__device__ __forceinline__ void
+b_correct(VBlock vb, const DevGrid& g, const double* __restrict__ p, double dt, T* __restrict__ u,
           T* __restrict__ v, T* __restrict__ w, unsigned* __restrict__ flags) {
   const int a = blockIdx.y;
   T* dst = a == 0 ? u : (a == 1 ? v : w);
   const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
   const int stride = a == 0 ? 1 : (a == 1 ? g.nx : g.nx * g.ny);
-  const RowRange rr = block_rows(fy * fz);
+  const RowRange rr = block_rows(fy * fz, vb);
   const double acc = g.acc[a];
   unsigned bad = 0;
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
@@ -341,16 +362,22 @@ k_correct(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u,
   }
   if (bad) atomicOr(flags, bad);
 }
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_correct(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u,
+          T* __restrict__ v, T* __restrict__ w, unsigned* __restrict__ flags) {
+  b_correct<T>(hw_block(), g, p, dt, u, v, w, flags);
+}
 
 // divergence (src/operator.rs:513-568) and its all-cell maximum
 // (actual_divergence_max, src/pressure.rs:330-342):
 // div = (u_hi-u_lo)/h0 + (v_hi-v_lo)/h1 + (w_hi-w_lo)/h2.
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_divergence_max(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void
+b_divergence_max(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                  const T* __restrict__ w, Reduce red) {
   const int nx = g.nx, ny = g.ny;
-  const RowRange rr = block_rows(g.ny * g.nz);
+  const RowRange rr = block_rows(g.ny * g.nz, vb);
   double m[1] = {0.0};
   unsigned bad = 0;
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
@@ -368,25 +395,37 @@ k_divergence_max(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
   }
   if (bad) atomicOr(red.flags, bad);
   DD dd[1] = {DD{0.0, 0.0}};
-  block_publish<1, 1>(dd, m, red.pdd, red.pmx);
+  block_publish<1, 1>(dd, m, red.pdd, red.pmx, vb);
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_divergence_max(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                 const T* __restrict__ w, Reduce red) {
+  b_divergence_max<T>(hw_block(), g, u, v, w, red);
 }
 
 // velocity_rate (src/simulation.rs:1415-1429): sum_d max|u_d| / h_d.
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-k_velocity_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+__device__ __forceinline__ void
+b_velocity_rate(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                 const T* __restrict__ w, Reduce red) {
   double m[3] = {0.0, 0.0, 0.0};
   for (int a = 0; a < 3; ++a) {
     const T* src = a == 0 ? u : (a == 1 ? v : w);
     const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
-    const RowRange rr = block_rows(fy * fz);
+    const RowRange rr = block_rows(fy * fz, vb);
     for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock)
       for (int i = threadIdx.x; i < fx; i += kLanesX)
         m[a] = fmax(m[a], fabs((double)src[i + fx * row]));
   }
   DD dd[1] = {DD{0.0, 0.0}};
-  block_publish<1, 3>(dd, m, red.pdd, red.pmx);
+  block_publish<1, 3>(dd, m, red.pdd, red.pmx, vb);
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_velocity_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
+                const T* __restrict__ w, Reduce red) {
+  b_velocity_rate<T>(hw_block(), g, u, v, w, red);
 }
 
 // ------------------------------------------------------------ host launchers
@@ -471,6 +510,87 @@ void launch_zero_blocked(const DevGrid& g, void* u, void* v, void* w, hipStream_
   if (!g.mask) return;
   dim3 grid(grid_blocks((g.ny + 1) * (g.nz + 1)), 3), block(kLanesX, kRowsPerBlock);
   k_zero_blocked<T><<<grid, block, 0, s>>>(g, (T*)u, (T*)v, (T*)w);
+}
+
+// ---------------------------------------------------- batched (f32 faces)
+// Records: BFaceArgs; `state_ok` records skip scenes whose step has failed.
+
+#define RH_BATCH_PROLOGUE(ARGS)                                  \
+  const int s_ = batch_scene(prefix, n, blockIdx.x);            \
+  const ARGS& a = A[s_];                                        \
+  const VBlock vb = batch_block(prefix, s_);
+
+__global__ void __launch_bounds__(kBlock)
+kb_validate(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BFaceArgs)
+  b_validate_faces<float>(vb, *a.g, a.u, a.v, a.w, a.mode, a.flags);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_advect(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BFaceArgs)
+  b_advect_velocity<float>(vb, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow, a.flags);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_rhs(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BFaceArgs)
+  b_build_rhs<float>(vb, *a.g, a.u, a.v, a.w, *a.dt, a.cells, a.flags);
+}
+// correct / divergence / rate run only for scenes whose solve was accepted
+// (ctl->status == RH_OK after the quality gate).
+__global__ void __launch_bounds__(kBlock)
+kb_correct(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BFaceArgs)
+  if (a.ctl->status != RH_OK) return;
+  b_correct<float>(vb, *a.g, a.cells, *a.dt, a.ou, a.ov, a.ow, a.flags);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_divergence_max(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BFaceArgs)
+  if (a.ctl->status != RH_OK) return;
+  Reduce red;
+  red.pdd = a.pdd;
+  red.pmx = a.pmx;
+  red.out = nullptr;
+  red.flags = a.flags;
+  b_divergence_max<float>(vb, *a.g, a.u, a.v, a.w, red);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_velocity_rate(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BFaceArgs)
+  if (a.ctl->status != RH_OK) return;
+  Reduce red;
+  red.pdd = a.pdd;
+  red.pmx = a.pmx;
+  red.out = nullptr;
+  red.flags = a.flags;
+  b_velocity_rate<float>(vb, *a.g, a.u, a.v, a.w, red);
+}
+
+static dim3 face_batch_grid(const BatchLaunch& bl) { return dim3(bl.blocks, 3); }
+static dim3 block_shape() { return dim3(kLanesX, kRowsPerBlock); }
+
+void launch_b_validate(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_validate<<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_advect(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_advect<<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_rhs(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks) kb_rhs<<<bl.blocks, block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_correct(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_correct<<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_divergence_max(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_divergence_max<<<bl.blocks, block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_velocity_rate(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_velocity_rate<<<bl.blocks, block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
 }
 
 #define RH_INSTANTIATE(T)                                                                     \

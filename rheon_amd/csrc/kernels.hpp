@@ -70,6 +70,34 @@ struct PcgControl {
   double spare[7];
 };
 
+__device__ __forceinline__ void fail(PcgControl* ctl, int status, int it) {
+  ctl->done = 1;
+  ctl->status = status;
+  ctl->error_iteration = it;
+}
+
+// Compatibility sums and threshold (src/pressure.rs:603-633).
+struct FinRhsSums {
+  PcgControl* ctl;
+  __device__ bool active() const { return true; }
+  __device__ void operator()(const double* sum, const double* mx) const {
+    if (mx[0] != 0.0) return fail(ctl, RH_NONFINITE_INPUT, 0);
+    const double total = sum[0], total_abs = sum[1];
+    ctl->sum = total;
+    ctl->sum_abs = total_abs;
+    const double budget = (64.0 * 2.220446049250313e-16) * total_abs;
+    if (!isfinite(total) || !isfinite(total_abs)) return fail(ctl, RH_BREAKDOWN, 0);
+    if (fabs(total) > budget) {
+      ctl->error_value = total;
+      return fail(ctl, RH_INCOMPATIBLE_RHS, 0);
+    }
+    const double rhs_l2 = sqrt(sum[2]);
+    const double threshold = fmax(ctl->abs_tol, ctl->rel * rhs_l2);
+    if (!isfinite(rhs_l2) || !isfinite(threshold)) return fail(ctl, RH_BREAKDOWN, 0);
+    ctl->threshold = threshold;
+  }
+};
+
 struct PcgArrays {
   void* p;       // T
   void* r;       // T

@@ -75,6 +75,34 @@ void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, c
 void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,
                          float* xout, const PcgControl* ctl, hipStream_t s);
 
+// Acceptance of the outer defect-correction residual; every cycle is a
+// "candidate" (the true residual is computed each cycle).
+struct FinMgTrue {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done; }
+  __device__ void operator()(const double* sum, const double* mx) const {
+    if (mx[1] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
+    const double squared = sum[0], maximum = mx[0];
+    const double divergence = maximum * ctl->dt / ctl->volume;
+    if (!isfinite(squared) || !isfinite(divergence)) return fail(ctl, RH_BREAKDOWN, it);
+    const double l2 = sqrt(squared);
+    ctl->true_l2 = l2;
+    ctl->true_max = maximum;
+    ctl->pred_div = divergence;
+    ctl->iteration = it;
+    if (l2 <= ctl->threshold && divergence <= ctl->div_limit) {
+      ctl->done = 1;
+      ctl->status = RH_OK;
+      return;
+    }
+    if (it >= ctl->max_iterations) {
+      ctl->error_value = maximum;
+      return fail(ctl, RH_ITERATION_LIMIT, it);
+    }
+  }
+};
+
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
                        PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s);

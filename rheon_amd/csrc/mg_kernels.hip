@@ -24,18 +24,10 @@
 #include "profiler.hpp"
 #include "stencil.hpp"
 #include "mg_stencil.hpp"
+#include "batch.hpp"
 
 namespace rh {
 
-namespace {
-
-__device__ __forceinline__ void mg_fail(PcgControl* ctl, int status, int it) {
-  ctl->done = 1;
-  ctl->status = status;
-  ctl->error_iteration = it;
-}
-
-}  // namespace
 
 // ------------------------------------------------------------------ kernels
 
@@ -43,14 +35,15 @@ __device__ __forceinline__ void mg_fail(PcgControl* ctl, int status, int it) {
 // right-hand side b0 = S(r) with the first pre-smoothing sweep x0 = omega
 // b0/D fused in; partial sum r^2, max|r|, nonfinite flag.
 template <typename S>
-__global__ void __launch_bounds__(kBlock)
-k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
-                    S* __restrict__ b0, S* __restrict__ x0, double omega, const PcgControl* ctl,
-                    DD* pdd, double* pmx) {
+__device__ __forceinline__ void b_mg_outer_residual(VBlock vb, const DevGrid& g, const MgLevel& L0,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ b, S* __restrict__ b0,
+                                                    S* __restrict__ x0, const PcgControl* ctl, DD* pdd,
+                                                    double* pmx) {
   if (ctl->done) return;
   DD s[1] = {{0.0, 0.0}};
   double m[2] = {0.0, 0.0};
-  FOR_CELLS(g)
+  FOR_CELLS_VB(g, vb)
   const double v = apply_at<double>(g, p, c, i, j, k);
   if (!isfinite(v)) m[1] = 1.0;
   const double r = b[c] - v;
@@ -60,17 +53,24 @@ k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const d
   b0[c] = rs;
   x0[c] = (S)lv_odiag(L0, c, i, j, k) * rs;
   END_CELLS
-  block_publish<1, 2>(s, m, pdd, pmx);
+  block_publish<1, 2>(s, m, pdd, pmx, vb);
+}
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
+                    S* __restrict__ b0, S* __restrict__ x0, double omega, const PcgControl* ctl,
+                    DD* pdd, double* pmx) {
+  b_mg_outer_residual<S>(hw_block(), g, L0, p, b, b0, x0, ctl, pdd, pmx);
 }
 
 // One weighted-Jacobi sweep x_out = x_in + omega (b - K x_in)/D.  With
 // `p` non-null (finest level, last post-sweep) also p += x_out.
 template <typename S>
-__global__ void __launch_bounds__(kBlock)
-k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __restrict__ xout,
-            double omega, double* __restrict__ p, const PcgControl* ctl) {
+__device__ __forceinline__ void b_mg_smooth(VBlock vb, const MgLevel& L, const S* __restrict__ xin,
+                                            const S* __restrict__ b, S* __restrict__ xout,
+                                            double* __restrict__ p, const PcgControl* ctl) {
   if (ctl->done) return;
-  FOR_LEVEL_CELLS(L)
+  FOR_LEVEL_CELLS_VB(L, vb)
   const S ax = s_apply<S>(L, xin, c, i, j, k);
   const S xn = xin[c] + (S)lv_odiag(L, c, i, j, k) * (b[c] - ax);
   if (p) {
@@ -80,15 +80,22 @@ k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __
   }
   END_CELLS
 }
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __restrict__ xout,
+            double omega, double* __restrict__ p, const PcgControl* ctl) {
+  b_mg_smooth<S>(hw_block(), L, xin, b, xout, p, ctl);
+}
 
 // Restriction: coarse cell (I,J,K) sums the residual b - K x of its 8
 // children; writes b_c and the first coarse pre-smoothing sweep x_c.
 template <typename S>
-__global__ void __launch_bounds__(kBlock)
-k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restrict__ b,
-              S* __restrict__ bc, S* __restrict__ xc, double omega, const PcgControl* ctl) {
+__device__ __forceinline__ void b_mg_restrict(VBlock vb, const MgLevel& F, const MgLevel& Cl,
+                                              const S* __restrict__ x, const S* __restrict__ b,
+                                              S* __restrict__ bc, S* __restrict__ xc,
+                                              const PcgControl* ctl) {
   if (ctl->done) return;
-  FOR_LEVEL_CELLS(Cl)
+  FOR_LEVEL_CELLS_VB(Cl, vb)
   S sum = (S)0;
 #pragma unroll
   for (int dz = 0; dz < 2; ++dz)
@@ -104,16 +111,28 @@ k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restric
   xc[c] = (S)lv_odiag(Cl, c, i, j, k) * sum;
   END_CELLS
 }
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restrict__ b,
+              S* __restrict__ bc, S* __restrict__ xc, double omega, const PcgControl* ctl) {
+  b_mg_restrict<S>(hw_block(), F, Cl, x, b, bc, xc, ctl);
+}
 
 // Trilinear prolongation of the coarse correction, added to the fine x.
+template <typename S>
+__device__ __forceinline__ void b_mg_prolong(VBlock vb, const MgLevel& F, const MgLevel& Cl,
+                                             const S* __restrict__ xc, const S* __restrict__ xin,
+                                             S* __restrict__ xout, const PcgControl* ctl) {
+  if (ctl->done) return;
+  FOR_LEVEL_CELLS_VB(F, vb)
+  xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k, lv_fine_fluid(F, c));
+  END_CELLS
+}
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
 k_mg_prolong(MgLevel F, MgLevel Cl, const S* __restrict__ xc, const S* __restrict__ xin,
              S* __restrict__ xout, const PcgControl* ctl) {
-  if (ctl->done) return;
-  FOR_LEVEL_CELLS(F)
-  xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k, lv_fine_fluid(F, c));
-  END_CELLS
+  b_mg_prolong<S>(hw_block(), F, Cl, xc, xin, xout, ctl);
 }
 
 // Coarsest level: one workgroup solves K x = b - mean_a(b) with f64 CG in
@@ -122,8 +141,8 @@ k_mg_prolong(MgLevel F, MgLevel Cl, const S* __restrict__ xc, const S* __restric
 constexpr int kCoarseThreads = 1024;
 
 template <typename S>
-__global__ void __launch_bounds__(kCoarseThreads)
-k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const PcgControl* ctl) {
+__device__ __forceinline__ void b_mg_coarsest(const MgLevel& L, const S* __restrict__ b,
+                                              S* __restrict__ xout, const PcgControl* ctl) {
   if (ctl->done) return;
   __shared__ double xs[kMgCoarsestMax], rs[kMgCoarsestMax], ds[kMgCoarsestMax], qs[kMgCoarsestMax];
   __shared__ double red[kCoarseThreads / 64];
@@ -193,6 +212,11 @@ k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const Pc
   for (int c = t; c < n; c += kCoarseThreads)
     xout[c] = lv_active(L, c) ? (S)(xs[c] - xmean) : (S)0;
 }
+template <typename S>
+__global__ void __launch_bounds__(kCoarseThreads)
+k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const PcgControl* ctl) {
+  b_mg_coarsest<S>(L, b, xout, ctl);
+}
 
 // Coarsest level with the dense projected inverse: x = G b, one workgroup.
 constexpr int kDenseThreads = 512;
@@ -217,6 +241,14 @@ k_mg_coarsest_dense(MgLevel L, const double* __restrict__ minv, const S* __restr
   if (ctl->done) return;
   __shared__ double bs[kMgDenseMax];
   dense_solve<kDenseThreads, S>(L.nx * L.ny * L.nz, minv, b, xout, bs);
+}
+template <typename S>
+__global__ void __launch_bounds__(kDenseThreads)
+kb_mg_coarsest_dense(const BCoarseArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  const BCoarseArgs& a = A[batch_scene(prefix, n, blockIdx.x)];
+  if (a.ctl->done) return;
+  __shared__ double bs[kMgDenseMax];
+  dense_solve<kDenseThreads, S>(a.L->nx * a.L->ny * a.L->nz, a.minv, (const S*)a.b, (S*)a.x, bs);
 }
 
 // ------------------------------------------------------------- V-cycle tail
@@ -276,8 +308,8 @@ __device__ __forceinline__ void tail_copy(int n, const S* __restrict__ src, S* _
 }
 
 template <typename S>
-__global__ void __launch_bounds__(kTailThreads)
-k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
+__device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int l0,
+                                          const PcgControl* ctl) {
   if (ctl->done) return;
   // Level arrays of the tail live in LDS (offsets in H->tail_off); level l0
   // is loaded from and written back to global memory.
@@ -345,6 +377,17 @@ k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
     for (int c = threadIdx.x; c < n; c += kTailThreads) gx[c] = cur[l0][c];
   }
 }
+template <typename S>
+__global__ void __launch_bounds__(kTailThreads)
+k_mg_tail(const MgHierarchy* __restrict__ H, int l0, const PcgControl* ctl) {
+  b_mg_tail<S>(H, l0, ctl);
+}
+template <typename S>
+__global__ void __launch_bounds__(kTailThreads)
+kb_mg_tail(const BTailArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  const BTailArgs& a = A[batch_scene(prefix, n, blockIdx.x)];
+  b_mg_tail<S>(a.H, a.l0, a.ctl);
+}
 
 // p += x: a one-level hierarchy (no axis can be halved) solves the finest
 // level directly, so the correction is added here.
@@ -371,33 +414,59 @@ k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl) {
   END_CELLS
 }
 
-// Acceptance of the outer defect-correction residual; every cycle is a
-// "candidate" (the true residual is computed each cycle).
-struct FinMgTrue {
-  PcgControl* ctl;
-  int it;
-  __device__ bool active() const { return !ctl->done; }
-  __device__ void operator()(const double* sum, const double* mx) const {
-    if (mx[1] != 0.0) return mg_fail(ctl, RH_ARITHMETIC_FAILURE, it);
-    const double squared = sum[0], maximum = mx[0];
-    const double divergence = maximum * ctl->dt / ctl->volume;
-    if (!isfinite(squared) || !isfinite(divergence)) return mg_fail(ctl, RH_BREAKDOWN, it);
-    const double l2 = sqrt(squared);
-    ctl->true_l2 = l2;
-    ctl->true_max = maximum;
-    ctl->pred_div = divergence;
-    ctl->iteration = it;
-    if (l2 <= ctl->threshold && divergence <= ctl->div_limit) {
-      ctl->done = 1;
-      ctl->status = RH_OK;
-      return;
-    }
-    if (it >= ctl->max_iterations) {
-      ctl->error_value = maximum;
-      return mg_fail(ctl, RH_ITERATION_LIMIT, it);
-    }
-  }
-};
+
+// ------------------------------------------------------ batched (FP32)
+
+#define RH_BATCH_PROLOGUE(ARGS)                                  \
+  const int s_ = batch_scene(prefix, n, blockIdx.x);            \
+  const ARGS& a = A[s_];                                        \
+  const VBlock vb = batch_block(prefix, s_);
+
+__global__ void __launch_bounds__(kBlock)
+kb_mg_outer_residual(const BOuterArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BOuterArgs)
+  b_mg_outer_residual<float>(vb, *a.g, *a.L0, a.p, a.b, a.b0, a.x0, a.ctl, a.pdd, a.pmx);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_mg_smooth(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BSmoothArgs)
+  b_mg_smooth<float>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.ctl);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_mg_restrict(const BRestrictArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BRestrictArgs)
+  b_mg_restrict<float>(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.ctl);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_mg_prolong(const BProlongArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BProlongArgs)
+  b_mg_prolong<float>(vb, *a.F, *a.C, a.xc, a.xin, a.xout, a.ctl);
+}
+
+void launch_b_mg_outer(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_outer_residual<<<bl.blocks, block_shape(), 0, s>>>((const BOuterArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_smooth(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks) kb_mg_smooth<<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_restrict(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_restrict<<<bl.blocks, block_shape(), 0, s>>>((const BRestrictArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_prolong(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_prolong<<<bl.blocks, block_shape(), 0, s>>>((const BProlongArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_tail(const BatchLaunch& bl, int lds_bytes, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_tail<float><<<bl.blocks, kTailThreads, lds_bytes, s>>>((const BTailArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_coarsest_dense(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_coarsest_dense<float><<<bl.blocks, kDenseThreads, 0, s>>>((const BCoarseArgs*)bl.args, bl.prefix,
+                                                                     bl.n);
+}
 
 // ------------------------------------------------------------ launchers
 

@@ -9,6 +9,7 @@
 #include "kernels.hpp"
 #include "mg.hpp"
 #include "mg_stencil.hpp"
+#include "batch.hpp"
 #include "profiler.hpp"
 #include "stencil.hpp"
 
@@ -21,7 +22,7 @@ namespace {
 #define FOR_QUADS(NX, NY)                                                   \
   {                                                                         \
     const int cpr_ = (NX) >> 2;                                             \
-    const RowRange rr_ = block_rows((NY) * nz_);                            \
+    const RowRange rr_ = block_rows((NY) * nz_, vb);                        \
     const int t_end_ = rr_.end * cpr_;                                      \
     for (int t_ = rr_.begin * cpr_ + flat_tid(); t_ < t_end_; t_ += kBlock) { \
       const int row = t_ / cpr_, q = t_ - row * cpr_;                       \
@@ -125,8 +126,8 @@ __device__ __forceinline__ double quad_apply_d(const Quad<double>& Q, int u, con
 
 // Outer FP64 residual r = b - K p with the first pre-smoothing sweep fused
 // (as k_mg_outer_residual).
-__global__ void __launch_bounds__(kBlock)
-k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
+__device__ __forceinline__ void
+b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const double* __restrict__ p, const double* __restrict__ b,
                       float* __restrict__ b0, float* __restrict__ x0, const PcgControl* ctl, DD* pdd,
                       double* pmx) {
   if (ctl->done) return;
@@ -155,14 +156,14 @@ k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const
   store4(b0, c, rb);
   store4(x0, c, rx);
   END_QUADS
-  block_publish<1, 2>(s, m, pdd, pmx);
+  block_publish<1, 2>(s, m, pdd, pmx, vb);
 }
 
 // One weighted-Jacobi sweep (as k_mg_smooth); kUpdate: p += x_out instead of
 // storing x_out (the finest level's last sweep).
 template <bool kUpdate>
-__global__ void __launch_bounds__(kBlock)
-k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
+__device__ __forceinline__ void
+b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const float* __restrict__ b,
               float* __restrict__ xout, double* __restrict__ p, const PcgControl* ctl) {
   if (ctl->done) return;
   const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
@@ -196,8 +197,8 @@ k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict_
 // Restriction (as k_mg_restrict): each lane owns two coarse cells
 // (2 I0, 2 I0 + 1) of coarse row (J, K), i.e. one four-cell chunk in each of
 // the four fine rows (2J + dy, 2K + dz).  Sums keep the (dz, dy, dx) order.
-__global__ void __launch_bounds__(kBlock)
-k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float* __restrict__ b,
+__device__ __forceinline__ void
+b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __restrict__ x, const float* __restrict__ b,
                 float* __restrict__ bc, float* __restrict__ xc, const PcgControl* ctl) {
   if (ctl->done) return;
   const float w0 = (float)F.w[0], w1 = (float)F.w[1], w2 = (float)F.w[2];
@@ -238,8 +239,8 @@ k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float*
 
 // Prolongation (as k_mg_prolong) for four fine cells: parents I0 = i0/2 and
 // I0 + 1 and their x-neighbours I0 - 1 and I0 + 2 (mirrored at walls).
-__global__ void __launch_bounds__(kBlock)
-k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float* __restrict__ xin,
+__device__ __forceinline__ void
+b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __restrict__ xc, const float* __restrict__ xin,
                float* __restrict__ xout, const PcgControl* ctl) {
   if (ctl->done) return;
   const int nz_ = F.nz;
@@ -327,6 +328,79 @@ k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float*
   }
   store4(xout, c, out);
   END_QUADS
+}
+
+// ------------------------------------------------------- kernel wrappers
+
+__global__ void __launch_bounds__(kBlock)
+k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
+                      float* __restrict__ b0, float* __restrict__ x0, const PcgControl* ctl, DD* pdd,
+                      double* pmx) {
+  b_mg_outer_residual_q(hw_block(), g, L0, p, b, b0, x0, ctl, pdd, pmx);
+}
+template <bool kUpdate>
+__global__ void __launch_bounds__(kBlock)
+k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
+              float* __restrict__ xout, double* __restrict__ p, const PcgControl* ctl) {
+  b_mg_smooth_q<kUpdate>(hw_block(), L, xin, b, xout, p, ctl);
+}
+__global__ void __launch_bounds__(kBlock)
+k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float* __restrict__ b,
+                float* __restrict__ bc, float* __restrict__ xc, const PcgControl* ctl) {
+  b_mg_restrict_q(hw_block(), F, Cl, x, b, bc, xc, ctl);
+}
+__global__ void __launch_bounds__(kBlock)
+k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float* __restrict__ xin,
+               float* __restrict__ xout, const PcgControl* ctl) {
+  b_mg_prolong_q(hw_block(), F, Cl, xc, xin, xout, ctl);
+}
+
+// Batched: block -> scene, scene-local block, the scene's argument record.
+#define RH_BATCH_PROLOGUE(ARGS)                                  \
+  const int s_ = batch_scene(prefix, n, blockIdx.x);            \
+  const ARGS& a = A[s_];                                        \
+  const VBlock vb = batch_block(prefix, s_);
+
+__global__ void __launch_bounds__(kBlock)
+kb_mg_outer_residual_q(const BOuterArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BOuterArgs)
+  b_mg_outer_residual_q(vb, *a.g, *a.L0, a.p, a.b, a.b0, a.x0, a.ctl, a.pdd, a.pmx);
+}
+template <bool kUpdate>
+__global__ void __launch_bounds__(kBlock)
+kb_mg_smooth_q(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BSmoothArgs)
+  b_mg_smooth_q<kUpdate>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.ctl);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_mg_restrict_q(const BRestrictArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BRestrictArgs)
+  b_mg_restrict_q(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.ctl);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_mg_prolong_q(const BProlongArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BProlongArgs)
+  b_mg_prolong_q(vb, *a.F, *a.C, a.xc, a.xin, a.xout, a.ctl);
+}
+
+void launch_b_mg_outer_q(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_outer_residual_q<<<bl.blocks, block_shape(), 0, s>>>((const BOuterArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_smooth_q(const BatchLaunch& bl, bool update, hipStream_t s) {
+  if (!bl.blocks) return;
+  if (update)
+    kb_mg_smooth_q<true><<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
+  else
+    kb_mg_smooth_q<false><<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_restrict_q(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_restrict_q<<<bl.blocks, block_shape(), 0, s>>>((const BRestrictArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_prolong_q(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_prolong_q<<<bl.blocks, block_shape(), 0, s>>>((const BProlongArgs*)bl.args, bl.prefix, bl.n);
 }
 
 // ------------------------------------------------------------- launchers

@@ -136,9 +136,10 @@ __device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restric
   return den > (S)0 ? num / den : (S)0;
 }
 
-#define FOR_LEVEL_CELLS(L)                                                           \
+#define FOR_LEVEL_CELLS(L) FOR_LEVEL_CELLS_VB(L, hw_block())
+#define FOR_LEVEL_CELLS_VB(L, VB)                                                    \
   {                                                                                  \
-    const RowRange rr_ = block_rows((L).ny * (L).nz);                                \
+    const RowRange rr_ = block_rows((L).ny * (L).nz, (VB));                          \
     for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
       const int k = row / (L).ny, j = row - k * (L).ny;                              \
       for (int i = threadIdx.x; i < (L).nx; i += kLanesX) {                          \

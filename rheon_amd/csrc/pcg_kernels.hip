@@ -22,6 +22,7 @@
 // arithmetic) or float (halved traffic; scalars and arithmetic stay f64).
 #include "common.hpp"
 #include "kernels.hpp"
+#include "batch.hpp"
 #include "profiler.hpp"
 #include "stencil.hpp"
 
@@ -31,11 +32,6 @@ namespace rh {
 // this is the reference's division (src/pressure.rs:567).
 __device__ __forceinline__ double jdiv(double r, double d) { return d > 0.0 ? r / d : 0.0; }
 
-__device__ __forceinline__ void fail(PcgControl* ctl, int status, int it) {
-  ctl->done = 1;
-  ctl->status = status;
-  ctl->error_iteration = it;
-}
 
 // Which path a red-black stage belongs to: the speculative one right after
 // `update` (0) or the replacement / initial one after `replace` (1).
@@ -47,27 +43,6 @@ __device__ __forceinline__ bool sgs_active(const PcgControl* ctl, int it, int mo
 
 // -------------------------------------------------------------- finishers
 
-// Compatibility sums and threshold (src/pressure.rs:603-633).
-struct FinRhsSums {
-  PcgControl* ctl;
-  __device__ bool active() const { return true; }
-  __device__ void operator()(const double* sum, const double* mx) const {
-    if (mx[0] != 0.0) return fail(ctl, RH_NONFINITE_INPUT, 0);
-    const double total = sum[0], total_abs = sum[1];
-    ctl->sum = total;
-    ctl->sum_abs = total_abs;
-    const double budget = (64.0 * 2.220446049250313e-16) * total_abs;
-    if (!isfinite(total) || !isfinite(total_abs)) return fail(ctl, RH_BREAKDOWN, 0);
-    if (fabs(total) > budget) {
-      ctl->error_value = total;
-      return fail(ctl, RH_INCOMPATIBLE_RHS, 0);
-    }
-    const double rhs_l2 = sqrt(sum[2]);
-    const double threshold = fmax(ctl->abs_tol, ctl->rel * rhs_l2);
-    if (!isfinite(rhs_l2) || !isfinite(threshold)) return fail(ctl, RH_BREAKDOWN, 0);
-    ctl->threshold = threshold;
-  }
-};
 
 // curvature checks and alpha (src/pressure.rs:661-668).
 struct FinApplyDir {
@@ -158,18 +133,22 @@ __global__ void __launch_bounds__(kBlock) k_fill_zero(T* __restrict__ x, int n) 
 }
 
 // sum(b), sum(|b|), b.b and validate_cells (finite, src/operator.rs:840-848).
-__global__ void __launch_bounds__(kBlock)
-k_rhs_sums(DevGrid g, const double* __restrict__ b, DD* pdd, double* pmx) {
+__device__ __forceinline__ void b_rhs_sums(VBlock vb, const DevGrid& g, const double* __restrict__ b,
+                                           DD* pdd, double* pmx) {
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   double bad[1] = {0.0};
-  FOR_CELLS(g)
+  FOR_CELLS_VB(g, vb)
   const double v = b[c];
   if (!isfinite(v)) bad[0] = 1.0;
   dd_acc(s[0], v);
   dd_acc(s[1], fabs(v));
   dd_acc(s[2], v * v);
   END_CELLS
-  block_publish<3, 1>(s, bad, pdd, pmx);
+  block_publish<3, 1>(s, bad, pdd, pmx, vb);
+}
+__global__ void __launch_bounds__(kBlock)
+k_rhs_sums(DevGrid g, const double* __restrict__ b, DD* pdd, double* pmx) {
+  b_rhs_sums(hw_block(), g, b, pdd, pmx);
 }
 
 // q = K d, q[0] = 0 (gauge), partial d.q.
@@ -359,19 +338,79 @@ k_apply_full(DevGrid g, const double* __restrict__ x, double* __restrict__ y, un
 
 // Residual report of a supplied pressure (ProviderWorkspace qualification,
 // src/solver_provider.rs:246-254): sum (b - K p)^2, max|b - K p|, nonfinite.
-__global__ void __launch_bounds__(kBlock)
-k_residual_report(DevGrid g, const double* __restrict__ p, const double* __restrict__ b, DD* pdd,
-                  double* pmx) {
+__device__ __forceinline__ void b_residual_report(VBlock vb, const DevGrid& g,
+                                                  const double* __restrict__ p,
+                                                  const double* __restrict__ b, DD* pdd, double* pmx) {
   DD s[1] = {{0.0, 0.0}};
   double m[2] = {0.0, 0.0};
-  FOR_CELLS(g)
+  FOR_CELLS_VB(g, vb)
   const double v = apply_at<double>(g, p, c, i, j, k);
   if (!isfinite(v)) m[1] = 1.0;
   const double prod = b[c] - v;
   dd_acc(s[0], prod * prod);
   m[0] = fmax(m[0], fabs(prod));
   END_CELLS
-  block_publish<1, 2>(s, m, pdd, pmx);
+  block_publish<1, 2>(s, m, pdd, pmx, vb);
+}
+__global__ void __launch_bounds__(kBlock)
+k_residual_report(DevGrid g, const double* __restrict__ p, const double* __restrict__ b, DD* pdd,
+                  double* pmx) {
+  b_residual_report(hw_block(), g, p, b, pdd, pmx);
+}
+
+// ---------------------------------------------------------------- batched
+
+#define RH_BATCH_PROLOGUE(ARGS)                                  \
+  const int s_ = batch_scene(prefix, n, blockIdx.x);            \
+  const ARGS& a = A[s_];                                        \
+  const VBlock vb = batch_block(prefix, s_);
+
+__global__ void __launch_bounds__(kBlock)
+kb_rhs_sums(const BCellArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BCellArgs)
+  if (a.ctl->done) return;
+  b_rhs_sums(vb, *a.g, a.b, a.pdd, a.pmx);
+}
+// residual of the returned pressure: scenes whose solve ended RH_OK
+__global__ void __launch_bounds__(kBlock)
+kb_residual_report(const BCellArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BCellArgs)
+  if (a.ctl->status != RH_OK) return;
+  b_residual_report(vb, *a.g, a.p, a.b, a.pdd, a.pmx);
+}
+__global__ void __launch_bounds__(kBlock)
+kb_fill_pressure(const BCellArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BCellArgs)
+  const DevGrid& g = *a.g;
+  FOR_CELLS_VB(g, vb)
+  a.p[c] = 0.0;
+  END_CELLS
+}
+// p <- p - p[gauge] (captured in spare[0] when spare[1] != 0), solid cells 0
+__global__ void __launch_bounds__(kBlock)
+kb_regauge(const BCellArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BCellArgs)
+  if (a.ctl->spare[1] == 0.0) return;
+  const double p0 = a.ctl->spare[0];
+  const DevGrid& g = *a.g;
+  FOR_CELLS_VB(g, vb)
+  a.p[c] = is_fluid(g, c) ? a.p[c] - p0 : 0.0;
+  END_CELLS
+}
+
+void launch_b_rhs_sums(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks) kb_rhs_sums<<<bl.blocks, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_residual_report(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_residual_report<<<bl.blocks, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_fill_pressure(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_fill_pressure<<<bl.blocks, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_regauge(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks) kb_regauge<<<bl.blocks, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.prefix, bl.n);
 }
 
 // ------------------------------------------------------------ host launchers

@@ -65,6 +65,10 @@ class Solver {
   int gauge() const { return gauge_; }
   // Operator constants with this solver's obstacle mask and gauge.
   DevGrid dev_grid(double density) const;
+  // Changes whenever device-side operator data is replaced (obstacles,
+  // hierarchy uploads), so batched launch tables know to rebuild.
+  uint64_t generation() const { return generation_; }
+  const rh_solver_config& config() const { return config_; }
 
  protected:
   rh_status check_and_begin(double density, const double* d_rhs, double dt,
@@ -85,6 +89,7 @@ class Solver {
   std::vector<uint8_t> hmask_;      // host copy of the obstacle mask (empty = box)
   unsigned char* d_mask_ = nullptr;  // device obstacle mask
   int gauge_ = 0;
+  uint64_t generation_ = 0;
 };
 
 Solver* make_solver(rh_method method);
@@ -120,6 +125,11 @@ class MgSolver : public Solver {
   uint64_t allocated_bytes() const override;
   int levels() const { return h_.levels; }
   rh_status set_solid(const uint8_t* solid) override;
+  // Level data and coarsest inverse for `density` on the device (done by
+  // every solve; batched runners call it before building launch tables).
+  rh_status prepare(double density) { return set_density(density); }
+  const MgHierarchy& hierarchy() const { return h_; }
+  const MgHierarchy* device_hierarchy() const { return d_h_; }
 
  private:
   template <typename S>

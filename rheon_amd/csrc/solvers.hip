@@ -113,6 +113,7 @@ void Solver::release_mask() {
   d_mask_ = nullptr;
   hmask_.clear();
   gauge_ = 0;
+  ++generation_;
 }
 
 rh_status Solver::set_solid(const uint8_t* solid) {
@@ -128,6 +129,7 @@ rh_status Solver::set_solid(const uint8_t* solid) {
   RH_HIP(hipMemcpy(d_mask_, mask.data(), mask.size(), hipMemcpyHostToDevice));
   hmask_.swap(mask);
   gauge_ = gauge;
+  ++generation_;
   return RH_OK;
 }
 
@@ -579,6 +581,7 @@ rh_status MgSolver::set_density(double density) {
   RH_HIP(hipMemcpy(d_h_, &h_, sizeof(MgHierarchy), hipMemcpyHostToDevice));
   inv_density_ = density;
   dirty_ = false;
+  ++generation_;
   return RH_OK;
 }
 

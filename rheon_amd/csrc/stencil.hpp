@@ -7,9 +7,10 @@
 namespace rh {
 
 // Loop over this block's cells; defines (row, k, j, i, c) for the body.
-#define FOR_CELLS(g)                                                                 \
+#define FOR_CELLS(g) FOR_CELLS_VB(g, hw_block())
+#define FOR_CELLS_VB(g, VB)                                                          \
   {                                                                                  \
-    const RowRange rr_ = block_rows((g).ny * (g).nz);                                \
+    const RowRange rr_ = block_rows((g).ny * (g).nz, (VB));                          \
     for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
       const int k = row / (g).ny, j = row - k * (g).ny;                              \
       (void)k;                                                                       \

@@ -1,0 +1,94 @@
+"""Batched scenes: one launch per stage over many device-resident scenes
+(optionally one hipGraph replay per step).  Every scene's step must equal
+the one-scene step bit for bit: the batched kernels run the same bodies with
+the same per-scene block decomposition, so reductions and convergence
+decisions are identical."""
+import numpy as np
+import pytest
+
+from rheon_amd.scenes import GridSpec, centred_sphere, courant_dt, taylor_green_noise
+
+SHAPES = [(32, 32, 32), (40, 24, 16), (64, 32, 48), (48, 48, 48), (32, 64, 40)]
+
+
+def specs():
+    """Isotropic cells (h = 1/max n): boxes are non-cubic, cells are cubes."""
+    return [GridSpec(c, (1.0 / max(c),) * 3) for c in SHAPES]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_matches_single_scenes_bitwise(use_graph):
+    import rheon_amd as R
+    sp = specs()
+    st = R.PressureSettings(1e-8, 1e-14, 1e-3, 30)
+    batch = R.SceneBatch(sp, settings=st, use_graph=use_graph, actual_divergence_limit=1e-3)
+    singles, dts = [], []
+    solid = centred_sphere(sp[3], 0.2)
+    for i, s in enumerate(sp):
+        vel = taylor_green_noise(s, seed=100 + i)
+        dts.append(courant_dt(s, vel, 0.5))
+        sc = R.Scene(s, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e-3)
+        if i == 3:
+            sc.set_solid(solid)
+            batch.scene(i).set_solid(solid)
+        sc.set_velocity(vel)
+        batch.scene(i).set_velocity(vel)
+        singles.append(sc)
+    batch.set_dt(dts)
+    for _ in range(3):
+        reps = batch.step(1)
+        for i, sc in enumerate(singles):
+            r = sc.step(dts[i])
+            assert r.status == R.OK, (i, R.STATUS_NAMES.get(r.status))
+            assert reps[i].status == R.OK, (i, R.STATUS_NAMES.get(reps[i].status))
+            assert reps[i].pressure.iterations == r.pressure.iterations
+            assert reps[i].actual_divergence_max == r.actual_divergence_max
+            assert reps[i].pressure.true_residual_l2 == r.pressure.true_residual_l2
+    for i, sc in enumerate(singles):
+        a, b = batch.scene(i).get_velocity(), sc.get_velocity()
+        for k in range(3):
+            assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), (i, k)
+        assert np.array_equal(batch.scene(i).get_pressure().view(np.uint64), sc.get_pressure().view(np.uint64))
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_batch_failed_scene_keeps_its_faces():
+    import rheon_amd as R
+    sp = specs()[:3]
+    batch = R.SceneBatch(sp, use_graph=True)
+    vels = [taylor_green_noise(s, seed=7 + i) for i, s in enumerate(sp)]
+    bad = [v.copy() for v in vels[1]]
+    bad[0][sp[1].face_counts(0)[0] + 5] = np.nan  # an interior x-face
+    for i in range(3):
+        batch.scene(i).set_velocity(bad if i == 1 else vels[i])
+    batch.set_dt([courant_dt(s, v) for s, v in zip(sp, vels)])
+    reps = batch.step(1)
+    assert reps[0].status == R.OK and reps[2].status == R.OK
+    assert reps[1].status == R.NONFINITE_INPUT
+    got = batch.scene(1).get_velocity()
+    for k in range(3):
+        assert np.array_equal(got[k], bad[k], equal_nan=True)
+    # the others advanced
+    assert not np.array_equal(batch.scene(0).get_velocity()[0], vels[0][0])
+    batch.close()
+
+
+@pytest.mark.gpu
+def test_batch_scalar_variant_matches_default():
+    import rheon_amd as R
+    sp = specs()[:4]
+    out = []
+    for variant in (0, R.VARIANT_SCALAR_MG):
+        batch = R.SceneBatch(sp, use_graph=True, kernel_variant=variant)
+        vels = [taylor_green_noise(s, seed=21 + i) for i, s in enumerate(sp)]
+        for i in range(len(sp)):
+            batch.scene(i).set_velocity(vels[i])
+        batch.set_dt([courant_dt(s, v) for s, v in zip(sp, vels)])
+        reps = batch.step(2)
+        assert all(r.status == R.OK for r in reps)
+        out.append([batch.scene(i).get_pressure() for i in range(len(sp))])
+        batch.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
