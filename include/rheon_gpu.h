@@ -356,6 +356,21 @@ rh_status rh_batch_scene(rh_batch* batch, size_t i, rh_scene** out);
 rh_status rh_batch_set_dt(rh_batch* batch, const double* dt, size_t n);
 /* Kernel variant (RH_VARIANT_* bits) of the batched launches. */
 rh_status rh_batch_set_variant(rh_batch* batch, uint32_t kernel_variant);
+/* Launch shape of the batched stages: kernel variant and the divisor of
+ * each scene's one-scene block count (1 = the one-scene decomposition). */
+typedef struct rh_batch_tuning {
+  uint32_t kernel_variant;
+  int32_t block_divisor;
+  double seconds_per_step;  /* measured by rh_batch_autotune */
+} rh_batch_tuning;
+rh_status rh_batch_set_tuning(rh_batch* batch, const rh_batch_tuning* tuning);
+/* Autotuner: times every candidate shape for `steps` steps on the batch's
+ * own scenes (accepted faces saved before and restored after; pressure
+ * buffers hold the last trial's values), applies the fastest and returns it
+ * in `chosen`; `table` (table_len entries, may be NULL) receives every
+ * candidate tried and `tried` their number. */
+rh_status rh_batch_autotune(rh_batch* batch, int32_t steps, rh_batch_tuning* table,
+                            size_t table_len, size_t* tried, rh_batch_tuning* chosen);
 /* `steps` steps of every scene; reports/statuses (n entries each, may be
  * NULL) describe the last step.  A scene whose step fails keeps its
  * accepted faces (transactional, as rh_scene_step) and its status says why. */

@@ -601,8 +601,25 @@ class SceneBatch:
     def set_variant(self, kernel_variant: int) -> None:
         _check(lib().rh_batch_set_variant(self.handle, int(kernel_variant)), "rh_batch_set_variant")
 
+    def set_tuning(self, kernel_variant: int, block_divisor: int) -> None:
+        t = N.BatchTuning(int(kernel_variant), int(block_divisor), 0.0)
+        _check(lib().rh_batch_set_tuning(self.handle, C.byref(t)), "rh_batch_set_tuning")
+
+    def autotune(self, steps: int = 1):
+        """Time every candidate launch shape on these scenes; applies and
+        returns the fastest, with the table of all candidates."""
+        table = (N.BatchTuning * 16)()
+        tried = C.c_size_t()
+        best = N.BatchTuning()
+        _check(lib().rh_batch_autotune(self.handle, int(steps), table, 16, C.byref(tried), C.byref(best)),
+               "rh_batch_autotune")
+        rows = [{"kernel_variant": t.kernel_variant, "block_divisor": t.block_divisor,
+                 "seconds_per_step": t.seconds_per_step} for t in table[:tried.value]]
+        return {"kernel_variant": best.kernel_variant, "block_divisor": best.block_divisor,
+                "seconds_per_step": best.seconds_per_step}, rows
+
     def step(self, steps: int = 1):
-        """Run `steps` steps; returns (statuses, StepReports) of the last step."""
+        """Run `steps` steps; returns the StepReports of the last step."""
         n = len(self.specs)
         reps = (N.StepReport * n)()
         sts = (C.c_int * n)()

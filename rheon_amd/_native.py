@@ -121,6 +121,10 @@ class BatchConfig(C.Structure):
     ]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("kernel_variant", C.c_uint32), ("block_divisor", C.c_int32), ("seconds_per_step", C.c_double)]
+
+
 class StepReport(C.Structure):
     _fields_ = [
         ("dt", C.c_double),
@@ -198,6 +202,9 @@ SIGNATURES = {
     "rh_batch_set_dt": (_S, [_P, _P, C.c_size_t]),
     "rh_batch_set_variant": (_S, [_P, C.c_uint32]),
     "rh_batch_step": (_S, [_P, C.c_int32, C.POINTER(StepReport), C.POINTER(C.c_int)]),
+    "rh_batch_set_tuning": (_S, [_P, C.POINTER(BatchTuning)]),
+    "rh_batch_autotune": (_S, [_P, C.c_int32, C.POINTER(BatchTuning), C.c_size_t, C.POINTER(C.c_size_t),
+                               C.POINTER(BatchTuning)]),
 }
 
 _lib = None

@@ -245,6 +245,8 @@ class Batch {
   rh_status set_dt_all(const double* dt, size_t n);
   const rh_batch_config& config() const { return cfg_; }
   rh_status set_variant(uint32_t variant);
+  rh_status set_tuning(const rh_batch_tuning& t);
+  rh_status autotune(int steps, rh_batch_tuning* table, size_t cap, size_t* count, rh_batch_tuning* chosen);
 
  private:
   rh_status build();
@@ -257,6 +259,7 @@ class Batch {
   std::vector<uint64_t> gens_;
   rh_batch_config cfg_{};
   uint32_t variant_ = 0;
+  int block_div_ = 1;  // each scene's blocks per stage = its one-scene count / block_div_
   hipStream_t stream_ = nullptr;
   DevGrid* d_grids_ = nullptr;
   BatchScene* d_scenes_ = nullptr;
@@ -329,6 +332,77 @@ rh_status Batch::set_dt_all(const double* dt, size_t n) {
   return RH_OK;
 }
 
+rh_status Batch::set_tuning(const rh_batch_tuning& t) {
+  if (t.block_divisor < 1 || t.block_divisor > 64) return RH_INVALID_ARGUMENT;
+  if (t.kernel_variant != variant_ || t.block_divisor != block_div_) built_ = false;
+  variant_ = t.kernel_variant;
+  block_div_ = t.block_divisor;
+  return RH_OK;
+}
+
+// Times every candidate launch shape on this batch's own scenes and keeps
+// the fastest.  Candidates: MG kernel variant (four cells per lane where a
+// level allows / one cell per lane) x blocks per scene (the one-scene count
+// divided by 1, 2, 4, 8).  Each candidate starts from the same accepted
+// faces (saved first, restored after every run), runs one untimed step
+// (table build and graph capture) and `steps` timed steps.
+rh_status Batch::autotune(int steps, rh_batch_tuning* table, size_t cap, size_t* count,
+                          rh_batch_tuning* chosen) {
+  const size_t n = scenes_.size();
+  if (steps < 1) steps = 1;
+  for (size_t i = 0; i < n; ++i)
+    if (!(dt_[i] > 0.0)) return RH_INVALID_TIME_STEP;
+  size_t total = 0;
+  for (size_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) total += scenes_[i]->flen[a] * 4;
+  void* save = nullptr;
+  RH_HIP(hipMalloc(&save, total));
+  auto copy_faces = [&](bool restore) -> rh_status {
+    size_t off = 0;
+    for (size_t i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const size_t b = scenes_[i]->flen[a] * 4;
+        char* keep = (char*)save + off;
+        RH_HIP(hipMemcpyAsync(restore ? scenes_[i]->acc[a] : keep, restore ? keep : scenes_[i]->acc[a], b,
+                              hipMemcpyDeviceToDevice, stream_));
+        off += b;
+      }
+    RH_HIP(hipStreamSynchronize(stream_));
+    return RH_OK;
+  };
+  rh_status st = copy_faces(false);
+  std::vector<rh_batch_tuning> tried;
+  const uint32_t variants[2] = {0u, RH_VARIANT_SCALAR_MG};
+  const int divisors[4] = {1, 2, 4, 8};
+  for (uint32_t v : variants)
+    for (int d : divisors) {
+      if (st) break;
+      rh_batch_tuning t{v, d, 0.0};
+      if ((st = set_tuning(t))) break;
+      if ((st = copy_faces(true))) break;
+      if ((st = step(1, nullptr, nullptr))) break;  // build + capture
+      if ((st = copy_faces(true))) break;
+      const auto t0 = std::chrono::steady_clock::now();
+      if ((st = step(steps, nullptr, nullptr))) break;
+      t.seconds_per_step =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / steps;
+      tried.push_back(t);
+    }
+  rh_status rs = copy_faces(true);
+  (void)hipFree(save);
+  if (st) return st;
+  if (rs) return rs;
+  if (tried.empty()) return RH_EXECUTION;
+  const rh_batch_tuning best = *std::min_element(
+      tried.begin(), tried.end(),
+      [](const rh_batch_tuning& a, const rh_batch_tuning& b) { return a.seconds_per_step < b.seconds_per_step; });
+  if (count) *count = tried.size();
+  if (table)
+    for (size_t i = 0; i < tried.size() && i < cap; ++i) table[i] = tried[i];
+  if (chosen) *chosen = best;
+  return set_tuning(best);
+}
+
 rh_status Batch::set_variant(uint32_t variant) {
   if (variant == variant_) return RH_OK;
   variant_ = variant;
@@ -386,10 +460,9 @@ rh_status Batch::build() {
   Tables T;
   program_.clear();
   const bool quad_ok = (variant_ & RH_VARIANT_SCALAR_MG) == 0;
-  auto face_blocks = [&](size_t i) {
-    return grid_blocks((int)((grids[i].ny + 1) * (grids[i].nz + 1)));
-  };
-  auto cell_blocks = [&](size_t i) { return grid_blocks(grids[i].ny * grids[i].nz); };
+  auto nblk = [&](int rows) { return std::max(1, grid_blocks(rows) / block_div_); };
+  auto face_blocks = [&](size_t i) { return nblk((int)((grids[i].ny + 1) * (grids[i].nz + 1))); };
+  auto cell_blocks = [&](size_t i) { return nblk(grids[i].ny * grids[i].nz); };
   // a stage: records for all scenes + blocks per scene (0 = not taking part)
   auto stage = [&](Op op, const void* recs, size_t rec_size, const std::vector<int>& blocks, int extra = 0) {
     std::vector<int> prefix(n + 1, 0);
@@ -552,7 +625,7 @@ rh_status Batch::build() {
         r[i] = BSmoothArgs{&dh->lv[l], (const float*)(cx ? h.x[l] : h.y[l]), (const float*)h.b[l],
                            (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
                            scenes_[i]->solver->impl->scratch().ctl};
-        const int blocks = grid_blocks(h.lv[l].ny * h.lv[l].nz);
+        const int blocks = nblk(h.lv[l].ny * h.lv[l].nz);
         const bool q = quad_ok && mg_quad_level(h.lv[l]);
         (q ? bq : bs)[i] = blocks;
         if (!to_p) in_x[i][l] = !cx;
@@ -572,8 +645,8 @@ rh_status Batch::build() {
         r[i] = BRestrictArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)(in_x[i][l] ? h.x[l] : h.y[l]),
                              (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1],
                              scenes_[i]->solver->impl->scratch().ctl};
-        const int blocks = grid_blocks(h.lv[l + 1].ny * h.lv[l + 1].nz);
-        (quad_ok && mg_quad_level(h.lv[l]) ? bq : bs)[i] = blocks;
+        const int blocks = nblk(h.lv[l + 1].ny * h.lv[l + 1].nz);
+        (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]) ? bq : bs)[i] = blocks;
         in_x[i][l + 1] = true;
       }
       stage(Op::kRestrictQ, r.data(), sizeof(BRestrictArgs), bq);
@@ -614,8 +687,8 @@ rh_status Batch::build() {
         r[i] = BProlongArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)h.x[l + 1],
                             (const float*)(cx ? h.x[l] : h.y[l]), (float*)(cx ? h.y[l] : h.x[l]),
                             scenes_[i]->solver->impl->scratch().ctl};
-        const int blocks = grid_blocks(h.lv[l].ny * h.lv[l].nz);
-        (quad_ok && mg_quad_level(h.lv[l]) ? bq : bs)[i] = blocks;
+        const int blocks = nblk(h.lv[l].ny * h.lv[l].nz);
+        (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]) ? bq : bs)[i] = blocks;
         in_x[i][l] = !cx;
       }
       stage(Op::kProlongQ, r.data(), sizeof(BProlongArgs), bq);
@@ -857,4 +930,15 @@ extern "C" rh_status rh_batch_step(rh_batch* batch, int32_t steps, rh_step_repor
                                    rh_status* statuses) {
   if (!batch) return RH_INVALID_ARGUMENT;
   return batch->impl.step(steps, reports, statuses);
+}
+
+extern "C" rh_status rh_batch_set_tuning(rh_batch* batch, const rh_batch_tuning* tuning) {
+  if (!batch || !tuning) return RH_INVALID_ARGUMENT;
+  return batch->impl.set_tuning(*tuning);
+}
+
+extern "C" rh_status rh_batch_autotune(rh_batch* batch, int32_t steps, rh_batch_tuning* table,
+                                       size_t table_len, size_t* tried, rh_batch_tuning* chosen) {
+  if (!batch) return RH_INVALID_ARGUMENT;
+  return batch->impl.autotune(steps, table, table_len, tried, chosen);
 }

@@ -64,6 +64,7 @@ struct MgHierarchy {
 // Four-cells-per-lane FP32 kernels (levels with nx % 4 == 0 and no face
 // fractions); identical values to the scalar kernels.
 bool mg_quad_level(const MgLevel& L);
+bool mg_quad_pair(const MgLevel& F, const MgLevel& C);  // restriction / prolongation F <-> C
 bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, const double* b);
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
                                 float* b0, float* x0, const PcgControl* ctl, DD* pdd, double* pmx,

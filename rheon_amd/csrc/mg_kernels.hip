@@ -104,6 +104,7 @@ __device__ __forceinline__ void b_mg_restrict(VBlock vb, const MgLevel& F, const
 #pragma unroll
       for (int dx = 0; dx < 2; ++dx) {
         const int fi = 2 * i + dx, fj = 2 * j + dy, fk = 2 * k + dz;
+        if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;  // padded coarse cell
         const int fc = fi + F.nx * (fj + F.ny * fk);
         sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
       }
@@ -281,6 +282,7 @@ __device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& C
       for (int dy = 0; dy < 2; ++dy)
         for (int dx = 0; dx < 2; ++dx) {
           const int fi = 2 * i + dx, fj = 2 * j + dy, fk = 2 * k + dz;
+          if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;  // padded coarse cell
           const int fc = fi + F.nx * (fj + F.ny * fk);
           sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
         }
@@ -540,7 +542,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   }
   {
     prof::Scope ps(mg_slot(l, prof::kMgRestrict), s, l == 0 ? it : -2);
-    if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]))
+    if (sizeof(S) == 4 && h.quad && mg_quad_pair(h.lv[l], h.lv[l + 1]))
       launch_mg_restrict_q(h.lv[l], h.lv[l + 1], (const float*)cur, (const float*)b, (float*)h.b[l + 1],
                            (float*)h.x[l + 1], ctl, s);
     else
@@ -550,7 +552,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   vcycle<S>(h, l + 1, p, ctl, it, s);
   {
     prof::Scope ps(mg_slot(l, prof::kMgProlong), s, l == 0 ? it : -2);
-    if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]))
+    if (sizeof(S) == 4 && h.quad && mg_quad_pair(h.lv[l], h.lv[l + 1]))
       launch_mg_prolong_q(h.lv[l], h.lv[l + 1], (const float*)h.x[l + 1], (const float*)cur, (float*)nxt,
                           ctl, s);
     else

@@ -406,6 +406,9 @@ void launch_b_mg_prolong_q(const BatchLaunch& bl, hipStream_t s) {
 // ------------------------------------------------------------- launchers
 
 bool mg_quad_level(const MgLevel& L) { return L.nx % 4 == 0 && !L.frac; }
+bool mg_quad_pair(const MgLevel& F, const MgLevel& C) {
+  return mg_quad_level(F) && F.nx == 2 * C.nx && F.ny == 2 * C.ny && F.nz == 2 * C.nz;
+}
 
 static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 

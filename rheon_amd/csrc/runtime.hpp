@@ -136,7 +136,9 @@ class MgSolver : public Solver {
   rh_status run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
                 double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
   rh_status set_density(double density);
+  rh_status build_fractions();
   MgHierarchy h_;
+  int padded_level_ = 0;  // first level made by padded (ceil) coarsening; 0 = none
   unsigned* d_frac_ = nullptr;                // coarse-level face fractions (device)
   std::vector<std::vector<unsigned>> hfrac_;  // host copies per level (index = level)
   double fscale_[kMgMaxLevels] = {};

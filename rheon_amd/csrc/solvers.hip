@@ -252,16 +252,27 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
   h_.lv[0].ny = n[1];
   h_.lv[0].nz = n[2];
   const int want = config.mg_levels > 0 ? std::min(config.mg_levels, kMgMaxLevels) : kMgMaxLevels;
+  // A level with an odd axis that is still too large for the dense
+  // coarsest inverse is coarsened with ceil(n/2) cells per axis: the last
+  // coarse cell of an odd axis has one child, and the missing children act
+  // as solid cells (the coarse levels from there on carry face fractions).
+  padded_level_ = 0;
   while (levels < want) {
     const int* m = &h_.lv[levels - 1].nx;
     const long cells = (long)m[0] * m[1] * m[2];
     const bool even = (m[0] % 2 == 0) && (m[1] % 2 == 0) && (m[2] % 2 == 0);
     const int mn = std::min(m[0], std::min(m[1], m[2]));
-    if (!even || mn < 4) break;
-    if (config.mg_levels <= 0 && cells <= 64) break;
-    h_.lv[levels].nx = m[0] / 2;
-    h_.lv[levels].ny = m[1] / 2;
-    h_.lv[levels].nz = m[2] / 2;
+    const int mx = std::max(m[0], std::max(m[1], m[2]));
+    if (even) {
+      if (mn < 4) break;
+      if (config.mg_levels <= 0 && cells <= 64) break;
+    } else {
+      if (config.mg_levels > 0 || cells <= kMgDenseMax || mx < 3) break;
+      if (!padded_level_) padded_level_ = levels;
+    }
+    h_.lv[levels].nx = (m[0] + 1) / 2;
+    h_.lv[levels].ny = (m[1] + 1) / 2;
+    h_.lv[levels].nz = (m[2] + 1) / 2;
     ++levels;
   }
   h_.levels = levels;
@@ -288,6 +299,7 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
     h_.b[l] = p;
     p += sz;
   }
+  if (padded_level_) return build_fractions();
   return RH_OK;
 }
 
@@ -411,7 +423,15 @@ static bool dense_coarse_inverse(const LevelFaces& F, std::vector<double>& G) {
 rh_status MgSolver::set_solid(const uint8_t* solid) {
   rh_status st = Solver::set_solid(solid);
   if (st) return st;
+  return build_fractions();
+}
+
+// Level-0 mask and coarse face fractions: from the obstacle mask, or from
+// the plain box when only padded coarsening needs them (levels >= the first
+// padded one then carry fractions; the finer ones keep the box rule).
+rh_status MgSolver::build_fractions() {
   dirty_ = true;
+  ++generation_;
   hfrac_.assign(h_.levels, {});
   for (int l = 0; l < h_.levels; ++l) {
     h_.lv[l].mask = nullptr;
@@ -419,28 +439,41 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
     h_.lv[l].octant = nullptr;
     fscale_[l] = 0.0;
   }
-  if (!d_mask_) return RH_OK;
-  h_.lv[0].mask = d_mask_;
+  if (d_frac_) (void)hipFree(d_frac_);
+  d_frac_ = nullptr;
+  if (!d_mask_ && !padded_level_) return RH_OK;
+  if (d_mask_) h_.lv[0].mask = d_mask_;
   if (h_.levels == 1) return RH_OK;
+  const int first_frac = d_mask_ ? 1 : padded_level_;
+  std::vector<uint8_t> box;
+  if (!d_mask_) {  // open faces of the plain box, every cell fluid
+    const MgLevel& L = h_.lv[0];
+    box.assign((size_t)L.nx * L.ny * L.nz, 0);
+    for (int k = 0; k < L.nz; ++k)
+      for (int j = 0; j < L.ny; ++j)
+        for (int i = 0; i < L.nx; ++i)
+          box[i + (size_t)L.nx * (j + (size_t)L.ny * k)] =
+              (uint8_t)(kFluidBit | (i > 0) | ((i + 1 < L.nx) << 1) | ((j > 0) << 2) | ((j + 1 < L.ny) << 3) |
+                        ((k > 0) << 4) | ((k + 1 < L.nz) << 5));
+  }
+  const std::vector<uint8_t>& mask0 = d_mask_ ? hmask_ : box;
   // open sub-face counts of each level's three lower faces, fluid flags
   std::vector<uint32_t> cnt[3];
   std::vector<uint8_t> fluid;
   {
-    const size_t n = hmask_.size();
+    const size_t n = mask0.size();
     for (int a = 0; a < 3; ++a) {
       cnt[a].resize(n);
-      for (size_t c = 0; c < n; ++c) cnt[a][c] = (hmask_[c] >> (2 * a)) & 1u;
+      for (size_t c = 0; c < n; ++c) cnt[a][c] = (mask0[c] >> (2 * a)) & 1u;
     }
     fluid.resize(n);
-    for (size_t c = 0; c < n; ++c) fluid[c] = (hmask_[c] & kFluidBit) ? 1 : 0;
+    for (size_t c = 0; c < n; ++c) fluid[c] = (mask0[c] & kFluidBit) ? 1 : 0;
   }
   size_t total = 0;
   for (int l = 1; l < h_.levels; ++l) {
     const size_t nc = (size_t)h_.lv[l].nx * h_.lv[l].ny * h_.lv[l].nz;
     total += (nc + 63) / 64 * 64 + (nc + 255) / 256 * 64;  // words + octant bytes
   }
-  if (d_frac_) (void)hipFree(d_frac_);
-  d_frac_ = nullptr;
   RH_HIP(hipMalloc(&d_frac_, sizeof(unsigned) * total));
   size_t offset = 0;
   for (int l = 1; l < h_.levels; ++l) {
@@ -457,6 +490,7 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
           for (int dz = 0; dz < 2; ++dz)
             for (int dy = 0; dy < 2; ++dy)
               for (int dx = 0; dx < 2; ++dx) {
+                if (2 * I + dx >= F.nx || 2 * J + dy >= F.ny || 2 * K + dz >= F.nz) continue;  // padding
                 const size_t f =
                     (2 * I + dx) + (size_t)F.nx * ((2 * J + dy) + (size_t)F.ny * (2 * K + dz));
                 cf[c] |= fluid[f];
@@ -504,10 +538,14 @@ rh_status MgSolver::set_solid(const uint8_t* solid) {
           oct[I + (size_t)C.nx * (J + (size_t)C.ny * K)] = bits;
         }
     RH_HIP(hipMemcpy(d_frac_ + offset, q.data(), sizeof(unsigned) * nc, hipMemcpyHostToDevice));
-    h_.lv[l].frac = d_frac_ + offset;
     unsigned char* d_oct = (unsigned char*)(d_frac_ + offset + (nc + 63) / 64 * 64);
     RH_HIP(hipMemcpy(d_oct, oct.data(), nc, hipMemcpyHostToDevice));
-    h_.lv[l].octant = d_oct;
+    if (l >= first_frac) {
+      h_.lv[l].frac = d_frac_ + offset;
+      h_.lv[l].octant = d_oct;
+    } else {
+      q.clear();  // a box level: the host view keeps the box rule too
+    }
     offset += (nc + 63) / 64 * 64 + (nc + 255) / 256 * 64;
     hfrac_[l].swap(q);
     for (int a = 0; a < 3; ++a) cnt[a].swap(cc[a]);
@@ -546,7 +584,7 @@ rh_status MgSolver::set_density(double density) {
   std::vector<double> inv;
   const int cl = h_.levels - 1;
   const LevelFaces faces{last, (cl == 0 && !hmask_.empty()) ? &hmask_ : nullptr,
-                         (cl > 0 && !hmask_.empty()) ? &hfrac_[cl] : nullptr, fscale_[cl]};
+                         (cl > 0 && h_.lv[cl].frac) ? &hfrac_[cl] : nullptr, fscale_[cl]};
   if (n <= kMgDenseMax && dense_coarse_inverse(faces, inv)) {
     if (!d_inv_) RH_HIP(hipMalloc(&d_inv_, sizeof(double) * (size_t)kMgDenseMax * kMgDenseMax));
     RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));

@@ -246,6 +246,20 @@ def close_obstacle_faces(spec: GridSpec, solid: np.ndarray, fields):
     return out
 
 
+def log_uniform_sizes(seed: int, count: int, lo: int, hi: int, multiple: int = 8):
+    """``count`` grid shapes, each axis log-uniform on [lo, hi] and rounded
+    to a multiple of ``multiple`` (draws from the LCG of ``lcg_uniform``)."""
+    u = lcg_uniform(seed, 3 * count) + 0.5
+    n = np.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))
+    n = np.clip(np.rint(n / multiple) * multiple, lo, hi).astype(int)
+    return [tuple(int(v) for v in n[3 * i:3 * i + 3]) for i in range(count)]
+
+
+def isotropic_spec(counts) -> GridSpec:
+    """Cubic cells of side 1/max(counts): the longest box side is 1."""
+    return GridSpec(tuple(counts), (1.0 / max(counts),) * 3)
+
+
 def courant_dt(spec: GridSpec, fields, courant: float = 0.5) -> float:
     """dt = courant * h / max|u| (the explicit step of the workload recipes)."""
     return courant * min(spec.spacing) / max_abs(fields)

@@ -148,3 +148,34 @@ def test_device_mg_quad_kernels_match_scalar(counts, sphere):
         ws.close()
     assert out[0][0] == out[1][0]
     assert np.array_equal(out[0][1].view(np.uint64), out[1][1].view(np.uint64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("counts,sphere", [((44, 28, 60), False), ((44, 28, 60), True), ((33, 17, 7), False)])
+def test_device_mg_padded_coarsening(counts, sphere):
+    """Odd axes on a level too large for the dense coarsest inverse are
+    coarsened with ceil(n/2) (the missing children act as solid cells); the
+    solve still reaches the reference method's converged pressure."""
+    from rheon_amd.scenes import centred_sphere, close_obstacle_faces, isotropic_spec
+    spec = isotropic_spec(counts)
+    vel = fourier_modes(spec, 13, modes=8, kmax=6, dtype=np.float32)
+    solid = centred_sphere(spec, 0.15) if sphere else None
+    if solid is not None:
+        vel = close_obstacle_faces(spec, solid, vel)
+    dt = courant_dt(spec, vel)
+    g = ogrid(spec)
+    rhs = O.build_rhs(g, vel, dt)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32)
+    if solid is not None:
+        ws.set_solid(solid)
+    rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-10, 1e-14, 1e-3, 40))
+    assert rep.ok, R.STATUS_NAMES.get(rep.status)
+    assert rep.iterations <= 30, rep.iterations
+    p = ws.get_pressure()
+    if solid is None:
+        ref = O.solve_rhs(g, 1.0, rhs, dt, O.Settings(1e-12, 1e-16, 1e-3, 4000), "rbsgs-pcg")
+    else:
+        ob = O.obstacle_mask(g, solid)
+        ref = O.solve_rhs_masked(g, 1.0, ob, rhs, dt, O.Settings(1e-12, 1e-16, 1e-3, 4000), "rbsgs-pcg")
+    assert ref.status == O.OK
+    assert rel_err(p, ref.pressure) < 1e-7
