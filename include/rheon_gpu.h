@@ -377,6 +377,64 @@ rh_status rh_batch_autotune(rh_batch* batch, int32_t steps, rh_batch_tuning* tab
 rh_status rh_batch_step(rh_batch* batch, int32_t steps, rh_step_report* reports,
                         rh_status* statuses);
 
+/* ------------------------------------------------------------ serving */
+/* Continuous-batching server over device-resident scenes (no counterpart in
+ * the reference).  Requests are admitted in arrival order while their
+ * planned device bytes fit hbm_budget (and max_active); each tick runs one
+ * batched step (rh_batch) of every running request, retires finished ones
+ * (results kept on the host) and admits queued ones, whose set-up overlaps
+ * the device's step.  A request's result does not depend on the mix. */
+typedef struct rh_server rh_server;
+typedef struct rh_server_config {
+  rh_scene_config scene;     /* f32 faces, RH_MIXED_MG with f32 V-cycles */
+  uint64_t hbm_budget;       /* bytes of scene state held at once (0 = 80% of free) */
+  int32_t max_active;        /* running requests at once (0 = memory-bound only) */
+  int32_t block_divisor;     /* batched launch shape (rh_batch_tuning; 0 = 1) */
+  uint32_t kernel_variant;
+  int32_t reserved;
+} rh_server_config;
+typedef struct rh_request {
+  rh_grid grid;
+  const float* u;            /* initial faces, host (copied at submit) */
+  const float* v;
+  const float* w;
+  const uint8_t* solid;      /* solid cell flags or NULL */
+  double dt;
+  int32_t steps;
+  int32_t reserved;
+} rh_request;
+typedef struct rh_request_state {
+  int32_t state;             /* 0 queued, 1 running, 2 done, 3 failed */
+  rh_status status;
+  int32_t steps_done;
+  int32_t reserved;
+  double submitted, admitted, finished;  /* seconds since server creation */
+  uint64_t bytes;            /* planned device bytes while running */
+  rh_step_report last;       /* report of the last step taken */
+} rh_request_state;
+typedef struct rh_tick_report {
+  int32_t stepped;           /* requests that took a step this tick */
+  int32_t admitted;          /* requests admitted (they step from the next tick) */
+  int32_t finished;          /* requests retired (done or failed) */
+  int32_t active;            /* running after the tick */
+  int32_t queued;
+  int32_t reserved;
+  int64_t cells;             /* cells stepped */
+  uint64_t bytes_in_use;
+  double seconds;            /* host wall time of the tick */
+  double device_seconds;     /* device time of the batched step */
+} rh_tick_report;
+rh_status rh_server_create(const rh_server_config* config, rh_server** out);
+void rh_server_destroy(rh_server* server);
+rh_status rh_server_submit(rh_server* server, const rh_request* request, int64_t* id);
+rh_status rh_server_tick(rh_server* server, rh_tick_report* report);
+rh_status rh_server_state(rh_server* server, int64_t id, rh_request_state* out);
+/* Final faces (host, scene precision f32) and pressure of a finished request. */
+rh_status rh_server_result(rh_server* server, int64_t id, float* u, float* v, float* w,
+                           double* pressure);
+/* Planned device bytes of one request's scene (the admission figure). */
+uint64_t rh_server_plan_bytes(const rh_grid* grid, const rh_scene_config* config);
+
 #ifdef __cplusplus
 }
 #endif

@@ -608,10 +608,10 @@ class SceneBatch:
     def autotune(self, steps: int = 1):
         """Time every candidate launch shape on these scenes; applies and
         returns the fastest, with the table of all candidates."""
-        table = (N.BatchTuning * 16)()
+        table = (N.BatchTuning * 32)()
         tried = C.c_size_t()
         best = N.BatchTuning()
-        _check(lib().rh_batch_autotune(self.handle, int(steps), table, 16, C.byref(tried), C.byref(best)),
+        _check(lib().rh_batch_autotune(self.handle, int(steps), table, 32, C.byref(tried), C.byref(best)),
                "rh_batch_autotune")
         rows = [{"kernel_variant": t.kernel_variant, "block_divisor": t.block_divisor,
                  "seconds_per_step": t.seconds_per_step} for t in table[:tried.value]]
@@ -636,6 +636,78 @@ class SceneBatch:
             sc.handle = None
         if self.handle:
             lib().rh_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+REQUEST_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
+class ProjectionServer:
+    """Continuous-batching serving loop over device-resident scenes (see
+    rh_server_* in include/rheon_gpu.h): submit() queues a projection run,
+    tick() runs one batched step of every running request, admits queued
+    ones by planned device bytes and retires finished ones."""
+
+    def __init__(self, density: float = 1.0, settings: PressureSettings | None = None,
+                 actual_divergence_limit: float = 1e-6, max_courant: float = 0.0, hbm_budget: int = 0,
+                 max_active: int = 0, block_divisor: int = 1, kernel_variant: int = 0):
+        settings = settings or PressureSettings(1e-8, 1e-14, 5e-7, 40)
+        self.scene_config = N.SceneConfig(
+            density, F32, N.SolverConfig(MIXED_MG, F32, 0, 0, 0, 0.0, 0, kernel_variant, 0), settings.native(),
+            actual_divergence_limit, max_courant)
+        cfg = N.ServerConfig(self.scene_config, int(hbm_budget), int(max_active), int(block_divisor),
+                             int(kernel_variant), 0)
+        h = C.c_void_p()
+        _check(lib().rh_server_create(C.byref(cfg), C.byref(h)), "rh_server_create")
+        self.handle = h.value
+        self.specs = {}
+
+    def plan_bytes(self, spec: GridSpec) -> int:
+        g = make_grid(spec)
+        return int(lib().rh_server_plan_bytes(C.byref(g), C.byref(self.scene_config)))
+
+    def submit(self, spec: GridSpec, vel, dt: float, steps: int, solid=None) -> int:
+        arrs = [np.ascontiguousarray(v, np.float32) for v in vel]
+        for a in range(3):
+            assert arrs[a].size == spec.face_len(a)
+        sol = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
+        req = N.Request(make_grid(spec), arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data,
+                        None if sol is None else sol.ctypes.data, float(dt), int(steps), 0)
+        rid = C.c_int64()
+        _check(lib().rh_server_submit(self.handle, C.byref(req), C.byref(rid)), "rh_server_submit")
+        self.specs[rid.value] = spec
+        return rid.value
+
+    def tick(self) -> dict:
+        t = N.TickReport()
+        _check(lib().rh_server_tick(self.handle, C.byref(t)), "rh_server_tick")
+        return {k: getattr(t, k) for k, _ in N.TickReport._fields_ if k != "reserved"}
+
+    def state(self, rid: int) -> dict:
+        s = N.RequestState()
+        _check(lib().rh_server_state(self.handle, int(rid), C.byref(s)), "rh_server_state")
+        return {"state": REQUEST_STATES[s.state], "status": s.status, "steps_done": s.steps_done,
+                "submitted": s.submitted, "admitted": s.admitted, "finished": s.finished, "bytes": s.bytes,
+                "actual_divergence_max": s.last.actual_divergence_max,
+                "iterations": s.last.pressure.iterations}
+
+    def result(self, rid: int):
+        spec = self.specs[rid]
+        out = [np.empty(spec.face_len(a), np.float32) for a in range(3)]
+        p = np.empty(spec.cell_len, np.float64)
+        _check(lib().rh_server_result(self.handle, int(rid), out[0].ctypes.data, out[1].ctypes.data,
+                                      out[2].ctypes.data, p.ctypes.data), "rh_server_result")
+        return out, p
+
+    def close(self):
+        if self.handle:
+            lib().rh_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -139,6 +139,60 @@ class StepReport(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("scene", SceneConfig),
+        ("hbm_budget", C.c_uint64),
+        ("max_active", C.c_int32),
+        ("block_divisor", C.c_int32),
+        ("kernel_variant", C.c_uint32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("grid", Grid),
+        ("u", C.c_void_p),
+        ("v", C.c_void_p),
+        ("w", C.c_void_p),
+        ("solid", C.c_void_p),
+        ("dt", C.c_double),
+        ("steps", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class RequestState(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("status", C.c_int),
+        ("steps_done", C.c_int32),
+        ("reserved", C.c_int32),
+        ("submitted", C.c_double),
+        ("admitted", C.c_double),
+        ("finished", C.c_double),
+        ("bytes", C.c_uint64),
+        ("last", StepReport),
+    ]
+
+
+class TickReport(C.Structure):
+    _fields_ = [
+        ("stepped", C.c_int32),
+        ("admitted", C.c_int32),
+        ("finished", C.c_int32),
+        ("active", C.c_int32),
+        ("queued", C.c_int32),
+        ("reserved", C.c_int32),
+        ("cells", C.c_int64),
+        ("bytes_in_use", C.c_uint64),
+        ("seconds", C.c_double),
+        ("device_seconds", C.c_double),
+    ]
+
+
+
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 _P = C.c_void_p
@@ -203,6 +257,13 @@ SIGNATURES = {
     "rh_batch_set_variant": (_S, [_P, C.c_uint32]),
     "rh_batch_step": (_S, [_P, C.c_int32, C.POINTER(StepReport), C.POINTER(C.c_int)]),
     "rh_batch_set_tuning": (_S, [_P, C.POINTER(BatchTuning)]),
+    "rh_server_create": (_S, [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "rh_server_destroy": (None, [_P]),
+    "rh_server_submit": (_S, [_P, C.POINTER(Request), C.POINTER(C.c_int64)]),
+    "rh_server_tick": (_S, [_P, C.POINTER(TickReport)]),
+    "rh_server_state": (_S, [_P, C.c_int64, C.POINTER(RequestState)]),
+    "rh_server_result": (_S, [_P, C.c_int64, _P, _P, _P, _P]),
+    "rh_server_plan_bytes": (C.c_uint64, [C.POINTER(Grid), C.POINTER(SceneConfig)]),
     "rh_batch_autotune": (_S, [_P, C.c_int32, C.POINTER(BatchTuning), C.c_size_t, C.POINTER(C.c_size_t),
                                C.POINTER(BatchTuning)]),
 }

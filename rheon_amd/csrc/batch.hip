@@ -239,7 +239,13 @@ class Batch {
   ~Batch();
   rh_status create(const rh_grid* grids, size_t n, const rh_batch_config& cfg);
   rh_status step(int steps, rh_step_report* reports, rh_status* statuses);
-  rh_scene* scene(size_t i) { return i < scenes_.size() ? scenes_[i].get() : nullptr; }
+  rh_status launch(int steps);  // enqueue (tables rebuilt if needed), no wait
+  rh_status wait(rh_step_report* reports, rh_status* statuses);
+  double last_seconds() const { return last_seconds_; }  // device time of the last launch
+  rh_scene* scene(size_t i) { return i < scenes_.size() ? scenes_[i] : nullptr; }
+  // Non-owning membership (serving): the batch steps exactly these scenes.
+  rh_status set_members(const std::vector<rh_scene*>& scenes, const std::vector<double>& dt);
+  rh_status init_empty(const rh_batch_config& cfg);
   size_t size() const { return scenes_.size(); }
   uint64_t bytes() const;
   rh_status set_dt_all(const double* dt, size_t n);
@@ -250,11 +256,13 @@ class Batch {
 
  private:
   rh_status build();
+  rh_status reserve(size_t n);
   void run_program(hipStream_t s);
   void release_graph();
   MgSolver* mg(size_t i) { return static_cast<MgSolver*>(scenes_[i]->solver->impl.get()); }
 
-  std::vector<std::unique_ptr<rh_scene>> scenes_;
+  std::vector<rh_scene*> scenes_;                 // members (owned or not)
+  std::vector<std::unique_ptr<rh_scene>> owned_;  // scenes this batch created
   std::vector<double> dt_;
   std::vector<uint64_t> gens_;
   rh_batch_config cfg_{};
@@ -263,6 +271,7 @@ class Batch {
   hipStream_t stream_ = nullptr;
   DevGrid* d_grids_ = nullptr;
   BatchScene* d_scenes_ = nullptr;
+  size_t cap_ = 0;  // scene records the device tables hold
   std::vector<BatchScene> h_scenes_;
   unsigned char* d_tables_ = nullptr;
   size_t tables_bytes_ = 0;
@@ -271,9 +280,15 @@ class Batch {
   hipGraphExec_t exec_ = nullptr;
   bool built_ = false;
   int max_cycles_ = 0;
+  int launched_steps_ = 0;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+  double last_seconds_ = 0.0;
 };
 
 Batch::~Batch() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : ev_)
+    if (e) (void)hipEventDestroy(e);
   release_graph();
   if (d_tables_) (void)hipFree(d_tables_);
   if (d_grids_) (void)hipFree(d_grids_);
@@ -307,7 +322,8 @@ rh_status Batch::create(const rh_grid* grids, size_t n, const rh_batch_config& c
     rh_scene* raw = nullptr;
     rh_status s = rh_scene_create(&grids[i], &sc, &raw);
     if (s) return s;
-    scenes_.emplace_back(raw);
+    owned_.emplace_back(raw);
+    scenes_.push_back(raw);
     if (mg(i)->levels() < 2) {
       set_last_error("a batched scene needs at least two multigrid levels", "rh_batch_create");
       return RH_UNSUPPORTED;
@@ -316,10 +332,46 @@ rh_status Batch::create(const rh_grid* grids, size_t n, const rh_batch_config& c
   dt_.assign(n, 0.0);
   gens_.assign(n, ~0ull);
   RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  RH_HIP(hipMalloc(&d_grids_, sizeof(DevGrid) * n));
-  RH_HIP(hipMalloc(&d_scenes_, sizeof(BatchScene) * n));
-  h_scenes_.assign(n, BatchScene{});
+  return reserve(n);
+}
+
+rh_status Batch::reserve(size_t n) {
+  if (n <= cap_) return RH_OK;
+  if (d_grids_) (void)hipFree(d_grids_);
+  if (d_scenes_) (void)hipFree(d_scenes_);
+  d_grids_ = nullptr;
+  d_scenes_ = nullptr;
+  cap_ = std::max<size_t>(n, 2 * cap_);
+  RH_HIP(hipMalloc(&d_grids_, sizeof(DevGrid) * cap_));
+  RH_HIP(hipMalloc(&d_scenes_, sizeof(BatchScene) * cap_));
   return RH_OK;
+}
+
+rh_status Batch::init_empty(const rh_batch_config& cfg) {
+  cfg_ = cfg;
+  const rh_scene_config& sc = cfg.scene;
+  if (sc.precision != RH_F32 || sc.solver.method != RH_MIXED_MG || sc.solver.precision != RH_F32)
+    return RH_UNSUPPORTED;
+  variant_ = sc.solver.kernel_variant;
+  RH_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  return RH_OK;
+}
+
+rh_status Batch::set_members(const std::vector<rh_scene*>& scenes, const std::vector<double>& dt) {
+  if (scenes.size() != dt.size()) return RH_INVALID_ARGUMENT;
+  for (rh_scene* s : scenes) {
+    if (!s || s->config.precision != RH_F32 || s->config.solver.method != RH_MIXED_MG ||
+        s->config.solver.precision != RH_F32)
+      return RH_UNSUPPORTED;
+    if (static_cast<MgSolver*>(s->solver->impl.get())->levels() < 2) return RH_UNSUPPORTED;
+  }
+  if (scenes != scenes_ || dt != dt_) {
+    built_ = false;
+    scenes_ = scenes;
+    dt_ = dt;
+    gens_.assign(scenes_.size(), ~0ull);
+  }
+  return reserve(scenes_.size());
 }
 
 rh_status Batch::set_dt_all(const double* dt, size_t n) {
@@ -373,7 +425,7 @@ rh_status Batch::autotune(int steps, rh_batch_tuning* table, size_t cap, size_t*
   rh_status st = copy_faces(false);
   std::vector<rh_batch_tuning> tried;
   const uint32_t variants[2] = {0u, RH_VARIANT_SCALAR_MG};
-  const int divisors[4] = {1, 2, 4, 8};
+  const int divisors[6] = {1, 2, 4, 8, 16, 32};
   for (uint32_t v : variants)
     for (int d : divisors) {
       if (st) break;
@@ -414,6 +466,8 @@ rh_status Batch::set_variant(uint32_t variant) {
 // device state (hierarchies, masks, buffers).
 rh_status Batch::build() {
   const size_t n = scenes_.size();
+  h_scenes_.assign(n, BatchScene{});
+  if (gens_.size() != n) gens_.assign(n, ~0ull);
   const rh_scene_config& sc = cfg_.scene;
   const double density = sc.density;
   release_graph();
@@ -433,7 +487,7 @@ rh_status Batch::build() {
   max_cycles_ = (int)cycles;
   // per-scene records
   for (size_t i = 0; i < n; ++i) {
-    rh_scene* s = scenes_[i].get();
+    rh_scene* s = scenes_[i];
     Scratch& scr = s->solver->impl->scratch();
     BatchScene& b = h_scenes_[i];
     b = BatchScene{};
@@ -486,7 +540,7 @@ rh_status Batch::build() {
   std::vector<BFaceArgs> acc(n), adv(n), cand(n), rhs(n), cor(n), div(n), rate(n), commit(n);
   std::vector<int> fb(n), cb(n), rb(n);
   for (size_t i = 0; i < n; ++i) {
-    rh_scene* s = scenes_[i].get();
+    rh_scene* s = scenes_[i];
     Scratch& scr = s->solver->impl->scratch();
     BFaceArgs base{};
     base.g = d_grids_ + i;
@@ -536,7 +590,7 @@ rh_status Batch::build() {
   // --- cell records
   std::vector<BCellArgs> cells(n);
   for (size_t i = 0; i < n; ++i) {
-    rh_scene* s = scenes_[i].get();
+    rh_scene* s = scenes_[i];
     Scratch& scr = s->solver->impl->scratch();
     cells[i] = BCellArgs{d_grids_ + i, s->rhs, s->p, scr.pdd, scr.pmx, scr.ctl};
   }
@@ -835,8 +889,16 @@ void Batch::run_program(hipStream_t s) {
 }
 
 rh_status Batch::step(int steps, rh_step_report* reports, rh_status* statuses) {
+  rh_status s = launch(steps);
+  if (s) return s;
+  return wait(reports, statuses);
+}
+
+rh_status Batch::launch(int steps) {
   const size_t n = scenes_.size();
   if (steps < 1) return RH_INVALID_ARGUMENT;
+  launched_steps_ = 0;
+  if (n == 0) return RH_OK;
   for (size_t i = 0; i < n; ++i)
     if (!(dt_[i] > 0.0)) return RH_INVALID_TIME_STEP;
   for (size_t i = 0; i < n; ++i)
@@ -847,10 +909,11 @@ rh_status Batch::step(int steps, rh_step_report* reports, rh_status* statuses) {
   }
   // every scene's own stream must be idle (uploads went through them)
   for (size_t i = 0; i < n; ++i) RH_HIP(hipStreamSynchronize(scenes_[i]->solver->impl->scratch().stream));
-  hipEvent_t e0, e1;
-  RH_HIP(hipEventCreate(&e0));
-  RH_HIP(hipEventCreate(&e1));
-  RH_HIP(hipEventRecord(e0, stream_));
+  if (!ev_[0]) {
+    RH_HIP(hipEventCreate(&ev_[0]));
+    RH_HIP(hipEventCreate(&ev_[1]));
+  }
+  RH_HIP(hipEventRecord(ev_[0], stream_));
   for (int k = 0; k < steps; ++k) {
     if (exec_) {
       RH_HIP(hipGraphLaunch(exec_, stream_));
@@ -858,15 +921,24 @@ rh_status Batch::step(int steps, rh_step_report* reports, rh_status* statuses) {
       run_program(stream_);
     }
   }
-  RH_HIP(hipEventRecord(e1, stream_));
+  RH_HIP(hipEventRecord(ev_[1], stream_));
   RH_HIP(hipGetLastError());
   RH_HIP(hipMemcpyAsync(h_scenes_.data(), d_scenes_, sizeof(BatchScene) * n, hipMemcpyDeviceToHost,
                         stream_));
+  launched_steps_ = steps;
+  return RH_OK;
+}
+
+// Waits for the launched steps and unpacks the per-scene reports.
+rh_status Batch::wait(rh_step_report* reports, rh_status* statuses) {
+  const size_t n = scenes_.size();
+  const int steps = launched_steps_;
+  if (n == 0 || steps == 0) return RH_OK;
+  launched_steps_ = 0;
   RH_HIP(hipStreamSynchronize(stream_));
   float ms = 0.0f;
-  RH_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  RH_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+  last_seconds_ = ms * 1e-3;
   for (size_t i = 0; i < n; ++i) {
     const BatchScene& b = h_scenes_[i];
     if (statuses) statuses[i] = (rh_status)b.status;
@@ -941,4 +1013,287 @@ extern "C" rh_status rh_batch_autotune(rh_batch* batch, int32_t steps, rh_batch_
                                        size_t table_len, size_t* tried, rh_batch_tuning* chosen) {
   if (!batch) return RH_INVALID_ARGUMENT;
   return batch->impl.autotune(steps, table, table_len, tried, chosen);
+}
+
+// ================================================================ serving
+//
+// A serving loop over device-resident scenes with continuous batching:
+// requests are admitted in arrival order while their planned device bytes
+// fit the budget (and the active-scene cap); every tick runs one batched
+// step of every running request; requests leave as they finish and queued
+// ones join at the next tick.  Admission work (scene creation, obstacle
+// setup, uploads) for newcomers runs on the host while the device executes
+// the current tick.  The batched step of a scene does not depend on which
+// other scenes share the launch, so a request's result is the same alone
+// and in any mix (for a fixed launch shape).
+
+namespace rh {
+
+struct ServerRequest {
+  int64_t id = 0;
+  rh_grid grid{};
+  std::vector<float> u, v, w;
+  std::vector<uint8_t> solid;
+  double dt = 0.0;
+  int steps = 0;
+  uint64_t plan = 0;
+  rh_request_state st{};
+  std::unique_ptr<rh_scene> scene;
+  std::vector<float> ru, rv, rw;
+  std::vector<double> rp;
+};
+
+class Server {
+ public:
+  rh_status create(const rh_server_config& cfg);
+  rh_status submit(const rh_request& r, int64_t* id);
+  rh_status tick(rh_tick_report* rep);
+  rh_status state(int64_t id, rh_request_state* out);
+  rh_status result(int64_t id, float* u, float* v, float* w, double* p);
+
+ private:
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+  }
+  rh_status admit(ServerRequest* r);
+  void retire(ServerRequest* r, bool keep_result);
+  rh_server_config cfg_{};
+  Batch batch_;
+  std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
+  int64_t next_id_ = 1;
+  uint64_t budget_ = 0, used_ = 0;
+  std::vector<std::unique_ptr<ServerRequest>> all_;
+  std::vector<ServerRequest*> queue_, active_;
+};
+
+rh_status Server::create(const rh_server_config& cfg) {
+  cfg_ = cfg;
+  rh_batch_config bc{};
+  bc.scene = cfg.scene;
+  bc.max_cycles = 0;
+  bc.use_graph = 0;
+  rh_status s = batch_.init_empty(bc);
+  if (s) return s;
+  rh_batch_tuning t{cfg.kernel_variant, cfg.block_divisor > 0 ? cfg.block_divisor : 1, 0.0};
+  if ((s = batch_.set_tuning(t))) return s;
+  budget_ = cfg.hbm_budget;
+  if (!budget_) {
+    size_t free_b = 0, total_b = 0;
+    RH_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget_ = (uint64_t)(0.8 * (double)free_b);
+  }
+  return RH_OK;
+}
+
+rh_status Server::submit(const rh_request& r, int64_t* id) {
+  size_t cells = 0;
+  rh_status s = validate_grid(&r.grid, &cells);
+  if (s) return s;
+  if ((s = check_indexable(&r.grid))) return s;
+  if (!r.u || !r.v || !r.w || r.steps < 1 || !std::isfinite(r.dt) || r.dt <= 0.0) return RH_INVALID_ARGUMENT;
+  auto q = std::make_unique<ServerRequest>();
+  q->id = next_id_++;
+  q->grid = r.grid;
+  const size_t f[3] = {(size_t)(r.grid.counts[0] + 1) * r.grid.counts[1] * r.grid.counts[2],
+                       (size_t)r.grid.counts[0] * (r.grid.counts[1] + 1) * r.grid.counts[2],
+                       (size_t)r.grid.counts[0] * r.grid.counts[1] * (r.grid.counts[2] + 1)};
+  q->u.assign(r.u, r.u + f[0]);
+  q->v.assign(r.v, r.v + f[1]);
+  q->w.assign(r.w, r.w + f[2]);
+  if (r.solid) q->solid.assign(r.solid, r.solid + cells);
+  q->dt = r.dt;
+  q->steps = r.steps;
+  uint64_t faces = 0;
+  for (size_t a = 0; a < 3; ++a) faces += 2 * ((f[a] * 4 + 255) / 256 * 256);
+  q->plan = faces + 2 * ((cells * 8 + 255) / 256 * 256) + MgSolver::plan_bytes(r.grid, cfg_.scene.solver);
+  if (q->plan > budget_) return RH_BUFFER_LIMIT;
+  q->st.state = 0;
+  q->st.status = RH_OK;
+  q->st.submitted = now();
+  q->st.bytes = q->plan;
+  if (id) *id = q->id;
+  queue_.push_back(q.get());
+  all_.push_back(std::move(q));
+  return RH_OK;
+}
+
+// Scene creation, obstacles and initial faces of a newly admitted request.
+rh_status Server::admit(ServerRequest* r) {
+  rh_scene* raw = nullptr;
+  rh_status s = rh_scene_create(&r->grid, &cfg_.scene, &raw);
+  if (s) return s;
+  r->scene.reset(raw);
+  if (!r->solid.empty() && (s = rh_scene_set_solid(raw, r->solid.data(), r->solid.size()))) return s;
+  if ((s = rh_scene_set_velocity(raw, r->u.data(), r->v.data(), r->w.data()))) return s;
+  MgSolver* mg = static_cast<MgSolver*>(raw->solver->impl.get());
+  if (mg->levels() < 2) return RH_UNSUPPORTED;
+  if ((s = mg->prepare(cfg_.scene.density))) return s;
+  return RH_OK;
+}
+
+void Server::retire(ServerRequest* r, bool keep_result) {
+  if (keep_result && r->scene) {
+    const size_t cells = r->scene->n;
+    r->ru.resize(r->scene->flen[0]);
+    r->rv.resize(r->scene->flen[1]);
+    r->rw.resize(r->scene->flen[2]);
+    r->rp.resize(cells);
+    (void)rh_scene_get_velocity(r->scene.get(), r->ru.data(), r->rv.data(), r->rw.data());
+    (void)rh_scene_get_pressure(r->scene.get(), r->rp.data(), cells);
+  }
+  r->scene.reset();
+  used_ -= std::min(used_, r->st.bytes);
+  r->st.finished = now();
+  r->u.clear();
+  r->v.clear();
+  r->w.clear();
+  r->solid.clear();
+}
+
+rh_status Server::tick(rh_tick_report* rep) {
+  rh_tick_report local{};
+  rh_tick_report& T = rep ? *rep : local;
+  std::memset(&T, 0, sizeof(T));
+  const double t_start = now();
+  // 1. the current members' step runs on the device ...
+  std::vector<rh_scene*> members;
+  std::vector<double> dts;
+  for (ServerRequest* r : active_) {
+    members.push_back(r->scene.get());
+    dts.push_back(r->dt);
+  }
+  rh_status s = batch_.set_members(members, dts);
+  if (s) return s;
+  if (!members.empty() && (s = batch_.launch(1))) return s;
+  // 2. ... while newcomers are admitted (arrival order, memory permitting)
+  std::vector<ServerRequest*> joined;
+  while (!queue_.empty()) {
+    ServerRequest* r = queue_.front();
+    const size_t running = active_.size() + joined.size();
+    if (cfg_.max_active > 0 && running >= (size_t)cfg_.max_active) break;
+    if (used_ + r->plan > budget_) break;
+    queue_.erase(queue_.begin());
+    used_ += r->plan;
+    r->st.admitted = now();
+    rh_status a = admit(r);
+    if (a) {
+      r->st.state = 3;
+      r->st.status = a;
+      retire(r, false);
+      continue;
+    }
+    r->st.state = 1;
+    joined.push_back(r);
+    ++T.admitted;
+  }
+  // 3. wait, account, retire finished requests
+  std::vector<rh_step_report> reps(members.size());
+  std::vector<rh_status> sts(members.size(), RH_OK);
+  if (!members.empty() && (s = batch_.wait(reps.data(), sts.data()))) return s;
+  std::vector<ServerRequest*> still;
+  for (size_t i = 0; i < active_.size(); ++i) {
+    ServerRequest* r = active_[i];
+    r->st.last = reps[i];
+    T.cells += (int64_t)r->scene->n;
+    if (sts[i] != RH_OK) {
+      r->st.state = 3;
+      r->st.status = sts[i];
+      retire(r, true);
+      ++T.finished;
+      continue;
+    }
+    if (++r->st.steps_done >= r->steps) {
+      r->st.state = 2;
+      retire(r, true);
+      ++T.finished;
+      continue;
+    }
+    still.push_back(r);
+  }
+  for (ServerRequest* r : joined) still.push_back(r);
+  active_.swap(still);
+  T.stepped = (int32_t)members.size();
+  T.active = (int32_t)active_.size();
+  T.queued = (int32_t)queue_.size();
+  T.bytes_in_use = used_;
+  T.device_seconds = members.empty() ? 0.0 : batch_.last_seconds();
+  T.seconds = now() - t_start;
+  return RH_OK;
+}
+
+rh_status Server::state(int64_t id, rh_request_state* out) {
+  for (auto& r : all_)
+    if (r->id == id) {
+      *out = r->st;
+      return RH_OK;
+    }
+  return RH_INVALID_ARGUMENT;
+}
+
+rh_status Server::result(int64_t id, float* u, float* v, float* w, double* p) {
+  for (auto& r : all_)
+    if (r->id == id) {
+      if (r->st.state < 2 || r->ru.empty()) return RH_INVALID_ARGUMENT;
+      if (u) std::memcpy(u, r->ru.data(), r->ru.size() * 4);
+      if (v) std::memcpy(v, r->rv.data(), r->rv.size() * 4);
+      if (w) std::memcpy(w, r->rw.data(), r->rw.size() * 4);
+      if (p) std::memcpy(p, r->rp.data(), r->rp.size() * 8);
+      return RH_OK;
+    }
+  return RH_INVALID_ARGUMENT;
+}
+
+}  // namespace rh
+
+struct rh_server {
+  rh::Server impl;
+};
+
+extern "C" rh_status rh_server_create(const rh_server_config* config, rh_server** out) {
+  if (!config || !out) return RH_INVALID_ARGUMENT;
+  *out = nullptr;
+  auto* s = new (std::nothrow) rh_server();
+  if (!s) return RH_DEVICE_ERROR;
+  rh_status st = s->impl.create(*config);
+  if (st) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  return RH_OK;
+}
+
+extern "C" void rh_server_destroy(rh_server* server) { delete server; }
+
+extern "C" rh_status rh_server_submit(rh_server* server, const rh_request* request, int64_t* id) {
+  if (!server || !request) return RH_INVALID_ARGUMENT;
+  return server->impl.submit(*request, id);
+}
+
+extern "C" rh_status rh_server_tick(rh_server* server, rh_tick_report* report) {
+  if (!server) return RH_INVALID_ARGUMENT;
+  return server->impl.tick(report);
+}
+
+extern "C" rh_status rh_server_state(rh_server* server, int64_t id, rh_request_state* out) {
+  if (!server || !out) return RH_INVALID_ARGUMENT;
+  return server->impl.state(id, out);
+}
+
+extern "C" rh_status rh_server_result(rh_server* server, int64_t id, float* u, float* v, float* w,
+                                      double* pressure) {
+  if (!server) return RH_INVALID_ARGUMENT;
+  return server->impl.result(id, u, v, w, pressure);
+}
+
+extern "C" uint64_t rh_server_plan_bytes(const rh_grid* grid, const rh_scene_config* config) {
+  if (!grid || !config || rh::validate_grid(grid, nullptr)) return 0;
+  const uint64_t n = grid->counts[0] * grid->counts[1] * grid->counts[2];
+  uint64_t faces = 0;
+  for (int a = 0; a < 3; ++a) {
+    uint64_t f = 1;
+    for (int d = 0; d < 3; ++d) f *= grid->counts[d] + (d == a ? 1 : 0);
+    faces += 2 * ((f * 4 + 255) / 256 * 256);
+  }
+  return faces + 2 * ((n * 8 + 255) / 256 * 256) + rh::MgSolver::plan_bytes(*grid, config->solver);
 }

@@ -124,6 +124,8 @@ class MgSolver : public Solver {
                   void* ctx) override;
   uint64_t allocated_bytes() const override;
   int levels() const { return h_.levels; }
+  static int plan_levels(const rh_grid& grid, const rh_solver_config& config, int dims[][3], int* padded);
+  static uint64_t plan_bytes(const rh_grid& grid, const rh_solver_config& config);
   rh_status set_solid(const uint8_t* solid) override;
   // Level data and coarsest inverse for `density` on the device (done by
   // every solve; batched runners call it before building launch tables).
@@ -145,6 +147,7 @@ class MgSolver : public Solver {
   bool dirty_ = true;                         // hierarchy must be re-uploaded
   void* arena_ = nullptr;
   double* d_inv_ = nullptr;   // dense coarsest inverse (device)
+  size_t inv_cap_ = 0;        // doubles allocated at d_inv_
   MgHierarchy* d_h_ = nullptr;  // device copy of h_ read by the tail kernel
   double inv_density_ = 0.0;  // density the inverse was built for
   uint64_t bytes_ = 0;

@@ -233,6 +233,62 @@ rh_status PcgSolver::solve(double density, const double* d_rhs, double dt,
 
 // -------------------------------------------------------------- multigrid
 
+// Level shapes of the hierarchy for this grid (shared by create and the
+// memory plan): returns the level count; *padded = first padded level.
+int MgSolver::plan_levels(const rh_grid& grid, const rh_solver_config& config, int dims[][3],
+                          int* padded) {
+  int n[3] = {(int)grid.counts[0], (int)grid.counts[1], (int)grid.counts[2]};
+  int levels = 1;
+  dims[0][0] = n[0];
+  dims[0][1] = n[1];
+  dims[0][2] = n[2];
+  const int want = config.mg_levels > 0 ? std::min(config.mg_levels, kMgMaxLevels) : kMgMaxLevels;
+  // A level with an odd axis (or an axis below 4) above 64 cells is
+  // coarsened with ceil(n/2) cells per axis: the last coarse cell of an odd
+  // axis has one child, and the missing children act as solid cells (the
+  // coarse levels from there on carry face fractions).  The coarsest level
+  // then has at most 64 cells and a small dense inverse.
+  *padded = 0;
+  while (levels < want) {
+    const int* m = &dims[levels - 1][0];
+    const long cells = (long)m[0] * m[1] * m[2];
+    const bool even = (m[0] % 2 == 0) && (m[1] % 2 == 0) && (m[2] % 2 == 0);
+    const int mn = std::min(m[0], std::min(m[1], m[2]));
+    const int mx = std::max(m[0], std::max(m[1], m[2]));
+    if (even && mn >= 4) {
+      if (config.mg_levels <= 0 && cells <= 64) break;
+    } else {
+      // odd axis (or an axis below 4): keep coarsening with padding down to
+      // the small dense coarsest level
+      if (config.mg_levels > 0 || cells <= 64 || mx < 3) break;
+      if (!*padded) *padded = levels;
+    }
+    dims[levels][0] = (m[0] + 1) / 2;
+    dims[levels][1] = (m[1] + 1) / 2;
+    dims[levels][2] = (m[2] + 1) / 2;
+    ++levels;
+  }
+  return levels;
+}
+
+// Device bytes a scene of this grid holds with this solver (admission plan).
+uint64_t MgSolver::plan_bytes(const rh_grid& grid, const rh_solver_config& config) {
+  int dims[kMgMaxLevels][3];
+  int padded = 0;
+  const int levels = plan_levels(grid, config, dims, &padded);
+  const size_t s = config.precision == RH_F64 ? 8 : 4;
+  uint64_t total = 0;
+  for (int l = 0; l < levels; ++l) {
+    const uint64_t cells = (uint64_t)dims[l][0] * dims[l][1] * dims[l][2];
+    total += 3 * ((cells * s + 255) / 256 * 256);
+    if (l > 0) total += (cells + 63) / 64 * 256 + (cells + 255) / 256 * 256;  // fractions, octants
+    if (l == 0) total += cells;                                                // obstacle mask
+  }
+  const uint64_t coarse = (uint64_t)dims[levels - 1][0] * dims[levels - 1][1] * dims[levels - 1][2];
+  total += coarse * coarse * 8 + sizeof(MgHierarchy) + (256u << 10);  // dense inverse, tables, scratch
+  return total;
+}
+
 rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) {
   grid_ = grid;
   config_ = config;
@@ -246,34 +302,12 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
   h_.quad = (config.kernel_variant & RH_VARIANT_SCALAR_MG) == 0;
   // Coarsen while every axis is even and the level is above the one-
   // workgroup coarsest size (or up to the requested level count).
-  int n[3] = {(int)grid.counts[0], (int)grid.counts[1], (int)grid.counts[2]};
-  int levels = 1;
-  h_.lv[0].nx = n[0];
-  h_.lv[0].ny = n[1];
-  h_.lv[0].nz = n[2];
-  const int want = config.mg_levels > 0 ? std::min(config.mg_levels, kMgMaxLevels) : kMgMaxLevels;
-  // A level with an odd axis that is still too large for the dense
-  // coarsest inverse is coarsened with ceil(n/2) cells per axis: the last
-  // coarse cell of an odd axis has one child, and the missing children act
-  // as solid cells (the coarse levels from there on carry face fractions).
-  padded_level_ = 0;
-  while (levels < want) {
-    const int* m = &h_.lv[levels - 1].nx;
-    const long cells = (long)m[0] * m[1] * m[2];
-    const bool even = (m[0] % 2 == 0) && (m[1] % 2 == 0) && (m[2] % 2 == 0);
-    const int mn = std::min(m[0], std::min(m[1], m[2]));
-    const int mx = std::max(m[0], std::max(m[1], m[2]));
-    if (even) {
-      if (mn < 4) break;
-      if (config.mg_levels <= 0 && cells <= 64) break;
-    } else {
-      if (config.mg_levels > 0 || cells <= kMgDenseMax || mx < 3) break;
-      if (!padded_level_) padded_level_ = levels;
-    }
-    h_.lv[levels].nx = (m[0] + 1) / 2;
-    h_.lv[levels].ny = (m[1] + 1) / 2;
-    h_.lv[levels].nz = (m[2] + 1) / 2;
-    ++levels;
+  int dims[kMgMaxLevels][3];
+  int levels = plan_levels(grid, config, dims, &padded_level_);
+  for (int l = 0; l < levels; ++l) {
+    h_.lv[l].nx = dims[l][0];
+    h_.lv[l].ny = dims[l][1];
+    h_.lv[l].nz = dims[l][2];
   }
   h_.levels = levels;
   const MgLevel& last = h_.lv[levels - 1];
@@ -586,7 +620,14 @@ rh_status MgSolver::set_density(double density) {
   const LevelFaces faces{last, (cl == 0 && !hmask_.empty()) ? &hmask_ : nullptr,
                          (cl > 0 && h_.lv[cl].frac) ? &hfrac_[cl] : nullptr, fscale_[cl]};
   if (n <= kMgDenseMax && dense_coarse_inverse(faces, inv)) {
-    if (!d_inv_) RH_HIP(hipMalloc(&d_inv_, sizeof(double) * (size_t)kMgDenseMax * kMgDenseMax));
+    if (d_inv_ && inv_cap_ < inv.size()) {
+      (void)hipFree(d_inv_);
+      d_inv_ = nullptr;
+    }
+    if (!d_inv_) {
+      RH_HIP(hipMalloc(&d_inv_, sizeof(double) * inv.size()));
+      inv_cap_ = inv.size();
+    }
     RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
     h_.coarse_inv = d_inv_;
   }
