@@ -286,6 +286,122 @@ def bench_batch(args) -> int:
     return 0
 
 
+def serve_requests(count: int, seed: int):
+    """Requests of the serving workload: each axis log-uniform on [32, 192]
+    and a multiple of 8 (cubic cells, longest side 1), run length uniform on
+    {10..200}, 16 Fourier modes (|k| <= 8) with max|u| = 0.02 (so that the
+    f32 rounding floor of max|div u|, ~6 * 2^-25 |u| / h, stays below 1e-6),
+    and 0-3 solid spheres of radius U[0.05, 0.2] kept clear of the walls."""
+    from rheon_amd.scenes import (close_obstacle_faces, isotropic_spec, lcg_uniform, log_uniform_sizes,
+                                  spheres_solid)
+
+    sizes = log_uniform_sizes(seed, count, 32, 192, 8)
+    u = lcg_uniform(seed + 1, 8 * count) + 0.5
+    out = []
+    for i, counts in enumerate(sizes):
+        spec = isotropic_spec(counts)
+        r = u[8 * i:8 * i + 8]
+        steps = 10 + int(r[0] * 191) % 191
+        nsph = int(r[1] * 4) % 4
+        ext = [spec.counts[d] * spec.spacing[d] for d in range(3)]
+        sph = []
+        su = lcg_uniform(seed + 100 + i, 4 * nsph) + 0.5
+        for k in range(nsph):
+            rad = min(0.05 + 0.15 * su[4 * k], 0.45 * min(ext) - 0.05)
+            c = [rad + 0.05 + su[4 * k + 1 + d] * (ext[d] - 2 * rad - 0.1) for d in range(3)]
+            sph.append((c[0], c[1], c[2], rad))
+        solid = spheres_solid(spec, sph) if sph else None
+        vel = fourier_modes(spec, seed + 1000 + i, modes=16, kmax=8, amplitude=0.02, dtype=np.float32)
+        if solid is not None:
+            vel = close_obstacle_faces(spec, solid, vel)
+        out.append((spec, vel, courant_dt(spec, vel, 0.5), steps, solid, len(sph)))
+    return out
+
+
+def bench_serve(args) -> int:
+    """Serving workload: Poisson arrivals at a fixed rate, continuous
+    batching, admission by planned device bytes; per-step latency p50/p99
+    and total steps/s at max|div u| <= 1e-6."""
+    rng = np.random.default_rng(args.seed)
+    t_gen = time.perf_counter()
+    reqs = serve_requests(args.requests, args.seed)
+    t_gen = time.perf_counter() - t_gen
+    gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
+    arrivals = np.cumsum(gaps) - gaps[0]
+    server = R.ProjectionServer(settings=R.PressureSettings(1e-8, 1e-14, 5e-7, 40), actual_divergence_limit=1e-6,
+                                hbm_budget=int(args.hbm_budget_gb * (1 << 30)), block_divisor=args.block_divisor)
+    ids, submitted = [], 0
+    ticks = []
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while submitted < len(reqs) and arrivals[submitted] <= now:
+            spec, vel, dt, steps, solid, _ = reqs[submitted]
+            ids.append(server.submit(spec, vel, dt, steps, solid))
+            submitted += 1
+        states = [server.state(i)["state"] for i in ids]
+        if submitted == len(reqs) and all(s_ in ("done", "failed") for s_ in states):
+            break
+        if not any(s_ in ("queued", "running") for s_ in states):
+            time.sleep(max(0.0, arrivals[submitted] - now))
+            continue
+        tk = server.tick()
+        tk["t"] = time.perf_counter() - t0
+        ticks.append(tk)
+    makespan = time.perf_counter() - t0
+    st = [server.state(i) for i in ids]
+    failed = [(i, R.STATUS_NAMES.get(s_["status"])) for i, s_ in zip(ids, st) if s_["state"] != "done"]
+    step_lat = np.array([tk["seconds"] for tk in ticks if tk["stepped"] > 0]) * 1e3
+    steps_total = int(sum(tk["stepped"] for tk in ticks))
+    cells_total = int(sum(tk["cells"] for tk in ticks))
+    done = [s_ for s_ in st if s_["state"] == "done"]
+    wait = np.array([s_["admitted"] - s_["submitted"] for s_ in done]) * 1e3
+    turn = np.array([s_["finished"] - s_["submitted"] for s_ in done])
+    divs = [s_["actual_divergence_max"] for s_ in done]
+    result = {
+        "metric": "serving: per-step latency p50/p99 and total steps/s at max|div u| <= 1e-6",
+        "value": steps_total / makespan,
+        "unit": "scene-steps/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32 faces + f32 V-cycles, f64 outer residual/pressure",
+        "data": ("synthetic: 16 Fourier modes |k|<=8, max|u|=0.02, 0-3 spheres r in [0.05,0.2]; "
+                 f"seed {args.seed}"),
+        "config": {
+            "workload": (f"{len(reqs)} requests, Poisson arrivals at {args.rate:g}/s, axes log-uniform [32,192] "
+                         f"multiple of 8, run length U{{10..200}}, mixed-mg-dc rel 1e-8 + predicted div 5e-7, "
+                         f"actual max|div u| gate 1e-6"),
+            "hbm_budget_gb": args.hbm_budget_gb,
+            "block_divisor": args.block_divisor,
+            "latency_ms_p50": float(np.percentile(step_lat, 50)),
+            "latency_ms_p99": float(np.percentile(step_lat, 99)),
+            "latency_ms_max": float(step_lat.max()),
+            "cell_updates_per_s": cells_total / makespan,
+            "makespan_s": makespan,
+            "ticks": len(ticks),
+            "mean_active": float(np.mean([tk["stepped"] for tk in ticks])),
+            "max_active": int(max(tk["stepped"] for tk in ticks)),
+            "max_bytes_in_use_gb": max(tk["bytes_in_use"] for tk in ticks) / (1 << 30),
+            "queue_wait_ms_p50": float(np.percentile(wait, 50)) if len(wait) else None,
+            "queue_wait_ms_p99": float(np.percentile(wait, 99)) if len(wait) else None,
+            "turnaround_s_p50": float(np.percentile(turn, 50)) if len(turn) else None,
+            "turnaround_s_p99": float(np.percentile(turn, 99)) if len(turn) else None,
+            "max_div_u": max(divs) if divs else None,
+            "failed": failed,
+            "generation_s": t_gen,
+            "spheres": int(sum(r[5] for r in reqs)),
+            "scene_steps": steps_total,
+        },
+    }
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    server.close()
+    return 0 if not failed else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,9 +419,15 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once with the plain copy kernel (the known-size "
                          "reference for hardware traffic counters) and exit")
-    ap.add_argument("--workload", default="scene", choices=["scene", "batch"],
-                    help="scene: the 256^3 projection step (default); batch: 64 batched scenes")
+    ap.add_argument("--workload", default="scene", choices=["scene", "batch", "serve"],
+                    help="scene: the 256^3 projection step (default); batch: 64 batched scenes; "
+                         "serve: the continuous-batching serving loop")
     ap.add_argument("--scenes", type=int, default=64)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=8.0, help="serving: Poisson arrival rate (requests/s)")
+    ap.add_argument("--hbm-budget-gb", type=float, default=16.0)
+    ap.add_argument("--block-divisor", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=5)
     args = ap.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
@@ -313,6 +435,8 @@ def main() -> int:
         print("bench.py measures one GPU: a projection scene lives on one device and many scenes shard "
               "across GPUs by whole scenes (no collectives); run with --gpus 1.", file=sys.stderr)
         return 2
+    if args.workload == "serve":
+        return bench_serve(args)
     if args.workload == "batch":
         if args.steps == 5 and "--steps" not in sys.argv:
             args.steps = 10
