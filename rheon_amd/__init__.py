@@ -278,10 +278,12 @@ class PressureWorkspace:
     """PressureWorkspace on the device (src/pressure.rs:243-740)."""
 
     def __init__(self, spec: GridSpec, method: str = "jacobi-pcg-v1", precision: int = F64,
-                 memory_limit: int = 0, mg_levels: int = 0, kernel_variant: int = 0):
+                 memory_limit: int = 0, mg_levels: int = 0, kernel_variant: int = 0, mg_sweeps=(0, 0),
+                 mg_omega: float = 0.0):
         self.spec = spec
         self.grid = make_grid(spec)
-        cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, 0, 0, 0.0, memory_limit, kernel_variant, 0)
+        cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, int(mg_sweeps[0]), int(mg_sweeps[1]),
+                             float(mg_omega), memory_limit, kernel_variant, 0)
         h = C.c_void_p()
         _check(lib().rh_solver_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_solver_create")
         self.handle = h.value

@@ -161,6 +161,13 @@ __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restr
   return 0;
 }
 
+// Minimum waves per SIMD requested for the advection kernels (their f64
+// sampling is latency-bound; more waves in flight hide it).
+#ifndef RH_ADVECT_WAVES
+#define RH_ADVECT_WAVES 4
+#endif
+constexpr int kAdvectWaves = RH_ADVECT_WAVES;
+
 // advect_velocity (src/advection.rs:28-68).  blockIdx.y = component.  Wall
 // normal faces are exactly zero; every other face samples its own component
 // at the RK2 departure of its own face position (src/geometry.rs:240-248).
@@ -205,7 +212,7 @@ b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T*
   if (bad) atomicOr(flags, bad);
 }
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kAdvectWaves)
 k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
                   T* __restrict__ ow, unsigned* __restrict__ flags) {
@@ -525,7 +532,7 @@ kb_validate(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int
   RH_BATCH_PROLOGUE(BFaceArgs)
   b_validate_faces<float>(vb, *a.g, a.u, a.v, a.w, a.mode, a.flags);
 }
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kAdvectWaves)
 kb_advect(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BFaceArgs)
   b_advect_velocity<float>(vb, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow, a.flags);

@@ -99,7 +99,7 @@ struct FinRhsSums {
 };
 
 struct PcgArrays {
-  void* p;       // T
+  void* p;       // f64 solution (= p64)
   void* r;       // T
   void* d;       // T
   void* q;       // T (also holds the true residual)

@@ -109,7 +109,7 @@ class PcgSolver : public Solver {
   rh_status run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
                 double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
   void* arena_ = nullptr;
-  void *r_ = nullptr, *d_ = nullptr, *q_ = nullptr, *z_ = nullptr, *p32_ = nullptr;
+  void *r_ = nullptr, *d_ = nullptr, *q_ = nullptr, *z_ = nullptr;
   size_t n_ = 0;
   uint64_t bytes_ = 0;
 };

@@ -161,8 +161,9 @@ rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config)
   if (st) return st;
   n_ = (size_t)grid.counts[0] * grid.counts[1] * grid.counts[2];
   const size_t s = config.precision == RH_F64 ? 8 : 4;
-  // r, d, q (+ z for RB-SGS) (+ p when vectors are f32; f64 p is the output).
-  int vectors = 3 + (config.method == RH_RBSGS_PCG ? 1 : 0) + (config.precision == RH_F32 ? 1 : 0);
+  // r, d, q (+ z for RB-SGS); the solution accumulates in the caller's f64
+  // pressure for either vector precision.
+  int vectors = 3 + (config.method == RH_RBSGS_PCG ? 1 : 0);
   bytes_ = s * n_ * vectors;
   if (config.memory_limit && bytes_ > config.memory_limit) return RH_BUFFER_LIMIT;
   RH_HIP(hipMalloc(&arena_, bytes_ + 256 * vectors));
@@ -176,7 +177,7 @@ rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config)
   d_ = take();
   q_ = take();
   z_ = config.method == RH_RBSGS_PCG ? take() : nullptr;
-  p32_ = config.precision == RH_F32 ? take() : nullptr;
+
   return RH_OK;
 }
 
@@ -193,7 +194,7 @@ rh_status PcgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressur
                          void* ctx) {
   hipStream_t s = scratch_.stream;
   PcgArrays a;
-  a.p = sizeof(T) == 8 ? (void*)d_pressure : p32_;
+  a.p = d_pressure;
   a.r = r_;
   a.d = d_;
   a.q = q_;
