@@ -29,7 +29,7 @@ from rheon_amd.scenes import GridSpec, centred_sphere, courant_dt, fourier_modes
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 2, post: int = 2) -> float:
+def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 3, post: int = 1) -> float:
     """Compulsory bytes of one defect-correction cycle (outer f64 residual +
     V(pre, post) in storage of s bytes): per finest cell
       outer residual  read p, b (f64) + write b0, x0           16 + 2s
@@ -50,13 +50,13 @@ def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 2, post: int =
 
 
 def algorithmic_bytes(spec: GridSpec, face_bytes: int, method: str, solver_bytes: int, iterations: int,
-                      m: int = 0) -> dict:
+                      m: int = 0, sweeps=(3, 1)) -> dict:
     """Compulsory HBM bytes of one step (per-kernel minimal-fusion model)."""
     n = spec.cell_len
     F = spec.face_total
     sv = solver_bytes
     if method in ("mixed-mg-dc", "mg-f64-dc"):
-        per_iter = mg_cycle_bytes(spec, sv, m)
+        per_iter = mg_cycle_bytes(spec, sv, m, *sweeps)
     else:
         # Krylov vectors in sv bytes, the solution accumulated in f64 (16 B/cell per update)
         per_iter = {"jacobi-pcg-v1": (9 * sv + 16) * n + m * n, "rbsgs-pcg": (15 * sv + 16) * n + 3 * m * n}.get(method, 0)
@@ -531,6 +531,10 @@ def main() -> int:
                     choices=["jacobi-pcg-v1", "rbsgs-pcg", "mixed-mg-dc", "mg-f64-dc"])
     ap.add_argument("--rel", type=float, default=1e-10)
     ap.add_argument("--sphere", type=float, default=0.2, help="radius of the centred solid sphere (0 = none)")
+    ap.add_argument("--smoother", default="chebyshev", choices=["chebyshev", "jacobi"],
+                    help="multigrid smoother (Chebyshev weights or weighted Jacobi)")
+    ap.add_argument("--mg-sweeps", default="", metavar="PRE,POST",
+                    help="V-cycle sweeps (default 3,1 for Chebyshev, 2,2 for Jacobi)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--compare", action="store_true",
                     help="also time one solve of the step's RHS by every method and report pressure errors")
@@ -571,9 +575,13 @@ def main() -> int:
     mg = args.method in ("mixed-mg-dc", "mg-f64-dc")
     s_vec = 4 if args.method == "mixed-mg-dc" else 8  # bytes per Krylov / V-cycle value
     settings = R.PressureSettings(args.rel, 1e-14, 1e-4, 200 if mg else 20000, poll_interval=2 if mg else 32)
+    smoother = R.SMOOTHER_JACOBI if args.smoother == "jacobi" else R.SMOOTHER_CHEBYSHEV
+    sweeps = tuple(int(v) for v in args.mg_sweeps.split(",")) if args.mg_sweeps else (
+        (2, 2) if args.smoother == "jacobi" else (3, 1))
     scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method,
                     solver_precision=R.F32 if args.method == "mixed-mg-dc" else R.F64,
-                    settings=settings, actual_divergence_limit=1e-3, max_courant=0.0)
+                    settings=settings, actual_divergence_limit=1e-3, max_courant=0.0,
+                    mg_sweeps=sweeps, mg_smoother=smoother)
     solid = centred_sphere(spec, args.sphere) if args.sphere > 0 else None
     if solid is not None:
         scene.set_solid(solid)
@@ -637,7 +645,7 @@ def main() -> int:
     achieved = per_launch / avg / 1e9
 
     mean_iters = float(np.mean(iters))
-    ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)), m_bytes)
+    ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)), m_bytes, sweeps)
     step_bytes = sum(ab.values())
     ms = elapsed / args.steps * 1e3
 
@@ -660,6 +668,7 @@ def main() -> int:
             "workload": f"projection step {args.n}^3, {args.method} to rel {args.rel:g}",
             "grid": list(spec.counts),
             "method": args.method,
+            "mg_smoother": (f"{args.smoother} V({sweeps[0]},{sweeps[1]})" if mg else None),
             "relative_residual": args.rel,
             "obstacle": (f"sphere r={args.sphere:g} at (0.5,0.5,0.5), {int(solid.sum())} solid cells"
                          if solid is not None else "none"),

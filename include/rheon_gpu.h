@@ -183,13 +183,21 @@ typedef struct rh_solver_config {
   rh_method method;
   rh_precision precision;    /* Krylov vector storage (F64 = reference arithmetic) */
   int32_t mg_levels;         /* multigrid levels (0 = automatic) */
-  int32_t mg_pre_sweeps;     /* default 2 */
-  int32_t mg_post_sweeps;    /* default 2 */
-  double mg_omega;           /* weighted-Jacobi smoother weight (0 = default) */
+  int32_t mg_pre_sweeps;     /* 0 = default (Chebyshev 3, Jacobi 2); at most 8 */
+  int32_t mg_post_sweeps;    /* 0 = default (Chebyshev 1, Jacobi 2); at most 8 */
+  double mg_omega;           /* weighted-Jacobi smoother weight (0 = 6/7) */
   uint64_t memory_limit;     /* bytes; 0 = unlimited (PressureWorkspace limit) */
   uint32_t kernel_variant;   /* 0 = default kernels; RH_VARIANT_* bits select alternatives */
-  uint32_t reserved;
+  uint32_t mg_smoother;      /* RH_SMOOTHER_* (0 = Chebyshev) */
 } rh_solver_config;
+
+/* Multigrid smoothers.  Both sweep x <- x + tau_k (b - K x) / D; Jacobi uses
+ * tau_k = mg_omega, Chebyshev the reciprocal roots of the Chebyshev
+ * polynomial of the side's sweep count on the interval [lo, 2] of D^-1 K's
+ * spectrum (lo = 0.3 for up to 2 sweeps, 0.25 beyond), the same memory
+ * traffic per sweep with a smaller error-reduction factor. */
+#define RH_SMOOTHER_CHEBYSHEV 0u
+#define RH_SMOOTHER_JACOBI 1u
 
 /* Kernel variants (same results; for tuning and for cross-checking). */
 #define RH_VARIANT_SCALAR_MG 1u /* one cell per lane in every multigrid kernel */

@@ -8,18 +8,22 @@ can be checked cycle by cycle, and its convergence studied, on the CPU:
   than 64 cells (or a requested level count); spacing doubles; the operator
   is the box operator of src/operator.rs:58-97 rediscretised on each level
   (weights A_d/(rho h_d)), Neumann walls, applied as in :400-416.
-* V(nu1, nu2) with weighted Jacobi (omega = 6/7); first pre-sweep from zero
-  x = omega b/D; restriction = sum of the 8 children's residuals; trilinear
-  cell-centred prolongation (1D weights 3/4, 1/4; walls mirror the parent);
-  coarsest level solved exactly on the mean-free right-hand side.
+* V(nu1, nu2); sweep k of a side is x <- x + tau_k (b - K x)/D with
+  Chebyshev weights (default, V(3,1)): tau_k = 1/lambda_k, lambda_k the roots
+  of the degree-nu Chebyshev polynomial mapped to [lo, 2] (lo = 0.3 for
+  nu <= 2, else 0.25), or weighted Jacobi (tau_k = omega = 6/7, V(2,2));
+  first pre-sweep from zero x = tau_1 b/D; restriction = sum of the 8
+  children's residuals; trilinear cell-centred prolongation (1D weights 3/4,
+  1/4; walls mirror the parent); coarsest level solved exactly on the
+  mean-free right-hand side.
 * outer loop: p += V-cycle(b - K p) in f64 with the reference's acceptance
   rule (l2 <= max(abs, rel ||b||) and max|r| dt/V <= divergence limit); the
   returned pressure is shifted so p[0] = 0.
 
 Parity pinning: the operator is checked against the oracle's apply_full
 (bit-identical formula), and the solver's converged pressure against the
-oracle's jacobi-pcg-v1 at tight tolerance and against dense least squares
-(tests/test_mg_oracle.py).
+oracle's jacobi-pcg-v1 at tight tolerance, and the Chebyshev weights against
+the closed form of the Chebyshev polynomial's extrema (tests/test_mg.py).
 """
 from __future__ import annotations
 
@@ -109,24 +113,37 @@ def coarsest_solve(L: Level, b):
     return (x - x.mean()).reshape(b.shape)
 
 
-def vcycle(lv, l, b, pre=2, post=2, omega=6.0 / 7.0, dtype=np.float64):
+def sweep_weights(n, smoother="chebyshev", omega=6.0 / 7.0):
+    """Weights tau_1..tau_n of one side's sweeps."""
+    if smoother == "jacobi":
+        return [omega] * n
+    lo, hi = (0.3 if n <= 2 else 0.25), 2.0
+    c, h = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    return [1.0 / (c + h * np.cos(np.pi * (2 * k + 1) / (2 * n))) for k in range(n)]
+
+
+def vcycle(lv, l, b, tau_pre, tau_post, dtype=np.float64):
     L = lv[l]
     if l == len(lv) - 1:
         return coarsest_solve(L, b.astype(np.float64)).astype(dtype)
-    x = (omega * b / L.D).astype(dtype)
-    for _ in range(pre - 1):
-        x = (x + omega * (b - L.apply(x)) / L.D).astype(dtype)
+    x = (tau_pre[0] * b / L.D).astype(dtype)
+    for t in tau_pre[1:]:
+        x = (x + t * (b - L.apply(x)) / L.D).astype(dtype)
     bc = restrict(b - L.apply(x)).astype(dtype)
-    xc = vcycle(lv, l + 1, bc, pre, post, omega, dtype)
+    xc = vcycle(lv, l + 1, bc, tau_pre, tau_post, dtype)
     x = (x + prolong(xc, x.shape)).astype(dtype)
-    for _ in range(post):
-        x = (x + omega * (b - L.apply(x)) / L.D).astype(dtype)
+    for t in tau_post:
+        x = (x + t * (b - L.apply(x)) / L.D).astype(dtype)
     return x
 
 
 def solve(counts, spacing, density, rhs, dt, rel, abs_tol, div_limit, max_cycles=100, levels=0,
-          pre=2, post=2, omega=6.0 / 7.0, dtype=np.float64, history=None):
+          pre=0, post=0, omega=6.0 / 7.0, dtype=np.float64, history=None, smoother="chebyshev"):
     """Defect correction with V-cycles; returns (status, p, cycles, l2, rmax)."""
+    jac = smoother == "jacobi"
+    pre = pre or (2 if jac else 3)
+    post = post or (2 if jac else 1)
+    tau_pre, tau_post = sweep_weights(pre, smoother, omega), sweep_weights(post, smoother, omega)
     lv = build_levels(counts, spacing, density, levels)
     shape = (counts[2], counts[1], counts[0])
     b = np.asarray(rhs, np.float64).reshape(shape)
@@ -144,6 +161,6 @@ def solve(counts, spacing, density, rhs, dt, rel, abs_tol, div_limit, max_cycles
             return "ok", p.ravel(), cycle, l2, rmax
         if cycle == max_cycles:
             break
-        p = p + vcycle(lv, 0, r.astype(dtype), pre, post, omega, dtype).astype(np.float64)
+        p = p + vcycle(lv, 0, r.astype(dtype), tau_pre, tau_post, dtype).astype(np.float64)
     p = p - p.flat[0]
     return "iteration_limit", p.ravel(), max_cycles, l2, rmax

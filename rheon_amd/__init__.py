@@ -47,6 +47,8 @@ STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) a
 
 F64, F32 = 0, 1
 VARIANT_SCALAR_MG = 1
+SMOOTHER_CHEBYSHEV = 0  # multigrid smoothers (rh_solver_config.mg_smoother)
+SMOOTHER_JACOBI = 1
 JACOBI_PCG, RBSGS_PCG, MG_PCG, MIXED_MG = 0, 1, 2, 3
 METHOD_IDS = {"jacobi-pcg-v1": JACOBI_PCG, "rbsgs-pcg": RBSGS_PCG, "mg-pcg": MG_PCG,
               "mixed-mg-dc": MIXED_MG, "mg-f64-dc": MIXED_MG}
@@ -279,11 +281,11 @@ class PressureWorkspace:
 
     def __init__(self, spec: GridSpec, method: str = "jacobi-pcg-v1", precision: int = F64,
                  memory_limit: int = 0, mg_levels: int = 0, kernel_variant: int = 0, mg_sweeps=(0, 0),
-                 mg_omega: float = 0.0):
+                 mg_omega: float = 0.0, mg_smoother: int = 0):
         self.spec = spec
         self.grid = make_grid(spec)
         cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, int(mg_sweeps[0]), int(mg_sweeps[1]),
-                             float(mg_omega), memory_limit, kernel_variant, 0)
+                             float(mg_omega), memory_limit, kernel_variant, int(mg_smoother))
         h = C.c_void_p()
         _check(lib().rh_solver_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_solver_create")
         self.handle = h.value
@@ -483,14 +485,15 @@ class Scene:
     def __init__(self, spec: GridSpec, density: float = 1.0, precision: int = F32, method: str = "jacobi-pcg-v1",
                  solver_precision: int | None = None, settings: PressureSettings | None = None,
                  actual_divergence_limit: float = 1e-5, max_courant: float = 0.0, mg_levels: int = 0,
-                 kernel_variant: int = 0):
+                 kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0):
         self.spec = spec
         self.precision = precision
         self.grid = make_grid(spec)
         settings = settings or PressureSettings()
         sp = precision if solver_precision is None else solver_precision
         cfg = N.SceneConfig(density, precision,
-                            N.SolverConfig(METHOD_IDS[method], sp, mg_levels, 0, 0, 0.0, 0, kernel_variant, 0),
+                            N.SolverConfig(METHOD_IDS[method], sp, mg_levels, int(mg_sweeps[0]), int(mg_sweeps[1]),
+                                           0.0, 0, kernel_variant, int(mg_smoother)),
                             settings.native(), actual_divergence_limit, max_courant)
         h = C.c_void_p()
         _check(lib().rh_scene_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_scene_create")
@@ -570,11 +573,12 @@ class SceneBatch:
 
     def __init__(self, specs, density: float = 1.0, settings: PressureSettings | None = None,
                  actual_divergence_limit: float = 1e-3, max_courant: float = 0.0, max_cycles: int = 0,
-                 use_graph: bool = True, kernel_variant: int = 0):
+                 use_graph: bool = True, kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0):
         self.specs = list(specs)
         settings = settings or PressureSettings(1e-8, 1e-14, 1e-3, 40)
         cfg = N.BatchConfig(
-            N.SceneConfig(density, F32, N.SolverConfig(MIXED_MG, F32, 0, 0, 0, 0.0, 0, kernel_variant, 0),
+            N.SceneConfig(density, F32, N.SolverConfig(MIXED_MG, F32, 0, int(mg_sweeps[0]), int(mg_sweeps[1]), 0.0,
+                                                       0, kernel_variant, int(mg_smoother)),
                           settings.native(), actual_divergence_limit, max_courant),
             int(max_cycles), 1 if use_graph else 0)
         grids = (N.Grid * len(self.specs))(*[make_grid(sp) for sp in self.specs])

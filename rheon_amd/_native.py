@@ -55,7 +55,7 @@ class SolverConfig(C.Structure):
         ("mg_omega", C.c_double),
         ("memory_limit", C.c_uint64),
         ("kernel_variant", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("mg_smoother", C.c_uint32),
     ]
 
 

@@ -479,8 +479,11 @@ rh_status Batch::build() {
     grids[i] = mg(i)->dev_grid(density);
   }
   RH_HIP(hipMemcpy(d_grids_, grids.data(), sizeof(DevGrid) * n, hipMemcpyHostToDevice));
-  const int pre = std::max(1, sc.solver.mg_pre_sweeps > 0 ? sc.solver.mg_pre_sweeps : 2);
-  const int post = std::max(1, sc.solver.mg_post_sweeps > 0 ? sc.solver.mg_post_sweeps : 2);
+  int pre = 0, post = 0;
+  {
+    rh_status s = MgSolver::plan_sweeps(sc.solver, &pre, &post);
+    if (s) return s;
+  }
   int64_t cycles = sc.pressure.max_iterations;
   if (cfg_.max_cycles > 0) cycles = std::min<int64_t>(cycles, cfg_.max_cycles);
   if (cycles < 1) cycles = 1;
@@ -637,7 +640,7 @@ rh_status Batch::build() {
     const MgHierarchy* dh = mg(i)->device_hierarchy();
     Scratch& scr = scenes_[i]->solver->impl->scratch();
     outer[i] = BOuterArgs{d_grids_ + i, &dh->lv[0], scenes_[i]->p, scenes_[i]->rhs, (float*)h.b[0],
-                          (float*)h.x[0], scr.ctl, scr.pdd, scr.pmx};
+                          (float*)h.x[0], h.tau_pre[0], scr.ctl, scr.pdd, scr.pmx};
     const bool q = quad_ok && mg_quad_outer_ok(grids[i], h.lv[0], scenes_[i]->p, scenes_[i]->rhs);
     (q ? outer_q : outer_s)[i] = cb[i];
   }
@@ -668,7 +671,8 @@ rh_status Batch::build() {
     }
     // ping-pong state per scene and level: true = current iterate in x
     std::vector<std::vector<bool>> in_x(n, std::vector<bool>(kMgMaxLevels, true));
-    auto smooth_stage = [&](int l, bool to_p) {
+    // sweep `sw` of the pre (post = false) or post side of level l
+    auto smooth_stage = [&](int l, bool to_p, bool post_side, int sw) {
       std::vector<BSmoothArgs> r(n);
       std::vector<int> bq(n, 0), bs(n, 0);
       for (size_t i = 0; i < n; ++i) {
@@ -678,7 +682,7 @@ rh_status Batch::build() {
         const bool cx = in_x[i][l];
         r[i] = BSmoothArgs{&dh->lv[l], (const float*)(cx ? h.x[l] : h.y[l]), (const float*)h.b[l],
                            (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
-                           scenes_[i]->solver->impl->scratch().ctl};
+                           post_side ? h.tau_post[sw] : h.tau_pre[sw], scenes_[i]->solver->impl->scratch().ctl};
         const int blocks = nblk(h.lv[l].ny * h.lv[l].nz);
         const bool q = quad_ok && mg_quad_level(h.lv[l]);
         (q ? bq : bs)[i] = blocks;
@@ -688,7 +692,7 @@ rh_status Batch::build() {
       stage(Op::kSmoothS, r.data(), sizeof(BSmoothArgs), bs);  // the scalar body updates p itself
     };
     for (int l = 0; l < maxL; ++l) {
-      for (int sw = 1; sw < pre; ++sw) smooth_stage(l, false);
+      for (int sw = 1; sw < pre; ++sw) smooth_stage(l, false, false, sw);
       // restriction l -> l+1
       std::vector<BRestrictArgs> r(n);
       std::vector<int> bq(n, 0), bs(n, 0);
@@ -697,7 +701,7 @@ rh_status Batch::build() {
         const MgHierarchy& h = mg(i)->hierarchy();
         const MgHierarchy* dh = mg(i)->device_hierarchy();
         r[i] = BRestrictArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)(in_x[i][l] ? h.x[l] : h.y[l]),
-                             (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1],
+                             (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1], h.tau_pre[0],
                              scenes_[i]->solver->impl->scratch().ctl};
         const int blocks = nblk(h.lv[l + 1].ny * h.lv[l + 1].nz);
         (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]) ? bq : bs)[i] = blocks;
@@ -750,9 +754,9 @@ rh_status Batch::build() {
       for (int sw = 0; sw < post; ++sw) {
         const bool last = sw + 1 == post;
         if (last && l == 0) {
-          smooth_stage(0, true);
+          smooth_stage(0, true, true, sw);
         } else {
-          smooth_stage(l, false);
+          smooth_stage(l, false, true, sw);
         }
       }
       // a coarse level's result must end in x[l] for the parent's prolongation

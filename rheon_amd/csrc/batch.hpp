@@ -67,6 +67,7 @@ struct BOuterArgs {
   const double* b;
   float* b0;
   float* x0;
+  double tau;  // first pre-smoothing sweep weight
   const PcgControl* ctl;
   DD* pdd;
   double* pmx;
@@ -77,6 +78,7 @@ struct BSmoothArgs {
   const float* b;
   float* xout;
   double* p;  // non-null: p += x (finest level, last sweep)
+  double tau;  // sweep weight
   const PcgControl* ctl;
 };
 struct BRestrictArgs {
@@ -86,6 +88,7 @@ struct BRestrictArgs {
   const float* b;
   float* bc;
   float* xc;
+  double tau;  // first coarse pre-smoothing sweep weight
   const PcgControl* ctl;
 };
 struct BProlongArgs {

@@ -23,7 +23,7 @@ struct MgLevel {
   int nx, ny, nz, pad;
   double w[3];
   double diag[64];
-  double odiag[64];  // omega / diag (weighted-Jacobi scale); 0 for diag 0
+  double odiag[64];  // 1 / diag (Jacobi scale); 0 for diag 0
   const unsigned char* mask = nullptr;  // level 0 with obstacles
   const unsigned* frac = nullptr;       // levels >= 1 with obstacles
   // levels >= 1 with obstacles: bit o (o = x | y << 1 | z << 2 parities of a
@@ -31,18 +31,27 @@ struct MgLevel {
   // active coarse cells, so prolongation needs no renormalisation there
   const unsigned char* octant = nullptr;
   double wf[3] = {0.0, 0.0, 0.0};       // w[d] * fscale
-  double omega = 0.0;                   // smoother weight (for frac levels)
 };
 
 constexpr unsigned kFracFluid = 1u << 30;
 
+constexpr int kMgMaxSweeps = 8;   // smoothing sweeps per side of the V-cycle
+
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
 constexpr int kMgTailLds = 56 * 1024;  // LDS bytes for the tail levels' x, y, b arrays
 
+// Smoothing sweep k of a side is x <- x + tau[k] (b - K x) / D.  Jacobi:
+// every tau = omega.  Chebyshev (default): the taus are the reciprocals of
+// the roots of the degree-nu Chebyshev polynomial on [lo, 2] (D^-1 K of a
+// diagonally dominant operator has its spectrum in [0, 2]), so nu sweeps
+// damp every eigencomponent in that interval by 1 / T_nu((2 + lo) /
+// (2 - lo)); the sweeps commute, so their order is free.
 struct MgHierarchy {
   int levels = 0;
   int pre = 2, post = 2;
   double omega = 6.0 / 7.0;
+  double tau_pre[kMgMaxSweeps] = {};
+  double tau_post[kMgMaxSweeps] = {};
   // Coarsest level <= kMgDenseMax cells: G = M^-1 P with M = K + c 1_a 1_a^T
   // over the active (fluid) cells a and P the projection removing the mean
   // over a; column-major n x n f64 (rows and columns of inactive cells are
@@ -67,12 +76,12 @@ bool mg_quad_level(const MgLevel& L);
 bool mg_quad_pair(const MgLevel& F, const MgLevel& C);  // restriction / prolongation F <-> C
 bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, const double* b);
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
-                                float* b0, float* x0, const PcgControl* ctl, DD* pdd, double* pmx,
-                                hipStream_t s);
+                                float* b0, float* x0, double tau, const PcgControl* ctl, DD* pdd,
+                                double* pmx, hipStream_t s);
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
-                        const PcgControl* ctl, hipStream_t s);
+                        double tau, const PcgControl* ctl, hipStream_t s);
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
-                          float* xc, const PcgControl* ctl, hipStream_t s);
+                          float* xc, double tau, const PcgControl* ctl, hipStream_t s);
 void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,
                          float* xout, const PcgControl* ctl, hipStream_t s);
 

@@ -4,10 +4,11 @@
 // even), spacing doubles, and the operator is the box operator rediscretised
 // on the coarse spacing (weights A_d/(rho h_d), the integrated operator of
 // src/operator.rs:58-97 on that grid).  One V(nu1, nu2) cycle:
-//   pre-smooth  x = omega b/D (first sweep from zero), then nu1-1 weighted-
-//               Jacobi sweeps x <- x + omega (b - K x)/D
+//   pre-smooth  x = tau_1 b/D (first sweep from zero), then nu1-1 sweeps
+//               x <- x + tau_k (b - K x)/D (Chebyshev or weighted-Jacobi
+//               weights tau_k, see MgHierarchy)
 //   restrict    b_c = sum over the 8 children of the fine residual
-//               (integrated residuals add), x_c = omega b_c/D_c
+//               (integrated residuals add), x_c = tau_1 b_c/D_c
 //   recurse / coarsest solve
 //   prolong     x += trilinear interpolation of x_c (cell-centred weights
 //               27/64, 9/64, 3/64, 1/64; Neumann reflection at walls)
@@ -32,14 +33,14 @@ namespace rh {
 // ------------------------------------------------------------------ kernels
 
 // Outer FP64 residual r = b - K p (every cell), written as the finest V-cycle
-// right-hand side b0 = S(r) with the first pre-smoothing sweep x0 = omega
+// right-hand side b0 = S(r) with the first pre-smoothing sweep x0 = tau
 // b0/D fused in; partial sum r^2, max|r|, nonfinite flag.
 template <typename S>
 __device__ __forceinline__ void b_mg_outer_residual(VBlock vb, const DevGrid& g, const MgLevel& L0,
                                                     const double* __restrict__ p,
                                                     const double* __restrict__ b, S* __restrict__ b0,
-                                                    S* __restrict__ x0, const PcgControl* ctl, DD* pdd,
-                                                    double* pmx) {
+                                                    S* __restrict__ x0, double tau, const PcgControl* ctl,
+                                                    DD* pdd, double* pmx) {
   if (ctl->done) return;
   DD s[1] = {{0.0, 0.0}};
   double m[2] = {0.0, 0.0};
@@ -51,28 +52,28 @@ __device__ __forceinline__ void b_mg_outer_residual(VBlock vb, const DevGrid& g,
   m[0] = fmax(m[0], fabs(r));
   const S rs = (S)r;
   b0[c] = rs;
-  x0[c] = (S)lv_odiag(L0, c, i, j, k) * rs;
+  x0[c] = lv_scale<S>(L0, tau, c, i, j, k) * rs;
   END_CELLS
   block_publish<1, 2>(s, m, pdd, pmx, vb);
 }
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
 k_mg_outer_residual(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
-                    S* __restrict__ b0, S* __restrict__ x0, double omega, const PcgControl* ctl,
+                    S* __restrict__ b0, S* __restrict__ x0, double tau, const PcgControl* ctl,
                     DD* pdd, double* pmx) {
-  b_mg_outer_residual<S>(hw_block(), g, L0, p, b, b0, x0, ctl, pdd, pmx);
+  b_mg_outer_residual<S>(hw_block(), g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
 }
 
-// One weighted-Jacobi sweep x_out = x_in + omega (b - K x_in)/D.  With
+// One smoothing sweep x_out = x_in + tau (b - K x_in)/D.  With
 // `p` non-null (finest level, last post-sweep) also p += x_out.
 template <typename S>
 __device__ __forceinline__ void b_mg_smooth(VBlock vb, const MgLevel& L, const S* __restrict__ xin,
                                             const S* __restrict__ b, S* __restrict__ xout,
-                                            double* __restrict__ p, const PcgControl* ctl) {
+                                            double* __restrict__ p, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
   FOR_LEVEL_CELLS_VB(L, vb)
   const S ax = s_apply<S>(L, xin, c, i, j, k);
-  const S xn = xin[c] + (S)lv_odiag(L, c, i, j, k) * (b[c] - ax);
+  const S xn = xin[c] + lv_scale<S>(L, tau, c, i, j, k) * (b[c] - ax);
   if (p) {
     p[c] = p[c] + (double)xn;
   } else {
@@ -83,8 +84,8 @@ __device__ __forceinline__ void b_mg_smooth(VBlock vb, const MgLevel& L, const S
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __restrict__ xout,
-            double omega, double* __restrict__ p, const PcgControl* ctl) {
-  b_mg_smooth<S>(hw_block(), L, xin, b, xout, p, ctl);
+            double tau, double* __restrict__ p, const PcgControl* ctl) {
+  b_mg_smooth<S>(hw_block(), L, xin, b, xout, p, tau, ctl);
 }
 
 // Restriction: coarse cell (I,J,K) sums the residual b - K x of its 8
@@ -92,7 +93,7 @@ k_mg_smooth(MgLevel L, const S* __restrict__ xin, const S* __restrict__ b, S* __
 template <typename S>
 __device__ __forceinline__ void b_mg_restrict(VBlock vb, const MgLevel& F, const MgLevel& Cl,
                                               const S* __restrict__ x, const S* __restrict__ b,
-                                              S* __restrict__ bc, S* __restrict__ xc,
+                                              S* __restrict__ bc, S* __restrict__ xc, double tau,
                                               const PcgControl* ctl) {
   if (ctl->done) return;
   FOR_LEVEL_CELLS_VB(Cl, vb)
@@ -109,14 +110,14 @@ __device__ __forceinline__ void b_mg_restrict(VBlock vb, const MgLevel& F, const
         sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
       }
   bc[c] = sum;
-  xc[c] = (S)lv_odiag(Cl, c, i, j, k) * sum;
+  xc[c] = lv_scale<S>(Cl, tau, c, i, j, k) * sum;
   END_CELLS
 }
 template <typename S>
 __global__ void __launch_bounds__(kBlock)
 k_mg_restrict(MgLevel F, MgLevel Cl, const S* __restrict__ x, const S* __restrict__ b,
-              S* __restrict__ bc, S* __restrict__ xc, double omega, const PcgControl* ctl) {
-  b_mg_restrict<S>(hw_block(), F, Cl, x, b, bc, xc, ctl);
+              S* __restrict__ bc, S* __restrict__ xc, double tau, const PcgControl* ctl) {
+  b_mg_restrict<S>(hw_block(), F, Cl, x, b, bc, xc, tau, ctl);
 }
 
 // Trilinear prolongation of the coarse correction, added to the fine x.
@@ -260,12 +261,12 @@ constexpr int kTailThreads = 1024;
 
 template <typename S>
 __device__ __forceinline__ void tail_smooth(const MgLevel& L, const S* __restrict__ xin,
-                                            const S* __restrict__ b, S* __restrict__ xout) {
+                                            const S* __restrict__ b, S* __restrict__ xout, double tau) {
   const int n = L.nx * L.ny * L.nz;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
     const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
     const S ax = s_apply<S>(L, xin, c, i, j, k);
-    xout[c] = xin[c] + (S)lv_odiag(L, c, i, j, k) * (b[c] - ax);
+    xout[c] = xin[c] + lv_scale<S>(L, tau, c, i, j, k) * (b[c] - ax);
   }
   __syncthreads();
 }
@@ -273,7 +274,7 @@ __device__ __forceinline__ void tail_smooth(const MgLevel& L, const S* __restric
 template <typename S>
 __device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& Cl, const S* __restrict__ x,
                                               const S* __restrict__ b, S* __restrict__ bc,
-                                              S* __restrict__ xc) {
+                                              S* __restrict__ xc, double tau) {
   const int n = Cl.nx * Cl.ny * Cl.nz;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
     const int i = c % Cl.nx, j = (c / Cl.nx) % Cl.ny, k = c / (Cl.nx * Cl.ny);
@@ -287,7 +288,7 @@ __device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& C
           sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
         }
     bc[c] = sum;
-    xc[c] = (S)lv_odiag(Cl, c, i, j, k) * sum;
+    xc[c] = lv_scale<S>(Cl, tau, c, i, j, k) * sum;
   }
   __syncthreads();
 }
@@ -343,12 +344,12 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
   for (int l = l0; l < levels - 1; ++l) {
     const MgLevel& L = H->lv[l];
     for (int sweep = 1; sweep < H->pre; ++sweep) {
-      tail_smooth<S>(L, cur[l], B(l), nxt[l]);
+      tail_smooth<S>(L, cur[l], B(l), nxt[l], H->tau_pre[sweep]);
       S* t = cur[l];
       cur[l] = nxt[l];
       nxt[l] = t;
     }
-    tail_restrict<S>(L, H->lv[l + 1], cur[l], B(l), B(l + 1), X(l + 1));
+    tail_restrict<S>(L, H->lv[l + 1], cur[l], B(l), B(l + 1), X(l + 1), H->tau_pre[0]);
   }
   // coarsest: dense projected inverse (callers guarantee n <= kMgDenseMax)
   {
@@ -364,7 +365,7 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
       nxt[l] = t;
     }
     for (int sweep = 0; sweep < H->post; ++sweep) {
-      tail_smooth<S>(L, cur[l], B(l), nxt[l]);
+      tail_smooth<S>(L, cur[l], B(l), nxt[l], H->tau_post[sweep]);
       S* t = cur[l];
       cur[l] = nxt[l];
       nxt[l] = t;
@@ -427,17 +428,17 @@ k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl) {
 __global__ void __launch_bounds__(kBlock)
 kb_mg_outer_residual(const BOuterArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BOuterArgs)
-  b_mg_outer_residual<float>(vb, *a.g, *a.L0, a.p, a.b, a.b0, a.x0, a.ctl, a.pdd, a.pmx);
+  b_mg_outer_residual<float>(vb, *a.g, *a.L0, a.p, a.b, a.b0, a.x0, a.tau, a.ctl, a.pdd, a.pmx);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_mg_smooth(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BSmoothArgs)
-  b_mg_smooth<float>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.ctl);
+  b_mg_smooth<float>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_mg_restrict(const BRestrictArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BRestrictArgs)
-  b_mg_restrict<float>(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.ctl);
+  b_mg_restrict<float>(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_mg_prolong(const BProlongArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
@@ -481,10 +482,11 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
   {
     prof::Scope ps(prof::kOuterResidual, s, it);
     if (sizeof(S) == 4 && h.quad && mg_quad_outer_ok(g, h.lv[0], p, b))
-      launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], (float*)h.x[0], ctl, pdd, pmx, s);
+      launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], (float*)h.x[0], h.tau_pre[0], ctl, pdd,
+                                 pmx, s);
     else
       k_mg_outer_residual<S><<<grid, block_shape(), 0, s>>>(g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
-                                                            h.omega, ctl, pdd, pmx);
+                                                            h.tau_pre[0], ctl, pdd, pmx);
   }
   launch_finish<1, 2>(pdd, pmx, grid.x, FinMgTrue{ctl, it}, s);
 }
@@ -494,14 +496,14 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
 static inline int mg_slot(int l, int finest_slot) { return l == 0 ? finest_slot : prof::kMgCoarse; }
 
 template <typename S>
-static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p,
+static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p, double tau,
                    const PcgControl* ctl, int it, hipStream_t s) {
   prof::Scope ps(mg_slot(l, p ? prof::kMgSmoothUpdate : prof::kMgSmooth), s, l == 0 ? it : -2);
   if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]) && ((uintptr_t)p & 15u) == 0)
-    launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, ctl, s);
+    launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, tau, ctl, s);
   else
     k_mg_smooth<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], xin, (const S*)h.b[l], xout,
-                                                                 h.omega, p, ctl);
+                                                                 tau, p, ctl);
 }
 
 // V-cycle from level l; on entry x[l] holds the first pre-smoothing sweep.
@@ -535,7 +537,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   S* cur = x;
   S* nxt = y;
   for (int sweep = 1; sweep < h.pre; ++sweep) {
-    smooth<S>(h, l, cur, nxt, nullptr, ctl, it, s);
+    smooth<S>(h, l, cur, nxt, nullptr, h.tau_pre[sweep], ctl, it, s);
     S* t = cur;
     cur = nxt;
     nxt = t;
@@ -544,10 +546,10 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
     prof::Scope ps(mg_slot(l, prof::kMgRestrict), s, l == 0 ? it : -2);
     if (sizeof(S) == 4 && h.quad && mg_quad_pair(h.lv[l], h.lv[l + 1]))
       launch_mg_restrict_q(h.lv[l], h.lv[l + 1], (const float*)cur, (const float*)b, (float*)h.b[l + 1],
-                           (float*)h.x[l + 1], ctl, s);
+                           (float*)h.x[l + 1], h.tau_pre[0], ctl, s);
     else
       k_mg_restrict<S><<<level_grid(h.lv[l + 1]), block_shape(), 0, s>>>(
-          h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.omega, ctl);
+          h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.tau_pre[0], ctl);
   }
   vcycle<S>(h, l + 1, p, ctl, it, s);
   {
@@ -567,9 +569,9 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   for (int sweep = 0; sweep < h.post; ++sweep) {
     const bool last = sweep + 1 == h.post;
     if (last && l == 0) {
-      smooth<S>(h, l, cur, nxt, p, ctl, it, s);  // p += x (finest, last sweep)
+      smooth<S>(h, l, cur, nxt, p, h.tau_post[sweep], ctl, it, s);  // p += x (finest, last sweep)
     } else {
-      smooth<S>(h, l, cur, nxt, nullptr, ctl, it, s);
+      smooth<S>(h, l, cur, nxt, nullptr, h.tau_post[sweep], ctl, it, s);
       S* t = cur;
       cur = nxt;
       nxt = t;

@@ -128,8 +128,8 @@ __device__ __forceinline__ double quad_apply_d(const Quad<double>& Q, int u, con
 // (as k_mg_outer_residual).
 __device__ __forceinline__ void
 b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const double* __restrict__ p, const double* __restrict__ b,
-                      float* __restrict__ b0, float* __restrict__ x0, const PcgControl* ctl, DD* pdd,
-                      double* pmx) {
+                      float* __restrict__ b0, float* __restrict__ x0, double tau, const PcgControl* ctl,
+                      DD* pdd, double* pmx) {
   if (ctl->done) return;
   DD s[1] = {{0.0, 0.0}};
   double m[2] = {0.0, 0.0};
@@ -151,7 +151,7 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
     dd_acc(s[0], r * r);
     m[0] = fmax(m[0], fabs(r));
     rb[u] = (float)r;
-    rx[u] = (float)L0.odiag[mk] * rb[u];
+    rx[u] = (float)(tau * L0.odiag[mk]) * rb[u];
   }
   store4(b0, c, rb);
   store4(x0, c, rx);
@@ -159,12 +159,12 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   block_publish<1, 2>(s, m, pdd, pmx, vb);
 }
 
-// One weighted-Jacobi sweep (as k_mg_smooth); kUpdate: p += x_out instead of
+// One smoothing sweep (as k_mg_smooth); kUpdate: p += x_out instead of
 // storing x_out (the finest level's last sweep).
 template <bool kUpdate>
 __device__ __forceinline__ void
 b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const float* __restrict__ b,
-              float* __restrict__ xout, double* __restrict__ p, const PcgControl* ctl) {
+              float* __restrict__ xout, double* __restrict__ p, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
   const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
   const int nz_ = L.nz;
@@ -182,7 +182,7 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
     const int mk = quad_mask(L.mask, m4, L.nx, L.ny, L.nz, i0 + u, j, k, u);
     if (L.mask) close_faces(Q, u, mk);
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
-    xn[u] = Q.c[u] + (float)L.odiag[mk] * (bb[u] - ax);
+    xn[u] = Q.c[u] + (float)(tau * L.odiag[mk]) * (bb[u] - ax);
   }
   if (kUpdate) {
 #pragma unroll
@@ -199,7 +199,7 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
 // the four fine rows (2J + dy, 2K + dz).  Sums keep the (dz, dy, dx) order.
 __device__ __forceinline__ void
 b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __restrict__ x, const float* __restrict__ b,
-                float* __restrict__ bc, float* __restrict__ xc, const PcgControl* ctl) {
+                float* __restrict__ bc, float* __restrict__ xc, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
   const float w0 = (float)F.w[0], w1 = (float)F.w[1], w2 = (float)F.w[2];
   const int nz_ = Cl.nz;
@@ -232,7 +232,7 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
     }
   const int cc = I0 + Cl.nx * (J + Cl.ny * K);
   *reinterpret_cast<float2*>(bc + cc) = make_float2(s0, s1);
-  const float o0 = (float)lv_odiag(Cl, cc, I0, J, K), o1 = (float)lv_odiag(Cl, cc + 1, I0 + 1, J, K);
+  const float o0 = lv_scale<float>(Cl, tau, cc, I0, J, K), o1 = lv_scale<float>(Cl, tau, cc + 1, I0 + 1, J, K);
   *reinterpret_cast<float2*>(xc + cc) = make_float2(o0 * s0, o1 * s1);
   END_QUADS
 }
@@ -334,20 +334,20 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
 
 __global__ void __launch_bounds__(kBlock)
 k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
-                      float* __restrict__ b0, float* __restrict__ x0, const PcgControl* ctl, DD* pdd,
-                      double* pmx) {
-  b_mg_outer_residual_q(hw_block(), g, L0, p, b, b0, x0, ctl, pdd, pmx);
+                      float* __restrict__ b0, float* __restrict__ x0, double tau, const PcgControl* ctl,
+                      DD* pdd, double* pmx) {
+  b_mg_outer_residual_q(hw_block(), g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
 }
 template <bool kUpdate>
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
-              float* __restrict__ xout, double* __restrict__ p, const PcgControl* ctl) {
-  b_mg_smooth_q<kUpdate>(hw_block(), L, xin, b, xout, p, ctl);
+              float* __restrict__ xout, double* __restrict__ p, double tau, const PcgControl* ctl) {
+  b_mg_smooth_q<kUpdate>(hw_block(), L, xin, b, xout, p, tau, ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float* __restrict__ b,
-                float* __restrict__ bc, float* __restrict__ xc, const PcgControl* ctl) {
-  b_mg_restrict_q(hw_block(), F, Cl, x, b, bc, xc, ctl);
+                float* __restrict__ bc, float* __restrict__ xc, double tau, const PcgControl* ctl) {
+  b_mg_restrict_q(hw_block(), F, Cl, x, b, bc, xc, tau, ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float* __restrict__ xin,
@@ -364,18 +364,18 @@ k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float*
 __global__ void __launch_bounds__(kBlock)
 kb_mg_outer_residual_q(const BOuterArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BOuterArgs)
-  b_mg_outer_residual_q(vb, *a.g, *a.L0, a.p, a.b, a.b0, a.x0, a.ctl, a.pdd, a.pmx);
+  b_mg_outer_residual_q(vb, *a.g, *a.L0, a.p, a.b, a.b0, a.x0, a.tau, a.ctl, a.pdd, a.pmx);
 }
 template <bool kUpdate>
 __global__ void __launch_bounds__(kBlock)
 kb_mg_smooth_q(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BSmoothArgs)
-  b_mg_smooth_q<kUpdate>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.ctl);
+  b_mg_smooth_q<kUpdate>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_mg_restrict_q(const BRestrictArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BRestrictArgs)
-  b_mg_restrict_q(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.ctl);
+  b_mg_restrict_q(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_mg_prolong_q(const BProlongArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
@@ -417,22 +417,23 @@ bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, cons
 }
 
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
-                                float* b0, float* x0, const PcgControl* ctl, DD* pdd, double* pmx,
-                                hipStream_t s) {
-  k_mg_outer_residual_q<<<cells_grid(g), block_shape(), 0, s>>>(g, L0, p, b, b0, x0, ctl, pdd, pmx);
+                                float* b0, float* x0, double tau, const PcgControl* ctl, DD* pdd,
+                                double* pmx, hipStream_t s) {
+  k_mg_outer_residual_q<<<cells_grid(g), block_shape(), 0, s>>>(g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
 }
 
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
-                        const PcgControl* ctl, hipStream_t s) {
+                        double tau, const PcgControl* ctl, hipStream_t s) {
+  const dim3 grid(grid_blocks(L.ny * L.nz));
   if (p)
-    k_mg_smooth_q<true><<<dim3(grid_blocks(L.ny * L.nz)), block_shape(), 0, s>>>(L, xin, b, xout, p, ctl);
+    k_mg_smooth_q<true><<<grid, block_shape(), 0, s>>>(L, xin, b, xout, p, tau, ctl);
   else
-    k_mg_smooth_q<false><<<dim3(grid_blocks(L.ny * L.nz)), block_shape(), 0, s>>>(L, xin, b, xout, p, ctl);
+    k_mg_smooth_q<false><<<grid, block_shape(), 0, s>>>(L, xin, b, xout, p, tau, ctl);
 }
 
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
-                          float* xc, const PcgControl* ctl, hipStream_t s) {
-  k_mg_restrict_q<<<dim3(grid_blocks(Cl.ny * Cl.nz)), block_shape(), 0, s>>>(F, Cl, x, b, bc, xc, ctl);
+                          float* xc, double tau, const PcgControl* ctl, hipStream_t s) {
+  k_mg_restrict_q<<<dim3(grid_blocks(Cl.ny * Cl.nz)), block_shape(), 0, s>>>(F, Cl, x, b, bc, xc, tau, ctl);
 }
 
 void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,

@@ -77,15 +77,21 @@ __device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, 
   return w0 * ((xc - a) + (xc - b)) + w1 * ((xc - cy) + (xc - d)) + w2 * ((xc - e) + (xc - f));
 }
 
-// omega / D (weighted-Jacobi scale) of the cell; 0 where D = 0 (solid).
+// 1 / D (Jacobi scale) of the cell; 0 where D = 0 (solid).
 __device__ __forceinline__ double lv_odiag(const MgLevel& L, int c, int i, int j, int k) {
   if (L.frac) {
     const Frac6 f = lv_frac6(L, c, i, j, k);
     const double d = L.wf[0] * (double)(f.xm + f.xp) + L.wf[1] * (double)(f.ym + f.yp) +
                      L.wf[2] * (double)(f.zm + f.zp);
-    return d > 0.0 ? L.omega / d : 0.0;
+    return d > 0.0 ? 1.0 / d : 0.0;
   }
   return L.odiag[lv_mask(L, c, i, j, k)];
+}
+
+// Sweep scale tau / D in the V-cycle precision.
+template <typename S>
+__device__ __forceinline__ S lv_scale(const MgLevel& L, double tau, int c, int i, int j, int k) {
+  return (S)(tau * lv_odiag(L, c, i, j, k));
 }
 
 // Fine cell c of level F holds fluid (always in the plain box).

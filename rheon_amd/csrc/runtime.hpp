@@ -126,6 +126,7 @@ class MgSolver : public Solver {
   int levels() const { return h_.levels; }
   static int plan_levels(const rh_grid& grid, const rh_solver_config& config, int dims[][3], int* padded);
   static uint64_t plan_bytes(const rh_grid& grid, const rh_solver_config& config);
+  static rh_status plan_sweeps(const rh_solver_config& config, int* pre, int* post);
   rh_status set_solid(const uint8_t* solid) override;
   // Level data and coarsest inverse for `density` on the device (done by
   // every solve; batched runners call it before building launch tables).

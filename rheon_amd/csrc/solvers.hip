@@ -234,6 +234,27 @@ rh_status PcgSolver::solve(double density, const double* d_rhs, double dt,
 
 // -------------------------------------------------------------- multigrid
 
+// Sweep counts of the V-cycle: explicit, or the smoother's default.
+rh_status MgSolver::plan_sweeps(const rh_solver_config& config, int* pre, int* post) {
+  if (config.mg_smoother != RH_SMOOTHER_CHEBYSHEV && config.mg_smoother != RH_SMOOTHER_JACOBI)
+    return RH_INVALID_ARGUMENT;
+  const bool jacobi = config.mg_smoother == RH_SMOOTHER_JACOBI;
+  *pre = config.mg_pre_sweeps > 0 ? config.mg_pre_sweeps : (jacobi ? 2 : 3);
+  *post = config.mg_post_sweeps > 0 ? config.mg_post_sweeps : (jacobi ? 2 : 1);
+  if (*pre > kMgMaxSweeps || *post > kMgMaxSweeps || config.mg_omega < 0.0) return RH_INVALID_ARGUMENT;
+  return RH_OK;
+}
+
+// Weights tau_k of n sweeps: weighted Jacobi (all omega) or the reciprocal
+// roots lambda_k = c + h cos((2k + 1) pi / 2n), k = 0..n-1, of the degree-n
+// Chebyshev polynomial on [lo, 2] (c, h = centre and half-width).
+static void sweep_weights(int n, bool jacobi, double omega, double* tau) {
+  const double lo = n <= 2 ? 0.3 : 0.25, hi = 2.0;
+  const double c = 0.5 * (hi + lo), h = 0.5 * (hi - lo);
+  for (int k = 0; k < n; ++k)
+    tau[k] = jacobi ? omega : 1.0 / (c + h * std::cos(M_PI * (2.0 * k + 1.0) / (2.0 * n)));
+}
+
 // Level shapes of the hierarchy for this grid (shared by create and the
 // memory plan): returns the level count; *padded = first padded level.
 int MgSolver::plan_levels(const rh_grid& grid, const rh_solver_config& config, int dims[][3],
@@ -297,9 +318,11 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
   if (config.precision != RH_F64 && config.precision != RH_F32) return RH_INVALID_ARGUMENT;
   rh_status st = scratch_.init();
   if (st) return st;
-  h_.pre = config.mg_pre_sweeps > 0 ? config.mg_pre_sweeps : 2;
-  h_.post = config.mg_post_sweeps > 0 ? config.mg_post_sweeps : 2;
+  st = plan_sweeps(config, &h_.pre, &h_.post);
+  if (st) return st;
   h_.omega = config.mg_omega > 0.0 ? config.mg_omega : 6.0 / 7.0;
+  sweep_weights(h_.pre, config.mg_smoother == RH_SMOOTHER_JACOBI, h_.omega, h_.tau_pre);
+  sweep_weights(h_.post, config.mg_smoother == RH_SMOOTHER_JACOBI, h_.omega, h_.tau_post);
   h_.quad = (config.kernel_variant & RH_VARIANT_SCALAR_MG) == 0;
   // Coarsen while every axis is even and the level is above the one-
   // workgroup coarsest size (or up to the requested level count).
@@ -598,13 +621,12 @@ rh_status MgSolver::set_density(double density) {
     const double area[3] = {h[1] * h[2], h[0] * h[2], h[0] * h[1]};
     for (int a = 0; a < 3; ++a) L.w[a] = area[a] * (1.0 / (density * h[a]));
     for (int a = 0; a < 3; ++a) L.wf[a] = L.w[a] * (l > 0 ? fscale_[l] : 1.0);
-    L.omega = h_.omega;
     for (int mask = 0; mask < 64; ++mask) {
       double v = 0.0;
       for (int side = 0; side < 6; ++side)
         if (mask & (1 << side)) v += L.w[side / 2];
       L.diag[mask] = v;
-      L.odiag[mask] = v > 0.0 ? h_.omega / v : 0.0;
+      L.odiag[mask] = v > 0.0 ? 1.0 / v : 0.0;
     }
     for (int a = 0; a < 3; ++a) h[a] *= 2.0;
   }

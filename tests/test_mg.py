@@ -34,6 +34,36 @@ def test_mg_statement_levels_and_transfers():
     assert abs(M.restrict(r).sum() - r.sum()) < 1e-12
 
 
+def test_mg_statement_chebyshev_weights():
+    """n sweeps with the Chebyshev weights apply prod_k (1 - tau_k lam) to
+    an eigencomponent lam of D^-1 K: on [lo, 2] its largest magnitude is
+    1 / T_n((2 + lo) / (2 - lo)) (T_n = cosh(n acosh .)), reached with
+    alternating signs at the n + 1 mapped extrema of T_n."""
+    for n in range(1, 7):
+        tau = M.sweep_weights(n)
+        lo = 0.3 if n <= 2 else 0.25
+        poly = lambda lam: np.prod([1.0 - t * lam for t in tau], axis=0)
+        bound = 1.0 / np.cosh(n * np.arccosh((2 + lo) / (2 - lo)))
+        lam = np.linspace(lo, 2.0, 20001)
+        assert abs(np.max(np.abs(poly(lam))) - bound) < 1e-9
+        ext = 0.5 * (2 + lo) + 0.5 * (2 - lo) * np.cos(np.pi * np.arange(n + 1) / n)
+        vals = poly(ext)
+        assert np.allclose(np.abs(vals), bound, rtol=1e-12)
+        assert np.all(np.sign(vals[1:]) != np.sign(vals[:-1]))
+        assert poly(np.array([0.0]))[0] == 1.0  # the constant (coarse) mode is left to the coarse grid
+    assert M.sweep_weights(3, "jacobi") == [6.0 / 7.0] * 3
+
+
+def test_mg_statement_chebyshev_beats_jacobi():
+    """Same sweep count per cycle, fewer cycles: Chebyshev V(2,2) vs Jacobi V(2,2)."""
+    spec, vel, dt = config1(32, seed=3)
+    rhs = O.build_rhs(ogrid(spec), vel, dt)
+    args = (spec.counts, spec.spacing, 1.0, rhs, dt, 1e-10, 1e-14, 1.0, 60)
+    _, _, cj, _, _ = M.solve(*args, pre=2, post=2, smoother="jacobi")
+    _, _, cc, _, _ = M.solve(*args, pre=2, post=2)
+    assert cc < cj
+
+
 def test_mg_statement_converges_to_reference_pressure():
     spec, vel, dt = config1(32, seed=3)
     g = ogrid(spec)
