@@ -518,6 +518,7 @@ rh_status Batch::build() {
   program_.clear();
   const bool quad_ok = (variant_ & RH_VARIANT_SCALAR_MG) == 0;
   auto nblk = [&](int rows) { return std::max(1, grid_blocks(rows) / block_div_); };
+  auto qblk = [&](int blocks) { return std::max(1, blocks / block_div_); };  // quad launches
   auto face_blocks = [&](size_t i) { return nblk((int)((grids[i].ny + 1) * (grids[i].nz + 1))); };
   auto cell_blocks = [&](size_t i) { return nblk(grids[i].ny * grids[i].nz); };
   // a stage: records for all scenes + blocks per scene (0 = not taking part)
@@ -642,10 +643,13 @@ rh_status Batch::build() {
     outer[i] = BOuterArgs{d_grids_ + i, &dh->lv[0], scenes_[i]->p, scenes_[i]->rhs, (float*)h.b[0],
                           (float*)h.x[0], h.tau_pre[0], scr.ctl, scr.pdd, scr.pmx};
     const bool q = quad_ok && mg_quad_outer_ok(grids[i], h.lv[0], scenes_[i]->p, scenes_[i]->rhs);
-    (q ? outer_q : outer_s)[i] = cb[i];
+    if (q)
+      outer_q[i] = qblk(mg_quad_outer_blocks(grids[i]));
+    else
+      outer_s[i] = cb[i];
   }
   std::vector<int> outer_nb(n);
-  for (size_t i = 0; i < n; ++i) outer_nb[i] = cb[i];
+  for (size_t i = 0; i < n; ++i) outer_nb[i] = outer_q[i] + outer_s[i];
   auto outer_stage = [&](int it) {
     stage(Op::kOuterQ, outer.data(), sizeof(BOuterArgs), outer_q);
     stage(Op::kOuterS, outer.data(), sizeof(BOuterArgs), outer_s);
@@ -683,9 +687,11 @@ rh_status Batch::build() {
         r[i] = BSmoothArgs{&dh->lv[l], (const float*)(cx ? h.x[l] : h.y[l]), (const float*)h.b[l],
                            (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
                            post_side ? h.tau_post[sw] : h.tau_pre[sw], scenes_[i]->solver->impl->scratch().ctl};
-        const int blocks = nblk(h.lv[l].ny * h.lv[l].nz);
         const bool q = quad_ok && mg_quad_level(h.lv[l]);
-        (q ? bq : bs)[i] = blocks;
+        if (q)
+          bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
+        else
+          bs[i] = nblk(h.lv[l].ny * h.lv[l].nz);
         if (!to_p) in_x[i][l] = !cx;
       }
       stage(to_p ? Op::kSmoothQUpdate : Op::kSmoothQ, r.data(), sizeof(BSmoothArgs), bq);
@@ -703,8 +709,10 @@ rh_status Batch::build() {
         r[i] = BRestrictArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)(in_x[i][l] ? h.x[l] : h.y[l]),
                              (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1], h.tau_pre[0],
                              scenes_[i]->solver->impl->scratch().ctl};
-        const int blocks = nblk(h.lv[l + 1].ny * h.lv[l + 1].nz);
-        (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]) ? bq : bs)[i] = blocks;
+        if (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]))
+          bq[i] = qblk(mg_quad_restrict_blocks(h.lv[l + 1]));
+        else
+          bs[i] = nblk(h.lv[l + 1].ny * h.lv[l + 1].nz);
         in_x[i][l + 1] = true;
       }
       stage(Op::kRestrictQ, r.data(), sizeof(BRestrictArgs), bq);
@@ -745,8 +753,10 @@ rh_status Batch::build() {
         r[i] = BProlongArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)h.x[l + 1],
                             (const float*)(cx ? h.x[l] : h.y[l]), (float*)(cx ? h.y[l] : h.x[l]),
                             scenes_[i]->solver->impl->scratch().ctl};
-        const int blocks = nblk(h.lv[l].ny * h.lv[l].nz);
-        (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]) ? bq : bs)[i] = blocks;
+        if (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]))
+          bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
+        else
+          bs[i] = nblk(h.lv[l].ny * h.lv[l].nz);
         in_x[i][l] = !cx;
       }
       stage(Op::kProlongQ, r.data(), sizeof(BProlongArgs), bq);

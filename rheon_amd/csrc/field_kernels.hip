@@ -7,6 +7,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 #include "batch.hpp"
+#include "profiler.hpp"
 
 namespace rh {
 
@@ -441,7 +442,7 @@ void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w
                    void* ou, void* ov, void* ow, unsigned* flags, hipStream_t s) {
   const int rows = (g.ny + 1) * (g.nz + 1);
   dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
-  k_advect_velocity<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, dt,
+  launch(k_advect_velocity<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
                                               (T*)ou, (T*)ov, (T*)ow, flags);
 }
 
@@ -458,7 +459,7 @@ template <typename T>
 void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
                 double* out, unsigned* flags, hipStream_t s) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
-  k_build_rhs<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
+  launch(k_build_rhs<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
 }
 
 template <typename T>
@@ -466,14 +467,14 @@ void launch_correct(const DevGrid& g, const double* p, double dt, void* u, void*
                     unsigned* flags, hipStream_t s) {
   const int rows = (g.ny + 1) * (g.nz + 1);
   dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
-  k_correct<T><<<grid, block, 0, s>>>(g, p, dt, (T*)u, (T*)v, (T*)w, flags);
+  launch(k_correct<T>, grid, block, 0, s, g, p, dt, (T*)u, (T*)v, (T*)w, flags);
 }
 
 template <typename T>
 void launch_divergence_max(const DevGrid& g, const void* u, const void* v, const void* w,
                            const Reduce& red, hipStream_t s) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
-  k_divergence_max<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, red);
+  launch(k_divergence_max<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, red);
   launch_finish<1, 1>(red.pdd, red.pmx, grid.x, FinStore{red.out, 0, 0.0, 0.0, 0.0}, s);
 }
 

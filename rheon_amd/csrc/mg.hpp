@@ -75,6 +75,11 @@ struct MgHierarchy {
 bool mg_quad_level(const MgLevel& L);
 bool mg_quad_pair(const MgLevel& F, const MgLevel& C);  // restriction / prolongation F <-> C
 bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, const double* b);
+// Block counts of the four-cells-per-lane launches (one-scene grids; a
+// batched stage uses the same counts per scene).
+int mg_quad_outer_blocks(const DevGrid& g);
+inline int mg_quad_level_blocks(const MgLevel& L) { return quad_blocks(L.nx / 4, L.ny * L.nz); }
+inline int mg_quad_restrict_blocks(const MgLevel& C) { return quad_blocks(C.nx / 2, C.ny * C.nz); }
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
                                 float* b0, float* x0, double tau, const PcgControl* ctl, DD* pdd,
                                 double* pmx, hipStream_t s);

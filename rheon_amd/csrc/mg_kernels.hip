@@ -478,14 +478,16 @@ static inline dim3 level_grid(const MgLevel& L) { return dim3(grid_blocks(L.ny *
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
                        PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s) {
-  const dim3 grid = cells_grid(g);
+  dim3 grid = cells_grid(g);
   {
     prof::Scope ps(prof::kOuterResidual, s, it);
-    if (sizeof(S) == 4 && h.quad && mg_quad_outer_ok(g, h.lv[0], p, b))
+    const bool quad = sizeof(S) == 4 && h.quad && mg_quad_outer_ok(g, h.lv[0], p, b);
+    if (quad) grid = dim3(mg_quad_outer_blocks(g));
+    if (quad)
       launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], (float*)h.x[0], h.tau_pre[0], ctl, pdd,
                                  pmx, s);
     else
-      k_mg_outer_residual<S><<<grid, block_shape(), 0, s>>>(g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
+      launch(k_mg_outer_residual<S>, grid, block_shape(), 0, s, g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
                                                             h.tau_pre[0], ctl, pdd, pmx);
   }
   launch_finish<1, 2>(pdd, pmx, grid.x, FinMgTrue{ctl, it}, s);
@@ -502,7 +504,7 @@ static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p
   if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]) && ((uintptr_t)p & 15u) == 0)
     launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, tau, ctl, s);
   else
-    k_mg_smooth<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], xin, (const S*)h.b[l], xout,
+    launch(k_mg_smooth<S>, level_grid(h.lv[l]), block_shape(), 0, s, h.lv[l], xin, (const S*)h.b[l], xout,
                                                                  tau, p, ctl);
 }
 
@@ -515,15 +517,15 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   const S* b = (const S*)h.b[l];
   if (h.tail_level > 0 && l == h.tail_level) {
     prof::Scope ps(prof::kMgCoarse, s, -2);
-    k_mg_tail<S><<<1, kTailThreads, h.tail_lds, s>>>(h.d_self, l, ctl);
+    launch(k_mg_tail<S>, 1, kTailThreads, h.tail_lds, s, h.d_self, l, ctl);
     return;
   }
   if (l == h.levels - 1) {
     prof::Scope ps(prof::kMgCoarse, s, -2);
     if (h.coarse_inv)
-      k_mg_coarsest_dense<S><<<1, kDenseThreads, 0, s>>>(h.lv[l], h.coarse_inv, b, x, ctl);
+      launch(k_mg_coarsest_dense<S>, 1, kDenseThreads, 0, s, h.lv[l], h.coarse_inv, b, x, ctl);
     else
-      k_mg_coarsest<S><<<1, kCoarseThreads, 0, s>>>(h.lv[l], b, x, (PcgControl*)ctl);
+      launch(k_mg_coarsest<S>, 1, kCoarseThreads, 0, s, h.lv[l], b, x, (PcgControl*)ctl);
     if (l == 0) {
       DevGrid g{};
       g.nx = h.lv[0].nx;
@@ -548,7 +550,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
       launch_mg_restrict_q(h.lv[l], h.lv[l + 1], (const float*)cur, (const float*)b, (float*)h.b[l + 1],
                            (float*)h.x[l + 1], h.tau_pre[0], ctl, s);
     else
-      k_mg_restrict<S><<<level_grid(h.lv[l + 1]), block_shape(), 0, s>>>(
+      launch(k_mg_restrict<S>, level_grid(h.lv[l + 1]), block_shape(), 0, s,
           h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.tau_pre[0], ctl);
   }
   vcycle<S>(h, l + 1, p, ctl, it, s);
@@ -558,7 +560,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
       launch_mg_prolong_q(h.lv[l], h.lv[l + 1], (const float*)h.x[l + 1], (const float*)cur, (float*)nxt,
                           ctl, s);
     else
-      k_mg_prolong<S><<<level_grid(h.lv[l]), block_shape(), 0, s>>>(h.lv[l], h.lv[l + 1],
+      launch(k_mg_prolong<S>, level_grid(h.lv[l]), block_shape(), 0, s, h.lv[l], h.lv[l + 1],
                                                                      (const S*)h.x[l + 1], cur, nxt, ctl);
   }
   {

@@ -22,9 +22,8 @@ namespace {
 #define FOR_QUADS(NX, NY)                                                   \
   {                                                                         \
     const int cpr_ = (NX) >> 2;                                             \
-    const RowRange rr_ = block_rows((NY) * nz_, vb);                        \
-    const int t_end_ = rr_.end * cpr_;                                      \
-    for (int t_ = rr_.begin * cpr_ + flat_tid(); t_ < t_end_; t_ += kBlock) { \
+    const RowRange qr_ = block_span(cpr_ * (NY) * nz_, vb);                 \
+    for (int t_ = qr_.begin + flat_tid(); t_ < qr_.end; t_ += kBlock) {     \
       const int row = t_ / cpr_, q = t_ - row * cpr_;                       \
       const int k = row / (NY), j = row - k * (NY);                         \
       const int i0 = q << 2;                                                \
@@ -416,29 +415,34 @@ bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, cons
   return mg_quad_level(L0) && g.nx == L0.nx && aligned16(p) && aligned16(b);
 }
 
+int mg_quad_outer_blocks(const DevGrid& g) { return quad_blocks(g.nx / 4, g.ny * g.nz); }
+
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
                                 float* b0, float* x0, double tau, const PcgControl* ctl, DD* pdd,
                                 double* pmx, hipStream_t s) {
-  k_mg_outer_residual_q<<<cells_grid(g), block_shape(), 0, s>>>(g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
+  launch(k_mg_outer_residual_q, dim3(mg_quad_outer_blocks(g)), block_shape(), 0, s, g, L0, p, b, b0, x0, tau,
+         ctl, pdd, pmx);
 }
 
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
                         double tau, const PcgControl* ctl, hipStream_t s) {
-  const dim3 grid(grid_blocks(L.ny * L.nz));
+  const dim3 grid(quad_blocks(L.nx / 4, L.ny * L.nz));
   if (p)
-    k_mg_smooth_q<true><<<grid, block_shape(), 0, s>>>(L, xin, b, xout, p, tau, ctl);
+    launch(k_mg_smooth_q<true>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
   else
-    k_mg_smooth_q<false><<<grid, block_shape(), 0, s>>>(L, xin, b, xout, p, tau, ctl);
+    launch(k_mg_smooth_q<false>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
 }
 
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
                           float* xc, double tau, const PcgControl* ctl, hipStream_t s) {
-  k_mg_restrict_q<<<dim3(grid_blocks(Cl.ny * Cl.nz)), block_shape(), 0, s>>>(F, Cl, x, b, bc, xc, tau, ctl);
+  launch(k_mg_restrict_q, dim3(quad_blocks(Cl.nx / 2, Cl.ny * Cl.nz)), block_shape(), 0, s, F, Cl, x, b, bc, xc,
+         tau, ctl);
 }
 
 void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,
                          float* xout, const PcgControl* ctl, hipStream_t s) {
-  k_mg_prolong_q<<<dim3(grid_blocks(F.ny * F.nz)), block_shape(), 0, s>>>(F, Cl, xc, xin, xout, ctl);
+  launch(k_mg_prolong_q, dim3(quad_blocks(F.nx / 4, F.ny * F.nz)), block_shape(), 0, s, F, Cl, xc, xin, xout,
+         ctl);
 }
 
 }  // namespace rh

@@ -419,7 +419,7 @@ void pcg_prologue(const DevGrid& g, const PcgArrays& a, PcgControl* ctl, DD* pdd
   k_rhs_sums<<<grid, block_shape(), 0, s>>>(g, a.b, pdd, pmx);
   launch_finish<3, 1>(pdd, pmx, grid.x, FinRhsSums{ctl}, s);
   k_fill_zero<double><<<fill_blocks, block_shape(), 0, s>>>(a.p64, n);
-  k_true_residual<T><<<grid, block_shape(), 0, s>>>(g, a.p64, a.b, (T*)a.q, ctl, pdd, pmx, 0);
+  launch(k_true_residual<T>, grid, block_shape(), 0, s, g, a.p64, a.b, (T*)a.q, ctl, pdd, pmx, 0);
   launch_finish<1, 2>(pdd, pmx, grid.x, FinTrue{ctl, 0}, s);
 }
 
@@ -437,61 +437,61 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
   if (it > 0) {
     {
       prof::Scope ps(prof::kApplyDir, s, it);
-      k_apply_dir<T><<<grid, block, 0, s>>>(g, d, q, ctl, pdd, pmx);
+      launch(k_apply_dir<T>, grid, block, 0, s, g, d, q, ctl, pdd, pmx);
     }
     launch_finish<1, 1>(pdd, pmx, nb, FinApplyDir{ctl, it}, s);
     {
       prof::Scope ps(prof::kUpdate, s, it);
       if (sgs)
-        k_update<T, true><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx);
+        launch(k_update<T, true>, grid, block, 0, s, g, p, r, d, q, z, ctl, pdd, pmx);
       else
-        k_update<T, false><<<grid, block, 0, s>>>(g, p, r, d, q, z, ctl, pdd, pmx);
+        launch(k_update<T, false>, grid, block, 0, s, g, p, r, d, q, z, ctl, pdd, pmx);
     }
     launch_finish<3, 1>(pdd, pmx, nb, FinUpdate{ctl, it, sgs ? 0 : 1}, s);
     if (sgs) {
       {
         prof::Scope ps(prof::kSgsBlack, s, it);
-        k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 0);
+        launch(k_sgs_black<T>, grid, block, 0, s, g, r, z, ctl, it, 0);
       }
       {
         prof::Scope ps(prof::kSgsRed, s, it);
-        k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 0);
+        launch(k_sgs_red<T>, grid, block, 0, s, g, r, z, ctl, pdd, pmx, it, 0);
       }
       launch_finish<1, 1>(pdd, pmx, nb, FinSgsRed{ctl, it, 0}, s);
     }
     {
       prof::Scope ps(prof::kTrueResidual, s, it);
-      k_true_residual<T><<<grid, block, 0, s>>>(g, p, a.b, q, ctl, pdd, pmx, it);
+      launch(k_true_residual<T>, grid, block, 0, s, g, p, a.b, q, ctl, pdd, pmx, it);
     }
     launch_finish<1, 2>(pdd, pmx, nb, FinTrue{ctl, it}, s);
   }
   if (sgs) {
     {
       prof::Scope ps(prof::kReplace, s, it);
-      k_replace<T, true><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
+      launch(k_replace<T, true>, grid, block, 0, s, g, r, q, z, ctl, pdd, pmx, it);
     }
     {
       prof::Scope ps(prof::kSgsBlack, s, it);
-      k_sgs_black<T><<<grid, block, 0, s>>>(g, r, z, ctl, it, 1);
+      launch(k_sgs_black<T>, grid, block, 0, s, g, r, z, ctl, it, 1);
     }
     {
       prof::Scope ps(prof::kSgsRed, s, it);
-      k_sgs_red<T><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, it, 1);
+      launch(k_sgs_red<T>, grid, block, 0, s, g, r, z, ctl, pdd, pmx, it, 1);
     }
     launch_finish<1, 1>(pdd, pmx, nb, FinSgsRed{ctl, it, 1}, s);
     {
       prof::Scope ps(prof::kDirection, s, it);
-      k_direction<T, true><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+      launch(k_direction<T, true>, grid, block, 0, s, g, r, z, d, ctl, it);
     }
   } else {
     {
       prof::Scope ps(prof::kReplace, s, it);
-      k_replace<T, false><<<grid, block, 0, s>>>(g, r, q, z, ctl, pdd, pmx, it);
+      launch(k_replace<T, false>, grid, block, 0, s, g, r, q, z, ctl, pdd, pmx, it);
     }
     launch_finish<1, 1>(pdd, pmx, nb, FinReplace{ctl, it}, s);
     {
       prof::Scope ps(prof::kDirection, s, it);
-      k_direction<T, false><<<grid, block, 0, s>>>(g, r, z, d, ctl, it);
+      launch(k_direction<T, false>, grid, block, 0, s, g, r, z, d, ctl, it);
     }
   }
 }

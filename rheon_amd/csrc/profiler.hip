@@ -16,6 +16,7 @@ struct Pair {
   hipEvent_t a = nullptr, b = nullptr;
   int slot = -1;
   int tag = -1;
+  bool taken = false;  // attached to a dispatch
 };
 
 struct State {
@@ -23,7 +24,6 @@ struct State {
   bool on = false;
   int focus = -1;            // slot timed alone, or -1 = every slot
   std::vector<Pair> pool;    // reusable event pairs
-  std::vector<int> open;     // pool index of the open pair per slot
   size_t used = 0;
   int64_t launches[kNumSlots] = {};
   double seconds[kNumSlots] = {};
@@ -33,6 +33,8 @@ State& st() {
   static State s;
   return s;
 }
+
+thread_local int armed_pair = -1;  // pool index armed on this thread
 
 const char* kNames[kNumSlots] = {
     "pcg_apply_dir", "pcg_update",      "pcg_true_residual", "pcg_replace",   "pcg_direction",
@@ -51,10 +53,10 @@ bool enabled_for(int slot) {
 
 const char* name(int slot) { return slot >= 0 && slot < kNumSlots ? kNames[slot] : "?"; }
 
-void start(int slot, hipStream_t s, int tag) {
+void arm(int slot, int tag) {
   State& S = st();
   std::lock_guard<std::mutex> lk(S.mu);
-  if (S.open.empty()) S.open.assign(kNumSlots, -1);
+  armed_pair = -1;
   if (S.used == S.pool.size()) {
     Pair p;
     if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) return;
@@ -63,17 +65,23 @@ void start(int slot, hipStream_t s, int tag) {
   Pair& p = S.pool[S.used];
   p.slot = slot;
   p.tag = tag;
-  (void)hipEventRecord(p.a, s);
-  S.open[slot] = (int)S.used;
+  p.taken = false;
+  armed_pair = (int)S.used;
   ++S.used;
 }
 
-void stop(int slot, hipStream_t s) {
+void disarm() { armed_pair = -1; }
+
+bool take(hipEvent_t* a, hipEvent_t* b) {
+  if (armed_pair < 0) return false;
   State& S = st();
   std::lock_guard<std::mutex> lk(S.mu);
-  if (S.open.empty() || S.open[slot] < 0) return;
-  (void)hipEventRecord(S.pool[S.open[slot]].b, s);
-  S.open[slot] = -1;
+  Pair& p = S.pool[armed_pair];
+  armed_pair = -1;
+  p.taken = true;
+  *a = p.a;
+  *b = p.b;
+  return true;
 }
 
 void settle(int keep_max_tag) {
@@ -81,7 +89,7 @@ void settle(int keep_max_tag) {
   std::lock_guard<std::mutex> lk(S.mu);
   for (size_t i = 0; i < S.used; ++i) {
     Pair& p = S.pool[i];
-    if (p.tag >= 0 && p.tag > keep_max_tag) continue;
+    if (!p.taken || (p.tag >= 0 && p.tag > keep_max_tag)) continue;
     if (hipEventSynchronize(p.b) != hipSuccess) continue;
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) continue;

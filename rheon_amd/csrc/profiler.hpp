@@ -1,11 +1,14 @@
-// Live per-kernel timing with HIP events recorded on the launching stream.
-// Disabled by default (one branch per launch).  Every event pair costs the
-// stream a few microseconds, so a timed run can focus on one slot: only
-// that slot's launches are bracketed.  Entries carry the solver
+// Live per-kernel timing with HIP events.  Disabled by default (one branch
+// per launch).  A Scope arms an event pair that the next rh::launch on the
+// same thread attaches to the kernel's own dispatch (hipExtLaunchKernelGGL:
+// the events take that dispatch's start and end timestamps, so no marker
+// packets sit between kernels and the figure is the kernel's duration).  A
+// timed run can still focus on one slot.  Entries carry the solver
 // iteration they belong to, so launches enqueued past convergence (which
 // return at once) can be discarded before they are counted.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 namespace rh {
 namespace prof {
@@ -37,25 +40,38 @@ enum Slot {
 bool enabled();
 // Timing on for this slot (profiling on and no focus, or focus == slot).
 bool enabled_for(int slot);
-// Record a start event for `slot` on stream s; `tag` = solver iteration or -1.
-void start(int slot, hipStream_t s, int tag);
-void stop(int slot, hipStream_t s);
+// Arm an event pair for `slot` (`tag` = solver iteration or -1) that the
+// next launch on this thread takes; disarm drops it if no launch took it.
+void arm(int slot, int tag);
+void disarm();
+// The armed pair, if any, marked as taken by a launch.
+bool take(hipEvent_t* a, hipEvent_t* b);
 // Resolve recorded pairs (synchronises on their events): pairs with
 // tag <= keep_max_tag (or tag < 0) are accumulated, others dropped.
 void settle(int keep_max_tag);
 const char* name(int slot);
 
 struct Scope {
-  int slot;
-  hipStream_t s;
   bool on;
-  Scope(int sl, hipStream_t st, int tag) : slot(sl), s(st), on(enabled_for(sl)) {
-    if (on) start(sl, st, tag);
+  Scope(int sl, hipStream_t, int tag) : on(enabled_for(sl)) {
+    if (on) arm(sl, tag);
   }
   ~Scope() {
-    if (on) stop(slot, s);
+    if (on) disarm();
   }
 };
 
 }  // namespace prof
+
+// Kernel launch that carries the armed event pair of the enclosing Scope.
+template <typename... KArgs, typename... Args>
+inline void launch(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem, hipStream_t s,
+                   Args... args) {
+  hipEvent_t a, b;
+  if (prof::enabled() && prof::take(&a, &b))
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, static_cast<KArgs>(args)...);
+  else
+    kernel<<<grid, block, shmem, s>>>(static_cast<KArgs>(args)...);
+}
+
 }  // namespace rh

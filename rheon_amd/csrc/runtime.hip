@@ -26,12 +26,15 @@ void set_last_error(const char* what, const char* where) {
 const char* last_error() { return g_last_error.c_str(); }
 
 int grid_blocks(int rows) {
-  int groups = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
-  if (groups < 1) groups = 1;
-  if (groups > kMaxBlocks) return kMaxBlocks;
-  // Multiples of 8 let block_rows() give each XCD a contiguous slab.
-  if (groups >= 64) groups &= ~7;
-  return groups;
+  const long long groups = (rows + kRowsPerBlock - 1) / kRowsPerBlock;
+  const long long blocks = (groups + kGroupsPerBlock - 1) / kGroupsPerBlock;
+  return blocks < 1 ? 1 : (blocks > kMaxBlocks ? kMaxBlocks : (int)blocks);
+}
+
+int quad_blocks(int quads_per_row, int rows) {
+  const long long quads = (long long)quads_per_row * rows;
+  const long long blocks = (quads + kGroupsPerBlock * kBlock - 1) / (kGroupsPerBlock * kBlock);
+  return blocks < 1 ? 1 : (blocks > kMaxBlocks ? kMaxBlocks : (int)blocks);
 }
 
 // GridGeometry::new (src/geometry.rs:114-191), plus the 32-bit indexing
@@ -191,8 +194,7 @@ rh_status build_obstacle_mask(const rh_grid& g, const uint8_t* solid, std::vecto
 rh_status Scratch::init() {
   if (stream) return RH_OK;
   RH_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  const size_t bytes = sizeof(DD) * 4 * kMaxBlocks + sizeof(double) * 4 * kMaxBlocks +
-                       sizeof(unsigned) * 64 + sizeof(double) * 64 + 2 * sizeof(PcgControl) + 1024;
+  const size_t bytes = kDeviceBytes;
   RH_HIP(hipMalloc(&base, bytes));
   RH_HIP(hipMemsetAsync(base, 0, bytes, stream));
   char* p = (char*)base;

@@ -35,6 +35,10 @@ struct Scratch {
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
   size_t bytes_ = 0;
 
+  // device bytes of one Scratch (reduction partials for kMaxBlocks blocks)
+  static constexpr size_t kDeviceBytes = sizeof(DD) * 4 * kMaxBlocks + sizeof(double) * 4 * kMaxBlocks +
+                                         sizeof(unsigned) * 64 + sizeof(double) * 64 +
+                                         2 * sizeof(PcgControl) + 1024;
   rh_status init();
   void release();
   Reduce reduce(int out_slot);

@@ -307,7 +307,7 @@ uint64_t MgSolver::plan_bytes(const rh_grid& grid, const rh_solver_config& confi
     if (l == 0) total += cells;                                                // obstacle mask
   }
   const uint64_t coarse = (uint64_t)dims[levels - 1][0] * dims[levels - 1][1] * dims[levels - 1][2];
-  total += coarse * coarse * 8 + sizeof(MgHierarchy) + (256u << 10);  // dense inverse, tables, scratch
+  total += coarse * coarse * 8 + sizeof(MgHierarchy) + Scratch::kDeviceBytes;  // dense inverse, tables, scratch
   return total;
 }
 
