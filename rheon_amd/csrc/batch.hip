@@ -519,6 +519,9 @@ rh_status Batch::build() {
   const bool quad_ok = (variant_ & RH_VARIANT_SCALAR_MG) == 0;
   auto nblk = [&](int rows) { return std::max(1, grid_blocks(rows) / block_div_); };
   auto qblk = [&](int blocks) { return std::max(1, blocks / block_div_); };  // quad launches
+  auto lblk = [&](const MgLevel& L) {  // flat level launches
+    return std::max(1, level_blocks((long long)L.nx * L.ny * L.nz) / block_div_);
+  };
   auto face_blocks = [&](size_t i) { return nblk((int)((grids[i].ny + 1) * (grids[i].nz + 1))); };
   auto cell_blocks = [&](size_t i) { return nblk(grids[i].ny * grids[i].nz); };
   // a stage: records for all scenes + blocks per scene (0 = not taking part)
@@ -691,7 +694,7 @@ rh_status Batch::build() {
         if (q)
           bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
         else
-          bs[i] = nblk(h.lv[l].ny * h.lv[l].nz);
+          bs[i] = lblk(h.lv[l]);
         if (!to_p) in_x[i][l] = !cx;
       }
       stage(to_p ? Op::kSmoothQUpdate : Op::kSmoothQ, r.data(), sizeof(BSmoothArgs), bq);
@@ -712,7 +715,7 @@ rh_status Batch::build() {
         if (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]))
           bq[i] = qblk(mg_quad_restrict_blocks(h.lv[l + 1]));
         else
-          bs[i] = nblk(h.lv[l + 1].ny * h.lv[l + 1].nz);
+          bs[i] = lblk(h.lv[l + 1]);
         in_x[i][l + 1] = true;
       }
       stage(Op::kRestrictQ, r.data(), sizeof(BRestrictArgs), bq);
@@ -756,7 +759,7 @@ rh_status Batch::build() {
         if (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]))
           bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
         else
-          bs[i] = nblk(h.lv[l].ny * h.lv[l].nz);
+          bs[i] = lblk(h.lv[l]);
         in_x[i][l] = !cx;
       }
       stage(Op::kProlongQ, r.data(), sizeof(BProlongArgs), bq);

@@ -194,11 +194,28 @@ __device__ __forceinline__ void finish_body(const DD* __restrict__ pdd, const do
   for (int q = 0; q < ND; ++q) a[q] = DD{0.0, 0.0};
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = 0.0;
-  for (int b = t; b < nb; b += kBlock) {
+  // kUnroll partials per thread are loaded before any is added, so their
+  // loads are in flight together (the partials come from other XCDs' L2s)
+  constexpr int kUnroll = 8;
+  for (int b0 = t; b0 < nb; b0 += kUnroll * kBlock) {
+    DD v[kUnroll][ND];
+    double w[kUnroll][NM];
 #pragma unroll
-    for (int q = 0; q < ND; ++q) a[q] = dd_add(a[q], pdd[q * kMaxBlocks + b]);
+    for (int u = 0; u < kUnroll; ++u) {
+      const int b = b0 + u * kBlock;
+      const bool in = b < nb;
 #pragma unroll
-    for (int q = 0; q < NM; ++q) m[q] = fmax(m[q], pmx[q * kMaxBlocks + b]);
+      for (int q = 0; q < ND; ++q) v[u][q] = in ? pdd[q * kMaxBlocks + b] : DD{0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < NM; ++q) w[u][q] = in ? pmx[q * kMaxBlocks + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+#pragma unroll
+      for (int q = 0; q < ND; ++q) a[q] = dd_add(a[q], v[u][q]);
+#pragma unroll
+      for (int q = 0; q < NM; ++q) m[q] = fmax(m[q], w[u][q]);
+    }
   }
   block_reduce_dd<ND>(a);
   block_reduce_max<NM>(m);
@@ -294,6 +311,7 @@ __device__ __forceinline__ VBlock batch_block(const int* __restrict__ prefix, in
 // Host helpers ---------------------------------------------------------------
 int grid_blocks(int rows);               // blocks of a row launch (kRowsPerBlock rows each)
 int quad_blocks(int quads_per_row, int rows);  // blocks of a four-cell-per-lane launch
+int level_blocks(long long cells);             // blocks of a flat one-cell-per-lane launch
 DevGrid make_dev_grid(const rh_grid& g, double density);
 rh_status validate_grid(const rh_grid* g, size_t* cells_out);
 

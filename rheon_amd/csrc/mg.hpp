@@ -38,7 +38,10 @@ constexpr unsigned kFracFluid = 1u << 30;
 constexpr int kMgMaxSweeps = 8;   // smoothing sweeps per side of the V-cycle
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
-constexpr int kMgTailLds = 56 * 1024;  // LDS bytes for the tail levels' x, y, b arrays
+// Dynamic LDS bytes for the tail levels' arrays (x, y, b, face fractions,
+// octants); the tail kernel's static LDS (8 KB) comes on top, within the
+// 160 KB a gfx950 workgroup may hold.
+constexpr int kMgTailLds = 120 * 1024;
 
 // Smoothing sweep k of a side is x <- x + tau[k] (b - K x) / D.  Jacobi:
 // every tau = omega.  Chebyshev (default): the taus are the reciprocals of
@@ -62,7 +65,8 @@ struct MgHierarchy {
   bool quad = true;  // use the four-cells-per-lane kernels where a level allows
   int tail_level = 0;
   int tail_lds = 0;                  // dynamic LDS bytes of the tail launch
-  int tail_off[kMgMaxLevels][3] = {};  // LDS byte offsets of x, y, b per tail level
+  // LDS byte offsets per tail level of x, y, b, fractions, octants (-1 = none)
+  int tail_off[kMgMaxLevels][5] = {};
   const MgHierarchy* d_self = nullptr;
   MgLevel lv[kMgMaxLevels];
   void* x[kMgMaxLevels] = {};  // V-cycle iterate (S)
@@ -124,5 +128,6 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
 template <typename S>
 void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s);
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s);
+int mg_tail_lds_limit();  // dynamic LDS bytes the tail kernels may use
 
 }  // namespace rh

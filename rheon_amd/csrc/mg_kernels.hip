@@ -221,17 +221,40 @@ k_mg_coarsest(MgLevel L, const S* __restrict__ b, S* __restrict__ xout, const Pc
 }
 
 // Coarsest level with the dense projected inverse: x = G b, one workgroup.
+// Row c of G b is split over P = min(kThreads, kDenseRed) / n threads,
+// each summing a contiguous run of columns (loads issued together,
+// coalesced over c); the P partial sums of a row are added in part order
+// through LDS (`red`, kDenseRed doubles), so the result is deterministic.
 constexpr int kDenseThreads = 512;
+constexpr int kDenseRed = 512;
 
 template <int kThreads, typename S>
 __device__ __forceinline__ void dense_solve(int n, const double* __restrict__ G, const S* __restrict__ b,
-                                            S* __restrict__ x, double* bs) {
+                                            S* __restrict__ x, double* bs, double* red) {
   for (int c = threadIdx.x; c < n; c += kThreads) bs[c] = (double)b[c];
   __syncthreads();
-  for (int c = threadIdx.x; c < n; c += kThreads) {
+  constexpr int kSpan = kThreads < kDenseRed ? kThreads : kDenseRed;
+  const int parts = n > 0 && n <= kSpan ? kSpan / n : 1;
+  const int t = threadIdx.x;
+  if (parts > 1) {
+    const int c = t % n, part = t / n;
+    const int j0 = (int)((long long)part * n / parts), j1 = (int)((long long)(part + 1) * n / parts);
     double acc = 0.0;
-    for (int j2 = 0; j2 < n; ++j2) acc += G[(size_t)j2 * n + c] * bs[j2];
-    x[c] = (S)acc;
+    if (part < parts)
+      for (int j2 = j0; j2 < j1; ++j2) acc += G[(size_t)j2 * n + c] * bs[j2];
+    if (part < parts) red[part * n + c] = acc;
+    __syncthreads();
+    if (t < n) {
+      double sum = 0.0;
+      for (int q = 0; q < parts; ++q) sum += red[q * n + t];
+      x[t] = (S)sum;
+    }
+  } else {
+    for (int c = t; c < n; c += kThreads) {
+      double acc = 0.0;
+      for (int j2 = 0; j2 < n; ++j2) acc += G[(size_t)j2 * n + c] * bs[j2];
+      x[c] = (S)acc;
+    }
   }
   __syncthreads();
 }
@@ -242,7 +265,8 @@ k_mg_coarsest_dense(MgLevel L, const double* __restrict__ minv, const S* __restr
                     S* __restrict__ xout, const PcgControl* ctl) {
   if (ctl->done) return;
   __shared__ double bs[kMgDenseMax];
-  dense_solve<kDenseThreads, S>(L.nx * L.ny * L.nz, minv, b, xout, bs);
+  __shared__ double red[kDenseRed];
+  dense_solve<kDenseThreads, S>(L.nx * L.ny * L.nz, minv, b, xout, bs, red);
 }
 template <typename S>
 __global__ void __launch_bounds__(kDenseThreads)
@@ -250,34 +274,43 @@ kb_mg_coarsest_dense(const BCoarseArgs* __restrict__ A, const int* __restrict__ 
   const BCoarseArgs& a = A[batch_scene(prefix, n, blockIdx.x)];
   if (a.ctl->done) return;
   __shared__ double bs[kMgDenseMax];
-  dense_solve<kDenseThreads, S>(a.L->nx * a.L->ny * a.L->nz, a.minv, (const S*)a.b, (S*)a.x, bs);
+  __shared__ double red[kDenseRed];
+  dense_solve<kDenseThreads, S>(a.L->nx * a.L->ny * a.L->nz, a.minv, (const S*)a.b, (S*)a.x, bs, red);
 }
 
 // ------------------------------------------------------------- V-cycle tail
-// Levels l0..L-1 (each <= kMgTailCells cells) run inside one workgroup of
-// kTailThreads threads: the level arrays stay in L2, and each phase is
-// separated by a workgroup barrier instead of a kernel launch.
+// Levels l0..L-1 (together within kMgTailLds bytes) run inside one
+// workgroup of kTailThreads threads with every level array in LDS: the
+// iterate x, its ping-pong partner y and the right-hand side b (V-cycle
+// precision), and for obstacle levels the face-fraction words and the
+// octant bytes; phases are separated by workgroup barriers instead of
+// kernel launches.  The V-cycle runs as one loop over its phases with each
+// phase kind written once, so every level and sweep executes the same code
+// (separate inlined copies per call site each started with a cold
+// instruction cache, several microseconds per phase).
 constexpr int kTailThreads = 1024;
 
 template <typename S>
-__device__ __forceinline__ void tail_smooth(const MgLevel& L, const S* __restrict__ xin,
-                                            const S* __restrict__ b, S* __restrict__ xout, double tau) {
+__device__ __forceinline__ void tail_smooth(const MgLevel& L, const unsigned* fr, const S* xin, const S* b,
+                                         S* xout, double tau) {
   const int n = L.nx * L.ny * L.nz;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
-    const int i = c % L.nx, j = (c / L.nx) % L.ny, k = c / (L.nx * L.ny);
-    const S ax = s_apply<S>(L, xin, c, i, j, k);
-    xout[c] = xin[c] + lv_scale<S>(L, tau, c, i, j, k) * (b[c] - ax);
+    const int row = c / L.nx, i = c - row * L.nx;
+    const int k = row / L.ny, j = row - k * L.ny;
+    const S ax = s_apply_fr<S>(L, fr, xin, c, i, j, k);
+    xout[c] = xin[c] + (S)(tau * lv_odiag_fr(L, fr, c, i, j, k)) * (b[c] - ax);
   }
   __syncthreads();
 }
 
 template <typename S>
-__device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& Cl, const S* __restrict__ x,
-                                              const S* __restrict__ b, S* __restrict__ bc,
-                                              S* __restrict__ xc, double tau) {
+__device__ __forceinline__ void tail_restrict(const MgLevel& F, const unsigned* ffr, const MgLevel& Cl,
+                                           const unsigned* cfr, const S* x, const S* b, S* bc, S* xc,
+                                           double tau) {
   const int n = Cl.nx * Cl.ny * Cl.nz;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
-    const int i = c % Cl.nx, j = (c / Cl.nx) % Cl.ny, k = c / (Cl.nx * Cl.ny);
+    const int row = c / Cl.nx, i = c - row * Cl.nx;
+    const int k = row / Cl.ny, j = row - k * Cl.ny;
     S sum = (S)0;
     for (int dz = 0; dz < 2; ++dz)
       for (int dy = 0; dy < 2; ++dy)
@@ -285,27 +318,32 @@ __device__ __forceinline__ void tail_restrict(const MgLevel& F, const MgLevel& C
           const int fi = 2 * i + dx, fj = 2 * j + dy, fk = 2 * k + dz;
           if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;  // padded coarse cell
           const int fc = fi + F.nx * (fj + F.ny * fk);
-          sum += b[fc] - s_apply<S>(F, x, fc, fi, fj, fk);
+          sum += b[fc] - s_apply_fr<S>(F, ffr, x, fc, fi, fj, fk);
         }
     bc[c] = sum;
-    xc[c] = lv_scale<S>(Cl, tau, c, i, j, k) * sum;
+    xc[c] = (S)(tau * lv_odiag_fr(Cl, cfr, c, i, j, k)) * sum;
   }
   __syncthreads();
 }
 
+// xout = xin + trilinear prolongation of xc (prolong_value with the coarse
+// level's fractions and octants read from LDS)
 template <typename S>
-__device__ __forceinline__ void tail_prolong(const MgLevel& F, const MgLevel& Cl, const S* __restrict__ xc,
-                                             const S* __restrict__ xin, S* __restrict__ xout) {
+__device__ __forceinline__ void tail_prolong(const MgLevel& F, const unsigned* ffr, const MgLevel& Cl,
+                                          const unsigned* cfr, const unsigned char* coct, const S* xc,
+                                          const S* xin, S* xout) {
   const int n = F.nx * F.ny * F.nz;
   for (int c = threadIdx.x; c < n; c += kTailThreads) {
-    const int i = c % F.nx, j = (c / F.nx) % F.ny, k = c / (F.nx * F.ny);
-    xout[c] = xin[c] + prolong_value<S>(Cl, xc, i, j, k, lv_fine_fluid(F, c));
+    const int row = c / F.nx, i = c - row * F.nx;
+    const int k = row / F.ny, j = row - k * F.ny;
+    const bool fine_fluid = ffr ? (ffr[c] & kFracFluid) != 0 : lv_fine_fluid(F, c);
+    xout[c] = xin[c] + prolong_value_fr<S>(Cl, cfr, coct, xc, i, j, k, fine_fluid);
   }
   __syncthreads();
 }
 
 template <typename S>
-__device__ __forceinline__ void tail_copy(int n, const S* __restrict__ src, S* __restrict__ dst) {
+__device__ __forceinline__ void tail_copy(int n, const S* src, S* dst) {
   for (int c = threadIdx.x; c < n; c += kTailThreads) dst[c] = src[c];
   __syncthreads();
 }
@@ -314,69 +352,116 @@ template <typename S>
 __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int l0,
                                           const PcgControl* ctl) {
   if (ctl->done) return;
-  // Level arrays of the tail live in LDS (offsets in H->tail_off); level l0
-  // is loaded from and written back to global memory.
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double bs[kMgDenseMax];
+  __shared__ double red[kDenseRed];
   const int levels = H->levels;
-  auto X = [&](int l) { return (S*)(smem + H->tail_off[l][0]); };
-  auto Y = [&](int l) { return (S*)(smem + H->tail_off[l][1]); };
-  auto B = [&](int l) { return (S*)(smem + H->tail_off[l][2]); };
+  auto at = [&](int l, int q) -> void* { return H->tail_off[l][q] >= 0 ? smem + H->tail_off[l][q] : nullptr; };
+  auto X = [&](int l) { return (S*)at(l, 0); };
+  auto Y = [&](int l) { return (S*)at(l, 1); };
+  auto B = [&](int l) { return (S*)at(l, 2); };
+  auto FR = [&](int l) { return (const unsigned*)at(l, 3); };
+  auto OC = [&](int l) { return (const unsigned char*)at(l, 4); };
+  auto cells = [&](int l) { return H->lv[l].nx * H->lv[l].ny * H->lv[l].nz; };
+  // stage level l0's x and b, and the fractions and octants of every tail
+  // level, from global memory
   {
-    const MgLevel& L = H->lv[l0];
-    const int n = L.nx * L.ny * L.nz;
     const S* gb = (const S*)H->b[l0];
     const S* gx = (const S*)H->x[l0];
-    for (int c = threadIdx.x; c < n; c += kTailThreads) {
+    const int n0 = cells(l0);
+    for (int c = threadIdx.x; c < n0; c += kTailThreads) {
       B(l0)[c] = gb[c];
       X(l0)[c] = gx[c];
+    }
+    for (int l = l0; l < levels; ++l) {
+      const int n = cells(l);
+      if (H->tail_off[l][3] >= 0)
+        for (int c = threadIdx.x; c < n; c += kTailThreads) ((unsigned*)at(l, 3))[c] = H->lv[l].frac[c];
+      if (H->tail_off[l][4] >= 0)
+        for (int c = threadIdx.x; c < n; c += kTailThreads) ((unsigned char*)at(l, 4))[c] = H->lv[l].octant[c];
     }
     __syncthreads();
   }
   // Every tail level's iterate starts in X(l); when the tail is only the
-  // coarsest level (l0 == levels - 1) no loop below touches cur[l0].
+  // coarsest level (l0 == levels - 1) no phase below touches cur[l0].
   S* cur[kMgMaxLevels];
   S* nxt[kMgMaxLevels];
   for (int l = l0; l < levels; ++l) {
     cur[l] = X(l);
     nxt[l] = Y(l);
   }
-  for (int l = l0; l < levels - 1; ++l) {
+  // Phase loop.  Going down, level l takes pre - 1 sweeps then restricts
+  // to l + 1; the coarsest level is solved densely; going up, level l
+  // takes the prolongation from l + 1, post sweeps and (below l0) a copy
+  // of its result into X(l).
+  int l = l0, step = 0;
+  bool down = l0 < levels - 1;
+  for (;;) {
+    enum { kSmooth, kRestrict, kDense, kProlong, kCopy, kDone } op;
+    double tau = 0.0;
+    if (down) {
+      if (step < H->pre - 1) {
+        op = kSmooth;
+        tau = H->tau_pre[step + 1];
+      } else {
+        op = kRestrict;
+      }
+    } else if (l == levels - 1) {
+      op = kDense;
+    } else if (step == 0) {
+      op = kProlong;
+    } else if (step <= H->post) {
+      op = kSmooth;
+      tau = H->tau_post[step - 1];
+    } else {
+      op = (l > l0 && cur[l] != X(l)) ? kCopy : kDone;
+    }
+    if (!down && l == levels - 1) op = kDense;
     const MgLevel& L = H->lv[l];
-    for (int sweep = 1; sweep < H->pre; ++sweep) {
-      tail_smooth<S>(L, cur[l], B(l), nxt[l], H->tau_pre[sweep]);
+    switch (op) {
+      case kSmooth:
+        tail_smooth<S>(L, FR(l), cur[l], B(l), nxt[l], tau);
+        break;
+      case kRestrict:
+        tail_restrict<S>(L, FR(l), H->lv[l + 1], FR(l + 1), cur[l], B(l), B(l + 1), X(l + 1), H->tau_pre[0]);
+        break;
+      case kDense:
+        dense_solve<kTailThreads, S>(cells(l), H->coarse_inv, B(l), X(l), bs, red);
+        break;
+      case kProlong:
+        tail_prolong<S>(L, FR(l), H->lv[l + 1], FR(l + 1), OC(l + 1), X(l + 1), cur[l], nxt[l]);
+        break;
+      case kCopy:
+        tail_copy<S>(cells(l), cur[l], X(l));
+        break;
+      default:
+        break;
+    }
+    // advance
+    if (op == kSmooth || op == kProlong) {
       S* t = cur[l];
       cur[l] = nxt[l];
       nxt[l] = t;
+      ++step;
+    } else if (op == kRestrict) {
+      ++l;
+      step = 0;
+      if (l == levels - 1) down = false;
+    } else if (op == kDense) {
+      if (l == l0) break;  // the tail is only the coarsest level
+      --l;
+      step = 0;
+      down = false;
+    } else {  // kCopy or kDone: level l finished
+      if (l == l0) break;
+      --l;
+      step = 0;
     }
-    tail_restrict<S>(L, H->lv[l + 1], cur[l], B(l), B(l + 1), X(l + 1), H->tau_pre[0]);
-  }
-  // coarsest: dense projected inverse (callers guarantee n <= kMgDenseMax)
-  {
-    const MgLevel& L = H->lv[levels - 1];
-    dense_solve<kTailThreads, S>(L.nx * L.ny * L.nz, H->coarse_inv, B(levels - 1), X(levels - 1), bs);
-  }
-  for (int l = levels - 2; l >= l0; --l) {
-    const MgLevel& L = H->lv[l];
-    tail_prolong<S>(L, H->lv[l + 1], X(l + 1), cur[l], nxt[l]);
-    {
-      S* t = cur[l];
-      cur[l] = nxt[l];
-      nxt[l] = t;
-    }
-    for (int sweep = 0; sweep < H->post; ++sweep) {
-      tail_smooth<S>(L, cur[l], B(l), nxt[l], H->tau_post[sweep]);
-      S* t = cur[l];
-      cur[l] = nxt[l];
-      nxt[l] = t;
-    }
-    if (l > l0 && cur[l] != X(l)) tail_copy<S>(L.nx * L.ny * L.nz, cur[l], X(l));
   }
   // level l0's result back to global x[l0] for the parent's prolongation
   {
-    const MgLevel& L = H->lv[l0];
-    const int n = L.nx * L.ny * L.nz;
     S* gx = (S*)H->x[l0];
+    const int n = cells(l0);
     for (int c = threadIdx.x; c < n; c += kTailThreads) gx[c] = cur[l0][c];
   }
 }
@@ -461,6 +546,24 @@ void launch_b_mg_prolong(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks)
     kb_mg_prolong<<<bl.blocks, block_shape(), 0, s>>>((const BProlongArgs*)bl.args, bl.prefix, bl.n);
 }
+// Dynamic LDS the tail kernels may use: kMgTailLds once the runtime has
+// accepted that size for every tail kernel, else the 64 KB-workgroup share.
+int mg_tail_lds_limit() {
+  static const int limit = [] {
+    const int want = kMgTailLds;
+    const bool ok =
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail<float>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail<double>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&kb_mg_tail<float>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, want) == hipSuccess;
+    (void)hipGetLastError();
+    return ok ? want : 56 * 1024;
+  }();
+  return limit;
+}
+
 void launch_b_mg_tail(const BatchLaunch& bl, int lds_bytes, hipStream_t s) {
   if (bl.blocks)
     kb_mg_tail<float><<<bl.blocks, kTailThreads, lds_bytes, s>>>((const BTailArgs*)bl.args, bl.prefix, bl.n);
@@ -473,7 +576,7 @@ void launch_b_mg_coarsest_dense(const BatchLaunch& bl, hipStream_t s) {
 
 // ------------------------------------------------------------ launchers
 
-static inline dim3 level_grid(const MgLevel& L) { return dim3(grid_blocks(L.ny * L.nz)); }
+static inline dim3 level_grid(const MgLevel& L) { return dim3(level_blocks((long long)L.nx * L.ny * L.nz)); }
 
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,

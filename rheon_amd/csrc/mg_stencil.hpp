@@ -16,17 +16,19 @@ __device__ __forceinline__ bool lv_active(const MgLevel& L, int c) {
 }
 
 // Open fractions (in units of fscale) of the six faces of cell c on a level
-// with `frac`; upper faces are the neighbours' lower faces, walls are 0.
+// with face fractions `fr` (the level's `frac` array, or a copy of it);
+// upper faces are the neighbours' lower faces, walls are 0.
 struct Frac6 {
   int xm, xp, ym, yp, zm, zp;
 };
-__device__ __forceinline__ Frac6 lv_frac6(const MgLevel& L, int c, int i, int j, int k) {
+__device__ __forceinline__ Frac6 lv_frac6(const MgLevel& L, const unsigned* fr, int c, int i, int j,
+                                          int k) {
   const int sy = L.nx, sz = L.nx * L.ny;
   const bool hx = i + 1 < L.nx, hy = j + 1 < L.ny, hz = k + 1 < L.nz;
-  const unsigned q = L.frac[c];
-  const unsigned qx = L.frac[hx ? c + 1 : c];
-  const unsigned qy = L.frac[hy ? c + sy : c];
-  const unsigned qz = L.frac[hz ? c + sz : c];
+  const unsigned q = fr[c];
+  const unsigned qx = fr[hx ? c + 1 : c];
+  const unsigned qy = fr[hy ? c + sy : c];
+  const unsigned qz = fr[hz ? c + sz : c];
   Frac6 f;
   f.xm = (int)(q & 1023u);
   f.ym = (int)((q >> 10) & 1023u);
@@ -44,12 +46,13 @@ __device__ __forceinline__ int lv_mask(const MgLevel& L, int c, int i, int j, in
 // Level-local stencil in the V-cycle precision S (arithmetic in S).
 // Branch-free: all six neighbours are loaded at once from clamped
 // addresses (a wall or closed-face neighbour is the cell itself and
-// contributes 0).
+// contributes 0).  `fr` = the level's face fractions (null for box/mask
+// levels).
 template <typename S>
-__device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, int c, int i, int j,
-                                     int k) {
-  if (L.frac) {
-    const Frac6 f = lv_frac6(L, c, i, j, k);
+__device__ __forceinline__ S s_apply_fr(const MgLevel& L, const unsigned* fr, const S* __restrict__ x, int c,
+                                        int i, int j, int k) {
+  if (fr) {
+    const Frac6 f = lv_frac6(L, fr, c, i, j, k);
     const Nbr n = nbr_offsets(L.nx, L.ny, L.nz, i, j, k);
     const S xc = x[c];
     const S a = x[c + n.xm], b = x[c + n.xp], cy = x[c + n.ym], d = x[c + n.yp],
@@ -76,16 +79,25 @@ __device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, 
   const S w0 = (S)L.w[0], w1 = (S)L.w[1], w2 = (S)L.w[2];
   return w0 * ((xc - a) + (xc - b)) + w1 * ((xc - cy) + (xc - d)) + w2 * ((xc - e) + (xc - f));
 }
+template <typename S>
+__device__ __forceinline__ S s_apply(const MgLevel& L, const S* __restrict__ x, int c, int i, int j,
+                                     int k) {
+  return s_apply_fr<S>(L, L.frac, x, c, i, j, k);
+}
 
 // 1 / D (Jacobi scale) of the cell; 0 where D = 0 (solid).
-__device__ __forceinline__ double lv_odiag(const MgLevel& L, int c, int i, int j, int k) {
-  if (L.frac) {
-    const Frac6 f = lv_frac6(L, c, i, j, k);
+__device__ __forceinline__ double lv_odiag_fr(const MgLevel& L, const unsigned* fr, int c, int i, int j,
+                                              int k) {
+  if (fr) {
+    const Frac6 f = lv_frac6(L, fr, c, i, j, k);
     const double d = L.wf[0] * (double)(f.xm + f.xp) + L.wf[1] * (double)(f.ym + f.yp) +
                      L.wf[2] * (double)(f.zm + f.zp);
     return d > 0.0 ? 1.0 / d : 0.0;
   }
   return L.odiag[lv_mask(L, c, i, j, k)];
+}
+__device__ __forceinline__ double lv_odiag(const MgLevel& L, int c, int i, int j, int k) {
+  return lv_odiag_fr(L, L.frac, c, i, j, k);
 }
 
 // Sweep scale tau / D in the V-cycle precision.
@@ -105,8 +117,8 @@ __device__ __forceinline__ bool lv_fine_fluid(const MgLevel& F, int c) { return 
 // cell's value is never read by a fluid equation, so it keeps the plain
 // form).
 template <typename S>
-__device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restrict__ xc, int i, int j,
-                                           int k, bool fine_fluid) {
+__device__ __forceinline__ S prolong_value_fr(const MgLevel& Cl, const unsigned* cfr, const unsigned char* coct,
+                                              const S* __restrict__ xc, int i, int j, int k, bool fine_fluid) {
   const int I = i >> 1, J = j >> 1, K = k >> 1;
   int In = I + ((i & 1) ? 1 : -1), Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
   if (In < 0 || In >= Cl.nx) In = I;
@@ -121,9 +133,9 @@ __device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restric
   const S v01 = a * xc[I + sy * J + sz * Kn] + bq * xc[In + sy * J + sz * Kn];
   const S v11 = a * xc[I + sy * Jn + sz * Kn] + bq * xc[In + sy * Jn + sz * Kn];
   const S box = a * (a * v00 + bq * v10) + bq * (a * v01 + bq * v11);
-  if (!Cl.frac || !fine_fluid) return box;
+  if (!cfr || !fine_fluid) return box;
   const int o = (i & 1) | ((j & 1) << 1) | ((k & 1) << 2);
-  if ((Cl.octant[I + sy * J + sz * K] >> o) & 1) return box;
+  if ((coct[I + sy * J + sz * K] >> o) & 1) return box;
   const int ci[2] = {I, In}, cj[2] = {J, Jn}, ck[2] = {K, Kn};
   const S wt[2] = {a, bq};
   S num = (S)0, den = (S)0;
@@ -135,21 +147,28 @@ __device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restric
       for (int dx = 0; dx < 2; ++dx) {
         const int idx = ci[dx] + sy * cj[dy] + sz * ck[dz];
         const S w = wt[dx] * wt[dy] * wt[dz];
-        const bool act = (Cl.frac[idx] & kFracFluid) != 0;
+        const bool act = (cfr[idx] & kFracFluid) != 0;
         num += act ? w * xc[idx] : (S)0;
         den += act ? w : (S)0;
       }
   return den > (S)0 ? num / den : (S)0;
 }
+template <typename S>
+__device__ __forceinline__ S prolong_value(const MgLevel& Cl, const S* __restrict__ xc, int i, int j, int k,
+                                           bool fine_fluid) {
+  return prolong_value_fr<S>(Cl, Cl.frac, Cl.octant, xc, i, j, k, fine_fluid);
+}
 
 #define FOR_LEVEL_CELLS(L) FOR_LEVEL_CELLS_VB(L, hw_block())
+// Flat partition of a level's cells: consecutive threads take consecutive
+// cells (every lane busy however short the rows of a coarse level are).
 #define FOR_LEVEL_CELLS_VB(L, VB)                                                    \
   {                                                                                  \
-    const RowRange rr_ = block_rows((L).ny * (L).nz, (VB));                          \
-    for (int row = rr_.begin + threadIdx.y; row < rr_.end; row += kRowsPerBlock) {   \
-      const int k = row / (L).ny, j = row - k * (L).ny;                              \
-      for (int i = threadIdx.x; i < (L).nx; i += kLanesX) {                          \
-        const int c = i + (L).nx * row;
+    const RowRange cr_ = block_span((L).nx * (L).ny * (L).nz, (VB));                 \
+    for (int c = cr_.begin + flat_tid(); c < cr_.end; c += kBlock) {                 \
+      {                                                                              \
+        const int row = c / (L).nx, i = c - row * (L).nx;                            \
+        const int k = row / (L).ny, j = row - k * (L).ny;
 
 
 }  // namespace rh

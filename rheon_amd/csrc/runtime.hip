@@ -31,6 +31,14 @@ int grid_blocks(int rows) {
   return blocks < 1 ? 1 : (blocks > kMaxBlocks ? kMaxBlocks : (int)blocks);
 }
 
+// Small (latency-bound) levels get one cell per thread; from 2^21 cells on
+// (HBM-bound), four, as the row launches do.
+int level_blocks(long long cells) {
+  const long long per = cells >= (1ll << 21) ? 4ll * kBlock : kBlock;
+  const long long blocks = (cells + per - 1) / per;
+  return blocks < 1 ? 1 : (blocks > kMaxBlocks ? kMaxBlocks : (int)blocks);
+}
+
 int quad_blocks(int quads_per_row, int rows) {
   const long long quads = (long long)quads_per_row * rows;
   const long long blocks = (quads + kGroupsPerBlock * kBlock - 1) / (kGroupsPerBlock * kBlock);

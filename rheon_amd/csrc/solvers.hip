@@ -660,17 +660,30 @@ rh_status MgSolver::set_density(double density) {
   h_.tail_lds = 0;
   if (h_.coarse_inv) {
     const size_t s = config_.precision == RH_F64 ? 8 : 4;
+    auto a16 = [](size_t b) { return (b + 15) / 16 * 16; };
+    // per tail level: x, y, b; fractions if the level has them; octants
+    // where the level is the coarse side of a tail prolongation
+    auto sizes = [&](int l, int m, size_t q[5]) {
+      const size_t n = (size_t)h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz;
+      q[0] = q[1] = q[2] = a16(s * n);
+      q[3] = h_.lv[m].frac ? a16(4 * n) : 0;
+      q[4] = (m > l && h_.lv[m].octant) ? a16(n) : 0;
+    };
     for (int l = 1; l < h_.levels; ++l) {
       size_t bytes = 0;
-      for (int m = l; m < h_.levels; ++m)
-        bytes += 3 * ((s * h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz + 15) / 16 * 16);
-      if (bytes > (size_t)kMgTailLds) continue;
+      for (int m = l; m < h_.levels; ++m) {
+        size_t q[5];
+        sizes(l, m, q);
+        bytes += q[0] + q[1] + q[2] + q[3] + q[4];
+      }
+      if (bytes > (size_t)mg_tail_lds_limit()) continue;
       size_t off = 0;
       for (int m = l; m < h_.levels; ++m) {
-        const size_t a = (s * h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz + 15) / 16 * 16;
-        for (int q = 0; q < 3; ++q) {
-          h_.tail_off[m][q] = (int)off;
-          off += a;
+        size_t q[5];
+        sizes(l, m, q);
+        for (int k = 0; k < 5; ++k) {
+          h_.tail_off[m][k] = q[k] ? (int)off : -1;
+          off += q[k];
         }
       }
       h_.tail_level = l;
