@@ -208,7 +208,7 @@ namespace {
 
 enum class Op {
   kBeginStep, kValidateAcc, kAdvect, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kFill,
-  kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothS, kRestrictQ, kRestrictS,
+  kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothQFirst, kSmoothS, kRestrictQ, kRestrictS,
   kProlongQ, kProlongS, kTail, kCoarseDense, kEndSolve, kRegauge, kReport, kFinReport,
   kQuality, kCorrect, kDivMax, kFinDiv, kRate, kFinRate, kGate, kCommit
 };
@@ -639,12 +639,14 @@ rh_status Batch::build() {
   // outer residual records (quad / scalar split) and the V-cycle stages
   std::vector<BOuterArgs> outer(n);
   std::vector<int> outer_q(n, 0), outer_s(n, 0);
+  std::vector<bool> from_b(n, false);  // second finest pre-sweep from b0 alone (x0 not stored)
   for (size_t i = 0; i < n; ++i) {
     const MgHierarchy& h = mg(i)->hierarchy();
     const MgHierarchy* dh = mg(i)->device_hierarchy();
     Scratch& scr = scenes_[i]->solver->impl->scratch();
+    from_b[i] = quad_ok && mg_first_from_b(grids[i], h, scenes_[i]->p, scenes_[i]->rhs);
     outer[i] = BOuterArgs{d_grids_ + i, &dh->lv[0], scenes_[i]->p, scenes_[i]->rhs, (float*)h.b[0],
-                          (float*)h.x[0], h.tau_pre[0], scr.ctl, scr.pdd, scr.pmx};
+                          from_b[i] ? nullptr : (float*)h.x[0], h.tau_pre[0], scr.ctl, scr.pdd, scr.pmx};
     const bool q = quad_ok && mg_quad_outer_ok(grids[i], h.lv[0], scenes_[i]->p, scenes_[i]->rhs);
     if (q)
       outer_q[i] = qblk(mg_quad_outer_blocks(grids[i]));
@@ -681,7 +683,7 @@ rh_status Batch::build() {
     // sweep `sw` of the pre (post = false) or post side of level l
     auto smooth_stage = [&](int l, bool to_p, bool post_side, int sw) {
       std::vector<BSmoothArgs> r(n);
-      std::vector<int> bq(n, 0), bs(n, 0);
+      std::vector<int> bq(n, 0), bs(n, 0), bf(n, 0);
       for (size_t i = 0; i < n; ++i) {
         if (l >= bottom[i]) continue;
         const MgHierarchy& h = mg(i)->hierarchy();
@@ -689,15 +691,19 @@ rh_status Batch::build() {
         const bool cx = in_x[i][l];
         r[i] = BSmoothArgs{&dh->lv[l], (const float*)(cx ? h.x[l] : h.y[l]), (const float*)h.b[l],
                            (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
-                           post_side ? h.tau_post[sw] : h.tau_pre[sw], scenes_[i]->solver->impl->scratch().ctl};
+                           post_side ? h.tau_post[sw] : h.tau_pre[sw], scenes_[i]->solver->impl->scratch().ctl,
+                           h.tau_pre[0]};
         const bool q = quad_ok && mg_quad_level(h.lv[l]);
-        if (q)
+        if (l == 0 && !post_side && sw == 1 && from_b[i])
+          bf[i] = qblk(mg_quad_level_blocks(h.lv[l]));  // x0 recomputed from b0
+        else if (q)
           bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
         else
           bs[i] = lblk(h.lv[l]);
         if (!to_p) in_x[i][l] = !cx;
       }
       stage(to_p ? Op::kSmoothQUpdate : Op::kSmoothQ, r.data(), sizeof(BSmoothArgs), bq);
+      stage(Op::kSmoothQFirst, r.data(), sizeof(BSmoothArgs), bf);
       stage(Op::kSmoothS, r.data(), sizeof(BSmoothArgs), bs);  // the scalar body updates p itself
     };
     for (int l = 0; l < maxL; ++l) {
@@ -873,6 +879,7 @@ void Batch::run_program(hipStream_t s) {
         break;
       case Op::kSmoothQ: launch_b_mg_smooth_q(bl, false, s); break;
       case Op::kSmoothQUpdate: launch_b_mg_smooth_q(bl, true, s); break;
+      case Op::kSmoothQFirst: launch_b_mg_smooth_first_q(bl, s); break;
       case Op::kSmoothS: launch_b_mg_smooth(bl, s); break;
       case Op::kRestrictQ: launch_b_mg_restrict_q(bl, s); break;
       case Op::kRestrictS: launch_b_mg_restrict(bl, s); break;

@@ -80,6 +80,7 @@ struct BSmoothArgs {
   double* p;  // non-null: p += x (finest level, last sweep)
   double tau;  // sweep weight
   const PcgControl* ctl;
+  double tau_first;  // first-sweep weight (the sweep from b0 alone)
 };
 struct BRestrictArgs {
   const MgLevel* F;
@@ -114,6 +115,7 @@ struct BCoarseArgs {  // one block per scene: dense coarsest solve
 
 void launch_b_mg_outer_q(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_smooth_q(const BatchLaunch& bl, bool update, hipStream_t s);
+void launch_b_mg_smooth_first_q(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_restrict_q(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_prolong_q(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_outer(const BatchLaunch& bl, hipStream_t s);

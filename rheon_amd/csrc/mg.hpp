@@ -89,6 +89,12 @@ void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const doubl
                                 double* pmx, hipStream_t s);
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
                         double tau, const PcgControl* ctl, hipStream_t s);
+// Second finest pre-sweep from b0 alone (x0 = tau1 b0 / D recomputed per
+// position; the outer residual then does not store x0).
+void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
+                              const PcgControl* ctl, hipStream_t s);
+struct MgHierarchy;
+bool mg_first_from_b(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b);
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
                           float* xc, double tau, const PcgControl* ctl, hipStream_t s);
 void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,
@@ -122,11 +128,13 @@ struct FinMgTrue {
   }
 };
 
+// x0_implicit (FP32 only, mg_first_from_b): the outer residual does not
+// store the first pre-sweep and the cycle starts from b0 alone.
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
-                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s);
+                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit);
 template <typename S>
-void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s);
+void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit);
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s);
 int mg_tail_lds_limit();  // dynamic LDS bytes the tail kernels may use
 

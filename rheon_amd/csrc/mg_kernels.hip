@@ -580,15 +580,15 @@ static inline dim3 level_grid(const MgLevel& L) { return dim3(level_blocks((long
 
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
-                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s) {
+                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit) {
   dim3 grid = cells_grid(g);
   {
     prof::Scope ps(prof::kOuterResidual, s, it);
     const bool quad = sizeof(S) == 4 && h.quad && mg_quad_outer_ok(g, h.lv[0], p, b);
     if (quad) grid = dim3(mg_quad_outer_blocks(g));
     if (quad)
-      launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], (float*)h.x[0], h.tau_pre[0], ctl, pdd,
-                                 pmx, s);
+      launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], x0_implicit ? nullptr : (float*)h.x[0],
+                                 h.tau_pre[0], ctl, pdd, pmx, s);
     else
       launch(k_mg_outer_residual<S>, grid, block_shape(), 0, s, g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
                                                             h.tau_pre[0], ctl, pdd, pmx);
@@ -611,10 +611,12 @@ static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p
                                                                  tau, p, ctl);
 }
 
-// V-cycle from level l; on entry x[l] holds the first pre-smoothing sweep.
+// V-cycle from level l; on entry x[l] holds the first pre-smoothing sweep
+// (x0_implicit, level 0: only b[0] is there and the second sweep
+// recomputes the first from it).
 template <typename S>
 static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl, int it,
-                   hipStream_t s) {
+                   hipStream_t s, bool x0_implicit) {
   S* x = (S*)h.x[l];
   S* y = (S*)h.y[l];
   const S* b = (const S*)h.b[l];
@@ -642,7 +644,12 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   S* cur = x;
   S* nxt = y;
   for (int sweep = 1; sweep < h.pre; ++sweep) {
-    smooth<S>(h, l, cur, nxt, nullptr, h.tau_pre[sweep], ctl, it, s);
+    if (sweep == 1 && l == 0 && x0_implicit) {
+      prof::Scope ps(prof::kMgSmoothFirst, s, it);
+      launch_mg_smooth_first_q(h.lv[0], (const float*)b, (float*)nxt, h.tau_pre[0], h.tau_pre[1], ctl, s);
+    } else {
+      smooth<S>(h, l, cur, nxt, nullptr, h.tau_pre[sweep], ctl, it, s);
+    }
     S* t = cur;
     cur = nxt;
     nxt = t;
@@ -656,7 +663,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
       launch(k_mg_restrict<S>, level_grid(h.lv[l + 1]), block_shape(), 0, s,
           h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.tau_pre[0], ctl);
   }
-  vcycle<S>(h, l + 1, p, ctl, it, s);
+  vcycle<S>(h, l + 1, p, ctl, it, s, false);
   {
     prof::Scope ps(mg_slot(l, prof::kMgProlong), s, l == 0 ? it : -2);
     if (sizeof(S) == 4 && h.quad && mg_quad_pair(h.lv[l], h.lv[l + 1]))
@@ -691,8 +698,8 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
 }
 
 template <typename S>
-void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s) {
-  vcycle<S>(h, 0, p, ctl, it, s);
+void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit) {
+  vcycle<S>(h, 0, p, ctl, it, s, x0_implicit);
 }
 
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
@@ -701,10 +708,10 @@ void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
 }
 
 template void mg_outer_residual<float>(const DevGrid&, const MgHierarchy&, const double*,
-                                       const double*, PcgControl*, DD*, double*, int, hipStream_t);
+                                       const double*, PcgControl*, DD*, double*, int, hipStream_t, bool);
 template void mg_outer_residual<double>(const DevGrid&, const MgHierarchy&, const double*,
-                                        const double*, PcgControl*, DD*, double*, int, hipStream_t);
-template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t);
-template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t);
+                                        const double*, PcgControl*, DD*, double*, int, hipStream_t, bool);
+template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool);
+template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool);
 
 }  // namespace rh

@@ -124,7 +124,8 @@ __device__ __forceinline__ double quad_apply_d(const Quad<double>& Q, int u, con
 }  // namespace
 
 // Outer FP64 residual r = b - K p with the first pre-smoothing sweep fused
-// (as k_mg_outer_residual).
+// (as k_mg_outer_residual); x0 = null: the sweep's result is not stored
+// (b_mg_smooth_first_q recomputes it where it is needed).
 __device__ __forceinline__ void
 b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const double* __restrict__ p, const double* __restrict__ b,
                       float* __restrict__ b0, float* __restrict__ x0, double tau, const PcgControl* ctl,
@@ -153,7 +154,7 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
     rx[u] = (float)(tau * L0.odiag[mk]) * rb[u];
   }
   store4(b0, c, rb);
-  store4(x0, c, rx);
+  if (x0) store4(x0, c, rx);
   END_QUADS
   block_publish<1, 2>(s, m, pdd, pmx, vb);
 }
@@ -190,6 +191,96 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
   } else {
     store4(xout, c, xn);
   }
+  END_QUADS
+}
+
+// The finest level's second pre-smoothing sweep reading b0 only: the first
+// sweep's value x0 = (float)(tau1 / D) b0, which the outer residual no
+// longer stores, is recomputed at every stencil position from b0 and the
+// position's open-face mask (the same expression, so the same bits), then
+// x1 = x0 + (float)(tau2 / D) (b0 - K x0).  Saves the x0 store and load
+// (8 bytes per cell and cycle).
+__device__ __forceinline__ void
+b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, float* __restrict__ xout,
+                    double tau1, double tau2, const PcgControl* ctl) {
+  if (ctl->done) return;
+  __shared__ float s1[64], s2[64];  // (float)(tau / D) per open-face mask
+  {
+    const int t = flat_tid();
+    if (t < 64) {
+      s1[t] = (float)(tau1 * L.odiag[t]);
+      s2[t] = (float)(tau2 * L.odiag[t]);
+    }
+    __syncthreads();
+  }
+  const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
+  const int nz_ = L.nz;
+  const int sy = L.nx, sz = L.nx * L.ny;
+  FOR_QUADS(L.nx, L.ny)
+  Quad<float> Q;
+  load_quad(b, L.nx, L.ny, L.nz, i0, j, k, c, Q);
+  float bb[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) bb[u] = Q.c[u];
+  // positions of the loaded values (wall neighbours clamp to the cell)
+  const int jm = j > 0 ? j - 1 : j, jp = j + 1 < L.ny ? j + 1 : j;
+  const int km = k > 0 ? k - 1 : k, kp = k + 1 < L.nz ? k + 1 : k;
+  const int il = i0 > 0 ? i0 - 1 : i0, ir = i0 + 4 < L.nx ? i0 + 4 : i0 + 3;
+  int mc[4], mym[4], myp[4], mzm[4], mzp[4], ml, mr;
+  if (L.mask) {
+    const unsigned a = *reinterpret_cast<const unsigned*>(L.mask + c);
+    const unsigned ym4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (jm - j) * sy));
+    const unsigned yp4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (jp - j) * sy));
+    const unsigned zm4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (km - k) * sz));
+    const unsigned zp4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (kp - k) * sz));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      mc[u] = (int)((a >> (8 * u)) & 63u);
+      mym[u] = (int)((ym4 >> (8 * u)) & 63u);
+      myp[u] = (int)((yp4 >> (8 * u)) & 63u);
+      mzm[u] = (int)((zm4 >> (8 * u)) & 63u);
+      mzp[u] = (int)((zp4 >> (8 * u)) & 63u);
+    }
+    ml = L.mask[c + (il - i0)] & 63;
+    mr = L.mask[c + (ir - i0)] & 63;
+  } else {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      mc[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, k);
+      mym[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, jm, k);
+      myp[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, jp, k);
+      mzm[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, km);
+      mzp[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, kp);
+    }
+    ml = box_mask6(L.nx, L.ny, L.nz, il, j, k);
+    mr = box_mask6(L.nx, L.ny, L.nz, ir, j, k);
+  }
+  // b0 -> x0 at every position
+  const float xl = s1[ml] * Q.xm[0], xr = s1[mr] * Q.xp[3];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    Q.c[u] = s1[mc[u]] * Q.c[u];
+    Q.ym[u] = s1[mym[u]] * Q.ym[u];
+    Q.yp[u] = s1[myp[u]] * Q.yp[u];
+    Q.zm[u] = s1[mzm[u]] * Q.zm[u];
+    Q.zp[u] = s1[mzp[u]] * Q.zp[u];
+  }
+  Q.xm[0] = xl;
+  Q.xm[1] = Q.c[0];
+  Q.xm[2] = Q.c[1];
+  Q.xm[3] = Q.c[2];
+  Q.xp[0] = Q.c[1];
+  Q.xp[1] = Q.c[2];
+  Q.xp[2] = Q.c[3];
+  Q.xp[3] = xr;
+  float xn[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (L.mask) close_faces(Q, u, mc[u]);
+    const float ax = quad_apply_f(Q, u, w0, w1, w2);
+    xn[u] = Q.c[u] + s2[mc[u]] * (bb[u] - ax);
+  }
+  store4(xout, c, xn);
   END_QUADS
 }
 
@@ -344,6 +435,11 @@ k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict_
   b_mg_smooth_q<kUpdate>(hw_block(), L, xin, b, xout, p, tau, ctl);
 }
 __global__ void __launch_bounds__(kBlock)
+k_mg_smooth_first_q(MgLevel L, const float* __restrict__ b, float* __restrict__ xout, double tau1, double tau2,
+                    const PcgControl* ctl) {
+  b_mg_smooth_first_q(hw_block(), L, b, xout, tau1, tau2, ctl);
+}
+__global__ void __launch_bounds__(kBlock)
 k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float* __restrict__ b,
                 float* __restrict__ bc, float* __restrict__ xc, double tau, const PcgControl* ctl) {
   b_mg_restrict_q(hw_block(), F, Cl, x, b, bc, xc, tau, ctl);
@@ -372,6 +468,11 @@ kb_mg_smooth_q(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix
   b_mg_smooth_q<kUpdate>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
+kb_mg_smooth_first_q(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BSmoothArgs)
+  b_mg_smooth_first_q(vb, *a.L, a.b, a.xout, a.tau_first, a.tau, a.ctl);
+}
+__global__ void __launch_bounds__(kBlock)
 kb_mg_restrict_q(const BRestrictArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BRestrictArgs)
   b_mg_restrict_q(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.tau, a.ctl);
@@ -392,6 +493,10 @@ void launch_b_mg_smooth_q(const BatchLaunch& bl, bool update, hipStream_t s) {
     kb_mg_smooth_q<true><<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
   else
     kb_mg_smooth_q<false><<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_mg_smooth_first_q(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_smooth_first_q<<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
 }
 void launch_b_mg_restrict_q(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks)
@@ -431,6 +536,16 @@ void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, floa
     launch(k_mg_smooth_q<true>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
   else
     launch(k_mg_smooth_q<false>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
+}
+
+void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
+                              const PcgControl* ctl, hipStream_t s) {
+  launch(k_mg_smooth_first_q, dim3(quad_blocks(L.nx / 4, L.ny * L.nz)), block_shape(), 0, s, L, b, xout, tau1,
+         tau2, ctl);
+}
+
+bool mg_first_from_b(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b) {
+  return h.quad && h.pre >= 2 && h.levels > 1 && mg_quad_outer_ok(g, h.lv[0], p, b);
 }
 
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,

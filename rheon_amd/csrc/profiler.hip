@@ -40,7 +40,8 @@ const char* kNames[kNumSlots] = {
     "pcg_apply_dir", "pcg_update",      "pcg_true_residual", "pcg_replace",   "pcg_direction",
     "pcg_sgs_black", "pcg_sgs_red",     "advect_velocity",   "build_rhs",     "correct",
     "divergence_max", "mg_smooth",      "mg_residual",       "mg_restrict",   "mg_prolong",
-    "mg_coarse_levels", "mg_outer_residual", "outer_update",   "batch_step",    "mg_smooth_update"};
+    "mg_coarse_levels", "mg_outer_residual", "outer_update",   "batch_step",    "mg_smooth_update",
+    "mg_smooth_first"};
 
 }  // namespace
 

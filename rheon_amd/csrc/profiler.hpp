@@ -34,6 +34,7 @@ enum Slot {
   kOuterUpdate,
   kBatch,
   kMgSmoothUpdate,
+  kMgSmoothFirst,
   kNumSlots
 };
 

@@ -713,15 +713,17 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   // compatibility + threshold, p = 0, initial true residual (it = 0)
   launch_rhs_sums(g, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
   RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
-  mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s);
+  const bool x0_implicit = sizeof(S) == 4 && mg_first_from_b(g, h_, d_pressure, d_rhs);
+  mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s, x0_implicit);
   RH_HIP(hipGetLastError());
   bool cancelled = false;
   const int poll = st.poll_interval > 0 ? st.poll_interval : 2;
   rh_status rs = drive(
       h.max_iterations, poll,
       [&](int it) {
-        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s);
-        mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s);
+        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s, x0_implicit);
+        mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s,
+                             x0_implicit);
       },
       &cancelled, cancel, ctx);
   if (rs) return rs;
