@@ -85,6 +85,48 @@ def test_advect_velocity_bits(counts, dtype):
             assert bits(got[a]) == bits(want[a]), (a, dt)
 
 
+# Power-of-two spacings, where every face position is an exact lattice
+# point of the sampler's arithmetic (first RK2 stage weights exactly 0, 1/2
+# or 1), and one with an origin off the half-cell lattice.
+LATTICE_SPECS = [
+    GridSpec((32, 32, 32), (1 / 32,) * 3),
+    GridSpec((24, 16, 40), (1 / 16, 1 / 32, 1 / 8), (-0.5, 0.25, 1.0)),
+    GridSpec((17, 9, 33), (0.25, 0.5, 0.125), (-2.0, 4.0, 0.5)),
+    GridSpec((20, 12, 8), (1 / 16, 1 / 16, 1 / 16), (0.1, 0.3, -0.7)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("which", range(len(LATTICE_SPECS)))
+def test_advect_velocity_bits_lattice_grids(which, dtype):
+    spec = LATTICE_SPECS[which]
+    vel = wavy_faces(spec, dtype)
+    for dt in (0.001, 0.01, 0.5, 3.0):
+        got = R.advect_velocity(spec, vel, dt)
+        want = O.advect_velocity(ogrid(spec), vel, dt)
+        for a in range(3):
+            assert bits(got[a]) == bits(want[a]), (a, dt)
+
+
+@pytest.mark.gpu
+def test_advect_velocity_lattice_zero_and_nonfinite():
+    """Signed zeros and a non-finite value next to a sampled face."""
+    spec = LATTICE_SPECS[0]
+    vel = wavy_faces(spec, np.float32)
+    vel = [v.copy() for v in vel]
+    vel[1][::7] = -0.0
+    vel[2][::5] = 0.0
+    got = R.advect_velocity(spec, vel, 0.01)
+    want = O.advect_velocity(ogrid(spec), vel, 0.01)
+    for a in range(3):
+        assert bits(got[a]) == bits(want[a]), a
+    bad = [v.copy() for v in vel]
+    bad[1][spec.face_counts(1)[0] * 3 + 40] = np.inf
+    with pytest.raises(R.RheonError):
+        R.advect_velocity(spec, bad, 0.01)
+
+
 def test_advect_tracer_bits():
     # tests/advection_contract.rs:6-41 scene: sparse tracer, clamping, tiny donor
     spec = GridSpec((41, 29, 17), (0.07, 0.11, 0.19), (-2.0, 4.0, 0.3))
