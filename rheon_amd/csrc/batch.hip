@@ -208,7 +208,7 @@ namespace {
 
 enum class Op {
   kBeginStep, kValidateAcc, kAdvect, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kFill,
-  kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothQFirst, kSmoothS, kRestrictQ, kRestrictS,
+  kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothQFirst, kSmoothQFrac, kSmoothS, kRestrictQ, kRestrictQFrac, kRestrictS,
   kProlongQ, kProlongS, kTail, kCoarseDense, kEndSolve, kRegauge, kReport, kFinReport,
   kQuality, kCorrect, kDivMax, kFinDiv, kRate, kFinRate, kGate, kCommit
 };
@@ -683,7 +683,7 @@ rh_status Batch::build() {
     // sweep `sw` of the pre (post = false) or post side of level l
     auto smooth_stage = [&](int l, bool to_p, bool post_side, int sw) {
       std::vector<BSmoothArgs> r(n);
-      std::vector<int> bq(n, 0), bs(n, 0), bf(n, 0);
+      std::vector<int> bq(n, 0), bs(n, 0), bf(n, 0), bqf(n, 0);
       for (size_t i = 0; i < n; ++i) {
         if (l >= bottom[i]) continue;
         const MgHierarchy& h = mg(i)->hierarchy();
@@ -693,24 +693,27 @@ rh_status Batch::build() {
                            (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
                            post_side ? h.tau_post[sw] : h.tau_pre[sw], scenes_[i]->solver->impl->scratch().ctl,
                            h.tau_pre[0]};
-        const bool q = quad_ok && mg_quad_level(h.lv[l]);
+        const bool q = quad_ok && mg_quad_size_ok(h.lv[l], l) && mg_quad_level(h.lv[l]);
         if (l == 0 && !post_side && sw == 1 && from_b[i])
           bf[i] = qblk(mg_quad_level_blocks(h.lv[l]));  // x0 recomputed from b0
         else if (q)
           bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
+        else if (quad_ok && !to_p && mg_quad_size_ok(h.lv[l], l) && mg_quad_frac_level(h.lv[l]))
+          bqf[i] = qblk(mg_quad_level_blocks(h.lv[l]));
         else
           bs[i] = lblk(h.lv[l]);
         if (!to_p) in_x[i][l] = !cx;
       }
       stage(to_p ? Op::kSmoothQUpdate : Op::kSmoothQ, r.data(), sizeof(BSmoothArgs), bq);
       stage(Op::kSmoothQFirst, r.data(), sizeof(BSmoothArgs), bf);
+      stage(Op::kSmoothQFrac, r.data(), sizeof(BSmoothArgs), bqf);
       stage(Op::kSmoothS, r.data(), sizeof(BSmoothArgs), bs);  // the scalar body updates p itself
     };
     for (int l = 0; l < maxL; ++l) {
       for (int sw = 1; sw < pre; ++sw) smooth_stage(l, false, false, sw);
       // restriction l -> l+1
       std::vector<BRestrictArgs> r(n);
-      std::vector<int> bq(n, 0), bs(n, 0);
+      std::vector<int> bq(n, 0), bs(n, 0), bqf(n, 0);
       for (size_t i = 0; i < n; ++i) {
         if (l >= bottom[i]) continue;
         const MgHierarchy& h = mg(i)->hierarchy();
@@ -718,13 +721,14 @@ rh_status Batch::build() {
         r[i] = BRestrictArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)(in_x[i][l] ? h.x[l] : h.y[l]),
                              (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1], h.tau_pre[0],
                              scenes_[i]->solver->impl->scratch().ctl};
-        if (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]))
-          bq[i] = qblk(mg_quad_restrict_blocks(h.lv[l + 1]));
+        if (quad_ok && mg_quad_size_ok(h.lv[l], l) && mg_quad_pair(h.lv[l], h.lv[l + 1]))
+          (h.lv[l].frac ? bqf : bq)[i] = qblk(mg_quad_restrict_blocks(h.lv[l + 1]));
         else
           bs[i] = lblk(h.lv[l + 1]);
         in_x[i][l + 1] = true;
       }
       stage(Op::kRestrictQ, r.data(), sizeof(BRestrictArgs), bq);
+      stage(Op::kRestrictQFrac, r.data(), sizeof(BRestrictArgs), bqf);
       stage(Op::kRestrictS, r.data(), sizeof(BRestrictArgs), bs);
       // bottoms at level l + 1
       std::vector<BTailArgs> t(n);
@@ -762,7 +766,7 @@ rh_status Batch::build() {
         r[i] = BProlongArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)h.x[l + 1],
                             (const float*)(cx ? h.x[l] : h.y[l]), (float*)(cx ? h.y[l] : h.x[l]),
                             scenes_[i]->solver->impl->scratch().ctl};
-        if (quad_ok && mg_quad_pair(h.lv[l], h.lv[l + 1]))
+        if (quad_ok && mg_quad_size_ok(h.lv[l], l) && mg_quad_prolong_pair(h.lv[l], h.lv[l + 1]))
           bq[i] = qblk(mg_quad_level_blocks(h.lv[l]));
         else
           bs[i] = lblk(h.lv[l]);
@@ -880,8 +884,10 @@ void Batch::run_program(hipStream_t s) {
       case Op::kSmoothQ: launch_b_mg_smooth_q(bl, false, s); break;
       case Op::kSmoothQUpdate: launch_b_mg_smooth_q(bl, true, s); break;
       case Op::kSmoothQFirst: launch_b_mg_smooth_first_q(bl, s); break;
+      case Op::kSmoothQFrac: launch_b_mg_smooth_qf(bl, s); break;
       case Op::kSmoothS: launch_b_mg_smooth(bl, s); break;
-      case Op::kRestrictQ: launch_b_mg_restrict_q(bl, s); break;
+      case Op::kRestrictQ: launch_b_mg_restrict_q(bl, false, s); break;
+      case Op::kRestrictQFrac: launch_b_mg_restrict_q(bl, true, s); break;
       case Op::kRestrictS: launch_b_mg_restrict(bl, s); break;
       case Op::kProlongQ: launch_b_mg_prolong_q(bl, s); break;
       case Op::kProlongS: launch_b_mg_prolong(bl, s); break;

@@ -116,7 +116,8 @@ struct BCoarseArgs {  // one block per scene: dense coarsest solve
 void launch_b_mg_outer_q(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_smooth_q(const BatchLaunch& bl, bool update, hipStream_t s);
 void launch_b_mg_smooth_first_q(const BatchLaunch& bl, hipStream_t s);
-void launch_b_mg_restrict_q(const BatchLaunch& bl, hipStream_t s);
+void launch_b_mg_smooth_qf(const BatchLaunch& bl, hipStream_t s);
+void launch_b_mg_restrict_q(const BatchLaunch& bl, bool frac, hipStream_t s);
 void launch_b_mg_prolong_q(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_outer(const BatchLaunch& bl, hipStream_t s);
 void launch_b_mg_smooth(const BatchLaunch& bl, hipStream_t s);

@@ -77,7 +77,18 @@ struct MgHierarchy {
 // Four-cells-per-lane FP32 kernels (levels with nx % 4 == 0 and no face
 // fractions); identical values to the scalar kernels.
 bool mg_quad_level(const MgLevel& L);
-bool mg_quad_pair(const MgLevel& F, const MgLevel& C);  // restriction / prolongation F <-> C
+bool mg_quad_pair(const MgLevel& F, const MgLevel& C);  // restriction F -> C (exact halving, nx % 4 == 0)
+bool mg_quad_prolong_pair(const MgLevel& F, const MgLevel& C);  // prolongation C -> F (F may have fractions)
+bool mg_quad_frac_level(const MgLevel& L);  // smoothing sweeps of fraction levels, four cells per lane
+// Four cells per lane pays on the finest level and on coarse levels of at
+// least 2^20 cells; smaller coarse levels are latency-bound and run faster
+// with one cell per thread (more waves in flight).
+constexpr long long kMgQuadMinCells = 1ll << 20;
+inline bool mg_quad_size_ok(const MgLevel& L, int l) {
+  return l == 0 || (long long)L.nx * L.ny * L.nz >= kMgQuadMinCells;
+}
+void launch_mg_smooth_qf(const MgLevel& L, const float* xin, const float* b, float* xout, double tau,
+                         const PcgControl* ctl, hipStream_t s);
 bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, const double* b);
 // Block counts of the four-cells-per-lane launches (one-scene grids; a
 // batched stage uses the same counts per scene).

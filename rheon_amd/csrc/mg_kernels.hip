@@ -382,14 +382,12 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
     }
     __syncthreads();
   }
-  // Every tail level's iterate starts in X(l); when the tail is only the
-  // coarsest level (l0 == levels - 1) no phase below touches cur[l0].
-  S* cur[kMgMaxLevels];
-  S* nxt[kMgMaxLevels];
-  for (int l = l0; l < levels; ++l) {
-    cur[l] = X(l);
-    nxt[l] = Y(l);
-  }
+  // Every tail level's iterate starts in X(l); bit l of `in_y` says it has
+  // moved to Y(l) (a bit set instead of per-level pointer arrays, which
+  // would live in scratch memory).
+  unsigned in_y = 0;
+  auto cur = [&](int l) { return ((in_y >> l) & 1u) ? Y(l) : X(l); };
+  auto nxt = [&](int l) { return ((in_y >> l) & 1u) ? X(l) : Y(l); };
   // Phase loop.  Going down, level l takes pre - 1 sweeps then restricts
   // to l + 1; the coarsest level is solved densely; going up, level l
   // takes the prolongation from l + 1, post sweeps and (below l0) a copy
@@ -414,34 +412,32 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
       op = kSmooth;
       tau = H->tau_post[step - 1];
     } else {
-      op = (l > l0 && cur[l] != X(l)) ? kCopy : kDone;
+      op = (l > l0 && ((in_y >> l) & 1u)) ? kCopy : kDone;
     }
     if (!down && l == levels - 1) op = kDense;
     const MgLevel& L = H->lv[l];
     switch (op) {
       case kSmooth:
-        tail_smooth<S>(L, FR(l), cur[l], B(l), nxt[l], tau);
+        tail_smooth<S>(L, FR(l), cur(l), B(l), nxt(l), tau);
         break;
       case kRestrict:
-        tail_restrict<S>(L, FR(l), H->lv[l + 1], FR(l + 1), cur[l], B(l), B(l + 1), X(l + 1), H->tau_pre[0]);
+        tail_restrict<S>(L, FR(l), H->lv[l + 1], FR(l + 1), cur(l), B(l), B(l + 1), X(l + 1), H->tau_pre[0]);
         break;
       case kDense:
         dense_solve<kTailThreads, S>(cells(l), H->coarse_inv, B(l), X(l), bs, red);
         break;
       case kProlong:
-        tail_prolong<S>(L, FR(l), H->lv[l + 1], FR(l + 1), OC(l + 1), X(l + 1), cur[l], nxt[l]);
+        tail_prolong<S>(L, FR(l), H->lv[l + 1], FR(l + 1), OC(l + 1), X(l + 1), cur(l), nxt(l));
         break;
       case kCopy:
-        tail_copy<S>(cells(l), cur[l], X(l));
+        tail_copy<S>(cells(l), cur(l), X(l));
         break;
       default:
         break;
     }
     // advance
     if (op == kSmooth || op == kProlong) {
-      S* t = cur[l];
-      cur[l] = nxt[l];
-      nxt[l] = t;
+      in_y ^= 1u << l;
       ++step;
     } else if (op == kRestrict) {
       ++l;
@@ -462,7 +458,8 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
   {
     S* gx = (S*)H->x[l0];
     const int n = cells(l0);
-    for (int c = threadIdx.x; c < n; c += kTailThreads) gx[c] = cur[l0][c];
+    const S* r = cur(l0);
+    for (int c = threadIdx.x; c < n; c += kTailThreads) gx[c] = r[c];
   }
 }
 template <typename S>
@@ -604,7 +601,10 @@ template <typename S>
 static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p, double tau,
                    const PcgControl* ctl, int it, hipStream_t s) {
   prof::Scope ps(mg_slot(l, p ? prof::kMgSmoothUpdate : prof::kMgSmooth), s, l == 0 ? it : -2);
-  if (sizeof(S) == 4 && h.quad && mg_quad_level(h.lv[l]) && ((uintptr_t)p & 15u) == 0)
+  const bool sized = mg_quad_size_ok(h.lv[l], l);
+  if (sizeof(S) == 4 && h.quad && sized && !p && mg_quad_frac_level(h.lv[l]))
+    launch_mg_smooth_qf(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, tau, ctl, s);
+  else if (sizeof(S) == 4 && h.quad && sized && mg_quad_level(h.lv[l]) && ((uintptr_t)p & 15u) == 0)
     launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, tau, ctl, s);
   else
     launch(k_mg_smooth<S>, level_grid(h.lv[l]), block_shape(), 0, s, h.lv[l], xin, (const S*)h.b[l], xout,
@@ -656,7 +656,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   }
   {
     prof::Scope ps(mg_slot(l, prof::kMgRestrict), s, l == 0 ? it : -2);
-    if (sizeof(S) == 4 && h.quad && mg_quad_pair(h.lv[l], h.lv[l + 1]))
+    if (sizeof(S) == 4 && h.quad && mg_quad_size_ok(h.lv[l], l) && mg_quad_pair(h.lv[l], h.lv[l + 1]))
       launch_mg_restrict_q(h.lv[l], h.lv[l + 1], (const float*)cur, (const float*)b, (float*)h.b[l + 1],
                            (float*)h.x[l + 1], h.tau_pre[0], ctl, s);
     else
@@ -666,7 +666,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   vcycle<S>(h, l + 1, p, ctl, it, s, false);
   {
     prof::Scope ps(mg_slot(l, prof::kMgProlong), s, l == 0 ? it : -2);
-    if (sizeof(S) == 4 && h.quad && mg_quad_pair(h.lv[l], h.lv[l + 1]))
+    if (sizeof(S) == 4 && h.quad && mg_quad_size_ok(h.lv[l], l) && mg_quad_prolong_pair(h.lv[l], h.lv[l + 1]))
       launch_mg_prolong_q(h.lv[l], h.lv[l + 1], (const float*)h.x[l + 1], (const float*)cur, (float*)nxt,
                           ctl, s);
     else

@@ -194,6 +194,86 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
   END_QUADS
 }
 
+// Fraction words of a four-cell row chunk: the cells' own (lower faces),
+// the next cell's along x, and the rows above in y and z (upper faces).
+struct FracRow {
+  unsigned fw[5];  // cells 0..3 and the cell after the chunk
+  unsigned fy[4], fz[4];
+  bool hy, hz;
+};
+__device__ __forceinline__ void load_frac_row(const MgLevel& L, int i0, int j, int k, int c, FracRow& R) {
+  const int sy = L.nx, sz = L.nx * L.ny;
+  R.hy = j + 1 < L.ny;
+  R.hz = k + 1 < L.nz;
+  const uint4 q4 = *reinterpret_cast<const uint4*>(L.frac + c);
+  const uint4 y4 = *reinterpret_cast<const uint4*>(L.frac + (R.hy ? c + sy : c));
+  const uint4 z4 = *reinterpret_cast<const uint4*>(L.frac + (R.hz ? c + sz : c));
+  R.fw[0] = q4.x;
+  R.fw[1] = q4.y;
+  R.fw[2] = q4.z;
+  R.fw[3] = q4.w;
+  R.fw[4] = L.frac[i0 + 4 < L.nx ? c + 4 : c + 3];
+  R.fy[0] = y4.x;
+  R.fy[1] = y4.y;
+  R.fy[2] = y4.z;
+  R.fy[3] = y4.w;
+  R.fz[0] = z4.x;
+  R.fz[1] = z4.y;
+  R.fz[2] = z4.z;
+  R.fz[3] = z4.w;
+}
+// lv_frac6 of cell u of the chunk.
+__device__ __forceinline__ Frac6 frac6_of(const FracRow& R, int u, bool hx) {
+  Frac6 f;
+  f.xm = (int)(R.fw[u] & 1023u);
+  f.ym = (int)((R.fw[u] >> 10) & 1023u);
+  f.zm = (int)((R.fw[u] >> 20) & 1023u);
+  f.xp = hx ? (int)(R.fw[u + 1] & 1023u) : 0;
+  f.yp = R.hy ? (int)((R.fy[u] >> 10) & 1023u) : 0;
+  f.zp = R.hz ? (int)((R.fz[u] >> 20) & 1023u) : 0;
+  return f;
+}
+// s_apply's fraction form for cell u of the quad.
+__device__ __forceinline__ float quad_apply_frac(const MgLevel& L, const Quad<float>& Q, int u, const Frac6& f) {
+  const float xc = Q.c[u];
+  return (float)L.wf[0] * ((float)f.xm * (xc - Q.xm[u]) + (float)f.xp * (xc - Q.xp[u])) +
+         (float)L.wf[1] * ((float)f.ym * (xc - Q.ym[u]) + (float)f.yp * (xc - Q.yp[u])) +
+         (float)L.wf[2] * ((float)f.zm * (xc - Q.zm[u]) + (float)f.zp * (xc - Q.zp[u]));
+}
+
+// One smoothing sweep on a level with face fractions (coarse levels with
+// obstacles or padding), four cells per lane: the fraction words of the
+// cells and of their +x/+y/+z neighbours come in as 16-byte vectors.  The
+// stencil and the scale are s_apply's and lv_odiag's fraction forms
+// verbatim, so the values equal the one-cell kernel's.
+__device__ __forceinline__ void
+b_mg_smooth_qf(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const float* __restrict__ b,
+               float* __restrict__ xout, double tau, const PcgControl* ctl) {
+  if (ctl->done) return;
+  const int nz_ = L.nz;
+  const int sy = L.nx, sz = L.nx * L.ny;
+  FOR_QUADS(L.nx, L.ny)
+  Quad<float> Q;
+  load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);
+  float bb[4];
+  load4(b, c, bb);
+  FracRow R;
+  load_frac_row(L, i0, j, k, c, R);
+  float xn[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const Frac6 f = frac6_of(R, u, i0 + u + 1 < L.nx);
+    const float xc = Q.c[u];
+    const float ax = quad_apply_frac(L, Q, u, f);
+    const double d = L.wf[0] * (double)(f.xm + f.xp) + L.wf[1] * (double)(f.ym + f.yp) +
+                     L.wf[2] * (double)(f.zm + f.zp);
+    const double od = d > 0.0 ? 1.0 / d : 0.0;
+    xn[u] = xc + (float)(tau * od) * (bb[u] - ax);
+  }
+  store4(xout, c, xn);
+  END_QUADS
+}
+
 // The finest level's second pre-smoothing sweep reading b0 only: the first
 // sweep's value x0 = (float)(tau1 / D) b0, which the outer residual no
 // longer stores, is recomputed at every stencil position from b0 and the
@@ -287,6 +367,7 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
 // Restriction (as k_mg_restrict): each lane owns two coarse cells
 // (2 I0, 2 I0 + 1) of coarse row (J, K), i.e. one four-cell chunk in each of
 // the four fine rows (2J + dy, 2K + dz).  Sums keep the (dz, dy, dx) order.
+template <bool kFrac>
 __device__ __forceinline__ void
 b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __restrict__ x, const float* __restrict__ b,
                 float* __restrict__ bc, float* __restrict__ xc, double tau, const PcgControl* ctl) {
@@ -308,12 +389,19 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
       load_quad(x, F.nx, F.ny, F.nz, i0, fj, fk, fc, Q);
       float bb[4];
       load4(b, fc, bb);
-      const unsigned m4 = F.mask ? *reinterpret_cast<const unsigned*>(F.mask + fc) : 0u;
       float r[4];
+      if (kFrac) {  // fine level with face fractions
+        FracRow R;
+        load_frac_row(F, i0, fj, fk, fc, R);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (F.mask) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
-        r[u] = bb[u] - quad_apply_f(Q, u, w0, w1, w2);
+        for (int u = 0; u < 4; ++u) r[u] = bb[u] - quad_apply_frac(F, Q, u, frac6_of(R, u, i0 + u + 1 < F.nx));
+      } else {
+        const unsigned m4 = F.mask ? *reinterpret_cast<const unsigned*>(F.mask + fc) : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (F.mask) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
+          r[u] = bb[u] - quad_apply_f(Q, u, w0, w1, w2);
+        }
       }
       s0 += r[0];
       s0 += r[1];
@@ -362,12 +450,23 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
   unsigned slow = 0;
   if (Cl.frac) {
     const unsigned oc = *reinterpret_cast<const unsigned short*>(Cl.octant + I0 + rows[0]);
-    const unsigned m4 = F.mask ? *reinterpret_cast<const unsigned*>(F.mask + c) : 0x40404040u;
+    // fluid flags of the four fine cells (level-0 mask, fraction words, or all)
+    unsigned fl = 0xFu;
+    if (F.mask) {
+      const unsigned m4 = *reinterpret_cast<const unsigned*>(F.mask + c);
+      fl = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fl |= ((m4 >> (8 * u)) & kFluidBit) ? 1u << u : 0u;
+    } else if (F.frac) {
+      const uint4 f4 = *reinterpret_cast<const uint4*>(F.frac + c);
+      fl = ((f4.x & kFracFluid) ? 1u : 0u) | ((f4.y & kFracFluid) ? 2u : 0u) | ((f4.z & kFracFluid) ? 4u : 0u) |
+           ((f4.w & kFracFluid) ? 8u : 0u);
+    }
     const int oyz = ((j & 1) << 1) | ((k & 1) << 2);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const unsigned bits = (oc >> (8 * (u >> 1))) & 0xffu;
-      const bool fluid = (m4 >> (8 * u)) & kFluidBit;
+      const bool fluid = (fl >> u) & 1u;
       if (fluid && !((bits >> ((u & 1) | oyz)) & 1u)) slow |= 1u << u;
     }
   }
@@ -435,14 +534,20 @@ k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict_
   b_mg_smooth_q<kUpdate>(hw_block(), L, xin, b, xout, p, tau, ctl);
 }
 __global__ void __launch_bounds__(kBlock)
+k_mg_smooth_qf(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b, float* __restrict__ xout,
+               double tau, const PcgControl* ctl) {
+  b_mg_smooth_qf(hw_block(), L, xin, b, xout, tau, ctl);
+}
+__global__ void __launch_bounds__(kBlock)
 k_mg_smooth_first_q(MgLevel L, const float* __restrict__ b, float* __restrict__ xout, double tau1, double tau2,
                     const PcgControl* ctl) {
   b_mg_smooth_first_q(hw_block(), L, b, xout, tau1, tau2, ctl);
 }
+template <bool kFrac>
 __global__ void __launch_bounds__(kBlock)
 k_mg_restrict_q(MgLevel F, MgLevel Cl, const float* __restrict__ x, const float* __restrict__ b,
                 float* __restrict__ bc, float* __restrict__ xc, double tau, const PcgControl* ctl) {
-  b_mg_restrict_q(hw_block(), F, Cl, x, b, bc, xc, tau, ctl);
+  b_mg_restrict_q<kFrac>(hw_block(), F, Cl, x, b, bc, xc, tau, ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 k_mg_prolong_q(MgLevel F, MgLevel Cl, const float* __restrict__ xc, const float* __restrict__ xin,
@@ -468,14 +573,20 @@ kb_mg_smooth_q(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix
   b_mg_smooth_q<kUpdate>(vb, *a.L, a.xin, a.b, a.xout, a.p, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
+kb_mg_smooth_qf(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BSmoothArgs)
+  b_mg_smooth_qf(vb, *a.L, a.xin, a.b, a.xout, a.tau, a.ctl);
+}
+__global__ void __launch_bounds__(kBlock)
 kb_mg_smooth_first_q(const BSmoothArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BSmoothArgs)
   b_mg_smooth_first_q(vb, *a.L, a.b, a.xout, a.tau_first, a.tau, a.ctl);
 }
+template <bool kFrac>
 __global__ void __launch_bounds__(kBlock)
 kb_mg_restrict_q(const BRestrictArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BRestrictArgs)
-  b_mg_restrict_q(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.tau, a.ctl);
+  b_mg_restrict_q<kFrac>(vb, *a.F, *a.C, a.x, a.b, a.bc, a.xc, a.tau, a.ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_mg_prolong_q(const BProlongArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
@@ -494,13 +605,20 @@ void launch_b_mg_smooth_q(const BatchLaunch& bl, bool update, hipStream_t s) {
   else
     kb_mg_smooth_q<false><<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
 }
+void launch_b_mg_smooth_qf(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks)
+    kb_mg_smooth_qf<<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
+}
 void launch_b_mg_smooth_first_q(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks)
     kb_mg_smooth_first_q<<<bl.blocks, block_shape(), 0, s>>>((const BSmoothArgs*)bl.args, bl.prefix, bl.n);
 }
-void launch_b_mg_restrict_q(const BatchLaunch& bl, hipStream_t s) {
-  if (bl.blocks)
-    kb_mg_restrict_q<<<bl.blocks, block_shape(), 0, s>>>((const BRestrictArgs*)bl.args, bl.prefix, bl.n);
+void launch_b_mg_restrict_q(const BatchLaunch& bl, bool frac, hipStream_t s) {
+  if (!bl.blocks) return;
+  if (frac)
+    kb_mg_restrict_q<true><<<bl.blocks, block_shape(), 0, s>>>((const BRestrictArgs*)bl.args, bl.prefix, bl.n);
+  else
+    kb_mg_restrict_q<false><<<bl.blocks, block_shape(), 0, s>>>((const BRestrictArgs*)bl.args, bl.prefix, bl.n);
 }
 void launch_b_mg_prolong_q(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks)
@@ -510,9 +628,11 @@ void launch_b_mg_prolong_q(const BatchLaunch& bl, hipStream_t s) {
 // ------------------------------------------------------------- launchers
 
 bool mg_quad_level(const MgLevel& L) { return L.nx % 4 == 0 && !L.frac; }
-bool mg_quad_pair(const MgLevel& F, const MgLevel& C) {
-  return mg_quad_level(F) && F.nx == 2 * C.nx && F.ny == 2 * C.ny && F.nz == 2 * C.nz;
+bool mg_quad_frac_level(const MgLevel& L) { return L.nx % 4 == 0 && L.frac; }
+bool mg_quad_prolong_pair(const MgLevel& F, const MgLevel& C) {
+  return F.nx % 4 == 0 && F.nx == 2 * C.nx && F.ny == 2 * C.ny && F.nz == 2 * C.nz;
 }
+bool mg_quad_pair(const MgLevel& F, const MgLevel& C) { return mg_quad_prolong_pair(F, C); }
 
 static inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
@@ -538,6 +658,12 @@ void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, floa
     launch(k_mg_smooth_q<false>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
 }
 
+void launch_mg_smooth_qf(const MgLevel& L, const float* xin, const float* b, float* xout, double tau,
+                         const PcgControl* ctl, hipStream_t s) {
+  launch(k_mg_smooth_qf, dim3(quad_blocks(L.nx / 4, L.ny * L.nz)), block_shape(), 0, s, L, xin, b, xout, tau,
+         ctl);
+}
+
 void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
                               const PcgControl* ctl, hipStream_t s) {
   launch(k_mg_smooth_first_q, dim3(quad_blocks(L.nx / 4, L.ny * L.nz)), block_shape(), 0, s, L, b, xout, tau1,
@@ -550,8 +676,11 @@ bool mg_first_from_b(const DevGrid& g, const MgHierarchy& h, const double* p, co
 
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
                           float* xc, double tau, const PcgControl* ctl, hipStream_t s) {
-  launch(k_mg_restrict_q, dim3(quad_blocks(Cl.nx / 2, Cl.ny * Cl.nz)), block_shape(), 0, s, F, Cl, x, b, bc, xc,
-         tau, ctl);
+  const dim3 grid(quad_blocks(Cl.nx / 2, Cl.ny * Cl.nz));
+  if (F.frac)
+    launch(k_mg_restrict_q<true>, grid, block_shape(), 0, s, F, Cl, x, b, bc, xc, tau, ctl);
+  else
+    launch(k_mg_restrict_q<false>, grid, block_shape(), 0, s, F, Cl, x, b, bc, xc, tau, ctl);
 }
 
 void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, const float* xin,

@@ -31,11 +31,10 @@ int grid_blocks(int rows) {
   return blocks < 1 ? 1 : (blocks > kMaxBlocks ? kMaxBlocks : (int)blocks);
 }
 
-// Small (latency-bound) levels get one cell per thread; from 2^21 cells on
-// (HBM-bound), four, as the row launches do.
+// One cell per thread (coarse levels are latency-bound: the more waves in
+// flight the better), up to the block cap.
 int level_blocks(long long cells) {
-  const long long per = cells >= (1ll << 21) ? 4ll * kBlock : kBlock;
-  const long long blocks = (cells + per - 1) / per;
+  const long long blocks = (cells + kBlock - 1) / kBlock;
   return blocks < 1 ? 1 : (blocks > kMaxBlocks ? kMaxBlocks : (int)blocks);
 }
 
