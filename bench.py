@@ -93,6 +93,23 @@ KERNEL_BYTES_PER_CELL = {
 }
 
 
+def profiled_traffic(kernel: str):
+    """Calibrated HBM bytes per launch of `kernel` on the default workload,
+    from the committed rocprofv3 --pmc summary (profiles/*_traffic.md, line
+    `traffic_bytes_per_launch: name=bytes`), or None."""
+    import glob
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*_traffic.md"))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    for item in line.split(":", 1)[1].split():
+                        name, _, value = item.partition("=")
+                        if name == kernel:
+                            return float(value)
+    return None
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -660,6 +677,10 @@ def main() -> int:
         per_cell -= s_vec
     per_launch = per_cell * spec.cell_len
     achieved = per_launch / avg / 1e9
+    # measured HBM traffic (committed counter summary) applies to the default workload only
+    default_workload = (args.n == 256 and args.sphere == 0.2 and args.method == "mixed-mg-dc"
+                        and args.smoother == "chebyshev" and not args.mg_sweeps)
+    traffic = profiled_traffic(dom) if default_workload else None
 
     mean_iters = float(np.mean(iters))
     ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)), m_bytes, sweeps, x0_implicit)
@@ -705,7 +726,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes_per_launch": per_launch,
             "avg_launch_us": avg * 1e6,
             "timed_launches": launches,
