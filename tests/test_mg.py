@@ -154,10 +154,13 @@ def test_device_mg_fourier_projection_divergence():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("counts,sphere", [((64, 32, 48), False), ((64, 32, 48), True), ((40, 24, 16), False),
-                                           ((40, 24, 16), True)])
+                                           ((40, 24, 16), True), ((256, 256, 128), False),
+                                           ((256, 256, 128), True)])
 def test_device_mg_quad_kernels_match_scalar(counts, sphere):
     """The four-cells-per-lane FP32 kernels give the scalar kernels' values
-    bit for bit (same per-cell arithmetic)."""
+    bit for bit (same per-cell arithmetic).  256x256x128: its first coarse
+    level has 2^20 cells, so it also runs the four-cell kernels on a level
+    with face fractions (with the sphere)."""
     import rheon_amd as R
     from rheon_amd.scenes import centred_sphere, close_obstacle_faces
     spec = GridSpec((counts), tuple(1.0 / n for n in counts))
