@@ -275,7 +275,10 @@ def bench_batch(args) -> int:
     st = R.PressureSettings(args.rel, 1e-14, 1e-3, 40)
     vels = [taylor_green_noise(sp, seed=i) for i, sp in enumerate(specs)]
     dts = [courant_dt(sp, v, 0.5) for sp, v in zip(specs, vels)]
-    batch = R.SceneBatch(specs, settings=st, use_graph=True, actual_divergence_limit=1e-3)
+    # the graph holds a fixed number of V-cycles per step (converged scenes'
+    # blocks return at once); 20 is twice what these scenes need to 1e-8
+    cycles = 20
+    batch = R.SceneBatch(specs, settings=st, use_graph=True, actual_divergence_limit=1e-3, max_cycles=cycles)
     for i in range(len(specs)):
         batch.scene(i).set_velocity(vels[i])
     batch.set_dt(dts)
@@ -326,6 +329,7 @@ def bench_batch(args) -> int:
             "workload": f"{len(specs)} scenes, axes log-uniform [32,128] multiple of 8, {args.steps} steps, "
                         f"mixed-mg-dc to rel {args.rel:g}",
             "total_cells": cells,
+            "cycles_per_step_in_graph": cycles,
             "shapes": counts,
             "batch_seconds": batch_s,
             "looped_seconds": loop_s,

@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <vector>
@@ -239,7 +240,12 @@ class Batch {
   ~Batch();
   rh_status create(const rh_grid* grids, size_t n, const rh_batch_config& cfg);
   rh_status step(int steps, rh_step_report* reports, rh_status* statuses);
-  rh_status launch(int steps);  // enqueue (tables rebuilt if needed), no wait
+  // Enqueue `steps` steps (tables rebuilt if needed).  With a graph the
+  // whole step (max_cycles V-cycles) is one replay and nothing waits.
+  // Without one, the V-cycles go out in batches of kPollCycles and the host
+  // stops enqueuing once every scene's solve is done (polled one batch
+  // behind); `host_work`, if given, runs once while the device works.
+  rh_status launch(int steps, const std::function<rh_status()>& host_work = nullptr);
   rh_status wait(rh_step_report* reports, rh_status* statuses);
   double last_seconds() const { return last_seconds_; }  // device time of the last launch
   rh_scene* scene(size_t i) { return i < scenes_.size() ? scenes_[i] : nullptr; }
@@ -258,6 +264,8 @@ class Batch {
   rh_status build();
   rh_status reserve(size_t n);
   void run_program(hipStream_t s);
+  void run_range(size_t begin, size_t end, hipStream_t s);
+  rh_status run_polled(const std::function<rh_status()>* host_work);
   void release_graph();
   MgSolver* mg(size_t i) { return static_cast<MgSolver*>(scenes_[i]->solver->impl.get()); }
 
@@ -276,6 +284,11 @@ class Batch {
   unsigned char* d_tables_ = nullptr;
   size_t tables_bytes_ = 0;
   std::vector<Launch> program_;
+  std::vector<size_t> cycle_at_;  // program_ index of V-cycle it's first launch (it = 1..max_cycles_)
+  size_t post_at_ = 0;            // program_ index of the first launch after the solve
+  int* d_active_ = nullptr;       // [2] scenes still solving (device)
+  int* h_active_ = nullptr;       // [2] their pinned host copies
+  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   bool built_ = false;
@@ -289,6 +302,10 @@ Batch::~Batch() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto& e : ev_)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : poll_ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (d_active_) (void)hipFree(d_active_);
+  if (h_active_) (void)hipHostFree(h_active_);
   release_graph();
   if (d_tables_) (void)hipFree(d_tables_);
   if (d_grids_) (void)hipFree(d_grids_);
@@ -793,10 +810,13 @@ rh_status Batch::build() {
     program_.resize(mark);
   }
   outer_stage(0);
+  cycle_at_.assign(max_cycles_ + 1, 0);
   for (int it = 1; it <= max_cycles_; ++it) {
+    cycle_at_[it] = program_.size();
     program_.insert(program_.end(), cycle.begin(), cycle.end());
     outer_stage(it);
   }
+  post_at_ = program_.size();
   per_scene(Op::kEndSolve);
   stage(Op::kRegauge, cells.data(), sizeof(BCellArgs), cb);
   stage(Op::kReport, cells.data(), sizeof(BCellArgs), cb);
@@ -850,9 +870,12 @@ rh_status Batch::build() {
   return RH_OK;
 }
 
-void Batch::run_program(hipStream_t s) {
+void Batch::run_program(hipStream_t s) { run_range(0, program_.size(), s); }
+
+void Batch::run_range(size_t begin, size_t end, hipStream_t s) {
   const int n = (int)scenes_.size();
-  for (const Launch& L : program_) {
+  for (size_t li = begin; li < end; ++li) {
+    const Launch& L = program_[li];
     BatchLaunch bl;
     bl.blocks = L.blocks;
     bl.n = n;
@@ -918,13 +941,72 @@ void Batch::run_program(hipStream_t s) {
   }
 }
 
+// Scenes whose solve is still running (control block not done).
+__global__ void kb_count_active(const BatchScene* __restrict__ S, int n, int* out) {
+  __shared__ int count;
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int s = threadIdx.x; s < n; s += blockDim.x) mine += S[s].ctl->done ? 0 : 1;
+  if (mine) atomicAdd(&count, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) *out = count;
+}
+
+constexpr int kPollCycles = 2;
+
+// One step without a graph: the solve's V-cycles are enqueued kPollCycles
+// at a time; after each batch a one-block kernel counts the scenes still
+// solving and the count is copied to pinned host memory.  The host waits
+// for the previous batch's count (so the device always has one batch
+// queued) and stops enqueuing cycles once it is 0.  Skipped cycles would
+// only have launched kernels that return at once for every scene, so the
+// results equal the full program's.
+rh_status Batch::run_polled(const std::function<rh_status()>* host_work) {
+  const int n = (int)scenes_.size();
+  if (!d_active_) {
+    RH_HIP(hipMalloc(&d_active_, 2 * sizeof(int)));
+    RH_HIP(hipHostMalloc(&h_active_, 2 * sizeof(int), hipHostMallocDefault));
+    RH_HIP(hipEventCreateWithFlags(&poll_ev_[0], hipEventDisableTiming));
+    RH_HIP(hipEventCreateWithFlags(&poll_ev_[1], hipEventDisableTiming));
+  }
+  run_range(0, max_cycles_ >= 1 ? cycle_at_[1] : post_at_, stream_);
+  int it = 1, pending = -1;
+  bool worked = false;
+  for (int b = 0; it <= max_cycles_; ++b) {
+    const int last = std::min(max_cycles_, it + kPollCycles - 1);
+    run_range(cycle_at_[it], last < max_cycles_ ? cycle_at_[last + 1] : post_at_, stream_);
+    it = last + 1;
+    const int cur = b & 1;
+    kb_count_active<<<1, 256, 0, stream_>>>(d_scenes_, n, d_active_ + cur);
+    RH_HIP(hipMemcpyAsync(h_active_ + cur, d_active_ + cur, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    RH_HIP(hipEventRecord(poll_ev_[cur], stream_));
+    if (host_work && *host_work && !worked) {
+      worked = true;
+      rh_status w = (*host_work)();
+      if (w) return w;
+    }
+    if (pending >= 0) {
+      RH_HIP(hipEventSynchronize(poll_ev_[pending]));
+      if (h_active_[pending] == 0) break;
+    }
+    pending = cur;
+  }
+  if (host_work && *host_work && !worked) {
+    rh_status w = (*host_work)();
+    if (w) return w;
+  }
+  run_range(post_at_, program_.size(), stream_);
+  return RH_OK;
+}
+
 rh_status Batch::step(int steps, rh_step_report* reports, rh_status* statuses) {
   rh_status s = launch(steps);
   if (s) return s;
   return wait(reports, statuses);
 }
 
-rh_status Batch::launch(int steps) {
+rh_status Batch::launch(int steps, const std::function<rh_status()>& host_work) {
   const size_t n = scenes_.size();
   if (steps < 1) return RH_INVALID_ARGUMENT;
   launched_steps_ = 0;
@@ -948,8 +1030,13 @@ rh_status Batch::launch(int steps) {
     if (exec_) {
       RH_HIP(hipGraphLaunch(exec_, stream_));
     } else {
-      run_program(stream_);
+      rh_status r = run_polled(k == 0 ? &host_work : nullptr);
+      if (r) return r;
     }
+  }
+  if (exec_ && host_work) {
+    rh_status r = host_work();
+    if (r) return r;
   }
   RH_HIP(hipEventRecord(ev_[1], stream_));
   RH_HIP(hipGetLastError());
@@ -1194,27 +1281,35 @@ rh_status Server::tick(rh_tick_report* rep) {
   }
   rh_status s = batch_.set_members(members, dts);
   if (s) return s;
-  if (!members.empty() && (s = batch_.launch(1))) return s;
-  // 2. ... while newcomers are admitted (arrival order, memory permitting)
+  // 2. ... while newcomers are admitted (arrival order, memory permitting);
+  // the batch runs this once its first V-cycles are enqueued
   std::vector<ServerRequest*> joined;
-  while (!queue_.empty()) {
-    ServerRequest* r = queue_.front();
-    const size_t running = active_.size() + joined.size();
-    if (cfg_.max_active > 0 && running >= (size_t)cfg_.max_active) break;
-    if (used_ + r->plan > budget_) break;
-    queue_.erase(queue_.begin());
-    used_ += r->plan;
-    r->st.admitted = now();
-    rh_status a = admit(r);
-    if (a) {
-      r->st.state = 3;
-      r->st.status = a;
-      retire(r, false);
-      continue;
+  const std::function<rh_status()> admission = [&]() -> rh_status {
+    while (!queue_.empty()) {
+      ServerRequest* r = queue_.front();
+      const size_t running = active_.size() + joined.size();
+      if (cfg_.max_active > 0 && running >= (size_t)cfg_.max_active) break;
+      if (used_ + r->plan > budget_) break;
+      queue_.erase(queue_.begin());
+      used_ += r->plan;
+      r->st.admitted = now();
+      rh_status a = admit(r);
+      if (a) {
+        r->st.state = 3;
+        r->st.status = a;
+        retire(r, false);
+        continue;
+      }
+      r->st.state = 1;
+      joined.push_back(r);
+      ++T.admitted;
     }
-    r->st.state = 1;
-    joined.push_back(r);
-    ++T.admitted;
+    return RH_OK;
+  };
+  if (members.empty()) {
+    if ((s = admission())) return s;
+  } else if ((s = batch_.launch(1, admission))) {
+    return s;
   }
   // 3. wait, account, retire finished requests
   std::vector<rh_step_report> reps(members.size());
