@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
 def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 3, post: int = 1,
-                   x0_implicit: bool = True) -> float:
+                   x0_implicit: bool = True, cpre: int = 0, cpost: int = 0) -> float:
     """Compulsory bytes of one defect-correction cycle (outer f64 residual +
     V(pre, post) in storage of s bytes): per finest cell
       outer residual  read p, b (f64) + write b0, x0           16 + 2s
@@ -38,7 +38,8 @@ def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 3, post: int =
       restrict  read x, b + write coarse b, x (1/8 cells)       2s + 2s/8
       prolong   read x + coarse x (1/8) + write x               2s + s/8
       post-sweeps (post-1) x 3s, last one adds to p (f64 r/w)   2s + 16
-    and coarse levels l >= 1 the same minus the f64 terms, on N/8^l cells.
+    and coarse levels l >= 1 the same minus the f64 terms with their own
+    sweep counts (cpre, cpost; 0 = the finest level's), on N/8^l cells.
     x0_implicit (four-cell FP32 kernels, pre >= 2): the outer residual does
     not write x0 and the second pre-sweep reads b0 alone (s less each).
     With obstacles (m = 1) every finest stencil kernel also reads the 1-byte
@@ -48,20 +49,26 @@ def mg_cycle_bytes(spec: GridSpec, s: int, m: int = 0, pre: int = 3, post: int =
     if x0_implicit and pre >= 2:
         finest -= 2 * s
     finest += m * (1 + (pre - 1) + 1 + post)  # obstacle mask reads
-    coarse_per_cell = (pre - 1) * 3 * s + (2 * s + 2 * s / 8) + (2 * s + s / 8) + post * 3 * s
-    coarse_per_cell += 4 * m * (pre + 1 + post)  # face-fraction words
+    cpre, cpost = cpre or pre, cpost or post
+    coarse_per_cell = (cpre - 1) * 3 * s + (2 * s + 2 * s / 8) + (2 * s + s / 8) + cpost * 3 * s
+    coarse_per_cell += 4 * m * (cpre + 1 + cpost)  # face-fraction words
     coarse = sum(coarse_per_cell * n / 8 ** l for l in range(1, 8))
     return finest * n + coarse
 
 
+def coarse_sweeps(args) -> tuple:
+    """--mg-coarse-sweeps as (pre, post); (0, 0) = the finest level's counts."""
+    return tuple(int(v) for v in args.mg_coarse_sweeps.split(",")) if args.mg_coarse_sweeps else (0, 0)
+
+
 def algorithmic_bytes(spec: GridSpec, face_bytes: int, method: str, solver_bytes: int, iterations: int,
-                      m: int = 0, sweeps=(3, 1), x0_implicit: bool = False) -> dict:
+                      m: int = 0, sweeps=(3, 1), x0_implicit: bool = False, coarse=(0, 0)) -> dict:
     """Compulsory HBM bytes of one step (per-kernel minimal-fusion model)."""
     n = spec.cell_len
     F = spec.face_total
     sv = solver_bytes
     if method in ("mixed-mg-dc", "mg-f64-dc"):
-        per_iter = mg_cycle_bytes(spec, sv, m, *sweeps, x0_implicit=x0_implicit)
+        per_iter = mg_cycle_bytes(spec, sv, m, *sweeps, x0_implicit=x0_implicit, cpre=coarse[0], cpost=coarse[1])
     else:
         # Krylov vectors in sv bytes, the solution accumulated in f64 (16 B/cell per update)
         per_iter = {"jacobi-pcg-v1": (9 * sv + 16) * n + m * n, "rbsgs-pcg": (15 * sv + 16) * n + 3 * m * n}.get(method, 0)
@@ -278,7 +285,9 @@ def bench_batch(args) -> int:
     # the graph holds a fixed number of V-cycles per step (converged scenes'
     # blocks return at once); 20 is twice what these scenes need to 1e-8
     cycles = 20
-    batch = R.SceneBatch(specs, settings=st, use_graph=True, actual_divergence_limit=1e-3, max_cycles=cycles)
+    coarse = coarse_sweeps(args)
+    batch = R.SceneBatch(specs, settings=st, use_graph=True, actual_divergence_limit=1e-3, max_cycles=cycles,
+                         mg_coarse_sweeps=coarse)
     for i in range(len(specs)):
         batch.scene(i).set_velocity(vels[i])
     batch.set_dt(dts)
@@ -299,7 +308,8 @@ def bench_batch(args) -> int:
     # the one-scene path over the same scenes and steps
     singles = []
     for i, sp in enumerate(specs):
-        sc = R.Scene(sp, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e-3)
+        sc = R.Scene(sp, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e-3,
+                     mg_coarse_sweeps=coarse)
         sc.set_velocity(vels[i])
         sc.step(dts[i])  # warm (allocations, coarse inverse)
         sc.set_velocity(vels[i])
@@ -330,6 +340,7 @@ def bench_batch(args) -> int:
                         f"mixed-mg-dc to rel {args.rel:g}",
             "total_cells": cells,
             "cycles_per_step_in_graph": cycles,
+            "mg_coarse_sweeps": list(coarse) if args.mg_coarse_sweeps else "as finest",
             "shapes": counts,
             "batch_seconds": batch_s,
             "looped_seconds": loop_s,
@@ -568,6 +579,8 @@ def main() -> int:
                     help="multigrid smoother (Chebyshev weights or weighted Jacobi)")
     ap.add_argument("--mg-sweeps", default="", metavar="PRE,POST",
                     help="V-cycle sweeps (default 3,1 for Chebyshev, 2,2 for Jacobi)")
+    ap.add_argument("--mg-coarse-sweeps", default="", metavar="PRE,POST",
+                    help="sweeps of the levels below the finest (default: the finest level's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--compare", action="store_true",
                     help="also time one solve of the step's RHS by every method and report pressure errors")
@@ -614,7 +627,7 @@ def main() -> int:
     scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method,
                     solver_precision=R.F32 if args.method == "mixed-mg-dc" else R.F64,
                     settings=settings, actual_divergence_limit=1e-3, max_courant=0.0,
-                    mg_sweeps=sweeps, mg_smoother=smoother)
+                    mg_sweeps=sweeps, mg_smoother=smoother, mg_coarse_sweeps=coarse_sweeps(args))
     solid = centred_sphere(spec, args.sphere) if args.sphere > 0 else None
     if solid is not None:
         scene.set_solid(solid)
@@ -683,11 +696,12 @@ def main() -> int:
     achieved = per_launch / avg / 1e9
     # measured HBM traffic (committed counter summary) applies to the default workload only
     default_workload = (args.n == 256 and args.sphere == 0.2 and args.method == "mixed-mg-dc"
-                        and args.smoother == "chebyshev" and not args.mg_sweeps)
+                        and args.smoother == "chebyshev" and not args.mg_sweeps and not args.mg_coarse_sweeps)
     traffic = profiled_traffic(dom) if default_workload else None
 
     mean_iters = float(np.mean(iters))
-    ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)), m_bytes, sweeps, x0_implicit)
+    ab = algorithmic_bytes(spec, 4, args.method, s_vec, int(round(mean_iters)), m_bytes, sweeps, x0_implicit,
+                           coarse_sweeps(args))
     step_bytes = sum(ab.values())
     ms = elapsed / args.steps * 1e3
 
@@ -711,6 +725,7 @@ def main() -> int:
             "grid": list(spec.counts),
             "method": args.method,
             "mg_smoother": (f"{args.smoother} V({sweeps[0]},{sweeps[1]})" if mg else None),
+            "mg_coarse_sweeps": (list(coarse_sweeps(args)) if args.mg_coarse_sweeps else "as finest") if mg else None,
             "relative_residual": args.rel,
             "obstacle": (f"sphere r={args.sphere:g} at (0.5,0.5,0.5), {int(solid.sum())} solid cells"
                          if solid is not None else "none"),

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define RH_ABI_VERSION 1
+#define RH_ABI_VERSION 2
 
 /* Status codes.  0-6 mirror ProviderRejection (src/solver_provider.rs:54-62);
  * 7-19 mirror PressureError / OperatorError / SimulationError variants
@@ -189,6 +189,8 @@ typedef struct rh_solver_config {
   uint64_t memory_limit;     /* bytes; 0 = unlimited (PressureWorkspace limit) */
   uint32_t kernel_variant;   /* 0 = default kernels; RH_VARIANT_* bits select alternatives */
   uint32_t mg_smoother;      /* RH_SMOOTHER_* (0 = Chebyshev) */
+  int32_t mg_coarse_pre_sweeps;   /* levels below the finest; 0 = the finest level's count */
+  int32_t mg_coarse_post_sweeps;  /* levels below the finest; 0 = the finest level's count */
 } rh_solver_config;
 
 /* Multigrid smoothers.  Both sweep x <- x + tau_k (b - K x) / D; Jacobi uses

@@ -12,6 +12,8 @@ can be checked cycle by cycle, and its convergence studied, on the CPU:
   Chebyshev weights (default, V(3,1)): tau_k = 1/lambda_k, lambda_k the roots
   of the degree-nu Chebyshev polynomial mapped to [lo, 2] (lo = 0.3 for
   nu <= 2, else 0.25), or weighted Jacobi (tau_k = omega = 6/7, V(2,2));
+  the coarse levels may take their own sweep counts (coarse_pre,
+  coarse_post; default the finest level's);
   first pre-sweep from zero x = tau_1 b/D; restriction = sum of the 8
   children's residuals; trilinear cell-centred prolongation (1D weights 3/4,
   1/4; walls mirror the parent); coarsest level solved exactly on the
@@ -122,28 +124,36 @@ def sweep_weights(n, smoother="chebyshev", omega=6.0 / 7.0):
     return [1.0 / (c + h * np.cos(np.pi * (2 * k + 1) / (2 * n))) for k in range(n)]
 
 
-def vcycle(lv, l, b, tau_pre, tau_post, dtype=np.float64):
+def vcycle(lv, l, b, tau_pre, tau_post, dtype=np.float64, tau_cpre=None, tau_cpost=None):
+    """V-cycle from level l; levels below the finest use (tau_cpre,
+    tau_cpost) when given."""
     L = lv[l]
     if l == len(lv) - 1:
         return coarsest_solve(L, b.astype(np.float64)).astype(dtype)
-    x = (tau_pre[0] * b / L.D).astype(dtype)
-    for t in tau_pre[1:]:
+    tp = tau_pre if l == 0 or tau_cpre is None else tau_cpre
+    tq = tau_post if l == 0 or tau_cpost is None else tau_cpost
+    x = (tp[0] * b / L.D).astype(dtype)
+    for t in tp[1:]:
         x = (x + t * (b - L.apply(x)) / L.D).astype(dtype)
     bc = restrict(b - L.apply(x)).astype(dtype)
-    xc = vcycle(lv, l + 1, bc, tau_pre, tau_post, dtype)
+    xc = vcycle(lv, l + 1, bc, tau_pre, tau_post, dtype, tau_cpre, tau_cpost)
     x = (x + prolong(xc, x.shape)).astype(dtype)
-    for t in tau_post:
+    for t in tq:
         x = (x + t * (b - L.apply(x)) / L.D).astype(dtype)
     return x
 
 
 def solve(counts, spacing, density, rhs, dt, rel, abs_tol, div_limit, max_cycles=100, levels=0,
-          pre=0, post=0, omega=6.0 / 7.0, dtype=np.float64, history=None, smoother="chebyshev"):
+          pre=0, post=0, omega=6.0 / 7.0, dtype=np.float64, history=None, smoother="chebyshev",
+          coarse_pre=0, coarse_post=0):
     """Defect correction with V-cycles; returns (status, p, cycles, l2, rmax)."""
     jac = smoother == "jacobi"
     pre = pre or (2 if jac else 3)
     post = post or (2 if jac else 1)
+    coarse_pre, coarse_post = coarse_pre or pre, coarse_post or post
     tau_pre, tau_post = sweep_weights(pre, smoother, omega), sweep_weights(post, smoother, omega)
+    tau_cpre = sweep_weights(coarse_pre, smoother, omega)
+    tau_cpost = sweep_weights(coarse_post, smoother, omega)
     lv = build_levels(counts, spacing, density, levels)
     shape = (counts[2], counts[1], counts[0])
     b = np.asarray(rhs, np.float64).reshape(shape)
@@ -161,6 +171,6 @@ def solve(counts, spacing, density, rhs, dt, rel, abs_tol, div_limit, max_cycles
             return "ok", p.ravel(), cycle, l2, rmax
         if cycle == max_cycles:
             break
-        p = p + vcycle(lv, 0, r.astype(dtype), tau_pre, tau_post, dtype).astype(np.float64)
+        p = p + vcycle(lv, 0, r.astype(dtype), tau_pre, tau_post, dtype, tau_cpre, tau_cpost).astype(np.float64)
     p = p - p.flat[0]
     return "iteration_limit", p.ravel(), max_cycles, l2, rmax

@@ -281,11 +281,12 @@ class PressureWorkspace:
 
     def __init__(self, spec: GridSpec, method: str = "jacobi-pcg-v1", precision: int = F64,
                  memory_limit: int = 0, mg_levels: int = 0, kernel_variant: int = 0, mg_sweeps=(0, 0),
-                 mg_omega: float = 0.0, mg_smoother: int = 0):
+                 mg_omega: float = 0.0, mg_smoother: int = 0, mg_coarse_sweeps=(0, 0)):
         self.spec = spec
         self.grid = make_grid(spec)
         cfg = N.SolverConfig(METHOD_IDS[method], precision, mg_levels, int(mg_sweeps[0]), int(mg_sweeps[1]),
-                             float(mg_omega), memory_limit, kernel_variant, int(mg_smoother))
+                             float(mg_omega), memory_limit, kernel_variant, int(mg_smoother),
+                             int(mg_coarse_sweeps[0]), int(mg_coarse_sweeps[1]))
         h = C.c_void_p()
         _check(lib().rh_solver_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_solver_create")
         self.handle = h.value
@@ -485,7 +486,7 @@ class Scene:
     def __init__(self, spec: GridSpec, density: float = 1.0, precision: int = F32, method: str = "jacobi-pcg-v1",
                  solver_precision: int | None = None, settings: PressureSettings | None = None,
                  actual_divergence_limit: float = 1e-5, max_courant: float = 0.0, mg_levels: int = 0,
-                 kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0):
+                 kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0, mg_coarse_sweeps=(0, 0)):
         self.spec = spec
         self.precision = precision
         self.grid = make_grid(spec)
@@ -493,7 +494,8 @@ class Scene:
         sp = precision if solver_precision is None else solver_precision
         cfg = N.SceneConfig(density, precision,
                             N.SolverConfig(METHOD_IDS[method], sp, mg_levels, int(mg_sweeps[0]), int(mg_sweeps[1]),
-                                           0.0, 0, kernel_variant, int(mg_smoother)),
+                                           0.0, 0, kernel_variant, int(mg_smoother),
+                                           int(mg_coarse_sweeps[0]), int(mg_coarse_sweeps[1])),
                             settings.native(), actual_divergence_limit, max_courant)
         h = C.c_void_p()
         _check(lib().rh_scene_create(C.byref(self.grid), C.byref(cfg), C.byref(h)), "rh_scene_create")
@@ -573,12 +575,14 @@ class SceneBatch:
 
     def __init__(self, specs, density: float = 1.0, settings: PressureSettings | None = None,
                  actual_divergence_limit: float = 1e-3, max_courant: float = 0.0, max_cycles: int = 0,
-                 use_graph: bool = True, kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0):
+                 use_graph: bool = True, kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0,
+                 mg_coarse_sweeps=(0, 0)):
         self.specs = list(specs)
         settings = settings or PressureSettings(1e-8, 1e-14, 1e-3, 40)
         cfg = N.BatchConfig(
             N.SceneConfig(density, F32, N.SolverConfig(MIXED_MG, F32, 0, int(mg_sweeps[0]), int(mg_sweeps[1]), 0.0,
-                                                       0, kernel_variant, int(mg_smoother)),
+                                                       0, kernel_variant, int(mg_smoother),
+                                                       int(mg_coarse_sweeps[0]), int(mg_coarse_sweeps[1])),
                           settings.native(), actual_divergence_limit, max_courant),
             int(max_cycles), 1 if use_graph else 0)
         grids = (N.Grid * len(self.specs))(*[make_grid(sp) for sp in self.specs])

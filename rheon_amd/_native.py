@@ -56,6 +56,8 @@ class SolverConfig(C.Structure):
         ("memory_limit", C.c_uint64),
         ("kernel_variant", C.c_uint32),
         ("mg_smoother", C.c_uint32),
+        ("mg_coarse_pre_sweeps", C.c_int32),
+        ("mg_coarse_post_sweeps", C.c_int32),
     ]
 
 

@@ -496,9 +496,9 @@ rh_status Batch::build() {
     grids[i] = mg(i)->dev_grid(density);
   }
   RH_HIP(hipMemcpy(d_grids_, grids.data(), sizeof(DevGrid) * n, hipMemcpyHostToDevice));
-  int pre = 0, post = 0;
+  int pre = 0, post = 0, cpre = 0, cpost = 0;
   {
-    rh_status s = MgSolver::plan_sweeps(sc.solver, &pre, &post);
+    rh_status s = MgSolver::plan_sweeps(sc.solver, &pre, &post, &cpre, &cpost);
     if (s) return s;
   }
   int64_t cycles = sc.pressure.max_iterations;
@@ -706,10 +706,10 @@ rh_status Batch::build() {
         const MgHierarchy& h = mg(i)->hierarchy();
         const MgHierarchy* dh = mg(i)->device_hierarchy();
         const bool cx = in_x[i][l];
+        const double* tau = l == 0 ? (post_side ? h.tau_post : h.tau_pre) : (post_side ? h.tau_cpost : h.tau_cpre);
         r[i] = BSmoothArgs{&dh->lv[l], (const float*)(cx ? h.x[l] : h.y[l]), (const float*)h.b[l],
-                           (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr,
-                           post_side ? h.tau_post[sw] : h.tau_pre[sw], scenes_[i]->solver->impl->scratch().ctl,
-                           h.tau_pre[0]};
+                           (float*)(cx ? h.y[l] : h.x[l]), to_p ? scenes_[i]->p : nullptr, tau[sw],
+                           scenes_[i]->solver->impl->scratch().ctl, h.tau_pre[0]};
         const bool q = quad_ok && mg_quad_size_ok(h.lv[l], l) && mg_quad_level(h.lv[l]);
         if (l == 0 && !post_side && sw == 1 && from_b[i])
           bf[i] = qblk(mg_quad_level_blocks(h.lv[l]));  // x0 recomputed from b0
@@ -727,7 +727,7 @@ rh_status Batch::build() {
       stage(Op::kSmoothS, r.data(), sizeof(BSmoothArgs), bs);  // the scalar body updates p itself
     };
     for (int l = 0; l < maxL; ++l) {
-      for (int sw = 1; sw < pre; ++sw) smooth_stage(l, false, false, sw);
+      for (int sw = 1; sw < (l == 0 ? pre : cpre); ++sw) smooth_stage(l, false, false, sw);
       // restriction l -> l+1
       std::vector<BRestrictArgs> r(n);
       std::vector<int> bq(n, 0), bs(n, 0), bqf(n, 0);
@@ -735,14 +735,16 @@ rh_status Batch::build() {
         if (l >= bottom[i]) continue;
         const MgHierarchy& h = mg(i)->hierarchy();
         const MgHierarchy* dh = mg(i)->device_hierarchy();
+        const bool first_in_y = mg_first_in_y(h, l + 1);
         r[i] = BRestrictArgs{&dh->lv[l], &dh->lv[l + 1], (const float*)(in_x[i][l] ? h.x[l] : h.y[l]),
-                             (const float*)h.b[l], (float*)h.b[l + 1], (float*)h.x[l + 1], h.tau_pre[0],
+                             (const float*)h.b[l], (float*)h.b[l + 1],
+                             (float*)(first_in_y ? h.y[l + 1] : h.x[l + 1]), h.tau_cpre[0],
                              scenes_[i]->solver->impl->scratch().ctl};
         if (quad_ok && mg_quad_size_ok(h.lv[l], l) && mg_quad_pair(h.lv[l], h.lv[l + 1]))
           (h.lv[l].frac ? bqf : bq)[i] = qblk(mg_quad_restrict_blocks(h.lv[l + 1]));
         else
           bs[i] = lblk(h.lv[l + 1]);
-        in_x[i][l + 1] = true;
+        in_x[i][l + 1] = !first_in_y;
       }
       stage(Op::kRestrictQ, r.data(), sizeof(BRestrictArgs), bq);
       stage(Op::kRestrictQFrac, r.data(), sizeof(BRestrictArgs), bqf);
@@ -791,8 +793,9 @@ rh_status Batch::build() {
       }
       stage(Op::kProlongQ, r.data(), sizeof(BProlongArgs), bq);
       stage(Op::kProlongS, r.data(), sizeof(BProlongArgs), bs);
-      for (int sw = 0; sw < post; ++sw) {
-        const bool last = sw + 1 == post;
+      const int npost = l == 0 ? post : cpost;
+      for (int sw = 0; sw < npost; ++sw) {
+        const bool last = sw + 1 == npost;
         if (last && l == 0) {
           smooth_stage(0, true, true, sw);
         } else {

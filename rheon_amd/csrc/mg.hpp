@@ -39,8 +39,8 @@ constexpr int kMgMaxSweeps = 8;   // smoothing sweeps per side of the V-cycle
 
 constexpr int kMgDenseMax = 512;  // coarsest levels up to this size use a dense inverse
 // Dynamic LDS bytes for the tail levels' arrays (x, y, b, face fractions,
-// octants); the tail kernel's static LDS (8 KB) comes on top, within the
-// 160 KB a gfx950 workgroup may hold.
+// octants, 1/D); the tail kernel's static LDS (8 KB) comes on top, within
+// the 160 KB a gfx950 workgroup may hold.
 constexpr int kMgTailLds = 120 * 1024;
 
 // Smoothing sweep k of a side is x <- x + tau[k] (b - K x) / D.  Jacobi:
@@ -49,12 +49,18 @@ constexpr int kMgTailLds = 120 * 1024;
 // diagonally dominant operator has its spectrum in [0, 2]), so nu sweeps
 // damp every eigencomponent in that interval by 1 / T_nu((2 + lo) /
 // (2 - lo)); the sweeps commute, so their order is free.
+// The finest level takes (pre, post) sweeps, every coarser level (cpre,
+// cpost); a coarse level's first pre-sweep (x = tau_cpre[0] b / D) is fused
+// into the restriction that produces its right-hand side.
 struct MgHierarchy {
   int levels = 0;
   int pre = 2, post = 2;
+  int cpre = 2, cpost = 2;
   double omega = 6.0 / 7.0;
   double tau_pre[kMgMaxSweeps] = {};
   double tau_post[kMgMaxSweeps] = {};
+  double tau_cpre[kMgMaxSweeps] = {};
+  double tau_cpost[kMgMaxSweeps] = {};
   // Coarsest level <= kMgDenseMax cells: G = M^-1 P with M = K + c 1_a 1_a^T
   // over the active (fluid) cells a and P the projection removing the mean
   // over a; column-major n x n f64 (rows and columns of inactive cells are
@@ -65,14 +71,29 @@ struct MgHierarchy {
   bool quad = true;  // use the four-cells-per-lane kernels where a level allows
   int tail_level = 0;
   int tail_lds = 0;                  // dynamic LDS bytes of the tail launch
-  // LDS byte offsets per tail level of x, y, b, fractions, octants (-1 = none)
-  int tail_off[kMgMaxLevels][5] = {};
+  // LDS byte offsets per tail level of x, y, b, fractions, octants and 1/D
+  // (per cell on fraction levels, the 64-entry table on box levels); -1 =
+  // not staged (1/D is then computed per use)
+  int tail_off[kMgMaxLevels][6] = {};
+  // per-cell 1/D in device memory of the tail fraction levels that stage it
+  // (lv_odiag's values, computed once per hierarchy)
+  const double* tail_odc[kMgMaxLevels] = {};
   const MgHierarchy* d_self = nullptr;
   MgLevel lv[kMgMaxLevels];
   void* x[kMgMaxLevels] = {};  // V-cycle iterate (S)
   void* y[kMgMaxLevels] = {};  // Jacobi ping-pong partner (S)
   void* b[kMgMaxLevels] = {};  // level right-hand side (S)
 };
+
+// A coarse level l (above the bottom: the tail level, else the coarsest)
+// must end its V-cycle with its result in x[l] for the parent's
+// prolongation.  Its iterate moves between x and y once per sweep after the
+// first and once in the prolongation, (cpre - 1) + 1 + cpost moves in all;
+// when that count is odd the restriction stores the first sweep in y[l].
+inline bool mg_first_in_y(const MgHierarchy& h, int l) {
+  const int bottom = h.tail_level > 0 ? h.tail_level : h.levels - 1;
+  return l >= 1 && l < bottom && ((h.cpre + h.cpost) & 1) != 0;
+}
 
 // Four-cells-per-lane FP32 kernels (levels with nx % 4 == 0 and no face
 // fractions); identical values to the scalar kernels.

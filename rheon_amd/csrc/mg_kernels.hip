@@ -290,38 +290,83 @@ kb_mg_coarsest_dense(const BCoarseArgs* __restrict__ A, const int* __restrict__ 
 // instruction cache, several microseconds per phase).
 constexpr int kTailThreads = 1024;
 
+// Cells t, t + T, t + 2T, ... (T = kTailThreads) of an nx x ny level with
+// (i, j, k) carried from cell to cell: the tail runs on one compute unit, so
+// its phases are bound by instruction issue, and two runtime integer
+// divisions per cell cost more than the stencil itself.
+struct TailWalk {
+  int c, i, j, k, di, dj, dk;
+  __device__ __forceinline__ TailWalk(int nx, int ny) {
+    c = threadIdx.x;
+    const int row = c / nx;
+    i = c - row * nx;
+    k = row / ny;
+    j = row - k * ny;
+    const int r = kTailThreads / nx;
+    di = kTailThreads - r * nx;
+    dk = r / ny;
+    dj = r - dk * ny;
+  }
+  // c += T: i + di < 2 nx and j + carry + dj < 2 ny, so one wrap each
+  __device__ __forceinline__ void next(int nx, int ny) {
+    c += kTailThreads;
+    i += di;
+    if (i >= nx) {
+      i -= nx;
+      ++j;
+    }
+    j += dj;
+    if (j >= ny) {
+      j -= ny;
+      ++k;
+    }
+    k += dk;
+  }
+};
+
+// 1 / D of a tail cell: the per-cell values staged in LDS (levels with
+// fractions), the level's 64-entry table staged in LDS (box levels), or
+// computed as in the level kernels when neither fitted.
+__device__ __forceinline__ double tail_odiag(const MgLevel& L, const unsigned* fr, const double* od,
+                                             const TailWalk& w) {
+  if (od) return fr ? od[w.c] : od[box_mask6(L.nx, L.ny, L.nz, w.i, w.j, w.k)];
+  return lv_odiag_fr(L, fr, w.c, w.i, w.j, w.k);
+}
+
 template <typename S>
-__device__ __forceinline__ void tail_smooth(const MgLevel& L, const unsigned* fr, const S* xin, const S* b,
-                                         S* xout, double tau) {
+__device__ __forceinline__ void tail_smooth(const MgLevel& L, const unsigned* fr, const double* od, const S* xin,
+                                         const S* b, S* xout, double tau) {
   const int n = L.nx * L.ny * L.nz;
-  for (int c = threadIdx.x; c < n; c += kTailThreads) {
-    const int row = c / L.nx, i = c - row * L.nx;
-    const int k = row / L.ny, j = row - k * L.ny;
-    const S ax = s_apply_fr<S>(L, fr, xin, c, i, j, k);
-    xout[c] = xin[c] + (S)(tau * lv_odiag_fr(L, fr, c, i, j, k)) * (b[c] - ax);
+  for (TailWalk w(L.nx, L.ny); w.c < n; w.next(L.nx, L.ny)) {
+    const S ax = s_apply_fr<S>(L, fr, xin, w.c, w.i, w.j, w.k);
+    xout[w.c] = xin[w.c] + (S)(tau * tail_odiag(L, fr, od, w)) * (b[w.c] - ax);
   }
   __syncthreads();
 }
 
+// Restriction in two passes: the fine residuals b - K x into r (every
+// thread busy), then the eight-children sums in (dz, dy, dx) order (the
+// same values and order as k_mg_restrict).
 template <typename S>
 __device__ __forceinline__ void tail_restrict(const MgLevel& F, const unsigned* ffr, const MgLevel& Cl,
-                                           const unsigned* cfr, const S* x, const S* b, S* bc, S* xc,
-                                           double tau) {
+                                           const unsigned* cfr, const double* cod, const S* x, const S* b,
+                                           S* r, S* bc, S* xc, double tau) {
+  const int nf = F.nx * F.ny * F.nz;
+  for (TailWalk w(F.nx, F.ny); w.c < nf; w.next(F.nx, F.ny))
+    r[w.c] = b[w.c] - s_apply_fr<S>(F, ffr, x, w.c, w.i, w.j, w.k);
+  __syncthreads();
   const int n = Cl.nx * Cl.ny * Cl.nz;
-  for (int c = threadIdx.x; c < n; c += kTailThreads) {
-    const int row = c / Cl.nx, i = c - row * Cl.nx;
-    const int k = row / Cl.ny, j = row - k * Cl.ny;
+  for (TailWalk w(Cl.nx, Cl.ny); w.c < n; w.next(Cl.nx, Cl.ny)) {
     S sum = (S)0;
     for (int dz = 0; dz < 2; ++dz)
       for (int dy = 0; dy < 2; ++dy)
         for (int dx = 0; dx < 2; ++dx) {
-          const int fi = 2 * i + dx, fj = 2 * j + dy, fk = 2 * k + dz;
+          const int fi = 2 * w.i + dx, fj = 2 * w.j + dy, fk = 2 * w.k + dz;
           if (fi >= F.nx || fj >= F.ny || fk >= F.nz) continue;  // padded coarse cell
-          const int fc = fi + F.nx * (fj + F.ny * fk);
-          sum += b[fc] - s_apply_fr<S>(F, ffr, x, fc, fi, fj, fk);
+          sum += r[fi + F.nx * (fj + F.ny * fk)];
         }
-    bc[c] = sum;
-    xc[c] = (S)(tau * lv_odiag_fr(Cl, cfr, c, i, j, k)) * sum;
+    bc[w.c] = sum;
+    xc[w.c] = (S)(tau * tail_odiag(Cl, cfr, cod, w)) * sum;
   }
   __syncthreads();
 }
@@ -333,11 +378,9 @@ __device__ __forceinline__ void tail_prolong(const MgLevel& F, const unsigned* f
                                           const unsigned* cfr, const unsigned char* coct, const S* xc,
                                           const S* xin, S* xout) {
   const int n = F.nx * F.ny * F.nz;
-  for (int c = threadIdx.x; c < n; c += kTailThreads) {
-    const int row = c / F.nx, i = c - row * F.nx;
-    const int k = row / F.ny, j = row - k * F.ny;
-    const bool fine_fluid = ffr ? (ffr[c] & kFracFluid) != 0 : lv_fine_fluid(F, c);
-    xout[c] = xin[c] + prolong_value_fr<S>(Cl, cfr, coct, xc, i, j, k, fine_fluid);
+  for (TailWalk w(F.nx, F.ny); w.c < n; w.next(F.nx, F.ny)) {
+    const bool fine_fluid = ffr ? (ffr[w.c] & kFracFluid) != 0 : lv_fine_fluid(F, w.c);
+    xout[w.c] = xin[w.c] + prolong_value_fr<S>(Cl, cfr, coct, xc, w.i, w.j, w.k, fine_fluid);
   }
   __syncthreads();
 }
@@ -347,6 +390,59 @@ __device__ __forceinline__ void tail_copy(int n, const S* src, S* dst) {
   for (int c = threadIdx.x; c < n; c += kTailThreads) dst[c] = src[c];
   __syncthreads();
 }
+
+// Dense coarsest solve x = G b inside the tail (dense_solve's partition and
+// summation order); up to kTailDenseRegs entries of G per thread are loaded
+// into registers when the tail starts, so their latency overlaps the
+// staging instead of stalling the solve.
+constexpr int kTailDenseRegs = 8;
+struct TailDense {
+  int n, c, part, parts, j0, j1;
+  bool pre;  // every thread's share fits kTailDenseRegs
+  double g[kTailDenseRegs];
+  __device__ __forceinline__ void load(int cells, const double* __restrict__ G) {
+    n = cells;
+    parts = n > 0 && n <= kDenseRed ? kDenseRed / n : 1;
+    pre = parts > 1 && n <= kDenseRed / kTailDenseRegs;
+    const int t = threadIdx.x;
+    c = pre ? t % n : 0;
+    part = pre ? t / n : parts;
+    j0 = (int)((long long)part * n / parts);
+    j1 = (int)((long long)(part + 1) * n / parts);
+#pragma unroll
+    for (int q = 0; q < kTailDenseRegs; ++q)
+      g[q] = (pre && part < parts && j0 + q < j1) ? G[(size_t)(j0 + q) * n + c] : 0.0;
+  }
+};
+
+template <typename S>
+__device__ __forceinline__ void tail_dense(const TailDense& D, const double* __restrict__ G, const S* b, S* x,
+                                        double* bs, double* red) {
+  if (!D.pre) {
+    dense_solve<kTailThreads, S>(D.n, G, b, x, bs, red);
+    return;
+  }
+  for (int c = threadIdx.x; c < D.n; c += kTailThreads) bs[c] = (double)b[c];
+  __syncthreads();
+  if (D.part < D.parts) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < kTailDenseRegs; ++q)
+      if (D.j0 + q < D.j1) acc += D.g[q] * bs[D.j0 + q];
+    red[D.part * D.n + D.c] = acc;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < D.n) {
+    double sum = 0.0;
+    for (int q = 0; q < D.parts; ++q) sum += red[q * D.n + t];
+    x[t] = (S)sum;
+  }
+  __syncthreads();
+}
+
+// Lower tail levels staged in one pass (their loads in flight together).
+constexpr int kTailStageLower = 4;
 
 template <typename S>
 __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int l0,
@@ -362,24 +458,79 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
   auto B = [&](int l) { return (S*)at(l, 2); };
   auto FR = [&](int l) { return (const unsigned*)at(l, 3); };
   auto OC = [&](int l) { return (const unsigned char*)at(l, 4); };
+  auto OD = [&](int l) { return (const double*)at(l, 5); };
   auto cells = [&](int l) { return H->lv[l].nx * H->lv[l].ny * H->lv[l].nz; };
-  // stage level l0's x and b, and the fractions and octants of every tail
-  // level, from global memory
+  TailDense dense;
+  dense.load(cells(levels - 1), H->coarse_inv);
+  // Stage from global memory: level l0's b and x (the parent's restriction
+  // stored the first pre-smoothing sweep there), every level's fractions,
+  // octants and per-cell 1/D, and the 64-entry 1/D tables of box levels.
   {
+    const int n0 = cells(l0);
     const S* gb = (const S*)H->b[l0];
     const S* gx = (const S*)H->x[l0];
-    const int n0 = cells(l0);
-    for (int c = threadIdx.x; c < n0; c += kTailThreads) {
-      B(l0)[c] = gb[c];
-      X(l0)[c] = gx[c];
+    const unsigned* gf = H->tail_off[l0][3] >= 0 ? H->lv[l0].frac : nullptr;
+    const double* go = H->tail_off[l0][5] >= 0 ? H->tail_odc[l0] : nullptr;
+    for (int c0 = threadIdx.x; c0 < n0; c0 += 4 * kTailThreads) {
+      S vb[4], vx[4];
+      unsigned vf[4];
+      double vo[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + u * kTailThreads;
+        if (c < n0) {
+          vb[u] = gb[c];
+          vx[u] = gx[c];
+          vf[u] = gf ? gf[c] : 0u;
+          vo[u] = go ? go[c] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = c0 + u * kTailThreads;
+        if (c < n0) {
+          B(l0)[c] = vb[u];
+          X(l0)[c] = vx[u];
+          if (gf) ((unsigned*)at(l0, 3))[c] = vf[u];
+          if (go) ((double*)at(l0, 5))[c] = vo[u];
+        }
+      }
     }
-    for (int l = l0; l < levels; ++l) {
+    const int n1 = l0 + 1 < levels ? cells(l0 + 1) : 0;
+    for (int c = threadIdx.x; c < n1; c += kTailThreads) {
+      unsigned vf[kTailStageLower];
+      unsigned char vc[kTailStageLower];
+      double vo[kTailStageLower];
+#pragma unroll
+      for (int q = 0; q < kTailStageLower; ++q) {
+        const int l = l0 + 1 + q;
+        const bool in = l < levels && c < cells(l);
+        vf[q] = in && H->tail_off[l][3] >= 0 ? H->lv[l].frac[c] : 0u;
+        vc[q] = in && H->tail_off[l][4] >= 0 ? H->lv[l].octant[c] : (unsigned char)0;
+        vo[q] = in && H->tail_odc[l] ? H->tail_odc[l][c] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kTailStageLower; ++q) {
+        const int l = l0 + 1 + q;
+        if (l < levels && c < cells(l)) {
+          if (H->tail_off[l][3] >= 0) ((unsigned*)at(l, 3))[c] = vf[q];
+          if (H->tail_off[l][4] >= 0) ((unsigned char*)at(l, 4))[c] = vc[q];
+          if (H->tail_odc[l]) ((double*)at(l, 5))[c] = vo[q];
+        }
+      }
+    }
+    for (int l = l0 + 1 + kTailStageLower; l < levels; ++l) {
       const int n = cells(l);
       if (H->tail_off[l][3] >= 0)
         for (int c = threadIdx.x; c < n; c += kTailThreads) ((unsigned*)at(l, 3))[c] = H->lv[l].frac[c];
       if (H->tail_off[l][4] >= 0)
         for (int c = threadIdx.x; c < n; c += kTailThreads) ((unsigned char*)at(l, 4))[c] = H->lv[l].octant[c];
+      if (H->tail_odc[l])
+        for (int c = threadIdx.x; c < n; c += kTailThreads) ((double*)at(l, 5))[c] = H->tail_odc[l][c];
     }
+    if (threadIdx.x < 64)
+      for (int l = l0; l < levels; ++l)
+        if (H->tail_off[l][5] >= 0 && !H->lv[l].frac) ((double*)at(l, 5))[threadIdx.x] = H->lv[l].odiag[threadIdx.x];
     __syncthreads();
   }
   // Every tail level's iterate starts in X(l); bit l of `in_y` says it has
@@ -398,9 +549,9 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
     enum { kSmooth, kRestrict, kDense, kProlong, kCopy, kDone } op;
     double tau = 0.0;
     if (down) {
-      if (step < H->pre - 1) {
+      if (step < H->cpre - 1) {
         op = kSmooth;
-        tau = H->tau_pre[step + 1];
+        tau = H->tau_cpre[step + 1];
       } else {
         op = kRestrict;
       }
@@ -408,9 +559,9 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
       op = kDense;
     } else if (step == 0) {
       op = kProlong;
-    } else if (step <= H->post) {
+    } else if (step <= H->cpost) {
       op = kSmooth;
-      tau = H->tau_post[step - 1];
+      tau = H->tau_cpost[step - 1];
     } else {
       op = (l > l0 && ((in_y >> l) & 1u)) ? kCopy : kDone;
     }
@@ -418,13 +569,15 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
     const MgLevel& L = H->lv[l];
     switch (op) {
       case kSmooth:
-        tail_smooth<S>(L, FR(l), cur(l), B(l), nxt(l), tau);
+        tail_smooth<S>(L, FR(l), OD(l), cur(l), B(l), nxt(l), tau);
         break;
       case kRestrict:
-        tail_restrict<S>(L, FR(l), H->lv[l + 1], FR(l + 1), cur(l), B(l), B(l + 1), X(l + 1), H->tau_pre[0]);
+        // nxt(l) is free until the prolongation: it holds the residuals
+        tail_restrict<S>(L, FR(l), H->lv[l + 1], FR(l + 1), OD(l + 1), cur(l), B(l), nxt(l), B(l + 1),
+                         X(l + 1), H->tau_cpre[0]);
         break;
       case kDense:
-        dense_solve<kTailThreads, S>(cells(l), H->coarse_inv, B(l), X(l), bs, red);
+        tail_dense<S>(dense, H->coarse_inv, B(l), X(l), bs, red);
         break;
       case kProlong:
         tail_prolong<S>(L, FR(l), H->lv[l + 1], FR(l + 1), OC(l + 1), X(l + 1), cur(l), nxt(l));
@@ -641,27 +794,31 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
     return;
   }
   // pre-smoothing: sweeps 2..nu1 (ping-pong x <-> y), result in `cur`
-  S* cur = x;
-  S* nxt = y;
-  for (int sweep = 1; sweep < h.pre; ++sweep) {
+  S* cur = mg_first_in_y(h, l) ? y : x;
+  S* nxt = cur == x ? y : x;
+  const int pre = l == 0 ? h.pre : h.cpre, post = l == 0 ? h.post : h.cpost;
+  const double* tau_pre = l == 0 ? h.tau_pre : h.tau_cpre;
+  const double* tau_post = l == 0 ? h.tau_post : h.tau_cpost;
+  for (int sweep = 1; sweep < pre; ++sweep) {
     if (sweep == 1 && l == 0 && x0_implicit) {
       prof::Scope ps(prof::kMgSmoothFirst, s, it);
       launch_mg_smooth_first_q(h.lv[0], (const float*)b, (float*)nxt, h.tau_pre[0], h.tau_pre[1], ctl, s);
     } else {
-      smooth<S>(h, l, cur, nxt, nullptr, h.tau_pre[sweep], ctl, it, s);
+      smooth<S>(h, l, cur, nxt, nullptr, tau_pre[sweep], ctl, it, s);
     }
     S* t = cur;
     cur = nxt;
     nxt = t;
   }
   {
+    S* first = (S*)(mg_first_in_y(h, l + 1) ? h.y[l + 1] : h.x[l + 1]);  // level l + 1's first sweep
     prof::Scope ps(mg_slot(l, prof::kMgRestrict), s, l == 0 ? it : -2);
     if (sizeof(S) == 4 && h.quad && mg_quad_size_ok(h.lv[l], l) && mg_quad_pair(h.lv[l], h.lv[l + 1]))
       launch_mg_restrict_q(h.lv[l], h.lv[l + 1], (const float*)cur, (const float*)b, (float*)h.b[l + 1],
-                           (float*)h.x[l + 1], h.tau_pre[0], ctl, s);
+                           (float*)first, h.tau_cpre[0], ctl, s);
     else
       launch(k_mg_restrict<S>, level_grid(h.lv[l + 1]), block_shape(), 0, s,
-          h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], (S*)h.x[l + 1], h.tau_pre[0], ctl);
+          h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], first, h.tau_cpre[0], ctl);
   }
   vcycle<S>(h, l + 1, p, ctl, it, s, false);
   {
@@ -678,12 +835,12 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
     cur = nxt;
     nxt = t;
   }
-  for (int sweep = 0; sweep < h.post; ++sweep) {
-    const bool last = sweep + 1 == h.post;
+  for (int sweep = 0; sweep < post; ++sweep) {
+    const bool last = sweep + 1 == post;
     if (last && l == 0) {
-      smooth<S>(h, l, cur, nxt, p, h.tau_post[sweep], ctl, it, s);  // p += x (finest, last sweep)
+      smooth<S>(h, l, cur, nxt, p, tau_post[sweep], ctl, it, s);  // p += x (finest, last sweep)
     } else {
-      smooth<S>(h, l, cur, nxt, nullptr, h.tau_post[sweep], ctl, it, s);
+      smooth<S>(h, l, cur, nxt, nullptr, tau_post[sweep], ctl, it, s);
       S* t = cur;
       cur = nxt;
       nxt = t;

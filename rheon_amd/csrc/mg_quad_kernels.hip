@@ -251,7 +251,6 @@ b_mg_smooth_qf(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const
                float* __restrict__ xout, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
   const int nz_ = L.nz;
-  const int sy = L.nx, sz = L.nx * L.ny;
   FOR_QUADS(L.nx, L.ny)
   Quad<float> Q;
   load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);

@@ -130,7 +130,9 @@ class MgSolver : public Solver {
   int levels() const { return h_.levels; }
   static int plan_levels(const rh_grid& grid, const rh_solver_config& config, int dims[][3], int* padded);
   static uint64_t plan_bytes(const rh_grid& grid, const rh_solver_config& config);
-  static rh_status plan_sweeps(const rh_solver_config& config, int* pre, int* post);
+  // Sweeps per side of the finest level (pre, post) and of the coarse
+  // levels (cpre, cpost).
+  static rh_status plan_sweeps(const rh_solver_config& config, int* pre, int* post, int* cpre, int* cpost);
   rh_status set_solid(const uint8_t* solid) override;
   // Level data and coarsest inverse for `density` on the device (done by
   // every solve; batched runners call it before building launch tables).
@@ -153,6 +155,8 @@ class MgSolver : public Solver {
   void* arena_ = nullptr;
   double* d_inv_ = nullptr;   // dense coarsest inverse (device)
   size_t inv_cap_ = 0;        // doubles allocated at d_inv_
+  double* d_odc_ = nullptr;   // per-cell 1/D of the tail's fraction levels (device)
+  size_t odc_cap_ = 0;        // doubles allocated at d_odc_
   MgHierarchy* d_h_ = nullptr;  // device copy of h_ read by the tail kernel
   double inv_density_ = 0.0;  // density the inverse was built for
   uint64_t bytes_ = 0;

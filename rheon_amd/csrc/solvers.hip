@@ -235,13 +235,18 @@ rh_status PcgSolver::solve(double density, const double* d_rhs, double dt,
 // -------------------------------------------------------------- multigrid
 
 // Sweep counts of the V-cycle: explicit, or the smoother's default.
-rh_status MgSolver::plan_sweeps(const rh_solver_config& config, int* pre, int* post) {
+rh_status MgSolver::plan_sweeps(const rh_solver_config& config, int* pre, int* post, int* cpre, int* cpost) {
   if (config.mg_smoother != RH_SMOOTHER_CHEBYSHEV && config.mg_smoother != RH_SMOOTHER_JACOBI)
     return RH_INVALID_ARGUMENT;
   const bool jacobi = config.mg_smoother == RH_SMOOTHER_JACOBI;
   *pre = config.mg_pre_sweeps > 0 ? config.mg_pre_sweeps : (jacobi ? 2 : 3);
   *post = config.mg_post_sweeps > 0 ? config.mg_post_sweeps : (jacobi ? 2 : 1);
-  if (*pre > kMgMaxSweeps || *post > kMgMaxSweeps || config.mg_omega < 0.0) return RH_INVALID_ARGUMENT;
+  *cpre = config.mg_coarse_pre_sweeps > 0 ? config.mg_coarse_pre_sweeps : *pre;
+  *cpost = config.mg_coarse_post_sweeps > 0 ? config.mg_coarse_post_sweeps : *post;
+  if (*pre > kMgMaxSweeps || *post > kMgMaxSweeps || *cpre > kMgMaxSweeps || *cpost > kMgMaxSweeps ||
+      config.mg_pre_sweeps < 0 || config.mg_post_sweeps < 0 || config.mg_coarse_pre_sweeps < 0 ||
+      config.mg_coarse_post_sweeps < 0 || config.mg_omega < 0.0)
+    return RH_INVALID_ARGUMENT;
   return RH_OK;
 }
 
@@ -307,7 +312,8 @@ uint64_t MgSolver::plan_bytes(const rh_grid& grid, const rh_solver_config& confi
     if (l == 0) total += cells;                                                // obstacle mask
   }
   const uint64_t coarse = (uint64_t)dims[levels - 1][0] * dims[levels - 1][1] * dims[levels - 1][2];
-  total += coarse * coarse * 8 + sizeof(MgHierarchy) + Scratch::kDeviceBytes;  // dense inverse, tables, scratch
+  // dense inverse, tables, the tail's per-cell 1/D (at most its LDS), scratch
+  total += coarse * coarse * 8 + sizeof(MgHierarchy) + kMgTailLds + Scratch::kDeviceBytes;
   return total;
 }
 
@@ -318,11 +324,16 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
   if (config.precision != RH_F64 && config.precision != RH_F32) return RH_INVALID_ARGUMENT;
   rh_status st = scratch_.init();
   if (st) return st;
-  st = plan_sweeps(config, &h_.pre, &h_.post);
+  st = plan_sweeps(config, &h_.pre, &h_.post, &h_.cpre, &h_.cpost);
   if (st) return st;
   h_.omega = config.mg_omega > 0.0 ? config.mg_omega : 6.0 / 7.0;
-  sweep_weights(h_.pre, config.mg_smoother == RH_SMOOTHER_JACOBI, h_.omega, h_.tau_pre);
-  sweep_weights(h_.post, config.mg_smoother == RH_SMOOTHER_JACOBI, h_.omega, h_.tau_post);
+  {
+    const bool jacobi = config.mg_smoother == RH_SMOOTHER_JACOBI;
+    sweep_weights(h_.pre, jacobi, h_.omega, h_.tau_pre);
+    sweep_weights(h_.post, jacobi, h_.omega, h_.tau_post);
+    sweep_weights(h_.cpre, jacobi, h_.omega, h_.tau_cpre);
+    sweep_weights(h_.cpost, jacobi, h_.omega, h_.tau_cpost);
+  }
   h_.quad = (config.kernel_variant & RH_VARIANT_SCALAR_MG) == 0;
   // Coarsen while every axis is even and the level is above the one-
   // workgroup coarsest size (or up to the requested level count).
@@ -365,6 +376,7 @@ MgSolver::~MgSolver() {
   if (arena_) (void)hipFree(arena_);
   if (d_frac_) (void)hipFree(d_frac_);
   if (d_inv_) (void)hipFree(d_inv_);
+  if (d_odc_) (void)hipFree(d_odc_);
   if (d_h_) (void)hipFree(d_h_);
   scratch_.release();
 }
@@ -654,40 +666,90 @@ rh_status MgSolver::set_density(double density) {
     RH_HIP(hipMemcpy(d_inv_, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice));
     h_.coarse_inv = d_inv_;
   }
-  // Tail: the first level below the finest that fits kMgTailCells (the tail
-  // needs the dense coarsest inverse).
+  // Tail: the first level below the finest whose x, y, b, fractions and
+  // octants fit the tail's LDS (the tail needs the dense coarsest inverse);
+  // 1/D is staged as well where the remaining room allows (box tables
+  // first, then per-cell values from the largest level down).
   h_.tail_level = 0;
   h_.tail_lds = 0;
+  for (int l = 0; l < kMgMaxLevels; ++l) h_.tail_odc[l] = nullptr;
   if (h_.coarse_inv) {
     const size_t s = config_.precision == RH_F64 ? 8 : 4;
+    const size_t limit = (size_t)mg_tail_lds_limit();
     auto a16 = [](size_t b) { return (b + 15) / 16 * 16; };
     // per tail level: x, y, b; fractions if the level has them; octants
     // where the level is the coarse side of a tail prolongation
-    auto sizes = [&](int l, int m, size_t q[5]) {
+    auto sizes = [&](int l, int m, size_t q[6]) {
       const size_t n = (size_t)h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz;
       q[0] = q[1] = q[2] = a16(s * n);
       q[3] = h_.lv[m].frac ? a16(4 * n) : 0;
       q[4] = (m > l && h_.lv[m].octant) ? a16(n) : 0;
+      q[5] = 0;
     };
     for (int l = 1; l < h_.levels; ++l) {
+      size_t q[kMgMaxLevels][6];
       size_t bytes = 0;
       for (int m = l; m < h_.levels; ++m) {
-        size_t q[5];
-        sizes(l, m, q);
-        bytes += q[0] + q[1] + q[2] + q[3] + q[4];
+        sizes(l, m, q[m]);
+        for (int k = 0; k < 5; ++k) bytes += q[m][k];
       }
-      if (bytes > (size_t)mg_tail_lds_limit()) continue;
-      size_t off = 0;
+      if (bytes > limit) continue;
+      for (int m = l; m < h_.levels; ++m)
+        if (!h_.lv[m].frac && bytes + 64 * 8 <= limit) {
+          q[m][5] = 64 * 8;
+          bytes += q[m][5];
+        }
       for (int m = l; m < h_.levels; ++m) {
-        size_t q[5];
-        sizes(l, m, q);
-        for (int k = 0; k < 5; ++k) {
-          h_.tail_off[m][k] = q[k] ? (int)off : -1;
-          off += q[k];
+        const size_t n = (size_t)h_.lv[m].nx * h_.lv[m].ny * h_.lv[m].nz;
+        if (h_.lv[m].frac && bytes + a16(8 * n) <= limit) {
+          q[m][5] = a16(8 * n);
+          bytes += q[m][5];
         }
       }
+      size_t off = 0;
+      for (int m = l; m < h_.levels; ++m)
+        for (int k = 0; k < 6; ++k) {
+          h_.tail_off[m][k] = q[m][k] ? (int)off : -1;
+          off += q[m][k];
+        }
       h_.tail_level = l;
       h_.tail_lds = (int)bytes;
+      // per-cell 1/D of the staged fraction levels: lv_odiag's fraction
+      // form (IEEE products, sum and quotient: the device's values)
+      std::vector<double> od;
+      size_t at[kMgMaxLevels] = {};
+      for (int m = l; m < h_.levels; ++m) {
+        if (!h_.lv[m].frac || !q[m][5]) continue;
+        const MgLevel& L = h_.lv[m];
+        const std::vector<unsigned>& fr = hfrac_[m];
+        at[m] = od.size();
+        for (int k = 0; k < L.nz; ++k)
+          for (int j = 0; j < L.ny; ++j)
+            for (int i = 0; i < L.nx; ++i) {
+              const size_t c = i + (size_t)L.nx * (j + (size_t)L.ny * k);
+              const unsigned w = fr[c];
+              const unsigned xp = i + 1 < L.nx ? fr[c + 1] & 1023u : 0u;
+              const unsigned yp = j + 1 < L.ny ? (fr[c + L.nx] >> 10) & 1023u : 0u;
+              const unsigned zp = k + 1 < L.nz ? (fr[c + (size_t)L.nx * L.ny] >> 20) & 1023u : 0u;
+              const int sx = (int)(w & 1023u) + (int)xp, sy = (int)((w >> 10) & 1023u) + (int)yp,
+                        sz = (int)((w >> 20) & 1023u) + (int)zp;
+              const double d = L.wf[0] * (double)sx + L.wf[1] * (double)sy + L.wf[2] * (double)sz;
+              od.push_back(d > 0.0 ? 1.0 / d : 0.0);
+            }
+      }
+      if (!od.empty()) {
+        if (d_odc_ && odc_cap_ < od.size()) {
+          (void)hipFree(d_odc_);
+          d_odc_ = nullptr;
+        }
+        if (!d_odc_) {
+          RH_HIP(hipMalloc(&d_odc_, sizeof(double) * od.size()));
+          odc_cap_ = od.size();
+        }
+        RH_HIP(hipMemcpy(d_odc_, od.data(), sizeof(double) * od.size(), hipMemcpyHostToDevice));
+        for (int m = l; m < h_.levels; ++m)
+          if (h_.lv[m].frac && q[m][5]) h_.tail_odc[m] = d_odc_ + at[m];
+      }
       break;
     }
   }

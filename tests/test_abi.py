@@ -19,7 +19,7 @@ def test_library_exports_every_header_symbol():
     assert len(declared) >= 30
     assert [s for s in declared if not hasattr(lib, s)] == []
     assert set(declared) == set(N.SIGNATURES)
-    assert lib.rh_abi_version() == 1
+    assert lib.rh_abi_version() == 2
 
 
 @pytest.mark.parametrize("counts,spacing,origin,ok", [

@@ -17,18 +17,20 @@ def specs():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_batch_matches_single_scenes_bitwise(use_graph):
+@pytest.mark.parametrize("use_graph,coarse", [(False, (0, 0)), (True, (0, 0)), (True, (6, 3))])
+def test_batch_matches_single_scenes_bitwise(use_graph, coarse):
     import rheon_amd as R
     sp = specs()
     st = R.PressureSettings(1e-8, 1e-14, 1e-3, 30)
-    batch = R.SceneBatch(sp, settings=st, use_graph=use_graph, actual_divergence_limit=1e-3)
+    batch = R.SceneBatch(sp, settings=st, use_graph=use_graph, actual_divergence_limit=1e-3,
+                         mg_coarse_sweeps=coarse)
     singles, dts = [], []
     solid = centred_sphere(sp[3], 0.2)
     for i, s in enumerate(sp):
         vel = taylor_green_noise(s, seed=100 + i)
         dts.append(courant_dt(s, vel, 0.5))
-        sc = R.Scene(s, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e-3)
+        sc = R.Scene(s, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e-3,
+                     mg_coarse_sweeps=coarse)
         if i == 3:
             sc.set_solid(solid)
             batch.scene(i).set_solid(solid)
