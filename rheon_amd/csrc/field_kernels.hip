@@ -96,7 +96,13 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
     hi[d] = lo[d] + 1 < dims[d] - 1 ? lo[d] + 1 : dims[d] - 1;
     t[d] = q - (double)lo[d];
   }
+  // corner (x, y, z) sits at c0 + {0, ox} + {0, oy} + {0, oz}: two
+  // integer multiplies per sample instead of four (hi[d] - lo[d] is 0 or 1)
   const int sy = dims[0], sz = dims[0] * dims[1];
+  const int c0 = (lo[2] * dims[1] + lo[1]) * dims[0] + lo[0];
+  const int ox = hi[0] - lo[0];
+  const int oy = hi[1] != lo[1] ? sy : 0;
+  const int oz = hi[2] != lo[2] ? sz : 0;
   double vals8[8];
 #pragma unroll
   for (int z = 0; z < 2; ++z)
@@ -104,8 +110,7 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int x = 0; x < 2; ++x)
-        vals8[x + 2 * y + 4 * z] =
-            (double)vals[(x ? hi[0] : lo[0]) + sy * (y ? hi[1] : lo[1]) + sz * (z ? hi[2] : lo[2])];
+        vals8[x + 2 * y + 4 * z] = (double)vals[c0 + (x ? ox : 0) + (y ? oy : 0) + (z ? oz : 0)];
   double result = 0.0, minimum = INFINITY, maximum = -INFINITY;
 #pragma unroll
   for (int z = 0; z < 2; ++z)
