@@ -47,18 +47,6 @@ __device__ __forceinline__ void load4(const double* __restrict__ p, int c, doubl
   v[2] = b.x;
   v[3] = b.y;
 }
-// Nontemporal load: b of the outer residual is read once per cycle, and
-// keeping it out of the caches leaves them to the arrays the next kernels
-// read again (measured: 0.5% off the 256^3 step).
-__device__ __forceinline__ void load4_stream(const double* __restrict__ p, int c, double v[4]) {
-  typedef double d2v __attribute__((ext_vector_type(2)));
-  const d2v a = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p + c));
-  const d2v b = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p + c + 2));
-  v[0] = a.x;
-  v[1] = a.y;
-  v[2] = b.x;
-  v[3] = b.y;
-}
 __device__ __forceinline__ void store4(float* __restrict__ p, int c, const float v[4]) {
   *reinterpret_cast<float4*>(p + c) = make_float4(v[0], v[1], v[2], v[3]);
 }
@@ -150,7 +138,7 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   Quad<double> Q;
   load_quad(p, g.nx, g.ny, g.nz, i0, j, k, c, Q);
   double bb[4];
-  load4_stream(b, c, bb);
+  load4(b, c, bb);
   const unsigned m4 = g.mask ? *reinterpret_cast<const unsigned*>(g.mask + c) : 0u;
   float rb[4], rx[4];
 #pragma unroll
