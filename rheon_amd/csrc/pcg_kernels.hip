@@ -116,6 +116,44 @@ struct FinReplace {
   __device__ void operator()(const double* sum, const double*) const { ctl->rz[(it + 1) & 1] = sum[0]; }
 };
 
+// Jacobi PCG, iterations >= 1, with the replacement folded into the next
+// iteration's apply (the launch sequence is apply -> finish -> update ->
+// finish -> direction + true residual -> finish): when the previous
+// iteration was replaced, its new rz (partial r.z published beside d.q) is
+// stored and breakdown-checked for that iteration first, as the separate
+// replace, finish and direction kernels did (src/pressure.rs:705-719).
+struct FinApplyDirJ {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done; }
+  __device__ void operator()(const double* sum, const double* mx) const {
+    if (ctl->replaced) {
+      const double next_rz = sum[1];
+      ctl->rz[it & 1] = next_rz;
+      if (!isfinite(next_rz) || next_rz <= 0.0) return fail(ctl, RH_BREAKDOWN, it - 1);
+      ctl->iteration = it - 1;
+    }
+    FinApplyDir{ctl, it}(sum, mx);
+  }
+};
+
+// FinTrue for the folded sequence: the direction kernel ran before this
+// decision, so an iteration that is neither accepted, replaced nor at the
+// limit takes the next-rz breakdown check here (replaced ones take it in
+// FinApplyDirJ on their replacement rz).
+struct FinTrueJ {
+  PcgControl* ctl;
+  int it;
+  __device__ bool active() const { return !ctl->done && ctl->candidate; }
+  __device__ void operator()(const double* sum, const double* mx) const {
+    FinTrue{ctl, it}(sum, mx);
+    if (ctl->done || ctl->replaced) return;
+    const double next_rz = ctl->rz[(it + 1) & 1];
+    if (!isfinite(next_rz) || next_rz <= 0.0) return fail(ctl, RH_BREAKDOWN, it);
+    ctl->iteration = it;
+  }
+};
+
 // next rz after a red-black preconditioner application.
 struct FinSgsRed {
   PcgControl* ctl;
@@ -318,6 +356,115 @@ k_direction(DevGrid g, const T* __restrict__ r, const T* __restrict__ z, T* __re
 }
 
 
+// Jacobi PCG apply for iterations >= 1.  Not replaced: k_apply_dir.  After
+// a replacement (ctl->replaced, set by the previous iteration's true
+// residual held in `prod`): r = prod (r[gauge] = 0) and d = z = r / D are
+// formed on the fly (the values k_replace and k_direction stored), q = K d,
+// and the partial r.z of the new rz is published beside d.q.
+template <typename T>
+__device__ __forceinline__ T jdir_from(const DevGrid& g, const T* __restrict__ prod, int c, int i, int j, int k) {
+  return c == g.gauge ? (T)0.0 : (T)jdiv(ld(prod, c), diag_at(g, c, i, j, k));
+}
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_apply_dir_j(DevGrid g, T* __restrict__ r, T* __restrict__ d, T* __restrict__ q, const T* __restrict__ prod,
+              const PcgControl* ctl, DD* pdd, double* pmx) {
+  if (ctl->done) return;
+  const bool rep = ctl->replaced;
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  double v, dc;
+  if (!rep) {
+    v = apply_at<T>(g, d, c, i, j, k);
+    dc = ld(d, c);
+  } else {
+    // apply_at over the virtual direction d = z(prod): box-clamped
+    // neighbours (offset 0 = the cell itself), closed faces select d_c
+    const Nbr n = nbr_offsets(g.nx, g.ny, g.nz, i, j, k);
+    const T dT = jdir_from(g, prod, c, i, j, k);
+    const double xc = (double)dT;
+    double a = (double)jdir_from(g, prod, c + n.xm, i + n.xm, j, k);
+    double b = (double)jdir_from(g, prod, c + n.xp, i + n.xp, j, k);
+    double cy = (double)jdir_from(g, prod, c + n.ym, i, n.ym ? j - 1 : j, k);
+    double dd = (double)jdir_from(g, prod, c + n.yp, i, n.yp ? j + 1 : j, k);
+    double e = (double)jdir_from(g, prod, c + n.zm, i, j, n.zm ? k - 1 : k);
+    double f = (double)jdir_from(g, prod, c + n.zp, i, j, n.zp ? k + 1 : k);
+    if (g.mask) {
+      const int m = g.mask[c];
+      a = (m & 1) ? a : xc;
+      b = (m & 2) ? b : xc;
+      cy = (m & 4) ? cy : xc;
+      dd = (m & 8) ? dd : xc;
+      e = (m & 16) ? e : xc;
+      f = (m & 32) ? f : xc;
+    }
+    v = 0.0;
+    v += g.w[0] * (xc - a);
+    v += g.w[0] * (xc - b);
+    v += g.w[1] * (xc - cy);
+    v += g.w[1] * (xc - dd);
+    v += g.w[2] * (xc - e);
+    v += g.w[2] * (xc - f);
+    dc = xc;
+    d[c] = dT;
+    if (c == g.gauge) {
+      r[c] = (T)0.0;
+    } else {
+      const T rn = prod[c];
+      r[c] = rn;
+      const double rv = (double)rn;
+      dd_acc(s[1], rv * jdiv(rv, diag_at(g, c, i, j, k)));
+    }
+  }
+  if (!isfinite(v)) bad[0] = 1.0;
+  if (c == g.gauge) v = 0.0;
+  const T qs = (T)v;
+  q[c] = qs;
+  dd_acc(s[0], dc * (double)qs);
+  END_CELLS
+  block_publish<2, 1>(s, bad, pdd, pmx);
+}
+
+// Jacobi PCG direction for iterations >= 1 (not replaced: FinUpdate cleared
+// the flag), with the true residual of a candidate iteration in the same
+// pass: prod = b - K p into `prod` and its partials (k_true_residual).  A
+// candidate's next-rz check waits for FinTrueJ / FinApplyDirJ.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_direction_j(DevGrid g, const double* __restrict__ p, const double* __restrict__ bvec, const T* __restrict__ r,
+              T* __restrict__ d, T* __restrict__ prod, PcgControl* ctl, DD* pdd, double* pmx, int it) {
+  if (ctl->done) return;
+  const bool cand = ctl->candidate;
+  const double next_rz = ctl->rz[(it + 1) & 1];
+  if (!cand && (!isfinite(next_rz) || next_rz <= 0.0)) {
+    if (blockIdx.x == 0 && flat_tid() == 0) fail(ctl, RH_BREAKDOWN, it);
+    return;
+  }
+  const double beta = next_rz / ctl->rz[it & 1];
+  DD s[1] = {{0.0, 0.0}};
+  double m[2] = {0.0, 0.0};
+  FOR_CELLS(g)
+  if (cand) {
+    const double v = apply_at<double>(g, p, c, i, j, k);
+    if (!isfinite(v)) m[1] = 1.0;
+    const double pr = bvec[c] - v;
+    prod[c] = (T)pr;
+    dd_acc(s[0], pr * pr);
+    m[0] = fmax(m[0], fabs(pr));
+  }
+  if (c == g.gauge) {
+    d[c] = (T)0.0;
+    continue;
+  }
+  const double zv = jdiv(ld(r, c), diag_at(g, c, i, j, k));
+  d[c] = (T)(zv + beta * ld(d, c));
+  END_CELLS
+  if (cand) block_publish<1, 2>(s, m, pdd, pmx);
+  if (!cand && blockIdx.x == 0 && flat_tid() == 0) ctl->iteration = it;
+}
+
+
 // Stateless operator kernels for the C ABI -----------------------------------
 
 __global__ void __launch_bounds__(kBlock)
@@ -434,6 +581,27 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
   T* q = (T*)a.q;
   T* z = (T*)a.z;
   const bool sgs = method == RH_RBSGS_PCG;
+  if (!sgs && it > 0) {
+    // apply (+ the previous iteration's replacement) -> update -> direction
+    // (+ this iteration's true residual when it is a candidate); the true
+    // residual goes to z, which the Jacobi iteration does not store
+    {
+      prof::Scope ps(prof::kApplyDir, s, it);
+      launch(k_apply_dir_j<T>, grid, block, 0, s, g, r, d, q, (const T*)z, ctl, pdd, pmx);
+    }
+    launch_finish<2, 1>(pdd, pmx, nb, FinApplyDirJ{ctl, it}, s);
+    {
+      prof::Scope ps(prof::kUpdate, s, it);
+      launch(k_update<T, false>, grid, block, 0, s, g, p, r, d, q, z, ctl, pdd, pmx);
+    }
+    launch_finish<3, 1>(pdd, pmx, nb, FinUpdate{ctl, it, 1}, s);
+    {
+      prof::Scope ps(prof::kDirection, s, it);
+      launch(k_direction_j<T>, grid, block, 0, s, g, p, a.b, r, d, z, ctl, pdd, pmx, it);
+    }
+    launch_finish<1, 2>(pdd, pmx, nb, FinTrueJ{ctl, it}, s);
+    return;
+  }
   if (it > 0) {
     {
       prof::Scope ps(prof::kApplyDir, s, it);

@@ -161,9 +161,11 @@ rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config)
   if (st) return st;
   n_ = (size_t)grid.counts[0] * grid.counts[1] * grid.counts[2];
   const size_t s = config.precision == RH_F64 ? 8 : 4;
-  // r, d, q (+ z for RB-SGS); the solution accumulates in the caller's f64
-  // pressure for either vector precision.
-  int vectors = 3 + (config.method == RH_RBSGS_PCG ? 1 : 0);
+  // r, d, q, z (RB-SGS: the preconditioned residual; Jacobi: the true
+  // residual of a candidate iteration, held until the next apply folds a
+  // replacement in); the solution accumulates in the caller's f64 pressure
+  // for either vector precision.
+  int vectors = 4;
   bytes_ = s * n_ * vectors;
   if (config.memory_limit && bytes_ > config.memory_limit) return RH_BUFFER_LIMIT;
   RH_HIP(hipMalloc(&arena_, bytes_ + 256 * vectors));
@@ -176,7 +178,7 @@ rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config)
   r_ = take();
   d_ = take();
   q_ = take();
-  z_ = config.method == RH_RBSGS_PCG ? take() : nullptr;
+  z_ = take();
 
   return RH_OK;
 }

@@ -185,6 +185,30 @@ def test_pcg_iterates_bit_for_bit(method):
         assert rep.true_residual_l2 == ref.true_residual_l2
 
 
+@pytest.mark.parametrize("method", ["jacobi-pcg-v1", "rbsgs-pcg"])
+def test_pcg_replacement_path_bit_for_bit(method):
+    """A threshold below the attainable residual (rel 1e-17): from about
+    iteration 370 on, candidate iterations take the true residual, are not
+    accepted and restart from it (the oracle's iterates then differ from
+    the rel = 0 run); the iterate, status and residual equal the oracle's
+    bit for bit."""
+    spec, vel, dt = config1(32, seed=3)
+    g = ogrid(spec)
+    rhs = O.build_rhs(g, O.advect_velocity(g, vel, dt), dt)
+    ws = R.PressureWorkspace(spec, method, R.F64)
+    for k in (380, 420, 500):
+        s = jpcg_settings(relative_residual=1e-17, absolute_residual=0.0, max_iterations=k)
+        rep = ws.solve_rhs(1.0, rhs, dt, s)
+        ref = O.solve_rhs(g, 1.0, rhs, dt, osettings(s), method=method)
+        assert rep.status == R.ITERATION_LIMIT == ref.status
+        assert rep.error_iteration == k
+        assert bits(ws.get_pressure()) == bits(ref.pressure), k
+        assert rep.true_residual_l2 == ref.true_residual_l2
+    plain = O.solve_rhs(g, 1.0, rhs, dt, osettings(jpcg_settings(relative_residual=0.0, absolute_residual=0.0,
+                                                                  max_iterations=500)), method=method)
+    assert bits(plain.pressure) != bits(ref.pressure)  # the replacement path was taken
+
+
 @pytest.mark.parametrize("n", [32, 64])
 def test_steady_scene_golden_on_device(n):
     """Reference-published run (docs/plans/accepted-pressure-warm-start): the
