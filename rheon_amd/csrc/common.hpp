@@ -195,8 +195,11 @@ __device__ __forceinline__ void finish_body(const DD* __restrict__ pdd, const do
 #pragma unroll
   for (int q = 0; q < NM; ++q) m[q] = 0.0;
   // kUnroll partials per thread are loaded before any is added, so their
-  // loads are in flight together (the partials come from other XCDs' L2s)
-  constexpr int kUnroll = 8;
+  // loads are in flight together (the partials come from other XCDs' L2s):
+  // 32 (8192 partials in one round) when they fit 256 registers, else 16.
+  // Each thread adds the partials b = t (mod kBlock) in ascending order
+  // whatever kUnroll is, so the sums do not depend on it.
+  constexpr int kUnroll = (4 * ND + 2 * NM) * 32 <= 256 ? 32 : 16;
   for (int b0 = t; b0 < nb; b0 += kUnroll * kBlock) {
     DD v[kUnroll][ND];
     double w[kUnroll][NM];
