@@ -25,6 +25,7 @@
 #include "profiler.hpp"
 #include "stencil.hpp"
 #include "mg_stencil.hpp"
+#include "mg_quad.hpp"
 #include "batch.hpp"
 
 namespace rh {
@@ -344,6 +345,61 @@ __device__ __forceinline__ void tail_smooth(const MgLevel& L, const unsigned* fr
   __syncthreads();
 }
 
+// FP32 tail levels with nx % 4 == 0 take four cells per lane: 16-byte LDS
+// reads of the cells, their y/z neighbours, b and the fraction words, and the
+// level kernels' quad expressions (b_mg_smooth_qf, k_mg_smooth_q), so each
+// value equals tail_smooth's.  kResidual: out = b - K x (the restriction's
+// first pass) instead of the sweep.
+template <bool kResidual>
+__device__ __forceinline__ void tail_sweep_q(const MgLevel& L, const unsigned* fr, const double* od,
+                                          const float* xin, const float* b, float* out, double tau) {
+  const int qx = L.nx / 4;
+  const int nq = qx * L.ny * L.nz;
+  const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
+  for (TailWalk w(qx, L.ny); w.c < nq; w.next(qx, L.ny)) {
+    const int i0 = 4 * w.i, j = w.j, k = w.k, c = 4 * w.c;
+    Quad<float> Q;
+    load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);
+    float bb[4], xn[4];
+    load4(b, c, bb);
+    if (fr) {
+      FracRow R;
+      load_frac_row(fr, L, i0, j, k, c, R);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const Frac6 f = frac6_of(R, u, i0 + u + 1 < L.nx);
+        const float ax = quad_apply_frac(L, Q, u, f);
+        if (kResidual) {
+          xn[u] = bb[u] - ax;
+        } else {
+          double o;
+          if (od) {
+            o = od[c + u];
+          } else {
+            const double d = L.wf[0] * (double)(f.xm + f.xp) + L.wf[1] * (double)(f.ym + f.yp) +
+                             L.wf[2] * (double)(f.zm + f.zp);
+            o = d > 0.0 ? 1.0 / d : 0.0;
+          }
+          xn[u] = Q.c[u] + (float)(tau * o) * (bb[u] - ax);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float ax = quad_apply_f(Q, u, w0, w1, w2);
+        if (kResidual) {
+          xn[u] = bb[u] - ax;
+        } else {
+          const int m = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, k);
+          xn[u] = Q.c[u] + (float)(tau * (od ? od[m] : L.odiag[m])) * (bb[u] - ax);
+        }
+      }
+    }
+    store4(out, c, xn);
+  }
+  __syncthreads();
+}
+
 // Restriction in two passes: the fine residuals b - K x into r (every
 // thread busy), then the eight-children sums in (dz, dy, dx) order (the
 // same values and order as k_mg_restrict).
@@ -351,10 +407,21 @@ template <typename S>
 __device__ __forceinline__ void tail_restrict(const MgLevel& F, const unsigned* ffr, const MgLevel& Cl,
                                            const unsigned* cfr, const double* cod, const S* x, const S* b,
                                            S* r, S* bc, S* xc, double tau) {
-  const int nf = F.nx * F.ny * F.nz;
-  for (TailWalk w(F.nx, F.ny); w.c < nf; w.next(F.nx, F.ny))
-    r[w.c] = b[w.c] - s_apply_fr<S>(F, ffr, x, w.c, w.i, w.j, w.k);
-  __syncthreads();
+  if constexpr (sizeof(S) == 4) {
+    if (F.nx % 4 == 0) {
+      tail_sweep_q<true>(F, ffr, nullptr, x, b, r, 0.0);
+    } else {
+      const int nf = F.nx * F.ny * F.nz;
+      for (TailWalk w(F.nx, F.ny); w.c < nf; w.next(F.nx, F.ny))
+        r[w.c] = b[w.c] - s_apply_fr<S>(F, ffr, x, w.c, w.i, w.j, w.k);
+      __syncthreads();
+    }
+  } else {
+    const int nf = F.nx * F.ny * F.nz;
+    for (TailWalk w(F.nx, F.ny); w.c < nf; w.next(F.nx, F.ny))
+      r[w.c] = b[w.c] - s_apply_fr<S>(F, ffr, x, w.c, w.i, w.j, w.k);
+    __syncthreads();
+  }
   const int n = Cl.nx * Cl.ny * Cl.nz;
   for (TailWalk w(Cl.nx, Cl.ny); w.c < n; w.next(Cl.nx, Cl.ny)) {
     S sum = (S)0;
@@ -569,7 +636,14 @@ __device__ __forceinline__ void b_mg_tail(const MgHierarchy* __restrict__ H, int
     const MgLevel& L = H->lv[l];
     switch (op) {
       case kSmooth:
-        tail_smooth<S>(L, FR(l), OD(l), cur(l), B(l), nxt(l), tau);
+        if constexpr (sizeof(S) == 4) {
+          if (L.nx % 4 == 0)
+            tail_sweep_q<false>(L, FR(l), OD(l), cur(l), B(l), nxt(l), tau);
+          else
+            tail_smooth<S>(L, FR(l), OD(l), cur(l), B(l), nxt(l), tau);
+        } else {
+          tail_smooth<S>(L, FR(l), OD(l), cur(l), B(l), nxt(l), tau);
+        }
         break;
       case kRestrict:
         // nxt(l) is free until the prolongation: it holds the residuals
