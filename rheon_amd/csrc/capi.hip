@@ -820,9 +820,8 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
     }
     {
       rh::prof::Scope ps(rh::prof::kDivMax, st, -1);
-      rh::launch_divergence_max<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(0), st);
+      rh::launch_divergence_rate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(0), st);
     }
-    rh::launch_velocity_rate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(1), st);
     return RH_OK;
   });
   if (s) return s;

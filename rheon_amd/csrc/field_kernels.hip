@@ -441,6 +441,46 @@ k_velocity_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
   b_velocity_rate<T>(hw_block(), g, u, v, w, red);
 }
 
+// divergence maximum and velocity rate in one pass over the corrected
+// faces (the scene step needs both): per cell the divergence of
+// b_divergence_max, and the maxima of |u|, |v|, |w| over every face (each
+// cell takes its three lower faces, the last cell of a row / column /
+// pillar also the upper wall face), as b_velocity_rate.  Maxima do not
+// depend on the order they are taken in, so the values are those of the
+// two kernels.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_divergence_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+                  Reduce red) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const RowRange rr = block_rows(ny * nz);
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / ny, j = row - k * ny;
+    for (int i = threadIdx.x; i < nx; i += kLanesX) {
+      const int xu = i + (nx + 1) * row;
+      const int yv = i + nx * (j + (ny + 1) * k);
+      const int zw = i + nx * row;
+      const double u0 = (double)u[xu], u1 = (double)u[xu + 1];
+      const double v0 = (double)v[yv], v1 = (double)v[yv + nx];
+      const double w0 = (double)w[zw], w1 = (double)w[zw + nx * ny];
+      const double value = (u1 - u0) / g.h[0] + (v1 - v0) / g.h[1] + (w1 - w0) / g.h[2];
+      if (!isfinite(value)) bad |= kFlagArithmetic;
+      m[0] = fmax(m[0], fabs(value));
+      m[1] = fmax(m[1], fabs(u0));
+      m[2] = fmax(m[2], fabs(v0));
+      m[3] = fmax(m[3], fabs(w0));
+      if (i + 1 == nx) m[1] = fmax(m[1], fabs(u1));
+      if (j + 1 == ny) m[2] = fmax(m[2], fabs(v1));
+      if (k + 1 == nz) m[3] = fmax(m[3], fabs(w1));
+    }
+  }
+  if (bad) atomicOr(red.flags, bad);
+  DD dd[1] = {DD{0.0, 0.0}};
+  block_publish<1, 4>(dd, m, red.pdd, red.pmx);
+}
+
 // ------------------------------------------------------------ host launchers
 template <typename T>
 void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
@@ -490,6 +530,17 @@ void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const 
   dim3 grid(grid_blocks(rows)), block(kLanesX, kRowsPerBlock);
   k_velocity_rate<T><<<grid, block, 0, s>>>(g, (const T*)u, (const T*)v, (const T*)w, red);
   launch_finish<1, 3>(red.pdd, red.pmx, grid.x, FinStore{red.out, 1, g.h[0], g.h[1], g.h[2]}, s);
+}
+
+// out[0] = max|div u|, out[1] = velocity rate, out[2..4] = max|u|, |v|, |w|
+// (the slots launch_divergence_max into out and launch_velocity_rate into
+// out + 1 fill).
+template <typename T>
+void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, const void* w, const Reduce& red,
+                            hipStream_t s) {
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  launch(k_divergence_rate<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, red);
+  launch_finish<1, 4>(red.pdd, red.pmx, grid.x, FinStore{red.out, 3, g.h[0], g.h[1], g.h[2]}, s);
 }
 
 void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,
@@ -623,7 +674,9 @@ void launch_b_velocity_rate(const BatchLaunch& bl, hipStream_t s) {
   template void launch_divergence_max<T>(const DevGrid&, const void*, const void*, const void*, \
                                          const Reduce&, hipStream_t);                          \
   template void launch_velocity_rate<T>(const DevGrid&, const void*, const void*, const void*,  \
-                                        const Reduce&, hipStream_t);
+                                        const Reduce&, hipStream_t);                           \
+  template void launch_divergence_rate<T>(const DevGrid&, const void*, const void*, const void*, \
+                                          const Reduce&, hipStream_t);
 RH_INSTANTIATE(float)
 RH_INSTANTIATE(double)
 #undef RH_INSTANTIATE

@@ -39,6 +39,11 @@ void launch_divergence_max(const DevGrid& g, const void* u, const void* v, const
 template <typename T>
 void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const void* w,
                           const Reduce& red, hipStream_t s);
+// Both in one pass: divergence maximum into red.out[0], the velocity rate
+// (and the three component maxima) into red.out[1..4].
+template <typename T>
+void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, const void* w,
+                            const Reduce& red, hipStream_t s);
 template <typename T>
 void launch_zero_blocked(const DevGrid& g, void* u, void* v, void* w, hipStream_t s);
 void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,
