@@ -116,11 +116,15 @@ bool mg_quad_outer_ok(const DevGrid& g, const MgLevel& L0, const double* p, cons
 int mg_quad_outer_blocks(const DevGrid& g);
 inline int mg_quad_level_blocks(const MgLevel& L) { return quad_blocks(L.nx / 4, L.ny * L.nz); }
 inline int mg_quad_restrict_blocks(const MgLevel& C) { return quad_blocks(C.nx / 2, C.ny * C.nz); }
+// p_zero: p is known to be 0 and is not read (the solve's first residual,
+// the first cycle's update p = 0 + x).
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
                                 float* b0, float* x0, double tau, const PcgControl* ctl, DD* pdd,
-                                double* pmx, hipStream_t s);
+                                double* pmx, hipStream_t s, bool p_zero = false);
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
-                        double tau, const PcgControl* ctl, hipStream_t s);
+                        double tau, const PcgControl* ctl, hipStream_t s, bool p_zero = false);
+// The finest level's last sweep (p += x) takes the four-cell kernel.
+bool mg_quad_update(const MgHierarchy& h, const double* p);
 // Second finest pre-sweep from b0 alone (x0 = tau1 b0 / D recomputed per
 // position; the outer residual then does not store x0).
 void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
@@ -162,11 +166,15 @@ struct FinMgTrue {
 
 // x0_implicit (FP32 only, mg_first_from_b): the outer residual does not
 // store the first pre-sweep and the cycle starts from b0 alone.
+// p_zero (four-cell FP32 path only): p holds 0 and is not read (see
+// launch_mg_outer_residual_q); the caller fills p when no cycle runs.
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
-                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit);
+                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit,
+                       bool p_zero = false);
 template <typename S>
-void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit);
+void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit,
+              bool p_zero = false);
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s);
 int mg_tail_lds_limit();  // dynamic LDS bytes the tail kernels may use
 

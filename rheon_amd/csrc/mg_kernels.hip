@@ -804,7 +804,8 @@ static inline dim3 level_grid(const MgLevel& L) { return dim3(level_blocks((long
 
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
-                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit) {
+                       PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit,
+                       bool p_zero) {
   dim3 grid = cells_grid(g);
   {
     prof::Scope ps(prof::kOuterResidual, s, it);
@@ -812,7 +813,7 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
     if (quad) grid = dim3(mg_quad_outer_blocks(g));
     if (quad)
       launch_mg_outer_residual_q(g, h.lv[0], p, b, (float*)h.b[0], x0_implicit ? nullptr : (float*)h.x[0],
-                                 h.tau_pre[0], ctl, pdd, pmx, s);
+                                 h.tau_pre[0], ctl, pdd, pmx, s, p_zero);
     else
       launch(k_mg_outer_residual<S>, grid, block_shape(), 0, s, g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
                                                             h.tau_pre[0], ctl, pdd, pmx);
@@ -824,15 +825,19 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
 // slots (tagged with the iteration); all coarser work shares one slot.
 static inline int mg_slot(int l, int finest_slot) { return l == 0 ? finest_slot : prof::kMgCoarse; }
 
+bool mg_quad_update(const MgHierarchy& h, const double* p) {
+  return h.quad && mg_quad_level(h.lv[0]) && ((uintptr_t)p & 15u) == 0;
+}
+
 template <typename S>
 static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p, double tau,
-                   const PcgControl* ctl, int it, hipStream_t s) {
+                   const PcgControl* ctl, int it, hipStream_t s, bool p_zero = false) {
   prof::Scope ps(mg_slot(l, p ? prof::kMgSmoothUpdate : prof::kMgSmooth), s, l == 0 ? it : -2);
   const bool sized = mg_quad_size_ok(h.lv[l], l);
   if (sizeof(S) == 4 && h.quad && sized && !p && mg_quad_frac_level(h.lv[l]))
     launch_mg_smooth_qf(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, tau, ctl, s);
   else if (sizeof(S) == 4 && h.quad && sized && mg_quad_level(h.lv[l]) && ((uintptr_t)p & 15u) == 0)
-    launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, tau, ctl, s);
+    launch_mg_smooth_q(h.lv[l], (const float*)xin, (const float*)h.b[l], (float*)xout, p, tau, ctl, s, p_zero);
   else
     launch(k_mg_smooth<S>, level_grid(h.lv[l]), block_shape(), 0, s, h.lv[l], xin, (const S*)h.b[l], xout,
                                                                  tau, p, ctl);
@@ -843,7 +848,7 @@ static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p
 // recomputes the first from it).
 template <typename S>
 static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl, int it,
-                   hipStream_t s, bool x0_implicit) {
+                   hipStream_t s, bool x0_implicit, bool p_zero) {
   S* x = (S*)h.x[l];
   S* y = (S*)h.y[l];
   const S* b = (const S*)h.b[l];
@@ -894,7 +899,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
       launch(k_mg_restrict<S>, level_grid(h.lv[l + 1]), block_shape(), 0, s,
           h.lv[l], h.lv[l + 1], cur, b, (S*)h.b[l + 1], first, h.tau_cpre[0], ctl);
   }
-  vcycle<S>(h, l + 1, p, ctl, it, s, false);
+  vcycle<S>(h, l + 1, p, ctl, it, s, false, false);
   {
     prof::Scope ps(mg_slot(l, prof::kMgProlong), s, l == 0 ? it : -2);
     if (sizeof(S) == 4 && h.quad && mg_quad_size_ok(h.lv[l], l) && mg_quad_prolong_pair(h.lv[l], h.lv[l + 1]))
@@ -912,7 +917,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   for (int sweep = 0; sweep < post; ++sweep) {
     const bool last = sweep + 1 == post;
     if (last && l == 0) {
-      smooth<S>(h, l, cur, nxt, p, tau_post[sweep], ctl, it, s);  // p += x (finest, last sweep)
+      smooth<S>(h, l, cur, nxt, p, tau_post[sweep], ctl, it, s, p_zero);  // p += x (finest, last sweep)
     } else {
       smooth<S>(h, l, cur, nxt, nullptr, tau_post[sweep], ctl, it, s);
       S* t = cur;
@@ -929,8 +934,9 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
 }
 
 template <typename S>
-void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit) {
-  vcycle<S>(h, 0, p, ctl, it, s, x0_implicit);
+void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit,
+              bool p_zero) {
+  vcycle<S>(h, 0, p, ctl, it, s, x0_implicit, p_zero);
 }
 
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
@@ -939,10 +945,10 @@ void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
 }
 
 template void mg_outer_residual<float>(const DevGrid&, const MgHierarchy&, const double*,
-                                       const double*, PcgControl*, DD*, double*, int, hipStream_t, bool);
+                                       const double*, PcgControl*, DD*, double*, int, hipStream_t, bool, bool);
 template void mg_outer_residual<double>(const DevGrid&, const MgHierarchy&, const double*,
-                                        const double*, PcgControl*, DD*, double*, int, hipStream_t, bool);
-template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool);
-template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool);
+                                        const double*, PcgControl*, DD*, double*, int, hipStream_t, bool, bool);
+template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool, bool);
+template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool, bool);
 
 }  // namespace rh

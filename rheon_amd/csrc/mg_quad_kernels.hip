@@ -18,7 +18,10 @@ namespace rh {
 
 // Outer FP64 residual r = b - K p with the first pre-smoothing sweep fused
 // (as k_mg_outer_residual); x0 = null: the sweep's result is not stored
-// (b_mg_smooth_first_q recomputes it where it is needed).
+// (b_mg_smooth_first_q recomputes it where it is needed).  kZeroP: p is
+// known to be 0 (the solve's first residual), so it is not read; K 0 =
+// +0.0 exactly, the values of the general form.
+template <bool kZeroP = false>
 __device__ __forceinline__ void
 b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const double* __restrict__ p, const double* __restrict__ b,
                       float* __restrict__ b0, float* __restrict__ x0, double tau, const PcgControl* ctl,
@@ -29,7 +32,12 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   const int nz_ = g.nz;
   FOR_QUADS(g.nx, g.ny)
   Quad<double> Q;
-  load_quad(p, g.nx, g.ny, g.nz, i0, j, k, c, Q);
+  if (kZeroP) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) Q.c[u] = Q.xm[u] = Q.xp[u] = Q.ym[u] = Q.yp[u] = Q.zm[u] = Q.zp[u] = 0.0;
+  } else {
+    load_quad(p, g.nx, g.ny, g.nz, i0, j, k, c, Q);
+  }
   double bb[4];
   load4(b, c, bb);
   const unsigned m4 = g.mask ? *reinterpret_cast<const unsigned*>(g.mask + c) : 0u;
@@ -53,8 +61,9 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
 }
 
 // One smoothing sweep (as k_mg_smooth); kUpdate: p += x_out instead of
-// storing x_out (the finest level's last sweep).
-template <bool kUpdate>
+// storing x_out (the finest level's last sweep).  kZeroP: p is known to be
+// 0 (the first cycle), so p = 0.0 + x_out is formed without reading it.
+template <bool kUpdate, bool kZeroP = false>
 __device__ __forceinline__ void
 b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const float* __restrict__ b,
               float* __restrict__ xout, double* __restrict__ p, double tau, const PcgControl* ctl) {
@@ -67,7 +76,14 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
   float bb[4];
   load4(b, c, bb);
   double pv[4];
-  if (kUpdate) load4(p, c, pv);
+  if (kUpdate) {
+    if (kZeroP) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) pv[u] = 0.0;
+    } else {
+      load4(p, c, pv);
+    }
+  }
   const unsigned m4 = L.mask ? *reinterpret_cast<const unsigned*>(L.mask + c) : 0u;
   float xn[4];
 #pragma unroll
@@ -367,17 +383,18 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
 
 // ------------------------------------------------------- kernel wrappers
 
+template <bool kZeroP>
 __global__ void __launch_bounds__(kBlock)
 k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const double* __restrict__ b,
                       float* __restrict__ b0, float* __restrict__ x0, double tau, const PcgControl* ctl,
                       DD* pdd, double* pmx) {
-  b_mg_outer_residual_q(hw_block(), g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
+  b_mg_outer_residual_q<kZeroP>(hw_block(), g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
 }
-template <bool kUpdate>
+template <bool kUpdate, bool kZeroP>
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
               float* __restrict__ xout, double* __restrict__ p, double tau, const PcgControl* ctl) {
-  b_mg_smooth_q<kUpdate>(hw_block(), L, xin, b, xout, p, tau, ctl);
+  b_mg_smooth_q<kUpdate, kZeroP>(hw_block(), L, xin, b, xout, p, tau, ctl);
 }
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth_qf(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b, float* __restrict__ xout,
@@ -490,18 +507,24 @@ int mg_quad_outer_blocks(const DevGrid& g) { return quad_blocks(g.nx / 4, g.ny *
 
 void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const double* p, const double* b,
                                 float* b0, float* x0, double tau, const PcgControl* ctl, DD* pdd,
-                                double* pmx, hipStream_t s) {
-  launch(k_mg_outer_residual_q, dim3(mg_quad_outer_blocks(g)), block_shape(), 0, s, g, L0, p, b, b0, x0, tau,
-         ctl, pdd, pmx);
+                                double* pmx, hipStream_t s, bool p_zero) {
+  if (p_zero)
+    launch(k_mg_outer_residual_q<true>, dim3(mg_quad_outer_blocks(g)), block_shape(), 0, s, g, L0, p, b, b0, x0,
+           tau, ctl, pdd, pmx);
+  else
+    launch(k_mg_outer_residual_q<false>, dim3(mg_quad_outer_blocks(g)), block_shape(), 0, s, g, L0, p, b, b0, x0,
+           tau, ctl, pdd, pmx);
 }
 
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,
-                        double tau, const PcgControl* ctl, hipStream_t s) {
+                        double tau, const PcgControl* ctl, hipStream_t s, bool p_zero) {
   const dim3 grid(quad_blocks(L.nx / 4, L.ny * L.nz));
-  if (p)
-    launch(k_mg_smooth_q<true>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
+  if (p && p_zero)
+    launch(k_mg_smooth_q<true, true>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
+  else if (p)
+    launch(k_mg_smooth_q<true, false>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
   else
-    launch(k_mg_smooth_q<false>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
+    launch(k_mg_smooth_q<false, false>, grid, block_shape(), 0, s, L, xin, b, xout, p, tau, ctl);
 }
 
 void launch_mg_smooth_qf(const MgLevel& L, const float* xin, const float* b, float* xout, double tau,

@@ -776,16 +776,21 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   RH_HIP(hipEventRecord(scratch_.ev0, s));
   // compatibility + threshold, p = 0, initial true residual (it = 0)
   launch_rhs_sums(g, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
-  RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
   const bool x0_implicit = sizeof(S) == 4 && mg_first_from_b(g, h_, d_pressure, d_rhs);
-  mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s, x0_implicit);
+  // Four-cell FP32 path: p = 0 is never read -- the first residual is b
+  // and the first cycle's update writes p = 0 + x over every cell -- so p
+  // is filled only if no cycle runs (below).
+  const bool p_zero = x0_implicit && mg_quad_update(h_, d_pressure);
+  if (!p_zero) RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
+  mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s, x0_implicit,
+                       p_zero);
   RH_HIP(hipGetLastError());
   bool cancelled = false;
   const int poll = st.poll_interval > 0 ? st.poll_interval : 2;
   rh_status rs = drive(
       h.max_iterations, poll,
       [&](int it) {
-        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s, x0_implicit);
+        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s, x0_implicit, p_zero && it == 1);
         mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s,
                              x0_implicit);
       },
@@ -795,6 +800,8 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   // report the true residual of the returned pressure.
   PcgControl c;
   rs = finish_report(report, cancelled, &c);
+  if (p_zero && c.iteration == 0)  // no cycle ran: the result is p = 0
+    RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
   if (rs == RH_ITERATION_LIMIT) mg_regauge(g, d_pressure, scratch_.ctl, s);
   if (rs != RH_OK) return rs;
   mg_regauge(g, d_pressure, scratch_.ctl, s);

@@ -146,6 +146,27 @@ def test_device_mg_cycles_track_the_cpu_statement(coarse):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", [R.F32, R.F64])
+def test_device_mg_no_cycle_leaves_zero_pressure(precision):
+    """A solve that ends before its first cycle (zero right-hand side, or no
+    iterations allowed) returns p = 0 exactly, whatever the output buffer
+    held (the FP32 four-cell path does not read p = 0 and fills it only in
+    that case)."""
+    spec, vel, dt = config1(64, seed=4, dtype=np.float32)
+    rhs = O.build_rhs(ogrid(spec), vel, dt)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", precision)
+    assert ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-8, 1e-14, 1.0, 100)).ok
+    assert np.any(ws.get_pressure() != 0.0)
+    rep = ws.solve_rhs(1.0, np.zeros_like(rhs), dt, R.PressureSettings(1e-8, 1e-14, 1.0, 100))
+    assert rep.ok and rep.iterations == 0
+    assert np.all(ws.get_pressure().view(np.uint64) == 0)
+    assert ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-8, 1e-14, 1.0, 100)).ok
+    rep = ws.solve_rhs(1.0, rhs, dt, R.PressureSettings(1e-8, 1e-14, 1.0, 0))
+    assert rep.status == R.ITERATION_LIMIT
+    assert np.all(ws.get_pressure().view(np.uint64) == 0)
+
+
+@pytest.mark.gpu
 def test_device_mg_four_levels_config2_shape():
     """128^3 FP32 V-cycles, 4 levels (128 -> 16), to rel 1e-5."""
     spec, vel, dt = config1(128, seed=6, dtype=np.float32)
