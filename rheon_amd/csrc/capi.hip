@@ -797,9 +797,8 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   RH_HIP(hipEventRecord(sc->ev[1], st));
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
-    rh::launch_validate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], 1, scr.flags, st);
     rh::prof::Scope ps(rh::prof::kRhs, st, -1);
-    rh::launch_rhs<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], dt, sc->rhs, scr.flags, st);
+    rh::launch_rhs<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], dt, sc->rhs, scr.flags, st, true);
     return RH_OK;
   });
   if (s) return s;

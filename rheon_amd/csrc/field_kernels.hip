@@ -290,8 +290,11 @@ k_validate_faces(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
 }
 
 // build_rhs (src/operator.rs:460-509): b = (sum_d A_d (u_lo - u_hi)) / dt,
-// axis order X, Y, Z, f64 from T faces.
-template <typename T>
+// axis order X, Y, Z, f64 from T faces.  kValidate: also b_validate_faces
+// with the wall check, each face checked by the cell that reads it as its
+// lower face (or, for the last cell along an axis, its upper wall face);
+// same flags, one pass fewer.
+template <typename T, bool kValidate = false>
 __device__ __forceinline__ void
 b_build_rhs(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
             const T* __restrict__ w, double dt, double* __restrict__ out,
@@ -308,6 +311,22 @@ b_build_rhs(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __res
       const double ulo = (double)u[xu], uhi = (double)u[xu + 1];
       const double vlo = (double)v[yv], vhi = (double)v[yv + nx];
       const double wlo = (double)w[zw], whi = (double)w[zw + nx * ny];
+      if (kValidate) {
+        if (!isfinite(ulo) || !isfinite(vlo) || !isfinite(wlo)) bad |= kFlagNonfiniteInput;
+        if ((i == 0 && ulo != 0.0) || (j == 0 && vlo != 0.0) || (k == 0 && wlo != 0.0)) bad |= kFlagNonzeroWall;
+        if (i + 1 == nx) {
+          if (!isfinite(uhi)) bad |= kFlagNonfiniteInput;
+          if (uhi != 0.0) bad |= kFlagNonzeroWall;
+        }
+        if (j + 1 == ny) {
+          if (!isfinite(vhi)) bad |= kFlagNonfiniteInput;
+          if (vhi != 0.0) bad |= kFlagNonzeroWall;
+        }
+        if (k + 1 == g.nz) {
+          if (!isfinite(whi)) bad |= kFlagNonfiniteInput;
+          if (whi != 0.0) bad |= kFlagNonzeroWall;
+        }
+      }
       double flux = 0.0;
       flux += g.area[0] * (ulo - uhi);
       flux += g.area[1] * (vlo - vhi);
@@ -319,12 +338,12 @@ b_build_rhs(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __res
   }
   if (bad) atomicOr(flags, bad);
 }
-template <typename T>
+template <typename T, bool kValidate>
 __global__ void __launch_bounds__(kBlock)
 k_build_rhs(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
             const T* __restrict__ w, double dt, double* __restrict__ out,
             unsigned* __restrict__ flags) {
-  b_build_rhs<T>(hw_block(), g, u, v, w, dt, out, flags);
+  b_build_rhs<T, kValidate>(hw_block(), g, u, v, w, dt, out, flags);
 }
 
 // correct_candidate_in_place (src/operator.rs:627-668) for f32 storage:
@@ -380,6 +399,59 @@ __global__ void __launch_bounds__(kBlock)
 k_correct(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u,
           T* __restrict__ v, T* __restrict__ w, unsigned* __restrict__ flags) {
   b_correct<T>(hw_block(), g, p, dt, u, v, w, flags);
+}
+
+// correct_candidate_in_place with the three components in one pass over the
+// cells: cell (i, j, k) takes its three lower faces (the face is interior
+// when its own index along the axis is >= 1; the upper wall faces belong to
+// no cell and stay untouched), so p and its lower neighbours are read once
+// for all three components (k_correct reads p once per component).  Each
+// face gets b_correct's expressions and checks.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_correct_cells(DevGrid g, const double* __restrict__ p, double dt, T* __restrict__ u, T* __restrict__ v,
+                T* __restrict__ w, unsigned* __restrict__ flags) {
+  const int nx = g.nx, ny = g.ny;
+  const RowRange rr = block_rows(ny * g.nz);
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / ny, j = row - k * ny;
+    for (int i = threadIdx.x; i < nx; i += kLanesX) {
+      const int c = i + nx * row;
+      const double pc = p[c];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int pa = a == 0 ? i : (a == 1 ? j : k);
+        if (pa == 0) continue;
+        T* dst = a == 0 ? u : (a == 1 ? v : w);
+        const int index = a == 0 ? i + (nx + 1) * row : (a == 1 ? i + nx * (j + (ny + 1) * k) : c);
+        if (!face_open(g, a, i, j, k)) {  // closed obstacle face stays 0
+          dst[index] = (T)0.0;
+          continue;
+        }
+        const int stride = a == 0 ? 1 : (a == 1 ? nx : nx * ny);
+        const double difference = pc - p[c - stride];
+        const double acc = g.acc[a];
+        double value;
+        if (sizeof(T) == 4) {
+          value = (double)dst[index] - dt * acc * difference;
+          if (!isfinite(value) || fabs(value) > (double)FLT_MAX) {
+            bad |= kFlagArithmetic;
+            continue;
+          }
+        } else {
+          const double gradient = acc * difference;
+          value = (double)dst[index] - dt * gradient;
+          if (!isfinite(value)) {
+            bad |= kFlagArithmetic;
+            continue;
+          }
+        }
+        dst[index] = (T)value;
+      }
+    }
+  }
+  if (bad) atomicOr(flags, bad);
 }
 
 // divergence (src/operator.rs:513-568) and its all-cell maximum
@@ -502,17 +574,19 @@ void launch_validate(const DevGrid& g, const void* u, const void* v, const void*
 
 template <typename T>
 void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
-                double* out, unsigned* flags, hipStream_t s) {
+                double* out, unsigned* flags, hipStream_t s, bool validate) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
-  launch(k_build_rhs<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
+  if (validate)
+    launch(k_build_rhs<T, true>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
+  else
+    launch(k_build_rhs<T, false>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
 }
 
 template <typename T>
 void launch_correct(const DevGrid& g, const double* p, double dt, void* u, void* v, void* w,
                     unsigned* flags, hipStream_t s) {
-  const int rows = (g.ny + 1) * (g.nz + 1);
-  dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
-  launch(k_correct<T>, grid, block, 0, s, g, p, dt, (T*)u, (T*)v, (T*)w, flags);
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  launch(k_correct_cells<T>, grid, block, 0, s, g, p, dt, (T*)u, (T*)v, (T*)w, flags);
 }
 
 template <typename T>
@@ -668,7 +742,7 @@ void launch_b_velocity_rate(const BatchLaunch& bl, hipStream_t s) {
   template void launch_validate<T>(const DevGrid&, const void*, const void*, const void*, int, \
                                    unsigned*, hipStream_t);                                    \
   template void launch_rhs<T>(const DevGrid&, const void*, const void*, const void*, double,   \
-                              double*, unsigned*, hipStream_t);                                \
+                              double*, unsigned*, hipStream_t, bool);                          \
   template void launch_correct<T>(const DevGrid&, const double*, double, void*, void*, void*,  \
                                   unsigned*, hipStream_t);                                     \
   template void launch_divergence_max<T>(const DevGrid&, const void*, const void*, const void*, \

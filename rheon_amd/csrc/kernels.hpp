@@ -27,9 +27,10 @@ void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w
 template <typename T>
 void launch_validate(const DevGrid& g, const void* u, const void* v, const void* w,
                      int check_walls, unsigned* flags, hipStream_t s);
+// validate: also launch_validate's checks with the wall test, in the same pass.
 template <typename T>
 void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
-                double* out, unsigned* flags, hipStream_t s);
+                double* out, unsigned* flags, hipStream_t s, bool validate = false);
 template <typename T>
 void launch_correct(const DevGrid& g, const double* p, double dt, void* u, void* v, void* w,
                     unsigned* flags, hipStream_t s);
