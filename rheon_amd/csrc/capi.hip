@@ -730,6 +730,7 @@ extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const vo
   if (!sc || !u || !v || !w) return RH_INVALID_ARGUMENT;
   const void* src[3] = {u, v, w};
   const size_t fb = prec_bytes(sc->config.precision);
+  sc->acc_checked = false;
   for (int a = 0; a < 3; ++a)
     RH_HIP(hipMemcpy(sc->acc[a], src[a], sc->flen[a] * fb, hipMemcpyHostToDevice));
   return zero_blocked(sc);
@@ -738,6 +739,7 @@ extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const vo
 extern "C" rh_status rh_scene_set_solid(rh_scene* sc, const uint8_t* solid, size_t len) {
   if (!sc) return RH_INVALID_ARGUMENT;
   if (solid && len != sc->n) return RH_INVALID_ARGUMENT;
+  sc->acc_checked = false;
   rh_status s = sc->solver->impl->set_solid(solid);
   if (s) return s;
   return zero_blocked(sc);
@@ -760,6 +762,7 @@ extern "C" rh_status rh_scene_get_pressure(const rh_scene* sc, double* out, size
 
 extern "C" rh_status rh_scene_device_velocity(rh_scene* sc, void** d_u, void** d_v, void** d_w) {
   if (!sc || !d_u || !d_v || !d_w) return RH_INVALID_ARGUMENT;
+  sc->external = true;
   *d_u = sc->acc[0];
   *d_v = sc->acc[1];
   *d_w = sc->acc[2];
@@ -787,7 +790,8 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   RH_HIP(hipEventRecord(sc->ev[0], st));
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
-    rh::launch_validate<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], 0, scr.flags, st);
+    if (!sc->acc_checked || sc->external)
+      rh::launch_validate<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], 0, scr.flags, st);
     rh::prof::Scope ps(rh::prof::kAdvect, st, -1);
     rh::launch_advect<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], dt, sc->cand[0], sc->cand[1],
                          sc->cand[2], scr.flags, st);
@@ -845,5 +849,6 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   if (sc->config.max_courant > 0.0 && (!std::isfinite(rep->courant) || rep->courant > sc->config.max_courant))
     return RH_COURANT_LIMIT;
   for (int a = 0; a < 3; ++a) std::swap(sc->acc[a], sc->cand[a]);
+  sc->acc_checked = true;
   return RH_OK;
 }

@@ -23,6 +23,12 @@ struct rh_scene {
   uint64_t bytes = 0;
   std::unique_ptr<rh_solver> solver;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // The accepted faces are known finite: they were committed by a step
+  // (whose correction checked every face it wrote) and nothing has written
+  // them since.  `external`: their device pointers were handed out, so any
+  // step validates them again.
+  bool acc_checked = false;
+  bool external = false;
   ~rh_scene() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);

@@ -278,6 +278,24 @@ def test_scene_step_matches_oracle_and_is_transactional():
     assert all(bits(a) == bits(b) for a, b in zip(strict.get_velocity(), vel))
 
 
+def test_scene_step_validates_uploaded_faces():
+    """Accepted faces committed by a step are not re-validated, faces set
+    from the host always are: a non-finite upload after good steps is
+    rejected by the next step, and the accepted faces stay as uploaded."""
+    spec, vel, dt = config1(32, seed=2, dtype=np.float32)
+    s = R.PressureSettings(1e-8, 1e-12, 1e-6, 400)
+    scene = R.Scene(spec, 1.0, R.F32, "mixed-mg-dc", settings=s, actual_divergence_limit=1e-3)
+    scene.set_velocity(vel)
+    for _ in range(2):
+        assert scene.step(dt).status == R.OK
+    bad = [np.array(a, copy=True) for a in scene.get_velocity()]
+    bad[1][len(bad[1]) // 2] = np.nan
+    scene.set_velocity(bad)
+    assert scene.step(dt).status == R.NONFINITE_INPUT
+    got = scene.get_velocity()
+    assert np.isnan(got[1][len(bad[1]) // 2])
+
+
 def test_pcg_status_paths():
     spec = GridSpec((2, 1, 1), (1.0,) * 3)
     ws = R.PressureWorkspace(spec)
