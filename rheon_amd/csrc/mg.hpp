@@ -125,6 +125,10 @@ void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, floa
                         double tau, const PcgControl* ctl, hipStream_t s, bool p_zero = false);
 // The finest level's last sweep (p += x) takes the four-cell kernel.
 bool mg_quad_update(const MgHierarchy& h, const double* p);
+// launch_residual_report (sum of squares, max, nonfinite flag of b - K p)
+// four cells per lane; needs mg_quad_outer_ok.
+void launch_residual_report_q(const DevGrid& g, const double* p, const double* b, const Reduce& red,
+                              hipStream_t s);
 // Second finest pre-sweep from b0 alone (x0 = tau1 b0 / D recomputed per
 // position; the outer residual then does not store x0).
 void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,

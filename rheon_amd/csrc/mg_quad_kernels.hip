@@ -20,13 +20,15 @@ namespace rh {
 // (as k_mg_outer_residual); x0 = null: the sweep's result is not stored
 // (b_mg_smooth_first_q recomputes it where it is needed).  kZeroP: p is
 // known to be 0 (the solve's first residual), so it is not read; K 0 =
-// +0.0 exactly, the values of the general form.
-template <bool kZeroP = false>
+// +0.0 exactly, the values of the general form.  kReport: only the
+// reductions (the residual report of a returned pressure: b_residual_report's
+// sums and maxima, four cells per lane), no stores, no control block.
+template <bool kZeroP = false, bool kReport = false>
 __device__ __forceinline__ void
 b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const double* __restrict__ p, const double* __restrict__ b,
                       float* __restrict__ b0, float* __restrict__ x0, double tau, const PcgControl* ctl,
                       DD* pdd, double* pmx) {
-  if (ctl->done) return;
+  if (!kReport && ctl->done) return;
   DD s[1] = {{0.0, 0.0}};
   double m[2] = {0.0, 0.0};
   const int nz_ = g.nz;
@@ -51,11 +53,15 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
     const double r = bb[u] - v;
     dd_acc(s[0], r * r);
     m[0] = fmax(m[0], fabs(r));
-    rb[u] = (float)r;
-    rx[u] = (float)(tau * L0.odiag[mk]) * rb[u];
+    if (!kReport) {
+      rb[u] = (float)r;
+      rx[u] = (float)(tau * L0.odiag[mk]) * rb[u];
+    }
   }
-  store4(b0, c, rb);
-  if (x0) store4(x0, c, rx);
+  if (!kReport) {
+    store4(b0, c, rb);
+    if (x0) store4(x0, c, rx);
+  }
   END_QUADS
   block_publish<1, 2>(s, m, pdd, pmx, vb);
 }
@@ -390,6 +396,11 @@ k_mg_outer_residual_q(DevGrid g, MgLevel L0, const double* __restrict__ p, const
                       DD* pdd, double* pmx) {
   b_mg_outer_residual_q<kZeroP>(hw_block(), g, L0, p, b, b0, x0, tau, ctl, pdd, pmx);
 }
+__global__ void __launch_bounds__(kBlock)
+k_residual_report_q(DevGrid g, const double* __restrict__ p, const double* __restrict__ b, DD* pdd, double* pmx) {
+  const MgLevel unused{};
+  b_mg_outer_residual_q<false, true>(hw_block(), g, unused, p, b, nullptr, nullptr, 0.0, nullptr, pdd, pmx);
+}
 template <bool kUpdate, bool kZeroP>
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth_q(MgLevel L, const float* __restrict__ xin, const float* __restrict__ b,
@@ -514,6 +525,13 @@ void launch_mg_outer_residual_q(const DevGrid& g, const MgLevel& L0, const doubl
   else
     launch(k_mg_outer_residual_q<false>, dim3(mg_quad_outer_blocks(g)), block_shape(), 0, s, g, L0, p, b, b0, x0,
            tau, ctl, pdd, pmx);
+}
+
+void launch_residual_report_q(const DevGrid& g, const double* p, const double* b, const Reduce& red,
+                              hipStream_t s) {
+  const int nb = mg_quad_outer_blocks(g);
+  launch(k_residual_report_q, dim3(nb), block_shape(), 0, s, g, p, b, red.pdd, red.pmx);
+  launch_finish<1, 2>(red.pdd, red.pmx, nb, FinStore{red.out, 2, 0.0, 0.0, 0.0}, s);
 }
 
 void launch_mg_smooth_q(const MgLevel& L, const float* xin, const float* b, float* xout, double* p,

@@ -805,7 +805,10 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   if (rs == RH_ITERATION_LIMIT) mg_regauge(g, d_pressure, scratch_.ctl, s);
   if (rs != RH_OK) return rs;
   mg_regauge(g, d_pressure, scratch_.ctl, s);
-  launch_residual_report(g, d_pressure, d_rhs, scratch_.reduce(0), s);
+  if (h_.quad && mg_quad_outer_ok(g, h_.lv[0], d_pressure, d_rhs))
+    launch_residual_report_q(g, d_pressure, d_rhs, scratch_.reduce(0), s);
+  else
+    launch_residual_report(g, d_pressure, d_rhs, scratch_.reduce(0), s);
   double out[3];
   rs = scratch_.read_scalars(out, 3, 0);
   if (rs) return rs;
