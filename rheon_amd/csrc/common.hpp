@@ -58,6 +58,16 @@ struct DevGrid {
 
 constexpr unsigned char kFluidBit = 64;
 
+// Every byte offset into each face array of g (elements of `elem` bytes)
+// fits in 32 bits, so kernels may address them from a uniform base.
+inline bool face_offsets_32(const DevGrid& g, size_t elem) {
+  const unsigned long long x = g.nx, y = g.ny, z = g.nz;
+  unsigned long long m = (x + 1) * y * z;
+  if (x * (y + 1) * z > m) m = x * (y + 1) * z;
+  if (x * y * (z + 1) > m) m = x * y * (z + 1);
+  return m * elem <= (1ull << 32);
+}
+
 // Interior face of axis a with face lattice index (i, j, k): its upper cell
 // is (i, j, k), and the face is open when both cells are fluid (the lower-
 // face bit of the upper cell).  Always open in the plain box.

@@ -12,72 +12,44 @@
 namespace rh {
 
 // ---------------------------------------------------------------- sampling
-// ScalarSampler::sample (src/sampling.rs:74-122).  comp 0..2 = face lattice
-// of that axis, 3 = cell centres.  f64 arithmetic on T samples.
-template <typename T>
-__device__ __forceinline__ unsigned sample(const DevGrid& g, int comp, const T* __restrict__ vals,
-                                           const double pos[3], double& out) {
-  const int n[3] = {g.nx, g.ny, g.nz};
-  int lo[3], hi[3], dims[3];
-  double t[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    dims[d] = n[d] + (comp == d ? 1 : 0);
-    const double off = (comp == d) ? 0.0 : 0.5;
-    double x = pos[d];
-    if (x < g.origin[d]) x = g.origin[d];
-    if (x > g.upper[d]) x = g.upper[d];
-    double q = (x - g.origin[d]) / g.h[d] - off;
-    const double top = (double)(dims[d] - 1);
-    if (q < 0.0) q = 0.0;
-    if (q > top) q = top;
-    if (!isfinite(q)) return kFlagArithmetic;  // NaN survives the clamps
-    lo[d] = (int)floor(q);
-    hi[d] = lo[d] + 1 < dims[d] - 1 ? lo[d] + 1 : dims[d] - 1;
-    t[d] = q - (double)lo[d];
-  }
-  double result = 0.0, minimum = INFINITY, maximum = -INFINITY;
-#pragma unroll
-  for (int z = 0; z < 2; ++z)
-#pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        const int px = x ? hi[0] : lo[0];
-        const int py = y ? hi[1] : lo[1];
-        const int pz = z ? hi[2] : lo[2];
-        double weight = (x ? t[0] : 1.0 - t[0]);
-        weight = weight * (y ? t[1] : 1.0 - t[1]);
-        weight = weight * (z ? t[2] : 1.0 - t[2]);
-        const double value = (double)vals[px + dims[0] * (py + dims[1] * pz)];
-        minimum = fmin(minimum, value);
-        maximum = fmax(maximum, value);
-        result += weight * value;
-      }
-  if (!isfinite(result)) return kFlagArithmetic;
-  if (result < minimum) result = minimum;
-  if (result > maximum) result = maximum;
-  out = result;
-  return 0;
-}
-
 // (x - origin) / h of a world position clamped to the box, per axis: the
 // part of ScalarSampler::sample (src/sampling.rs:87-88) shared by the three
 // velocity components at one position.  When h is a power of two the
 // reciprocal is exact and the product equals the quotient bit for bit.
+// kInBox: the position is already clamped to the box (the RK2 midpoint and
+// departure are), so the clamp is the identity and is left out.  The clamp
+// is written as max/min: positions here are finite, where both forms agree
+// (up to the sign of a zero coordinate, which cannot change a sample).
+template <bool kInBox = false>
 __device__ __forceinline__ void base_coords(const DevGrid& g, const double pos[3], double out[3]) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     double x = pos[d];
-    if (x < g.origin[d]) x = g.origin[d];
-    if (x > g.upper[d]) x = g.upper[d];
+    if (!kInBox) x = fmin(fmax(x, g.origin[d]), g.upper[d]);
     out[d] = g.pow2[d] ? (x - g.origin[d]) * g.inv_h[d] : (x - g.origin[d]) / g.h[d];
   }
 }
 
-// ScalarSampler::sample from precomputed base coordinates: identical
-// arithmetic to sample() above, the division hoisted out.
-template <typename T>
+// Element `off` bytes past `vals`.  kOff32: every byte offset into the array
+// fits in 32 bits (the host checks the array size), so the load takes the
+// uniform base plus a 32-bit lane offset instead of a 64-bit address per lane.
+template <typename T, bool kOff32>
+__device__ __forceinline__ T load_elem(const T* __restrict__ vals, long long idx) {
+  if (kOff32)
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(vals) + (unsigned)idx * (unsigned)sizeof(T));
+  return vals[idx];
+}
+
+// ScalarSampler::sample (src/sampling.rs:74-122) from precomputed base
+// coordinates: comp 0..2 = face lattice of that axis, 3 = cell centres; f64
+// arithmetic on T samples, in the reference's weight and summation order.  The clamp of q to
+// [0, top] is written as max/min after a NaN test: the reference flags only a
+// NaN q (an infinite one clamps to a finite end), and the two forms agree
+// elsewhere up to the sign of a zero q, which only changes zero-weight terms.
+// t = q - floor(q) equals q - (double)lo exactly.  For T = float the corner
+// minimum and maximum are taken in float (exact: the widening is monotone;
+// they matter only when every corner is finite, else the sum is not).
+template <typename T, bool kOff32 = false>
 __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const T* __restrict__ vals,
                                               const double base[3], double& out) {
   const int n[3] = {g.nx, g.ny, g.nz};
@@ -89,12 +61,12 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
     const double off = (comp == d) ? 0.0 : 0.5;
     double q = base[d] - off;
     const double top = (double)(dims[d] - 1);
-    if (q < 0.0) q = 0.0;
-    if (q > top) q = top;
-    if (!isfinite(q)) return kFlagArithmetic;
-    lo[d] = (int)floor(q);
+    if (isnan(q)) return kFlagArithmetic;
+    q = fmin(fmax(q, 0.0), top);
+    const double fl = floor(q);
+    lo[d] = (int)fl;
     hi[d] = lo[d] + 1 < dims[d] - 1 ? lo[d] + 1 : dims[d] - 1;
-    t[d] = q - (double)lo[d];
+    t[d] = q - fl;
   }
   // corner (x, y, z) sits at c0 + {0, ox} + {0, oy} + {0, oz}: two
   // integer multiplies per sample instead of four (hi[d] - lo[d] is 0 or 1)
@@ -103,15 +75,33 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
   const int ox = hi[0] - lo[0];
   const int oy = hi[1] != lo[1] ? sy : 0;
   const int oz = hi[2] != lo[2] ? sz : 0;
-  double vals8[8];
+  T raw[8];
 #pragma unroll
   for (int z = 0; z < 2; ++z)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int x = 0; x < 2; ++x)
-        vals8[x + 2 * y + 4 * z] = (double)vals[c0 + (x ? ox : 0) + (y ? oy : 0) + (z ? oz : 0)];
-  double result = 0.0, minimum = INFINITY, maximum = -INFINITY;
+        raw[x + 2 * y + 4 * z] = load_elem<T, kOff32>(vals, c0 + (x ? ox : 0) + (y ? oy : 0) + (z ? oz : 0));
+  double result = 0.0, minimum, maximum;
+  if (sizeof(T) == 4) {
+    T lo4 = raw[0], hi4 = raw[0];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) {
+      lo4 = fminf(lo4, raw[q]);
+      hi4 = fmaxf(hi4, raw[q]);
+    }
+    minimum = (double)lo4;
+    maximum = (double)hi4;
+  } else {
+    minimum = INFINITY;
+    maximum = -INFINITY;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      minimum = fmin(minimum, (double)raw[q]);
+      maximum = fmax(maximum, (double)raw[q]);
+    }
+  }
 #pragma unroll
   for (int z = 0; z < 2; ++z)
 #pragma unroll
@@ -121,10 +111,7 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
         double weight = (x ? t[0] : 1.0 - t[0]);
         weight = weight * (y ? t[1] : 1.0 - t[1]);
         weight = weight * (z ? t[2] : 1.0 - t[2]);
-        const double value = vals8[x + 2 * y + 4 * z];
-        minimum = fmin(minimum, value);
-        maximum = fmax(maximum, value);
-        result += weight * value;
+        result += weight * (double)raw[x + 2 * y + 4 * z];
       }
   if (!isfinite(result)) return kFlagArithmetic;
   if (result < minimum) result = minimum;
@@ -134,35 +121,32 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
 }
 
 // VelocitySampler::backtrace / departure (src/sampling.rs:151-175).
-template <typename T>
+// The clamps to the box follow a finite test, so max/min is the same clamp.
+template <typename T, bool kOff32 = false>
 __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restrict__ u,
                                               const T* __restrict__ v, const T* __restrict__ w,
                                               const double pos[3], double dt, double out[3]) {
   double vel[3], mid[3], base[3];
   base_coords(g, pos, base);
-  unsigned f = sample_at<T>(g, 0, u, base, vel[0]) | sample_at<T>(g, 1, v, base, vel[1]) |
-               sample_at<T>(g, 2, w, base, vel[2]);
+  unsigned f = sample_at<T, kOff32>(g, 0, u, base, vel[0]) | sample_at<T, kOff32>(g, 1, v, base, vel[1]) |
+               sample_at<T, kOff32>(g, 2, w, base, vel[2]);
   if (f) return f;
   const double half = 0.5 * dt;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double value = pos[d] - half * vel[d];
+    const double value = pos[d] - half * vel[d];
     if (!isfinite(value)) return kFlagArithmetic;
-    if (value < g.origin[d]) value = g.origin[d];
-    if (value > g.upper[d]) value = g.upper[d];
-    mid[d] = value;
+    mid[d] = fmin(fmax(value, g.origin[d]), g.upper[d]);
   }
-  base_coords(g, mid, base);
-  f = sample_at<T>(g, 0, u, base, vel[0]) | sample_at<T>(g, 1, v, base, vel[1]) |
-      sample_at<T>(g, 2, w, base, vel[2]);
+  base_coords<true>(g, mid, base);
+  f = sample_at<T, kOff32>(g, 0, u, base, vel[0]) | sample_at<T, kOff32>(g, 1, v, base, vel[1]) |
+      sample_at<T, kOff32>(g, 2, w, base, vel[2]);
   if (f) return f;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double value = pos[d] - dt * vel[d];
+    const double value = pos[d] - dt * vel[d];
     if (!isfinite(value)) return kFlagArithmetic;
-    if (value < g.origin[d]) value = g.origin[d];
-    if (value > g.upper[d]) value = g.upper[d];
-    out[d] = value;
+    out[d] = fmin(fmax(value, g.origin[d]), g.upper[d]);
   }
   return 0;
 }
@@ -177,7 +161,7 @@ constexpr int kAdvectWaves = RH_ADVECT_WAVES;
 // advect_velocity (src/advection.rs:28-68).  blockIdx.y = component.  Wall
 // normal faces are exactly zero; every other face samples its own component
 // at the RK2 departure of its own face position (src/geometry.rs:240-248).
-template <typename T>
+template <typename T, bool kOff32>
 __device__ __forceinline__ void
 b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
@@ -202,11 +186,11 @@ b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T*
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
-      unsigned f = backtrace<T>(g, u, v, w, pos, dt, dep);
+      unsigned f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
       if (!f) {
         double base[3];
-        base_coords(g, dep, base);
-        f = sample_at<T>(g, a, src, base, value);
+        base_coords<true>(g, dep, base);
+        f = sample_at<T, kOff32>(g, a, src, base, value);
       }
       if (f) {
         bad |= f;
@@ -217,12 +201,12 @@ b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T*
   }
   if (bad) atomicOr(flags, bad);
 }
-template <typename T>
+template <typename T, bool kOff32>
 __global__ void __launch_bounds__(kBlock, kAdvectWaves)
 k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
                   T* __restrict__ ow, unsigned* __restrict__ flags) {
-  b_advect_velocity<T>(hw_block(), g, u, v, w, dt, ou, ov, ow, flags);
+  b_advect_velocity<T, kOff32>(hw_block(), g, u, v, w, dt, ou, ov, ow, flags);
 }
 
 // advect_tracer (src/advection.rs:72-102), every cell evaluated (the
@@ -243,7 +227,7 @@ k_advect_tracer(DevGrid g, const float* __restrict__ old, const float* __restric
       unsigned f = backtrace<float>(g, u, v, w, pos, dt, dep);
       if (!f) {
         double base[3];
-        base_coords(g, dep, base);
+        base_coords<true>(g, dep, base);
         f = sample_at<float>(g, 3, old, base, value);
       }
       if (f) {
@@ -559,8 +543,12 @@ void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w
                    void* ou, void* ov, void* ow, unsigned* flags, hipStream_t s) {
   const int rows = (g.ny + 1) * (g.nz + 1);
   dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
-  launch(k_advect_velocity<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
-                                              (T*)ou, (T*)ov, (T*)ow, flags);
+  if (face_offsets_32(g, sizeof(T)))
+    launch(k_advect_velocity<T, true>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
+                                                  (T*)ou, (T*)ov, (T*)ow, flags);
+  else
+    launch(k_advect_velocity<T, false>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
+                                                   (T*)ou, (T*)ov, (T*)ow, flags);
 }
 
 template <typename T>
@@ -667,10 +655,11 @@ kb_validate(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int
 // through the argument record costs registers, and under the hint the
 // compiler spilled to scratch (measured 0.9 ms slower per 256^3 step than
 // 171 VGPRs at 2 waves per SIMD).
+template <bool kOff32>
 __global__ void __launch_bounds__(kBlock)
 kb_advect(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BFaceArgs)
-  b_advect_velocity<float>(vb, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow, a.flags);
+  b_advect_velocity<float, kOff32>(vb, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow, a.flags);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_rhs(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
@@ -715,9 +704,12 @@ void launch_b_validate(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks)
     kb_validate<<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
 }
-void launch_b_advect(const BatchLaunch& bl, hipStream_t s) {
-  if (bl.blocks)
-    kb_advect<<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+void launch_b_advect(const BatchLaunch& bl, hipStream_t s, bool off32) {
+  if (!bl.blocks) return;
+  if (off32)
+    kb_advect<true><<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+  else
+    kb_advect<false><<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
 }
 void launch_b_rhs(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks) kb_rhs<<<bl.blocks, block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
