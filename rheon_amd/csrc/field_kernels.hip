@@ -49,19 +49,23 @@ __device__ __forceinline__ T load_elem(const T* __restrict__ vals, long long idx
 // t = q - floor(q) equals q - (double)lo exactly.  For T = float the corner
 // minimum and maximum are taken in float (exact: the widening is monotone;
 // they matter only when every corner is finite, else the sum is not).
+// Branch-free: a failing sample still completes (max/min maps a NaN q to 0,
+// so every corner index stays inside the array) and returns the flag; the
+// caller discards the value of a flagged face.
 template <typename T, bool kOff32 = false>
 __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const T* __restrict__ vals,
                                               const double base[3], double& out) {
   const int n[3] = {g.nx, g.ny, g.nz};
   int lo[3], hi[3], dims[3];
   double t[3];
+  bool bad = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     dims[d] = n[d] + (comp == d ? 1 : 0);
     const double off = (comp == d) ? 0.0 : 0.5;
     double q = base[d] - off;
     const double top = (double)(dims[d] - 1);
-    if (isnan(q)) return kFlagArithmetic;
+    bad = bad || isnan(q);
     q = fmin(fmax(q, 0.0), top);
     const double fl = floor(q);
     lo[d] = (int)fl;
@@ -113,15 +117,18 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
         weight = weight * (z ? t[2] : 1.0 - t[2]);
         result += weight * (double)raw[x + 2 * y + 4 * z];
       }
-  if (!isfinite(result)) return kFlagArithmetic;
+  bad = bad || !isfinite(result);
   if (result < minimum) result = minimum;
   if (result > maximum) result = maximum;
   out = result;
-  return 0;
+  return bad ? kFlagArithmetic : 0u;
 }
 
 // VelocitySampler::backtrace / departure (src/sampling.rs:151-175).
-// The clamps to the box follow a finite test, so max/min is the same clamp.
+// The clamps to the box follow a finite test, so max/min is the same clamp
+// for every accepted position.  Branch-free like sample_at: a NaN or
+// infinite position is flagged and max/min puts it on the box (so later
+// samples stay in bounds); the departure of a flagged face is discarded.
 template <typename T, bool kOff32 = false>
 __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restrict__ u,
                                               const T* __restrict__ v, const T* __restrict__ w,
@@ -130,31 +137,32 @@ __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restr
   base_coords(g, pos, base);
   unsigned f = sample_at<T, kOff32>(g, 0, u, base, vel[0]) | sample_at<T, kOff32>(g, 1, v, base, vel[1]) |
                sample_at<T, kOff32>(g, 2, w, base, vel[2]);
-  if (f) return f;
   const double half = 0.5 * dt;
+  bool bad = false;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double value = pos[d] - half * vel[d];
-    if (!isfinite(value)) return kFlagArithmetic;
+    bad = bad || !isfinite(value);
     mid[d] = fmin(fmax(value, g.origin[d]), g.upper[d]);
   }
   base_coords<true>(g, mid, base);
-  f = sample_at<T, kOff32>(g, 0, u, base, vel[0]) | sample_at<T, kOff32>(g, 1, v, base, vel[1]) |
-      sample_at<T, kOff32>(g, 2, w, base, vel[2]);
-  if (f) return f;
+  f |= sample_at<T, kOff32>(g, 0, u, base, vel[0]) | sample_at<T, kOff32>(g, 1, v, base, vel[1]) |
+       sample_at<T, kOff32>(g, 2, w, base, vel[2]);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     const double value = pos[d] - dt * vel[d];
-    if (!isfinite(value)) return kFlagArithmetic;
+    bad = bad || !isfinite(value);
     out[d] = fmin(fmax(value, g.origin[d]), g.upper[d]);
   }
-  return 0;
+  return f | (bad ? kFlagArithmetic : 0u);
 }
 
-// Minimum waves per SIMD requested for the advection kernels (their f64
-// sampling is latency-bound; more waves in flight hide it).
+// Minimum waves per SIMD requested for the advection kernels.  The
+// branch-free sampling keeps a face's 24 gathers of one RK2 stage in flight
+// at once; 3 waves (up to 168 VGPRs) hold them without spilling, which
+// measured faster than 4 waves with spills.
 #ifndef RH_ADVECT_WAVES
-#define RH_ADVECT_WAVES 4
+#define RH_ADVECT_WAVES 3
 #endif
 constexpr int kAdvectWaves = RH_ADVECT_WAVES;
 
@@ -187,11 +195,9 @@ b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T*
       for (int d = 0; d < 3; ++d)
         pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
       unsigned f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
-      if (!f) {
-        double base[3];
-        base_coords<true>(g, dep, base);
-        f = sample_at<T, kOff32>(g, a, src, base, value);
-      }
+      double base[3];
+      base_coords<true>(g, dep, base);
+      f |= sample_at<T, kOff32>(g, a, src, base, value);
       if (f) {
         bad |= f;
         continue;
