@@ -247,7 +247,9 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
   (void)c;
   const int J = j, K = k, I0 = i0 >> 1;
   float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
+  // dz rolled: the two rows of one dz in flight at a time, so the pass runs
+  // at higher occupancy
+#pragma unroll 1
   for (int dz = 0; dz < 2; ++dz)
 #pragma unroll
     for (int dy = 0; dy < 2; ++dy) {
