@@ -82,6 +82,20 @@ __device__ __forceinline__ int quad_mask(const unsigned char* mask, unsigned m4,
   return mask ? (int)((m4 >> (8 * u)) & 63u) : box_mask6(nx, ny, nz, i, j, k);
 }
 
+// Row flags of the level-0 obstacle mask, stored after it (one byte per
+// (j, k) row at mask + nx ny nz): kRowPlain = every cell of the row is fluid
+// with the plain box's open faces; kRowPlainNear = kRowPlain holds for the
+// row and its four neighbour rows (j +- 1, k +- 1, clamped at walls).  A row
+// with the flag takes the box form of a stencil (no mask load), which gives
+// the same values: its masks are box_mask6, and at walls the box form's
+// clamped neighbour is the cell itself, as close_faces makes it.
+constexpr unsigned kRowPlain = 1u, kRowPlainNear = 2u;
+__device__ __forceinline__ const unsigned char* row_mask(const unsigned char* mask, int nx, int ny, int nz,
+                                                         int row, unsigned flag) {
+  if (!mask) return nullptr;
+  return (mask[(long long)nx * ny * nz + row] & flag) ? nullptr : mask;
+}
+
 // Closed faces take the cell's own value (contribution exactly 0).
 template <typename T>
 __device__ __forceinline__ void close_faces(Quad<T>& Q, int u, int m) {

@@ -42,12 +42,13 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   }
   double bb[4];
   load4(b, c, bb);
-  const unsigned m4 = g.mask ? *reinterpret_cast<const unsigned*>(g.mask + c) : 0u;
+  const unsigned char* const mrow = row_mask(g.mask, g.nx, g.ny, g.nz, row, kRowPlain);
+  const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
   float rb[4], rx[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int mk = quad_mask(g.mask, m4, g.nx, g.ny, g.nz, i0 + u, j, k, u);
-    if (g.mask) close_faces(Q, u, mk);
+    const int mk = quad_mask(mrow, m4, g.nx, g.ny, g.nz, i0 + u, j, k, u);
+    if (mrow) close_faces(Q, u, mk);
     const double v = quad_apply_d(Q, u, g);
     if (!isfinite(v)) m[1] = 1.0;
     const double r = bb[u] - v;
@@ -90,12 +91,13 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
       load4(p, c, pv);
     }
   }
-  const unsigned m4 = L.mask ? *reinterpret_cast<const unsigned*>(L.mask + c) : 0u;
+  const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlain);
+  const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
   float xn[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int mk = quad_mask(L.mask, m4, L.nx, L.ny, L.nz, i0 + u, j, k, u);
-    if (L.mask) close_faces(Q, u, mk);
+    const int mk = quad_mask(mrow, m4, L.nx, L.ny, L.nz, i0 + u, j, k, u);
+    if (mrow) close_faces(Q, u, mk);
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
     xn[u] = Q.c[u] + (float)(tau * L.odiag[mk]) * (bb[u] - ax);
   }
@@ -175,12 +177,13 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
   const int km = k > 0 ? k - 1 : k, kp = k + 1 < L.nz ? k + 1 : k;
   const int il = i0 > 0 ? i0 - 1 : i0, ir = i0 + 4 < L.nx ? i0 + 4 : i0 + 3;
   int mc[4], mym[4], myp[4], mzm[4], mzp[4], ml, mr;
-  if (L.mask) {
-    const unsigned a = *reinterpret_cast<const unsigned*>(L.mask + c);
-    const unsigned ym4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (jm - j) * sy));
-    const unsigned yp4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (jp - j) * sy));
-    const unsigned zm4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (km - k) * sz));
-    const unsigned zp4 = *reinterpret_cast<const unsigned*>(L.mask + (c + (kp - k) * sz));
+  const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlainNear);
+  if (mrow) {
+    const unsigned a = *reinterpret_cast<const unsigned*>(mrow + c);
+    const unsigned ym4 = *reinterpret_cast<const unsigned*>(mrow + (c + (jm - j) * sy));
+    const unsigned yp4 = *reinterpret_cast<const unsigned*>(mrow + (c + (jp - j) * sy));
+    const unsigned zm4 = *reinterpret_cast<const unsigned*>(mrow + (c + (km - k) * sz));
+    const unsigned zp4 = *reinterpret_cast<const unsigned*>(mrow + (c + (kp - k) * sz));
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       mc[u] = (int)((a >> (8 * u)) & 63u);
@@ -189,8 +192,8 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
       mzm[u] = (int)((zm4 >> (8 * u)) & 63u);
       mzp[u] = (int)((zp4 >> (8 * u)) & 63u);
     }
-    ml = L.mask[c + (il - i0)] & 63;
-    mr = L.mask[c + (ir - i0)] & 63;
+    ml = mrow[c + (il - i0)] & 63;
+    mr = mrow[c + (ir - i0)] & 63;
   } else {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -224,7 +227,7 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
   float xn[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    if (L.mask) close_faces(Q, u, mc[u]);
+    if (mrow) close_faces(Q, u, mc[u]);
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
     xn[u] = Q.c[u] + s2[mc[u]] * (bb[u] - ax);
   }
@@ -266,10 +269,11 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
 #pragma unroll
         for (int u = 0; u < 4; ++u) r[u] = bb[u] - quad_apply_frac(F, Q, u, frac6_of(R, u, i0 + u + 1 < F.nx));
       } else {
-        const unsigned m4 = F.mask ? *reinterpret_cast<const unsigned*>(F.mask + fc) : 0u;
+        const unsigned char* const mrow = row_mask(F.mask, F.nx, F.ny, F.nz, fj + F.ny * fk, kRowPlain);
+        const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + fc) : 0u;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          if (F.mask) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
+          if (mrow) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
           r[u] = bb[u] - quad_apply_f(Q, u, w0, w1, w2);
         }
       }
@@ -322,12 +326,13 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
     const unsigned oc = *reinterpret_cast<const unsigned short*>(Cl.octant + I0 + rows[0]);
     // fluid flags of the four fine cells (level-0 mask, fraction words, or all)
     unsigned fl = 0xFu;
-    if (F.mask) {
-      const unsigned m4 = *reinterpret_cast<const unsigned*>(F.mask + c);
+    const unsigned char* const mrow = row_mask(F.mask, F.nx, F.ny, F.nz, row, kRowPlain);
+    if (mrow) {
+      const unsigned m4 = *reinterpret_cast<const unsigned*>(mrow + c);
       fl = 0;
 #pragma unroll
       for (int u = 0; u < 4; ++u) fl |= ((m4 >> (8 * u)) & kFluidBit) ? 1u << u : 0u;
-    } else if (F.frac) {
+    } else if (!F.mask && F.frac) {
       const uint4 f4 = *reinterpret_cast<const uint4*>(F.frac + c);
       fl = ((f4.x & kFracFluid) ? 1u : 0u) | ((f4.y & kFracFluid) ? 2u : 0u) | ((f4.z & kFracFluid) ? 4u : 0u) |
            ((f4.w & kFracFluid) ? 8u : 0u);

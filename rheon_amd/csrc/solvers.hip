@@ -116,6 +116,37 @@ void Solver::release_mask() {
   ++generation_;
 }
 
+// Row flags stored on the device right after the level-0 mask (one byte per
+// (j, k) row): bit 0 = every cell of the row is fluid with the plain box's
+// open faces, bit 1 = bit 0 holds for the row and its four neighbour rows
+// (clamped at walls).  The four-cell kernels skip the mask load on such rows
+// (kRowPlain / kRowPlainNear in mg_quad.hpp).
+static std::vector<uint8_t> mask_row_flags(const rh_grid& g, const std::vector<uint8_t>& mask) {
+  const int nx = (int)g.counts[0], ny = (int)g.counts[1], nz = (int)g.counts[2];
+  std::vector<uint8_t> plain((size_t)ny * nz), out((size_t)ny * nz);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      const size_t row = (size_t)j + (size_t)ny * k;
+      bool ok = true;
+      for (int i = 0; i < nx && ok; ++i) {
+        const int box = (i > 0) | ((i + 1 < nx) << 1) | ((j > 0) << 2) | ((j + 1 < ny) << 3) | ((k > 0) << 4) |
+                        ((k + 1 < nz) << 5);
+        ok = mask[(size_t)i + (size_t)nx * row] == (uint8_t)(box | kFluidBit);
+      }
+      plain[row] = ok ? 1 : 0;
+    }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      const size_t row = (size_t)j + (size_t)ny * k;
+      const int jm = j > 0 ? j - 1 : j, jp = j + 1 < ny ? j + 1 : j;
+      const int km = k > 0 ? k - 1 : k, kp = k + 1 < nz ? k + 1 : k;
+      const bool near = plain[row] && plain[(size_t)jm + (size_t)ny * k] && plain[(size_t)jp + (size_t)ny * k] &&
+                        plain[(size_t)j + (size_t)ny * km] && plain[(size_t)j + (size_t)ny * kp];
+      out[row] = (uint8_t)(plain[row] | (near ? 2 : 0));
+    }
+  return out;
+}
+
 rh_status Solver::set_solid(const uint8_t* solid) {
   if (!solid) {
     release_mask();
@@ -125,8 +156,10 @@ rh_status Solver::set_solid(const uint8_t* solid) {
   int gauge = 0;
   rh_status st = build_obstacle_mask(grid_, solid, mask, &gauge);
   if (st) return st;
-  if (!d_mask_) RH_HIP(hipMalloc(&d_mask_, mask.size()));
+  const std::vector<uint8_t> rows = mask_row_flags(grid_, mask);
+  if (!d_mask_) RH_HIP(hipMalloc(&d_mask_, mask.size() + rows.size()));
   RH_HIP(hipMemcpy(d_mask_, mask.data(), mask.size(), hipMemcpyHostToDevice));
+  RH_HIP(hipMemcpy(d_mask_ + mask.size(), rows.data(), rows.size(), hipMemcpyHostToDevice));
   hmask_.swap(mask);
   gauge_ = gauge;
   ++generation_;
@@ -311,7 +344,7 @@ uint64_t MgSolver::plan_bytes(const rh_grid& grid, const rh_solver_config& confi
     const uint64_t cells = (uint64_t)dims[l][0] * dims[l][1] * dims[l][2];
     total += 3 * ((cells * s + 255) / 256 * 256);
     if (l > 0) total += (cells + 63) / 64 * 256 + (cells + 255) / 256 * 256;  // fractions, octants
-    if (l == 0) total += cells;                                                // obstacle mask
+    if (l == 0) total += cells + (uint64_t)dims[0][1] * dims[0][2];            // obstacle mask, row flags
   }
   const uint64_t coarse = (uint64_t)dims[levels - 1][0] * dims[levels - 1][1] * dims[levels - 1][2];
   // dense inverse, tables, the tail's per-cell 1/D (at most its LDS), scratch

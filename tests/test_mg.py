@@ -192,20 +192,37 @@ def test_device_mg_fourier_projection_divergence():
     assert R.divergence_max(spec, out) < 64 * 64 * 6e-8
 
 
+def wall_blocks(spec):
+    """Solid blocks against the walls and one in the interior: rows next to a
+    wall or a block are not plain box rows, their neighbours are plain rows
+    with non-plain neighbour rows, and one block sits in a single z-plane."""
+    nx, ny, nz = spec.counts
+    s = np.zeros((nz, ny, nx), np.uint8)
+    s[:, :3, :4] = 1             # along the x- / y- edge, every plane
+    s[nz // 2, ny // 2, nx - 4:] = 1   # one row's end against the x+ wall
+    s[1:3, ny - 2:, 8:12] = 1    # against the y+ wall near the z- wall
+    s[nz // 3, ny // 3:ny // 3 + 2, nx // 2:nx // 2 + 4] = 1
+    return s.ravel()
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("counts,sphere", [((64, 32, 48), False), ((64, 32, 48), True), ((40, 24, 16), False),
                                            ((40, 24, 16), True), ((256, 256, 128), False),
-                                           ((256, 256, 128), True)])
+                                           ((256, 256, 128), True), ((64, 32, 48), "walls"),
+                                           ((40, 24, 16), "walls")])
 def test_device_mg_quad_kernels_match_scalar(counts, sphere):
     """The four-cells-per-lane FP32 kernels give the scalar kernels' values
     bit for bit (same per-cell arithmetic).  256x256x128: its first coarse
     level has 2^20 cells, so it also runs the four-cell kernels on a level
-    with face fractions (with the sphere)."""
+    with face fractions (with the sphere).  On obstacle grids the four-cell
+    kernels take the box form on rows flagged plain (no mask load) and the
+    mask form elsewhere; "walls" puts solids against the walls so that
+    both forms, and rows whose neighbour rows differ, occur."""
     import rheon_amd as R
     from rheon_amd.scenes import centred_sphere, close_obstacle_faces
     spec = GridSpec((counts), tuple(1.0 / n for n in counts))
     vel = fourier_modes(spec, 9, modes=8, kmax=6, dtype=np.float32)
-    solid = centred_sphere(spec, 0.25) if sphere else None
+    solid = wall_blocks(spec) if sphere == "walls" else (centred_sphere(spec, 0.25) if sphere else None)
     if solid is not None:
         vel = close_obstacle_faces(spec, solid, vel)
     dt = courant_dt(spec, vel)
