@@ -76,10 +76,23 @@ __device__ __forceinline__ void load_quad(const T* __restrict__ x, int nx, int n
   Q.xp[3] = right;
 }
 
-// Open-face bits of cell i0+u: obstacle mask byte or the box rule.
-__device__ __forceinline__ int quad_mask(const unsigned char* mask, unsigned m4, int nx, int ny, int nz,
-                                         int i, int j, int k, int u) {
-  return mask ? (int)((m4 >> (8 * u)) & 63u) : box_mask6(nx, ny, nz, i, j, k);
+// box_mask6's y/z bits of row (j, k).
+__device__ __forceinline__ int box_yz(int ny, int nz, int j, int k) {
+  return ((j > 0) << 2) | ((j + 1 < ny) << 3) | ((k > 0) << 4) | ((k + 1 < nz) << 5);
+}
+
+// box_mask6 of cell i0 + u of a four-cell chunk (nx % 4 == 0, i0 % 4 == 0)
+// in a row with y/z bits yz: cells 1 and 2 have both x faces open; only cell
+// 0 (x- at i0 = 0) and cell 3 (x+ at i0 + 4 = nx) can touch an x wall.
+__device__ __forceinline__ int quad_box_mask(int nx, int i0, int yz, int u) {
+  const int xb = u == 0 ? ((i0 > 0) | 2) : (u == 3 ? (1 | ((i0 + 4 < nx) << 1)) : 3);
+  return yz | xb;
+}
+
+// Open-face bits of cell i0+u: obstacle mask byte or the box rule (yz =
+// box_yz of the chunk's row).
+__device__ __forceinline__ int quad_mask(const unsigned char* mask, unsigned m4, int nx, int i0, int yz, int u) {
+  return mask ? (int)((m4 >> (8 * u)) & 63u) : quad_box_mask(nx, i0, yz, u);
 }
 
 // Row flags of the level-0 obstacle mask, stored after it (one byte per

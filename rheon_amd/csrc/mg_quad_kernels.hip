@@ -44,10 +44,11 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   load4(b, c, bb);
   const unsigned char* const mrow = row_mask(g.mask, g.nx, g.ny, g.nz, row, kRowPlain);
   const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
+  const int yz = box_yz(g.ny, g.nz, j, k);
   float rb[4], rx[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int mk = quad_mask(mrow, m4, g.nx, g.ny, g.nz, i0 + u, j, k, u);
+    const int mk = quad_mask(mrow, m4, g.nx, i0, yz, u);
     if (mrow) close_faces(Q, u, mk);
     const double v = quad_apply_d(Q, u, g);
     if (!isfinite(v)) m[1] = 1.0;
@@ -93,10 +94,11 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
   }
   const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlain);
   const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
+  const int yz = box_yz(L.ny, L.nz, j, k);
   float xn[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int mk = quad_mask(mrow, m4, L.nx, L.ny, L.nz, i0 + u, j, k, u);
+    const int mk = quad_mask(mrow, m4, L.nx, i0, yz, u);
     if (mrow) close_faces(Q, u, mk);
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
     xn[u] = Q.c[u] + (float)(tau * L.odiag[mk]) * (bb[u] - ax);
@@ -195,16 +197,20 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
     ml = mrow[c + (il - i0)] & 63;
     mr = mrow[c + (ir - i0)] & 63;
   } else {
+    // the box rule per row (quad_box_mask); the x-edge neighbours il = i0 - 1
+    // and ir = i0 + 4 are interior in x unless they clamp onto cells 0 / 3
+    const int yc = box_yz(L.ny, L.nz, j, k), yym = box_yz(L.ny, L.nz, jm, k), yyp = box_yz(L.ny, L.nz, jp, k);
+    const int yzm = box_yz(L.ny, L.nz, j, km), yzp = box_yz(L.ny, L.nz, j, kp);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      mc[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, k);
-      mym[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, jm, k);
-      myp[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, jp, k);
-      mzm[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, km);
-      mzp[u] = box_mask6(L.nx, L.ny, L.nz, i0 + u, j, kp);
+      mc[u] = quad_box_mask(L.nx, i0, yc, u);
+      mym[u] = quad_box_mask(L.nx, i0, yym, u);
+      myp[u] = quad_box_mask(L.nx, i0, yyp, u);
+      mzm[u] = quad_box_mask(L.nx, i0, yzm, u);
+      mzp[u] = quad_box_mask(L.nx, i0, yzp, u);
     }
-    ml = box_mask6(L.nx, L.ny, L.nz, il, j, k);
-    mr = box_mask6(L.nx, L.ny, L.nz, ir, j, k);
+    ml = i0 > 0 ? (yc | 3) : mc[0];
+    mr = i0 + 4 < L.nx ? (yc | 3) : mc[3];
   }
   // b0 -> x0 at every position
   const float xl = s1[ml] * Q.xm[0], xr = s1[mr] * Q.xp[3];
