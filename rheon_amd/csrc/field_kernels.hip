@@ -40,6 +40,12 @@ __device__ __forceinline__ T load_elem(const T* __restrict__ vals, long long idx
   return vals[idx];
 }
 
+// x / h[d]: the product with the exact reciprocal where h is a power of two
+// (the same value bit for bit), the quotient otherwise.
+__device__ __forceinline__ double over_h(const DevGrid& g, int d, double x) {
+  return g.pow2[d] ? x * g.inv_h[d] : x / g.h[d];
+}
+
 // ScalarSampler::sample (src/sampling.rs:74-122) from precomputed base
 // coordinates: comp 0..2 = face lattice of that axis, 3 = cell centres; f64
 // arithmetic on T samples, in the reference's weight and summation order.  The clamp of q to
@@ -461,9 +467,9 @@ b_divergence_max(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* 
       const int xu = i + (nx + 1) * row;
       const int yv = i + nx * (j + (ny + 1) * k);
       const int zw = i + nx * row;
-      const double value = ((double)u[xu + 1] - (double)u[xu]) / g.h[0] +
-                           ((double)v[yv + nx] - (double)v[yv]) / g.h[1] +
-                           ((double)w[zw + nx * ny] - (double)w[zw]) / g.h[2];
+      const double value = over_h(g, 0, (double)u[xu + 1] - (double)u[xu]) +
+                           over_h(g, 1, (double)v[yv + nx] - (double)v[yv]) +
+                           over_h(g, 2, (double)w[zw + nx * ny] - (double)w[zw]);
       if (!isfinite(value)) bad |= kFlagArithmetic;
       m[0] = fmax(m[0], fabs(value));
     }
@@ -527,7 +533,7 @@ k_divergence_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v, c
       const double u0 = (double)u[xu], u1 = (double)u[xu + 1];
       const double v0 = (double)v[yv], v1 = (double)v[yv + nx];
       const double w0 = (double)w[zw], w1 = (double)w[zw + nx * ny];
-      const double value = (u1 - u0) / g.h[0] + (v1 - v0) / g.h[1] + (w1 - w0) / g.h[2];
+      const double value = over_h(g, 0, u1 - u0) + over_h(g, 1, v1 - v0) + over_h(g, 2, w1 - w0);
       if (!isfinite(value)) bad |= kFlagArithmetic;
       m[0] = fmax(m[0], fabs(value));
       m[1] = fmax(m[1], fabs(u0));
