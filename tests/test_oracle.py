@@ -477,3 +477,30 @@ def test_steady_scene_golden_bits(n):
     xyz = [i % n, (i // n) % n, i // (n * n)]
     exact = sum(1.25 * acc[d] * xyz[d] * spec.spacing[d] for d in range(3))
     assert float(np.max(np.abs(r.pressure - exact))) == gold["perr"]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_velocity_rate_planted_maxima(dtype):
+    # src/simulation.rs:1415-1429: rate = sum_d max_i |f64(u_d[i])| / h_d over every
+    # face (walls included), d = 0, 1, 2.  Each component holds a planted extreme
+    # (u: a negative one, v: at the last face, w: at the first face) above a random
+    # background, so the maxima are known by construction; the spacings differ per
+    # axis, so a transposed h, a signed max or a max over all components fails.
+    g = O.Grid((7, 5, 3), (0.3, 0.125, 0.7), (-0.5, 1.0, 2.0))
+    rng = np.random.default_rng(5)
+    vel = [rng.uniform(-0.5, 0.5, g.face_len(d)).astype(dtype) for d in range(3)]
+    vel[0][17] = -9.5
+    vel[0][3] = 9.25
+    vel[1][-1] = 3.25
+    vel[2][0] = -0.75
+    vel[2][5] = 0.5
+    expect = 0.0
+    for m, h in zip((9.5, 3.25, 0.75), g.spacing):
+        expect += m / h
+    assert O.velocity_rate(g, vel) == expect
+    # independent numpy statement of the same sum
+    alt = sum(float(np.max(np.abs(v.astype(np.float64)))) / h for v, h in zip(vel, g.spacing))
+    assert O.velocity_rate(g, vel) == alt
+    # all-zero field: rate 0 (the Courant dt rule then takes the requested dt)
+    zero = [np.zeros(g.face_len(d), dtype) for d in range(3)]
+    assert O.velocity_rate(g, zero) == 0.0
