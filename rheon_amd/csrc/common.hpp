@@ -38,7 +38,12 @@ constexpr int kGroupsPerBlock = 2;
 // Validated box geometry and operator constants, passed by value.
 struct DevGrid {
   int nx, ny, nz;
-  int pad;
+  // Nonzero when every face and cell-centre position of the lattice maps
+  // back to its exact lattice coordinate: (o + (p + s) h - o) / h == p + s
+  // for every p in 0..n and s in {0, 1/2} on each axis (true for a
+  // power-of-two h and a modest origin).  Sampling at such a position has
+  // weights in {0, 1/2, 1} only.
+  int lattice;
   double h[3];
   double origin[3];
   double upper[3];
@@ -59,13 +64,16 @@ struct DevGrid {
 constexpr unsigned char kFluidBit = 64;
 
 // Every byte offset into each face array of g (elements of `elem` bytes)
-// fits in 32 bits, so kernels may address them from a uniform base.
+// fits in 32 bits, so kernels may address them from a uniform base; and
+// the flat index (i + fx (j + fy k)) of every face can be formed with
+// 24-bit multiplies (fx < 2^24 and fy fz <= 2^24 for every face lattice).
 inline bool face_offsets_32(const DevGrid& g, size_t elem) {
   const unsigned long long x = g.nx, y = g.ny, z = g.nz;
   unsigned long long m = (x + 1) * y * z;
   if (x * (y + 1) * z > m) m = x * (y + 1) * z;
   if (x * y * (z + 1) > m) m = x * y * (z + 1);
-  return m * elem <= (1ull << 32);
+  const unsigned long long yz = (y + 1) * (z + 1);
+  return m * elem <= (1ull << 32) && x + 1 < (1ull << 24) && yz <= (1ull << 24);
 }
 
 // Interior face of axis a with face lattice index (i, j, k): its upper cell

@@ -81,7 +81,10 @@ __device__ __forceinline__ unsigned sample_at(const DevGrid& g, int comp, const 
   // corner (x, y, z) sits at c0 + {0, ox} + {0, oy} + {0, oz}: two
   // integer multiplies per sample instead of four (hi[d] - lo[d] is 0 or 1)
   const int sy = dims[0], sz = dims[0] * dims[1];
-  const int c0 = (lo[2] * dims[1] + lo[1]) * dims[0] + lo[0];
+  // kOff32 also guarantees 24-bit operands (face_offsets_32): full-rate
+  // 24-bit multiplies give the same index
+  const int c0 = kOff32 ? (int)__umul24(__umul24(lo[2], dims[1]) + lo[1], dims[0]) + lo[0]
+                        : (lo[2] * dims[1] + lo[1]) * dims[0] + lo[0];
   const int ox = hi[0] - lo[0];
   const int oy = hi[1] != lo[1] ? sy : 0;
   const int oz = hi[2] != lo[2] ? sz : 0;
@@ -163,24 +166,111 @@ __device__ __forceinline__ unsigned backtrace(const DevGrid& g, const T* __restr
   return f | (bad ? kFlagArithmetic : 0u);
 }
 
+// First RK2 stage at a face position of a lattice grid (DevGrid::lattice,
+// f32 faces).  For the face (i, j, k) of axis a every base coordinate is an
+// exact lattice coordinate, so ScalarSampler::sample's weights are products
+// of 0, 1/2 and 1 (src/sampling.rs:89-113):
+//  * component a samples its own lattice point: weight 1 on the face itself
+//    and exactly +0 on the other seven corners, so the sum is 0.0 + u;
+//  * component c != a has t = 1/2 on axes a and c (lower corners p_a - 1 and
+//    p_c, both upper corners inside the support) and t = 0 on the third axis:
+//    four corners of weight exactly 1/4, taken in the reference's z, y, x
+//    order, and four of weight +0.
+// The terms of weight +0 are +-0 and leave the running sum unchanged: it
+// starts at +0.0 and a round-to-nearest sum is -0 only when both operands
+// are, so it never holds -0.  The clamp to the corners' [min, max] is the
+// identity: the k-th partial sum is at most round(k/4 * max) = k/4 * max
+// (representable for an f32 max) and, likewise, at least k/4 * min.  With
+// finite faces (validated before sampling) every sum is finite.  So the
+// value is the one the general sample returns, bit for bit.
+template <bool kOff32>
+__device__ __forceinline__ double lattice_half_sample(const DevGrid& g, int a, int c, const float* __restrict__ vals,
+                                                      int i, int j, int k) {
+  const int dx = g.nx + (c == 0), dy = g.ny + (c == 1);
+  const int p[3] = {i - (a == 0), j - (a == 1), k - (a == 2)};
+  const int c0 = kOff32 ? p[0] + (int)__umul24(dx, p[1] + (int)__umul24(dy, p[2])) : p[0] + dx * (p[1] + dy * p[2]);
+  const int lo_ax = a < c ? a : c, hi_ax = a < c ? c : a;
+  const int s_in = lo_ax == 0 ? 1 : dx;       // inner (lower-index) active axis
+  const int s_out = hi_ax == 1 ? dx : dx * dy;  // outer active axis
+  const double v0 = (double)load_elem<float, kOff32>(vals, c0);
+  const double v1 = (double)load_elem<float, kOff32>(vals, c0 + s_in);
+  const double v2 = (double)load_elem<float, kOff32>(vals, c0 + s_out);
+  const double v3 = (double)load_elem<float, kOff32>(vals, c0 + s_in + s_out);
+  double r = 0.0;
+  r += 0.25 * v0;
+  r += 0.25 * v1;
+  r += 0.25 * v2;
+  r += 0.25 * v3;
+  return r;
+}
+
+// backtrace from the face (i, j, k) of axis a at `pos`, with the first
+// stage's samples taken by lattice_half_sample (f32 faces, lattice grid);
+// the rest as backtrace().
+template <bool kOff32>
+__device__ __forceinline__ unsigned backtrace_lattice_face(const DevGrid& g, int a, const float* __restrict__ u,
+                                                           const float* __restrict__ v, const float* __restrict__ w,
+                                                           int i, int j, int k, const double pos[3], double dt,
+                                                           double out[3]) {
+  const float* comp[3] = {u, v, w};
+  double vel[3], mid[3], base[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c == a) {
+      const int fx = g.nx + (a == 0), fy = g.ny + (a == 1);
+      vel[c] = 0.0 + (double)load_elem<float, kOff32>(comp[c], i + fx * (j + fy * k));
+    } else {
+      vel[c] = lattice_half_sample<kOff32>(g, a, c, comp[c], i, j, k);
+    }
+  }
+  const double half = 0.5 * dt;
+  bool bad = false;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double value = pos[d] - half * vel[d];
+    bad = bad || !isfinite(value);
+    mid[d] = fmin(fmax(value, g.origin[d]), g.upper[d]);
+  }
+  base_coords<true>(g, mid, base);
+  unsigned f = sample_at<float, kOff32>(g, 0, u, base, vel[0]) | sample_at<float, kOff32>(g, 1, v, base, vel[1]) |
+               sample_at<float, kOff32>(g, 2, w, base, vel[2]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double value = pos[d] - dt * vel[d];
+    bad = bad || !isfinite(value);
+    out[d] = fmin(fmax(value, g.origin[d]), g.upper[d]);
+  }
+  return f | (bad ? kFlagArithmetic : 0u);
+}
+
 // Minimum waves per SIMD requested for the advection kernels.  The
 // branch-free sampling keeps a face's 24 gathers of one RK2 stage in flight
-// at once; 3 waves (up to 168 VGPRs) hold them without spilling, which
-// measured faster than 4 waves with spills.
+// at once; the general form at 3 waves (up to 168 VGPRs) holds them without
+// spilling, which measured faster than 4 waves with spills.  The lattice
+// form fits 4 waves (128 VGPRs): 1.00 -> 0.88 ms per 256^3 launch.
 #ifndef RH_ADVECT_WAVES
 #define RH_ADVECT_WAVES 3
 #endif
+#ifndef RH_ADVECT_LATTICE_WAVES
+#define RH_ADVECT_LATTICE_WAVES 4
+#endif
 constexpr int kAdvectWaves = RH_ADVECT_WAVES;
+constexpr int kAdvectLatticeWaves = RH_ADVECT_LATTICE_WAVES;
 
 // advect_velocity (src/advection.rs:28-68).  blockIdx.y = component.  Wall
 // normal faces are exactly zero; every other face samples its own component
 // at the RK2 departure of its own face position (src/geometry.rs:240-248).
-template <typename T, bool kOff32>
+// kLattice (f32 faces): 1 = the grid is a lattice grid, 2 = decided per
+// grid at run time (batched launches); the first RK2 stage is then taken by
+// backtrace_lattice_face, the same values with a fraction of the work.
+// Component a of block vb.
+template <typename T, bool kOff32, int kLattice = 0>
 __device__ __forceinline__ void
-b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
+b_advect_velocity(VBlock vb, int a, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
                   T* __restrict__ ow, unsigned* __restrict__ flags) {
-  const int a = blockIdx.y;
+  static_assert(kLattice == 0 || sizeof(T) == 4, "lattice first stage is stated for f32 faces");
+  const bool lattice = kLattice == 1 || (kLattice == 2 && g.lattice);
   const T* src = a == 0 ? u : (a == 1 ? v : w);
   T* dst = a == 0 ? ou : (a == 1 ? ov : ow);
   const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
@@ -200,7 +290,15 @@ b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T*
 #pragma unroll
       for (int d = 0; d < 3; ++d)
         pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
-      unsigned f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
+      unsigned f;
+      if constexpr (kLattice != 0) {
+        if (lattice)
+          f = backtrace_lattice_face<kOff32>(g, a, u, v, w, i, j, k, pos, dt, dep);
+        else
+          f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
+      } else {
+        f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
+      }
       double base[3];
       base_coords<true>(g, dep, base);
       f |= sample_at<T, kOff32>(g, a, src, base, value);
@@ -213,12 +311,12 @@ b_advect_velocity(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T*
   }
   if (bad) atomicOr(flags, bad);
 }
-template <typename T, bool kOff32>
-__global__ void __launch_bounds__(kBlock, kAdvectWaves)
+template <typename T, bool kOff32, int kLattice = 0>
+__global__ void __launch_bounds__(kBlock, kLattice == 1 ? kAdvectLatticeWaves : kAdvectWaves)
 k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
                   T* __restrict__ ow, unsigned* __restrict__ flags) {
-  b_advect_velocity<T, kOff32>(hw_block(), g, u, v, w, dt, ou, ov, ow, flags);
+  b_advect_velocity<T, kOff32, kLattice>(hw_block(), (int)blockIdx.y, g, u, v, w, dt, ou, ov, ow, flags);
 }
 
 // advect_tracer (src/advection.rs:72-102), every cell evaluated (the
@@ -555,7 +653,19 @@ void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w
                    void* ou, void* ov, void* ow, unsigned* flags, hipStream_t s) {
   const int rows = (g.ny + 1) * (g.nz + 1);
   dim3 grid(grid_blocks(rows), 3), block(kLanesX, kRowsPerBlock);
-  if (face_offsets_32(g, sizeof(T)))
+  const bool off32 = face_offsets_32(g, sizeof(T));
+  if constexpr (sizeof(T) == 4) {
+    if (g.lattice) {
+      if (off32)
+        launch(k_advect_velocity<T, true, 1>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
+               (T*)ou, (T*)ov, (T*)ow, flags);
+      else
+        launch(k_advect_velocity<T, false, 1>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
+               (T*)ou, (T*)ov, (T*)ow, flags);
+      return;
+    }
+  }
+  if (off32)
     launch(k_advect_velocity<T, true>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt,
                                                   (T*)ou, (T*)ov, (T*)ow, flags);
   else
@@ -667,11 +777,12 @@ kb_validate(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int
 // through the argument record costs registers, and under the hint the
 // compiler spilled to scratch (measured 0.9 ms slower per 256^3 step than
 // 171 VGPRs at 2 waves per SIMD).
-template <bool kOff32>
+template <bool kOff32, int kLattice>
 __global__ void __launch_bounds__(kBlock)
 kb_advect(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BFaceArgs)
-  b_advect_velocity<float, kOff32>(vb, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow, a.flags);
+  b_advect_velocity<float, kOff32, kLattice>(vb, (int)blockIdx.y, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow,
+                                             a.flags);
 }
 __global__ void __launch_bounds__(kBlock)
 kb_rhs(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
@@ -716,12 +827,17 @@ void launch_b_validate(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks)
     kb_validate<<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
 }
-void launch_b_advect(const BatchLaunch& bl, hipStream_t s, bool off32) {
+void launch_b_advect(const BatchLaunch& bl, hipStream_t s, bool off32, bool lattice) {
   if (!bl.blocks) return;
-  if (off32)
-    kb_advect<true><<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+  const BFaceArgs* A = (const BFaceArgs*)bl.args;
+  if (off32 && lattice)
+    kb_advect<true, 2><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
+  else if (off32)
+    kb_advect<true, 0><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
+  else if (lattice)
+    kb_advect<false, 2><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
   else
-    kb_advect<false><<<face_batch_grid(bl), block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);
+    kb_advect<false, 0><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
 }
 void launch_b_rhs(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks) kb_rhs<<<bl.blocks, block_shape(), 0, s>>>((const BFaceArgs*)bl.args, bl.prefix, bl.n);

@@ -109,6 +109,21 @@ DevGrid make_dev_grid(const rh_grid& g, double density) {
     d.inv_h[a] = inv;
     d.pow2[a] = (m == 0.5 && std::isnormal(inv) && g.spacing[a] * inv == 1.0) ? 1 : 0;
   }
+  // lattice: the base coordinate of field kernels (field_kernels.hip,
+  // base_coords) recovers every lattice coordinate p + s exactly.
+  d.lattice = 1;
+  for (int a = 0; a < 3 && d.lattice; ++a)
+    for (uint64_t p = 0; p <= g.counts[a] && d.lattice; ++p)
+      for (int half = 0; half < (p < g.counts[a] ? 2 : 1); ++half) {
+        const double s = (double)p + (half ? 0.5 : 0.0);
+        double x = d.origin[a] + s * d.h[a];
+        x = std::fmin(std::fmax(x, d.origin[a]), d.upper[a]);
+        const double q = d.pow2[a] ? (x - d.origin[a]) * d.inv_h[a] : (x - d.origin[a]) / d.h[a];
+        if (q != s) {
+          d.lattice = 0;
+          break;
+        }
+      }
   const double* h = g.spacing;
   d.area[0] = h[1] * h[2];
   d.area[1] = h[0] * h[2];

@@ -13,7 +13,7 @@ import pytest
 
 import oracle as O
 import rheon_amd as R
-from rheon_amd.scenes import GridSpec, fourier_modes, lcg_faces, taylor_green_noise
+from rheon_amd.scenes import GridSpec, courant_dt, fourier_modes, lcg_faces, taylor_green_noise
 
 from helpers import config1, ogrid, rel_err, wavy_faces
 
@@ -110,6 +110,20 @@ def test_advect_velocity_bits_lattice_grids(which, dtype):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [64, 96])
+def test_advect_velocity_bits_fourier_lattice(n):
+    """The config-3 field recipe (Fourier modes) on unit boxes: 64^3 is a
+    lattice grid, 96^3 (h = 1/96) is not; both bit for bit."""
+    spec = GridSpec.unit_box(n)
+    vel = fourier_modes(spec, seed=5, modes=32, kmax=16, amplitude=1.0, dtype=np.float32)
+    dt = courant_dt(spec, vel, 0.5)
+    got = R.advect_velocity(spec, vel, dt)
+    want = O.advect_velocity(ogrid(spec), vel, dt, threads=8)
+    for a in range(3):
+        assert bits(got[a]) == bits(want[a]), a
+
+
+@pytest.mark.gpu
 def test_advect_velocity_lattice_zero_and_nonfinite():
     """Signed zeros and a non-finite value next to a sampled face."""
     spec = LATTICE_SPECS[0]
@@ -119,6 +133,16 @@ def test_advect_velocity_lattice_zero_and_nonfinite():
     vel[2][::5] = 0.0
     got = R.advect_velocity(spec, vel, 0.01)
     want = O.advect_velocity(ogrid(spec), vel, 0.01)
+    for a in range(3):
+        assert bits(got[a]) == bits(want[a]), a
+    # a component that is -0.0 almost everywhere: every first-stage sum of it
+    # is the +0.0 the reference's running sum gives
+    negz = [v.copy() for v in vel]
+    negz[0][:] = -0.0
+    negz[0][::11] = 0.25
+    negz[2][:] = -0.0
+    got = R.advect_velocity(spec, negz, 0.3)
+    want = O.advect_velocity(ogrid(spec), negz, 0.3)
     for a in range(3):
         assert bits(got[a]) == bits(want[a]), a
     bad = [v.copy() for v in vel]
