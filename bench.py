@@ -100,6 +100,41 @@ KERNEL_BYTES_PER_CELL = {
 }
 
 
+# Algorithmic bytes per launch of the face-field kernels (f32 faces, F faces,
+# N cells): advection reads three face arrays and writes three; the RHS pass
+# reads the faces and writes f64 cells; the correction reads p and reads and
+# writes the faces; the divergence/rate pass reads the faces.
+FACE_KERNEL_BYTES = {
+    "advect_velocity": lambda spec, m: 2 * 4 * spec.face_total,
+    "build_rhs": lambda spec, m: 4 * spec.face_total + 8 * spec.cell_len,
+    "correct": lambda spec, m: 8 * spec.cell_len + 2 * 4 * spec.face_total + m * spec.cell_len,
+    "divergence_max": lambda spec, m: 4 * spec.face_total + m * spec.cell_len,
+}
+# Timed groups that are many kernels (not candidates for the dominant kernel).
+AGGREGATES = ("mg_coarse_levels",)
+
+
+def kernel_bytes_per_launch(name: str, spec: GridSpec, s: int, m: int) -> float:
+    if name in FACE_KERNEL_BYTES:
+        return FACE_KERNEL_BYTES[name](spec, m)
+    return KERNEL_BYTES_PER_CELL[name](s, m) * spec.cell_len
+
+
+def profiled_counters(kernel: str) -> dict:
+    """`name=value` items of the committed instruction-counter summary line
+    `counters_per_launch: kernel ...` (profiles/*_counters.md), or {}."""
+    import glob
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*_counters.md"))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith("counters_per_launch:"):
+                    items = line.split(":", 1)[1].split()
+                    if items and items[0] == kernel:
+                        return {k: float(v) for k, _, v in (it.partition("=") for it in items[1:])}
+    return {}
+
+
 def profiled_traffic(kernel: str):
     """Calibrated HBM bytes per launch of `kernel` on the default workload,
     from the committed rocprofv3 --pmc summary (profiles/*_traffic.md, line
@@ -200,6 +235,26 @@ def cpu_baseline(spec: GridSpec, vel, dt: float, iterations_needed: int, solid=N
     }
 
 
+def advection_line(spec: GridSpec, kernels: dict, default_workload: bool) -> dict:
+    """Advection is bound by f64 instruction issue, not HBM: its survey
+    time, HBM fraction of the algorithmic 2 s_f F bytes, and (default
+    workload) VALU instructions per face and issue fraction from the
+    committed counter summary."""
+    if "advect_velocity" not in kernels:
+        return None
+    n, sec = kernels["advect_velocity"]
+    avg = sec / n
+    out = {"avg_us": avg * 1e6,
+           "algorithmic_bytes_per_launch": 8 * spec.face_total,
+           "hbm_frac": 8 * spec.face_total / avg / 1e9 / HBM_PEAK_GBS}
+    c = profiled_counters("advect_velocity") if default_workload else {}
+    if c:
+        out["valu_insts_per_face"] = c.get("valu_insts_per_face")
+        out["valu_issue_frac"] = c.get("valu_issue_frac")
+        out["counters_source"] = "profiles/advect_velocity_256_counters.md"
+    return out
+
+
 def jacobi_settings(st: R.PressureSettings) -> R.PressureSettings:
     return R.PressureSettings(st.relative_residual, st.absolute_residual, st.divergence_limit, 20000,
                               poll_interval=32)
@@ -269,6 +324,16 @@ def copy_probe(mib: int) -> int:
 
 
 def bench_batch(args) -> int:
+    result = run_batch(args)
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if result.get("ok") else 1
+
+
+def run_batch(args) -> dict:
     """Batched workload: 64 scenes, each axis log-uniform on [32, 128] and a
     multiple of 8 (cubic cells, longest side 1), Taylor-Green + noise with
     seed = scene id, 10 projection steps with the mixed-precision multigrid
@@ -303,8 +368,7 @@ def bench_batch(args) -> int:
     batch_s = time.perf_counter() - t0
     bad = [i for i, r in enumerate(reps) if r.status != R.OK]
     if bad:
-        print(f"batched scenes failed: {[(i, R.STATUS_NAMES.get(reps[i].status)) for i in bad]}", file=sys.stderr)
-        return 1
+        return {"ok": False, "failed": [(i, R.STATUS_NAMES.get(reps[i].status)) for i in bad]}
     # the one-scene path over the same scenes and steps
     singles = []
     for i, sp in enumerate(specs):
@@ -320,8 +384,7 @@ def bench_batch(args) -> int:
         for i, sc in enumerate(singles):
             r = sc.step(dts[i])
             if r.status != R.OK:
-                print(f"single scene {i} failed: {R.STATUS_NAMES.get(r.status)}", file=sys.stderr)
-                return 1
+                return {"ok": False, "failed_single": (i, R.STATUS_NAMES.get(r.status))}
     loop_s = time.perf_counter() - t0
     same = all(np.array_equal(batch.scene(i).get_pressure().view(np.uint64),
                               singles[i].get_pressure().view(np.uint64)) for i in range(len(specs)))
@@ -350,13 +413,12 @@ def bench_batch(args) -> int:
             "autotune": {"chosen": best, "candidates": table, "seconds": tune_s},
             "iterations_last_step": [r.pressure.iterations for r in reps],
         },
+        "ok": bool(same),
     }
-    line = json.dumps(result)
-    print(line, flush=True)
-    if args.json_out:
-        with open(args.json_out, "w") as f:
-            f.write(line + "\n")
-    return 0
+    for sc in singles:
+        sc.close()
+    batch.close()
+    return result
 
 
 def serve_requests(count: int, seed: int):
@@ -392,6 +454,16 @@ def serve_requests(count: int, seed: int):
 
 
 def bench_serve(args) -> int:
+    result = run_serve(args)
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0 if not result["config"]["failed"] else 1
+
+
+def run_serve(args) -> dict:
     """Serving workload: Poisson arrivals at a fixed rate, continuous
     batching, admission by planned device bytes; per-step latency p50/p99
     and total steps/s at max|div u| <= 1e-6."""
@@ -466,13 +538,8 @@ def bench_serve(args) -> int:
             "scene_steps": steps_total,
         },
     }
-    line = json.dumps(result)
-    print(line, flush=True)
-    if args.json_out:
-        with open(args.json_out, "w") as f:
-            f.write(line + "\n")
     server.close()
-    return 0 if not failed else 1
+    return result
 
 
 def bench_config2(args) -> int:
@@ -585,6 +652,8 @@ def main() -> int:
     ap.add_argument("--compare", action="store_true",
                     help="also time one solve of the step's RHS by every method and report pressure errors")
     ap.add_argument("--json-out", default=None)
+    ap.add_argument("--headline-only", action="store_true",
+                    help="default workload without the method comparison and the batch/serve summaries")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once with the plain copy kernel (the known-size "
                          "reference for hardware traffic counters) and exit")
@@ -645,8 +714,7 @@ def main() -> int:
 
     def kernel_table():
         out = {}
-        for name in list(KERNEL_BYTES_PER_CELL) + ["advect_velocity", "build_rhs", "correct", "divergence_max",
-                                                   "mg_coarse_levels"]:
+        for name in list(KERNEL_BYTES_PER_CELL) + list(FACE_KERNEL_BYTES) + list(AGGREGATES):
             n_l, sec = C.c_int64(), C.c_double()
             if lib.rh_profile_query(name.encode(), C.byref(n_l), C.byref(sec)) == 0 and n_l.value:
                 out[name] = (n_l.value, sec.value)
@@ -663,7 +731,8 @@ def main() -> int:
         return 1
     kernels = kernel_table()
     lib.rh_profile_enable(0)
-    dom = max((k for k in kernels if k in KERNEL_BYTES_PER_CELL), key=lambda k: kernels[k][1])
+    # the dominant kernel: the most device time per step among single kernels
+    dom = max((k for k in kernels if k not in AGGREGATES), key=lambda k: kernels[k][1])
 
     lib.rh_profile_focus(dom.encode())
     lib.rh_profile_enable(1)
@@ -687,12 +756,14 @@ def main() -> int:
 
     avg = secs / launches
     m_bytes = 1 if solid is not None else 0
-    per_cell = KERNEL_BYTES_PER_CELL[dom](s_vec, m_bytes)
     # four-cell FP32 path with >= 2 pre-sweeps: the outer residual does not store x0
     x0_implicit = args.method == "mixed-mg-dc" and sweeps[0] >= 2 and spec.counts[0] % 4 == 0
-    if dom == "mg_outer_residual" and x0_implicit:
-        per_cell -= s_vec
-    per_launch = per_cell * spec.cell_len
+
+    def launch_bytes(name):
+        b = kernel_bytes_per_launch(name, spec, s_vec, m_bytes)
+        return b - s_vec * spec.cell_len if name == "mg_outer_residual" and x0_implicit else b
+
+    per_launch = launch_bytes(dom)
     total_bytes = per_launch * launches
     first_launch = None
     if dom == "mg_outer_residual" and x0_implicit:
@@ -741,8 +812,12 @@ def main() -> int:
             "latency_ms_p99": float(np.percentile(np.array(lat) * 1e3, 99)),
             "algorithmic_GBps": step_bytes / (elapsed / args.steps) / 1e9,
             "cell_updates_per_s": spec.cell_len * args.steps / elapsed,
-            "kernels_survey_step": {k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e6}
-                                    for k, v in kernels.items()},
+            "kernels_survey_step": {
+                k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e6, "total_us": v[1] * 1e6,
+                    **({"hbm_frac": launch_bytes(k) / (v[1] / v[0]) / 1e9 / HBM_PEAK_GBS}
+                       if k not in AGGREGATES else {})}
+                for k, v in sorted(kernels.items(), key=lambda kv: -kv[1][1])},
+            "advection": advection_line(spec, kernels, default_workload),
         },
         "roofline": {
             "bound": "hbm",
@@ -772,8 +847,28 @@ def main() -> int:
             need = ws.solve_rhs(1.0, rhs, dt, jacobi_settings(settings)).iterations
             ws.close()
         result["cpu_baseline"] = cpu_baseline(spec, scene.get_velocity(), dt, need, solid)
-    if args.compare:
+    if args.compare or (default_workload and not args.headline_only):
         result["config"]["compare"] = compare_methods(spec, scene, dt, solid, settings)
+    if default_workload and not args.headline_only:
+        scene.close()
+        # the batched and serving workloads, summarised (each has its own
+        # --workload line with every detail)
+        b = run_batch(argparse.Namespace(scenes=64, steps=10, rel=1e-8, mg_coarse_sweeps="", json_out=None))
+        c = b.get("config", {})
+        result["batch"] = {k: v for k, v in {
+            "value": b.get("value"), "unit": b.get("unit"), "workload": c.get("workload"),
+            "looped_cell_updates_per_s": c.get("looped_cell_updates_per_s"),
+            "speedup_vs_looped": c.get("speedup_vs_looped"),
+            "batch_equals_looped_bitwise": c.get("batch_equals_looped_bitwise"),
+            "autotune_chosen": c.get("autotune", {}).get("chosen"), "ok": b.get("ok"),
+            "failed": b.get("failed")}.items() if v is not None}
+        sv = run_serve(argparse.Namespace(requests=40, rate=8.0, seed=5, hbm_budget_gb=16.0, block_divisor=8,
+                                          json_out=None))
+        c = sv["config"]
+        result["serve"] = {"value": sv["value"], "unit": sv["unit"], "workload": c["workload"],
+                           **{k: c[k] for k in ("latency_ms_p50", "latency_ms_p99", "latency_ms_max",
+                                                "cell_updates_per_s", "queue_wait_ms_p50", "queue_wait_ms_p99",
+                                                "max_bytes_in_use_gb", "max_div_u", "failed", "scene_steps")}}
     line = json.dumps(result)
     print(line, flush=True)
     if args.json_out:
