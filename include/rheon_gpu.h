@@ -60,6 +60,7 @@ typedef enum rh_status {
   RH_ARITHMETIC_FAILURE = 14,  /* OperatorError/SamplingError::ArithmeticFailure */
   RH_INVALID_TIME_STEP = 15,   /* OperatorError::InvalidTimeStep */
   RH_INVALID_DENSITY = 16,     /* OperatorError::InvalidDensity */
+  RH_TIME_RESOLUTION = 17,     /* SimulationError::TimeResolution */
   RH_DIVERGENCE_LIMIT = 18,    /* SimulationError::DivergenceLimit */
   RH_COURANT_LIMIT = 19,       /* SimulationError::CourantLimit */
   RH_GEOMETRY = 20,            /* GeometryError::* */
@@ -311,7 +312,7 @@ typedef struct rh_scene_config {
   double max_courant;               /* 0 disables the Courant gate */
 } rh_scene_config;
 
-/* StepReport (src/simulation.rs:44-53) subset + timings. */
+/* StepReport (src/simulation.rs:44-53) + timings. */
 typedef struct rh_step_report {
   double dt;
   rh_pressure_report pressure;
@@ -319,6 +320,10 @@ typedef struct rh_step_report {
   double courant;
   double seconds;          /* device time of the whole step */
   double advect_seconds, rhs_seconds, solve_seconds, correct_seconds;
+  double time;             /* scene time after the step (0 + sum of committed dt) */
+  uint64_t generation;     /* committed steps after the step */
+  double tracer_integral;  /* sum of the candidate tracer * cell volume (0 without a tracer) */
+  double kinetic_energy;   /* 0.5 rho V sum of squared candidate faces */
 } rh_step_report;
 
 rh_status rh_scene_create(const rh_grid* grid, const rh_scene_config* config, rh_scene** out);
@@ -336,10 +341,28 @@ rh_status rh_scene_get_pressure(const rh_scene* scene, double* out, size_t len);
 /* Device pointers of the accepted faces (for zero-copy producers). */
 rh_status rh_scene_device_velocity(rh_scene* scene, void** d_u, void** d_v, void** d_w);
 /* One projection step with an explicit dt: advect -> build_rhs -> solve ->
- * correct -> divergence gate -> Courant gate -> commit.  On any failure the
- * accepted faces are unchanged (transactional, src/simulation.rs:391-392). */
+ * correct -> divergence gate -> Courant gate -> tracer (when the scene has
+ * one) -> commit.  On any failure the accepted faces (and tracer) are
+ * unchanged (transactional, src/simulation.rs:391-392). */
 rh_status rh_scene_step(rh_scene* scene, double dt, rh_step_report* report,
                         rh_cancel_fn cancel, void* cancel_ctx);
+/* Simulation::step's time-step policy (src/simulation.rs:926-979, the fixed
+ * box without sources): dt = min(requested, 0.5 max_courant / rate) with
+ * rate = sum_d max|u_d| / h_d of the accepted faces (src/simulation.rs:
+ * 1415-1429), then rh_scene_step(dt).  max_courant must lie in (0, 1]
+ * (src/simulation.rs:293-295).  The rate of faces a step committed is kept
+ * from that step's Courant gate, so no extra pass runs. */
+rh_status rh_scene_step_requested(rh_scene* scene, double requested_dt, rh_step_report* report,
+                                  rh_cancel_fn cancel, void* cancel_ctx);
+/* Passive tracer of the scene (Simulation's accepted/candidate tracer,
+ * src/simulation.rs:1290-1297): one f32 per cell (host), or NULL to drop
+ * it.  With a tracer, every step advects it with the corrected candidate
+ * velocity (advect_tracer, src/advection.rs:72-162, cells outside the
+ * displaced support of its nonzero cells are exactly 0), rejects a value
+ * outside [0, 1] or non-finite (src/simulation.rs:1322-1331) and commits it
+ * with the faces. */
+rh_status rh_scene_set_tracer(rh_scene* scene, const float* tracer, size_t len);
+rh_status rh_scene_get_tracer(const rh_scene* scene, float* out, size_t len);
 
 /* ------------------------------------------------------ batched scenes */
 /* Many device-resident scenes advanced together: each stage of the step is

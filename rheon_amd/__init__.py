@@ -37,6 +37,7 @@ NONZERO_WALL_VELOCITY = 13
 ARITHMETIC_FAILURE = 14
 INVALID_TIME_STEP = 15
 INVALID_DENSITY = 16
+TIME_RESOLUTION = 17
 DIVERGENCE_LIMIT = 18
 COURANT_LIMIT = 19
 GEOMETRY = 20
@@ -476,6 +477,10 @@ class StepReport:
     courant: float
     seconds: float
     stage_seconds: dict = field(default_factory=dict)
+    time: float = 0.0
+    generation: int = 0
+    tracer_integral: float = 0.0
+    kinetic_energy: float = 0.0
 
 
 class Scene:
@@ -533,14 +538,40 @@ class Scene:
         _check(lib().rh_scene_get_pressure(self.handle, out.ctypes.data, out.size), "rh_scene_get_pressure")
         return out
 
+    def set_tracer(self, tracer) -> None:
+        """Passive tracer (one f32 per cell) advected and committed with every
+        step, or None to drop it."""
+        if tracer is None:
+            _check(lib().rh_scene_set_tracer(self.handle, None, 0), "rh_scene_set_tracer")
+            return
+        a = np.ascontiguousarray(tracer, np.float32).reshape(-1)
+        _check(lib().rh_scene_set_tracer(self.handle, a.ctypes.data, a.size), "rh_scene_set_tracer")
+
+    def get_tracer(self) -> np.ndarray:
+        out = np.empty(self.spec.cell_len, np.float32)
+        _check(lib().rh_scene_get_tracer(self.handle, out.ctypes.data, out.size), "rh_scene_get_tracer")
+        return out
+
+    @staticmethod
+    def _report(status: int, rep) -> StepReport:
+        return StepReport(status, rep.dt, PressureReport.from_native(status, rep.pressure),
+                          rep.actual_divergence_max, rep.courant, rep.seconds,
+                          {"advect": rep.advect_seconds, "rhs": rep.rhs_seconds, "solve": rep.solve_seconds,
+                           "correct": rep.correct_seconds},
+                          rep.time, int(rep.generation), rep.tracer_integral, rep.kinetic_energy)
+
     def step(self, dt: float, cancelled=None) -> StepReport:
         rep = N.StepReport()
         cb, _keep = _cancel_cb(cancelled)
         status = lib().rh_scene_step(self.handle, dt, C.byref(rep), cb, None)
-        return StepReport(status, rep.dt, PressureReport.from_native(status, rep.pressure),
-                          rep.actual_divergence_max, rep.courant, rep.seconds,
-                          {"advect": rep.advect_seconds, "rhs": rep.rhs_seconds, "solve": rep.solve_seconds,
-                           "correct": rep.correct_seconds})
+        return self._report(status, rep)
+
+    def step_requested(self, requested_dt: float, cancelled=None) -> StepReport:
+        """Simulation::step's dt policy: min(requested, 0.5 max_courant / rate)."""
+        rep = N.StepReport()
+        cb, _keep = _cancel_cb(cancelled)
+        status = lib().rh_scene_step_requested(self.handle, requested_dt, C.byref(rep), cb, None)
+        return self._report(status, rep)
 
     @classmethod
     def _borrowed(cls, handle: int, spec: GridSpec, precision: int) -> "Scene":

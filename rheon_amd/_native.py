@@ -138,6 +138,10 @@ class StepReport(C.Structure):
         ("rhs_seconds", C.c_double),
         ("solve_seconds", C.c_double),
         ("correct_seconds", C.c_double),
+        ("time", C.c_double),
+        ("generation", C.c_uint64),
+        ("tracer_integral", C.c_double),
+        ("kinetic_energy", C.c_double),
     ]
 
 
@@ -251,6 +255,9 @@ SIGNATURES = {
     "rh_scene_get_pressure": (_S, [_P, _P, C.c_size_t]),
     "rh_scene_device_velocity": (_S, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rh_scene_step": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
+    "rh_scene_step_requested": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
+    "rh_scene_set_tracer": (_S, [_P, _P, C.c_size_t]),
+    "rh_scene_get_tracer": (_S, [_P, _P, C.c_size_t]),
     "rh_batch_create": (_S, [C.POINTER(Grid), C.c_size_t, C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
     "rh_batch_destroy": (None, [_P]),
     "rh_batch_allocated_bytes": (C.c_uint64, [_P]),

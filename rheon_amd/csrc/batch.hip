@@ -342,6 +342,7 @@ rh_status Batch::create(const rh_grid* grids, size_t n, const rh_batch_config& c
     rh_status s = rh_scene_create(&grids[i], &sc, &raw);
     if (s) return s;
     owned_.emplace_back(raw);
+    raw->batched = true;
     scenes_.push_back(raw);
     if (mg(i)->levels() < 2) {
       set_last_error("a batched scene needs at least two multigrid levels", "rh_batch_create");
@@ -383,6 +384,13 @@ rh_status Batch::set_members(const std::vector<rh_scene*>& scenes, const std::ve
         s->config.solver.precision != RH_F32)
       return RH_UNSUPPORTED;
     if (static_cast<MgSolver*>(s->solver->impl.get())->levels() < 2) return RH_UNSUPPORTED;
+  }
+  for (rh_scene* s : scenes) {
+    if (s->tracer[0]) {
+      set_last_error("scenes stepped by a batch carry no tracer", "rh_batch");
+      return RH_UNSUPPORTED;
+    }
+    s->batched = true;
   }
   if (scenes != scenes_ || dt != dt_) {
     built_ = false;
@@ -1030,6 +1038,8 @@ rh_status Batch::launch(int steps, const std::function<rh_status()>& host_work) 
   }
   // every scene's own stream must be idle (uploads went through them)
   for (size_t i = 0; i < n; ++i) RH_HIP(hipStreamSynchronize(scenes_[i]->solver->impl->scratch().stream));
+  // the batch commits faces without the one-scene step's bookkeeping
+  for (size_t i = 0; i < n; ++i) scenes_[i]->rate_known = false;
   if (!ev_[0]) {
     RH_HIP(hipEventCreate(&ev_[0]));
     RH_HIP(hipEventCreate(&ev_[1]));

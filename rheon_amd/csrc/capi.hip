@@ -731,6 +731,7 @@ extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const vo
   const void* src[3] = {u, v, w};
   const size_t fb = prec_bytes(sc->config.precision);
   sc->acc_checked = false;
+  sc->rate_known = false;
   for (int a = 0; a < 3; ++a)
     RH_HIP(hipMemcpy(sc->acc[a], src[a], sc->flen[a] * fb, hipMemcpyHostToDevice));
   return zero_blocked(sc);
@@ -740,6 +741,7 @@ extern "C" rh_status rh_scene_set_solid(rh_scene* sc, const uint8_t* solid, size
   if (!sc) return RH_INVALID_ARGUMENT;
   if (solid && len != sc->n) return RH_INVALID_ARGUMENT;
   sc->acc_checked = false;
+  sc->rate_known = false;
   rh_status s = sc->solver->impl->set_solid(solid);
   if (s) return s;
   return zero_blocked(sc);
@@ -763,15 +765,17 @@ extern "C" rh_status rh_scene_get_pressure(const rh_scene* sc, double* out, size
 extern "C" rh_status rh_scene_device_velocity(rh_scene* sc, void** d_u, void** d_v, void** d_w) {
   if (!sc || !d_u || !d_v || !d_w) return RH_INVALID_ARGUMENT;
   sc->external = true;
+  sc->rate_known = false;
   *d_u = sc->acc[0];
   *d_v = sc->acc[1];
   *d_w = sc->acc[2];
   return RH_OK;
 }
 
-// Simulation::prepare_impl carrier sequence (src/simulation.rs:988-1277):
+// Simulation::prepare_impl carrier sequence (src/simulation.rs:988-1331):
 // advect -> build_rhs -> pressure solve -> correct_candidate_in_place ->
-// actual divergence gate -> Courant gate -> commit swap (:805-811).
+// actual divergence gate -> Courant gate -> tracer (advect_tracer with the
+// candidate velocity, range check, integral) -> commit swap (:805-811).
 extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* report,
                                    rh_cancel_fn cancel, void* cancel_ctx) {
   if (!sc) return RH_INVALID_ARGUMENT;
@@ -779,6 +783,8 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   rh_step_report* rep = report ? report : &local;
   std::memset(rep, 0, sizeof(*rep));
   rep->dt = dt;
+  rep->time = sc->time;
+  rep->generation = sc->generation;
   if (!std::isfinite(dt) || dt <= 0.0) return RH_INVALID_TIME_STEP;
   if (cancel && cancel(cancel_ctx)) return RH_CANCELLED;
   rh::Scratch& scr = sc->solver->impl->scratch();
@@ -832,8 +838,9 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   RH_HIP(hipGetLastError());
   if ((s = scr.read_flags(&f))) return s;
   if (f) return rh::flags_to_status(f);
-  double sv[2];
-  if ((s = scr.read_scalars(sv, 2, 0))) return s;
+  // [0] max|div u|, [1] rate, [2..4] component maxima, [5] sum of squared faces
+  double sv[6];
+  if ((s = scr.read_scalars(sv, 6, 0))) return s;
   if (rh::prof::enabled()) rh::prof::settle(INT32_MAX);
   float ms[4] = {0, 0, 0, 0}, total = 0;
   for (int i = 0; i < 4; ++i) RH_HIP(hipEventElapsedTime(&ms[i], sc->ev[i], sc->ev[i + 1]));
@@ -846,9 +853,129 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   rep->actual_divergence_max = sv[0];
   rep->courant = dt * sv[1];
   if (sv[0] > sc->config.actual_divergence_limit) return RH_DIVERGENCE_LIMIT;
-  if (sc->config.max_courant > 0.0 && (!std::isfinite(rep->courant) || rep->courant > sc->config.max_courant))
-    return RH_COURANT_LIMIT;
+  if (!std::isfinite(sv[1]) || !std::isfinite(rep->courant)) return RH_ARITHMETIC_FAILURE;
+  if (sc->config.max_courant > 0.0 && rep->courant > sc->config.max_courant) return RH_COURANT_LIMIT;
+  if (cancel && cancel(cancel_ctx)) return RH_CANCELLED;  // StepStage::BeforeTracer
+  const double volume = g.volume;
+  double tracer_sum = 0.0;
+  if (sc->tracer[0]) {
+    // TracerSupport (src/advection.rs:119-153): nonzero bounding box of the
+    // accepted tracer grown by the largest RK2 displacement plus two cells;
+    // the whole box when a displaced coordinate would not be finite.
+    rh::launch_tracer_bounds(g, sc->tracer[0], scr.counters, scr.flags, st);
+    unsigned b[6];
+    RH_HIP(hipMemcpyAsync(scr.host, scr.counters, sizeof(b), hipMemcpyDeviceToHost, st));
+    RH_HIP(hipStreamSynchronize(st));
+    std::memcpy(b, scr.host, sizeof(b));
+    if ((s = scr.read_flags(&f))) return s;
+    if (f) return rh::flags_to_status(f);
+    const int n[3] = {g.nx, g.ny, g.nz};
+    rh::TracerBox box{{0, 0, 0}, {n[0], n[1], n[2]}};
+    bool dense = false;
+    double margin[3];
+    for (int d = 0; d < 3; ++d) {
+      const double displacement = dt * sv[2 + d];
+      const double world = std::fmax(std::fabs(g.origin[d]), std::fabs(g.upper[d]));
+      if (!std::isfinite(world + displacement)) dense = true;
+      margin[d] = std::ceil(displacement / g.h[d]) + 2.0;
+    }
+    if (!dense) {
+      const bool empty = b[3] == 0 && b[4] == 0 && b[5] == 0;
+      for (int d = 0; d < 3; ++d) {
+        if (empty) {
+          box.lo[d] = n[d];
+          box.hi[d] = 0;
+          continue;
+        }
+        const double lo = (double)b[d] - margin[d], hi = (double)b[3 + d] + margin[d];
+        box.lo[d] = lo <= 0.0 ? 0 : (int)lo;
+        box.hi[d] = hi >= (double)n[d] ? n[d] : (int)hi;
+      }
+    }
+    rh::launch_advect_tracer(g, sc->tracer[0], (const float*)sc->cand[0], (const float*)sc->cand[1],
+                             (const float*)sc->cand[2], dt, sc->tracer[1], scr.flags, st, &box);
+    rh::launch_tracer_sum(g, sc->tracer[1], scr.reduce(8), st);
+    RH_HIP(hipGetLastError());
+    if ((s = scr.read_flags(&f))) return s;
+    if (f) return rh::flags_to_status(f);
+    if ((s = scr.read_scalars(&tracer_sum, 1, 8))) return s;
+  }
+  rep->tracer_integral = tracer_sum * volume;
+  rep->kinetic_energy = 0.5 * sc->config.density * volume * sv[5];
+  if (!std::isfinite(rep->tracer_integral) || !std::isfinite(rep->kinetic_energy)) return RH_ARITHMETIC_FAILURE;
+  const double next_time = sc->time + dt;
+  if (cancel && cancel(cancel_ctx)) return RH_CANCELLED;  // StepStage::BeforeCommit
   for (int a = 0; a < 3; ++a) std::swap(sc->acc[a], sc->cand[a]);
+  if (sc->tracer[0]) std::swap(sc->tracer[0], sc->tracer[1]);
   sc->acc_checked = true;
+  sc->acc_rate = sv[1];
+  sc->rate_known = !sc->external;
+  sc->time = next_time;
+  sc->generation += 1;
+  rep->time = sc->time;
+  rep->generation = sc->generation;
+  return RH_OK;
+}
+
+// Simulation::step's dt (src/simulation.rs:926-979) for the fixed box with
+// no source, forces or emitter: the acceleration rate is 0, so
+// dt = min(requested, 0.5 max_courant / rate) when rate > 0.
+extern "C" rh_status rh_scene_step_requested(rh_scene* sc, double requested_dt, rh_step_report* report,
+                                             rh_cancel_fn cancel, void* cancel_ctx) {
+  if (!sc) return RH_INVALID_ARGUMENT;
+  if (report) std::memset(report, 0, sizeof(*report));
+  const double c = sc->config.max_courant;
+  if (!std::isfinite(c) || c <= 0.0 || c > 1.0) return RH_INVALID_SETTINGS;
+  if (!std::isfinite(requested_dt) || requested_dt <= 0.0) return RH_INVALID_TIME_STEP;
+  double rate = sc->acc_rate;
+  if (!sc->rate_known || sc->external) {
+    rh::Scratch& scr = sc->solver->impl->scratch();
+    const DevGrid g = sc->solver->impl->dev_grid(sc->config.density);
+    rh_status s = dispatch_precision(sc->config.precision, [&](auto* tag) -> rh_status {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      rh::launch_velocity_rate<T>(g, sc->acc[0], sc->acc[1], sc->acc[2], scr.reduce(16), scr.stream);
+      return RH_OK;
+    });
+    if (s) return s;
+    RH_HIP(hipGetLastError());
+    if ((s = scr.read_scalars(&rate, 1, 16))) return s;
+  }
+  if (!std::isfinite(rate)) return RH_ARITHMETIC_FAILURE;
+  double dt = requested_dt;
+  if (rate > 0.0) dt = std::fmin(dt, 0.5 * c / rate);
+  const double next_time = sc->time + dt;
+  if (dt <= 0.0 || !std::isfinite(next_time) || next_time <= sc->time) return RH_TIME_RESOLUTION;
+  return rh_scene_step(sc, dt, report, cancel, cancel_ctx);
+}
+
+extern "C" rh_status rh_scene_set_tracer(rh_scene* sc, const float* tracer, size_t len) {
+  if (!sc) return RH_INVALID_ARGUMENT;
+  if (!tracer) {
+    if (sc->tracer[0]) (void)hipFree(sc->tracer[0]);
+    sc->tracer[0] = sc->tracer[1] = nullptr;
+    return RH_OK;
+  }
+  if (len != sc->n) return RH_INVALID_ARGUMENT;
+  if (sc->batched) {
+    rh::set_last_error("scenes stepped by a batch carry no tracer", "rh_scene_set_tracer");
+    return RH_UNSUPPORTED;
+  }
+  for (size_t i = 0; i < len; ++i)
+    if (!std::isfinite(tracer[i])) return RH_NONFINITE_INPUT;
+  if (!sc->tracer[0]) {
+    const size_t b = (sc->n * sizeof(float) + 255) / 256 * 256;
+    void* p = nullptr;
+    RH_HIP(hipMalloc(&p, 2 * b));
+    sc->tracer[0] = (float*)p;
+    sc->tracer[1] = (float*)((char*)p + b);
+  }
+  RH_HIP(hipMemcpy(sc->tracer[0], tracer, len * sizeof(float), hipMemcpyHostToDevice));
+  return RH_OK;
+}
+
+extern "C" rh_status rh_scene_get_tracer(const rh_scene* sc, float* out, size_t len) {
+  if (!sc || !out || len != sc->n) return RH_INVALID_ARGUMENT;
+  if (!sc->tracer[0]) return RH_INVALID_ARGUMENT;
+  RH_HIP(hipMemcpy(out, sc->tracer[0], len * sizeof(float), hipMemcpyDeviceToHost));
   return RH_OK;
 }

@@ -263,7 +263,8 @@ inline void launch_finish(const DD* pdd, const double* pmx, int nb, const Fin& f
 struct FinStore {
   double* out;
   int mode;  // 0: out[0] = max0; 1: velocity rate; 2: out[0..2] = sum0, max0, max1;
-             // 3: out[0] = max0 and the velocity rate of max1..3 into out[1..4]
+             // 3: out[0] = max0 and the velocity rate of max1..3 into out[1..4],
+             //    out[5] = sum0; 4: out[0] = sum0
   double h0, h1, h2;
   __device__ bool active() const { return true; }
   __device__ void operator()(const double* sums, const double* mx) const {
@@ -278,9 +279,12 @@ struct FinStore {
       out[1] = mx[0];
       out[2] = mx[1];
       out[3] = mx[2];
+    } else if (mode == 4) {
+      out[0] = sums[0];
     } else if (mode == 3) {
       out[0] = mx[0];
       FinStore{out + 1, 1, h0, h1, h2}(sums, mx + 1);
+      out[5] = sums[0];
     } else {
       out[0] = sums[0];
       out[1] = mx[0];

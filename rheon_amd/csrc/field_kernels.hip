@@ -319,27 +319,78 @@ k_advect_velocity(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
   b_advect_velocity<T, kOff32, kLattice>(hw_block(), (int)blockIdx.y, g, u, v, w, dt, ou, ov, ow, flags);
 }
 
-// advect_tracer (src/advection.rs:72-102), every cell evaluated (the
-// reference's support skipping is exact: it writes the same zeros).
+// First RK2 stage at a cell centre of a lattice grid (DevGrid::lattice, f32
+// faces): component c has t = 1/2 on its own axis (corners p_c and p_c + 1)
+// and t = 0 on the other two, so ScalarSampler::sample reduces to the two
+// corners of weight exactly 1/2 in the reference's order; the other six
+// weigh +0 and the clamp is the identity (the argument of
+// lattice_half_sample with two terms).
+template <bool kOff32>
+__device__ __forceinline__ double lattice_cell_sample(const DevGrid& g, int c, const float* __restrict__ vals, int i,
+                                                      int j, int k) {
+  const int dx = g.nx + (c == 0), dy = g.ny + (c == 1);
+  const int c0 = kOff32 ? i + (int)__umul24(dx, j + (int)__umul24(dy, k)) : i + dx * (j + dy * k);
+  const int s = c == 0 ? 1 : (c == 1 ? dx : dx * dy);
+  double r = 0.0;
+  r += 0.5 * (double)load_elem<float, kOff32>(vals, c0);
+  r += 0.5 * (double)load_elem<float, kOff32>(vals, c0 + s);
+  return r;
+}
+
+// advect_tracer (src/advection.rs:72-162).  Cells outside the support box
+// [lo, hi) are exactly +0.0, the value the reference writes there without
+// sampling (TracerSupport, :104-153; the caller passes the whole box for
+// the reference's dense fallback).  kLattice: the first RK2 stage by
+// lattice_cell_sample (same values).
+template <bool kOff32, bool kLattice>
 __global__ void __launch_bounds__(kBlock)
 k_advect_tracer(DevGrid g, const float* __restrict__ old, const float* __restrict__ u,
                 const float* __restrict__ v, const float* __restrict__ w, double dt,
-                float* __restrict__ out, unsigned* __restrict__ flags) {
+                float* __restrict__ out, unsigned* __restrict__ flags, TracerBox box) {
   const RowRange rr = block_rows(g.ny * g.nz);
   unsigned bad = 0;
+  const float* comp[3] = {u, v, w};
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
     const int k = row / g.ny, j = row - k * g.ny;
+    const bool row_in = j >= box.lo[1] && j < box.hi[1] && k >= box.lo[2] && k < box.hi[2];
     for (int i = threadIdx.x; i < g.nx; i += kLanesX) {
+      if (!row_in || i < box.lo[0] || i >= box.hi[0]) {
+        out[i + g.nx * row] = 0.0f;
+        continue;
+      }
       const int p[3] = {i, j, k};
       double pos[3], dep[3], value = 0.0;
 #pragma unroll
       for (int d = 0; d < 3; ++d) pos[d] = g.origin[d] + ((double)p[d] + 0.5) * g.h[d];
-      unsigned f = backtrace<float>(g, u, v, w, pos, dt, dep);
-      if (!f) {
-        double base[3];
-        base_coords<true>(g, dep, base);
-        f = sample_at<float>(g, 3, old, base, value);
+      unsigned f;
+      if constexpr (kLattice) {
+        double vel[3], mid[3], base[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vel[c] = lattice_cell_sample<kOff32>(g, c, comp[c], i, j, k);
+        const double half = 0.5 * dt;
+        bool nf = false;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double x = pos[d] - half * vel[d];
+          nf = nf || !isfinite(x);
+          mid[d] = fmin(fmax(x, g.origin[d]), g.upper[d]);
+        }
+        base_coords<true>(g, mid, base);
+        f = sample_at<float, kOff32>(g, 0, u, base, vel[0]) | sample_at<float, kOff32>(g, 1, v, base, vel[1]) |
+            sample_at<float, kOff32>(g, 2, w, base, vel[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const double x = pos[d] - dt * vel[d];
+          nf = nf || !isfinite(x);
+          dep[d] = fmin(fmax(x, g.origin[d]), g.upper[d]);
+        }
+        f |= nf ? kFlagArithmetic : 0u;
+      } else {
+        f = backtrace<float, kOff32>(g, u, v, w, pos, dt, dep);
       }
+      double base[3];
+      base_coords<true>(g, dep, base);
+      f |= sample_at<float, kOff32>(g, 3, old, base, value);
       if (f) {
         bad |= f;
         continue;
@@ -348,6 +399,71 @@ k_advect_tracer(DevGrid g, const float* __restrict__ old, const float* __restric
     }
   }
   if (bad) atomicOr(flags, bad);
+}
+
+// Bounding box of the nonzero cells of a tracer (TracerSupport's exact
+// support, src/advection.rs:131-145: a cell counts when its value != 0,
+// so -0.0 does not) into bounds[0..5] = min i, j, k and max i, j, k (set to
+// n, n, n, 0, 0, 0 beforehand); non-finite values raise the input flag
+// (ScalarSampler::cells, src/sampling.rs:62-64).
+__global__ void __launch_bounds__(kBlock)
+k_tracer_bounds(DevGrid g, const float* __restrict__ t, unsigned* __restrict__ bounds,
+                unsigned* __restrict__ flags) {
+  const RowRange rr = block_rows(g.ny * g.nz);
+  unsigned lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  bool any = false, bad = false;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    for (int i = threadIdx.x; i < g.nx; i += kLanesX) {
+      const float x = t[i + g.nx * row];
+      if (!isfinite(x)) bad = true;
+      if (x != 0.0f) {
+        any = true;
+        lo[0] = min(lo[0], (unsigned)i);
+        lo[1] = min(lo[1], (unsigned)j);
+        lo[2] = min(lo[2], (unsigned)k);
+        hi[0] = max(hi[0], (unsigned)i + 1);
+        hi[1] = max(hi[1], (unsigned)j + 1);
+        hi[2] = max(hi[2], (unsigned)k + 1);
+      }
+    }
+  }
+  if (bad) atomicOr(flags, kFlagNonfiniteInput);
+  if (any) {
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&bounds[d], lo[d]);
+      atomicMax(&bounds[3 + d], hi[d]);
+    }
+  }
+}
+
+__global__ void k_tracer_bounds_init(DevGrid g, unsigned* bounds) {
+  if (threadIdx.x == 0) {
+    bounds[0] = (unsigned)g.nx;
+    bounds[1] = (unsigned)g.ny;
+    bounds[2] = (unsigned)g.nz;
+    bounds[3] = bounds[4] = bounds[5] = 0u;
+  }
+}
+
+// Candidate tracer check and sum (src/simulation.rs:1322-1331): every value
+// finite and in [0, 1] (else the arithmetic flag), and the double-double
+// sum of the values (the reference sums them serially; this is the sum
+// rounded once).
+__global__ void __launch_bounds__(kBlock)
+k_tracer_sum(DevGrid g, const float* __restrict__ t, Reduce red) {
+  const RowRange rr = block_rows(g.ny * g.nz);
+  DD dd[1] = {DD{0.0, 0.0}};
+  double mx[1] = {0.0};
+  bool bad = false;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock)
+    for (int i = threadIdx.x; i < g.nx; i += kLanesX) {
+      const float x = t[i + g.nx * row];
+      if (!isfinite(x) || !(x >= 0.0f && x <= 1.0f)) bad = true;
+      dd_acc(dd[0], (double)x);
+    }
+  if (bad) atomicOr(red.flags, kFlagArithmetic);
+  block_publish<1, 1>(dd, mx, red.pdd, red.pmx);
 }
 
 // Finite check of every face and (optionally) exact zero on wall-normal faces
@@ -621,6 +737,7 @@ k_divergence_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v, c
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const RowRange rr = block_rows(ny * nz);
   double m[4] = {0.0, 0.0, 0.0, 0.0};
+  DD e{0.0, 0.0};
   unsigned bad = 0;
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
     const int k = row / ny, j = row - k * ny;
@@ -640,10 +757,17 @@ k_divergence_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v, c
       if (i + 1 == nx) m[1] = fmax(m[1], fabs(u1));
       if (j + 1 == ny) m[2] = fmax(m[2], fabs(v1));
       if (k + 1 == nz) m[3] = fmax(m[3], fabs(w1));
+      // squared faces for the kinetic energy (each face once)
+      dd_acc(e, u0 * u0);
+      dd_acc(e, v0 * v0);
+      dd_acc(e, w0 * w0);
+      if (i + 1 == nx) dd_acc(e, u1 * u1);
+      if (j + 1 == ny) dd_acc(e, v1 * v1);
+      if (k + 1 == nz) dd_acc(e, w1 * w1);
     }
   }
   if (bad) atomicOr(red.flags, bad);
-  DD dd[1] = {DD{0.0, 0.0}};
+  DD dd[1] = {e};
   block_publish<1, 4>(dd, m, red.pdd, red.pmx);
 }
 
@@ -718,7 +842,7 @@ void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const 
 
 // out[0] = max|div u|, out[1] = velocity rate, out[2..4] = max|u|, |v|, |w|
 // (the slots launch_divergence_max into out and launch_velocity_rate into
-// out + 1 fill).
+// out + 1 fill), out[5] = sum of the squared faces (double-double).
 template <typename T>
 void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, const void* w, const Reduce& red,
                             hipStream_t s) {
@@ -728,9 +852,32 @@ void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, cons
 }
 
 void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,
-                          const float* w, double dt, float* out, unsigned* flags, hipStream_t s) {
+                          const float* w, double dt, float* out, unsigned* flags, hipStream_t s,
+                          const TracerBox* box) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
-  k_advect_tracer<<<grid, block, 0, s>>>(g, old, u, v, w, dt, out, flags);
+  const TracerBox all{{0, 0, 0}, {g.nx, g.ny, g.nz}};
+  const TracerBox b = box ? *box : all;
+  const bool off32 = face_offsets_32(g, sizeof(float));
+  if (g.lattice && off32)
+    launch(k_advect_tracer<true, true>, grid, block, 0, s, g, old, u, v, w, dt, out, flags, b);
+  else if (g.lattice)
+    launch(k_advect_tracer<false, true>, grid, block, 0, s, g, old, u, v, w, dt, out, flags, b);
+  else if (off32)
+    launch(k_advect_tracer<true, false>, grid, block, 0, s, g, old, u, v, w, dt, out, flags, b);
+  else
+    launch(k_advect_tracer<false, false>, grid, block, 0, s, g, old, u, v, w, dt, out, flags, b);
+}
+
+void launch_tracer_bounds(const DevGrid& g, const float* t, unsigned* bounds, unsigned* flags, hipStream_t s) {
+  k_tracer_bounds_init<<<1, 64, 0, s>>>(g, bounds);
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  k_tracer_bounds<<<grid, block, 0, s>>>(g, t, bounds, flags);
+}
+
+void launch_tracer_sum(const DevGrid& g, const float* t, const Reduce& red, hipStream_t s) {
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  launch(k_tracer_sum, grid, block, 0, s, g, t, red);
+  launch_finish<1, 1>(red.pdd, red.pmx, grid.x, FinStore{red.out, 4, 0.0, 0.0, 0.0}, s);
 }
 
 // Obstacle faces (both cells not fluid on either side) set to 0; walls and

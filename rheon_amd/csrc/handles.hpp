@@ -29,10 +29,22 @@ struct rh_scene {
   // step validates them again.
   bool acc_checked = false;
   bool external = false;
+  // Passive tracer (accepted, candidate), f32 per cell; null without one.
+  float* tracer[2] = {nullptr, nullptr};
+  // Simulation time and committed steps (src/simulation.rs StepReport).
+  double time = 0.0;
+  uint64_t generation = 0;
+  // Velocity rate of the accepted faces, kept from the Courant gate of the
+  // step that committed them (valid while nothing else wrote them).
+  double acc_rate = 0.0;
+  bool rate_known = false;
+  // Stepped by a batch (rh_batch / rh_server), which carries no tracer.
+  bool batched = false;
   ~rh_scene() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (arena) (void)hipFree(arena);
+    if (tracer[0]) (void)hipFree(tracer[0]);
   }
 };
 

@@ -47,8 +47,19 @@ void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, cons
                             const Reduce& red, hipStream_t s);
 template <typename T>
 void launch_zero_blocked(const DevGrid& g, void* u, void* v, void* w, hipStream_t s);
+// Cells [lo, hi) of a tracer advection; the rest are written +0.0.
+struct TracerBox {
+  int lo[3], hi[3];
+};
+// box: nullptr = every cell.
 void launch_advect_tracer(const DevGrid& g, const float* old, const float* u, const float* v,
-                          const float* w, double dt, float* out, unsigned* flags, hipStream_t s);
+                          const float* w, double dt, float* out, unsigned* flags, hipStream_t s,
+                          const TracerBox* box = nullptr);
+// bounds[0..5]: min i, j, k and one past max i, j, k of the nonzero cells
+// (n, n, n, 0, 0, 0 when none is nonzero).
+void launch_tracer_bounds(const DevGrid& g, const float* t, unsigned* bounds, unsigned* flags, hipStream_t s);
+// red.out[0] = sum of the tracer values; arithmetic flag on a value outside [0, 1].
+void launch_tracer_sum(const DevGrid& g, const float* t, const Reduce& red, hipStream_t s);
 
 // -------------------------------------------------------------- PCG kernels
 // Device-side solver control block: all scalar decisions of solve_loaded
