@@ -252,17 +252,18 @@ typedef struct rh_solver_metadata {
 typedef struct rh_provider_config {
   rh_solver_config solver;
   rh_pressure_settings settings;  /* the provider's own stopping rule */
+  uint64_t shape_xyz[3];          /* cells the provider serves; {0,0,0} = any shape */
 } rh_provider_config;
 
 /* impl PressureProvider for the GPU solver (trait src/solver_provider.rs:67-76). */
 typedef struct rh_provider rh_provider;
 rh_status rh_provider_create(const rh_provider_config* config, rh_provider** out);
 void rh_provider_destroy(rh_provider* provider);
-/* metadata(): shape_xyz taken from `grid` (the trait's metadata is per provider;
- * the shape is that of the request it will serve). */
-rh_status rh_provider_metadata(const rh_provider* provider, const rh_grid* grid,
-                               rh_solver_metadata* out);
-/* supports(): RH_OK or RH_UNSUPPORTED with a static reason string. */
+/* metadata() (src/solver_provider.rs:67-68): per provider; shape_xyz is the
+ * configured shape ({0,0,0} when the provider serves any shape). */
+rh_status rh_provider_metadata(const rh_provider* provider, rh_solver_metadata* out);
+/* supports(): RH_OK or RH_UNSUPPORTED with a static reason string (also for
+ * a request whose grid differs from a configured shape). */
 rh_status rh_provider_supports(const rh_provider* provider, const rh_pressure_request* request,
                                const char** reason);
 /* predict(): writes all N cells of `pressure` (host), p[0] == 0.  Returns
@@ -274,7 +275,8 @@ rh_status rh_provider_predict(rh_provider* provider, const rh_pressure_request* 
 /* ProviderReport (src/solver_provider.rs:78-92); durations in seconds. */
 typedef struct rh_provider_report {
   char selected[48];
-  int32_t attempted;                 /* a preferred provider was offered */
+  int32_t has_attempted;             /* a preferred provider was offered ... */
+  rh_solver_metadata attempted;      /* ... and this is its metadata (Option<SolverMetadata>) */
   rh_status rejection;               /* RH_OK if none */
   int32_t has_candidate;
   rh_pressure_report candidate;

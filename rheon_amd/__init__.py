@@ -361,23 +361,27 @@ class ProviderRejection(Exception):
         self.reason = reason
 
 
+def _metadata_dict(m) -> dict:
+    return {k: (getattr(m, k).decode() if isinstance(getattr(m, k), bytes) else list(getattr(m, k)))
+            for k, _ in N.SolverMetadata._fields_}
+
+
 class GpuPressureProvider:
     """impl PressureProvider (src/solver_provider.rs:67-76) on the device."""
 
     def __init__(self, method: str = "jacobi-pcg-v1", precision: int = F64,
-                 settings: PressureSettings | None = None):
+                 settings: PressureSettings | None = None, shape=None):
         settings = settings or PressureSettings()
-        cfg = N.ProviderConfig(N.SolverConfig(METHOD_IDS[method], precision, 0, 0, 0, 0.0, 0), settings.native())
+        cfg = N.ProviderConfig(N.SolverConfig(METHOD_IDS[method], precision, 0, 0, 0, 0.0, 0), settings.native(),
+                               (C.c_uint64 * 3)(*(shape or (0, 0, 0))))
         h = C.c_void_p()
         _check(lib().rh_provider_create(C.byref(cfg), C.byref(h)), "rh_provider_create")
         self.handle = h.value
 
-    def metadata(self, spec: GridSpec) -> dict:
+    def metadata(self) -> dict:
         m = N.SolverMetadata()
-        g = make_grid(spec)
-        _check(lib().rh_provider_metadata(self.handle, C.byref(g), C.byref(m)), "rh_provider_metadata")
-        return {k: (getattr(m, k).decode() if isinstance(getattr(m, k), bytes) else list(getattr(m, k)))
-                for k, _ in N.SolverMetadata._fields_}
+        _check(lib().rh_provider_metadata(self.handle, C.byref(m)), "rh_provider_metadata")
+        return _metadata_dict(m)
 
     def supports(self, request: PressureRequest) -> None:
         req, _keep = request.native()
@@ -412,7 +416,7 @@ class GpuPressureProvider:
 class ProviderReport:
     """ProviderReport (src/solver_provider.rs:78-92)."""
     selected: str
-    attempted: bool
+    attempted: dict | None  # the offered provider's metadata (Option<SolverMetadata>)
     rejection: int
     candidate: PressureReport | None
     accepted: PressureReport
@@ -445,7 +449,8 @@ class ProviderWorkspace:
         if status != OK:
             raise RheonError(status, "ProviderWorkspace::solve")
         cand = PressureReport.from_native(OK, rep.candidate) if rep.has_candidate else None
-        return ProviderReport(rep.selected.decode(), bool(rep.attempted), rep.rejection, cand,
+        return ProviderReport(rep.selected.decode(), _metadata_dict(rep.attempted) if rep.has_attempted else None,
+                              rep.rejection, cand,
                               PressureReport.from_native(OK, rep.accepted), rep.total, rep.inference,
                               rep.qualification, rep.fallback)
 

@@ -86,13 +86,14 @@ class SolverMetadata(C.Structure):
 
 
 class ProviderConfig(C.Structure):
-    _fields_ = [("solver", SolverConfig), ("settings", PressureSettings)]
+    _fields_ = [("solver", SolverConfig), ("settings", PressureSettings), ("shape_xyz", C.c_uint64 * 3)]
 
 
 class ProviderReport(C.Structure):
     _fields_ = [
         ("selected", C.c_char * 48),
-        ("attempted", C.c_int32),
+        ("has_attempted", C.c_int32),
+        ("attempted", SolverMetadata),
         ("rejection", C.c_int),
         ("has_candidate", C.c_int32),
         ("candidate", PressureReport),
@@ -237,7 +238,7 @@ SIGNATURES = {
                                  C.POINTER(PressureReport), CANCEL_FN, _P]),
     "rh_provider_create": (_S, [C.POINTER(ProviderConfig), C.POINTER(C.c_void_p)]),
     "rh_provider_destroy": (None, [_P]),
-    "rh_provider_metadata": (_S, [_P, _G, C.POINTER(SolverMetadata)]),
+    "rh_provider_metadata": (_S, [_P, C.POINTER(SolverMetadata)]),
     "rh_provider_supports": (_S, [_P, C.POINTER(PressureRequest), C.POINTER(C.c_char_p)]),
     "rh_provider_predict": (_S, [_P, C.POINTER(PressureRequest), _P, C.c_size_t, CANCEL_FN, _P,
                                  C.POINTER(C.c_char_p)]),

@@ -1325,6 +1325,17 @@ rh_status Server::tick(rh_tick_report* rep) {
     }
     return RH_OK;
   };
+  // Requests admitted this tick are running from now on, also when the
+  // launch or the wait below fails: they keep their scene and budget and
+  // join the members (or are retired with the server) like any other.
+  struct KeepJoined {
+    std::vector<ServerRequest*>& joined;
+    std::vector<ServerRequest*>& active;
+    bool armed = true;
+    ~KeepJoined() {
+      if (armed) active.insert(active.end(), joined.begin(), joined.end());
+    }
+  } keep{joined, active_};
   if (members.empty()) {
     if ((s = admission())) return s;
   } else if ((s = batch_.launch(1, admission))) {
@@ -1334,6 +1345,7 @@ rh_status Server::tick(rh_tick_report* rep) {
   std::vector<rh_step_report> reps(members.size());
   std::vector<rh_status> sts(members.size(), RH_OK);
   if (!members.empty() && (s = batch_.wait(reps.data(), sts.data()))) return s;
+  keep.armed = false;
   std::vector<ServerRequest*> still;
   for (size_t i = 0; i < active_.size(); ++i) {
     ServerRequest* r = active_[i];

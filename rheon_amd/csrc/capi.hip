@@ -412,16 +412,14 @@ extern "C" rh_status rh_provider_create(const rh_provider_config* config, rh_pro
 
 extern "C" void rh_provider_destroy(rh_provider* provider) { delete provider; }
 
-extern "C" rh_status rh_provider_metadata(const rh_provider* provider, const rh_grid* grid,
-                                          rh_solver_metadata* out) {
+extern "C" rh_status rh_provider_metadata(const rh_provider* provider, rh_solver_metadata* out) {
   if (!provider || !out) return RH_INVALID_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
   copy_str(out->id, sizeof(out->id),
            method_id(provider->config.solver.method, provider->config.solver.precision));
   copy_str(out->version, sizeof(out->version), "1");
   copy_str(out->stage, sizeof(out->stage), "pressure");
-  if (grid)
-    for (int d = 0; d < 3; ++d) out->shape_xyz[d] = grid->counts[d];
+  for (int d = 0; d < 3; ++d) out->shape_xyz[d] = provider->config.shape_xyz[d];
   copy_str(out->layout, sizeof(out->layout), "x-fastest-cell-centered");
   copy_str(out->scalar, sizeof(out->scalar), "f64");
   copy_str(out->units_in, sizeof(out->units_in), "m^3/s^2");
@@ -443,6 +441,10 @@ static rh_status provider_supports(const rh_provider* provider, const rh_pressur
   if (rh::check_density(*req->grid, req->density)) return why("density must be finite and positive");
   if (!std::isfinite(req->dt) || req->dt <= 0.0) return why("time step must be finite and positive");
   if (req->rhs_len != cells_of(*req->grid)) return why("rhs length does not match the grid");
+  const uint64_t* shape = provider->config.shape_xyz;
+  if ((shape[0] || shape[1] || shape[2]) &&
+      (shape[0] != req->grid->counts[0] || shape[1] != req->grid->counts[1] || shape[2] != req->grid->counts[2]))
+    return why("grid shape differs from the provider's shape");
   const rh_method m = provider->config.solver.method;
   if (m != RH_JACOBI_PCG && m != RH_RBSGS_PCG && m != RH_MIXED_MG)
     return why("method not available in this build");
@@ -594,9 +596,9 @@ extern "C" rh_status rh_provider_workspace_solve(rh_provider_workspace* ws,
   if (cancelled()) return RH_CANCELLED;
   copy_str(report->selected, sizeof(report->selected), method_id(ws->fallback, RH_F64));
   if (preferred) {
-    report->attempted = 1;
-    rh_solver_metadata meta;
-    rh_provider_metadata(preferred, &ws->grid, &meta);
+    report->has_attempted = 1;
+    rh_solver_metadata& meta = report->attempted;
+    rh_provider_metadata(preferred, &meta);
     const char* reason = nullptr;
     rh_status attempt = rh_provider_supports(preferred, req, &reason);
     if (attempt == RH_OK) {

@@ -398,9 +398,15 @@ def test_provider_contract_and_workspace_qualification():
     g = ogrid(spec)
     rhs = O.build_rhs(g, vel, dt)
     req = R.PressureRequest(spec, 1.0, dt, rhs)
-    prov = R.GpuPressureProvider("rbsgs-pcg", R.F64, R.PressureSettings(1e-10, 1e-12, 1e-7, 2000))
-    meta = prov.metadata(spec)
+    prov = R.GpuPressureProvider("rbsgs-pcg", R.F64, R.PressureSettings(1e-10, 1e-12, 1e-7, 2000),
+                                 shape=(32, 32, 32))
+    meta = prov.metadata()
     assert meta["stage"] == "pressure" and meta["shape_xyz"] == [32, 32, 32]
+    # a provider for any shape says so; one for another shape declines the request
+    assert R.GpuPressureProvider("jacobi-pcg-v1").metadata()["shape_xyz"] == [0, 0, 0]
+    with pytest.raises(R.ProviderRejection) as e:
+        R.GpuPressureProvider("jacobi-pcg-v1", shape=(16, 32, 32)).supports(req)
+    assert e.value.status == R.UNSUPPORTED
     assert meta["layout"] == "x-fastest-cell-centered" and meta["scalar"] == "f64"
     prov.supports(req)
     out = np.full(spec.cell_len, np.nan)
@@ -410,12 +416,14 @@ def test_provider_contract_and_workspace_qualification():
     ws = R.ProviderWorkspace(spec)
     rep = ws.solve(req, settings, prov)
     assert rep.selected == "rbsgs-pcg" and rep.rejection == R.OK and rep.candidate is not None
+    assert rep.attempted == meta
     assert rep.accepted.true_residual_l2 <= rep.candidate.threshold
     # a provider with a looser stopping rule fails the host quality gate and
     # the native device jacobi-pcg-v1 fallback answers, as the reference's.
     loose = R.GpuPressureProvider("jacobi-pcg-v1", R.F64, R.PressureSettings(1e-3, 1e-12, 1.0, 2000))
     rep = ws.solve(req, settings, loose)
     assert rep.rejection == R.QUALITY_GATE and rep.selected == "jacobi-pcg-v1"
+    assert rep.attempted == loose.metadata()
     ref = O.solve_rhs(g, 1.0, rhs, dt, osettings(settings))
     assert rep.accepted.iterations == ref.iterations
     assert bits(ws.pressure()) == bits(ref.pressure)
@@ -425,7 +433,7 @@ def test_provider_contract_and_workspace_qualification():
     assert e.value.status == R.UNSUPPORTED
     # no provider: straight fallback
     rep = ws.solve(req, settings, None)
-    assert not rep.attempted and rep.selected == "jacobi-pcg-v1"
+    assert rep.attempted is None and rep.selected == "jacobi-pcg-v1"
 
 
 def test_fourier_field_projection_is_divergence_free():
