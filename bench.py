@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 import rheon_amd as R  # noqa: E402
 from rheon_amd import _native as N  # noqa: E402
-from rheon_amd.scenes import GridSpec, centred_sphere, courant_dt, fourier_modes  # noqa: E402
+from rheon_amd.scenes import GridSpec, centred_sphere, courant_dt, fourier_modes, serving_requests  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
@@ -339,14 +339,12 @@ def run_batch(args) -> dict:
     seed = scene id, 10 projection steps with the mixed-precision multigrid
     solve, in one hipGraph replay per step with autotuned launch shapes;
     against looping the one-scene path over the same scenes."""
-    from rheon_amd.scenes import isotropic_spec, log_uniform_sizes, taylor_green_noise
+    from rheon_amd.scenes import batch_workload
 
-    counts = log_uniform_sizes(seed=4, count=args.scenes, lo=32, hi=128, multiple=8)
-    specs = [isotropic_spec(c) for c in counts]
+    specs, vels, dts = batch_workload(args.scenes)
+    counts = [list(sp.counts) for sp in specs]
     cells = sum(sp.cell_len for sp in specs)
     st = R.PressureSettings(args.rel, 1e-14, 1e-3, 40)
-    vels = [taylor_green_noise(sp, seed=i) for i, sp in enumerate(specs)]
-    dts = [courant_dt(sp, v, 0.5) for sp, v in zip(specs, vels)]
     # the graph holds a fixed number of V-cycles per step (converged scenes'
     # blocks return at once); 20 is twice what these scenes need to 1e-8
     cycles = 20
@@ -421,38 +419,6 @@ def run_batch(args) -> dict:
     return result
 
 
-def serve_requests(count: int, seed: int):
-    """Requests of the serving workload: each axis log-uniform on [32, 192]
-    and a multiple of 8 (cubic cells, longest side 1), run length uniform on
-    {10..200}, 16 Fourier modes (|k| <= 8) with max|u| = 0.02 (so that the
-    f32 rounding floor of max|div u|, ~6 * 2^-25 |u| / h, stays below 1e-6),
-    and 0-3 solid spheres of radius U[0.05, 0.2] kept clear of the walls."""
-    from rheon_amd.scenes import (close_obstacle_faces, isotropic_spec, lcg_uniform, log_uniform_sizes,
-                                  spheres_solid)
-
-    sizes = log_uniform_sizes(seed, count, 32, 192, 8)
-    u = lcg_uniform(seed + 1, 8 * count) + 0.5
-    out = []
-    for i, counts in enumerate(sizes):
-        spec = isotropic_spec(counts)
-        r = u[8 * i:8 * i + 8]
-        steps = 10 + int(r[0] * 191) % 191
-        nsph = int(r[1] * 4) % 4
-        ext = [spec.counts[d] * spec.spacing[d] for d in range(3)]
-        sph = []
-        su = lcg_uniform(seed + 100 + i, 4 * nsph) + 0.5
-        for k in range(nsph):
-            rad = min(0.05 + 0.15 * su[4 * k], 0.45 * min(ext) - 0.05)
-            c = [rad + 0.05 + su[4 * k + 1 + d] * (ext[d] - 2 * rad - 0.1) for d in range(3)]
-            sph.append((c[0], c[1], c[2], rad))
-        solid = spheres_solid(spec, sph) if sph else None
-        vel = fourier_modes(spec, seed + 1000 + i, modes=16, kmax=8, amplitude=0.02, dtype=np.float32)
-        if solid is not None:
-            vel = close_obstacle_faces(spec, solid, vel)
-        out.append((spec, vel, courant_dt(spec, vel, 0.5), steps, solid, len(sph)))
-    return out
-
-
 def bench_serve(args) -> int:
     result = run_serve(args)
     line = json.dumps(result)
@@ -469,7 +435,7 @@ def run_serve(args) -> dict:
     and total steps/s at max|div u| <= 1e-6."""
     rng = np.random.default_rng(args.seed)
     t_gen = time.perf_counter()
-    reqs = serve_requests(args.requests, args.seed)
+    reqs = serving_requests(args.requests, args.seed)
     t_gen = time.perf_counter() - t_gen
     gaps = rng.exponential(1.0 / args.rate, size=len(reqs))
     arrivals = np.cumsum(gaps) - gaps[0]

@@ -263,3 +263,44 @@ def isotropic_spec(counts) -> GridSpec:
 def courant_dt(spec: GridSpec, fields, courant: float = 0.5) -> float:
     """dt = courant * h / max|u| (the explicit step of the workload recipes)."""
     return courant * min(spec.spacing) / max_abs(fields)
+
+
+def serving_requests(count: int, seed: int):
+    """Requests of the serving workload: each axis log-uniform on [32, 192]
+    and a multiple of 8 (cubic cells, longest side 1), run length uniform on
+    {10..200}, 16 Fourier modes (|k| <= 8) with max|u| = 0.02 (so that the
+    f32 rounding floor of max|div u|, ~6 * 2^-25 |u| / h, stays below 1e-6),
+    and 0-3 solid spheres of radius U[0.05, 0.2] kept clear of the walls."""
+    sizes = log_uniform_sizes(seed, count, 32, 192, 8)
+    u = lcg_uniform(seed + 1, 8 * count) + 0.5
+    out = []
+    for i, counts in enumerate(sizes):
+        spec = isotropic_spec(counts)
+        r = u[8 * i:8 * i + 8]
+        steps = 10 + int(r[0] * 191) % 191
+        nsph = int(r[1] * 4) % 4
+        ext = [spec.counts[d] * spec.spacing[d] for d in range(3)]
+        sph = []
+        su = lcg_uniform(seed + 100 + i, 4 * nsph) + 0.5
+        for k in range(nsph):
+            rad = min(0.05 + 0.15 * su[4 * k], 0.45 * min(ext) - 0.05)
+            c = [rad + 0.05 + su[4 * k + 1 + d] * (ext[d] - 2 * rad - 0.1) for d in range(3)]
+            sph.append((c[0], c[1], c[2], rad))
+        solid = spheres_solid(spec, sph) if sph else None
+        vel = fourier_modes(spec, seed + 1000 + i, modes=16, kmax=8, amplitude=0.02, dtype=np.float32)
+        if solid is not None:
+            vel = close_obstacle_faces(spec, solid, vel)
+        out.append((spec, vel, courant_dt(spec, vel, 0.5), steps, solid, len(sph)))
+    return out
+
+
+def batch_workload(count: int = 64, seed: int = 4):
+    """Scenes of the batched workload: each axis log-uniform on [32, 128] and
+    a multiple of 8 (cubic cells, longest side 1), Taylor-Green + LCG noise
+    with seed = scene index, dt = 0.5 h / max|u|.  Returns (specs, fields,
+    dts)."""
+    counts = log_uniform_sizes(seed=seed, count=count, lo=32, hi=128, multiple=8)
+    specs = [isotropic_spec(c) for c in counts]
+    vels = [taylor_green_noise(sp, seed=i) for i, sp in enumerate(specs)]
+    dts = [courant_dt(sp, v, 0.5) for sp, v in zip(specs, vels)]
+    return specs, vels, dts
