@@ -166,6 +166,16 @@ rh_status rh_correct_velocity(const rh_grid* grid, rh_precision precision, doubl
 /* actual_divergence_max (src/pressure.rs:330-342 with src/operator.rs:513-568). */
 rh_status rh_divergence_max(const rh_grid* grid, rh_precision precision, const void* d_u,
                             const void* d_v, const void* d_w, double* out_max);
+/* rh_divergence_max without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  Same checks and result as rh_divergence_max, on the
+ * null stream, with its reduction scratch (partials for kMaxBlocks = 16384
+ * blocks, flags, result: 1.5 MiB) in a fixed device array that each GPU holds
+ * until the process ends; every grid that passes the 32-bit indexing check
+ * fits.  Calls no hipMalloc/hipFree.  Calls that overlap on one GPU race on
+ * that GPU's array. */
+rh_status rh_divergence_max_noalloc(const rh_grid* grid, rh_precision precision, const void* d_u,
+                                    const void* d_v, const void* d_w, double* out_max);
 /* velocity_rate (src/simulation.rs:1415-1429): sum_d max|u_d| / h_d. */
 rh_status rh_velocity_rate(const rh_grid* grid, rh_precision precision, const void* d_u,
                            const void* d_v, const void* d_w, double* out_rate);

@@ -228,6 +228,7 @@ SIGNATURES = {
     "rh_build_rhs": (_S, [_G, C.c_int, _P, _P, _P, C.c_double, _P]),
     "rh_correct_velocity": (_S, [_G, C.c_int, C.c_double, _P, C.c_double, _P, _P, _P]),
     "rh_divergence_max": (_S, [_G, C.c_int, _P, _P, _P, C.POINTER(C.c_double)]),
+    "rh_divergence_max_noalloc": (_S, [_G, C.c_int, _P, _P, _P, C.POINTER(C.c_double)]),
     "rh_velocity_rate": (_S, [_G, C.c_int, _P, _P, _P, C.POINTER(C.c_double)]),
     "rh_precondition": (_S, [_G, C.c_double, C.c_int, _P, _P, C.POINTER(C.c_double)]),
     "rh_solver_create": (_S, [_G, C.POINTER(SolverConfig), C.POINTER(C.c_void_p)]),
