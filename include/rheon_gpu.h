@@ -65,7 +65,8 @@ typedef enum rh_status {
   RH_COURANT_LIMIT = 19,       /* SimulationError::CourantLimit */
   RH_GEOMETRY = 20,            /* GeometryError::* */
   RH_BUFFER_LIMIT = 21,        /* PressureError::BufferLimit / GeometryError::BufferLimit */
-  RH_DEVICE_ERROR = 22         /* HIP runtime failure (message in rh_last_error) */
+  RH_DEVICE_ERROR = 22,        /* HIP runtime failure (message in rh_last_error) */
+  RH_REFUSED = 23              /* ProjectionError::Refused (the kind is in rh_projection_report.refusal) */
 } rh_status;
 
 /* GridGeometry::new inputs (src/geometry.rs:114-118). */
@@ -310,6 +311,102 @@ rh_status rh_provider_workspace_solve(rh_provider_workspace* ws,
 /* pressure(): the last accepted pressure (host copy, N f64). */
 rh_status rh_provider_workspace_pressure(const rh_provider_workspace* ws, double* out,
                                          size_t len);
+
+/* ------------------------------- supplied-projection qualification (device) */
+/* ProjectionWorkspace::qualify and its request (src/solver_provider/composition/
+ * projection.rs:24-131, :238-542): a supplied (pressure, corrected velocity)
+ * pair for a sealed-box projection is accepted only when the velocity is, bit
+ * for bit, the input velocity corrected by the supplied pressure (the momentum
+ * witness), the pressure passes the provider workspace's requalification and
+ * the corrected faces pass the actual-divergence gate; otherwise the native
+ * device solve answers.  Arrays are host memory; the work runs on the device. */
+typedef enum rh_projection_domain {
+  RH_DOMAIN_SEALED_BOX = 0, RH_DOMAIN_FREE_SURFACE = 1, RH_DOMAIN_EMBEDDED_OBSTACLE = 2,
+  RH_DOMAIN_MOVING_WALL = 3
+} rh_projection_domain;
+/* ProjectionInput (:31-42); SessionIdentity = (run_id, source_revision). */
+typedef struct rh_projection_input {
+  const rh_grid* grid;
+  double density;
+  double interval_start;
+  double dt;
+  uint64_t run_id;
+  const char* source_revision;      /* UTF-8, 1..4096 bytes, no control characters */
+  uint64_t generation;
+  const float* velocity[3];         /* provisional faces (host) */
+  size_t velocity_len[3];
+  rh_pressure_settings settings;
+  double actual_divergence_limit;
+  rh_projection_domain domain;
+} rh_projection_input;
+/* TensorMetadata and ProjectionRepresentation (:150-179). */
+typedef struct rh_tensor_metadata {
+  uint64_t shape_xyz[3];
+  char layout[32];
+  char scalar[8];
+  char units[16];
+} rh_tensor_metadata;
+typedef struct rh_projection_representation {
+  uint32_t version;
+  char operator_id[64];
+  rh_tensor_metadata pressure;
+  rh_tensor_metadata velocity[3];
+} rh_projection_representation;
+/* SuppliedProjection (:187-193) with its ProjectionCandidateIdentity. */
+typedef struct rh_projection_candidate {
+  uint8_t input_id[32];
+  rh_projection_representation representation;
+  const char* algorithm;            /* 1..256 bytes */
+  const char* version;              /* 1..256 bytes */
+  const char* source_revision;      /* 1..4096 bytes */
+  const double* pressure;
+  size_t pressure_len;
+  const float* velocity[3];
+  size_t velocity_len[3];
+} rh_projection_candidate;
+/* ProjectionSelection, ProjectionRejection, ProjectionRefusal (:195-219). */
+enum { RH_SELECT_SUPPLIED = 0, RH_SELECT_NATIVE_BASELINE = 1, RH_SELECT_NATIVE_FALLBACK = 2 };
+enum { RH_REJECT_NONE = 0, RH_REJECT_FIELD = 1, RH_REJECT_MOMENTUM_WITNESS = 2, RH_REJECT_PRESSURE = 3,
+       RH_REJECT_ACTUAL_DIVERGENCE = 4 };
+enum { RH_REFUSE_NONE = 0, RH_REFUSE_DOMAIN = 1, RH_REFUSE_WORKSPACE_GRID = 2, RH_REFUSE_INPUT_BINDING = 3,
+       RH_REFUSE_REPRESENTATION = 4, RH_REFUSE_IDENTITY = 5 };
+/* ProjectionReport (:256-270). */
+typedef struct rh_projection_report {
+  uint8_t input_id[32];
+  int32_t selection;
+  int32_t refusal;                  /* set with RH_REFUSED */
+  int32_t rejection;                /* RH_REJECT_* */
+  rh_status rejection_status;       /* FIELD: the OperatorError; PRESSURE: the ProviderRejection */
+  int32_t witness_axis;             /* MOMENTUM_WITNESS: first differing face (axis, index); */
+  int32_t wall_axis;                /* FIELD + NONZERO_WALL_VELOCITY: the offending face */
+  int64_t witness_index;
+  int64_t wall_index;
+  double rejected_divergence;       /* ACTUAL_DIVERGENCE: the maximum that failed */
+  int32_t has_supplied_identity;
+  int32_t has_discarded;
+  rh_provider_report pressure;
+  rh_provider_report discarded_candidate_pressure;  /* valid when has_discarded */
+  double actual_divergence_max;
+} rh_projection_report;
+typedef struct rh_projection_workspace rh_projection_workspace;
+/* ProjectionRequest::new (:58-131): validates the input and returns its
+ * input id (SHA-256 of the contract, representation, grid, identity,
+ * settings and every face's bits, in the reference's byte order). */
+rh_status rh_projection_input_id(const rh_projection_input* input, uint8_t out[32]);
+/* ProjectionRepresentation::sealed_v1 (:158-170). */
+rh_status rh_projection_representation_of(const rh_grid* grid, rh_projection_representation* out);
+/* ProjectionWorkspace::new (:283-335): limit = byte cap (16 N + 4 F + the
+ * provider workspace), 0 = none; fallback = RH_JACOBI_PCG or RH_RBSGS_PCG. */
+rh_status rh_projection_workspace_create(const rh_grid* grid, uint64_t limit, rh_method fallback,
+                                         rh_projection_workspace** out);
+void rh_projection_workspace_destroy(rh_projection_workspace* ws);
+/* qualify (:350-479).  candidate may be NULL (native baseline).  On RH_OK the
+ * qualified pressure and faces are read with rh_projection_result. */
+rh_status rh_projection_qualify(rh_projection_workspace* ws, const rh_projection_input* input,
+                                const rh_projection_candidate* candidate, rh_cancel_fn cancel,
+                                void* cancel_ctx, rh_projection_report* report);
+rh_status rh_projection_result(const rh_projection_workspace* ws, double* pressure, size_t pressure_len,
+                               float* u, float* v, float* w);
 
 /* --------------------------------------------- device-resident scene step */
 /* A fixed-box carrier with accepted and candidate faces on the device

@@ -43,6 +43,7 @@ COURANT_LIMIT = 19
 GEOMETRY = 20
 BUFFER_LIMIT = 21
 DEVICE_ERROR = 22
+REFUSED = 23
 
 STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) and k.isupper()}
 
@@ -754,6 +755,137 @@ class ProjectionServer:
     def close(self):
         if self.handle:
             lib().rh_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------- supplied-projection qualification
+DOMAIN_SEALED_BOX, DOMAIN_FREE_SURFACE, DOMAIN_EMBEDDED_OBSTACLE, DOMAIN_MOVING_WALL = 0, 1, 2, 3
+SELECTIONS = {0: "supplied", 1: "native_baseline", 2: "native_fallback"}
+REJECTIONS = {0: None, 1: "field", 2: "momentum_witness", 3: "pressure", 4: "actual_divergence"}
+REFUSALS = {0: None, 1: "domain", 2: "workspace_grid", 3: "input_binding", 4: "representation", 5: "identity"}
+
+
+@dataclass
+class ProjectionInput:
+    """ProjectionInput (src/solver_provider/composition/projection.rs:31-42)."""
+    spec: GridSpec
+    velocity: list
+    density: float = 1.25
+    interval_start: float = 0.0
+    dt: float = 0.01
+    run_id: int = 1
+    source_revision: str = "rheon-amd"
+    generation: int = 0
+    settings: PressureSettings = field(default_factory=lambda: PressureSettings(1e-10, 1e-12, 1e-8, 1000))
+    actual_divergence_limit: float = 1e-5
+    domain: int = DOMAIN_SEALED_BOX
+
+    def native(self):
+        g = make_grid(self.spec)
+        vel = [np.ascontiguousarray(v, np.float32) for v in self.velocity]
+        keep = [g, vel, self.source_revision.encode()]
+        n = N.ProjectionInput(C.pointer(g), self.density, self.interval_start, self.dt, int(self.run_id), keep[2],
+                              int(self.generation), (C.c_void_p * 3)(*[v.ctypes.data for v in vel]),
+                              (C.c_size_t * 3)(*[v.size for v in vel]), self.settings.native(),
+                              self.actual_divergence_limit, int(self.domain))
+        return n, keep
+
+
+def projection_input_id(inp: ProjectionInput) -> bytes:
+    """ProjectionRequest::new: the input id (raises RheonError on an invalid input)."""
+    n, _keep = inp.native()
+    out = (C.c_uint8 * 32)()
+    _check(lib().rh_projection_input_id(C.byref(n), out), "rh_projection_input_id")
+    return bytes(out)
+
+
+def projection_representation(spec: GridSpec):
+    r = N.ProjectionRepresentation()
+    g = make_grid(spec)
+    _check(lib().rh_projection_representation_of(C.byref(g), C.byref(r)), "rh_projection_representation_of")
+    return r
+
+
+@dataclass
+class ProjectionCandidate:
+    """SuppliedProjection (projection.rs:187-193)."""
+    input_id: bytes
+    pressure: np.ndarray
+    velocity: list
+    representation: object = None
+    algorithm: str = "classical-supplied-projection"
+    version: str = "1"
+    source_revision: str = "rheon-amd-supplied"
+
+    def native(self, spec: GridSpec):
+        p = np.ascontiguousarray(self.pressure, np.float64)
+        vel = [np.ascontiguousarray(v, np.float32) for v in self.velocity]
+        rep = self.representation if self.representation is not None else projection_representation(spec)
+        texts = [self.algorithm.encode(), self.version.encode(), self.source_revision.encode()]
+        n = N.ProjectionCandidate((C.c_uint8 * 32)(*self.input_id), rep, texts[0], texts[1], texts[2],
+                                  p.ctypes.data, p.size, (C.c_void_p * 3)(*[v.ctypes.data for v in vel]),
+                                  (C.c_size_t * 3)(*[v.size for v in vel]))
+        return n, (p, vel, texts, rep)
+
+
+def _provider_report(r) -> ProviderReport:
+    cand = PressureReport.from_native(OK, r.candidate) if r.has_candidate else None
+    return ProviderReport(r.selected.decode(), _metadata_dict(r.attempted) if r.has_attempted else None, r.rejection,
+                          cand, PressureReport.from_native(OK, r.accepted), r.total, r.inference, r.qualification,
+                          r.fallback)
+
+
+class ProjectionWorkspace:
+    """ProjectionWorkspace (projection.rs:275-479) on the device: qualify()
+    accepts a supplied projection only through the momentum witness, the
+    pressure requalification and the actual-divergence gate, else answers
+    with the native device solve."""
+
+    def __init__(self, spec: GridSpec, limit: int = 0, fallback: str = "jacobi-pcg-v1"):
+        self.spec = spec
+        g = make_grid(spec)
+        h = C.c_void_p()
+        _check(lib().rh_projection_workspace_create(C.byref(g), int(limit), METHOD_IDS[fallback], C.byref(h)),
+               "rh_projection_workspace_create")
+        self.handle = h.value
+
+    def qualify(self, inp: ProjectionInput, candidate: ProjectionCandidate | None = None, cancelled=None):
+        """Returns (status, report dict)."""
+        n, _k1 = inp.native()
+        if candidate is not None:
+            c, _k2 = candidate.native(self.spec)
+            cp = C.byref(c)
+        else:
+            cp = None
+        rep = N.ProjectionReport()
+        cb, _k3 = _cancel_cb(cancelled)
+        st = lib().rh_projection_qualify(self.handle, C.byref(n), cp, cb, None, C.byref(rep))
+        out = {"input_id": bytes(rep.input_id), "selection": SELECTIONS.get(rep.selection),
+               "refusal": REFUSALS.get(rep.refusal), "rejection": REJECTIONS.get(rep.rejection),
+               "rejection_status": rep.rejection_status, "witness": (rep.witness_axis, rep.witness_index),
+               "wall": (rep.wall_axis, rep.wall_index), "rejected_divergence": rep.rejected_divergence,
+               "supplied_identity": bool(rep.has_supplied_identity),
+               "pressure": _provider_report(rep.pressure),
+               "discarded": _provider_report(rep.discarded_candidate_pressure) if rep.has_discarded else None,
+               "actual_divergence_max": rep.actual_divergence_max}
+        return st, out
+
+    def result(self):
+        p = np.empty(self.spec.cell_len, np.float64)
+        vel = [np.empty(self.spec.face_len(a), np.float32) for a in range(3)]
+        _check(lib().rh_projection_result(self.handle, p.ctypes.data, p.size, vel[0].ctypes.data,
+                                          vel[1].ctypes.data, vel[2].ctypes.data), "rh_projection_result")
+        return p, vel
+
+    def close(self):
+        if self.handle:
+            lib().rh_projection_workspace_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -200,6 +200,67 @@ class TickReport(C.Structure):
 
 
 
+class ProjectionInput(C.Structure):
+    _fields_ = [
+        ("grid", C.POINTER(Grid)),
+        ("density", C.c_double),
+        ("interval_start", C.c_double),
+        ("dt", C.c_double),
+        ("run_id", C.c_uint64),
+        ("source_revision", C.c_char_p),
+        ("generation", C.c_uint64),
+        ("velocity", C.c_void_p * 3),
+        ("velocity_len", C.c_size_t * 3),
+        ("settings", PressureSettings),
+        ("actual_divergence_limit", C.c_double),
+        ("domain", C.c_int),
+    ]
+
+
+class TensorMetadata(C.Structure):
+    _fields_ = [("shape_xyz", C.c_uint64 * 3), ("layout", C.c_char * 32), ("scalar", C.c_char * 8),
+                ("units", C.c_char * 16)]
+
+
+class ProjectionRepresentation(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("operator_id", C.c_char * 64), ("pressure", TensorMetadata),
+                ("velocity", TensorMetadata * 3)]
+
+
+class ProjectionCandidate(C.Structure):
+    _fields_ = [
+        ("input_id", C.c_uint8 * 32),
+        ("representation", ProjectionRepresentation),
+        ("algorithm", C.c_char_p),
+        ("version", C.c_char_p),
+        ("source_revision", C.c_char_p),
+        ("pressure", C.c_void_p),
+        ("pressure_len", C.c_size_t),
+        ("velocity", C.c_void_p * 3),
+        ("velocity_len", C.c_size_t * 3),
+    ]
+
+
+class ProjectionReport(C.Structure):
+    _fields_ = [
+        ("input_id", C.c_uint8 * 32),
+        ("selection", C.c_int32),
+        ("refusal", C.c_int32),
+        ("rejection", C.c_int32),
+        ("rejection_status", C.c_int),
+        ("witness_axis", C.c_int32),
+        ("wall_axis", C.c_int32),
+        ("witness_index", C.c_int64),
+        ("wall_index", C.c_int64),
+        ("rejected_divergence", C.c_double),
+        ("has_supplied_identity", C.c_int32),
+        ("has_discarded", C.c_int32),
+        ("pressure", ProviderReport),
+        ("discarded_candidate_pressure", ProviderReport),
+        ("actual_divergence_max", C.c_double),
+    ]
+
+
 CANCEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 _P = C.c_void_p
@@ -257,6 +318,13 @@ SIGNATURES = {
     "rh_scene_get_pressure": (_S, [_P, _P, C.c_size_t]),
     "rh_scene_device_velocity": (_S, [_P, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "rh_scene_step": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
+    "rh_projection_input_id": (_S, [C.POINTER(ProjectionInput), C.c_uint8 * 32]),
+    "rh_projection_representation_of": (_S, [_G, C.POINTER(ProjectionRepresentation)]),
+    "rh_projection_workspace_create": (_S, [_G, C.c_uint64, C.c_int, C.POINTER(C.c_void_p)]),
+    "rh_projection_workspace_destroy": (None, [_P]),
+    "rh_projection_qualify": (_S, [_P, C.POINTER(ProjectionInput), C.POINTER(ProjectionCandidate), CANCEL_FN, _P,
+                                   C.POINTER(ProjectionReport)]),
+    "rh_projection_result": (_S, [_P, _P, C.c_size_t, _P, _P, _P]),
     "rh_scene_step_requested": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
     "rh_scene_set_tracer": (_S, [_P, _P, C.c_size_t]),
     "rh_scene_get_tracer": (_S, [_P, _P, C.c_size_t]),

@@ -387,18 +387,6 @@ extern "C" rh_status rh_solver_set_solid(rh_solver* solver, const uint8_t* solid
 
 // ------------------------------------------------------------- provider
 
-struct rh_provider {
-  rh_provider_config config{};
-  std::unique_ptr<rh_solver> solver;
-  rh_grid grid{};
-  double* d_rhs = nullptr;
-  double* d_p = nullptr;
-  size_t cap = 0;
-  ~rh_provider() {
-    if (d_rhs) (void)hipFree(d_rhs);
-    if (d_p) (void)hipFree(d_p);
-  }
-};
 
 extern "C" rh_status rh_provider_create(const rh_provider_config* config, rh_provider** out) {
   if (!config || !out) return RH_INVALID_ARGUMENT;
@@ -416,7 +404,8 @@ extern "C" rh_status rh_provider_metadata(const rh_provider* provider, rh_solver
   if (!provider || !out) return RH_INVALID_ARGUMENT;
   std::memset(out, 0, sizeof(*out));
   copy_str(out->id, sizeof(out->id),
-           method_id(provider->config.solver.method, provider->config.solver.precision));
+           provider->supplied ? kSuppliedPressureId
+                              : method_id(provider->config.solver.method, provider->config.solver.precision));
   copy_str(out->version, sizeof(out->version), "1");
   copy_str(out->stage, sizeof(out->stage), "pressure");
   for (int d = 0; d < 3; ++d) out->shape_xyz[d] = provider->config.shape_xyz[d];
@@ -424,7 +413,8 @@ extern "C" rh_status rh_provider_metadata(const rh_provider* provider, rh_solver
   copy_str(out->scalar, sizeof(out->scalar), "f64");
   copy_str(out->units_in, sizeof(out->units_in), "m^3/s^2");
   copy_str(out->units_out, sizeof(out->units_out), "Pa");
-  copy_str(out->domain, sizeof(out->domain), "sealed-constant-density-mac-box");
+  copy_str(out->domain, sizeof(out->domain),
+           provider->supplied ? "sealed box supplied-array witness only" : "sealed-constant-density-mac-box");
   return RH_OK;
 }
 
@@ -436,6 +426,7 @@ static rh_status provider_supports(const rh_provider* provider, const rh_pressur
   };
   if (reason) *reason = nullptr;
   if (!provider || !req || !req->grid) return RH_INVALID_ARGUMENT;
+  if (provider->supplied) return RH_OK;  // SuppliedPressure::supports (projection.rs:581-583)
   if (rh::validate_grid(req->grid, nullptr)) return why("invalid grid geometry");
   if (rh::check_indexable(req->grid)) return why("grid exceeds 32-bit device indexing");
   if (rh::check_density(*req->grid, req->density)) return why("density must be finite and positive");
@@ -464,6 +455,11 @@ extern "C" rh_status rh_provider_predict(rh_provider* provider, const rh_pressur
   if (s) return s;
   if (!pressure || pressure_len != req->rhs_len || !req->rhs) return RH_INVALID_ARGUMENT;
   const size_t n = req->rhs_len;
+  if (provider->supplied) {  // SuppliedPressure::predict: the supplied values (projection.rs:584-592)
+    if (provider->supplied_len != n) return RH_INVALID_ARGUMENT;
+    std::memcpy(pressure, provider->supplied, n * sizeof(double));
+    return RH_OK;
+  }
   auto exec = [&](const char* r) {
     if (reason) *reason = r;
     return RH_EXECUTION;
@@ -498,18 +494,6 @@ extern "C" rh_status rh_provider_predict(rh_provider* provider, const rh_pressur
 
 // ---------------------------------------------------- provider workspace
 
-struct rh_provider_workspace {
-  rh_grid grid{};
-  rh_method fallback = RH_JACOBI_PCG;
-  std::unique_ptr<rh_solver> solver;  // native fallback on the device
-  double* d_rhs = nullptr;
-  double* d_p = nullptr;
-  std::vector<double> pressure;  // last result (host)
-  ~rh_provider_workspace() {
-    if (d_rhs) (void)hipFree(d_rhs);
-    if (d_p) (void)hipFree(d_p);
-  }
-};
 
 extern "C" rh_status rh_provider_workspace_create(const rh_grid* grid, uint64_t limit,
                                                   rh_method fallback,

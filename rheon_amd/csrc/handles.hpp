@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <vector>
 
 #include "runtime.hpp"
 
@@ -48,3 +49,38 @@ struct rh_scene {
   }
 };
 
+// impl PressureProvider for the GPU solver (src/solver_provider.rs:67-76),
+// or, with `supplied` set, the supplied-array pressure witness of the
+// projection qualification (SuppliedPressure, src/solver_provider/
+// composition/projection.rs:554-593): predict() returns the supplied values.
+struct rh_provider {
+  rh_provider_config config{};
+  std::unique_ptr<rh_solver> solver;
+  rh_grid grid{};
+  double* d_rhs = nullptr;
+  double* d_p = nullptr;
+  size_t cap = 0;
+  const double* supplied = nullptr;
+  size_t supplied_len = 0;
+  ~rh_provider() {
+    if (d_rhs) (void)hipFree(d_rhs);
+    if (d_p) (void)hipFree(d_p);
+  }
+};
+
+// ProviderWorkspace (src/solver_provider.rs:94-327) on the device.
+struct rh_provider_workspace {
+  rh_grid grid{};
+  rh_method fallback = RH_JACOBI_PCG;
+  std::unique_ptr<rh_solver> solver;  // native fallback on the device
+  double* d_rhs = nullptr;
+  double* d_p = nullptr;             // the last selected pressure (device)
+  std::vector<double> pressure;      // the last selected pressure (host)
+  ~rh_provider_workspace() {
+    if (d_rhs) (void)hipFree(d_rhs);
+    if (d_p) (void)hipFree(d_p);
+  }
+};
+
+// SolverMetadata id of the supplied-array pressure witness.
+constexpr const char* kSuppliedPressureId = "supplied-projection-pressure-witness-v1";
