@@ -88,7 +88,8 @@ typedef enum rh_method {
   RH_JACOBI_PCG = 0,
   RH_RBSGS_PCG = 1,
   RH_MG_PCG = 2,
-  RH_MIXED_MG = 3
+  RH_MIXED_MG = 3,
+  RH_SGS_PCG = 4   /* the reference's "sgs-pcg-v1" (src/pressure.rs:467-532), iterate-for-iterate; box only */
 } rh_method;
 
 /* PressureSettings (src/pressure.rs:33-39) + device polling cadence. */

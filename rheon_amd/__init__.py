@@ -51,9 +51,9 @@ F64, F32 = 0, 1
 VARIANT_SCALAR_MG = 1
 SMOOTHER_CHEBYSHEV = 0  # multigrid smoothers (rh_solver_config.mg_smoother)
 SMOOTHER_JACOBI = 1
-JACOBI_PCG, RBSGS_PCG, MG_PCG, MIXED_MG = 0, 1, 2, 3
+JACOBI_PCG, RBSGS_PCG, MG_PCG, MIXED_MG, SGS_PCG = 0, 1, 2, 3, 4
 METHOD_IDS = {"jacobi-pcg-v1": JACOBI_PCG, "rbsgs-pcg": RBSGS_PCG, "mg-pcg": MG_PCG,
-              "mixed-mg-dc": MIXED_MG, "mg-f64-dc": MIXED_MG}
+              "mixed-mg-dc": MIXED_MG, "mg-f64-dc": MIXED_MG, "sgs-pcg-v1": SGS_PCG}
 
 
 class RheonError(RuntimeError):

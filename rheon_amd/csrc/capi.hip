@@ -56,6 +56,7 @@ const char* method_id(rh_method m, rh_precision p) {
   switch (m) {
     case RH_JACOBI_PCG: return p == RH_F64 ? "jacobi-pcg-v1" : "jacobi-pcg-f32";
     case RH_RBSGS_PCG: return p == RH_F64 ? "rbsgs-pcg" : "rbsgs-pcg-f32";
+    case RH_SGS_PCG: return p == RH_F64 ? "sgs-pcg-v1" : "sgs-pcg-f32";
     case RH_MG_PCG: return p == RH_F64 ? "mg-pcg" : "mg-pcg-f32";
     case RH_MIXED_MG: return p == RH_F32 ? "mixed-mg-dc" : "mg-f64-dc";
   }
@@ -324,19 +325,33 @@ extern "C" rh_status rh_precondition(const rh_grid* grid, double density, rh_met
   rh_status s = grid_ok(grid);
   if (s) return s;
   if (!d_r || !d_z) return RH_INVALID_ARGUMENT;
-  if (method != RH_JACOBI_PCG && method != RH_RBSGS_PCG) return RH_UNSUPPORTED;
+  if (method != RH_JACOBI_PCG && method != RH_RBSGS_PCG && method != RH_SGS_PCG) return RH_UNSUPPORTED;
   if ((s = rh::check_density(*grid, density))) return s;
   std::unique_lock<std::mutex> lock;
   rh::Scratch& sc = rh::global_scratch(lock);
   if ((s = sc.init())) return s;
   const DevGrid g = rh::make_dev_grid(*grid, density);
+  std::vector<rh::SgsTile> tiles;
+  std::vector<int> wave_start;
+  rh::SgsTile* d_tiles = nullptr;
+  if (method == RH_SGS_PCG) {
+    rh::sgs_plan(g, tiles, wave_start);
+    RH_HIP(hipMalloc(&d_tiles, sizeof(rh::SgsTile) * tiles.size()));
+    RH_HIP(hipMemcpy(d_tiles, tiles.data(), sizeof(rh::SgsTile) * tiles.size(), hipMemcpyHostToDevice));
+  }
   RH_HIP(hipMemsetAsync(sc.ctl, 0, sizeof(rh::PcgControl), sc.stream));
-  rh::launch_precondition(g, method, d_r, d_z, sc.ctl, sc.pdd, sc.pmx, sc.stream);
-  RH_HIP(hipGetLastError());
+  rh::launch_precondition(g, method, d_r, d_z, sc.ctl, sc.pdd, sc.pmx, sc.stream, d_tiles, wave_start.data(),
+                          (int)wave_start.size() - 1);
+  const hipError_t le = hipGetLastError();
   rh::PcgControl c;
-  RH_HIP(hipMemcpyAsync(sc.host, sc.ctl, sizeof(c), hipMemcpyDeviceToHost, sc.stream));
-  RH_HIP(hipStreamSynchronize(sc.stream));
+  const hipError_t ce = hipMemcpyAsync(sc.host, sc.ctl, sizeof(c), hipMemcpyDeviceToHost, sc.stream);
+  const hipError_t se = hipStreamSynchronize(sc.stream);
+  if (d_tiles) (void)hipFree(d_tiles);
+  RH_HIP(le);
+  RH_HIP(ce);
+  RH_HIP(se);
   std::memcpy(&c, sc.host, sizeof(c));
+  if (c.done && c.status == RH_BREAKDOWN) return RH_BREAKDOWN;
   if (out_rz) *out_rz = c.rz[1];
   return RH_OK;
 }
@@ -437,7 +452,7 @@ static rh_status provider_supports(const rh_provider* provider, const rh_pressur
       (shape[0] != req->grid->counts[0] || shape[1] != req->grid->counts[1] || shape[2] != req->grid->counts[2]))
     return why("grid shape differs from the provider's shape");
   const rh_method m = provider->config.solver.method;
-  if (m != RH_JACOBI_PCG && m != RH_RBSGS_PCG && m != RH_MIXED_MG)
+  if (m != RH_JACOBI_PCG && m != RH_RBSGS_PCG && m != RH_SGS_PCG && m != RH_MIXED_MG)
     return why("method not available in this build");
   return RH_OK;
 }
@@ -502,7 +517,7 @@ extern "C" rh_status rh_provider_workspace_create(const rh_grid* grid, uint64_t 
   *out = nullptr;
   rh_status s = grid_ok(grid);
   if (s) return s;
-  if (fallback != RH_JACOBI_PCG && fallback != RH_RBSGS_PCG) return RH_UNSUPPORTED;
+  if (fallback != RH_JACOBI_PCG && fallback != RH_RBSGS_PCG && fallback != RH_SGS_PCG) return RH_UNSUPPORTED;
   const size_t n = cells_of(*grid);
   // ProviderWorkspace::new (src/solver_provider.rs:105-144): 16 N extra plus
   // the 48 N reference workspace must fit the limit.

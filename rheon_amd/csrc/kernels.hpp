@@ -1,5 +1,7 @@
 // Kernel launch interface shared by the host code of rheon_amd.
 #pragma once
+#include <vector>
+
 #include "common.hpp"
 
 namespace rh {
@@ -115,14 +117,25 @@ struct FinRhsSums {
   }
 };
 
+// One tile of the lexicographic SGS sweeps (tile coordinates).
+struct SgsTile {
+  int bx, by, bz, pad;
+};
+constexpr int kSgsTileEdge = 8;
+
 struct PcgArrays {
   void* p;       // f64 solution (= p64)
   void* r;       // T
   void* d;       // T
   void* q;       // T (also holds the true residual)
-  void* z;       // T (RB-SGS only)
+  void* z;       // T (SGS: the preconditioned residual)
   const double* b;
   double* p64;   // f64 output pressure when T = float (may alias p when T = double)
+  // lexicographic SGS: the tiles of wave w are tiles[wave_start[w] ..
+  // wave_start[w + 1]) (device array; host offsets)
+  const SgsTile* tiles = nullptr;
+  const int* wave_start = nullptr;
+  int waves = 0;
 };
 
 template <typename T>
@@ -136,8 +149,13 @@ void pcg_export_pressure(const DevGrid& g, const PcgArrays& a, hipStream_t s);
 
 void launch_apply_full(const DevGrid& g, const double* x, double* y, unsigned* flags,
                        hipStream_t s);
+// tiles/wave_start/waves: the lexicographic SGS plan (RH_SGS_PCG only).
 void launch_precondition(const DevGrid& g, int method, double* r, double* z, PcgControl* ctl,
-                         DD* pdd, double* pmx, hipStream_t s);
+                         DD* pdd, double* pmx, hipStream_t s, const SgsTile* tiles = nullptr,
+                         const int* wave_start = nullptr, int waves = 0);
+// The tile-wave plan of the lexicographic SGS sweeps: tiles of kSgsTileEdge
+// cells per axis, grouped by wave bx + by + bz (host).
+void sgs_plan(const DevGrid& g, std::vector<SgsTile>& tiles, std::vector<int>& wave_start);
 void launch_residual_report(const DevGrid& g, const double* p, const double* b, const Reduce& red,
                             hipStream_t s);
 void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,

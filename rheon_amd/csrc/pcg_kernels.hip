@@ -330,6 +330,132 @@ k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgContro
   block_publish<1, 1>(s, none, pdd, pmx);
 }
 
+// ------------------------------------------------ lexicographic SGS (exact)
+// precondition_sgs (src/pressure.rs:467-532, the sgs-pcg-v1 preconditioner):
+// forward z_i = (r_i + sum_{d, lower} w_d z_{i - s_d}) / D_i in row order,
+// then reverse z_i += (sum_{d, upper} w_d z_{i + s_d}) / D_i in reverse row
+// order, the gauge row skipped (z = 0).  A cell's forward value depends only
+// on its lower neighbours' final values, so any order that finishes them
+// first gives the same bits: tiles of 8^3 cells run in waves bx + by + bz =
+// w (one launch per wave, src/sgs.rs:56-128 uses the same tile waves on CPU
+// threads), and inside a tile one wavefront walks its local planes
+// li + lj + lk = l with the tile's values in LDS.  Box operator only.
+template <typename T, bool kReverse>
+__global__ void __launch_bounds__(64)
+k_sgs_lex_wave(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const SgsTile* __restrict__ tiles,
+               const PcgControl* ctl, int it, int mode) {
+  if (!sgs_active(ctl, it, mode)) return;
+  constexpr int E = kSgsTileEdge;
+  __shared__ T tz[E * E * E];
+  const SgsTile t = tiles[blockIdx.x];
+  const int x0 = t.bx * E, y0 = t.by * E, z0 = t.bz * E;
+  const int ex = min(E, g.nx - x0), ey = min(E, g.ny - y0), ez = min(E, g.nz - z0);
+  const int cells = ex * ey * ez, exy = ex * ey;
+  const int sy = g.nx, sz = g.nx * g.ny;
+  const int lane = threadIdx.x;
+  if (kReverse)
+    for (int q = lane; q < cells; q += 64) {
+      const int li = q % ex, lj = (q / ex) % ey, lk = q / exy;
+      tz[q] = z[(x0 + li) + sy * (y0 + lj) + sz * (z0 + lk)];
+    }
+  __syncthreads();
+  const int planes = ex + ey + ez - 2;
+  for (int step = 0; step < planes; ++step) {
+    const int l = kReverse ? planes - 1 - step : step;
+    for (int q = lane; q < cells; q += 64) {
+      const int li = q % ex, lj = (q / ex) % ey, lk = q / exy;
+      if (li + lj + lk != l) continue;
+      const int i = x0 + li, j = y0 + lj, k = z0 + lk;
+      const int c = i + sy * j + sz * k;
+      if (c == g.gauge) {
+        tz[q] = (T)0.0;
+        continue;
+      }
+      const double diag = diag_at(g, c, i, j, k);
+      if (!kReverse) {
+        double v = (double)r[c];
+        if (i > 0) v += g.w[0] * (double)(li > 0 ? tz[q - 1] : z[c - 1]);
+        if (j > 0) v += g.w[1] * (double)(lj > 0 ? tz[q - ex] : z[c - sy]);
+        if (k > 0) v += g.w[2] * (double)(lk > 0 ? tz[q - exy] : z[c - sz]);
+        tz[q] = (T)(v / diag);
+      } else {
+        double corr = 0.0;
+        if (i + 1 < g.nx) corr += g.w[0] * (double)(li + 1 < ex ? tz[q + 1] : z[c + 1]);
+        if (j + 1 < g.ny) corr += g.w[1] * (double)(lj + 1 < ey ? tz[q + ex] : z[c + sy]);
+        if (k + 1 < g.nz) corr += g.w[2] * (double)(lk + 1 < ez ? tz[q + exy] : z[c + sz]);
+        tz[q] = (T)((double)tz[q] + corr / diag);
+      }
+    }
+    __syncthreads();
+  }
+  for (int q = lane; q < cells; q += 64) {
+    const int li = q % ex, lj = (q / ex) % ey, lk = q / exy;
+    z[(x0 + li) + sy * (y0 + lj) + sz * (z0 + lk)] = tz[q];
+  }
+}
+
+// r.z after the lexicographic sweeps (the reference's preconditioned.dot
+// (residual)) and a non-finite z flag (the sweeps' Breakdown checks).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_sgs_lex_dot(DevGrid g, const T* __restrict__ r, const T* __restrict__ z, const PcgControl* ctl, DD* pdd,
+              double* pmx, int it, int mode) {
+  if (!sgs_active(ctl, it, mode)) return;
+  DD s[1] = {{0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  const double zv = ld(z, c);
+  if (!isfinite(zv)) bad[0] = 1.0;
+  dd_acc(s[0], zv * ld(r, c));
+  END_CELLS
+  block_publish<1, 1>(s, bad, pdd, pmx);
+}
+
+// next rz after the lexicographic SGS; a non-finite value is the
+// preconditioner's Breakdown { iteration: 0 } (src/pressure.rs:493-498).
+struct FinSgsLex {
+  PcgControl* ctl;
+  int it;
+  int mode;
+  __device__ bool active() const { return sgs_active(ctl, it, mode); }
+  __device__ void operator()(const double* sum, const double* mx) const {
+    if (mx[0] != 0.0) return fail(ctl, RH_BREAKDOWN, 0);
+    ctl->rz[(it + 1) & 1] = sum[0];
+  }
+};
+
+template <typename T>
+void sgs_lex_apply(const DevGrid& g, const T* r, T* z, const SgsTile* tiles, const int* wave_start, int waves,
+                   PcgControl* ctl, DD* pdd, double* pmx, int it, int mode, hipStream_t s) {
+  for (int w = 0; w < waves; ++w) {
+    const int n = wave_start[w + 1] - wave_start[w];
+    if (n > 0) k_sgs_lex_wave<T, false><<<n, 64, 0, s>>>(g, r, z, tiles + wave_start[w], ctl, it, mode);
+  }
+  for (int w = waves - 1; w >= 0; --w) {
+    const int n = wave_start[w + 1] - wave_start[w];
+    if (n > 0) k_sgs_lex_wave<T, true><<<n, 64, 0, s>>>(g, r, z, tiles + wave_start[w], ctl, it, mode);
+  }
+  const dim3 grid = cells_grid(g);
+  k_sgs_lex_dot<T><<<grid, block_shape(), 0, s>>>(g, r, z, ctl, pdd, pmx, it, mode);
+  launch_finish<1, 1>(pdd, pmx, grid.x, FinSgsLex{ctl, it, mode}, s);
+}
+
+void sgs_plan(const DevGrid& g, std::vector<SgsTile>& tiles, std::vector<int>& wave_start) {
+  const int E = kSgsTileEdge;
+  const int tx = (g.nx + E - 1) / E, ty = (g.ny + E - 1) / E, tz = (g.nz + E - 1) / E;
+  const int waves = tx + ty + tz - 2;
+  tiles.clear();
+  wave_start.assign(1, 0);
+  for (int w = 0; w < waves; ++w) {
+    for (int bx = 0; bx < tx; ++bx)
+      for (int by = 0; by < ty; ++by) {
+        const int bz = w - bx - by;
+        if (bz >= 0 && bz < tz) tiles.push_back(SgsTile{bx, by, bz, 0});
+      }
+    wave_start.push_back((int)tiles.size());
+  }
+}
+
 // d = z + beta d (i >= 1), or d = z after a replacement / at it = 0;
 // next_rz breakdown check (src/pressure.rs:716-719).
 template <typename T, bool SGS>
@@ -580,7 +706,8 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
   T* d = (T*)a.d;
   T* q = (T*)a.q;
   T* z = (T*)a.z;
-  const bool sgs = method == RH_RBSGS_PCG;
+  const bool lex = method == RH_SGS_PCG;
+  const bool sgs = method == RH_RBSGS_PCG || lex;
   if (!sgs && it > 0) {
     // apply (+ the previous iteration's replacement) -> update -> direction
     // (+ this iteration's true residual when it is a candidate); the true
@@ -616,7 +743,10 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
         launch(k_update<T, false>, grid, block, 0, s, g, p, r, d, q, z, ctl, pdd, pmx);
     }
     launch_finish<3, 1>(pdd, pmx, nb, FinUpdate{ctl, it, sgs ? 0 : 1}, s);
-    if (sgs) {
+    if (lex) {
+      prof::Scope ps(prof::kSgsRed, s, it);
+      sgs_lex_apply<T>(g, r, z, a.tiles, a.wave_start, a.waves, ctl, pdd, pmx, it, 0, s);
+    } else if (sgs) {
       {
         prof::Scope ps(prof::kSgsBlack, s, it);
         launch(k_sgs_black<T>, grid, block, 0, s, g, r, z, ctl, it, 0);
@@ -638,15 +768,20 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
       prof::Scope ps(prof::kReplace, s, it);
       launch(k_replace<T, true>, grid, block, 0, s, g, r, q, z, ctl, pdd, pmx, it);
     }
-    {
-      prof::Scope ps(prof::kSgsBlack, s, it);
-      launch(k_sgs_black<T>, grid, block, 0, s, g, r, z, ctl, it, 1);
-    }
-    {
+    if (lex) {
       prof::Scope ps(prof::kSgsRed, s, it);
-      launch(k_sgs_red<T>, grid, block, 0, s, g, r, z, ctl, pdd, pmx, it, 1);
+      sgs_lex_apply<T>(g, r, z, a.tiles, a.wave_start, a.waves, ctl, pdd, pmx, it, 1, s);
+    } else {
+      {
+        prof::Scope ps(prof::kSgsBlack, s, it);
+        launch(k_sgs_black<T>, grid, block, 0, s, g, r, z, ctl, it, 1);
+      }
+      {
+        prof::Scope ps(prof::kSgsRed, s, it);
+        launch(k_sgs_red<T>, grid, block, 0, s, g, r, z, ctl, pdd, pmx, it, 1);
+      }
+      launch_finish<1, 1>(pdd, pmx, nb, FinSgsRed{ctl, it, 1}, s);
     }
-    launch_finish<1, 1>(pdd, pmx, nb, FinSgsRed{ctl, it, 1}, s);
     {
       prof::Scope ps(prof::kDirection, s, it);
       launch(k_direction<T, true>, grid, block, 0, s, g, r, z, d, ctl, it);
@@ -692,9 +827,13 @@ void launch_apply_full(const DevGrid& g, const double* x, double* y, unsigned* f
 // Reuses the it = 0 path of the PCG kernels with prod aliasing r; `ctl`
 // must be zeroed by the caller.
 void launch_precondition(const DevGrid& g, int method, double* r, double* z, PcgControl* ctl,
-                         DD* pdd, double* pmx, hipStream_t s) {
+                         DD* pdd, double* pmx, hipStream_t s, const SgsTile* tiles, const int* wave_start,
+                         int waves) {
   const dim3 grid = cells_grid(g), block = block_shape();
-  if (method == RH_RBSGS_PCG) {
+  if (method == RH_SGS_PCG) {
+    k_replace<double, true><<<grid, block, 0, s>>>(g, r, r, z, ctl, pdd, pmx, 0);
+    sgs_lex_apply<double>(g, r, z, tiles, wave_start, waves, ctl, pdd, pmx, 0, 1, s);
+  } else if (method == RH_RBSGS_PCG) {
     k_replace<double, true><<<grid, block, 0, s>>>(g, r, r, z, ctl, pdd, pmx, 0);
     k_sgs_black<double><<<grid, block, 0, s>>>(g, r, z, ctl, 0, 1);
     k_sgs_red<double><<<grid, block, 0, s>>>(g, r, z, ctl, pdd, pmx, 0, 1);

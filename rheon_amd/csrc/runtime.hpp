@@ -107,12 +107,15 @@ class PcgSolver : public Solver {
                   double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
                   void* ctx) override;
   uint64_t allocated_bytes() const override;
+  rh_status set_solid(const uint8_t* solid) override;
 
  private:
   template <typename T>
   rh_status run(const DevGrid& g, const double* d_rhs, const rh_pressure_settings& st,
                 double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel, void* ctx);
   void* arena_ = nullptr;
+  SgsTile* d_tiles_ = nullptr;    // lexicographic SGS tile waves (device)
+  std::vector<int> wave_start_;   // their offsets per wave (host)
   void *r_ = nullptr, *d_ = nullptr, *q_ = nullptr, *z_ = nullptr;
   size_t n_ = 0;
   uint64_t bytes_ = 0;

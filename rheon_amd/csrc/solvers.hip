@@ -21,7 +21,8 @@ namespace rh {
 Solver* make_solver(rh_method method) {
   switch (method) {
     case RH_JACOBI_PCG:
-    case RH_RBSGS_PCG: return new PcgSolver();
+    case RH_RBSGS_PCG:
+    case RH_SGS_PCG: return new PcgSolver();
     case RH_MIXED_MG: return new MgSolver();
     default: return nullptr;
   }
@@ -188,7 +189,8 @@ static void init_control(PcgControl& h, const rh_pressure_settings& st, double d
 rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config) {
   grid_ = grid;
   config_ = config;
-  if (config.method != RH_JACOBI_PCG && config.method != RH_RBSGS_PCG) return RH_UNSUPPORTED;
+  if (config.method != RH_JACOBI_PCG && config.method != RH_RBSGS_PCG && config.method != RH_SGS_PCG)
+    return RH_UNSUPPORTED;
   if (config.precision != RH_F64 && config.precision != RH_F32) return RH_INVALID_ARGUMENT;
   rh_status st = scratch_.init();
   if (st) return st;
@@ -212,13 +214,29 @@ rh_status PcgSolver::create(const rh_grid& grid, const rh_solver_config& config)
   d_ = take();
   q_ = take();
   z_ = take();
-
+  if (config.method == RH_SGS_PCG) {
+    std::vector<SgsTile> tiles;
+    sgs_plan(make_dev_grid(grid, 1.0), tiles, wave_start_);
+    RH_HIP(hipMalloc(&d_tiles_, sizeof(SgsTile) * tiles.size()));
+    RH_HIP(hipMemcpy(d_tiles_, tiles.data(), sizeof(SgsTile) * tiles.size(), hipMemcpyHostToDevice));
+    bytes_ += sizeof(SgsTile) * tiles.size();
+  }
   return RH_OK;
 }
 
 PcgSolver::~PcgSolver() {
   if (arena_) (void)hipFree(arena_);
+  if (d_tiles_) (void)hipFree(d_tiles_);
   scratch_.release();
+}
+
+// The lexicographic SGS (sgs-pcg-v1) is stated for the sealed box.
+rh_status PcgSolver::set_solid(const uint8_t* solid) {
+  if (solid && config_.method == RH_SGS_PCG) {
+    set_last_error("sgs-pcg-v1 runs on the sealed box only", "rh_solver_set_solid");
+    return RH_UNSUPPORTED;
+  }
+  return Solver::set_solid(solid);
 }
 
 uint64_t PcgSolver::allocated_bytes() const { return bytes_ + scratch_.bytes_; }
@@ -236,6 +254,9 @@ rh_status PcgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressur
   a.z = z_;
   a.b = d_rhs;
   a.p64 = d_pressure;
+  a.tiles = d_tiles_;
+  a.wave_start = wave_start_.data();
+  a.waves = (int)wave_start_.size() - 1;
   PcgControl h;
   init_control(h, st, dt_, g.volume);
   std::memcpy(scratch_.host, &h, sizeof(h));

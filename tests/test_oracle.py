@@ -504,3 +504,13 @@ def test_velocity_rate_planted_maxima(dtype):
     # all-zero field: rate 0 (the Courant dt rule then takes the requested dt)
     zero = [np.zeros(g.face_len(d), dtype) for d in range(3)]
     assert O.velocity_rate(g, zero) == 0.0
+
+
+def test_steady_scene_sgs_iteration_count():
+    # docs/plans/accepted-pressure-warm-start/measurement-tables.md:79: the 32^3
+    # steady scene with sgs-pcg-v1 (1 worker) takes 1552 iterations over its 16
+    # identical solves, 97 per solve (the jacobi-pcg-v1 row beside it, 3968 =
+    # 16 x 248, is the 248 of steady-cold.csv)
+    spec, g, vel, dt, acc = steady_scene(32)
+    r = O.solve_velocity(g, 1.25, vel, dt, O.Settings(1e-8, 1e-12, 1e-7, 3000), method="sgs-pcg-v1", threads=8)
+    assert r.ok and r.iterations == 97
