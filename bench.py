@@ -463,6 +463,12 @@ def run_serve(args) -> dict:
     st = [server.state(i) for i in ids]
     failed = [(i, R.STATUS_NAMES.get(s_["status"])) for i, s_ in zip(ids, st) if s_["state"] != "done"]
     step_lat = np.array([tk["seconds"] for tk in ticks if tk["stepped"] > 0]) * 1e3
+
+    def pct(key):
+        v = np.array([tk[key] for tk in ticks if tk["stepped"] > 0]) * 1e3
+        return {"p50": float(np.percentile(v, 50)), "p99": float(np.percentile(v, 99)), "max": float(v.max())}
+
+    breakdown = {k: pct(k) for k in ("admit_seconds", "build_seconds", "wait_seconds", "device_seconds")}
     steps_total = int(sum(tk["stepped"] for tk in ticks))
     cells_total = int(sum(tk["cells"] for tk in ticks))
     done = [s_ for s_ in st if s_["state"] == "done"]
@@ -487,6 +493,7 @@ def run_serve(args) -> dict:
             "latency_ms_p50": float(np.percentile(step_lat, 50)),
             "latency_ms_p99": float(np.percentile(step_lat, 99)),
             "latency_ms_max": float(step_lat.max()),
+            "tick_breakdown_ms": breakdown,
             "cell_updates_per_s": cells_total / makespan,
             "makespan_s": makespan,
             "ticks": len(ticks),
@@ -828,13 +835,23 @@ def main() -> int:
             "batch_equals_looped_bitwise": c.get("batch_equals_looped_bitwise"),
             "autotune_chosen": c.get("autotune", {}).get("chosen"), "ok": b.get("ok"),
             "failed": b.get("failed")}.items() if v is not None}
+        keys = ("latency_ms_p50", "latency_ms_p99", "latency_ms_max", "tick_breakdown_ms", "cell_updates_per_s",
+                "queue_wait_ms_p50", "queue_wait_ms_p99", "max_bytes_in_use_gb", "hbm_budget_gb", "max_div_u",
+                "failed", "scene_steps", "makespan_s")
         sv = run_serve(argparse.Namespace(requests=40, rate=8.0, seed=5, hbm_budget_gb=16.0, block_divisor=8,
                                           json_out=None))
         c = sv["config"]
         result["serve"] = {"value": sv["value"], "unit": sv["unit"], "workload": c["workload"],
-                           **{k: c[k] for k in ("latency_ms_p50", "latency_ms_p99", "latency_ms_max",
-                                                "cell_updates_per_s", "queue_wait_ms_p50", "queue_wait_ms_p99",
-                                                "max_bytes_in_use_gb", "max_div_u", "failed", "scene_steps")}}
+                           **{k: c[k] for k in keys}}
+        # the same stream under an HBM budget of 1.25x its largest request's
+        # plan, so admission by device bytes binds (requests queue for memory)
+        server = R.ProjectionServer()
+        biggest = max(server.plan_bytes(r[0]) for r in serving_requests(40, 5))
+        server.close()
+        sv = run_serve(argparse.Namespace(requests=40, rate=8.0, seed=5, hbm_budget_gb=1.25 * biggest / (1 << 30),
+                                          block_divisor=8, json_out=None))
+        c = sv["config"]
+        result["serve_budget_bound"] = {"value": sv["value"], "unit": sv["unit"], **{k: c[k] for k in keys}}
     line = json.dumps(result)
     print(line, flush=True)
     if args.json_out:

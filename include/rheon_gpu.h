@@ -523,10 +523,13 @@ rh_status rh_batch_step(rh_batch* batch, int32_t steps, rh_step_report* reports,
 /* ------------------------------------------------------------ serving */
 /* Continuous-batching server over device-resident scenes (no counterpart in
  * the reference).  Requests are admitted in arrival order while their
- * planned device bytes fit hbm_budget (and max_active); each tick runs one
- * batched step (rh_batch) of every running request, retires finished ones
- * (results kept on the host) and admits queued ones, whose set-up overlaps
- * the device's step.  A request's result does not depend on the mix. */
+ * planned device bytes fit hbm_budget (and max_active) by a worker thread of
+ * the server (scene creation, obstacles, uploads), off the ticks' path; each
+ * tick runs one batched step (rh_batch) of every running request, retires
+ * finished ones (results kept on the host) and lets admitted ones join; a
+ * tick with nothing running admits on the caller's thread.  A request's
+ * result does not depend on the mix.  One thread at a time may call a given
+ * server's functions. */
 typedef struct rh_server rh_server;
 typedef struct rh_server_config {
   rh_scene_config scene;     /* f32 faces, RH_MIXED_MG with f32 V-cycles */
@@ -557,7 +560,7 @@ typedef struct rh_request_state {
 } rh_request_state;
 typedef struct rh_tick_report {
   int32_t stepped;           /* requests that took a step this tick */
-  int32_t admitted;          /* requests admitted (they step from the next tick) */
+  int32_t admitted;          /* requests that joined the running set this tick */
   int32_t finished;          /* requests retired (done or failed) */
   int32_t active;            /* running after the tick */
   int32_t queued;
@@ -566,6 +569,9 @@ typedef struct rh_tick_report {
   uint64_t bytes_in_use;
   double seconds;            /* host wall time of the tick */
   double device_seconds;     /* device time of the batched step */
+  double admit_seconds;      /* host time spent admitting requests during the tick */
+  double build_seconds;      /* host time rebuilding the batch's launch tables */
+  double wait_seconds;       /* host time waiting for the step after enqueuing it */
 } rh_tick_report;
 rh_status rh_server_create(const rh_server_config* config, rh_server** out);
 void rh_server_destroy(rh_server* server);

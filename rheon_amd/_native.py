@@ -196,6 +196,9 @@ class TickReport(C.Structure):
         ("bytes_in_use", C.c_uint64),
         ("seconds", C.c_double),
         ("device_seconds", C.c_double),
+        ("admit_seconds", C.c_double),
+        ("build_seconds", C.c_double),
+        ("wait_seconds", C.c_double),
     ]
 
 

@@ -14,6 +14,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -248,6 +251,7 @@ class Batch {
   rh_status launch(int steps, const std::function<rh_status()>& host_work = nullptr);
   rh_status wait(rh_step_report* reports, rh_status* statuses);
   double last_seconds() const { return last_seconds_; }  // device time of the last launch
+  double last_build_seconds() const { return last_build_seconds_; }  // host time of the last table build
   rh_scene* scene(size_t i) { return i < scenes_.size() ? scenes_[i] : nullptr; }
   // Non-owning membership (serving): the batch steps exactly these scenes.
   rh_status set_members(const std::vector<rh_scene*>& scenes, const std::vector<double>& dt);
@@ -281,6 +285,8 @@ class Batch {
   BatchScene* d_scenes_ = nullptr;
   size_t cap_ = 0;  // scene records the device tables hold
   std::vector<BatchScene> h_scenes_;
+  BatchScene* pinned_ = nullptr;  // pinned readback of the per-scene records
+  size_t pinned_cap_ = 0;
   unsigned char* d_tables_ = nullptr;
   size_t tables_bytes_ = 0;
   std::vector<Launch> program_;
@@ -298,6 +304,7 @@ class Batch {
   int launched_steps_ = 0;
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double last_seconds_ = 0.0;
+  double last_build_seconds_ = 0.0;
 };
 
 Batch::~Batch() {
@@ -308,6 +315,7 @@ Batch::~Batch() {
     if (e) (void)hipEventDestroy(e);
   if (d_active_) (void)hipFree(d_active_);
   if (h_active_) (void)hipHostFree(h_active_);
+  if (pinned_) (void)hipHostFree(pinned_);
   release_graph();
   if (d_tables_) (void)hipFree(d_tables_);
   if (d_grids_) (void)hipFree(d_grids_);
@@ -1032,8 +1040,11 @@ rh_status Batch::launch(int steps, const std::function<rh_status()>& host_work) 
     if (!(dt_[i] > 0.0)) return RH_INVALID_TIME_STEP;
   for (size_t i = 0; i < n; ++i)
     if (scenes_[i]->solver->impl->generation() != gens_[i]) built_ = false;
+  last_build_seconds_ = 0.0;
   if (!built_) {
+    const auto t0 = std::chrono::steady_clock::now();
     rh_status s = build();
+    last_build_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (s) return s;
   }
   // every scene's own stream must be idle (uploads went through them)
@@ -1059,7 +1070,16 @@ rh_status Batch::launch(int steps, const std::function<rh_status()>& host_work) 
   }
   RH_HIP(hipEventRecord(ev_[1], stream_));
   RH_HIP(hipGetLastError());
-  RH_HIP(hipMemcpyAsync(h_scenes_.data(), d_scenes_, sizeof(BatchScene) * n, hipMemcpyDeviceToHost,
+  // the records come back through pinned memory: a pageable copy would queue
+  // behind other threads' pageable transfers (a serving admission upload)
+  if (pinned_cap_ < n) {
+    if (pinned_) (void)hipHostFree(pinned_);
+    pinned_ = nullptr;
+    pinned_cap_ = 0;
+    RH_HIP(hipHostMalloc(&pinned_, sizeof(BatchScene) * n, hipHostMallocDefault));
+    pinned_cap_ = n;
+  }
+  RH_HIP(hipMemcpyAsync(pinned_, d_scenes_, sizeof(BatchScene) * n, hipMemcpyDeviceToHost,
                         stream_));
   launched_steps_ = steps;
   return RH_OK;
@@ -1072,6 +1092,7 @@ rh_status Batch::wait(rh_step_report* reports, rh_status* statuses) {
   if (n == 0 || steps == 0) return RH_OK;
   launched_steps_ = 0;
   RH_HIP(hipStreamSynchronize(stream_));
+  std::memcpy(h_scenes_.data(), pinned_, sizeof(BatchScene) * n);
   float ms = 0.0f;
   RH_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
   last_seconds_ = ms * 1e-3;
@@ -1156,12 +1177,14 @@ extern "C" rh_status rh_batch_autotune(rh_batch* batch, int32_t steps, rh_batch_
 // A serving loop over device-resident scenes with continuous batching:
 // requests are admitted in arrival order while their planned device bytes
 // fit the budget (and the active-scene cap); every tick runs one batched
-// step of every running request; requests leave as they finish and queued
-// ones join at the next tick.  Admission work (scene creation, obstacle
-// setup, uploads) for newcomers runs on the host while the device executes
-// the current tick.  The batched step of a scene does not depend on which
-// other scenes share the launch, so a request's result is the same alone
-// and in any mix (for a fixed launch shape).
+// step of every running request; requests leave as they finish and admitted
+// ones join at the next tick.  Admission (scene creation, obstacle set-up,
+// the coarse levels' host preparation, uploads) runs on a worker thread of
+// the server, so a tick's latency is its batched step and nothing else; a
+// tick with no running request admits on its own thread (the queue never
+// waits for the worker to wake).  The batched step of a scene does not
+// depend on which other scenes share the launch, so a request's result is
+// the same alone and in any mix (for a fixed launch shape).
 
 namespace rh {
 
@@ -1181,6 +1204,7 @@ struct ServerRequest {
 
 class Server {
  public:
+  ~Server();
   rh_status create(const rh_server_config& cfg);
   rh_status submit(const rh_request& r, int64_t* id);
   rh_status tick(rh_tick_report* rep);
@@ -1192,15 +1216,57 @@ class Server {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
   }
   rh_status admit(ServerRequest* r);
+  // Host -> device copy through the worker's pinned staging buffer.
+  rh_status upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+  // Frees the request's scene (after copying its result when keep_result)
+  // and returns its bytes to the budget.
   void retire(ServerRequest* r, bool keep_result);
+  // The next queued request may be admitted now (caller holds mu_).
+  bool admissible() const;
+  // Admits the request at the queue head: reserve under mu_, build outside.
+  void admit_front(std::unique_lock<std::mutex>& lk);
+  void worker();
   rh_server_config cfg_{};
   Batch batch_;
   std::chrono::steady_clock::time_point t0_ = std::chrono::steady_clock::now();
+  int device_ = 0;
+  // guarded by mu_
+  std::mutex mu_;
+  std::condition_variable cv_;
   int64_t next_id_ = 1;
   uint64_t budget_ = 0, used_ = 0;
   std::vector<std::unique_ptr<ServerRequest>> all_;
-  std::vector<ServerRequest*> queue_, active_;
+  std::vector<ServerRequest*> queue_, ready_;
+  // finished requests whose results the worker copies out before their
+  // state turns final (final state, status, keep the result)
+  struct Retiring {
+    ServerRequest* r;
+    int32_t state;
+    rh_status status;
+  };
+  std::vector<Retiring> retiring_;
+  int retiring_busy_ = 0;  // retirements the worker is carrying out
+  size_t running_ = 0;    // requests in active_ (the tick thread's list)
+  int admitting_ = 0;
+  bool stop_ = false;
+  // tick thread only
+  std::vector<ServerRequest*> active_;
+  std::thread thread_;
+  // admission staging (pinned; used by one admission at a time)
+  std::mutex stage_mu_;
+  void* stage_ = nullptr;
+  size_t stage_bytes_ = 0;
 };
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  cv_.notify_all();
+  if (thread_.joinable()) thread_.join();
+  if (stage_) (void)hipHostFree(stage_);
+}
 
 rh_status Server::create(const rh_server_config& cfg) {
   cfg_ = cfg;
@@ -1218,6 +1284,13 @@ rh_status Server::create(const rh_server_config& cfg) {
     RH_HIP(hipMemGetInfo(&free_b, &total_b));
     budget_ = (uint64_t)(0.8 * (double)free_b);
   }
+  RH_HIP(hipGetDevice(&device_));
+  try {
+    thread_ = std::thread([this] { worker(); });
+  } catch (...) {
+    set_last_error("could not start the admission thread", "rh_server_create");
+    return RH_EXECUTION;
+  }
   return RH_OK;
 }
 
@@ -1228,7 +1301,6 @@ rh_status Server::submit(const rh_request& r, int64_t* id) {
   if ((s = check_indexable(&r.grid))) return s;
   if (!r.u || !r.v || !r.w || r.steps < 1 || !std::isfinite(r.dt) || r.dt <= 0.0) return RH_INVALID_ARGUMENT;
   auto q = std::make_unique<ServerRequest>();
-  q->id = next_id_++;
   q->grid = r.grid;
   const size_t f[3] = {(size_t)(r.grid.counts[0] + 1) * r.grid.counts[1] * r.grid.counts[2],
                        (size_t)r.grid.counts[0] * (r.grid.counts[1] + 1) * r.grid.counts[2],
@@ -1242,28 +1314,56 @@ rh_status Server::submit(const rh_request& r, int64_t* id) {
   uint64_t faces = 0;
   for (size_t a = 0; a < 3; ++a) faces += 2 * ((f[a] * 4 + 255) / 256 * 256);
   q->plan = faces + 2 * ((cells * 8 + 255) / 256 * 256) + MgSolver::plan_bytes(r.grid, cfg_.scene.solver);
-  if (q->plan > budget_) return RH_BUFFER_LIMIT;
-  q->st.state = 0;
-  q->st.status = RH_OK;
-  q->st.submitted = now();
-  q->st.bytes = q->plan;
-  if (id) *id = q->id;
-  queue_.push_back(q.get());
-  all_.push_back(std::move(q));
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    if (q->plan > budget_) return RH_BUFFER_LIMIT;
+    q->id = next_id_++;
+    q->st.state = 0;
+    q->st.status = RH_OK;
+    q->st.submitted = now();
+    q->st.bytes = q->plan;
+    if (id) *id = q->id;
+    queue_.push_back(q.get());
+    all_.push_back(std::move(q));
+  }
+  cv_.notify_all();
+  return RH_OK;
+}
+
+rh_status Server::upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(stage_mu_);
+  if (!stage_) {
+    RH_HIP(hipHostMalloc(&stage_, 8u << 20, hipHostMallocDefault));
+    stage_bytes_ = 8u << 20;
+  }
+  for (size_t off = 0; off < bytes; off += stage_bytes_) {
+    const size_t n = std::min(stage_bytes_, bytes - off);
+    std::memcpy(stage_, (const char*)src + off, n);
+    RH_HIP(hipMemcpyAsync((char*)dst + off, stage_, n, hipMemcpyHostToDevice, s));
+    RH_HIP(hipStreamSynchronize(s));
+  }
   return RH_OK;
 }
 
 // Scene creation, obstacles and initial faces of a newly admitted request.
+// The faces go straight into the scene's accepted arrays through the pinned
+// staging buffer; the obstacle set-up that follows closes obstacle faces.
 rh_status Server::admit(ServerRequest* r) {
   rh_scene* raw = nullptr;
   rh_status s = rh_scene_create(&r->grid, &cfg_.scene, &raw);
   if (s) return s;
   r->scene.reset(raw);
+  const hipStream_t st = raw->solver->impl->scratch().stream;
+  const std::vector<float>* src[3] = {&r->u, &r->v, &r->w};
+  for (int a = 0; a < 3; ++a)
+    if ((s = upload(raw->acc[a], src[a]->data(), raw->flen[a] * sizeof(float), st))) return s;
+  raw->acc_checked = false;
+  raw->rate_known = false;
   if (!r->solid.empty() && (s = rh_scene_set_solid(raw, r->solid.data(), r->solid.size()))) return s;
-  if ((s = rh_scene_set_velocity(raw, r->u.data(), r->v.data(), r->w.data()))) return s;
   MgSolver* mg = static_cast<MgSolver*>(raw->solver->impl.get());
   if (mg->levels() < 2) return RH_UNSUPPORTED;
   if ((s = mg->prepare(cfg_.scene.density))) return s;
+  RH_HIP(hipStreamSynchronize(raw->solver->impl->scratch().stream));
   return RH_OK;
 }
 
@@ -1278,12 +1378,67 @@ void Server::retire(ServerRequest* r, bool keep_result) {
     (void)rh_scene_get_pressure(r->scene.get(), r->rp.data(), cells);
   }
   r->scene.reset();
-  used_ -= std::min(used_, r->st.bytes);
-  r->st.finished = now();
   r->u.clear();
   r->v.clear();
   r->w.clear();
   r->solid.clear();
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    used_ -= std::min(used_, r->st.bytes);
+    r->st.finished = now();
+  }
+  cv_.notify_all();
+}
+
+bool Server::admissible() const {
+  if (queue_.empty()) return false;
+  const size_t running = running_ + ready_.size() + (size_t)admitting_;
+  if (cfg_.max_active > 0 && running >= (size_t)cfg_.max_active) return false;
+  return used_ + queue_.front()->plan <= budget_;
+}
+
+void Server::admit_front(std::unique_lock<std::mutex>& lk) {
+  ServerRequest* r = queue_.front();
+  queue_.erase(queue_.begin());
+  used_ += r->plan;
+  r->st.admitted = now();
+  ++admitting_;
+  lk.unlock();
+  const rh_status a = admit(r);
+  if (a) retire(r, false);
+  lk.lock();
+  --admitting_;
+  if (a) {
+    r->st.state = 3;
+    r->st.status = a;
+  } else {
+    r->st.state = 1;
+    ready_.push_back(r);
+  }
+}
+
+void Server::worker() {
+  (void)hipSetDevice(device_);
+  std::unique_lock<std::mutex> lk(mu_);
+  while (true) {
+    cv_.wait(lk, [&] { return stop_ || !retiring_.empty() || admissible(); });
+    if (stop_) return;
+    if (!retiring_.empty()) {
+      // retirements first: they return device memory to the budget
+      const Retiring t = retiring_.front();
+      retiring_.erase(retiring_.begin());
+      ++retiring_busy_;
+      lk.unlock();
+      retire(t.r, true);
+      lk.lock();
+      --retiring_busy_;
+      t.r->st.state = t.state;
+      t.r->st.status = t.status;
+    } else {
+      admit_front(lk);
+    }
+    cv_.notify_all();
+  }
 }
 
 rh_status Server::tick(rh_tick_report* rep) {
@@ -1291,43 +1446,39 @@ rh_status Server::tick(rh_tick_report* rep) {
   rh_tick_report& T = rep ? *rep : local;
   std::memset(&T, 0, sizeof(T));
   const double t_start = now();
-  // 1. the current members' step runs on the device ...
+  std::vector<ServerRequest*> joined;
+  {
+    std::unique_lock<std::mutex> lk(mu_);
+    if (active_.empty()) {
+      // nothing runs: admit what fits here rather than wait for the worker
+      const double a0 = now();
+      // with nothing running, finish pending retirements (they free budget
+      // and make results final) and admissions before going on
+      while (!retiring_.empty() || retiring_busy_ > 0 || admitting_ > 0 || admissible()) {
+        if (admissible() && retiring_.empty() && retiring_busy_ == 0)
+          admit_front(lk);
+        else
+          cv_.wait(lk);
+      }
+      T.admit_seconds = now() - a0;
+    }
+    joined.swap(ready_);
+    T.admitted = (int32_t)joined.size();
+    running_ = active_.size() + joined.size();
+    if (active_.empty()) {
+      // the joined requests step from this tick on
+      active_.swap(joined);
+    }
+  }
+  // 1. one batched step of the current members
   std::vector<rh_scene*> members;
   std::vector<double> dts;
   for (ServerRequest* r : active_) {
     members.push_back(r->scene.get());
     dts.push_back(r->dt);
   }
-  rh_status s = batch_.set_members(members, dts);
-  if (s) return s;
-  // 2. ... while newcomers are admitted (arrival order, memory permitting);
-  // the batch runs this once its first V-cycles are enqueued
-  std::vector<ServerRequest*> joined;
-  const std::function<rh_status()> admission = [&]() -> rh_status {
-    while (!queue_.empty()) {
-      ServerRequest* r = queue_.front();
-      const size_t running = active_.size() + joined.size();
-      if (cfg_.max_active > 0 && running >= (size_t)cfg_.max_active) break;
-      if (used_ + r->plan > budget_) break;
-      queue_.erase(queue_.begin());
-      used_ += r->plan;
-      r->st.admitted = now();
-      rh_status a = admit(r);
-      if (a) {
-        r->st.state = 3;
-        r->st.status = a;
-        retire(r, false);
-        continue;
-      }
-      r->st.state = 1;
-      joined.push_back(r);
-      ++T.admitted;
-    }
-    return RH_OK;
-  };
-  // Requests admitted this tick are running from now on, also when the
-  // launch or the wait below fails: they keep their scene and budget and
-  // join the members (or are retired with the server) like any other.
+  // Requests taken from ready_ join active_ also when the step fails (they
+  // keep their scene and budget like any other running request).
   struct KeepJoined {
     std::vector<ServerRequest*>& joined;
     std::vector<ServerRequest*>& active;
@@ -1336,31 +1487,39 @@ rh_status Server::tick(rh_tick_report* rep) {
       if (armed) active.insert(active.end(), joined.begin(), joined.end());
     }
   } keep{joined, active_};
-  if (members.empty()) {
-    if ((s = admission())) return s;
-  } else if ((s = batch_.launch(1, admission))) {
-    return s;
-  }
-  // 3. wait, account, retire finished requests
+  rh_status s = batch_.set_members(members, dts);
+  if (s) return s;
+  if (!members.empty() && (s = batch_.launch(1))) return s;
+  // 2. wait, account, retire finished requests
   std::vector<rh_step_report> reps(members.size());
   std::vector<rh_status> sts(members.size(), RH_OK);
+  const double w0 = now();
   if (!members.empty() && (s = batch_.wait(reps.data(), sts.data()))) return s;
+  T.wait_seconds = now() - w0;
+  T.build_seconds = members.empty() ? 0.0 : batch_.last_build_seconds();
   keep.armed = false;
   std::vector<ServerRequest*> still;
   for (size_t i = 0; i < active_.size(); ++i) {
     ServerRequest* r = active_[i];
-    r->st.last = reps[i];
-    T.cells += (int64_t)r->scene->n;
-    if (sts[i] != RH_OK) {
-      r->st.state = 3;
-      r->st.status = sts[i];
-      retire(r, true);
-      ++T.finished;
-      continue;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      r->st.last = reps[i];
     }
-    if (++r->st.steps_done >= r->steps) {
-      r->st.state = 2;
-      retire(r, true);
+    T.cells += (int64_t)r->scene->n;
+    // a finished request's results are copied out by the worker, off the
+    // tick's path; its state turns final once they are on the host
+    bool done;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      if (sts[i] != RH_OK) {
+        retiring_.push_back(Retiring{r, 3, sts[i]});
+        done = true;
+      } else {
+        done = ++r->st.steps_done >= r->steps;
+        if (done) retiring_.push_back(Retiring{r, 2, RH_OK});
+      }
+    }
+    if (done) {
       ++T.finished;
       continue;
     }
@@ -1368,16 +1527,22 @@ rh_status Server::tick(rh_tick_report* rep) {
   }
   for (ServerRequest* r : joined) still.push_back(r);
   active_.swap(still);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    running_ = active_.size();
+    T.active = (int32_t)active_.size();
+    T.queued = (int32_t)queue_.size();
+    T.bytes_in_use = used_;
+  }
+  cv_.notify_all();
   T.stepped = (int32_t)members.size();
-  T.active = (int32_t)active_.size();
-  T.queued = (int32_t)queue_.size();
-  T.bytes_in_use = used_;
   T.device_seconds = members.empty() ? 0.0 : batch_.last_seconds();
   T.seconds = now() - t_start;
   return RH_OK;
 }
 
 rh_status Server::state(int64_t id, rh_request_state* out) {
+  std::lock_guard<std::mutex> lk(mu_);
   for (auto& r : all_)
     if (r->id == id) {
       *out = r->st;
@@ -1387,6 +1552,7 @@ rh_status Server::state(int64_t id, rh_request_state* out) {
 }
 
 rh_status Server::result(int64_t id, float* u, float* v, float* w, double* p) {
+  std::lock_guard<std::mutex> lk(mu_);
   for (auto& r : all_)
     if (r->id == id) {
       if (r->st.state < 2 || r->ru.empty()) return RH_INVALID_ARGUMENT;
