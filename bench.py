@@ -659,7 +659,14 @@ def main() -> int:
 
     spec = GridSpec.unit_box(args.n)
     vel = fourier_modes(spec, seed=2026, modes=32, kmax=16, amplitude=1.0, dtype=np.float32)
-    dt = courant_dt(spec, vel, 0.5)
+    # Simulation::step's dt policy (src/simulation.rs:926-979): the recipe's
+    # requested dt = 0.5 h / max|u|, clipped to 0.5 max_courant / rate with
+    # rate = sum_d max|u_d| / h_d and max_courant = 1; the Courant gate then
+    # checks every step's candidate against max_courant.
+    max_courant = 1.0
+    dt_requested = courant_dt(spec, vel, 0.5)
+    rate0 = sum(float(np.max(np.abs(vel[d]))) / spec.spacing[d] for d in range(3))
+    dt = min(dt_requested, 0.5 * max_courant / rate0)
     mg = args.method in ("mixed-mg-dc", "mg-f64-dc")
     s_vec = 4 if args.method == "mixed-mg-dc" else 8  # bytes per Krylov / V-cycle value
     settings = R.PressureSettings(args.rel, 1e-14, 1e-4, 200 if mg else 20000, poll_interval=2 if mg else 32)
@@ -668,7 +675,7 @@ def main() -> int:
         (2, 2) if args.smoother == "jacobi" else (3, 1))
     scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method,
                     solver_precision=R.F32 if args.method == "mixed-mg-dc" else R.F64,
-                    settings=settings, actual_divergence_limit=1e-3, max_courant=0.0,
+                    settings=settings, actual_divergence_limit=1e-3, max_courant=max_courant,
                     mg_sweeps=sweeps, mg_smoother=smoother, mg_coarse_sweeps=coarse_sweeps(args))
     solid = centred_sphere(spec, args.sphere) if args.sphere > 0 else None
     if solid is not None:
@@ -677,7 +684,7 @@ def main() -> int:
     N.lib().rh_device_synchronize()
 
     for _ in range(args.warmup):
-        rep = scene.step(dt)
+        rep = scene.step_requested(dt_requested)
         if rep.status != R.OK:
             print(f"warmup step failed: {R.STATUS_NAMES.get(rep.status)}", file=sys.stderr)
             return 1
@@ -698,7 +705,7 @@ def main() -> int:
     # each, so the timed steps bracket only the dominant kernel.
     lib.rh_profile_focus(None)
     lib.rh_profile_enable(1)
-    rep = scene.step(dt)
+    rep = scene.step_requested(dt_requested)
     if rep.status != R.OK:
         print(f"survey step failed: {R.STATUS_NAMES.get(rep.status)}", file=sys.stderr)
         return 1
@@ -710,17 +717,18 @@ def main() -> int:
     lib.rh_profile_focus(dom.encode())
     lib.rh_profile_enable(1)
     lib.rh_device_synchronize()
-    lat, iters, divs = [], [], []
+    lat, iters, divs, dts = [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        rep = scene.step(dt)
+        rep = scene.step_requested(dt_requested)
         lat.append(time.perf_counter() - s0)
         if rep.status != R.OK:
             print(f"step failed: {R.STATUS_NAMES.get(rep.status)}", file=sys.stderr)
             return 1
         iters.append(rep.pressure.iterations)
         divs.append(rep.actual_divergence_max)
+        dts.append((rep.dt, rep.courant))
     lib.rh_device_synchronize()
     elapsed = time.perf_counter() - t0
     launches, secs = kernel_table()[dom]
@@ -779,6 +787,9 @@ def main() -> int:
             "relative_residual": args.rel,
             "obstacle": (f"sphere r={args.sphere:g} at (0.5,0.5,0.5), {int(solid.sum())} solid cells"
                          if solid is not None else "none"),
+            "dt_policy": {"requested": dt_requested, "max_courant": max_courant,
+                          "dt_mean": float(np.mean([d for d, _ in dts])),
+                          "courant_max": float(max(c for _, c in dts))},
             "iterations_mean": mean_iters,
             "max_div_u": max(divs),
             "latency_ms_p50": float(np.percentile(np.array(lat) * 1e3, 50)),
