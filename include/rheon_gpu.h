@@ -298,7 +298,8 @@ typedef struct rh_provider_report {
 
 /* ProviderWorkspace (src/solver_provider.rs:94-327): input checks, NaN
  * pre-fill, predict, residual qualification (on the device), and the native
- * fallback -- the device "jacobi-pcg-v1" -- when the candidate is rejected. */
+ * fallback -- the device "jacobi-pcg-v1" (or "sgs-pcg-v1", "rbsgs-pcg") -- when
+ * the candidate is rejected. */
 typedef struct rh_provider_workspace rh_provider_workspace;
 rh_status rh_provider_workspace_create(const rh_grid* grid, uint64_t limit,
                                        rh_method fallback, rh_provider_workspace** out);
@@ -397,7 +398,8 @@ rh_status rh_projection_input_id(const rh_projection_input* input, uint8_t out[3
 /* ProjectionRepresentation::sealed_v1 (:158-170). */
 rh_status rh_projection_representation_of(const rh_grid* grid, rh_projection_representation* out);
 /* ProjectionWorkspace::new (:283-335): limit = byte cap (16 N + 4 F + the
- * provider workspace), 0 = none; fallback = RH_JACOBI_PCG or RH_RBSGS_PCG. */
+ * provider workspace), 0 = none; fallback = RH_JACOBI_PCG, RH_SGS_PCG or
+ * RH_RBSGS_PCG. */
 rh_status rh_projection_workspace_create(const rh_grid* grid, uint64_t limit, rh_method fallback,
                                          rh_projection_workspace** out);
 void rh_projection_workspace_destroy(rh_projection_workspace* ws);
