@@ -33,7 +33,33 @@ constexpr int kBlock = kLanesX * kRowsPerBlock;
 #define RH_MAX_BLOCKS 16384
 #endif
 constexpr int kMaxBlocks = RH_MAX_BLOCKS;
-constexpr int kGroupsPerBlock = 2;
+#ifndef RH_GROUPS_PER_BLOCK
+#define RH_GROUPS_PER_BLOCK 2
+#endif
+constexpr int kGroupsPerBlock = RH_GROUPS_PER_BLOCK;
+
+// floor(n / d) for 0 <= n < 2^31 and a divisor d >= 1 fixed per launch,
+// as a multiply-high, an add and a shift (Granlund and Montgomery, "Division
+// by invariant integers using multiplication", 1994, the round-up method):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, then
+// floor(n / d) = (mulhi(m, n) + n) >> l; the sum stays below 2^32 because
+// mulhi(m, n) <= n < 2^31.  A runtime integer division costs ~18 VALU
+// instructions; kernels that map a flat index to (row, chunk) or (j, k)
+// every iteration take this form instead.
+struct FastDiv {
+  unsigned m;
+  int l;
+};
+inline FastDiv make_fast_div(long long d) {
+  FastDiv f{0u, 0};
+  if (d <= 0 || d > 0x7fffffffll) return f;  // never used for such a divisor
+  while ((1ll << f.l) < d) ++f.l;
+  f.m = (unsigned)(((1ull << 32) * ((1ull << f.l) - (unsigned long long)d)) / (unsigned long long)d + 1ull);
+  return f;
+}
+__device__ __forceinline__ int fast_div(int n, FastDiv f) {
+  return (int)((__umulhi(f.m, (unsigned)n) + (unsigned)n) >> f.l);
+}
 
 // Validated box geometry and operator constants, passed by value.
 struct DevGrid {
@@ -59,6 +85,9 @@ struct DevGrid {
   // x-, x+, y-, y+, z-, z+ open (the neighbour bits of the cached diagonal
   // table), bit 6 = fluid cell.  nullptr = the plain box.
   const unsigned char* mask;
+  // Divisors of the four-cell chunk loops: nx / 4 chunks per row, ny rows
+  // per plane (set by make_dev_grid).
+  FastDiv div_q, div_y;
 };
 
 constexpr unsigned char kFluidBit = 64;

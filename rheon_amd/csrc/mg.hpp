@@ -31,7 +31,16 @@ struct MgLevel {
   // active coarse cells, so prolongation needs no renormalisation there
   const unsigned char* octant = nullptr;
   double wf[3] = {0.0, 0.0, 0.0};       // w[d] * fscale
+  // Divisors of the four-cell chunk loops (set_level_divisors): nx / 4 and
+  // nx / 2 chunks per row (sweeps; the restriction onto this level), ny.
+  FastDiv div_q{}, div_h{}, div_y{};
 };
+
+inline void set_level_divisors(MgLevel& L) {
+  L.div_q = make_fast_div(L.nx / 4);
+  L.div_h = make_fast_div(L.nx / 2);
+  L.div_y = make_fast_div(L.ny);
+}
 
 constexpr unsigned kFracFluid = 1u << 30;
 

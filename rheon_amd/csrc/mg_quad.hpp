@@ -11,14 +11,15 @@
 namespace rh {
 
 // Chunk loop over this block's rows: chunk t covers cells 4q..4q+3 of one
-// row; consecutive threads take consecutive chunks (coalesced).
-#define FOR_QUADS(NX, NY)                                                   \
+// row; consecutive threads take consecutive chunks (coalesced).  DQ and DY
+// divide by the chunks per row ((NX) / 4) and by NY.
+#define FOR_QUADS(NX, NY, DQ, DY)                                           \
   {                                                                         \
     const int cpr_ = (NX) >> 2;                                             \
     const RowRange qr_ = block_span(cpr_ * (NY) * nz_, vb);                 \
     for (int t_ = qr_.begin + flat_tid(); t_ < qr_.end; t_ += kBlock) {     \
-      const int row = t_ / cpr_, q = t_ - row * cpr_;                       \
-      const int k = row / (NY), j = row - k * (NY);                         \
+      const int row = fast_div(t_, DQ), q = t_ - row * cpr_;                \
+      const int k = fast_div(row, DY), j = row - k * (NY);                  \
       const int i0 = q << 2;                                                \
       const int c = i0 + (NX) * row;
 #define END_QUADS \

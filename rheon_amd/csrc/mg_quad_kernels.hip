@@ -32,7 +32,7 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   DD s[1] = {{0.0, 0.0}};
   double m[2] = {0.0, 0.0};
   const int nz_ = g.nz;
-  FOR_QUADS(g.nx, g.ny)
+  FOR_QUADS(g.nx, g.ny, g.div_q, g.div_y)
   Quad<double> Q;
   if (kZeroP) {
 #pragma unroll
@@ -44,25 +44,27 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   load4(b, c, bb);
   const unsigned char* const mrow = row_mask(g.mask, g.nx, g.ny, g.nz, row, kRowPlain);
   const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
-  const int yz = box_yz(g.ny, g.nz, j, k);
-  float rb[4], rx[4];
+  float rb[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int mk = quad_mask(mrow, m4, g.nx, i0, yz, u);
-    if (mrow) close_faces(Q, u, mk);
+    if (mrow) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
     const double v = quad_apply_d(Q, u, g);
     if (!isfinite(v)) m[1] = 1.0;
     const double r = bb[u] - v;
     dd_acc(s[0], r * r);
     m[0] = fmax(m[0], fabs(r));
-    if (!kReport) {
-      rb[u] = (float)r;
-      rx[u] = (float)(tau * L0.odiag[mk]) * rb[u];
-    }
+    if (!kReport) rb[u] = (float)r;
   }
   if (!kReport) {
     store4(b0, c, rb);
-    if (x0) store4(x0, c, rx);
+    if (x0) {  // the first sweep's value, where it is stored
+      const int yz = box_yz(g.ny, g.nz, j, k);
+      float rx[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        rx[u] = (float)(tau * L0.odiag[quad_mask(mrow, m4, g.nx, i0, yz, u)]) * rb[u];
+      store4(x0, c, rx);
+    }
   }
   END_QUADS
   block_publish<1, 2>(s, m, pdd, pmx, vb);
@@ -77,8 +79,11 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
               float* __restrict__ xout, double* __restrict__ p, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
   const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
+  // scales of an open cell and of the x-wall cells of a box row
+  const float a63 = (float)(tau * L.odiag[63]), a62 = (float)(tau * L.odiag[62]);
+  const float a61 = (float)(tau * L.odiag[61]);
   const int nz_ = L.nz;
-  FOR_QUADS(L.nx, L.ny)
+  FOR_QUADS(L.nx, L.ny, L.div_q, L.div_y)
   Quad<float> Q;
   load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);
   float bb[4];
@@ -93,15 +98,30 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
     }
   }
   const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlain);
-  const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
-  const int yz = box_yz(L.ny, L.nz, j, k);
+  // an interior box-rule row: the cells' masks are 63 but for the x-wall
+  // cells (quad_box_mask), so the scales are the three entries above
+  const bool plain = !mrow && j > 0 && j + 1 < L.ny && k > 0 && k + 1 < L.nz;
+  float sc[4];
+  if (plain) {
+    sc[0] = i0 > 0 ? a63 : a62;
+    sc[1] = a63;
+    sc[2] = a63;
+    sc[3] = i0 + 4 < L.nx ? a63 : a61;
+  } else {
+    const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
+    const int yz = box_yz(L.ny, L.nz, j, k);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int mk = quad_mask(mrow, m4, L.nx, i0, yz, u);
+      if (mrow) close_faces(Q, u, mk);
+      sc[u] = (float)(tau * L.odiag[mk]);
+    }
+  }
   float xn[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int mk = quad_mask(mrow, m4, L.nx, i0, yz, u);
-    if (mrow) close_faces(Q, u, mk);
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
-    xn[u] = Q.c[u] + (float)(tau * L.odiag[mk]) * (bb[u] - ax);
+    xn[u] = Q.c[u] + sc[u] * (bb[u] - ax);
   }
   if (kUpdate) {
 #pragma unroll
@@ -124,7 +144,7 @@ b_mg_smooth_qf(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const
                float* __restrict__ xout, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
   const int nz_ = L.nz;
-  FOR_QUADS(L.nx, L.ny)
+  FOR_QUADS(L.nx, L.ny, L.div_q, L.div_y)
   Quad<float> Q;
   load_quad(xin, L.nx, L.ny, L.nz, i0, j, k, c, Q);
   float bb[4];
@@ -166,61 +186,94 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
     __syncthreads();
   }
   const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
+  // the scales of an open cell (mask 63) and of the x-wall cells of a box
+  // row (62: x- closed, 61: x+ closed), the table's entries
+  const float a63 = (float)(tau1 * L.odiag[63]), a62 = (float)(tau1 * L.odiag[62]);
+  const float a61 = (float)(tau1 * L.odiag[61]);
+  const float c63 = (float)(tau2 * L.odiag[63]), c62 = (float)(tau2 * L.odiag[62]);
+  const float c61 = (float)(tau2 * L.odiag[61]);
   const int nz_ = L.nz;
   const int sy = L.nx, sz = L.nx * L.ny;
-  FOR_QUADS(L.nx, L.ny)
+  FOR_QUADS(L.nx, L.ny, L.div_q, L.div_y)
   Quad<float> Q;
   load_quad(b, L.nx, L.ny, L.nz, i0, j, k, c, Q);
   float bb[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) bb[u] = Q.c[u];
-  // positions of the loaded values (wall neighbours clamp to the cell)
-  const int jm = j > 0 ? j - 1 : j, jp = j + 1 < L.ny ? j + 1 : j;
-  const int km = k > 0 ? k - 1 : k, kp = k + 1 < L.nz ? k + 1 : k;
-  const int il = i0 > 0 ? i0 - 1 : i0, ir = i0 + 4 < L.nx ? i0 + 4 : i0 + 3;
-  int mc[4], mym[4], myp[4], mzm[4], mzp[4], ml, mr;
   const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlainNear);
-  if (mrow) {
-    const unsigned a = *reinterpret_cast<const unsigned*>(mrow + c);
-    const unsigned ym4 = *reinterpret_cast<const unsigned*>(mrow + (c + (jm - j) * sy));
-    const unsigned yp4 = *reinterpret_cast<const unsigned*>(mrow + (c + (jp - j) * sy));
-    const unsigned zm4 = *reinterpret_cast<const unsigned*>(mrow + (c + (km - k) * sz));
-    const unsigned zp4 = *reinterpret_cast<const unsigned*>(mrow + (c + (kp - k) * sz));
+  // b0 -> x0 = (float)(tau1 / D) b0 at every loaded position (cells, y/z
+  // neighbours, the two x-edge neighbours), scaled in place; t2 = the
+  // cells' (float)(tau2 / D), mc = their masks (obstacle rows)
+  float t2[4], xl, xr;
+  int mc[4] = {63, 63, 63, 63};
+  // A box-rule row whose four neighbour rows are interior too (2 <= j <
+  // ny - 2, 2 <= k < nz - 2): every position's mask is 63 but for the
+  // x-wall cells (62 at i0 = 0, 61 at i0 + 4 = nx), which the same
+  // positions of the neighbour rows share; the clamped x-edge neighbour of
+  // a wall cell is the cell itself.  The box-rule branch below gives these
+  // same masks, so both give the same values.
+  const bool plain = !mrow && j >= 2 && j + 2 < L.ny && k >= 2 && k + 2 < L.nz;
+  if (plain) {
+    const float e0 = i0 > 0 ? a63 : a62, e3 = i0 + 4 < L.nx ? a63 : a61;
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      mc[u] = (int)((a >> (8 * u)) & 63u);
-      mym[u] = (int)((ym4 >> (8 * u)) & 63u);
-      myp[u] = (int)((yp4 >> (8 * u)) & 63u);
-      mzm[u] = (int)((zm4 >> (8 * u)) & 63u);
-      mzp[u] = (int)((zp4 >> (8 * u)) & 63u);
+      const float e = u == 0 ? e0 : (u == 3 ? e3 : a63);
+      Q.c[u] = e * Q.c[u];
+      Q.ym[u] = e * Q.ym[u];
+      Q.yp[u] = e * Q.yp[u];
+      Q.zm[u] = e * Q.zm[u];
+      Q.zp[u] = e * Q.zp[u];
     }
-    ml = mrow[c + (il - i0)] & 63;
-    mr = mrow[c + (ir - i0)] & 63;
+    xl = e0 * Q.xm[0];
+    xr = e3 * Q.xp[3];
+    t2[0] = i0 > 0 ? c63 : c62;
+    t2[1] = c63;
+    t2[2] = c63;
+    t2[3] = i0 + 4 < L.nx ? c63 : c61;
   } else {
-    // the box rule per row (quad_box_mask); the x-edge neighbours il = i0 - 1
-    // and ir = i0 + 4 are interior in x unless they clamp onto cells 0 / 3
-    const int yc = box_yz(L.ny, L.nz, j, k), yym = box_yz(L.ny, L.nz, jm, k), yyp = box_yz(L.ny, L.nz, jp, k);
-    const int yzm = box_yz(L.ny, L.nz, j, km), yzp = box_yz(L.ny, L.nz, j, kp);
+    // positions of the loaded values (wall neighbours clamp to the cell)
+    const int jm = j > 0 ? j - 1 : j, jp = j + 1 < L.ny ? j + 1 : j;
+    const int km = k > 0 ? k - 1 : k, kp = k + 1 < L.nz ? k + 1 : k;
+    const int il = i0 > 0 ? i0 - 1 : i0, ir = i0 + 4 < L.nx ? i0 + 4 : i0 + 3;
+    if (mrow) {
+      const unsigned a = *reinterpret_cast<const unsigned*>(mrow + c);
+      const unsigned ym4 = *reinterpret_cast<const unsigned*>(mrow + (c + (jm - j) * sy));
+      const unsigned yp4 = *reinterpret_cast<const unsigned*>(mrow + (c + (jp - j) * sy));
+      const unsigned zm4 = *reinterpret_cast<const unsigned*>(mrow + (c + (km - k) * sz));
+      const unsigned zp4 = *reinterpret_cast<const unsigned*>(mrow + (c + (kp - k) * sz));
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      mc[u] = quad_box_mask(L.nx, i0, yc, u);
-      mym[u] = quad_box_mask(L.nx, i0, yym, u);
-      myp[u] = quad_box_mask(L.nx, i0, yyp, u);
-      mzm[u] = quad_box_mask(L.nx, i0, yzm, u);
-      mzp[u] = quad_box_mask(L.nx, i0, yzp, u);
+      for (int u = 0; u < 4; ++u) {
+        mc[u] = (int)((a >> (8 * u)) & 63u);
+        Q.c[u] = s1[mc[u]] * Q.c[u];
+        Q.ym[u] = s1[(ym4 >> (8 * u)) & 63u] * Q.ym[u];
+        Q.yp[u] = s1[(yp4 >> (8 * u)) & 63u] * Q.yp[u];
+        Q.zm[u] = s1[(zm4 >> (8 * u)) & 63u] * Q.zm[u];
+        Q.zp[u] = s1[(zp4 >> (8 * u)) & 63u] * Q.zp[u];
+        t2[u] = s2[mc[u]];
+      }
+      xl = s1[mrow[c + (il - i0)] & 63] * Q.xm[0];
+      xr = s1[mrow[c + (ir - i0)] & 63] * Q.xp[3];
+    } else {
+      // the box rule per row (quad_box_mask); the x-edge neighbours il = i0 - 1
+      // and ir = i0 + 4 are interior in x unless they clamp onto cells 0 / 3
+      const int yc = box_yz(L.ny, L.nz, j, k), yym = box_yz(L.ny, L.nz, jm, k), yyp = box_yz(L.ny, L.nz, jp, k);
+      const int yzm = box_yz(L.ny, L.nz, j, km), yzp = box_yz(L.ny, L.nz, j, kp);
+      int m0 = 63, m3 = 63;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int mcu = quad_box_mask(L.nx, i0, yc, u);
+        if (u == 0) m0 = mcu;
+        if (u == 3) m3 = mcu;
+        Q.c[u] = s1[mcu] * Q.c[u];
+        Q.ym[u] = s1[quad_box_mask(L.nx, i0, yym, u)] * Q.ym[u];
+        Q.yp[u] = s1[quad_box_mask(L.nx, i0, yyp, u)] * Q.yp[u];
+        Q.zm[u] = s1[quad_box_mask(L.nx, i0, yzm, u)] * Q.zm[u];
+        Q.zp[u] = s1[quad_box_mask(L.nx, i0, yzp, u)] * Q.zp[u];
+        t2[u] = s2[mcu];
+      }
+      xl = s1[i0 > 0 ? (yc | 3) : m0] * Q.xm[0];
+      xr = s1[i0 + 4 < L.nx ? (yc | 3) : m3] * Q.xp[3];
     }
-    ml = i0 > 0 ? (yc | 3) : mc[0];
-    mr = i0 + 4 < L.nx ? (yc | 3) : mc[3];
-  }
-  // b0 -> x0 at every position
-  const float xl = s1[ml] * Q.xm[0], xr = s1[mr] * Q.xp[3];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    Q.c[u] = s1[mc[u]] * Q.c[u];
-    Q.ym[u] = s1[mym[u]] * Q.ym[u];
-    Q.yp[u] = s1[myp[u]] * Q.yp[u];
-    Q.zm[u] = s1[mzm[u]] * Q.zm[u];
-    Q.zp[u] = s1[mzp[u]] * Q.zp[u];
   }
   Q.xm[0] = xl;
   Q.xm[1] = Q.c[0];
@@ -235,7 +288,7 @@ b_mg_smooth_first_q(VBlock vb, const MgLevel& L, const float* __restrict__ b, fl
   for (int u = 0; u < 4; ++u) {
     if (mrow) close_faces(Q, u, mc[u]);
     const float ax = quad_apply_f(Q, u, w0, w1, w2);
-    xn[u] = Q.c[u] + s2[mc[u]] * (bb[u] - ax);
+    xn[u] = Q.c[u] + t2[u] * (bb[u] - ax);
   }
   store4(xout, c, xn);
   END_QUADS
@@ -252,7 +305,7 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
   const float w0 = (float)F.w[0], w1 = (float)F.w[1], w2 = (float)F.w[2];
   const int nz_ = Cl.nz;
   // coarse pairs: Cl.nx = F.nx / 2 is even, two coarse cells per lane
-  FOR_QUADS(2 * Cl.nx, Cl.ny)
+  FOR_QUADS(2 * Cl.nx, Cl.ny, Cl.div_h, Cl.div_y)
   (void)c;
   const int J = j, K = k, I0 = i0 >> 1;
   float s0 = 0.0f, s1 = 0.0f;
@@ -304,7 +357,7 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
   const int nz_ = F.nz;
   const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
   const float a = 0.75f, bq = 0.25f;
-  FOR_QUADS(F.nx, F.ny)
+  FOR_QUADS(F.nx, F.ny, F.div_q, F.div_y)
   const int I0 = i0 >> 1, J = j >> 1, K = k >> 1;
   int Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
   if (Jn < 0 || Jn >= Cl.ny) Jn = J;

@@ -96,6 +96,8 @@ DevGrid make_dev_grid(const rh_grid& g, double density) {
   d.nx = (int)g.counts[0];
   d.ny = (int)g.counts[1];
   d.nz = (int)g.counts[2];
+  d.div_q = make_fast_div(d.nx / 4);
+  d.div_y = make_fast_div(d.ny);
   for (int a = 0; a < 3; ++a) {
     d.h[a] = g.spacing[a];
     d.origin[a] = g.origin[a];

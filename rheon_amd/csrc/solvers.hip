@@ -399,6 +399,7 @@ rh_status MgSolver::create(const rh_grid& grid, const rh_solver_config& config) 
     h_.lv[l].nx = dims[l][0];
     h_.lv[l].ny = dims[l][1];
     h_.lv[l].nz = dims[l][2];
+    set_level_divisors(h_.lv[l]);
   }
   h_.levels = levels;
   const MgLevel& last = h_.lv[levels - 1];
