@@ -233,9 +233,10 @@ rh_status rh_solver_solve_rhs(rh_solver* solver, double density, const double* d
  * solid, host, len N) or NULL for the sealed box.  The discretisation of
  * StaticObstaclePressure (src/obstacle_pressure.rs:224-309) with open areas in
  * {0, A} and fluid volumes in {0, V}: a face is open iff both cells are fluid,
- * solid cells are not unknowns, and the gauge is the lowest-index fluid cell
- * (p there is exactly 0; p of solid cells is returned as 0).  The fluid must
- * be one connected region (UNSUPPORTED otherwise). */
+ * solid cells are not unknowns, and every connected fluid region has one
+ * gauge, its lowest-index cell (p there is exactly 0; p of solid cells is
+ * returned as 0), and its own compatibility check (src/obstacle_pressure.rs:
+ * 310-323, :455-472).  More than 254 regions: UNSUPPORTED. */
 rh_status rh_solver_set_solid(rh_solver* solver, const uint8_t* solid, size_t len);
 
 /* ------------------------------------------------- provider (host memory) */

@@ -129,6 +129,8 @@ def _declare(L):
     L.or_precondition_jacobi.argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double), C.c_int]
     L.or_solve_rhs.argtypes = [_G, C.c_double, _P, C.c_double, C.POINTER(_Settings), C.c_int, C.c_int, _P, C.POINTER(_Report)]
     L.or_obstacle_mask.argtypes = [_G, _P, _P, C.POINTER(C.c_int64)]
+    L.or_obstacle_components.argtypes = [_G, _P, _P]
+    L.or_obstacle_components.restype = C.c_int32
     L.or_apply_masked.argtypes = [_G, C.c_double, _P, _P, _P]
     L.or_precondition_masked.argtypes = [_G, C.c_double, _P, C.c_int64, C.c_int, _P, _P, C.POINTER(C.c_double)]
     L.or_solve_rhs_masked.argtypes = [_G, C.c_double, _P, C.c_int64, _P, C.c_double, C.POINTER(_Settings), C.c_int,
@@ -373,8 +375,13 @@ def solve_velocity(grid: Grid, density: float, vel, dt: float, settings: Setting
 
 @dataclass
 class Obstacle:
-    mask: np.ndarray  # uint8 per cell: bits 0..5 open faces x-,x+,y-,y+,z-,z+; bit 6 fluid
-    gauge: int
+    mask: np.ndarray  # uint8 per cell: bits 0..5 open faces x-,x+,y-,y+,z-,z+; bit 6 fluid; bit 7 gauge
+    gauge: int        # the first gauge (lowest-index fluid cell)
+
+    @property
+    def gauges(self) -> np.ndarray:
+        """One gauge per connected fluid region, in label order."""
+        return np.flatnonzero(self.mask & 128)
 
 
 def obstacle_mask(grid: Grid, solid: np.ndarray) -> Obstacle:
@@ -386,6 +393,16 @@ def obstacle_mask(grid: Grid, solid: np.ndarray) -> Obstacle:
     if st != OK:
         raise OracleError(st, "obstacle_mask")
     return Obstacle(mask, gauge.value)
+
+
+def obstacle_components(grid: Grid, mask: np.ndarray):
+    """(labels, count): connected fluid regions of a mask (-1 = solid)."""
+    mask = np.ascontiguousarray(mask, np.uint8)
+    labels = np.empty(grid.cell_len, np.int32)
+    count = lib().or_obstacle_components(C.byref(grid.s), _ptr(mask), _ptr(labels))
+    if count < 0:
+        raise OracleError(INVALID_ARGUMENT, "obstacle_components")
+    return labels, int(count)
 
 
 def apply_masked(grid: Grid, density: float, ob: Obstacle, x: np.ndarray) -> np.ndarray:

@@ -280,11 +280,37 @@ typedef struct op_coeff {
   double area[3], acc[3], w[3], diag[64];
   const uint8_t* mask;
   int64_t gauge;
+  /* several fluid components: one gauge per component (mask bit 128 on each
+   * component's gauge cell), labels[row] = component or -1 (solid) */
+  const int32_t* labels;
+  int32_t components;
 } op_coeff;
+
+#define OR_GAUGE_BIT 128
+
+/* The eliminated rows: the gauge cell, and with an obstacle every cell that
+ * carries the gauge bit (one per connected fluid region). */
+static inline int is_gauge(const op_coeff* c, int64_t row) {
+  return row == c->gauge || (c->mask && (c->mask[row] & OR_GAUGE_BIT));
+}
+
+/* r = z = 0 on the eliminated rows (src/pressure.rs:534-541; per component,
+ * src/obstacle_pressure.rs:385-388, :415-425). */
+static void zero_gauge_rows(const or_grid* g, const op_coeff* c, double* r, double* z) {
+  if (!c->mask) {
+    r[c->gauge] = 0.0;
+    z[c->gauge] = 0.0;
+    return;
+  }
+  for (int64_t row = 0; row < g->cell_len; ++row)
+    if (is_gauge(c, row)) r[row] = z[row] = 0.0;
+}
 
 static int op_init(const or_grid* g, double density, op_coeff* c) {
   c->mask = NULL;
   c->gauge = 0;
+  c->labels = NULL;
+  c->components = 1;
   if (!isfinite(density) || density <= 0.0) return OR_INVALID_DENSITY;
   const double* h = g->h;
   c->area[0] = h[1] * h[2];
@@ -605,10 +631,8 @@ double or_dot(const double* a, const double* b, int64_t n) {
 
 static int jacobi_c(const or_grid* g, const op_coeff* c, double* r, double* z, double* rz,
                     int threads) {
-  const int64_t gauge = c->gauge;
-  r[gauge] = 0.0;
-  z[gauge] = 0.0;
   int64_t n = g->cell_len;
+  zero_gauge_rows(g, c, r, z);
   const int64_t nx = g->n[0], ny = g->n[1];
   int status = OR_OK;
   int64_t chunks = (n + CHUNK_VALUES - 1) / CHUNK_VALUES;
@@ -617,7 +641,7 @@ static int jacobi_c(const or_grid* g, const op_coeff* c, double* r, double* z, d
   for (int64_t ch = 0; ch < chunks; ++ch) {
     int64_t e = (ch + 1) * CHUNK_VALUES < n ? (ch + 1) * CHUNK_VALUES : n;
     for (int64_t row = ch * CHUNK_VALUES; row < e; ++row) {
-      if (row == gauge) continue;
+      if (is_gauge(c, row)) continue;
       if (c->mask && !(c->mask[row] & 64)) {  // solid cell: not an unknown
         z[row] = 0.0;
         continue;
@@ -688,15 +712,13 @@ int or_precondition_sgs(const or_grid* g, double density, double* r, double* z, 
  *   red reverse:   z = y + (sum_nbr w*z) / D  (neighbours are black)
  * Neighbour order: axis 0..2, lower then upper. */
 static int rbsgs_c(const or_grid* g, const op_coeff* c, double* r, double* z, double* rz) {
-  const int64_t gauge = c->gauge;
-  r[gauge] = 0.0;
-  z[gauge] = 0.0;
   const int64_t nx = g->n[0], ny = g->n[1], n = g->cell_len;
+  zero_gauge_rows(g, c, r, z);
   const int64_t stride[3] = {1, nx, nx * ny};
   for (int pass = 0; pass < 3; ++pass) {
     int colour = pass == 1 ? 1 : 0;
     for (int64_t row = 0; row < n; ++row) {
-      if (row == gauge) continue;
+      if (is_gauge(c, row)) continue;
       int64_t p[3] = {row % nx, (row / nx) % ny, row / (nx * ny)};
       if (((p[0] + p[1] + p[2]) & 1) != colour) continue;
       if (c->mask && !(c->mask[row] & 64)) {  // solid cell: not an unknown
@@ -801,6 +823,26 @@ static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, doubl
     rep->error_value = sum;
     return OR_INCOMPATIBLE_RHS;
   }
+  if (c->components > 1) {
+    /* each component's sum within 64 eps of its absolute sum, components in
+     * label order, compensated sums in index order
+     * (src/obstacle_pressure.rs:455-472 with :164-183) */
+    for (int32_t comp = 0; comp < c->components; ++comp) {
+      double cs = 0.0, cc = 0.0, ca = 0.0, cac = 0.0;
+      for (int64_t i = 0; i < n; ++i)
+        if (c->labels[i] == comp) {
+          neumaier_add(&cs, &cc, rhs[i]);
+          neumaier_add(&ca, &cac, fabs(rhs[i]));
+        }
+      cs += cc;
+      ca += cac;
+      if (!isfinite(cs) || !isfinite(ca)) return OR_BREAKDOWN;
+      if (fabs(cs) > (64.0 * DBL_EPSILON) * ca) {
+        rep->error_value = cs;
+        return OR_INCOMPATIBLE_RHS;
+      }
+    }
+  }
   for (int64_t i = 0; i < n; ++i) p[i] = 0.0;
   double rhs_l2 = sqrt(or_dot(rhs, rhs, n));
   double threshold = maxd(st->absolute_residual, st->relative_residual * rhs_l2);
@@ -832,13 +874,17 @@ static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, doubl
   double rz;
   if ((status = precondition(g, c, method, w.r, w.z, &rz, threads))) goto done;
   memcpy(w.d, w.z, sizeof(double) * n);
-  /* element-wise updates skip the gauge row (src/pressure.rs:669-679) */
+  /* element-wise updates skip the gauge rows (src/pressure.rs:669-679) */
   const int64_t gauge = c->gauge;
+  const int multi = c->mask && c->components > 1;
   int par = use_parallel(n - 1, threads);
   int64_t chunks = (n + CHUNK_VALUES - 1) / CHUNK_VALUES;
   for (int64_t it = 1; it <= st->max_iterations; ++it) {
     if ((status = apply_full_c(g, c, w.d, w.q, threads))) goto done;
     w.q[gauge] = 0.0;
+    if (multi)
+      for (int64_t i = 0; i < n; ++i)
+        if (is_gauge(c, i)) w.q[i] = 0.0;
     double curvature = or_dot(w.d, w.q, n);
     if (!isfinite(curvature) || curvature <= 0.0 || !isfinite(rz) || rz <= 0.0) {
       rep->error_iteration = it;
@@ -856,7 +902,7 @@ static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, doubl
       int64_t b = ch * CHUNK_VALUES;
       int64_t e = b + CHUNK_VALUES < n ? b + CHUNK_VALUES : n;
       for (int64_t i = b; i < e; ++i) {
-        if (i == gauge) continue;
+        if (i == gauge || (multi && is_gauge(c, i))) continue;
         p[i] += alpha * w.d[i];
         w.r[i] -= alpha * w.q[i];
       }
@@ -898,7 +944,7 @@ static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, doubl
         int64_t b = ch * CHUNK_VALUES;
         int64_t e = b + CHUNK_VALUES < n ? b + CHUNK_VALUES : n;
         for (int64_t i = b; i < e; ++i)
-          if (i != gauge) w.d[i] = w.z[i] + beta * w.d[i];
+          if (!(i == gauge || (multi && is_gauge(c, i)))) w.d[i] = w.z[i] + beta * w.d[i];
       }
     }
     rz = next_rz;
@@ -922,10 +968,45 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
 /* Solid obstacles (builder-defined sphere scenes).  Discretisation of      */
 /* src/obstacle_pressure.rs:224-284 with open areas in {0, A} and fluid      */
 /* volumes in {0, V}: a face is open iff both cells are fluid; weights      */
-/* A/(rho h) on open faces; solid cells are not unknowns; one gauge at the  */
-/* lowest-index fluid cell (the largest-volume rule of :299-309 when all    */
-/* fluid volumes tie).  Summation order is the box operator's.              */
+/* A/(rho h) on open faces; solid cells are not unknowns.  The fluid cells  */
+/* are labelled by connected component (src/static_obstacle.rs:165-210:     */
+/* seeds in index order, breadth-first over open faces); each component    */
+/* has one gauge, its largest-volume cell with the lowest index on ties     */
+/* (src/obstacle_pressure.rs:310-323), i.e. its seed when all fluid volumes */
+/* tie, marked with OR_GAUGE_BIT in the mask.  Summation order is the box   */
+/* operator's.                                                              */
 /* ------------------------------------------------------------------------ */
+
+/* Component labels of the mask's fluid cells (-1 for solid cells); returns
+ * the number of components. */
+int32_t or_obstacle_components(const or_grid* g, const uint8_t* mask, int32_t* labels) {
+  const int64_t nx = g->n[0], ny = g->n[1], n = g->cell_len;
+  const int64_t off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
+  int64_t* queue = malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+  if (!queue) return -1;
+  for (int64_t row = 0; row < n; ++row) labels[row] = -1;
+  int32_t components = 0;
+  for (int64_t seed = 0; seed < n; ++seed) {
+    if (!(mask[seed] & 64) || labels[seed] >= 0) continue;
+    labels[seed] = components;
+    int64_t head = 0, tail = 0;
+    queue[tail++] = seed;
+    while (head < tail) {
+      const int64_t cell = queue[head++];
+      for (int side = 0; side < 6; ++side)
+        if ((mask[cell] >> side) & 1) {
+          const int64_t other = cell + off[side];
+          if (labels[other] < 0) {
+            labels[other] = components;
+            queue[tail++] = other;
+          }
+        }
+    }
+    ++components;
+  }
+  free(queue);
+  return components;
+}
 
 int or_obstacle_mask(const or_grid* g, const uint8_t* solid, uint8_t* mask, int64_t* gauge) {
   const int64_t nx = g->n[0], ny = g->n[1], n = g->cell_len;
@@ -944,7 +1025,34 @@ int or_obstacle_mask(const or_grid* g, const uint8_t* solid, uint8_t* mask, int6
     }
     mask[row] = m;
   }
-  return *gauge < 0 ? OR_INVALID_ARGUMENT : OR_OK;
+  if (*gauge < 0) return OR_INVALID_ARGUMENT;
+  int32_t* labels = malloc(sizeof(int32_t) * (size_t)n);
+  if (!labels) return OR_INVALID_ARGUMENT;
+  const int32_t components = or_obstacle_components(g, mask, labels);
+  /* the seed of each component (its lowest index) is its gauge */
+  int32_t next = 0;
+  for (int64_t row = 0; row < n && next < components; ++row)
+    if (labels[row] == next) {
+      mask[row] |= OR_GAUGE_BIT;
+      ++next;
+    }
+  free(labels);
+  return components < 1 ? OR_INVALID_ARGUMENT : OR_OK;
+}
+
+/* op_coeff of a masked operator: the labels of a mask with several gauges. */
+static int masked_coeff(const or_grid* g, const uint8_t* mask, int64_t gauge, op_coeff* c, int32_t** labels) {
+  c->mask = mask;
+  c->gauge = gauge;
+  *labels = NULL;
+  int64_t gauges = 0;
+  for (int64_t row = 0; row < g->cell_len; ++row) gauges += (mask[row] & OR_GAUGE_BIT) != 0;
+  if (gauges <= 1) return OR_OK;
+  *labels = malloc(sizeof(int32_t) * (size_t)g->cell_len);
+  if (!*labels) return OR_INVALID_ARGUMENT;
+  c->components = or_obstacle_components(g, mask, *labels);
+  c->labels = *labels;
+  return OR_OK;
 }
 
 int or_apply_masked(const or_grid* g, double density, const uint8_t* mask, const double* x,
@@ -961,10 +1069,12 @@ int or_precondition_masked(const or_grid* g, double density, const uint8_t* mask
   op_coeff c;
   int s = op_init(g, density, &c);
   if (s) return s;
-  c.mask = mask;
-  c.gauge = gauge;
   if (method == OR_SGS) return OR_INVALID_ARGUMENT;
-  return precondition(g, &c, method, r, z, rz, 1);
+  int32_t* labels = NULL;
+  if ((s = masked_coeff(g, mask, gauge, &c, &labels))) return s;
+  s = precondition(g, &c, method, r, z, rz, 1);
+  free(labels);
+  return s;
 }
 
 int or_solve_rhs_masked(const or_grid* g, double density, const uint8_t* mask, int64_t gauge,
@@ -975,9 +1085,11 @@ int or_solve_rhs_masked(const or_grid* g, double density, const uint8_t* mask, i
   int s = op_init(g, density, &c);
   if (s) return s;
   if (method == OR_SGS) return OR_INVALID_ARGUMENT;
-  c.mask = mask;
-  c.gauge = gauge;
-  return solve_c(g, &c, rhs, dt, st, method, threads, p, rep);
+  int32_t* labels = NULL;
+  if ((s = masked_coeff(g, mask, gauge, &c, &labels))) return s;
+  s = solve_c(g, &c, rhs, dt, st, method, threads, p, rep);
+  free(labels);
+  return s;
 }
 
 /* Corrected f32 faces with closed faces left at zero; max |div| over fluid

@@ -148,10 +148,14 @@ int or_actual_divergence_max_f32(const or_grid* g, const float* u, const float* 
 int or_actual_divergence_max_f64(const or_grid* g, const double* u, const double* v,
                                  const double* w, double* out);
 /* Solid obstacles: per-cell open-face mask from a solid-cell flag array
- * (bits 0..5 = faces x-,x+,y-,y+,z-,z+ open, bit 6 = fluid) and the gauge
- * (lowest-index fluid cell).  Discretisation of src/obstacle_pressure.rs:224-309
- * for all-or-nothing face apertures; summation order of the box operator. */
+ * (bits 0..5 = faces x-,x+,y-,y+,z-,z+ open, bit 6 = fluid, bit 7 = the
+ * gauge of a connected fluid region, its lowest-index cell) and the first
+ * gauge.  Discretisation of src/obstacle_pressure.rs:224-323 for
+ * all-or-nothing face apertures; summation order of the box operator.
+ * or_obstacle_components labels the fluid regions (src/static_obstacle.rs:
+ * 165-210; -1 = solid) and returns their count. */
 int or_obstacle_mask(const or_grid* g, const uint8_t* solid, uint8_t* mask, int64_t* gauge);
+int32_t or_obstacle_components(const or_grid* g, const uint8_t* mask, int32_t* labels);
 int or_apply_masked(const or_grid* g, double density, const uint8_t* mask, const double* x,
                     double* y);
 int or_precondition_masked(const or_grid* g, double density, const uint8_t* mask, int64_t gauge,

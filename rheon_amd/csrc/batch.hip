@@ -40,6 +40,7 @@ struct BatchScene {
   unsigned* flags;   // validation / arithmetic flags
   double* scal;      // [0..2] residual report, [3] max|div u|, [4..7] rate
   const double* p;
+  const DevGrid* g;  // the scene's operator record (regions of a split fluid)
   double volume, actual_limit, max_courant, dt;
   int gauge;
   int status;
@@ -107,6 +108,9 @@ __global__ void kb_end_solve(BatchScene* S, int n) {
   const bool shift = st == RH_OK || st == RH_ITERATION_LIMIT;
   c->spare[0] = shift ? S[s].p[S[s].gauge] : 0.0;
   c->spare[1] = shift ? 1.0 : 0.0;
+  const DevGrid& g = *S[s].g;
+  if (shift && g.ncomp > 1)
+    for (int k = 0; k < g.ncomp; ++k) g.comp_shift[k] = S[s].p[g.comp_gauge[k]];
   if (S[s].status == RH_OK) S[s].status = st;
 }
 
@@ -211,7 +215,7 @@ struct FinStatusOk {
 namespace {
 
 enum class Op {
-  kBeginStep, kValidateAcc, kAdvect, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kFill,
+  kBeginStep, kValidateAcc, kAdvect, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kCompSums, kCompCheck, kFill,
   kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothQFirst, kSmoothQFrac, kSmoothS, kRestrictQ, kRestrictQFrac, kRestrictS,
   kProlongQ, kProlongS, kTail, kCoarseDense, kEndSolve, kRegauge, kReport, kFinReport,
   kQuality, kCorrect, kDivMax, kFinDiv, kRate, kFinRate, kGate, kCommit
@@ -546,6 +550,7 @@ rh_status Batch::build() {
     b.max_courant = sc.max_courant;
     b.dt = dt_[i];
     b.gauge = grids[i].gauge;
+    b.g = d_grids_ + i;
   }
   RH_HIP(hipMemcpy(d_scenes_, h_scenes_.data(), sizeof(BatchScene) * n, hipMemcpyHostToDevice));
 
@@ -673,6 +678,22 @@ rh_status Batch::build() {
                                 return BFin<3, 1, FRhs>{scr.pdd, scr.pmx, nb, 0, FRhs{scr.ctl, FinRhsSums{scr.ctl}}};
                               },
                               3, 1, cb));
+  // per-region compatibility of split-fluid scenes
+  std::vector<int> compb(n, 0);
+  bool split = false;
+  for (size_t i = 0; i < n; ++i)
+    if (grids[i].ncomp > 1) {
+      compb[i] = grids[i].comp_nchunks;
+      split = true;
+    }
+  if (split) {
+    stage(Op::kCompSums, cells.data(), sizeof(BCellArgs), compb);
+    Launch L;
+    L.op = Op::kCompCheck;
+    L.args_off = T.add(cells.data(), sizeof(BCellArgs) * n);
+    L.blocks = (int)n;
+    program_.push_back(L);
+  }
   stage(Op::kFill, cells.data(), sizeof(BCellArgs), cb);
 
   // outer residual records (quad / scalar split) and the V-cycle stages
@@ -922,6 +943,8 @@ void Batch::run_range(size_t begin, size_t end, hipStream_t s) {
         kb_finish<3, 1, FinNotDone<FinRhsSums>><<<n, dim3(kLanesX, kRowsPerBlock), 0, s>>>(
             (const BFin<3, 1, FinNotDone<FinRhsSums>>*)bl.args);
         break;
+      case Op::kCompSums: launch_b_comp_sums(bl, s); break;
+      case Op::kCompCheck: launch_b_comp_check(bl, s); break;
       case Op::kFill: launch_b_fill_pressure(bl, s); break;
       case Op::kOuterQ: launch_b_mg_outer_q(bl, s); break;
       case Op::kOuterS: launch_b_mg_outer(bl, s); break;

@@ -59,6 +59,8 @@ void launch_b_rhs_sums(const BatchLaunch& bl, hipStream_t s);
 void launch_b_residual_report(const BatchLaunch& bl, hipStream_t s);
 void launch_b_fill_pressure(const BatchLaunch& bl, hipStream_t s);
 void launch_b_regauge(const BatchLaunch& bl, hipStream_t s);
+void launch_b_comp_sums(const BatchLaunch& bl, hipStream_t s);
+void launch_b_comp_check(const BatchLaunch& bl, hipStream_t s);  // bl.n blocks, one per scene
 
 // --------------------------------------------------------- multigrid
 struct BOuterArgs {

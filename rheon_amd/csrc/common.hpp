@@ -61,6 +61,11 @@ __device__ __forceinline__ int fast_div(int n, FastDiv f) {
   return (int)((__umulhi(f.m, (unsigned)n) + (unsigned)n) >> f.l);
 }
 
+// A chunk of one fluid region's cells in the region-sorted cell list.
+struct CompChunk {
+  int comp, begin, end, pad;
+};
+
 // Validated box geometry and operator constants, passed by value.
 struct DevGrid {
   int nx, ny, nz;
@@ -83,14 +88,29 @@ struct DevGrid {
   int gauge;       // eliminated gauge cell (0 in the sealed box)
   // Optional per-cell open-face mask for solid obstacles: bits 0..5 = faces
   // x-, x+, y-, y+, z-, z+ open (the neighbour bits of the cached diagonal
-  // table), bit 6 = fluid cell.  nullptr = the plain box.
+  // table), bit 6 = fluid cell, bit 7 = the gauge of a fluid region.
+  // nullptr = the plain box.
   const unsigned char* mask;
   // Divisors of the four-cell chunk loops: nx / 4 chunks per row, ny rows
   // per plane (set by make_dev_grid).
   FastDiv div_q, div_y;
+  // Obstacles that split the fluid into ncomp > 1 regions: one gauge per
+  // region (kGaugeBit in the mask), the region label of every cell (255 =
+  // solid), each region's gauge cell, a slot per region for the pressure
+  // captured at its gauge, and the region-sorted fluid cell list in chunks
+  // for the per-region compatibility sums.  ncomp <= 1: all unused.
+  int ncomp, comp_nchunks;
+  const unsigned char* comp_label;
+  const int* comp_gauge;
+  double* comp_shift;
+  const int* comp_cells;
+  const CompChunk* comp_chunks;
+  const int* comp_chunk_first;  // ncomp + 1 chunk offsets
 };
 
 constexpr unsigned char kFluidBit = 64;
+constexpr unsigned char kGaugeBit = 128;  // the gauge cell of a fluid region
+
 
 // Every byte offset into each face array of g (elements of `elem` bytes)
 // fits in 32 bits, so kernels may address them from a uniform base; and
@@ -103,6 +123,12 @@ inline bool face_offsets_32(const DevGrid& g, size_t elem) {
   if (x * y * (z + 1) > m) m = x * y * (z + 1);
   const unsigned long long yz = (y + 1) * (z + 1);
   return m * elem <= (1ull << 32) && x + 1 < (1ull << 24) && yz <= (1ull << 24);
+}
+
+// The eliminated rows: the gauge cell, and with a split fluid every region's
+// gauge (src/pressure.rs:534-541; src/obstacle_pressure.rs:385-388).
+__device__ __forceinline__ bool is_gauge(const DevGrid& g, int c) {
+  return c == g.gauge || (g.ncomp > 1 && (g.mask[c] & kGaugeBit));
 }
 
 // Interior face of axis a with face lattice index (i, j, k): its upper cell

@@ -158,6 +158,7 @@ void launch_precondition(const DevGrid& g, int method, double* r, double* z, Pcg
 void sgs_plan(const DevGrid& g, std::vector<SgsTile>& tiles, std::vector<int>& wave_start);
 void launch_residual_report(const DevGrid& g, const double* p, const double* b, const Reduce& red,
                             hipStream_t s);
+void launch_comp_check(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx, hipStream_t s);
 void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
                      hipStream_t s);
 

@@ -712,17 +712,21 @@ k_mg_add(DevGrid g, const S* __restrict__ x, double* __restrict__ p, const PcgCo
   END_CELLS
 }
 
-// p <- p - p0 with p0 = p[gauge] captured in ctl->spare[0] beforehand;
+// p <- p - p0 with p0 = p[gauge] captured in ctl->spare[0] beforehand (a
+// split fluid: each region's p at its own gauge, captured per region);
 // solid cells (not unknowns) are set to 0.
 __global__ void k_capture_gauge(const double* __restrict__ p, int gauge, PcgControl* ctl) {
   if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[gauge];
+}
+__global__ void k_capture_comp_gauges(DevGrid g, const double* __restrict__ p) {
+  for (int k = threadIdx.x; k < g.ncomp; k += blockDim.x) g.comp_shift[k] = p[g.comp_gauge[k]];
 }
 
 __global__ void __launch_bounds__(kBlock)
 k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl) {
   const double p0 = ctl->spare[0];
   FOR_CELLS(g)
-  p[c] = is_fluid(g, c) ? p[c] - p0 : 0.0;
+  p[c] = is_fluid(g, c) ? p[c] - (g.ncomp > 1 ? g.comp_shift[g.comp_label[c]] : p0) : 0.0;
   END_CELLS
 }
 
@@ -940,7 +944,10 @@ void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hi
 }
 
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
-  k_capture_gauge<<<1, 64, 0, s>>>(p, g.gauge, ctl);
+  if (g.ncomp > 1)
+    k_capture_comp_gauges<<<1, 256, 0, s>>>(g, p);
+  else
+    k_capture_gauge<<<1, 64, 0, s>>>(p, g.gauge, ctl);
   k_regauge<<<cells_grid(g), block_shape(), 0, s>>>(g, p, ctl);
 }
 

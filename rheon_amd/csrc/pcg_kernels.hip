@@ -189,6 +189,69 @@ k_rhs_sums(DevGrid g, const double* __restrict__ b, DD* pdd, double* pmx) {
   b_rhs_sums(hw_block(), g, b, pdd, pmx);
 }
 
+// Compatibility of a split fluid (src/obstacle_pressure.rs:455-472): every
+// region's right-hand side sums to zero within 64 eps of its absolute sum.
+// Chunk sums over the region-sorted cell list (one block per chunk, chunks
+// never span two regions), then one block adds each region's chunk partials
+// in chunk order and checks the regions in label order; the first failing
+// region reports its sum.
+__device__ __forceinline__ void b_comp_sums(const DevGrid& g, const double* __restrict__ b, VBlock vb, DD* pdd,
+                                            double* pmx) {
+  const CompChunk ch = g.comp_chunks[vb.b];
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double none[1] = {0.0};
+  for (int t = ch.begin + flat_tid(); t < ch.end; t += kBlock) {
+    const double v = b[g.comp_cells[t]];
+    dd_acc(s[0], v);
+    dd_acc(s[1], fabs(v));
+  }
+  block_publish<2, 1>(s, none, pdd, pmx, vb);
+}
+__device__ __forceinline__ void b_comp_check(const DevGrid& g, const DD* __restrict__ pdd, PcgControl* ctl) {
+  __shared__ int stop;
+  const int t = flat_tid();
+  for (int k = 0; k < g.ncomp; ++k) {
+    DD v[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int q = g.comp_chunk_first[k] + t; q < g.comp_chunk_first[k + 1]; q += kBlock) {
+      v[0] = dd_add(v[0], pdd[q]);
+      v[1] = dd_add(v[1], pdd[kMaxBlocks + q]);
+    }
+    block_reduce_dd<2>(v);
+    if (t == 0) {
+      stop = 0;
+      const double sum = dd_value(v[0]), sum_abs = dd_value(v[1]);
+      if (!isfinite(sum) || !isfinite(sum_abs)) {
+        fail(ctl, RH_BREAKDOWN, 0);
+        stop = 1;
+      } else if (fabs(sum) > (64.0 * 2.220446049250313e-16) * sum_abs) {
+        ctl->error_value = sum;
+        fail(ctl, RH_INCOMPATIBLE_RHS, 0);
+        stop = 1;
+      }
+    }
+    __syncthreads();
+    if (stop) return;
+    __syncthreads();
+  }
+}
+__global__ void __launch_bounds__(kBlock)
+k_comp_sums(DevGrid g, const double* __restrict__ b, const PcgControl* ctl, DD* pdd, double* pmx) {
+  if (ctl->done) return;
+  b_comp_sums(g, b, hw_block(), pdd, pmx);
+}
+__global__ void __launch_bounds__(kBlock) k_comp_check(DevGrid g, const DD* __restrict__ pdd, PcgControl* ctl) {
+  if (ctl->done) return;
+  b_comp_check(g, pdd, ctl);
+}
+
+// After the whole-domain compatibility sums: the per-region check of a
+// split fluid (nothing for one region).
+void launch_comp_check(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx, hipStream_t s) {
+  if (g.ncomp <= 1) return;
+  k_comp_sums<<<g.comp_nchunks, block_shape(), 0, s>>>(g, b, ctl, pdd, pmx);
+  k_comp_check<<<1, block_shape(), 0, s>>>(g, pdd, ctl);
+}
+
 // q = K d, q[0] = 0 (gauge), partial d.q.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -200,7 +263,7 @@ k_apply_dir(DevGrid g, const T* __restrict__ d, T* __restrict__ q, const PcgCont
   FOR_CELLS(g)
   double v = apply_at<T>(g, d, c, i, j, k);
   if (!isfinite(v)) bad[0] = 1.0;
-  if (c == g.gauge) v = 0.0;
+  if (is_gauge(g, c)) v = 0.0;
   const T qs = (T)v;
   q[c] = qs;
   dd_acc(s[0], ld(d, c) * (double)qs);
@@ -220,7 +283,7 @@ k_update(DevGrid g, double* __restrict__ p, T* __restrict__ r, const T* __restri
   DD s[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
   double none[1] = {0.0};
   FOR_CELLS(g)
-  if (c == g.gauge) {
+  if (is_gauge(g, c)) {
     if (SGS) z[c] = (T)0.0;
     continue;
   }
@@ -274,7 +337,7 @@ k_replace(DevGrid g, T* r, const T* prod, T* __restrict__ z, const PcgControl* c
   DD s[1] = {{0.0, 0.0}};
   double none[1] = {0.0};
   FOR_CELLS(g)
-  if (c == g.gauge) {
+  if (is_gauge(g, c)) {
     r[c] = (T)0.0;
     if (SGS) z[c] = (T)0.0;
     continue;
@@ -300,7 +363,7 @@ k_sgs_black(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgCont
             int mode) {
   if (!sgs_active(ctl, it, mode)) return;
   FOR_CELLS(g)
-  if (((i + j + k) & 1) == 0 || c == g.gauge) continue;
+  if (((i + j + k) & 1) == 0 || is_gauge(g, c)) continue;
   const double v = nbr_sum<T>(g, z, ld(r, c), c, i, j, k);
   z[c] = (T)jdiv(v, diag_at(g, c, i, j, k));
   END_CELLS
@@ -317,7 +380,7 @@ k_sgs_red(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const PcgContro
   double none[1] = {0.0};
   FOR_CELLS(g)
   double zv;
-  if (((i + j + k) & 1) == 0 && c != g.gauge) {
+  if (((i + j + k) & 1) == 0 && !is_gauge(g, c)) {
     const double corr = nbr_sum<T>(g, z, 0.0, c, i, j, k);
     const T zn = (T)(ld(z, c) + jdiv(corr, diag_at(g, c, i, j, k)));
     z[c] = zn;
@@ -367,7 +430,7 @@ k_sgs_lex_wave(DevGrid g, const T* __restrict__ r, T* __restrict__ z, const SgsT
       if (li + lj + lk != l) continue;
       const int i = x0 + li, j = y0 + lj, k = z0 + lk;
       const int c = i + sy * j + sz * k;
-      if (c == g.gauge) {
+      if (is_gauge(g, c)) {
         tz[q] = (T)0.0;
         continue;
       }
@@ -471,7 +534,7 @@ k_direction(DevGrid g, const T* __restrict__ r, const T* __restrict__ z, T* __re
   const bool copy = it == 0 || ctl->replaced;
   const double beta = copy ? 0.0 : next_rz / ctl->rz[it & 1];
   FOR_CELLS(g)
-  if (c == g.gauge) {
+  if (is_gauge(g, c)) {
     d[c] = (T)0.0;
     continue;
   }
@@ -489,7 +552,7 @@ k_direction(DevGrid g, const T* __restrict__ r, const T* __restrict__ z, T* __re
 // and the partial r.z of the new rz is published beside d.q.
 template <typename T>
 __device__ __forceinline__ T jdir_from(const DevGrid& g, const T* __restrict__ prod, int c, int i, int j, int k) {
-  return c == g.gauge ? (T)0.0 : (T)jdiv(ld(prod, c), diag_at(g, c, i, j, k));
+  return is_gauge(g, c) ? (T)0.0 : (T)jdiv(ld(prod, c), diag_at(g, c, i, j, k));
 }
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -534,7 +597,7 @@ k_apply_dir_j(DevGrid g, T* __restrict__ r, T* __restrict__ d, T* __restrict__ q
     v += g.w[2] * (xc - f);
     dc = xc;
     d[c] = dT;
-    if (c == g.gauge) {
+    if (is_gauge(g, c)) {
       r[c] = (T)0.0;
     } else {
       const T rn = prod[c];
@@ -544,7 +607,7 @@ k_apply_dir_j(DevGrid g, T* __restrict__ r, T* __restrict__ d, T* __restrict__ q
     }
   }
   if (!isfinite(v)) bad[0] = 1.0;
-  if (c == g.gauge) v = 0.0;
+  if (is_gauge(g, c)) v = 0.0;
   const T qs = (T)v;
   q[c] = qs;
   dd_acc(s[0], dc * (double)qs);
@@ -579,7 +642,7 @@ k_direction_j(DevGrid g, const double* __restrict__ p, const double* __restrict_
     dd_acc(s[0], pr * pr);
     m[0] = fmax(m[0], fabs(pr));
   }
-  if (c == g.gauge) {
+  if (is_gauge(g, c)) {
     d[c] = (T)0.0;
     continue;
   }
@@ -662,8 +725,21 @@ kb_regauge(const BCellArgs* __restrict__ A, const int* __restrict__ prefix, int 
   const double p0 = a.ctl->spare[0];
   const DevGrid& g = *a.g;
   FOR_CELLS_VB(g, vb)
-  a.p[c] = is_fluid(g, c) ? a.p[c] - p0 : 0.0;
+  a.p[c] = is_fluid(g, c) ? a.p[c] - (g.ncomp > 1 ? g.comp_shift[g.comp_label[c]] : p0) : 0.0;
   END_CELLS
+}
+// Per-region compatibility of split-fluid scenes (blocks = chunks; scenes
+// with one region have none), then one checking block per scene.
+__global__ void __launch_bounds__(kBlock)
+kb_comp_sums(const BCellArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
+  RH_BATCH_PROLOGUE(BCellArgs)
+  if (a.ctl->done) return;
+  b_comp_sums(*a.g, a.b, vb, a.pdd, a.pmx);
+}
+__global__ void __launch_bounds__(kBlock) kb_comp_check(const BCellArgs* __restrict__ A, int n) {
+  const BCellArgs& a = A[blockIdx.x];
+  if (a.g->ncomp <= 1 || a.ctl->done) return;
+  b_comp_check(*a.g, a.pdd, a.ctl);
 }
 
 void launch_b_rhs_sums(const BatchLaunch& bl, hipStream_t s) {
@@ -680,6 +756,12 @@ void launch_b_fill_pressure(const BatchLaunch& bl, hipStream_t s) {
 void launch_b_regauge(const BatchLaunch& bl, hipStream_t s) {
   if (bl.blocks) kb_regauge<<<bl.blocks, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.prefix, bl.n);
 }
+void launch_b_comp_sums(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.blocks) kb_comp_sums<<<bl.blocks, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.prefix, bl.n);
+}
+void launch_b_comp_check(const BatchLaunch& bl, hipStream_t s) {
+  if (bl.n) kb_comp_check<<<bl.n, block_shape(), 0, s>>>((const BCellArgs*)bl.args, bl.n);
+}
 
 // ------------------------------------------------------------ host launchers
 
@@ -691,6 +773,7 @@ void pcg_prologue(const DevGrid& g, const PcgArrays& a, PcgControl* ctl, DD* pdd
   const dim3 grid = cells_grid(g);
   k_rhs_sums<<<grid, block_shape(), 0, s>>>(g, a.b, pdd, pmx);
   launch_finish<3, 1>(pdd, pmx, grid.x, FinRhsSums{ctl}, s);
+  launch_comp_check(g, a.b, ctl, pdd, pmx, s);
   k_fill_zero<double><<<fill_blocks, block_shape(), 0, s>>>(a.p64, n);
   launch(k_true_residual<T>, grid, block_shape(), 0, s, g, a.p64, a.b, (T*)a.q, ctl, pdd, pmx, 0);
   launch_finish<1, 2>(pdd, pmx, grid.x, FinTrue{ctl, 0}, s);
@@ -858,6 +941,7 @@ void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd
   const dim3 grid = cells_grid(g);
   k_rhs_sums<<<grid, block_shape(), 0, s>>>(g, b, pdd, pmx);
   launch_finish<3, 1>(pdd, pmx, grid.x, FinRhsSums{ctl}, s);
+  launch_comp_check(g, b, ctl, pdd, pmx, s);
 }
 
 }  // namespace rh

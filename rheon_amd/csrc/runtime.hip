@@ -98,6 +98,7 @@ DevGrid make_dev_grid(const rh_grid& g, double density) {
   d.nz = (int)g.counts[2];
   d.div_q = make_fast_div(d.nx / 4);
   d.div_y = make_fast_div(d.ny);
+  d.ncomp = 1;
   for (int a = 0; a < 3; ++a) {
     d.h[a] = g.spacing[a];
     d.origin[a] = g.origin[a];
@@ -153,23 +154,20 @@ rh_status check_density(const rh_grid& g, double density) {
   return RH_OK;
 }
 
-// Solid obstacles (src/obstacle_pressure.rs:224-309 with all-or-nothing
-// apertures): open-face mask per cell from a solid flag per cell, and the
-// gauge at the lowest-index fluid cell.  The fluid must be one connected
-// region (the reference keeps one gauge per component; this solver keeps
-// one).
+// Solid obstacles (src/obstacle_pressure.rs:224-323 with all-or-nothing
+// apertures): open-face mask per cell from a solid flag per cell, the fluid
+// regions and one gauge per region (the first at *gauge).
 rh_status build_obstacle_mask(const rh_grid& g, const uint8_t* solid, std::vector<uint8_t>& mask,
-                              int* gauge) {
+                              int* gauge, ObstacleComponents* comps) {
   const int64_t nx = (int64_t)g.counts[0], ny = (int64_t)g.counts[1], nz = (int64_t)g.counts[2];
   const int64_t n = nx * ny * nz;
   mask.assign((size_t)n, 0);
-  int64_t first = -1, fluid = 0;
+  int64_t first = -1;
   for (int64_t k = 0; k < nz; ++k)
     for (int64_t j = 0; j < ny; ++j)
       for (int64_t i = 0; i < nx; ++i) {
         const int64_t c = i + nx * (j + ny * k);
         if (solid[c]) continue;
-        ++fluid;
         if (first < 0) first = c;
         uint8_t m = kFluidBit;
         if (i > 0 && !solid[c - 1]) m |= 1;
@@ -184,32 +182,48 @@ rh_status build_obstacle_mask(const rh_grid& g, const uint8_t* solid, std::vecto
     set_last_error("every cell is solid", "build_obstacle_mask");
     return RH_INVALID_ARGUMENT;
   }
-  // connectivity: flood fill over open faces from the gauge
-  std::vector<uint8_t> seen((size_t)n, 0);
-  std::vector<int64_t> stack;
-  stack.push_back(first);
-  seen[(size_t)first] = 1;
-  int64_t reached = 0;
+  // Fluid regions: seeds in index order, breadth-first over open faces
+  // (src/static_obstacle.rs:165-210).  Each region's gauge is its largest-
+  // volume cell, lowest index on ties (src/obstacle_pressure.rs:310-323): with
+  // every fluid volume equal, its seed.  The gauge cells carry kGaugeBit.
+  std::vector<uint8_t> label((size_t)n, kNoComponent);
+  std::vector<int64_t> queue;
+  queue.reserve(1024);
   const int64_t off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
-  while (!stack.empty()) {
-    const int64_t c = stack.back();
-    stack.pop_back();
-    ++reached;
-    const uint8_t m = mask[(size_t)c];
-    for (int side = 0; side < 6; ++side)
-      if ((m >> side) & 1) {
-        const int64_t d = c + off[side];
-        if (!seen[(size_t)d]) {
-          seen[(size_t)d] = 1;
-          stack.push_back(d);
+  std::vector<int> gauges;
+  for (int64_t seed = first; seed < n; ++seed) {
+    if (!(mask[(size_t)seed] & kFluidBit) || label[(size_t)seed] != kNoComponent) continue;
+    if ((int)gauges.size() == kMaxComponents) {
+      set_last_error("the fluid cells form more than 254 separate regions", "build_obstacle_mask");
+      return RH_UNSUPPORTED;
+    }
+    const uint8_t id = (uint8_t)gauges.size();
+    gauges.push_back((int)seed);
+    mask[(size_t)seed] |= kGaugeBit;
+    label[(size_t)seed] = id;
+    queue.clear();
+    queue.push_back(seed);
+    for (size_t head = 0; head < queue.size(); ++head) {
+      const int64_t c = queue[head];
+      const uint8_t m = mask[(size_t)c];
+      for (int side = 0; side < 6; ++side)
+        if ((m >> side) & 1) {
+          const int64_t d = c + off[side];
+          if (label[(size_t)d] == kNoComponent) {
+            label[(size_t)d] = id;
+            queue.push_back(d);
+          }
         }
-      }
-  }
-  if (reached != fluid) {
-    set_last_error("the fluid cells form more than one connected region", "build_obstacle_mask");
-    return RH_UNSUPPORTED;
+    }
   }
   *gauge = (int)first;
+  if (comps) {
+    comps->gauges = gauges;
+    if (gauges.size() > 1)
+      comps->labels.swap(label);
+    else
+      comps->labels.clear();
+  }
   return RH_OK;
 }
 

@@ -16,8 +16,16 @@ const char* last_error();
 rh_status check_indexable(const rh_grid* g);
 rh_status check_density(const rh_grid& g, double density);
 rh_status flags_to_status(unsigned f);
+// Fluid regions of an obstacle mask (src/static_obstacle.rs:165-210): label
+// per cell (kNoComponent for solid cells) and each region's gauge cell.
+constexpr uint8_t kNoComponent = 255;
+constexpr int kMaxComponents = 254;
+struct ObstacleComponents {
+  std::vector<int> gauges;       // lowest-index cell of each region, in label order
+  std::vector<uint8_t> labels;   // per cell; filled only with more than one region
+};
 rh_status build_obstacle_mask(const rh_grid& g, const uint8_t* solid, std::vector<uint8_t>& mask,
-                              int* gauge);
+                              int* gauge, ObstacleComponents* comps);
 
 // Per-object device scratch: a stream, reduction partials, counters, error
 // flags, result scalars, the solver control block and a pinned readback page.
@@ -63,10 +71,11 @@ class Solver {
   const rh_grid& grid() const { return grid_; }
   // Solid obstacles: `solid` holds one flag per cell (nonzero = solid) on
   // the host, or is null for the plain box.  Builds the open-face mask,
-  // checks that the fluid is connected and keeps the mask on the device.
+  // labels the fluid regions (one gauge each) and keeps the mask on the device.
   virtual rh_status set_solid(const uint8_t* solid);
   const unsigned char* device_mask() const { return d_mask_; }
   int gauge() const { return gauge_; }
+  int components() const { return ncomp_; }
   // Operator constants with this solver's obstacle mask and gauge.
   DevGrid dev_grid(double density) const;
   // Changes whenever device-side operator data is replaced (obstacles,
@@ -86,6 +95,8 @@ class Solver {
                   void* ctx);
   rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out);
   void release_mask();
+  void release_components();
+  rh_status upload_components(const ObstacleComponents& comps);
   rh_grid grid_{};
   rh_solver_config config_{};
   Scratch scratch_;
@@ -93,6 +104,18 @@ class Solver {
   std::vector<uint8_t> hmask_;      // host copy of the obstacle mask (empty = box)
   unsigned char* d_mask_ = nullptr;  // device obstacle mask
   int gauge_ = 0;
+  // Several fluid regions: one device allocation holding the region labels,
+  // gauge cells, captured gauge pressures, the region-sorted cell list and
+  // its chunk table (DevGrid's comp_* fields).
+  void* d_comp_ = nullptr;
+  int ncomp_ = 1, comp_nchunks_ = 0;
+  const unsigned char* comp_label_ = nullptr;
+  const int* comp_gauge_ = nullptr;
+  double* comp_shift_ = nullptr;
+  const int* comp_cells_ = nullptr;
+  const CompChunk* comp_chunks_ = nullptr;
+  const int* comp_chunk_first_ = nullptr;
+  std::vector<int> gauges_;  // host copy of the gauge cells
   uint64_t generation_ = 0;
 };
 

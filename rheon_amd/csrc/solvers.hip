@@ -114,7 +114,76 @@ void Solver::release_mask() {
   d_mask_ = nullptr;
   hmask_.clear();
   gauge_ = 0;
+  release_components();
   ++generation_;
+}
+
+void Solver::release_components() {
+  if (d_comp_) (void)hipFree(d_comp_);
+  d_comp_ = nullptr;
+  ncomp_ = 1;
+  comp_nchunks_ = 0;
+  comp_label_ = nullptr;
+  comp_gauge_ = nullptr;
+  comp_shift_ = nullptr;
+  comp_cells_ = nullptr;
+  comp_chunks_ = nullptr;
+  comp_chunk_first_ = nullptr;
+  gauges_.clear();
+}
+
+// Device tables of a fluid split into several regions: labels, gauges, one
+// captured-pressure slot per region, and the fluid cells sorted by region
+// (index order within a region) in chunks of at most `len` cells, none
+// spanning two regions, so every region's compatibility sums are formed
+// over its own cells.
+rh_status Solver::upload_components(const ObstacleComponents& comps) {
+  release_components();
+  const int nc = (int)comps.gauges.size();
+  if (nc <= 1) return RH_OK;
+  const size_t n = (size_t)grid_.counts[0] * grid_.counts[1] * grid_.counts[2];
+  std::vector<int> count(nc, 0);
+  for (size_t c = 0; c < n; ++c)
+    if (comps.labels[c] != kNoComponent) ++count[comps.labels[c]];
+  std::vector<int> start(nc + 1, 0);
+  for (int k = 0; k < nc; ++k) start[k + 1] = start[k] + count[k];
+  const int fluid = start[nc];
+  std::vector<int> cells(fluid), fill(start.begin(), start.end() - 1);
+  for (size_t c = 0; c < n; ++c)
+    if (comps.labels[c] != kNoComponent) cells[fill[comps.labels[c]]++] = (int)c;
+  const int len = std::max(2048, (fluid + 8191) / 8192);
+  std::vector<CompChunk> chunks;
+  std::vector<int> first(nc + 1, 0);
+  for (int k = 0; k < nc; ++k) {
+    first[k] = (int)chunks.size();
+    for (int b = start[k]; b < start[k + 1]; b += len) chunks.push_back(CompChunk{k, b, std::min(b + len, start[k + 1]), 0});
+  }
+  first[nc] = (int)chunks.size();
+  if ((int)chunks.size() > kMaxBlocks) return RH_UNSUPPORTED;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t o_label = 0, o_gauge = up(n), o_shift = o_gauge + up(sizeof(int) * nc),
+               o_cells = o_shift + up(sizeof(double) * nc), o_chunks = o_cells + up(sizeof(int) * (size_t)fluid),
+               o_first = o_chunks + up(sizeof(CompChunk) * chunks.size()),
+               total = o_first + up(sizeof(int) * (size_t)(nc + 1));
+  char* d = nullptr;
+  RH_HIP(hipMalloc(&d, total));
+  d_comp_ = d;
+  RH_HIP(hipMemset(d, 0, total));
+  RH_HIP(hipMemcpy(d + o_label, comps.labels.data(), n, hipMemcpyHostToDevice));
+  RH_HIP(hipMemcpy(d + o_gauge, comps.gauges.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+  RH_HIP(hipMemcpy(d + o_cells, cells.data(), sizeof(int) * (size_t)fluid, hipMemcpyHostToDevice));
+  RH_HIP(hipMemcpy(d + o_chunks, chunks.data(), sizeof(CompChunk) * chunks.size(), hipMemcpyHostToDevice));
+  RH_HIP(hipMemcpy(d + o_first, first.data(), sizeof(int) * (size_t)(nc + 1), hipMemcpyHostToDevice));
+  ncomp_ = nc;
+  comp_nchunks_ = (int)chunks.size();
+  comp_label_ = (const unsigned char*)(d + o_label);
+  comp_gauge_ = (const int*)(d + o_gauge);
+  comp_shift_ = (double*)(d + o_shift);
+  comp_cells_ = (const int*)(d + o_cells);
+  comp_chunks_ = (const CompChunk*)(d + o_chunks);
+  comp_chunk_first_ = (const int*)(d + o_first);
+  gauges_ = comps.gauges;
+  return RH_OK;
 }
 
 // Row flags stored on the device right after the level-0 mask (one byte per
@@ -155,8 +224,10 @@ rh_status Solver::set_solid(const uint8_t* solid) {
   }
   std::vector<uint8_t> mask;
   int gauge = 0;
-  rh_status st = build_obstacle_mask(grid_, solid, mask, &gauge);
+  ObstacleComponents comps;
+  rh_status st = build_obstacle_mask(grid_, solid, mask, &gauge, &comps);
   if (st) return st;
+  if ((st = upload_components(comps))) return st;
   const std::vector<uint8_t> rows = mask_row_flags(grid_, mask);
   if (!d_mask_) RH_HIP(hipMalloc(&d_mask_, mask.size() + rows.size()));
   RH_HIP(hipMemcpy(d_mask_, mask.data(), mask.size(), hipMemcpyHostToDevice));
@@ -171,6 +242,14 @@ DevGrid Solver::dev_grid(double density) const {
   DevGrid g = make_dev_grid(grid_, density);
   g.mask = d_mask_;
   g.gauge = gauge_;
+  g.ncomp = d_mask_ ? ncomp_ : 1;
+  g.comp_nchunks = comp_nchunks_;
+  g.comp_label = comp_label_;
+  g.comp_gauge = comp_gauge_;
+  g.comp_shift = comp_shift_;
+  g.comp_cells = comp_cells_;
+  g.comp_chunks = comp_chunks_;
+  g.comp_chunk_first = comp_chunk_first_;
   return g;
 }
 
@@ -469,37 +548,63 @@ struct LevelFaces {
   }
 };
 
-// G = M^-1 P over the active cells a: M = K + c 1_a 1_a^T (SPD on a
-// connected region; c = mean active diagonal / n_a) by Cholesky on the host,
-// P = I - 1_a 1_a^T / n_a.  For any b, x = G b solves K x = P b with
-// mean_a(x) = 0.  Inactive rows and columns of G are 0.
+// G = M^-1 P over the active cells, region by region: the active cells
+// split into regions a_r connected through open faces (one region unless the
+// obstacles cut the fluid apart at this level), M = K + sum_r c_r 1_r 1_r^T
+// (SPD: K is block-diagonal over the regions and each rank-one term lifts
+// its region's constant null vector; c_r = mean diagonal of a_r / n_r) by
+// Cholesky on the host, P = I - sum_r 1_r 1_r^T / n_r.  For any b, x = G b
+// solves K x = P b with every region's mean of x zero.  Inactive rows and
+// columns of G are 0.
 static bool dense_coarse_inverse(const LevelFaces& F, std::vector<double>& G) {
   const MgLevel& L = F.L;
   const int nx = L.nx, ny = L.ny, n = L.nx * L.ny * L.nz;
   std::vector<double> A((size_t)n * n, 0.0);
-  double dsum = 0.0;
-  int na = 0;
   const int off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
   for (int c = 0; c < n; ++c) {
     if (!F.active(c)) {
       A[(size_t)c * n + c] = 1.0;
       continue;
     }
-    ++na;
     for (int s = 0; s < 6; ++s) {
       const double w = F.weight(c, s);
       if (w == 0.0) continue;
       A[(size_t)c * n + c] += w;
       A[(size_t)c * n + c + off[s]] -= w;
     }
-    dsum += A[(size_t)c * n + c];
   }
-  if (na == 0) return false;
-  const double shift = dsum > 0.0 ? dsum / na / na : 1.0;
+  // regions of the active cells (breadth-first through open faces)
+  std::vector<int> region(n, -1), queue;
+  int regions = 0;
+  for (int seed = 0; seed < n; ++seed) {
+    if (!F.active(seed) || region[seed] >= 0) continue;
+    region[seed] = regions;
+    queue.assign(1, seed);
+    for (size_t h = 0; h < queue.size(); ++h) {
+      const int c = queue[h];
+      for (int s = 0; s < 6; ++s)
+        if (F.weight(c, s) != 0.0 && region[c + off[s]] < 0) {
+          region[c + off[s]] = regions;
+          queue.push_back(c + off[s]);
+        }
+    }
+    ++regions;
+  }
+  if (regions == 0) return false;
+  std::vector<double> dsum(regions, 0.0);
+  std::vector<int> na(regions, 0);
+  for (int c = 0; c < n; ++c)
+    if (region[c] >= 0) {
+      dsum[region[c]] += A[(size_t)c * n + c];
+      ++na[region[c]];
+    }
   for (int r = 0; r < n; ++r)
-    if (F.active(r))
+    if (region[r] >= 0) {
+      const int k = region[r];
+      const double shift = dsum[k] > 0.0 ? dsum[k] / na[k] / na[k] : 1.0;
       for (int c = 0; c < n; ++c)
-        if (F.active(c)) A[(size_t)r * n + c] += shift;
+        if (region[c] == k) A[(size_t)r * n + c] += shift;
+    }
   // Cholesky A = L L^T (lower, in place)
   for (int j = 0; j < n; ++j) {
     double d = A[(size_t)j * n + j];
@@ -529,17 +634,19 @@ static bool dense_coarse_inverse(const LevelFaces& F, std::vector<double>& G) {
     }
     for (int i = 0; i < n; ++i) inv[(size_t)col * n + i] = y[i];
   }
-  // G = M^-1 P: subtract the mean of the active columns from each active column
-  std::vector<double> mean_col(n, 0.0);
+  // G = M^-1 P: from each active column subtract the mean of its region's
+  // columns
+  std::vector<double> mean_col((size_t)regions * n, 0.0);
   for (int col = 0; col < n; ++col)
-    if (F.active(col))
-      for (int i = 0; i < n; ++i) mean_col[i] += inv[(size_t)col * n + i];
-  for (int i = 0; i < n; ++i) mean_col[i] /= na;
+    if (region[col] >= 0)
+      for (int i = 0; i < n; ++i) mean_col[(size_t)region[col] * n + i] += inv[(size_t)col * n + i];
+  for (int k = 0; k < regions; ++k)
+    for (int i = 0; i < n; ++i) mean_col[(size_t)k * n + i] /= na[k];
   G.assign((size_t)n * n, 0.0);
   for (int col = 0; col < n; ++col) {
-    if (!F.active(col)) continue;
+    if (region[col] < 0) continue;
     for (int i = 0; i < n; ++i)
-      if (F.active(i)) G[(size_t)col * n + i] = inv[(size_t)col * n + i] - mean_col[i];
+      if (F.active(i)) G[(size_t)col * n + i] = inv[(size_t)col * n + i] - mean_col[(size_t)region[col] * n + i];
   }
   return true;
 }

@@ -237,15 +237,20 @@ def test_gpu_mg_with_sphere_converges_to_oracle(n, prec):
 
 
 @pytest.mark.gpu
-def test_gpu_disconnected_fluid_is_rejected():
+def test_gpu_more_than_254_fluid_regions_are_rejected():
+    """Split fluid is solved region by region (tests/test_components.py); the
+    region labels are one byte, so at most 254 regions are accepted."""
     import rheon_amd as R
-    spec = GridSpec.unit_box(8)
-    solid = np.zeros((8, 8, 8), np.uint8)
-    solid[:, :, 4] = 1  # a wall across x splits the box in two
+    spec = GridSpec.unit_box(16)
+    solid = np.ones((16, 16, 16), np.uint8)
+    solid[::2, ::2, ::2] = 0  # 512 isolated fluid cells
     ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32)
     with pytest.raises(R.RheonError) as e:
         ws.set_solid(solid.ravel())
     assert e.value.status == R.UNSUPPORTED
+    solid = np.ones((16, 16, 16), np.uint8)
+    solid[::4, ::4, ::4] = 0  # 64 regions: accepted
+    ws.set_solid(solid.ravel())
 
 
 @pytest.mark.gpu
