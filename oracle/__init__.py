@@ -33,6 +33,7 @@ CANCELLED = 6
 INVALID_ARGUMENT = 7
 INVALID_SETTINGS = 8
 INCOMPATIBLE_RHS = 9
+INVALID_INITIAL_PRESSURE = 24
 BREAKDOWN = 10
 ITERATION_LIMIT = 11
 NONFINITE_INPUT = 12
@@ -128,6 +129,9 @@ def _declare(L):
         getattr(L, name).argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double)]
     L.or_precondition_jacobi.argtypes = [_G, C.c_double, _P, _P, C.POINTER(C.c_double), C.c_int]
     L.or_solve_rhs.argtypes = [_G, C.c_double, _P, C.c_double, C.POINTER(_Settings), C.c_int, C.c_int, _P, C.POINTER(_Report)]
+    L.or_solve_rhs_initial.argtypes = [_G, C.c_double, _P, _P, C.c_double, C.POINTER(_Settings), C.c_int, C.c_int, _P,
+                                       C.POINTER(_Report)]
+    L.or_apply_acceleration.argtypes = [_G, C.c_double, C.POINTER(C.c_double * 3), C.c_void_p, C.c_void_p, _P, _P, _P]
     L.or_obstacle_mask.argtypes = [_G, _P, _P, C.POINTER(C.c_int64)]
     L.or_obstacle_components.argtypes = [_G, _P, _P]
     L.or_obstacle_components.restype = C.c_int32
@@ -345,8 +349,9 @@ class SolveResult:
 
 
 def solve_rhs(grid: Grid, density: float, rhs: np.ndarray, dt: float, settings: Settings,
-              method: str = "jacobi-pcg-v1", threads: int = 1) -> SolveResult:
-    """PressureWorkspace::solve_rhs (src/pressure.rs:365-393 -> :577-740).
+              method: str = "jacobi-pcg-v1", threads: int = 1, initial: np.ndarray | None = None) -> SolveResult:
+    """PressureWorkspace::solve_rhs (src/pressure.rs:365-393 -> :577-740), or
+    solve_rhs_with_initial_pressure (:413-429) when ``initial`` is given.
 
     Returns the status instead of raising; ``pressure`` holds the last iterate
     (also on IterationLimit, as the reference's workspace does)."""
@@ -355,10 +360,28 @@ def solve_rhs(grid: Grid, density: float, rhs: np.ndarray, dt: float, settings: 
     s = _Settings(settings.relative_residual, settings.absolute_residual, settings.divergence_limit,
                   int(settings.max_iterations))
     rep = _Report()
-    st = lib().or_solve_rhs(C.byref(grid.s), density, _ptr(rhs), dt, C.byref(s), METHODS[method], threads,
-                            _ptr(p), C.byref(rep))
+    if initial is None:
+        st = lib().or_solve_rhs(C.byref(grid.s), density, _ptr(rhs), dt, C.byref(s), METHODS[method], threads,
+                                _ptr(p), C.byref(rep))
+    else:
+        initial = np.ascontiguousarray(initial, np.float64)
+        assert initial.size == grid.cell_len
+        st = lib().or_solve_rhs_initial(C.byref(grid.s), density, _ptr(rhs), _ptr(initial), dt, C.byref(s),
+                                        METHODS[method], threads, _ptr(p), C.byref(rep))
     return SolveResult(st, p, rep.iterations, rep.true_residual_l2, rep.true_residual_max,
                        rep.predicted_divergence_max, rep.error_iteration, rep.error_value)
+
+
+def apply_acceleration(grid: Grid, vel, dt: float, value, lower=None, upper=None):
+    """Copies of f32 faces with one body force applied (src/forces.rs:97-205)."""
+    outs = [np.array(x, np.float32, copy=True) for x in vel]
+    val = (C.c_double * 3)(*value)
+    lo = (C.c_double * 3)(*lower) if lower is not None else None
+    hi = (C.c_double * 3)(*upper) if upper is not None else None
+    st = lib().or_apply_acceleration(C.byref(grid.s), dt, val, lo, hi, _ptr(outs[0]), _ptr(outs[1]), _ptr(outs[2]))
+    if st != OK:
+        raise OracleError(st, "apply_acceleration")
+    return outs
 
 
 def solve_velocity(grid: Grid, density: float, vel, dt: float, settings: Settings,

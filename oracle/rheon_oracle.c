@@ -804,7 +804,7 @@ static int precondition(const or_grid* g, const op_coeff* c, int method, double*
   }
 }
 
-static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, double dt,
+static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, const double* initial, double dt,
                    const or_settings* st, int method, int threads, double* p, or_report* rep) {
   const int64_t n = g->cell_len;
   if (!finite_cells(rhs, n)) return OR_NONFINITE_INPUT;
@@ -843,7 +843,14 @@ static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, doubl
       }
     }
   }
-  for (int64_t i = 0; i < n; ++i) p[i] = 0.0;
+  if (initial) {
+    /* caller-supplied sealed pressure guess (src/pressure.rs:615-623) */
+    if (!finite_cells(initial, n)) return OR_NONFINITE_INPUT;
+    if (initial[c->gauge] != 0.0) return OR_INVALID_INITIAL_PRESSURE;
+    for (int64_t i = 0; i < n; ++i) p[i] = initial[i];
+  } else {
+    for (int64_t i = 0; i < n; ++i) p[i] = 0.0;
+  }
   double rhs_l2 = sqrt(or_dot(rhs, rhs, n));
   double threshold = maxd(st->absolute_residual, st->relative_residual * rhs_l2);
   if (!isfinite(rhs_l2) || !isfinite(threshold)) return OR_BREAKDOWN;
@@ -917,7 +924,16 @@ static int solve_c(const or_grid* g, const op_coeff* c, const double* rhs, doubl
       }
       *rep = report;
       if (ACCEPTED(report)) goto done;
-      if (report.true_residual_l2 > threshold) {
+      /* a seeded solve decides the replacement on the residual without the
+       * eliminated gauge row (src/pressure.rs:689-707) */
+      double replacement_l2 = report.true_residual_l2;
+      if (initial) {
+        double sum = 0.0, corr = 0.0;
+        for (int64_t i = 0; i < n; ++i)
+          if (!(i == gauge || (multi && is_gauge(c, i)))) neumaier_add(&sum, &corr, w.q[i] * w.q[i]);
+        replacement_l2 = sqrt(sum + corr);
+      }
+      if (replacement_l2 > threshold) {
         memcpy(w.r, w.q, sizeof(double) * n);
         replaced = 1;
       }
@@ -961,7 +977,44 @@ int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
   op_coeff c;
   int s = op_init(g, density, &c);
   if (s) return s;
-  return solve_c(g, &c, rhs, dt, st, method, threads, p, rep);
+  return solve_c(g, &c, rhs, NULL, dt, st, method, threads, p, rep);
+}
+
+int or_solve_rhs_initial(const or_grid* g, double density, const double* rhs, const double* initial,
+                         double dt, const or_settings* st, int method, int threads, double* p,
+                         or_report* rep) {
+  memset(rep, 0, sizeof(*rep));
+  op_coeff c;
+  int s = op_init(g, density, &c);
+  if (s) return s;
+  return solve_c(g, &c, rhs, initial, dt, st, method, threads, p, rep);
+}
+
+int or_apply_acceleration(const or_grid* g, double dt, const double value[3], const double* lower,
+                          const double* upper, float* u, float* v, float* w) {
+  float* vel[3] = {u, v, w};
+  for (int d = 0; d < 3; ++d) {
+    const int64_t* dims = g->fn[d];
+    for (int64_t k = 0; k < dims[2]; ++k)
+      for (int64_t j = 0; j < dims[1]; ++j)
+        for (int64_t i = 0; i < dims[0]; ++i) {
+          const int64_t q[3] = {i, j, k};
+          if (q[d] == 0 || q[d] == g->n[d]) continue;
+          double pos[3];
+          for (int e = 0; e < 3; ++e) pos[e] = g->origin[e] + ((double)q[e] + (e == d ? 0.0 : 0.5)) * g->h[e];
+          int inside = 1;
+          if (lower)
+            for (int e = 0; e < 3; ++e) inside = inside && pos[e] >= lower[e] && pos[e] < upper[e];
+          double a = 0.0;
+          if (inside) a += value[d];
+          const int64_t index = face_index(g, d, i, j, k);
+          const double old = (double)vel[d][index];
+          const double updated = old + dt * a;
+          if (!isfinite(updated) || fabs(updated) > (double)FLT_MAX) return OR_ARITHMETIC_FAILURE;
+          vel[d][index] = (float)updated;
+        }
+  }
+  return OR_OK;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1087,7 +1140,7 @@ int or_solve_rhs_masked(const or_grid* g, double density, const uint8_t* mask, i
   if (method == OR_SGS) return OR_INVALID_ARGUMENT;
   int32_t* labels = NULL;
   if ((s = masked_coeff(g, mask, gauge, &c, &labels))) return s;
-  s = solve_c(g, &c, rhs, dt, st, method, threads, p, rep);
+  s = solve_c(g, &c, rhs, NULL, dt, st, method, threads, p, rep);
   free(labels);
   return s;
 }

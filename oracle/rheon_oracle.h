@@ -40,6 +40,7 @@ enum {
   OR_NONFINITE_POSITION = 17,
   OR_DIVERGENCE_LIMIT = 18,
   OR_GEOMETRY = 20,
+  OR_INVALID_INITIAL_PRESSURE = 24,
   OR_CANCELLED = 6
 };
 
@@ -140,6 +141,17 @@ int or_precondition_jacobi(const or_grid* g, double density, double* r, double* 
                            double* rz, int threads);
 /* PressureWorkspace::solve_rhs -> solve_loaded (src/pressure.rs:365-393, :577-740).
  * p (len N) receives the last iterate also on IterationLimit. */
+/* solve_rhs_with_initial_pressure (src/pressure.rs:413-429, :615-623,
+ * :689-707): the solve seeded with a caller-supplied pressure (finite, 0 at
+ * the gauge). */
+int or_solve_rhs_initial(const or_grid* g, double density, const double* rhs, const double* initial,
+                         double dt, const or_settings* s, int method, int threads, double* p,
+                         or_report* rep);
+/* One body force on the interior faces (src/forces.rs:97-205): u = f32(f64(u)
+ * + dt a(x)) with a(x) = value when the face position x lies in the half-open
+ * region [lower, upper) (lower = upper = NULL: the whole box), else 0. */
+int or_apply_acceleration(const or_grid* g, double dt, const double value[3], const double* lower,
+                          const double* upper, float* u, float* v, float* w);
 int or_solve_rhs(const or_grid* g, double density, const double* rhs, double dt,
                  const or_settings* s, int method, int threads, double* p, or_report* rep);
 /* PressureWorkspace::actual_divergence_max (src/pressure.rs:330-342). */
