@@ -66,7 +66,8 @@ typedef enum rh_status {
   RH_GEOMETRY = 20,            /* GeometryError::* */
   RH_BUFFER_LIMIT = 21,        /* PressureError::BufferLimit / GeometryError::BufferLimit */
   RH_DEVICE_ERROR = 22,        /* HIP runtime failure (message in rh_last_error) */
-  RH_REFUSED = 23              /* ProjectionError::Refused (the kind is in rh_projection_report.refusal) */
+  RH_REFUSED = 23,             /* ProjectionError::Refused (the kind is in rh_projection_report.refusal) */
+  RH_INVALID_INITIAL_PRESSURE = 24 /* PressureError::InvalidInitialPressure */
 } rh_status;
 
 /* GridGeometry::new inputs (src/geometry.rs:114-118). */
@@ -228,6 +229,18 @@ rh_status rh_solver_solve_rhs(rh_solver* solver, double density, const double* d
                               double dt, const rh_pressure_settings* settings,
                               double* d_pressure, rh_pressure_report* report,
                               rh_cancel_fn cancel, void* cancel_ctx);
+/* solve_rhs_with_initial_pressure (src/pressure.rs:413-429, feature
+ * accepted-pressure-warm-start): the solve started from d_initial (N f64,
+ * device; may be d_pressure itself), which must be finite (NONFINITE_INPUT)
+ * and 0 at the gauge (INVALID_INITIAL_PRESSURE).  A seeded jacobi-pcg-v1 /
+ * sgs-pcg-v1 solve replaces its recursive residual only when the true
+ * residual without the gauge row exceeds the threshold (:689-707); the
+ * multigrid methods start their defect correction from the seed. */
+rh_status rh_solver_solve_rhs_initial(rh_solver* solver, double density, const double* d_rhs,
+                                      const double* d_initial, double dt,
+                                      const rh_pressure_settings* settings, double* d_pressure,
+                                      rh_pressure_report* report, rh_cancel_fn cancel,
+                                      void* cancel_ctx);
 
 /* Solid obstacles for this solver: solid = one flag per cell (nonzero =
  * solid, host, len N) or NULL for the sealed box.  The discretisation of
@@ -474,6 +487,14 @@ rh_status rh_scene_step_requested(rh_scene* scene, double requested_dt, rh_step_
  * displaced support of its nonzero cells are exactly 0), rejects a value
  * outside [0, 1] or non-finite (src/simulation.rs:1322-1331) and commits it
  * with the faces. */
+/* Pressure initialization of every later step's solve
+ * (PressureInitializationPolicy, feature accepted-pressure-warm-start,
+ * src/liquid_step.rs:800-850): COLD starts from p = 0 (the default);
+ * ACCEPTED_PREVIOUS seeds the solve with the pressure of the last committed
+ * step (solve_rhs_with_initial_pressure), cold when there is none (a new
+ * velocity or obstacle set clears it).  Batched scenes: UNSUPPORTED. */
+typedef enum rh_init_policy { RH_INIT_COLD = 0, RH_INIT_ACCEPTED_PREVIOUS = 1 } rh_init_policy;
+rh_status rh_scene_set_initialization(rh_scene* scene, rh_init_policy policy);
 rh_status rh_scene_set_tracer(rh_scene* scene, const float* tracer, size_t len);
 rh_status rh_scene_get_tracer(const rh_scene* scene, float* out, size_t len);
 

@@ -44,8 +44,14 @@ GEOMETRY = 20
 BUFFER_LIMIT = 21
 DEVICE_ERROR = 22
 REFUSED = 23
+INVALID_INITIAL_PRESSURE = 24
+
 
 STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) and k.isupper()}
+
+# PressureInitializationPolicy (rh_init_policy)
+INIT_COLD = 0
+INIT_ACCEPTED_PREVIOUS = 1
 
 F64, F32 = 0, 1
 VARIANT_SCALAR_MG = 1
@@ -307,15 +313,29 @@ class PressureWorkspace:
         a = np.ascontiguousarray(solid, np.uint8).reshape(-1)
         _check(lib().rh_solver_set_solid(self.handle, a.ctypes.data, a.size), "rh_solver_set_solid")
 
-    def solve_rhs(self, density: float, rhs, dt: float, settings: PressureSettings, cancelled=None) -> PressureReport:
-        """PressureWorkspace::solve_rhs (src/pressure.rs:365-393); returns the
-        report with its status (pressure() holds the last iterate)."""
+    def solve_rhs(self, density: float, rhs, dt: float, settings: PressureSettings, cancelled=None,
+                  initial=None) -> PressureReport:
+        """PressureWorkspace::solve_rhs (src/pressure.rs:365-393), or
+        solve_rhs_with_initial_pressure (:413-429) when `initial` is given (a
+        host array, a DeviceArray, or "pressure" for this workspace's last
+        pressure); returns the report with its status (pressure() holds the
+        last iterate)."""
         drhs = rhs if isinstance(rhs, DeviceArray) else DeviceArray.from_host(np.ascontiguousarray(rhs, np.float64))
         rep = N.PressureReport()
         st = settings.native()
         cb, _keep = _cancel_cb(cancelled)
-        status = lib().rh_solver_solve_rhs(self.handle, density, drhs.ptr, dt, C.byref(st), self.pressure.ptr,
-                                           C.byref(rep), cb, None)
+        if initial is None:
+            status = lib().rh_solver_solve_rhs(self.handle, density, drhs.ptr, dt, C.byref(st), self.pressure.ptr,
+                                               C.byref(rep), cb, None)
+        else:
+            if isinstance(initial, str) and initial == "pressure":
+                seed = self.pressure
+            elif isinstance(initial, DeviceArray):
+                seed = initial
+            else:
+                seed = DeviceArray.from_host(np.ascontiguousarray(initial, np.float64))
+            status = lib().rh_solver_solve_rhs_initial(self.handle, density, drhs.ptr, seed.ptr, dt, C.byref(st),
+                                                       self.pressure.ptr, C.byref(rep), cb, None)
         return PressureReport.from_native(status, rep)
 
     def solve_velocity(self, density: float, vel, dt: float, settings: PressureSettings, cancelled=None):
@@ -543,6 +563,12 @@ class Scene:
         out = np.empty(self.spec.cell_len, np.float64)
         _check(lib().rh_scene_get_pressure(self.handle, out.ctypes.data, out.size), "rh_scene_get_pressure")
         return out
+
+    def set_initialization(self, policy: int) -> None:
+        """PressureInitializationPolicy of every later step's solve: INIT_COLD
+        (p = 0) or INIT_ACCEPTED_PREVIOUS (seeded with the last committed
+        step's pressure, feature accepted-pressure-warm-start)."""
+        _check(lib().rh_scene_set_initialization(self.handle, policy), "rh_scene_set_initialization")
 
     def set_tracer(self, tracer) -> None:
         """Passive tracer (one f32 per cell) advected and committed with every

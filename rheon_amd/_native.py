@@ -301,6 +301,8 @@ SIGNATURES = {
     "rh_solver_set_solid": (_S, [_P, _P, C.c_size_t]),
     "rh_solver_solve_rhs": (_S, [_P, C.c_double, _P, C.c_double, C.POINTER(PressureSettings), _P,
                                  C.POINTER(PressureReport), CANCEL_FN, _P]),
+    "rh_solver_solve_rhs_initial": (_S, [_P, C.c_double, _P, _P, C.c_double, C.POINTER(PressureSettings), _P,
+                                         C.POINTER(PressureReport), CANCEL_FN, _P]),
     "rh_provider_create": (_S, [C.POINTER(ProviderConfig), C.POINTER(C.c_void_p)]),
     "rh_provider_destroy": (None, [_P]),
     "rh_provider_metadata": (_S, [_P, C.POINTER(SolverMetadata)]),
@@ -330,6 +332,7 @@ SIGNATURES = {
     "rh_projection_result": (_S, [_P, _P, C.c_size_t, _P, _P, _P]),
     "rh_scene_step_requested": (_S, [_P, C.c_double, C.POINTER(StepReport), CANCEL_FN, _P]),
     "rh_scene_set_tracer": (_S, [_P, _P, C.c_size_t]),
+    "rh_scene_set_initialization": (_S, [_P, C.c_int]),
     "rh_scene_get_tracer": (_S, [_P, _P, C.c_size_t]),
     "rh_batch_create": (_S, [C.POINTER(Grid), C.c_size_t, C.POINTER(BatchConfig), C.POINTER(C.c_void_p)]),
     "rh_batch_destroy": (None, [_P]),

@@ -355,6 +355,7 @@ rh_status Batch::create(const rh_grid* grids, size_t n, const rh_batch_config& c
     if (s) return s;
     owned_.emplace_back(raw);
     raw->batched = true;
+    raw->p_state = 0;
     scenes_.push_back(raw);
     if (mg(i)->levels() < 2) {
       set_last_error("a batched scene needs at least two multigrid levels", "rh_batch_create");
@@ -403,6 +404,8 @@ rh_status Batch::set_members(const std::vector<rh_scene*>& scenes, const std::ve
       return RH_UNSUPPORTED;
     }
     s->batched = true;
+    s->p_state = 0;
+    s->init_policy = RH_INIT_COLD;
   }
   if (scenes != scenes_ || dt != dt_) {
     built_ = false;

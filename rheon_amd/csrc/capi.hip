@@ -394,6 +394,18 @@ extern "C" rh_status rh_solver_solve_rhs(rh_solver* solver, double density, cons
   return solver->impl->solve(density, d_rhs, dt, *settings, d_pressure, report, cancel, cancel_ctx);
 }
 
+extern "C" rh_status rh_solver_solve_rhs_initial(rh_solver* solver, double density, const double* d_rhs,
+                                                 const double* d_initial, double dt,
+                                                 const rh_pressure_settings* settings, double* d_pressure,
+                                                 rh_pressure_report* report, rh_cancel_fn cancel,
+                                                 void* cancel_ctx) {
+  if (!solver || !settings || !d_initial) return RH_INVALID_ARGUMENT;
+  solver->impl->set_seed(d_initial);
+  const rh_status s = solver->impl->solve(density, d_rhs, dt, *settings, d_pressure, report, cancel, cancel_ctx);
+  solver->impl->set_seed(nullptr);
+  return s;
+}
+
 extern "C" rh_status rh_solver_set_solid(rh_solver* solver, const uint8_t* solid, size_t len) {
   if (!solver) return RH_INVALID_ARGUMENT;
   if (solid && len != cells_of(solver->impl->grid())) return RH_INVALID_ARGUMENT;
@@ -733,6 +745,7 @@ extern "C" rh_status rh_scene_set_velocity(rh_scene* sc, const void* u, const vo
   const size_t fb = prec_bytes(sc->config.precision);
   sc->acc_checked = false;
   sc->rate_known = false;
+  sc->p_state = 0;  // a new state: no accepted pressure yet
   for (int a = 0; a < 3; ++a)
     RH_HIP(hipMemcpy(sc->acc[a], src[a], sc->flen[a] * fb, hipMemcpyHostToDevice));
   return zero_blocked(sc);
@@ -743,6 +756,7 @@ extern "C" rh_status rh_scene_set_solid(rh_scene* sc, const uint8_t* solid, size
   if (solid && len != sc->n) return RH_INVALID_ARGUMENT;
   sc->acc_checked = false;
   sc->rate_known = false;
+  sc->p_state = 0;
   rh_status s = sc->solver->impl->set_solid(solid);
   if (s) return s;
   return zero_blocked(sc);
@@ -777,9 +791,31 @@ extern "C" rh_status rh_scene_device_velocity(rh_scene* sc, void** d_u, void** d
 // advect -> build_rhs -> pressure solve -> correct_candidate_in_place ->
 // actual divergence gate -> Courant gate -> tracer (advect_tracer with the
 // candidate velocity, range check, integral) -> commit swap (:805-811).
+static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_cancel_fn cancel,
+                            void* cancel_ctx);
+
 extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* report,
                                    rh_cancel_fn cancel, void* cancel_ctx) {
   if (!sc) return RH_INVALID_ARGUMENT;
+  sc->p_written = false;
+  const rh_status s = scene_step(sc, dt, report, cancel, cancel_ctx);
+  if (s == RH_OK)
+    sc->p_state = 1;  // p holds the accepted pressure
+  else if (sc->p_state == 1 && sc->p_written)
+    sc->p_state = 0;  // the failed attempt's solve overwrote it
+  return s;
+}
+
+extern "C" rh_status rh_scene_set_initialization(rh_scene* sc, rh_init_policy policy) {
+  if (!sc) return RH_INVALID_ARGUMENT;
+  if (policy != RH_INIT_COLD && policy != RH_INIT_ACCEPTED_PREVIOUS) return RH_INVALID_ARGUMENT;
+  if (sc->batched && policy != RH_INIT_COLD) return RH_UNSUPPORTED;
+  sc->init_policy = policy;
+  return RH_OK;
+}
+
+static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_cancel_fn cancel,
+                            void* cancel_ctx) {
   rh_step_report local;
   rh_step_report* rep = report ? report : &local;
   std::memset(rep, 0, sizeof(*rep));
@@ -818,8 +854,24 @@ extern "C" rh_status rh_scene_step(rh_scene* sc, double dt, rh_step_report* repo
   unsigned f = 0;
   if ((s = scr.read_flags(&f))) return s;
   if (f) return rh::flags_to_status(f);
-  s = rh_solver_solve_rhs(sc->solver.get(), sc->config.density, sc->rhs, dt, &sc->config.pressure,
-                          sc->p, &rep->pressure, cancel, cancel_ctx);
+  // AcceptedPreviousV1: seed with the last committed step's pressure, kept
+  // aside first so that a failed attempt does not lose it.
+  const double* seed = nullptr;
+  if (sc->init_policy == RH_INIT_ACCEPTED_PREVIOUS && sc->p_state != 0) {
+    if (!sc->p_acc) RH_HIP(hipMalloc(&sc->p_acc, sc->n * sizeof(double)));
+    if (sc->p_state == 1) {
+      RH_HIP(hipMemcpyAsync(sc->p_acc, sc->p, sc->n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      sc->p_state = 2;
+    }
+    seed = sc->p_acc;
+  }
+  sc->p_written = true;
+  if (seed)
+    s = rh_solver_solve_rhs_initial(sc->solver.get(), sc->config.density, sc->rhs, seed, dt, &sc->config.pressure,
+                                    sc->p, &rep->pressure, cancel, cancel_ctx);
+  else
+    s = rh_solver_solve_rhs(sc->solver.get(), sc->config.density, sc->rhs, dt, &sc->config.pressure, sc->p,
+                            &rep->pressure, cancel, cancel_ctx);
   if (s) return s;
   RH_HIP(hipEventRecord(sc->ev[3], st));
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {

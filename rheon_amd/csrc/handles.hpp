@@ -41,11 +41,23 @@ struct rh_scene {
   bool rate_known = false;
   // Stepped by a batch (rh_batch / rh_server), which carries no tracer.
   bool batched = false;
+  // Pressure initialization of the solve (rh_scene_set_initialization) and
+  // where the accepted pressure of the last committed step is: 0 none, 1 in
+  // p (the last solve's output), 2 in p_acc (kept aside by a warm step, so a
+  // failed attempt does not lose it).  p_acc is allocated by the first warm
+  // step.  A failed attempt keeps state 1 unless its solve wrote p
+  // (p_written); a cold scene whose attempt fails after its solve has no
+  // accepted pressure left to seed a later warm step with.
+  bool p_written = false;
+  int init_policy = RH_INIT_COLD;
+  int p_state = 0;
+  double* p_acc = nullptr;
   ~rh_scene() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (arena) (void)hipFree(arena);
     if (tracer[0]) (void)hipFree(tracer[0]);
+    if (p_acc) (void)hipFree(p_acc);
   }
 };
 

@@ -84,7 +84,7 @@ struct PcgControl {
   int error_iteration;
   int candidate;       // this iteration needs a true report
   int replaced;        // this iteration replaced r by the true residual
-  int pad;
+  int seeded;          // started from a caller-supplied pressure
   double sum_r;        // sum of r (multigrid gauge shift)
   double spare[7];
 };
@@ -124,6 +124,7 @@ struct SgsTile {
 constexpr int kSgsTileEdge = 8;
 
 struct PcgArrays {
+  const double* seed = nullptr;  // caller-supplied starting pressure (none: p = 0)
   void* p;       // f64 solution (= p64)
   void* r;       // T
   void* d;       // T
@@ -159,6 +160,9 @@ void sgs_plan(const DevGrid& g, std::vector<SgsTile>& tiles, std::vector<int>& w
 void launch_residual_report(const DevGrid& g, const double* p, const double* b, const Reduce& red,
                             hipStream_t s);
 void launch_comp_check(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx, hipStream_t s);
+// p = seed with the seed's checks (non-finite, nonzero gauge) into ctl.
+void launch_seed(const DevGrid& g, const double* seed, double* p, PcgControl* ctl, DD* pdd, double* pmx,
+                 hipStream_t s);
 void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
                      hipStream_t s);
 

@@ -99,7 +99,10 @@ struct FinTrue {
       ctl->status = RH_OK;
       return;
     }
-    if (it > 0 && l2 > ctl->threshold) ctl->replaced = 1;
+    // a seeded solve replaces on the residual without the eliminated gauge
+    // rows (src/pressure.rs:689-707)
+    const double replacement_l2 = ctl->seeded ? sqrt(sum[1]) : l2;
+    if (it > 0 && replacement_l2 > ctl->threshold) ctl->replaced = 1;
     if (it == ctl->max_iterations) {
       ctl->error_value = maximum;
       ctl->iteration = it;
@@ -187,6 +190,44 @@ __device__ __forceinline__ void b_rhs_sums(VBlock vb, const DevGrid& g, const do
 __global__ void __launch_bounds__(kBlock)
 k_rhs_sums(DevGrid g, const double* __restrict__ b, DD* pdd, double* pmx) {
   b_rhs_sums(hw_block(), g, b, pdd, pmx);
+}
+
+// A caller-supplied pressure guess (src/pressure.rs:615-623): p = seed,
+// every value finite (OperatorError::NonFiniteInput) and 0 at the gauge
+// (PressureError::InvalidInitialPressure; with a split fluid at every
+// region's gauge).  One pass copies and flags; the finisher decides.
+__global__ void __launch_bounds__(kBlock)
+k_seed_copy(DevGrid g, const double* seed, double* p, const PcgControl* ctl, DD* pdd,  // seed may be p
+            double* pmx) {
+  if (ctl->done) return;
+  DD none[1] = {{0.0, 0.0}};
+  double bad[1] = {0.0};
+  FOR_CELLS(g)
+  const double v = seed[c];
+  if (!isfinite(v)) bad[0] = 1.0;
+  p[c] = v;
+  END_CELLS
+  block_publish<1, 1>(none, bad, pdd, pmx);
+}
+struct FinSeed {
+  PcgControl* ctl;
+  const double* seed;
+  int gauge, ncomp;
+  const int* comp_gauge;
+  __device__ bool active() const { return !ctl->done; }
+  __device__ void operator()(const double*, const double* mx) const {
+    if (mx[0] != 0.0) return fail(ctl, RH_NONFINITE_INPUT, 0);
+    bool zero = seed[gauge] == 0.0;
+    if (ncomp > 1)
+      for (int k = 0; k < ncomp; ++k) zero = zero && seed[comp_gauge[k]] == 0.0;
+    if (!zero) return fail(ctl, RH_INVALID_INITIAL_PRESSURE, 0);
+  }
+};
+void launch_seed(const DevGrid& g, const double* seed, double* p, PcgControl* ctl, DD* pdd, double* pmx,
+                 hipStream_t s) {
+  const dim3 grid = cells_grid(g);
+  k_seed_copy<<<grid, block_shape(), 0, s>>>(g, seed, p, ctl, pdd, pmx);
+  launch_finish<1, 1>(pdd, pmx, grid.x, FinSeed{ctl, seed, g.gauge, g.ncomp, g.comp_gauge}, s);
 }
 
 // Compatibility of a split fluid (src/obstacle_pressure.rs:455-472): every
@@ -312,7 +353,7 @@ k_true_residual(DevGrid g, const double* __restrict__ p, const double* __restric
                 T* __restrict__ out, const PcgControl* ctl, DD* pdd, double* pmx, int it) {
   if (ctl->done) return;
   if (it > 0 && !ctl->candidate) return;
-  DD s[1] = {{0.0, 0.0}};
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
   double m[2] = {0.0, 0.0};
   FOR_CELLS(g)
   const double v = apply_at<double>(g, p, c, i, j, k);
@@ -320,9 +361,10 @@ k_true_residual(DevGrid g, const double* __restrict__ p, const double* __restric
   const double prod = b[c] - v;
   out[c] = (T)prod;
   dd_acc(s[0], prod * prod);
+  if (!is_gauge(g, c)) dd_acc(s[1], prod * prod);  // without the gauge rows
   m[0] = fmax(m[0], fabs(prod));
   END_CELLS
-  block_publish<1, 2>(s, m, pdd, pmx);
+  block_publish<2, 2>(s, m, pdd, pmx);
 }
 
 // Replacement r = product (r[0] = 0) and the first preconditioner stage.
@@ -631,7 +673,7 @@ k_direction_j(DevGrid g, const double* __restrict__ p, const double* __restrict_
     return;
   }
   const double beta = next_rz / ctl->rz[it & 1];
-  DD s[1] = {{0.0, 0.0}};
+  DD s[2] = {{0.0, 0.0}, {0.0, 0.0}};
   double m[2] = {0.0, 0.0};
   FOR_CELLS(g)
   if (cand) {
@@ -640,6 +682,7 @@ k_direction_j(DevGrid g, const double* __restrict__ p, const double* __restrict_
     const double pr = bvec[c] - v;
     prod[c] = (T)pr;
     dd_acc(s[0], pr * pr);
+    if (!is_gauge(g, c)) dd_acc(s[1], pr * pr);  // without the gauge rows
     m[0] = fmax(m[0], fabs(pr));
   }
   if (is_gauge(g, c)) {
@@ -649,7 +692,7 @@ k_direction_j(DevGrid g, const double* __restrict__ p, const double* __restrict_
   const double zv = jdiv(ld(r, c), diag_at(g, c, i, j, k));
   d[c] = (T)(zv + beta * ld(d, c));
   END_CELLS
-  if (cand) block_publish<1, 2>(s, m, pdd, pmx);
+  if (cand) block_publish<2, 2>(s, m, pdd, pmx);
   if (!cand && blockIdx.x == 0 && flat_tid() == 0) ctl->iteration = it;
 }
 
@@ -774,9 +817,12 @@ void pcg_prologue(const DevGrid& g, const PcgArrays& a, PcgControl* ctl, DD* pdd
   k_rhs_sums<<<grid, block_shape(), 0, s>>>(g, a.b, pdd, pmx);
   launch_finish<3, 1>(pdd, pmx, grid.x, FinRhsSums{ctl}, s);
   launch_comp_check(g, a.b, ctl, pdd, pmx, s);
-  k_fill_zero<double><<<fill_blocks, block_shape(), 0, s>>>(a.p64, n);
+  if (a.seed)
+    launch_seed(g, a.seed, a.p64, ctl, pdd, pmx, s);
+  else
+    k_fill_zero<double><<<fill_blocks, block_shape(), 0, s>>>(a.p64, n);
   launch(k_true_residual<T>, grid, block_shape(), 0, s, g, a.p64, a.b, (T*)a.q, ctl, pdd, pmx, 0);
-  launch_finish<1, 2>(pdd, pmx, grid.x, FinTrue{ctl, 0}, s);
+  launch_finish<2, 2>(pdd, pmx, grid.x, FinTrue{ctl, 0}, s);
 }
 
 template <typename T>
@@ -809,7 +855,7 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
       prof::Scope ps(prof::kDirection, s, it);
       launch(k_direction_j<T>, grid, block, 0, s, g, p, a.b, r, d, z, ctl, pdd, pmx, it);
     }
-    launch_finish<1, 2>(pdd, pmx, nb, FinTrueJ{ctl, it}, s);
+    launch_finish<2, 2>(pdd, pmx, nb, FinTrueJ{ctl, it}, s);
     return;
   }
   if (it > 0) {
@@ -844,7 +890,7 @@ void pcg_iteration(const DevGrid& g, int method, const PcgArrays& a, PcgControl*
       prof::Scope ps(prof::kTrueResidual, s, it);
       launch(k_true_residual<T>, grid, block, 0, s, g, p, a.b, q, ctl, pdd, pmx, it);
     }
-    launch_finish<1, 2>(pdd, pmx, nb, FinTrue{ctl, it}, s);
+    launch_finish<2, 2>(pdd, pmx, nb, FinTrue{ctl, it}, s);
   }
   if (sgs) {
     {

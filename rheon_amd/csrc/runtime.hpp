@@ -76,6 +76,8 @@ class Solver {
   const unsigned char* device_mask() const { return d_mask_; }
   int gauge() const { return gauge_; }
   int components() const { return ncomp_; }
+  // The next solve starts from this device pressure (null: from p = 0).
+  void set_seed(const double* seed) { seed_ = seed; }
   // Operator constants with this solver's obstacle mask and gauge.
   DevGrid dev_grid(double density) const;
   // Changes whenever device-side operator data is replaced (obstacles,
@@ -116,6 +118,7 @@ class Solver {
   const CompChunk* comp_chunks_ = nullptr;
   const int* comp_chunk_first_ = nullptr;
   std::vector<int> gauges_;  // host copy of the gauge cells
+  const double* seed_ = nullptr;
   uint64_t generation_ = 0;
 };
 

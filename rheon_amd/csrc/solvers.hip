@@ -336,8 +336,10 @@ rh_status PcgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressur
   a.tiles = d_tiles_;
   a.wave_start = wave_start_.data();
   a.waves = (int)wave_start_.size() - 1;
+  a.seed = seed_;
   PcgControl h;
   init_control(h, st, dt_, g.volume);
+  h.seeded = seed_ ? 1 : 0;
   std::memcpy(scratch_.host, &h, sizeof(h));
   RH_HIP(hipMemcpyAsync(scratch_.ctl, scratch_.host, sizeof(h), hipMemcpyHostToDevice, s));
   RH_HIP(hipEventRecord(scratch_.ev0, s));
@@ -942,8 +944,12 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   // Four-cell FP32 path: p = 0 is never read -- the first residual is b
   // and the first cycle's update writes p = 0 + x over every cell -- so p
   // is filled only if no cycle runs (below).
-  const bool p_zero = x0_implicit && mg_quad_update(h_, d_pressure);
-  if (!p_zero) RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
+  // A seed starts the defect correction from it (the first residual reads p).
+  const bool p_zero = !seed_ && x0_implicit && mg_quad_update(h_, d_pressure);
+  if (seed_)
+    launch_seed(g, seed_, d_pressure, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
+  else if (!p_zero)
+    RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
   mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s, x0_implicit,
                        p_zero);
   RH_HIP(hipGetLastError());

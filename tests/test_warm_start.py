@@ -300,3 +300,196 @@ def test_seeded_replacement_keeps_the_gauge_eliminated_recurrence():
         assert r.ok and r.iterations == want, (trial, r.iterations, want)
         checked += 1
     assert checked > 200
+
+
+# ---------------------------------------------------------------- GPU path
+
+
+def seeded_cases(count):
+    """The random seeded problems of the replacement-rule test above, in the
+    same order: (grid, rhs, seed, threshold)."""
+    rng = np.random.default_rng(0)
+    for trial in range(count):
+        n = int(rng.integers(3, 40))
+        dims = [(n, 1, 1), (n // 3 + 2, 3, 1), (4, 4, n // 8 + 2)][trial % 3]
+        g = O.Grid(dims, (1.0, 1.0, 1.0))
+        rhs = rng.standard_normal(g.cell_len)
+        rhs -= rhs.mean()
+        if trial % 2:
+            rhs[:] = 0.0
+        seed = (rng.standard_normal(g.cell_len) * 10 ** rng.uniform(-6, -2)
+                + rng.uniform(-1, 1) * 1e-4 * np.arange(g.cell_len))
+        seed[0] = 0.0
+        thr = 10 ** rng.uniform(-7, -3)
+        yield dims, g, rhs, seed, thr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,k", [("localized", 1), ("localized", 3), ("switch", 1), ("switch", 2),
+                                    ("switch", 3), ("steady", 1)])
+def test_gpu_seeded_solve_reproduces_the_published_warm_steps(mode, k):
+    """The device's seeded jacobi-pcg-v1 on the lab's warm step k (RHS and
+    seed from the oracle's replay): the published iteration count and
+    residuals, and the oracle's pressure bit for bit."""
+    import rheon_amd as R
+    s = list(oracle_lab(mode, True))[k]
+    ws = R.PressureWorkspace(s["spec"], "jacobi-pcg-v1", R.F64)
+    st = R.PressureSettings(1e-8, 1e-12, 1e-7, 3000)
+    rep = ws.solve_rhs(RHO, s["rhs"], s["dt"], st, initial=s["seed"])
+    want = PILOT[(mode, "warm")][k]
+    assert rep.ok, rep
+    assert rep.iterations == want[0]
+    assert (rep.true_residual_l2, rep.true_residual_max) == want[1:3]
+    assert np.array_equal(ws.get_pressure().view(np.uint64), s["pressure"].view(np.uint64))
+    ws.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["jacobi-pcg-v1", "sgs-pcg-v1", "rbsgs-pcg"])
+def test_gpu_seeded_solves_match_the_oracle_bitwise(method):
+    """The replacement-rule problems (chains, slabs and boxes with seeds
+    whose error sits mostly in the gauge row): status, iteration count and
+    pressure of the device equal the oracle's bit for bit."""
+    import rheon_amd as R
+    checked = 0
+    for trial, (dims, g, rhs, seed, thr) in enumerate(seeded_cases(90)):
+        st = (0.0, thr, 1e9, 60)
+        ref = O.solve_rhs(g, 1.0, rhs, 1.0, O.Settings(*st), method, initial=seed)
+        ws = R.PressureWorkspace(GridSpec(dims, (1.0, 1.0, 1.0)), method, R.F64)
+        rep = ws.solve_rhs(1.0, rhs, 1.0, R.PressureSettings(*st), initial=seed)
+        assert rep.status == ref.status, (trial, rep, ref.status)
+        assert rep.iterations == ref.iterations, (trial, rep.iterations, ref.iterations)
+        assert np.array_equal(ws.get_pressure().view(np.uint64), ref.pressure.view(np.uint64)), trial
+        checked += ref.ok
+        ws.close()
+    assert checked > 50
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,precision", [("jacobi-pcg-v1", 0), ("sgs-pcg-v1", 0), ("mixed-mg-dc", 1)])
+def test_gpu_seed_validation(method, precision):
+    """src/pressure.rs:615-623 on the device: a nonzero gauge entry is
+    InvalidInitialPressure, a non-finite entry NonFiniteInput; the exact
+    solution as a seed (also the output buffer itself) is accepted at once."""
+    import rheon_amd as R
+    first = next(oracle_lab("localized", False, n=16, steps=1))
+    spec = first["spec"]
+    ws = R.PressureWorkspace(spec, method, precision)
+    st = R.PressureSettings(1e-8, 1e-12, 1e-7, 3000)
+    seed = np.zeros(spec.cell_len)
+    seed[0] = 1e-3
+    assert ws.solve_rhs(RHO, first["rhs"], first["dt"], st, initial=seed).status == R.INVALID_INITIAL_PRESSURE
+    seed = np.zeros(spec.cell_len)
+    seed[9] = np.inf
+    assert ws.solve_rhs(RHO, first["rhs"], first["dt"], st, initial=seed).status == R.NONFINITE_INPUT
+    rep = ws.solve_rhs(RHO, first["rhs"], first["dt"], st)
+    assert rep.ok and rep.iterations > 0
+    again = ws.solve_rhs(RHO, first["rhs"], first["dt"], st, initial="pressure")
+    assert again.ok and again.iterations == 0
+    ws.close()
+
+
+def oracle_scene(spec, vel, dt, steps, warm, st):
+    """The scene step without forces: advect, RHS, jacobi-pcg-v1 (seeded
+    with the previous accepted pressure when warm), correction."""
+    g = O.Grid(spec.counts, spec.spacing)
+    prev, out = None, []
+    for _ in range(steps):
+        adv = O.advect_velocity(g, vel, dt, threads=8)
+        rhs = O.build_rhs(g, adv, dt)
+        res = O.solve_rhs(g, 1.0, rhs, dt, st, threads=8, initial=prev if warm else None)
+        assert res.ok
+        vel = O.correct(g, 1.0, res.pressure, dt, adv)
+        prev = res.pressure
+        out.append((res.iterations, res.pressure, vel))
+    return out
+
+
+@pytest.mark.gpu
+def test_gpu_scene_warm_steps_bitwise():
+    """Five scene steps under AcceptedPreviousV1: iteration counts, pressure
+    and faces equal the oracle's warm replay bit for bit.  Attempts that fail
+    (an invalid dt before the solve, a Courant refusal after it) leave the
+    accepted pressure in place for the next step, as a failed attempt leaves
+    the reference's accepted history (src/liquid_step.rs:924-991)."""
+    import rheon_amd as R
+    from helpers import config1
+    spec, vel, dt = config1(32, seed=2, dtype=np.float32)
+    dt *= 0.5  # the gate sums the three axes' rates: Courant 0.52 on the faces
+    st = O.Settings(1e-9, 1e-14, 1e-5, 3000)
+    ref = oracle_scene(spec, vel, dt, 5, True, st)
+    cold = oracle_scene(spec, vel, dt, 5, False, st)
+    # 317 iterations against 300 on the second step (the first step's
+    # pressure mostly removes the initial field's discrete divergence, a poor
+    # guess for the second), then 276, 273, 271 against 299
+    assert ref[1][0] > cold[1][0]
+    assert all(w[0] < c[0] for w, c in zip(ref[2:], cold[2:]))
+    sc = R.Scene(spec, 1.0, R.F32, "jacobi-pcg-v1", solver_precision=R.F64,
+                 settings=R.PressureSettings(1e-9, 1e-14, 1e-5, 3000), actual_divergence_limit=1e-3,
+                 max_courant=1.0)
+    sc.set_initialization(R.INIT_ACCEPTED_PREVIOUS)
+    sc.set_velocity(vel)
+    for k in range(5):
+        if k == 2:
+            assert sc.step(-1.0).status == R.INVALID_TIME_STEP
+        if k == 3:
+            assert sc.step(dt * 1e3).status in (R.COURANT_LIMIT, R.DIVERGENCE_LIMIT)
+        rep = sc.step(dt)
+        assert rep.status == R.OK, (k, rep)
+        assert rep.pressure.iterations == ref[k][0], k
+        assert np.array_equal(sc.get_pressure().view(np.uint64), ref[k][1].view(np.uint64)), k
+        got = sc.get_velocity()
+        for a in range(3):
+            assert np.array_equal(got[a].view(np.uint32), ref[k][2][a].view(np.uint32)), (k, a)
+    # new faces invalidate the accepted pressure: the next step is cold
+    sc.set_velocity(vel)
+    rep = sc.step(dt)
+    assert rep.pressure.iterations == cold[0][0]
+    assert np.array_equal(sc.get_pressure().view(np.uint64), cold[0][1].view(np.uint64))
+    sc.close()
+
+
+@pytest.mark.gpu
+def test_gpu_multigrid_warm_steps():
+    """Mixed-precision multigrid scenes around a sphere, cold and under
+    AcceptedPreviousV1.  Each warm step equals, bit for bit and cycle for
+    cycle, a workspace solve of that step's RHS (from the scene's accepted
+    faces, by the oracle's advection and RHS) seeded with the scene's
+    previous pressure.  The second step's seed is poor (13 + 14 cycles
+    against 13 + 12 cold, as for the jacobi scene); from the third step on
+    consecutive pressures are alike and the warm scene needs fewer cycles."""
+    import rheon_amd as R
+    from rheon_amd.scenes import centred_sphere, close_obstacle_faces
+    from helpers import config1
+    spec, vel, dt = config1(64, seed=3, dtype=np.float32)
+    g = O.Grid(spec.counts, spec.spacing)
+    solid = centred_sphere(spec, 0.2)
+    vel = close_obstacle_faces(spec, solid, vel)
+    st = R.PressureSettings(1e-10, 1e-14, 1e-3, 60)
+    scenes = []
+    for policy in (R.INIT_COLD, R.INIT_ACCEPTED_PREVIOUS):
+        sc = R.Scene(spec, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e-3)
+        sc.set_solid(solid)
+        sc.set_initialization(policy)
+        sc.set_velocity(vel)
+        scenes.append(sc)
+    ws = R.PressureWorkspace(spec, "mixed-mg-dc", R.F32)
+    ws.set_solid(solid)
+    cycles = [[], []]
+    for k in range(4):
+        faces = scenes[1].get_velocity()
+        prev = scenes[1].get_pressure()
+        for j, sc in enumerate(scenes):
+            rep = sc.step(dt)
+            assert rep.status == R.OK, (k, j, rep)
+            cycles[j].append(rep.pressure.iterations)
+        if k > 0:
+            rhs = O.build_rhs(g, close_obstacle_faces(spec, solid, O.advect_velocity(g, faces, dt, threads=8)), dt)
+            alone = ws.solve_rhs(1.0, rhs, dt, st, initial=prev)
+            assert alone.ok and alone.iterations == cycles[1][k], (k, alone)
+            assert np.array_equal(ws.get_pressure().view(np.uint64), scenes[1].get_pressure().view(np.uint64)), k
+    assert cycles[0][0] == cycles[1][0]
+    assert all(w < c for w, c in zip(cycles[1][2:], cycles[0][2:])), cycles
+    ws.close()
+    for sc in scenes:
+        sc.close()
