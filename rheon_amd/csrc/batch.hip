@@ -215,7 +215,7 @@ struct FinStatusOk {
 namespace {
 
 enum class Op {
-  kBeginStep, kValidateAcc, kAdvect, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kCompSums, kCompCheck, kFill,
+  kBeginStep, kValidateAcc, kAdvect, kAdvectLattice, kValidateCand, kRhs, kBeginSolve, kRhsSums, kFinRhs, kCompSums, kCompCheck, kFill,
   kOuterQ, kOuterS, kFinOuter, kSmoothQ, kSmoothQUpdate, kSmoothQFirst, kSmoothQFrac, kSmoothS, kRestrictQ, kRestrictQFrac, kRestrictS,
   kProlongQ, kProlongS, kTail, kCoarseDense, kEndSolve, kRegauge, kReport, kFinReport,
   kQuality, kCorrect, kDivMax, kFinDiv, kRate, kFinRate, kGate, kCommit
@@ -303,7 +303,6 @@ class Batch {
   hipGraphExec_t exec_ = nullptr;
   bool built_ = false;
   bool adv_off32_ = false;  // every scene's face arrays take 32-bit offsets
-  bool adv_lattice_ = false;  // some scene's grid is a lattice grid
   int max_cycles_ = 0;
   int launched_steps_ = 0;
   hipEvent_t ev_[2] = {nullptr, nullptr};
@@ -668,9 +667,14 @@ rh_status Batch::build() {
   stage(Op::kValidateAcc, acc.data(), sizeof(BFaceArgs), fb);
   adv_off32_ = true;
   for (size_t i = 0; i < n; ++i) adv_off32_ = adv_off32_ && face_offsets_32(grids[i], sizeof(float));
-  adv_lattice_ = false;
-  for (size_t i = 0; i < n; ++i) adv_lattice_ = adv_lattice_ || grids[i].lattice;
-  stage(Op::kAdvect, adv.data(), sizeof(BFaceArgs), fb);
+  // lattice grids take the cheaper first RK2 stage: one launch for them and
+  // one for the others, each kernel built for its own form
+  {
+    std::vector<int> fb_lat(n, 0), fb_gen(n, 0);
+    for (size_t i = 0; i < n; ++i) (grids[i].lattice ? fb_lat : fb_gen)[i] = fb[i];
+    stage(Op::kAdvectLattice, adv.data(), sizeof(BFaceArgs), fb_lat);
+    stage(Op::kAdvect, adv.data(), sizeof(BFaceArgs), fb_gen);
+  }
   stage(Op::kValidateCand, cand.data(), sizeof(BFaceArgs), fb);
   stage(Op::kRhs, rhs.data(), sizeof(BFaceArgs), cb);
   per_scene(Op::kBeginSolve);
@@ -939,7 +943,8 @@ void Batch::run_range(size_t begin, size_t end, hipStream_t s) {
       case Op::kGate: kb_gate<<<pb, 256, 0, s>>>(d_scenes_, n); break;
       case Op::kValidateAcc:
       case Op::kValidateCand: launch_b_validate(bl, s); break;
-      case Op::kAdvect: launch_b_advect(bl, s, adv_off32_, adv_lattice_); break;
+      case Op::kAdvect: launch_b_advect(bl, s, adv_off32_, false); break;
+      case Op::kAdvectLattice: launch_b_advect(bl, s, adv_off32_, true); break;
       case Op::kRhs: launch_b_rhs(bl, s); break;
       case Op::kRhsSums: launch_b_rhs_sums(bl, s); break;
       case Op::kFinRhs:

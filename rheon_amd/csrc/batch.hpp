@@ -39,7 +39,7 @@ struct BFaceArgs {  // validate / advect / rhs / correct / divergence / rate
 };
 
 void launch_b_validate(const BatchLaunch& bl, hipStream_t s);
-// lattice: some scene's grid is a lattice grid (DevGrid::lattice)
+// lattice: every scene of the launch has a lattice grid (DevGrid::lattice)
 void launch_b_advect(const BatchLaunch& bl, hipStream_t s, bool off32, bool lattice);
 void launch_b_rhs(const BatchLaunch& bl, hipStream_t s);
 void launch_b_correct(const BatchLaunch& bl, hipStream_t s);

@@ -92,8 +92,10 @@ struct DevGrid {
   // nullptr = the plain box.
   const unsigned char* mask;
   // Divisors of the four-cell chunk loops: nx / 4 chunks per row, ny rows
-  // per plane (set by make_dev_grid).
+  // per plane; of the flat face loops: nx and nx + 1 faces per row, ny + 1
+  // rows per plane (set by make_dev_grid).
   FastDiv div_q, div_y;
+  FastDiv div_x, div_x1, div_y1;
   // Obstacles that split the fluid into ncomp > 1 regions: one gauge per
   // region (kGaugeBit in the mask), the region label of every cell (255 =
   // solid), each region's gauge cell, a slot per region for the pressure
@@ -360,6 +362,38 @@ __device__ __forceinline__ RowRange block_span(int units, VBlock vb) {
   RowRange r;
   r.begin = (int)((long long)vb.b * units / vb.nb);
   r.end = (int)((long long)(vb.b + 1) * units / vb.nb);
+  return r;
+}
+
+// A copy of a record that no kernel writes (a batch's per-scene grid
+// record), read through the constant address space so that its fields are
+// loaded into scalar registers once per wave; read through a generic
+// pointer after stores, the compiler keeps them in vector registers (the
+// batched advection then ran at 2 or 3 waves per SIMD against 3 or 4 for
+// its one-scene form).  Only for kernels that do not index the record's
+// tables per lane: such an index would put the copy in scratch memory.
+template <typename R>
+__device__ __forceinline__ R load_record(const R* p) {
+  static_assert(sizeof(R) % 8 == 0 && alignof(R) >= 8, "records are copied in 8-byte words");
+  typedef const __attribute__((address_space(4))) unsigned long long* CWords;
+  const CWords src = (CWords)p;
+  R r;
+  unsigned long long* dst = reinterpret_cast<unsigned long long*>(&r);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(R) / 8); ++i) dst[i] = src[i];
+  return r;
+}
+
+// Contiguous partition of `units` into nb chunks of a whole number of
+// kBlock units (the last chunk short, trailing blocks possibly empty), so
+// that every pass of a block's threads is full except in the last chunk.
+__device__ __forceinline__ RowRange chunk_span(int units, VBlock vb) {
+  const long long per = ((long long)units + vb.nb - 1) / vb.nb;
+  const long long chunk = (per + kBlock - 1) / kBlock * kBlock;
+  const long long b = (long long)vb.b * chunk, e = b + chunk;
+  RowRange r;
+  r.begin = (int)(b < units ? b : units);
+  r.end = (int)(e < units ? e : units);
   return r;
 }
 

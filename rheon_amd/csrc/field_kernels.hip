@@ -257,57 +257,65 @@ __device__ __forceinline__ unsigned backtrace_lattice_face(const DevGrid& g, int
 constexpr int kAdvectWaves = RH_ADVECT_WAVES;
 constexpr int kAdvectLatticeWaves = RH_ADVECT_LATTICE_WAVES;
 
-// advect_velocity (src/advection.rs:28-68).  blockIdx.y = component.  Wall
-// normal faces are exactly zero; every other face samples its own component
-// at the RK2 departure of its own face position (src/geometry.rs:240-248).
+// advect_velocity (src/advection.rs:28-68) of face (i, j, k) of component
+// a, flat index idx: wall-normal faces are exactly zero; every other face
+// samples its own component at the RK2 departure of its own face position
+// (src/geometry.rs:240-248).  Returns the flags of a failed sample (the
+// face is then left unwritten).
 // kLattice (f32 faces): 1 = the grid is a lattice grid, 2 = decided per
-// grid at run time (batched launches); the first RK2 stage is then taken by
+// grid at run time; the first RK2 stage is then taken by
 // backtrace_lattice_face, the same values with a fraction of the work.
-// Component a of block vb.
+template <typename T, bool kOff32, int kLattice>
+__device__ __forceinline__ unsigned advect_face(const DevGrid& g, int a, int i, int j, int k, int idx, int top,
+                                                const T* __restrict__ u, const T* __restrict__ v,
+                                                const T* __restrict__ w, const T* __restrict__ src, double dt,
+                                                T* __restrict__ dst) {
+  const int p[3] = {i, j, k};
+  if (p[a] == 0 || p[a] == top || !face_open(g, a, i, j, k)) {
+    dst[idx] = (T)0.0;
+    return 0u;
+  }
+  double pos[3], dep[3], value = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
+  unsigned f;
+  if constexpr (kLattice != 0) {
+    if (kLattice == 1 || g.lattice)
+      f = backtrace_lattice_face<kOff32>(g, a, u, v, w, i, j, k, pos, dt, dep);
+    else
+      f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
+  } else {
+    f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
+  }
+  double base[3];
+  base_coords<true>(g, dep, base);
+  f |= sample_at<T, kOff32>(g, a, src, base, value);
+  if (f) return f;
+  dst[idx] = (T)value;
+  return 0u;
+}
+
+// Component a (blockIdx.y) of block vb: a contiguous chunk of the
+// component's faces in memory order, a whole number of passes of the
+// block's threads (rows as short as 33 faces keep every lane busy).
 template <typename T, bool kOff32, int kLattice = 0>
 __device__ __forceinline__ void
 b_advect_velocity(VBlock vb, int a, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
                   T* __restrict__ ow, unsigned* __restrict__ flags) {
   static_assert(kLattice == 0 || sizeof(T) == 4, "lattice first stage is stated for f32 faces");
-  const bool lattice = kLattice == 1 || (kLattice == 2 && g.lattice);
   const T* src = a == 0 ? u : (a == 1 ? v : w);
   T* dst = a == 0 ? ou : (a == 1 ? ov : ow);
   const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
-  const RowRange rr = block_rows(fy * fz, vb);
+  const FastDiv dx = a == 0 ? g.div_x1 : g.div_x, dy = a == 1 ? g.div_y1 : g.div_y;
+  const int top = (a == 0 ? fx : (a == 1 ? fy : fz)) - 1;
+  const RowRange fr = chunk_span(fx * fy * fz, vb);
   unsigned bad = 0;
-  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
-    const int k = row / fy, j = row - k * fy;
-    for (int i = threadIdx.x; i < fx; i += kLanesX) {
-      const int p[3] = {i, j, k};
-      const int idx = i + fx * row;
-      const int top = (a == 0 ? fx : (a == 1 ? fy : fz)) - 1;
-      if (p[a] == 0 || p[a] == top || !face_open(g, a, i, j, k)) {
-        dst[idx] = (T)0.0;
-        continue;
-      }
-      double pos[3], dep[3], value = 0.0;
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        pos[d] = g.origin[d] + ((double)p[d] + (d == a ? 0.0 : 0.5)) * g.h[d];
-      unsigned f;
-      if constexpr (kLattice != 0) {
-        if (lattice)
-          f = backtrace_lattice_face<kOff32>(g, a, u, v, w, i, j, k, pos, dt, dep);
-        else
-          f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
-      } else {
-        f = backtrace<T, kOff32>(g, u, v, w, pos, dt, dep);
-      }
-      double base[3];
-      base_coords<true>(g, dep, base);
-      f |= sample_at<T, kOff32>(g, a, src, base, value);
-      if (f) {
-        bad |= f;
-        continue;
-      }
-      dst[idx] = (T)value;
-    }
+  for (int t = fr.begin + flat_tid(); t < fr.end; t += kBlock) {
+    const int row = fast_div(t, dx), i = t - row * fx;
+    const int k = fast_div(row, dy), j = row - k * fy;
+    bad |= advect_face<T, kOff32, kLattice>(g, a, i, j, k, t, top, u, v, w, src, dt, dst);
   }
   if (bad) atomicOr(flags, bad);
 }
@@ -920,15 +928,15 @@ kb_validate(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int
   RH_BATCH_PROLOGUE(BFaceArgs)
   b_validate_faces<float>(vb, *a.g, a.u, a.v, a.w, a.mode, a.flags);
 }
-// No minimum-occupancy hint here: reading the scene's grid and arrays
-// through the argument record costs registers, and under the hint the
-// compiler spilled to scratch (measured 0.9 ms slower per 256^3 step than
-// 171 VGPRs at 2 waves per SIMD).
+// Lattice and general scenes go to separate launches (kLattice 1 or 0), so
+// each form fits the one-scene kernel's occupancy; the grid record is
+// copied into scalar registers (load_record) rather than read per use.
 template <bool kOff32, int kLattice>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, kLattice == 1 ? kAdvectLatticeWaves : kAdvectWaves)
 kb_advect(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BFaceArgs)
-  b_advect_velocity<float, kOff32, kLattice>(vb, (int)blockIdx.y, *a.g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow,
+  const DevGrid g = load_record(a.g);
+  b_advect_velocity<float, kOff32, kLattice>(vb, (int)blockIdx.y, g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow,
                                              a.flags);
 }
 __global__ void __launch_bounds__(kBlock)
@@ -978,11 +986,11 @@ void launch_b_advect(const BatchLaunch& bl, hipStream_t s, bool off32, bool latt
   if (!bl.blocks) return;
   const BFaceArgs* A = (const BFaceArgs*)bl.args;
   if (off32 && lattice)
-    kb_advect<true, 2><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
+    kb_advect<true, 1><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
   else if (off32)
     kb_advect<true, 0><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
   else if (lattice)
-    kb_advect<false, 2><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
+    kb_advect<false, 1><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
   else
     kb_advect<false, 0><<<face_batch_grid(bl), block_shape(), 0, s>>>(A, bl.prefix, bl.n);
 }

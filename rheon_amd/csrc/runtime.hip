@@ -98,6 +98,9 @@ DevGrid make_dev_grid(const rh_grid& g, double density) {
   d.nz = (int)g.counts[2];
   d.div_q = make_fast_div(d.nx / 4);
   d.div_y = make_fast_div(d.ny);
+  d.div_x = make_fast_div(d.nx);
+  d.div_x1 = make_fast_div(d.nx + 1);
+  d.div_y1 = make_fast_div(d.ny + 1);
   d.ncomp = 1;
   for (int a = 0; a < 3; ++a) {
     d.h[a] = g.spacing[a];
