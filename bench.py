@@ -337,7 +337,8 @@ def run_batch(args) -> dict:
     """Batched workload: 64 scenes, each axis log-uniform on [32, 128] and a
     multiple of 8 (cubic cells, longest side 1), Taylor-Green + noise with
     seed = scene id, 10 projection steps with the mixed-precision multigrid
-    solve, in one hipGraph replay per step with autotuned launch shapes;
+    solve, as hipGraph segments per step (the cycles the scenes need) with
+    autotuned launch shapes;
     against looping the one-scene path over the same scenes."""
     from rheon_amd.scenes import batch_workload
 
@@ -345,12 +346,12 @@ def run_batch(args) -> dict:
     counts = [list(sp.counts) for sp in specs]
     cells = sum(sp.cell_len for sp in specs)
     st = R.PressureSettings(args.rel, 1e-14, 1e-3, 40)
-    # the graph holds a fixed number of V-cycles per step (converged scenes'
-    # blocks return at once); 20 is twice what these scenes need to 1e-8
-    cycles = 20
+    # graph segments: the cycles the last step needed, then one more at a
+    # time while some scene is still solving, up to the solve's 40
+    cycles = 40
     coarse = coarse_sweeps(args)
-    batch = R.SceneBatch(specs, settings=st, use_graph=True, actual_divergence_limit=1e-3, max_cycles=cycles,
-                         mg_coarse_sweeps=coarse)
+    batch = R.SceneBatch(specs, settings=st, use_graph=True, adaptive=True, actual_divergence_limit=1e-3,
+                         max_cycles=cycles, mg_coarse_sweeps=coarse)
     for i in range(len(specs)):
         batch.scene(i).set_velocity(vels[i])
     batch.set_dt(dts)
@@ -400,7 +401,7 @@ def run_batch(args) -> dict:
             "workload": f"{len(specs)} scenes, axes log-uniform [32,128] multiple of 8, {args.steps} steps, "
                         f"mixed-mg-dc to rel {args.rel:g}",
             "total_cells": cells,
-            "cycles_per_step_in_graph": cycles,
+            "cycles_per_step": f"adaptive graph segments (at most {cycles})",
             "mg_coarse_sweeps": list(coarse) if args.mg_coarse_sweeps else "as finest",
             "shapes": counts,
             "batch_seconds": batch_s,

@@ -509,8 +509,14 @@ rh_status rh_scene_get_tracer(const rh_scene* scene, float* out, size_t len);
 typedef struct rh_batch rh_batch;
 typedef struct rh_batch_config {
   rh_scene_config scene;  /* shared: f32 faces, RH_MIXED_MG with f32 V-cycles */
-  int32_t max_cycles;     /* V-cycles per step (0 = scene.pressure.max_iterations) */
-  int32_t use_graph;      /* nonzero: capture one step in a hipGraph, replay it */
+  int32_t max_cycles;     /* V-cycles per step at most (0 = scene.pressure.max_iterations) */
+  /* 0: direct launches, the host stops enqueuing cycles once every scene's
+   *    solve is done (polled two cycles behind);
+   * 1: one hipGraph replay per step holding max_cycles V-cycles;
+   * 2: hipGraph segments per step: the cycles the last step needed, then
+   *    one more at a time while some scene is still solving (one host round
+   *    trip per check).  Every mode gives the same results. */
+  int32_t use_graph;
 } rh_batch_config;
 
 rh_status rh_batch_create(const rh_grid* grids, size_t n, const rh_batch_config* config,

@@ -639,7 +639,10 @@ class SceneBatch:
     def __init__(self, specs, density: float = 1.0, settings: PressureSettings | None = None,
                  actual_divergence_limit: float = 1e-3, max_courant: float = 0.0, max_cycles: int = 0,
                  use_graph: bool = True, kernel_variant: int = 0, mg_sweeps=(0, 0), mg_smoother: int = 0,
-                 mg_coarse_sweeps=(0, 0)):
+                 mg_coarse_sweeps=(0, 0), adaptive: bool = False):
+        """use_graph: one hipGraph replay per step with max_cycles V-cycles;
+        with adaptive, graph segments that run only the cycles the scenes
+        need (the same results); without a graph, direct launches."""
         self.specs = list(specs)
         settings = settings or PressureSettings(1e-8, 1e-14, 1e-3, 40)
         cfg = N.BatchConfig(
@@ -647,7 +650,7 @@ class SceneBatch:
                                                        0, kernel_variant, int(mg_smoother),
                                                        int(mg_coarse_sweeps[0]), int(mg_coarse_sweeps[1])),
                           settings.native(), actual_divergence_limit, max_courant),
-            int(max_cycles), 1 if use_graph else 0)
+            int(max_cycles), (2 if adaptive else 1) if use_graph else 0)
         grids = (N.Grid * len(self.specs))(*[make_grid(sp) for sp in self.specs])
         h = C.c_void_p()
         _check(lib().rh_batch_create(grids, len(self.specs), C.byref(cfg), C.byref(h)), "rh_batch_create")

@@ -301,6 +301,16 @@ class Batch {
   hipEvent_t poll_ev_[2] = {nullptr, nullptr};
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
+  // use_graph == 2: the step as segments (prologue, one cycle, one cycle
+  // then the count of scenes still solving, the post stage) and the
+  // cycles the last step needed
+  hipGraphExec_t seg_[4] = {nullptr, nullptr, nullptr, nullptr};
+  int* cycle_state_ = nullptr;      // [2] pinned: scenes still solving, largest cycle count
+  int* cycle_state_dev_ = nullptr;  // its device address
+  int next_cycles_ = 0;
+  hipEvent_t seg_ev_ = nullptr;
+  rh_status run_segments(const std::function<rh_status()>* host_work);
+  rh_status capture(size_t begin, size_t end, bool state, hipGraphExec_t* out);
   bool built_ = false;
   bool adv_off32_ = false;  // every scene's face arrays take 32-bit offsets
   int max_cycles_ = 0;
@@ -319,6 +329,8 @@ Batch::~Batch() {
   if (d_active_) (void)hipFree(d_active_);
   if (h_active_) (void)hipHostFree(h_active_);
   if (pinned_) (void)hipHostFree(pinned_);
+  if (cycle_state_) (void)hipHostFree(cycle_state_);
+  if (seg_ev_) (void)hipEventDestroy(seg_ev_);
   release_graph();
   if (d_tables_) (void)hipFree(d_tables_);
   if (d_grids_) (void)hipFree(d_grids_);
@@ -331,6 +343,10 @@ void Batch::release_graph() {
   if (graph_) (void)hipGraphDestroy(graph_);
   exec_ = nullptr;
   graph_ = nullptr;
+  for (auto& e : seg_) {
+    if (e) (void)hipGraphExecDestroy(e);
+    e = nullptr;
+  }
 }
 
 uint64_t Batch::bytes() const {
@@ -342,7 +358,7 @@ uint64_t Batch::bytes() const {
 rh_status Batch::create(const rh_grid* grids, size_t n, const rh_batch_config& cfg) {
   cfg_ = cfg;
   const rh_scene_config& sc = cfg.scene;
-  if (n == 0 || !grids) return RH_INVALID_ARGUMENT;
+  if (n == 0 || !grids || cfg.use_graph < 0 || cfg.use_graph > 2) return RH_INVALID_ARGUMENT;
   if (sc.precision != RH_F32 || sc.solver.method != RH_MIXED_MG || sc.solver.precision != RH_F32) {
     set_last_error("batched steps run f32 faces with the mixed-precision multigrid solve", "rh_batch_create");
     return RH_UNSUPPORTED;
@@ -728,7 +744,10 @@ rh_status Batch::build() {
     fin_launch(Op::kFinOuter, fin_table(
                                   [&](size_t i, int nb) {
                                     Scratch& scr = scenes_[i]->solver->impl->scratch();
-                                    return BFin<1, 2, FinMgTrue>{scr.pdd, scr.pmx, nb, 0, FinMgTrue{scr.ctl, it}};
+                                    // cycles take their iteration from the control block, so
+                                    // every cycle's launches are the same (segments replay)
+                                    return BFin<1, 2, FinMgTrue>{scr.pdd, scr.pmx, nb, 0,
+                                                                 FinMgTrue{scr.ctl, it == 0 ? 0 : -1}};
                                   },
                                   1, 2, outer_nb));
   };
@@ -909,7 +928,19 @@ rh_status Batch::build() {
   RH_HIP(hipMalloc(&d_tables_, std::max<size_t>(tables_bytes_, 16)));
   RH_HIP(hipMemcpy(d_tables_, T.bytes.data(), tables_bytes_, hipMemcpyHostToDevice));
   for (size_t i = 0; i < n; ++i) gens_[i] = scenes_[i]->solver->impl->generation();
-  if (cfg_.use_graph) {
+  if (cfg_.use_graph == 2) {
+    if (!cycle_state_) {
+      RH_HIP(hipHostMalloc(&cycle_state_, 2 * sizeof(int), hipHostMallocMapped));
+      RH_HIP(hipHostGetDevicePointer((void**)&cycle_state_dev_, cycle_state_, 0));
+    }
+    const size_t c1 = cycle_at_[1], c2 = max_cycles_ >= 2 ? cycle_at_[2] : post_at_;
+    rh_status s = capture(0, c1, false, &seg_[0]);
+    if (!s) s = capture(c1, c2, false, &seg_[1]);
+    if (!s) s = capture(c1, c2, true, &seg_[2]);
+    if (!s) s = capture(post_at_, program_.size(), false, &seg_[3]);
+    if (s) return s;
+    if (next_cycles_ < 1 || next_cycles_ > max_cycles_) next_cycles_ = std::max(1, max_cycles_ / 2);
+  } else if (cfg_.use_graph) {
     RH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     run_program(stream_);
     hipError_t e = hipStreamEndCapture(stream_, &graph_);
@@ -1011,6 +1042,79 @@ __global__ void kb_count_active(const BatchScene* __restrict__ S, int n, int* ou
 
 constexpr int kPollCycles = 2;
 
+// Scenes still solving and the largest outer iteration any scene reached,
+// written straight into mapped pinned memory (the host reads it after the
+// segment's event; no copy node in the graph).
+__global__ void kb_cycle_state(const BatchScene* __restrict__ S, int n, int* out) {
+  __shared__ int count, most;
+  if (threadIdx.x == 0) {
+    count = 0;
+    most = 0;
+  }
+  __syncthreads();
+  int mine = 0, hi = 0;
+  for (int s = threadIdx.x; s < n; s += blockDim.x) {
+    const PcgControl* c = S[s].ctl;
+    mine += c->done ? 0 : 1;
+    hi = max(hi, c->iteration);
+  }
+  if (mine) atomicAdd(&count, mine);
+  if (hi) atomicMax(&most, hi);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = count;
+    out[1] = most;
+  }
+}
+
+// Launches [begin, end) of the program, then (with `state`) the cycle
+// state, captured into an executable graph.
+rh_status Batch::capture(size_t begin, size_t end, bool state, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  RH_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  run_range(begin, end, stream_);
+  if (state) kb_cycle_state<<<1, 256, 0, stream_>>>(d_scenes_, (int)scenes_.size(), cycle_state_dev_);
+  hipError_t e = hipStreamEndCapture(stream_, &g);
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    set_last_error(hipGetErrorString(e), "Batch::capture");
+    return RH_DEVICE_ERROR;
+  }
+  return RH_OK;
+}
+
+// One step as graph segments: the prologue, the cycles the last step
+// needed (the last of them followed by the cycle state), then one more
+// cycle at a time while some scene is still solving, then the post stage.
+// Cycles after a scene's solve is done return at once for it, so the
+// results equal the fixed-cycle program's; only the launches of cycles no
+// scene needs are saved, at one host round trip per check.
+rh_status Batch::run_segments(const std::function<rh_status()>* host_work) {
+  if (!seg_ev_) RH_HIP(hipEventCreateWithFlags(&seg_ev_, hipEventDisableTiming));
+  RH_HIP(hipGraphLaunch(seg_[0], stream_));
+  int run = 0, want = std::min(std::max(next_cycles_, 1), max_cycles_);
+  bool worked = false;
+  volatile const int* st = cycle_state_;
+  for (;;) {
+    for (; run + 1 < want; ++run) RH_HIP(hipGraphLaunch(seg_[1], stream_));
+    RH_HIP(hipGraphLaunch(seg_[2], stream_));
+    ++run;
+    RH_HIP(hipEventRecord(seg_ev_, stream_));
+    if (host_work && *host_work && !worked) {
+      worked = true;
+      rh_status w = (*host_work)();
+      if (w) return w;
+    }
+    RH_HIP(hipEventSynchronize(seg_ev_));
+    if (st[0] == 0 || run >= max_cycles_) break;
+    want = run + 1;
+  }
+  next_cycles_ = std::max(1, std::min((int)st[1], max_cycles_));
+  RH_HIP(hipGraphLaunch(seg_[3], stream_));
+  return RH_OK;
+}
+
 // One step without a graph: the solve's V-cycles are enqueued kPollCycles
 // at a time; after each batch a one-block kernel counts the scenes still
 // solving and the count is copied to pinned host memory.  The host waits
@@ -1088,7 +1192,10 @@ rh_status Batch::launch(int steps, const std::function<rh_status()>& host_work) 
   }
   RH_HIP(hipEventRecord(ev_[0], stream_));
   for (int k = 0; k < steps; ++k) {
-    if (exec_) {
+    if (cfg_.use_graph == 2) {
+      rh_status r = run_segments(k == 0 ? &host_work : nullptr);
+      if (r) return r;
+    } else if (exec_) {
       RH_HIP(hipGraphLaunch(exec_, stream_));
     } else {
       rh_status r = run_polled(k == 0 ? &host_work : nullptr);

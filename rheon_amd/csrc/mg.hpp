@@ -153,9 +153,10 @@ void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, c
 // "candidate" (the true residual is computed each cycle).
 struct FinMgTrue {
   PcgControl* ctl;
-  int it;
+  int it;  // outer iteration; < 0: the one after ctl->iteration (a captured cycle replayed any number of times)
   __device__ bool active() const { return !ctl->done; }
   __device__ void operator()(const double* sum, const double* mx) const {
+    const int it = this->it >= 0 ? this->it : ctl->iteration + 1;
     if (mx[1] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
     const double squared = sum[0], maximum = mx[0];
     const double divergence = maximum * ctl->dt / ctl->volume;

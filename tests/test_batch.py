@@ -61,6 +61,53 @@ def test_batch_matches_single_scenes_bitwise(use_graph, coarse):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("max_cycles", [30, 4])
+def test_adaptive_cycles_match_the_fixed_graph(max_cycles):
+    """Graph segments that run only the cycles the scenes need: statuses,
+    cycle counts, residuals, faces and pressure equal the fixed-cycle graph
+    and the one-scene steps, bit for bit, over steps whose needs differ
+    (the first step starts from a guess of half the cap); with a cap of 4
+    every scene stops at the iteration limit in both."""
+    import rheon_amd as R
+    sp = specs()
+    st = R.PressureSettings(1e-8, 1e-14, 1e-3, 40)
+    batches = [R.SceneBatch(sp, settings=st, use_graph=True, adaptive=a, actual_divergence_limit=1e-3,
+                            max_cycles=max_cycles) for a in (False, True)]
+    singles, dts = [], []
+    for i, s in enumerate(sp):
+        vel = taylor_green_noise(s, seed=300 + i)
+        dts.append(courant_dt(s, vel, 0.5 if i % 2 else 0.2))
+        for b in batches:
+            b.scene(i).set_velocity(vel)
+        sc = R.Scene(s, 1.0, R.F32, "mixed-mg-dc", settings=R.PressureSettings(1e-8, 1e-14, 1e-3, max_cycles),
+                     actual_divergence_limit=1e-3)
+        sc.set_velocity(vel)
+        singles.append(sc)
+    for b in batches:
+        b.set_dt(dts)
+    for step in range(3):
+        fixed, adaptive = (b.step(1) for b in batches)
+        for i, sc in enumerate(singles):
+            r = sc.step(dts[i])
+            for rep in (fixed[i], adaptive[i]):
+                assert rep.status == r.status, (step, i, rep.status, r.status)
+                assert rep.pressure.iterations == r.pressure.iterations, (step, i)
+                assert rep.pressure.true_residual_l2 == r.pressure.true_residual_l2
+        if max_cycles == 4:
+            assert all(r.status == R.ITERATION_LIMIT for r in adaptive)
+            break
+    for i, sc in enumerate(singles):
+        want = sc.get_pressure().view(np.uint64)
+        for b in batches:
+            assert np.array_equal(b.scene(i).get_pressure().view(np.uint64), want), i
+            got = b.scene(i).get_velocity()
+            for a, ref in zip(got, sc.get_velocity()):
+                assert np.array_equal(a.view(np.uint32), ref.view(np.uint32)), i
+    for b in batches:
+        b.close()
+
+
+@pytest.mark.gpu
 def test_batch_failed_scene_keeps_its_faces():
     import rheon_amd as R
     sp = specs()[:3]
