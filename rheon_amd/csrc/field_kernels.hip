@@ -296,10 +296,13 @@ __device__ __forceinline__ unsigned advect_face(const DevGrid& g, int a, int i, 
   return 0u;
 }
 
-// Component a (blockIdx.y) of block vb: a contiguous chunk of the
-// component's faces in memory order, a whole number of passes of the
-// block's threads (rows as short as 33 faces keep every lane busy).
-template <typename T, bool kOff32, int kLattice = 0>
+// Component a (blockIdx.y) of block vb.  kFlat (batched launches): a
+// contiguous chunk of the component's faces in memory order, a whole
+// number of passes of the block's threads, so rows as short as 33 faces
+// keep every lane busy.  Otherwise rows of faces, lanes along x (one-scene
+// grids, whose rows are long; the flat form's index divisions cost 2% of a
+// 256^3 launch there).
+template <typename T, bool kOff32, int kLattice = 0, bool kFlat = false>
 __device__ __forceinline__ void
 b_advect_velocity(VBlock vb, int a, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
                   const T* __restrict__ w, double dt, T* __restrict__ ou, T* __restrict__ ov,
@@ -308,14 +311,23 @@ b_advect_velocity(VBlock vb, int a, const DevGrid& g, const T* __restrict__ u, c
   const T* src = a == 0 ? u : (a == 1 ? v : w);
   T* dst = a == 0 ? ou : (a == 1 ? ov : ow);
   const int fx = g.nx + (a == 0), fy = g.ny + (a == 1), fz = g.nz + (a == 2);
-  const FastDiv dx = a == 0 ? g.div_x1 : g.div_x, dy = a == 1 ? g.div_y1 : g.div_y;
   const int top = (a == 0 ? fx : (a == 1 ? fy : fz)) - 1;
-  const RowRange fr = chunk_span(fx * fy * fz, vb);
   unsigned bad = 0;
-  for (int t = fr.begin + flat_tid(); t < fr.end; t += kBlock) {
-    const int row = fast_div(t, dx), i = t - row * fx;
-    const int k = fast_div(row, dy), j = row - k * fy;
-    bad |= advect_face<T, kOff32, kLattice>(g, a, i, j, k, t, top, u, v, w, src, dt, dst);
+  if constexpr (kFlat) {
+    const FastDiv dx = a == 0 ? g.div_x1 : g.div_x, dy = a == 1 ? g.div_y1 : g.div_y;
+    const RowRange fr = chunk_span(fx * fy * fz, vb);
+    for (int t = fr.begin + flat_tid(); t < fr.end; t += kBlock) {
+      const int row = fast_div(t, dx), i = t - row * fx;
+      const int k = fast_div(row, dy), j = row - k * fy;
+      bad |= advect_face<T, kOff32, kLattice>(g, a, i, j, k, t, top, u, v, w, src, dt, dst);
+    }
+  } else {
+    const RowRange rr = block_rows(fy * fz, vb);
+    for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+      const int k = row / fy, j = row - k * fy;
+      for (int i = threadIdx.x; i < fx; i += kLanesX)
+        bad |= advect_face<T, kOff32, kLattice>(g, a, i, j, k, i + fx * row, top, u, v, w, src, dt, dst);
+    }
   }
   if (bad) atomicOr(flags, bad);
 }
@@ -936,7 +948,7 @@ __global__ void __launch_bounds__(kBlock, kLattice == 1 ? kAdvectLatticeWaves : 
 kb_advect(const BFaceArgs* __restrict__ A, const int* __restrict__ prefix, int n) {
   RH_BATCH_PROLOGUE(BFaceArgs)
   const DevGrid g = load_record(a.g);
-  b_advect_velocity<float, kOff32, kLattice>(vb, (int)blockIdx.y, g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow,
+  b_advect_velocity<float, kOff32, kLattice, true>(vb, (int)blockIdx.y, g, a.u, a.v, a.w, *a.dt, a.ou, a.ov, a.ow,
                                              a.flags);
 }
 __global__ void __launch_bounds__(kBlock)
