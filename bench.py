@@ -606,6 +606,39 @@ def bench_config2(args) -> int:
     return 0 if worst < 1e-5 else 1
 
 
+def run_warm_start(spec, vel, solid, dt_requested, settings, sweeps, smoother, args, max_courant) -> dict:
+    """The headline workload under AcceptedPreviousV1 (feature
+    accepted-pressure-warm-start of the reference: every solve seeded with
+    the last committed step's pressure): a fresh scene, the same untimed
+    steps before the same number of timed ones as the cold headline.  The
+    headline itself stays cold (the reference's default policy)."""
+    scene = R.Scene(spec, density=1.0, precision=R.F32, method=args.method,
+                    solver_precision=R.F32 if args.method == "mixed-mg-dc" else R.F64,
+                    settings=settings, actual_divergence_limit=1e-3, max_courant=max_courant,
+                    mg_sweeps=sweeps, mg_smoother=smoother, mg_coarse_sweeps=coarse_sweeps(args))
+    if solid is not None:
+        scene.set_solid(solid)
+    scene.set_initialization(R.INIT_ACCEPTED_PREVIOUS)
+    scene.set_velocity(vel)
+    for _ in range(args.warmup + 1):  # the cold run's warm-up and survey steps
+        if scene.step_requested(dt_requested).status != R.OK:
+            return {"failed": "untimed step"}
+    N.lib().rh_device_synchronize()
+    iters, divs = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rep = scene.step_requested(dt_requested)
+        if rep.status != R.OK:
+            return {"failed": R.STATUS_NAMES.get(rep.status)}
+        iters.append(rep.pressure.iterations)
+        divs.append(rep.actual_divergence_max)
+    N.lib().rh_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    scene.close()
+    return {"policy": "AcceptedPreviousV1", "steps_per_s": args.steps / elapsed,
+            "ms_per_step": elapsed / args.steps * 1e3, "iterations": iters, "max_div_u": max(divs)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -836,6 +869,8 @@ def main() -> int:
         result["config"]["compare"] = compare_methods(spec, scene, dt, solid, settings)
     if default_workload and not args.headline_only:
         scene.close()
+        result["config"]["warm_start"] = run_warm_start(spec, vel, solid, dt_requested, settings, sweeps, smoother,
+                                                        args, max_courant)
         # the batched and serving workloads, summarised (each has its own
         # --workload line with every detail)
         b = run_batch(argparse.Namespace(scenes=64, steps=10, rel=1e-8, mg_coarse_sweeps="", json_out=None))
