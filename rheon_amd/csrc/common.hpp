@@ -258,7 +258,10 @@ __device__ __forceinline__ void block_publish(DD (&dd)[ND], double (&mx)[NM], DD
 // `Fin` provides `bool active() const` (false when the producer returned
 // early without publishing) and `void operator()(const double* sums,
 // const double* maxima) const`, both __device__.
-template <int ND, int NM, typename Fin>
+// kU: partials loaded per round (0: the largest that fits, below); a
+// finisher run inside another kernel takes a small one to keep that
+// kernel's register budget.
+template <int ND, int NM, typename Fin, int kU = 0>
 __device__ __forceinline__ void finish_body(const DD* __restrict__ pdd, const double* __restrict__ pmx,
                                             int nb, const Fin& fin) {
   if (!fin.active()) return;
@@ -274,7 +277,7 @@ __device__ __forceinline__ void finish_body(const DD* __restrict__ pdd, const do
   // 32 (8192 partials in one round) when they fit 256 registers, else 16.
   // Each thread adds the partials b = t (mod kBlock) in ascending order
   // whatever kUnroll is, so the sums do not depend on it.
-  constexpr int kUnroll = (4 * ND + 2 * NM) * 32 <= 256 ? 32 : 16;
+  constexpr int kUnroll = kU > 0 ? kU : ((4 * ND + 2 * NM) * 32 <= 256 ? 32 : 16);
   for (int b0 = t; b0 < nb; b0 += kUnroll * kBlock) {
     DD v[kUnroll][ND];
     double w[kUnroll][NM];

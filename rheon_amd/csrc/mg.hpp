@@ -140,8 +140,9 @@ void launch_residual_report_q(const DevGrid& g, const double* p, const double* b
                               hipStream_t s);
 // Second finest pre-sweep from b0 alone (x0 = tau1 b0 / D recomputed per
 // position; the outer residual then does not store x0).
+struct MgFold;
 void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
-                              const PcgControl* ctl, hipStream_t s);
+                              const PcgControl* ctl, hipStream_t s, const MgFold* fold = nullptr);
 struct MgHierarchy;
 bool mg_first_from_b(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b);
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
@@ -154,8 +155,17 @@ void launch_mg_prolong_q(const MgLevel& F, const MgLevel& Cl, const float* xc, c
 struct FinMgTrue {
   PcgControl* ctl;
   int it;  // outer iteration; < 0: the one after ctl->iteration (a captured cycle replayed any number of times)
+  // Optional mapped pinned word the host paces its enqueueing on:
+  // ((iteration + 1) << 1) | done once this iteration is decided.
+  int* progress = nullptr;
   __device__ bool active() const { return !ctl->done; }
   __device__ void operator()(const double* sum, const double* mx) const {
+    decide(sum, mx);
+    if (progress)
+      __hip_atomic_store(progress, ((ctl->iteration + 1) << 1) | (ctl->done ? 1 : 0), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  __device__ void decide(const double* sum, const double* mx) const {
     const int it = this->it >= 0 ? this->it : ctl->iteration + 1;
     if (mx[1] != 0.0) return fail(ctl, RH_ARITHMETIC_FAILURE, it);
     const double squared = sum[0], maximum = mx[0];
@@ -178,17 +188,35 @@ struct FinMgTrue {
   }
 };
 
+// The previous outer residual's acceptance (FinMgTrue over its partials),
+// run by block 0 of the next cycle's first kernel instead of as a launch of
+// its own.  Kernels that run in the cycle before the decision lands do
+// scratch work only; the cycle's pressure update comes later and sees it.
+struct MgFold {
+  const DD* pdd;
+  const double* pmx;
+  int nb;
+  FinMgTrue fin;
+};
+// Blocks of the outer residual's launch (the acceptance finisher's partial count).
+template <typename S>
+int mg_outer_blocks(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b);
+
 // x0_implicit (FP32 only, mg_first_from_b): the outer residual does not
 // store the first pre-sweep and the cycle starts from b0 alone.
 // p_zero (four-cell FP32 path only): p holds 0 and is not read (see
 // launch_mg_outer_residual_q); the caller fills p when no cycle runs.
+// finish: launch the acceptance finisher after it (false: the next cycle's
+// first sweep carries it, MgFold); progress: FinMgTrue::progress.
 template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
                        PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit,
-                       bool p_zero = false);
+                       bool p_zero = false, bool finish = true, int* progress = nullptr);
+// fold: the previous outer residual's acceptance, run by the cycle's first
+// sweep (four-cell FP32 path with x0_implicit only).
 template <typename S>
 void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit,
-              bool p_zero = false);
+              bool p_zero = false, const MgFold* fold = nullptr);
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s);
 int mg_tail_lds_limit();  // dynamic LDS bytes the tail kernels may use
 

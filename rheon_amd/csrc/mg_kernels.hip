@@ -807,9 +807,15 @@ void launch_b_mg_coarsest_dense(const BatchLaunch& bl, hipStream_t s) {
 static inline dim3 level_grid(const MgLevel& L) { return dim3(level_blocks((long long)L.nx * L.ny * L.nz)); }
 
 template <typename S>
+int mg_outer_blocks(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b) {
+  const bool quad = sizeof(S) == 4 && h.quad && mg_quad_outer_ok(g, h.lv[0], p, b);
+  return quad ? mg_quad_outer_blocks(g) : (int)cells_grid(g).x;
+}
+
+template <typename S>
 void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b,
                        PcgControl* ctl, DD* pdd, double* pmx, int it, hipStream_t s, bool x0_implicit,
-                       bool p_zero) {
+                       bool p_zero, bool finish, int* progress) {
   dim3 grid = cells_grid(g);
   {
     prof::Scope ps(prof::kOuterResidual, s, it);
@@ -822,7 +828,7 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
       launch(k_mg_outer_residual<S>, grid, block_shape(), 0, s, g, h.lv[0], p, b, (S*)h.b[0], (S*)h.x[0],
                                                             h.tau_pre[0], ctl, pdd, pmx);
   }
-  launch_finish<1, 2>(pdd, pmx, grid.x, FinMgTrue{ctl, it}, s);
+  if (finish) launch_finish<1, 2>(pdd, pmx, grid.x, FinMgTrue{ctl, it, progress}, s);
 }
 
 // Profiler slot of a V-cycle kernel: finest-level kernels have their own
@@ -852,7 +858,7 @@ static void smooth(const MgHierarchy& h, int l, const S* xin, S* xout, double* p
 // recomputes the first from it).
 template <typename S>
 static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl, int it,
-                   hipStream_t s, bool x0_implicit, bool p_zero) {
+                   hipStream_t s, bool x0_implicit, bool p_zero, const MgFold* fold = nullptr) {
   S* x = (S*)h.x[l];
   S* y = (S*)h.y[l];
   const S* b = (const S*)h.b[l];
@@ -885,7 +891,7 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
   for (int sweep = 1; sweep < pre; ++sweep) {
     if (sweep == 1 && l == 0 && x0_implicit) {
       prof::Scope ps(prof::kMgSmoothFirst, s, it);
-      launch_mg_smooth_first_q(h.lv[0], (const float*)b, (float*)nxt, h.tau_pre[0], h.tau_pre[1], ctl, s);
+      launch_mg_smooth_first_q(h.lv[0], (const float*)b, (float*)nxt, h.tau_pre[0], h.tau_pre[1], ctl, s, fold);
     } else {
       smooth<S>(h, l, cur, nxt, nullptr, tau_pre[sweep], ctl, it, s);
     }
@@ -939,8 +945,8 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
 
 template <typename S>
 void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit,
-              bool p_zero) {
-  vcycle<S>(h, 0, p, ctl, it, s, x0_implicit, p_zero);
+              bool p_zero, const MgFold* fold) {
+  vcycle<S>(h, 0, p, ctl, it, s, x0_implicit, p_zero, fold);
 }
 
 void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
@@ -951,11 +957,17 @@ void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
   k_regauge<<<cells_grid(g), block_shape(), 0, s>>>(g, p, ctl);
 }
 
+template int mg_outer_blocks<float>(const DevGrid&, const MgHierarchy&, const double*, const double*);
+template int mg_outer_blocks<double>(const DevGrid&, const MgHierarchy&, const double*, const double*);
 template void mg_outer_residual<float>(const DevGrid&, const MgHierarchy&, const double*,
-                                       const double*, PcgControl*, DD*, double*, int, hipStream_t, bool, bool);
+                                       const double*, PcgControl*, DD*, double*, int, hipStream_t, bool, bool,
+                                       bool, int*);
 template void mg_outer_residual<double>(const DevGrid&, const MgHierarchy&, const double*,
-                                        const double*, PcgControl*, DD*, double*, int, hipStream_t, bool, bool);
-template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool, bool);
-template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool, bool);
+                                        const double*, PcgControl*, DD*, double*, int, hipStream_t, bool, bool,
+                                        bool, int*);
+template void mg_cycle<float>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool, bool,
+                              const MgFold*);
+template void mg_cycle<double>(const MgHierarchy&, double*, const PcgControl*, int, hipStream_t, bool, bool,
+                               const MgFold*);
 
 }  // namespace rh

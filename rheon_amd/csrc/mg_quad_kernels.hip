@@ -480,7 +480,14 @@ k_mg_smooth_qf(MgLevel L, const float* __restrict__ xin, const float* __restrict
 }
 __global__ void __launch_bounds__(kBlock)
 k_mg_smooth_first_q(MgLevel L, const float* __restrict__ b, float* __restrict__ xout, double tau1, double tau2,
-                    const PcgControl* ctl) {
+                    const PcgControl* ctl, MgFold fold) {
+  if (fold.fin.ctl && blockIdx.x == 0) {
+    // the previous outer residual's acceptance; this block then reads the
+    // control block after the decision (the barrier orders it), the others
+    // read it as it stands when they start
+    finish_body<1, 2, FinMgTrue, 4>(fold.pdd, fold.pmx, fold.nb, fold.fin);
+    __syncthreads();
+  }
   b_mg_smooth_first_q(hw_block(), L, b, xout, tau1, tau2, ctl);
 }
 template <bool kFrac>
@@ -618,9 +625,11 @@ void launch_mg_smooth_qf(const MgLevel& L, const float* xin, const float* b, flo
 }
 
 void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
-                              const PcgControl* ctl, hipStream_t s) {
+                              const PcgControl* ctl, hipStream_t s, const MgFold* fold) {
+  MgFold f{};
+  if (fold) f = *fold;
   launch(k_mg_smooth_first_q, dim3(quad_blocks(L.nx / 4, L.ny * L.nz)), block_shape(), 0, s, L, b, xout, tau1,
-         tau2, ctl);
+         tau2, ctl, f);
 }
 
 bool mg_first_from_b(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b) {

@@ -251,6 +251,9 @@ rh_status Scratch::init() {
   ctl = (PcgControl*)p;
   flags = counters + 32;
   RH_HIP(hipHostMalloc(&host, 4096, hipHostMallocDefault));
+  RH_HIP(hipHostMalloc(&progress, 64, hipHostMallocMapped));
+  RH_HIP(hipHostGetDevicePointer((void**)&progress_dev, progress, 0));
+  *(volatile int*)progress = 0;
   RH_HIP(hipEventCreate(&ev0));
   RH_HIP(hipEventCreate(&ev1));
   RH_HIP(hipEventCreateWithFlags(&poll_ev[0], hipEventDisableTiming));
@@ -262,6 +265,7 @@ rh_status Scratch::init() {
 
 void Scratch::release() {
   if (host) (void)hipHostFree(host);
+  if (progress) (void)hipHostFree(progress);
   if (base) (void)hipFree(base);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);

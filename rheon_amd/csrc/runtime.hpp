@@ -39,6 +39,10 @@ struct Scratch {
   double* scal = nullptr;
   PcgControl* ctl = nullptr;
   void* host = nullptr;
+  // One mapped pinned word the device writes a solve's progress into
+  // (FinMgTrue::progress): host view and device pointer.
+  int* progress = nullptr;
+  int* progress_dev = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t poll_ev[2] = {nullptr, nullptr};
   size_t bytes_ = 0;
@@ -95,6 +99,16 @@ class Solver {
   template <typename Step>
   rh_status drive(int max_iterations, int poll, Step step, bool* cancelled, rh_cancel_fn cancel,
                   void* ctx);
+  // Enqueues `step(it)` for it = 1.. paced on the device's progress word
+  // (scratch_.progress, written by each iteration's acceptance finisher):
+  // at least `ahead` iterations, then one beyond the iteration the device
+  // is running, so the device always has the next one queued and at most
+  // one iteration past convergence is enqueued (its launches return at
+  // once).  No copies or events per iteration; cancellation is polled
+  // every `poll` decided iterations.
+  template <typename Step>
+  rh_status drive_paced(int max_iterations, int poll, int ahead, Step step, bool* cancelled,
+                        rh_cancel_fn cancel, void* ctx, int* enqueued);
   rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out);
   void release_mask();
   void release_components();
@@ -188,6 +202,7 @@ class MgSolver : public Solver {
   size_t odc_cap_ = 0;        // doubles allocated at d_odc_
   MgHierarchy* d_h_ = nullptr;  // device copy of h_ read by the tail kernel
   double inv_density_ = 0.0;  // density the inverse was built for
+  int last_cycles_ = 0;       // cycles the previous solve took (enqueued without waiting next time)
   uint64_t bytes_ = 0;
 };
 

@@ -78,6 +78,54 @@ rh_status Solver::drive(int max_iterations, int poll, Step step, bool* cancelled
   return RH_OK;
 }
 
+template <typename Step>
+rh_status Solver::drive_paced(int max_iterations, int poll, int ahead, Step step, bool* cancelled,
+                              rh_cancel_fn cancel, void* ctx, int* enqueued) {
+  hipStream_t s = scratch_.stream;
+  volatile const int* progress = scratch_.progress;
+  *cancelled = false;
+  int it = 0;           // iterations enqueued
+  int polled = 0;       // decided iterations at the last cancellation poll
+  unsigned spins = 0;
+  bool drained = false;  // the stream had finished everything enqueued
+  for (;;) {
+    const int v = *progress;
+    if (v & 1) break;                  // terminal status on the device
+    const int seen = (v >> 1) - 1;     // last decided iteration (-1: none yet)
+    if (cancel && seen + 1 - polled >= poll) {
+      polled = seen + 1;
+      if (cancel(ctx)) {
+        *cancelled = true;
+        break;
+      }
+    }
+    if (it < max_iterations && it < std::max(ahead, seen + 2)) {
+      step(++it);
+      spins = 0;
+      drained = false;
+      continue;
+    }
+    // Waiting on the device.  Once the stream has drained (and the word,
+    // read again above, allows no further iteration -- e.g. all are
+    // enqueued and the last one's decision is not launched yet) the loop
+    // ends; the caller reads the control block itself.
+    if (drained) break;
+    if ((++spins & 1023u) == 0) {
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess) {
+        drained = true;
+      } else if (q != hipErrorNotReady) {
+        set_last_error(hipGetErrorString(q), "Solver::drive_paced");
+        return RH_DEVICE_ERROR;
+      }
+    }
+    __builtin_ia32_pause();
+  }
+  *enqueued = it;
+  RH_HIP(hipGetLastError());
+  return RH_OK;
+}
+
 rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out) {
   hipStream_t s = scratch_.stream;
   PcgControl* c = (PcgControl*)scratch_.host + 1;
@@ -950,24 +998,35 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
     launch_seed(g, seed_, d_pressure, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
   else if (!p_zero)
     RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
+  // Four-cell FP32 path: each outer residual's acceptance runs inside the
+  // next cycle's first sweep (MgFold), the last one after the loop.
+  const bool fold = x0_implicit;
+  int* progress = scratch_.progress_dev;
+  *(volatile int*)scratch_.progress = 0;
+  const int nb = mg_outer_blocks<S>(g, h_, d_pressure, d_rhs);
   mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, 0, s, x0_implicit,
-                       p_zero);
+                       p_zero, !fold, progress);
   RH_HIP(hipGetLastError());
   bool cancelled = false;
   const int poll = st.poll_interval > 0 ? st.poll_interval : 2;
-  rh_status rs = drive(
-      h.max_iterations, poll,
+  int enqueued = 0;
+  rh_status rs = drive_paced(
+      h.max_iterations, poll, std::max(0, last_cycles_ - 1),
       [&](int it) {
-        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s, x0_implicit, p_zero && it == 1);
+        const MgFold f{scratch_.pdd, scratch_.pmx, nb, FinMgTrue{scratch_.ctl, it - 1, progress}};
+        mg_cycle<S>(h_, d_pressure, scratch_.ctl, it, s, x0_implicit, p_zero && it == 1, fold ? &f : nullptr);
         mg_outer_residual<S>(g, h_, d_pressure, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, it, s,
-                             x0_implicit);
+                             x0_implicit, false, !fold, progress);
       },
-      &cancelled, cancel, ctx);
+      &cancelled, cancel, ctx, &enqueued);
   if (rs) return rs;
+  // the last enqueued outer residual's acceptance (a no-op once decided)
+  if (fold) launch_finish<1, 2>(scratch_.pdd, scratch_.pmx, nb, FinMgTrue{scratch_.ctl, enqueued, progress}, s);
   // Fix the gauge p[0] = 0 (the reference's eliminated gauge cell) and
   // report the true residual of the returned pressure.
   PcgControl c;
   rs = finish_report(report, cancelled, &c);
+  if (c.done) last_cycles_ = c.iteration;
   if (p_zero && c.iteration == 0)  // no cycle ran: the result is p = 0
     RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
   if (rs == RH_ITERATION_LIMIT) mg_regauge(g, d_pressure, scratch_.ctl, s);
