@@ -144,6 +144,18 @@ struct MgFold;
 void launch_mg_smooth_first_q(const MgLevel& L, const float* b, float* xout, double tau1, double tau2,
                               const PcgControl* ctl, hipStream_t s, const MgFold* fold = nullptr);
 struct MgHierarchy;
+// Fused legs of the small coarse levels (mg_fused_kernels.hip): FP32 levels
+// 1 <= l < the tail level (or the coarsest) below the four-cell size, with
+// up to kMgLegMaxSweeps coarse sweeps per side.  The down leg reads b[l],
+// stores the pre-smoothed x in y[l] and restricts onto b[l + 1]; the up
+// leg adds the prolonged x[l + 1] to y[l], post-smooths, and stores x[l].
+constexpr int kMgLegMaxSweeps = 3;
+struct MgLegTaus {
+  double t[kMgLegMaxSweeps];
+};
+bool mg_leg_fusable(const MgHierarchy& h, int l);
+void launch_mg_down_leg(const MgHierarchy& h, int l, const PcgControl* ctl, hipStream_t s);
+void launch_mg_up_leg(const MgHierarchy& h, int l, const PcgControl* ctl, hipStream_t s);
 bool mg_first_from_b(const DevGrid& g, const MgHierarchy& h, const double* p, const double* b);
 void launch_mg_restrict_q(const MgLevel& F, const MgLevel& Cl, const float* x, const float* b, float* bc,
                           float* xc, double tau, const PcgControl* ctl, hipStream_t s);

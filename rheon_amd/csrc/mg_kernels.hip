@@ -882,6 +882,17 @@ static void vcycle(const MgHierarchy& h, int l, double* p, const PcgControl* ctl
     }
     return;
   }
+  // small FP32 coarse level: one launch per leg (mg_fused_kernels.hip)
+  if (sizeof(S) == 4 && mg_leg_fusable(h, l)) {
+    {
+      prof::Scope ps(prof::kMgCoarse, s, -2);
+      launch_mg_down_leg(h, l, ctl, s);
+    }
+    vcycle<S>(h, l + 1, p, ctl, it, s, false, false);
+    prof::Scope ps(prof::kMgCoarse, s, -2);
+    launch_mg_up_leg(h, l, ctl, s);
+    return;
+  }
   // pre-smoothing: sweeps 2..nu1 (ping-pong x <-> y), result in `cur`
   S* cur = mg_first_in_y(h, l) ? y : x;
   S* nxt = cur == x ? y : x;
