@@ -70,6 +70,61 @@ b_mg_outer_residual_q(VBlock vb, const DevGrid& g, const MgLevel& L0, const doub
   block_publish<1, 2>(s, m, pdd, pmx, vb);
 }
 
+// Sweep scales of a box-rule level: an open cell (mask 63) and the x-wall
+// cells of a box row (62: x- closed, 61: x+ closed).
+struct QuadScales {
+  float w0, w1, w2, a63, a62, a61;
+  __device__ QuadScales(const MgLevel& L, double tau)
+      : w0((float)L.w[0]), w1((float)L.w[1]), w2((float)L.w[2]), a63((float)(tau * L.odiag[63])),
+        a62((float)(tau * L.odiag[62])), a61((float)(tau * L.odiag[61])) {}
+};
+
+// One sweep of the four cells i0..i0+3 of row (j, k) (cell index c, row
+// index row) whose neighbourhood Q is loaded: xn = x + (tau / D)(b - K x).
+// Closed obstacle faces are resolved in Q.
+__device__ __forceinline__ void smooth_quad(const MgLevel& L, const QuadScales& S, double tau, Quad<float>& Q,
+                                            const float bb[4], int i0, int j, int k, int c, int row,
+                                            float xn[4]) {
+  const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlain);
+  // an interior box-rule row: the cells' masks are 63 but for the x-wall
+  // cells (quad_box_mask), so the scales are the three entries above
+  const bool plain = !mrow && j > 0 && j + 1 < L.ny && k > 0 && k + 1 < L.nz;
+  float sc[4];
+  if (plain) {
+    sc[0] = i0 > 0 ? S.a63 : S.a62;
+    sc[1] = S.a63;
+    sc[2] = S.a63;
+    sc[3] = i0 + 4 < L.nx ? S.a63 : S.a61;
+  } else {
+    const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
+    const int yz = box_yz(L.ny, L.nz, j, k);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int mk = quad_mask(mrow, m4, L.nx, i0, yz, u);
+      if (mrow) close_faces(Q, u, mk);
+      sc[u] = (float)(tau * L.odiag[mk]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const float ax = quad_apply_f(Q, u, S.w0, S.w1, S.w2);
+    xn[u] = Q.c[u] + sc[u] * (bb[u] - ax);
+  }
+}
+
+// Residual b - K x of four cells of a level-0 (box or obstacle mask) row
+// whose neighbourhood Q is loaded.
+__device__ __forceinline__ void residual_quad(const MgLevel& F, Quad<float>& Q, const float bb[4], int fc, int row,
+                                              float w0, float w1, float w2, float r[4]) {
+  const unsigned char* const mrow = row_mask(F.mask, F.nx, F.ny, F.nz, row, kRowPlain);
+  const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + fc) : 0u;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    if (mrow) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
+    r[u] = bb[u] - quad_apply_f(Q, u, w0, w1, w2);
+  }
+}
+
 // One smoothing sweep (as k_mg_smooth); kUpdate: p += x_out instead of
 // storing x_out (the finest level's last sweep).  kZeroP: p is known to be
 // 0 (the first cycle), so p = 0.0 + x_out is formed without reading it.
@@ -78,10 +133,7 @@ __device__ __forceinline__ void
 b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const float* __restrict__ b,
               float* __restrict__ xout, double* __restrict__ p, double tau, const PcgControl* ctl) {
   if (ctl->done) return;
-  const float w0 = (float)L.w[0], w1 = (float)L.w[1], w2 = (float)L.w[2];
-  // scales of an open cell and of the x-wall cells of a box row
-  const float a63 = (float)(tau * L.odiag[63]), a62 = (float)(tau * L.odiag[62]);
-  const float a61 = (float)(tau * L.odiag[61]);
+  const QuadScales S(L, tau);
   const int nz_ = L.nz;
   FOR_QUADS(L.nx, L.ny, L.div_q, L.div_y)
   Quad<float> Q;
@@ -97,32 +149,8 @@ b_mg_smooth_q(VBlock vb, const MgLevel& L, const float* __restrict__ xin, const 
       load4(p, c, pv);
     }
   }
-  const unsigned char* const mrow = row_mask(L.mask, L.nx, L.ny, L.nz, row, kRowPlain);
-  // an interior box-rule row: the cells' masks are 63 but for the x-wall
-  // cells (quad_box_mask), so the scales are the three entries above
-  const bool plain = !mrow && j > 0 && j + 1 < L.ny && k > 0 && k + 1 < L.nz;
-  float sc[4];
-  if (plain) {
-    sc[0] = i0 > 0 ? a63 : a62;
-    sc[1] = a63;
-    sc[2] = a63;
-    sc[3] = i0 + 4 < L.nx ? a63 : a61;
-  } else {
-    const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + c) : 0u;
-    const int yz = box_yz(L.ny, L.nz, j, k);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int mk = quad_mask(mrow, m4, L.nx, i0, yz, u);
-      if (mrow) close_faces(Q, u, mk);
-      sc[u] = (float)(tau * L.odiag[mk]);
-    }
-  }
   float xn[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const float ax = quad_apply_f(Q, u, w0, w1, w2);
-    xn[u] = Q.c[u] + sc[u] * (bb[u] - ax);
-  }
+  smooth_quad(L, S, tau, Q, bb, i0, j, k, c, row, xn);
   if (kUpdate) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) pv[u] = pv[u] + (double)xn[u];
@@ -328,13 +356,7 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
 #pragma unroll
         for (int u = 0; u < 4; ++u) r[u] = bb[u] - quad_apply_frac(F, Q, u, frac6_of(R, u, i0 + u + 1 < F.nx));
       } else {
-        const unsigned char* const mrow = row_mask(F.mask, F.nx, F.ny, F.nz, fj + F.ny * fk, kRowPlain);
-        const unsigned m4 = mrow ? *reinterpret_cast<const unsigned*>(mrow + fc) : 0u;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (mrow) close_faces(Q, u, (int)((m4 >> (8 * u)) & 63u));
-          r[u] = bb[u] - quad_apply_f(Q, u, w0, w1, w2);
-        }
+        residual_quad(F, Q, bb, fc, fj + F.ny * fk, w0, w1, w2, r);
       }
       s0 += r[0];
       s0 += r[1];
@@ -350,14 +372,12 @@ b_mg_restrict_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __r
 
 // Prolongation (as k_mg_prolong) for four fine cells: parents I0 = i0/2 and
 // I0 + 1 and their x-neighbours I0 - 1 and I0 + 2 (mirrored at walls).
-__device__ __forceinline__ void
-b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __restrict__ xc, const float* __restrict__ xin,
-               float* __restrict__ xout, const PcgControl* ctl) {
-  if (ctl->done) return;
-  const int nz_ = F.nz;
+// out = xin + P xc for cells i0..i0+3 of row (j, k) (cell c, row index row).
+__device__ __forceinline__ void prolong_quad(const MgLevel& F, const MgLevel& Cl, const float* __restrict__ xc,
+                                             const float* __restrict__ xin, int i0, int j, int k, int c, int row,
+                                             float out[4]) {
   const int sy = Cl.nx, sz = Cl.nx * Cl.ny;
   const float a = 0.75f, bq = 0.25f;
-  FOR_QUADS(F.nx, F.ny, F.div_q, F.div_y)
   const int I0 = i0 >> 1, J = j >> 1, K = k >> 1;
   int Jn = J + ((j & 1) ? 1 : -1), Kn = K + ((k & 1) ? 1 : -1);
   if (Jn < 0 || Jn >= Cl.ny) Jn = J;
@@ -404,7 +424,6 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
       if (fluid && !((bits >> ((u & 1) | oyz)) & 1u)) slow |= 1u << u;
     }
   }
-  float out[4];
   // fine cell u: parent index P = 1 + (u >> 1), neighbour N = u even ? P-1 : P+1
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
@@ -449,6 +468,16 @@ b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __re
       out[u] = xi[u] + (den > 0.0f ? num / den : 0.0f);
     }
   }
+}
+
+__device__ __forceinline__ void
+b_mg_prolong_q(VBlock vb, const MgLevel& F, const MgLevel& Cl, const float* __restrict__ xc, const float* __restrict__ xin,
+               float* __restrict__ xout, const PcgControl* ctl) {
+  if (ctl->done) return;
+  const int nz_ = F.nz;
+  FOR_QUADS(F.nx, F.ny, F.div_q, F.div_y)
+  float out[4];
+  prolong_quad(F, Cl, xc, xin, i0, j, k, c, row, out);
   store4(xout, c, out);
   END_QUADS
 }
