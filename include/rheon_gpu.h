@@ -67,7 +67,8 @@ typedef enum rh_status {
   RH_BUFFER_LIMIT = 21,        /* PressureError::BufferLimit / GeometryError::BufferLimit */
   RH_DEVICE_ERROR = 22,        /* HIP runtime failure (message in rh_last_error) */
   RH_REFUSED = 23,             /* ProjectionError::Refused (the kind is in rh_projection_report.refusal) */
-  RH_INVALID_INITIAL_PRESSURE = 24 /* PressureError::InvalidInitialPressure */
+  RH_INVALID_INITIAL_PRESSURE = 24, /* PressureError::InvalidInitialPressure */
+  RH_INVALID_STATE = 25        /* Dense3dError::InvalidState (the reference's message in rh_last_error) */
 } rh_status;
 
 /* GridGeometry::new inputs (src/geometry.rs:114-118). */
@@ -613,6 +614,70 @@ rh_status rh_server_result(rh_server* server, int64_t id, float* u, float* v, fl
                            double* pressure);
 /* Planned device bytes of one request's scene (the admission figure). */
 uint64_t rh_server_plan_bytes(const rh_grid* grid, const rh_scene_config* config);
+
+/* ------------------------------------------------------- dense3d frames */
+/* write_dense3d_frame (src/dense3d_sequence.rs:412-564): one accepted
+ * transport state as one JSON line, every field number formatted on the
+ * device exactly as the reference's `{v:.9e}` (f32) and `{v:.17e}` (f64). */
+typedef struct rh_volume_stamp {
+  uint64_t id;
+  uint64_t version;
+} rh_volume_stamp;
+/* LiquidTransportView as the frame reads it: the grid's flat face and cell
+ * arrays in device memory (carrier faces and tracer f32, liquid fraction and
+ * pressure f64), the clocks and stamps, and whether the view is
+ * transport-only (no pressure surface, pressure columns, reconstructed
+ * surface or flat column MAC). */
+typedef struct rh_dense3d_view {
+  const float* velocity[3];
+  const float* tracer;
+  const double* fraction;
+  const double* pressure;
+  double carrier_time;
+  double liquid_time;
+  uint64_t carrier_generation;
+  rh_volume_stamp carrier_stamp;
+  rh_volume_stamp liquid_stamp;
+  int32_t transport_only;
+  int32_t reserved;
+} rh_dense3d_view;
+/* The LiquidStepReport fields a frame checks and prints (carrier.step: time,
+ * dt, generation, pressure.iterations, pressure.true_residual_max,
+ * actual_divergence_max; liquid: time, dt, stamp, flow and the volume
+ * accounting). */
+typedef struct rh_dense3d_report {
+  double carrier_time;
+  double carrier_dt;
+  uint64_t carrier_generation;
+  uint64_t pressure_iterations;
+  double pressure_true_residual_max;
+  double carrier_actual_divergence_max;
+  double liquid_time;
+  double liquid_dt;
+  rh_volume_stamp liquid_stamp;
+  rh_volume_stamp liquid_flow;
+  double liquid_actual_divergence_max;
+  double liquid_volume_before;
+  double liquid_volume_after;
+  double liquid_mass_after;
+  double inward_boundary_volume;
+  double outward_boundary_volume;
+  double source_volume;
+  double volume_balance_error;
+  double volume_rounding_budget;
+} rh_dense3d_report;
+/* write_dense3d_frame (src/dense3d_sequence.rs:415-564) into device memory
+ * d_out: the preflight checks in the reference's order (a failure is
+ * RH_INVALID_STATE, message in rh_last_error, nothing written), then the
+ * line, newline included; *bytes = its length.  report null: the
+ * constructor frame (frame 0 at time 0).  A null d_out or a capacity below
+ * the length returns RH_BUFFER_LIMIT with the length needed in *bytes. */
+rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view* view, uint64_t frame,
+                           const rh_dense3d_report* report, char* d_out, uint64_t capacity, uint64_t* bytes);
+/* The frame's number format on the host: finite v with `digits` (10 or 18)
+ * significant digits, NUL-terminated into out (>= 26 bytes); returns the
+ * length, -1 for a non-finite v or another digit count. */
+int32_t rh_format_sci(double v, int32_t digits, char* out);
 
 #ifdef __cplusplus
 }

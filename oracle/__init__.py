@@ -103,7 +103,19 @@ _P = C.c_void_p
 _G = C.POINTER(_Grid)
 
 
+class _D3Meta(C.Structure):
+    _fields_ = [("frame", C.c_uint64), ("time", C.c_double), ("has_report", C.c_int32), ("pad", C.c_int32),
+                ("dt", C.c_double), ("carrier_id", C.c_uint64), ("carrier_version", C.c_uint64),
+                ("liquid_id", C.c_uint64), ("liquid_version", C.c_uint64), ("pressure_iterations", C.c_uint64),
+                ("diag", C.c_double * 11)]
+
+
 def _declare(L):
+    L.or_format_sci.argtypes = [C.c_double, C.c_int, C.c_char_p]
+    L.or_format_sci.restype = C.c_int
+    L.or_dense3d_frame.argtypes = [C.POINTER(C.c_int64), _P, _P, _P, _P, _P, _P, C.POINTER(_D3Meta), _P, C.c_int64,
+                                   C.c_int]
+    L.or_dense3d_frame.restype = C.c_int64
     L.or_grid_init.argtypes = [_G, C.POINTER(C.c_uint64), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.or_buffer_plan.argtypes = [_G, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.or_buffer_plan.restype = C.c_int64
@@ -501,3 +513,77 @@ def zero_blocked_faces(grid: Grid, ob: Obstacle, vel):
         a[tuple(inner)] = np.where(open_, a[tuple(inner)], 0)
         outs.append(a.reshape(-1))
     return outs
+
+
+
+# ------------------------------------------------------------ dense3d frames
+class Dense3dInvalidState(Exception):
+    """Dense3dError::InvalidState (src/dense3d_sequence.rs:21-27)."""
+
+
+def format_sci(v: float, digits: int) -> str:
+    """Rust's `{v:.(digits-1)e}` (or_format_sci: glibc's exact conversion)."""
+    buf = C.create_string_buffer(64)
+    n = lib().or_format_sci(float(v), int(digits), buf)
+    return buf.raw[:n].decode()
+
+
+def dense3d_frame(counts, velocity, tracer, fraction, pressure, *, frame, carrier_time=0.0, liquid_time=0.0,
+                  carrier_generation=0, carrier_stamp=(0, 0), liquid_stamp=(0, 0), transport_only=True,
+                  report=None, threads: int = 1) -> bytes:
+    """write_dense3d_frame (src/dense3d_sequence.rs:415-564): the preflight
+    checks in the reference's order (:422-495), then the line (C
+    serialization).  ``report``: an object with the rh_dense3d_report field
+    names (carrier_time, carrier_dt, ...), or None for the constructor frame."""
+    n = [int(c) for c in counts]
+    cells = n[0] * n[1] * n[2]
+
+    def invalid(ok, msg):
+        if not ok:
+            raise Dense3dInvalidState(msg)
+
+    invalid(transport_only, "transport-only view required")
+    vel = []
+    for axis in range(3):
+        a = np.ascontiguousarray(velocity[axis], np.float32)
+        flen = (n[0] + (axis == 0)) * (n[1] + (axis == 1)) * (n[2] + (axis == 2))
+        invalid(a.size == flen, "face shape")
+        invalid(bool(np.all(np.isfinite(a))), "nonfinite velocity")
+        vel.append(a)
+    tr = np.ascontiguousarray(tracer, np.float32)
+    fr = np.ascontiguousarray(fraction, np.float64)
+    pr = np.ascontiguousarray(pressure, np.float64)
+    invalid(tr.size == cells and fr.size == cells and pr.size == cells, "cell shape")
+    invalid(bool(np.all(np.isfinite(tr)) and np.all(np.isfinite(pr)) and np.all(np.isfinite(fr))
+                 and np.all((fr >= 0.0) & (fr <= 1.0))), "nonfinite or invalid cell field")
+    invalid(np.isfinite(carrier_time) and carrier_time == liquid_time and carrier_stamp[1] == carrier_generation,
+            "state clock/stamp")
+    m = _D3Meta()
+    m.frame = int(frame)
+    m.time = float(carrier_time)
+    if report is not None:
+        r = report
+        invalid(r.carrier_time == carrier_time and r.liquid_time == r.carrier_time and r.carrier_dt == r.liquid_dt
+                and r.carrier_dt > 0.0 and r.carrier_generation == carrier_generation
+                and tuple(r.liquid_stamp) == tuple(liquid_stamp) and tuple(r.liquid_flow) == tuple(carrier_stamp),
+                "report continuity")
+        diag = [r.pressure_true_residual_max, r.carrier_actual_divergence_max, r.liquid_actual_divergence_max,
+                r.liquid_volume_before, r.liquid_volume_after, r.liquid_mass_after, r.inward_boundary_volume,
+                r.outward_boundary_volume, r.source_volume, r.volume_balance_error, r.volume_rounding_budget]
+        invalid(all(np.isfinite(x) for x in [r.carrier_dt] + diag), "nonfinite report")
+        m.has_report = 1
+        m.dt = float(r.carrier_dt)
+        m.pressure_iterations = int(r.pressure_iterations)
+        for k, x in enumerate(diag):
+            m.diag[k] = float(x)
+    else:
+        invalid(frame == 0 and carrier_time == 0.0 and carrier_generation == 0, "initial frame")
+    m.carrier_id, m.carrier_version = (int(x) for x in carrier_stamp)
+    m.liquid_id, m.liquid_version = (int(x) for x in liquid_stamp)
+    nn = (C.c_int64 * 3)(*n)
+    args = [nn] + [_ptr(a) for a in vel] + [_ptr(tr), _ptr(fr), _ptr(pr), C.byref(m)]
+    need = -lib().or_dense3d_frame(*args, None, 0, int(threads))
+    out = C.create_string_buffer(need)
+    got = lib().or_dense3d_frame(*args, out, need, int(threads))
+    assert got == need
+    return out.raw[:got]

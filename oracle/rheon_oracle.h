@@ -184,6 +184,29 @@ double or_velocity_rate_f32(const or_grid* g, const float* u, const float* v, co
 double or_velocity_rate_f64(const or_grid* g, const double* u, const double* v,
                             const double* w);
 
+/* ---- dense3d frames (dense3d_oracle.c; src/dense3d_sequence.rs:415-564) */
+/* The frame header and diagnostics values (already checked). */
+typedef struct or_d3_meta {
+  uint64_t frame;
+  double time;   /* state.carrier.time */
+  int32_t has_report;
+  int32_t pad;
+  double dt;     /* report.carrier.step.dt */
+  uint64_t carrier_id, carrier_version, liquid_id, liquid_version;
+  uint64_t pressure_iterations;
+  double diag[11]; /* pressure_residual, carrier and liquid divergence, volume before/after, mass after,
+                      inward, outward, source, balance, rounding budget (the diagnostics object's order) */
+} or_d3_meta;
+/* Rust's `{:.Pe}` of a finite double with `digits` = P + 1 significant digits. */
+int or_format_sci(double v, int digits, char* out);
+/* write_dense3d_frame's line (src/dense3d_sequence.rs:497-563); returns its
+   length, or minus the length when out is NULL or cap is smaller.  threads
+   > 1 formats the field arrays in 64 Ki-value chunks on that many OpenMP
+   threads (the same bytes). */
+int64_t or_dense3d_frame(const int64_t n[3], const float* u, const float* v, const float* w, const float* tracer,
+                         const double* fraction, const double* pressure, const or_d3_meta* meta, char* out,
+                         int64_t cap, int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,7 @@ BUFFER_LIMIT = 21
 DEVICE_ERROR = 22
 REFUSED = 23
 INVALID_INITIAL_PRESSURE = 24
+INVALID_STATE = 25
 
 
 STATUS_NAMES = {v: k for k, v in dict(globals()).items() if isinstance(v, int) and k.isupper()}
@@ -64,7 +65,8 @@ METHOD_IDS = {"jacobi-pcg-v1": JACOBI_PCG, "rbsgs-pcg": RBSGS_PCG, "mg-pcg": MG_
 
 class RheonError(RuntimeError):
     def __init__(self, status: int, where: str = ""):
-        msg = N.lib().rh_last_error().decode(errors="replace") if status == DEVICE_ERROR else ""
+        msg = (N.lib().rh_last_error().decode(errors="replace")
+               if status in (DEVICE_ERROR, INVALID_STATE) else "")
         super().__init__(f"{where}: {STATUS_NAMES.get(status, status)} {msg}".strip())
         self.status = status
 
@@ -922,3 +924,123 @@ class ProjectionWorkspace:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ dense3d frames
+@dataclass
+class Dense3dState:
+    """The LiquidTransportView fields a dense3d frame serializes
+    (src/dense3d_sequence.rs:415-495): the carrier faces and tracer (f32),
+    the liquid fraction and pressure (f64) as host arrays or DeviceArrays,
+    the clocks and the stamps ((id, version) pairs)."""
+    velocity: tuple
+    tracer: object
+    fraction: object
+    pressure: object
+    carrier_time: float = 0.0
+    liquid_time: float = 0.0
+    carrier_generation: int = 0
+    carrier_stamp: tuple = (0, 0)
+    liquid_stamp: tuple = (0, 0)
+    transport_only: bool = True
+
+
+@dataclass
+class Dense3dReport:
+    """The LiquidStepReport fields a frame checks and prints (carrier.step and
+    liquid of src/liquid_step.rs:112-131)."""
+    carrier_time: float
+    carrier_dt: float
+    carrier_generation: int
+    pressure_iterations: int = 0
+    pressure_true_residual_max: float = 0.0
+    carrier_actual_divergence_max: float = 0.0
+    liquid_time: float = 0.0
+    liquid_dt: float = 0.0
+    liquid_stamp: tuple = (0, 0)
+    liquid_flow: tuple = (0, 0)
+    liquid_actual_divergence_max: float = 0.0
+    liquid_volume_before: float = 0.0
+    liquid_volume_after: float = 0.0
+    liquid_mass_after: float = 0.0
+    inward_boundary_volume: float = 0.0
+    outward_boundary_volume: float = 0.0
+    source_volume: float = 0.0
+    volume_balance_error: float = 0.0
+    volume_rounding_budget: float = 0.0
+
+
+def format_sci(v: float, digits: int) -> str:
+    """Rust's `{v:.(digits-1)e}` on the host (the frame's number format)."""
+    buf = C.create_string_buffer(32)
+    n = lib().rh_format_sci(float(v), int(digits), buf)
+    if n < 0:
+        raise ValueError("finite value and 10 or 18 digits required")
+    return buf.value.decode()
+
+
+def _dense3d_structs(state: Dense3dState, report):
+    keep = []
+
+    def dev(a, dtype):
+        if isinstance(a, DeviceArray):
+            return a
+        d = DeviceArray.from_host(np.ascontiguousarray(a, dtype))
+        keep.append(d)
+        return d
+
+    view = N.Dense3dView()
+    for d in range(3):
+        view.velocity[d] = dev(state.velocity[d], np.float32).ptr
+    view.tracer = dev(state.tracer, np.float32).ptr
+    view.fraction = dev(state.fraction, np.float64).ptr
+    view.pressure = dev(state.pressure, np.float64).ptr
+    view.carrier_time = float(state.carrier_time)
+    view.liquid_time = float(state.liquid_time)
+    view.carrier_generation = int(state.carrier_generation)
+    view.carrier_stamp.id, view.carrier_stamp.version = (int(x) for x in state.carrier_stamp)
+    view.liquid_stamp.id, view.liquid_stamp.version = (int(x) for x in state.liquid_stamp)
+    view.transport_only = 1 if state.transport_only else 0
+    rep = None
+    if report is not None:
+        rep = N.Dense3dReport()
+        for name, _ in N.Dense3dReport._fields_:
+            val = getattr(report, name)
+            if name in ("liquid_stamp", "liquid_flow"):
+                getattr(rep, name).id, getattr(rep, name).version = (int(x) for x in val)
+            else:
+                setattr(rep, name, val)
+    return view, rep, keep
+
+
+def dense3d_frame_device(spec: GridSpec, state: Dense3dState, frame: int, report, out: "DeviceArray | None"):
+    """write_dense3d_frame into device memory (rh_dense3d_frame).  Returns
+    the line's length; with ``out`` None or too small raises BUFFER_LIMIT
+    after setting ``needed`` on the error."""
+    g = make_grid(spec)
+    view, rep, keep = _dense3d_structs(state, report)
+    n = C.c_uint64()
+    st = lib().rh_dense3d_frame(C.byref(g), C.byref(view), int(frame), C.byref(rep) if rep is not None else None,
+                                out.ptr if out is not None else None, out.nbytes if out is not None else 0,
+                                C.byref(n))
+    if st != OK:
+        err = RheonError(st, "rh_dense3d_frame")
+        err.needed = n.value
+        raise err
+    return n.value
+
+
+def write_dense3d_frame(spec: GridSpec, state: Dense3dState, frame: int, report=None) -> bytes:
+    """write_dense3d_frame (src/dense3d_sequence.rs:415-564): the frame's
+    JSONL line, formatted on the device and copied back.  A failed preflight
+    raises RheonError(INVALID_STATE) with the reference's message."""
+    try:
+        dense3d_frame_device(spec, state, frame, report, None)
+        raise AssertionError("rh_dense3d_frame wrote into a null buffer")
+    except RheonError as e:
+        if e.status != BUFFER_LIMIT:
+            raise
+        needed = e.needed
+    out = DeviceArray(needed, np.uint8)
+    n = dense3d_frame_device(spec, state, frame, report, out)
+    return out.download()[:n].tobytes()

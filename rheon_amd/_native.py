@@ -271,6 +271,51 @@ _G = C.POINTER(Grid)
 _S = C.c_int
 
 # name -> (restype, argtypes); every function declared in include/rheon_gpu.h
+
+class VolumeStamp(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("version", C.c_uint64)]
+
+
+class Dense3dView(C.Structure):
+    _fields_ = [
+        ("velocity", C.c_void_p * 3),
+        ("tracer", C.c_void_p),
+        ("fraction", C.c_void_p),
+        ("pressure", C.c_void_p),
+        ("carrier_time", C.c_double),
+        ("liquid_time", C.c_double),
+        ("carrier_generation", C.c_uint64),
+        ("carrier_stamp", VolumeStamp),
+        ("liquid_stamp", VolumeStamp),
+        ("transport_only", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class Dense3dReport(C.Structure):
+    _fields_ = [
+        ("carrier_time", C.c_double),
+        ("carrier_dt", C.c_double),
+        ("carrier_generation", C.c_uint64),
+        ("pressure_iterations", C.c_uint64),
+        ("pressure_true_residual_max", C.c_double),
+        ("carrier_actual_divergence_max", C.c_double),
+        ("liquid_time", C.c_double),
+        ("liquid_dt", C.c_double),
+        ("liquid_stamp", VolumeStamp),
+        ("liquid_flow", VolumeStamp),
+        ("liquid_actual_divergence_max", C.c_double),
+        ("liquid_volume_before", C.c_double),
+        ("liquid_volume_after", C.c_double),
+        ("liquid_mass_after", C.c_double),
+        ("inward_boundary_volume", C.c_double),
+        ("outward_boundary_volume", C.c_double),
+        ("source_volume", C.c_double),
+        ("volume_balance_error", C.c_double),
+        ("volume_rounding_budget", C.c_double),
+    ]
+
+
 SIGNATURES = {
     "rh_abi_version": (C.c_int32, []),
     "rh_last_error": (C.c_char_p, []),
@@ -351,6 +396,9 @@ SIGNATURES = {
     "rh_server_plan_bytes": (C.c_uint64, [C.POINTER(Grid), C.POINTER(SceneConfig)]),
     "rh_batch_autotune": (_S, [_P, C.c_int32, C.POINTER(BatchTuning), C.c_size_t, C.POINTER(C.c_size_t),
                                C.POINTER(BatchTuning)]),
+    "rh_dense3d_frame": (_S, [C.POINTER(Grid), C.POINTER(Dense3dView), C.c_uint64, C.POINTER(Dense3dReport),
+                              _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "rh_format_sci": (C.c_int32, [C.c_double, C.c_int32, C.c_char_p]),
 }
 
 _lib = None

@@ -41,7 +41,7 @@ const char* kNames[kNumSlots] = {
     "pcg_sgs_black", "pcg_sgs_red",     "advect_velocity",   "build_rhs",     "correct",
     "divergence_max", "mg_smooth",      "mg_residual",       "mg_restrict",   "mg_prolong",
     "mg_coarse_levels", "mg_outer_residual", "outer_update",   "batch_step",    "mg_smooth_update",
-    "mg_smooth_first"};
+    "mg_smooth_first", "dense3d_count", "dense3d_write"};
 
 }  // namespace
 

@@ -35,6 +35,8 @@ enum Slot {
   kBatch,
   kMgSmoothUpdate,
   kMgSmoothFirst,
+  kDense3dCount,
+  kDense3dWrite,
   kNumSlots
 };
 
