@@ -323,6 +323,103 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def run_dense3d(n: int = 256, reps: int = 5, cpu: bool = True) -> dict:
+    """Dense3d frame of an n^3 accepted transport state (write_dense3d_frame,
+    src/dense3d_sequence.rs:415-564): three f32 face arrays and the f32
+    tracer printed with 10 significant digits, the f64 fraction and pressure
+    with 18, every byte formatted on the device (rh_dense3d_frame) from
+    device-resident fields into a device buffer.  Synthetic fields: normal
+    faces and pressure, uniform tracer and fraction.  Timed: `reps` whole
+    calls (preflight pass, count, scan, text pieces, write; host wall clock
+    around the synchronous call) and the count and write kernels (HIP events).
+    CPU baseline: the oracle's serialization (glibc's exact conversion, one
+    thread, as the reference's writer) on a 64^3 frame, scaled per number."""
+    import ctypes as C
+    spec = GridSpec.unit_box(n)
+    rng = np.random.default_rng(256)
+    cells = n ** 3
+    flen = [(n + (a == 0)) * (n + (a == 1)) * (n + (a == 2)) for a in range(3)]
+    vel = [rng.standard_normal(flen[a], dtype=np.float32) for a in range(3)]
+    host = (vel, rng.random(cells, dtype=np.float32), rng.random(cells), rng.standard_normal(cells))
+    dev = [R.DeviceArray.from_host(a) for a in host[0]] + [R.DeviceArray.from_host(a) for a in host[1:]]
+    state = R.Dense3dState(velocity=tuple(dev[:3]), tracer=dev[3], fraction=dev[4], pressure=dev[5],
+                           carrier_stamp=(43, 0), liquid_stamp=(41, 0))
+    try:
+        R.dense3d_frame_device(spec, state, 0, None, None)
+    except R.RheonError as e:
+        needed = e.needed
+    out = R.DeviceArray(needed, np.uint8)
+    numbers = sum(flen) + 3 * cells
+    lib = N.lib()
+    R.dense3d_frame_device(spec, state, 0, None, out)  # warm-up
+    lib.rh_profile_focus(None)
+    lib.rh_profile_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nbytes = R.dense3d_frame_device(spec, state, 0, None, out)
+    wall = (time.perf_counter() - t0) / reps
+    ks = {}
+    for name in ("dense3d_count", "dense3d_write"):
+        n_l, sec = C.c_int64(), C.c_double()
+        if lib.rh_profile_query(name.encode(), C.byref(n_l), C.byref(sec)) == 0 and n_l.value:
+            ks[name] = sec.value / reps
+    lib.rh_profile_enable(0)
+    head = out.download()[:64].tobytes().decode(errors="replace")
+    # write kernel: reads every field value and writes the line's bytes
+    read_bytes = 4 * (sum(flen) + cells) + 8 * 2 * cells
+    write_s = ks.get("dense3d_write")
+    res = {
+        "value": 1.0 / wall, "unit": "frames/s",
+        "workload": f"dense3d frame {n}^3 ({numbers} numbers: 3 f32 face arrays + f32 tracer at 10 digits, "
+                    "f64 fraction + pressure at 18), device fields -> device line",
+        "frame_bytes": nbytes, "ms_per_frame": wall * 1e3, "numbers_per_s": numbers / wall,
+        "line_GBps": nbytes / wall / 1e9,
+        "kernels_ms": {k: v * 1e3 for k, v in ks.items()},
+        "head": head[:48],
+    }
+    if write_s:
+        ach = (read_bytes + nbytes) / write_s / 1e9
+        res["write_kernel_roofline"] = {"bound": "valu", "hbm_achieved_GBps": ach, "hbm_frac": ach / HBM_PEAK_GBS,
+                                        "algorithmic_bytes": read_bytes + nbytes}
+    if cpu:
+        import oracle as O
+        m = 64
+        sflen = [(m + (a == 0)) * (m + (a == 1)) * (m + (a == 2)) for a in range(3)]
+        srng = np.random.default_rng(64)
+        sv = [srng.standard_normal(sflen[a], dtype=np.float32) for a in range(3)]
+        sc = m ** 3
+        args_ = (sv, srng.random(sc, dtype=np.float32), srng.random(sc), srng.standard_normal(sc))
+        O.dense3d_frame((m, m, m), *args_, frame=0)  # warm-up
+        times = []
+        for _ in range(3):
+            c0 = time.perf_counter()
+            O.dense3d_frame((m, m, m), *args_, frame=0, threads=1)
+            times.append(time.perf_counter() - c0)
+        per_number = float(np.median(times)) / (sum(sflen) + 3 * sc)
+        res["cpu_baseline"] = {"value": 1.0 / (per_number * numbers), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"{m}^3 frame (oracle: glibc exact conversion, one thread, median of 3), "
+                                         f"scaled per number to {n}^3",
+                               "numbers_per_s": 1.0 / per_number}
+    for d in dev:
+        d.free()
+    out.free()
+    return res
+
+
+def bench_dense3d(args) -> int:
+    r = run_dense3d(args.n, args.steps, cpu=not args.no_cpu_baseline)
+    result = {"metric": "dense3d frames/s (write_dense3d_frame) at 256^3", "value": r["value"], "unit": "frames/s",
+              "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": r["ms_per_frame"],
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64 -> u8",
+              "data": "synthetic", "config": {k: v for k, v in r.items() if k not in ("value", "unit")}}
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    return 0
+
+
 def bench_batch(args) -> int:
     result = run_batch(args)
     line = json.dumps(result)
@@ -664,7 +761,7 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once with the plain copy kernel (the known-size "
                          "reference for hardware traffic counters) and exit")
-    ap.add_argument("--workload", default="scene", choices=["scene", "batch", "serve", "config2"],
+    ap.add_argument("--workload", default="scene", choices=["scene", "batch", "serve", "config2", "dense3d"],
                     help="scene: the 256^3 projection step (default); batch: 64 batched scenes; "
                          "serve: the continuous-batching serving loop")
     ap.add_argument("--scenes", type=int, default=64)
@@ -684,6 +781,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "config2":
         return bench_config2(args)
+    if args.workload == "dense3d":
+        return bench_dense3d(args)
     if args.workload == "batch":
         if args.steps == 5 and "--steps" not in sys.argv:
             args.steps = 10
@@ -899,6 +998,9 @@ def main() -> int:
                                           block_divisor=8, json_out=None))
         c = sv["config"]
         result["serve_budget_bound"] = {"value": sv["value"], "unit": sv["unit"], **{k: c[k] for k in keys}}
+        # dataset output: one dense3d frame of a 256^3 state (--workload dense3d)
+        d3 = run_dense3d(256, 5, cpu=not args.no_cpu_baseline)
+        result["dense3d"] = {k: v for k, v in d3.items() if k != "head"}
     line = json.dumps(result)
     print(line, flush=True)
     if args.json_out:

@@ -970,6 +970,15 @@ class Dense3dReport:
     volume_rounding_budget: float = 0.0
 
 
+class Dense3dShapeError(RheonError):
+    """A field array whose length is not the grid's (Dense3dError::InvalidState
+    "face shape" / "cell shape"); raised before anything reaches the device."""
+
+    def __init__(self, msg: str):
+        RuntimeError.__init__(self, f"write_dense3d_frame: INVALID_STATE {msg}")
+        self.status = INVALID_STATE
+
+
 def format_sci(v: float, digits: int) -> str:
     """Rust's `{v:.(digits-1)e}` on the host (the frame's number format)."""
     buf = C.create_string_buffer(32)
@@ -979,8 +988,30 @@ def format_sci(v: float, digits: int) -> str:
     return buf.value.decode()
 
 
-def _dense3d_structs(state: Dense3dState, report):
+def _dense3d_structs(spec: GridSpec, state: Dense3dState, report):
+    """The C view and report; every array's length is checked against the
+    grid first (the reference's "face shape" / "cell shape" refusals,
+    src/dense3d_sequence.rs:431-442), since the device reads each array to
+    the grid's length."""
     keep = []
+    n = [int(c) for c in spec.counts]
+
+    def length(a):
+        return a.count if isinstance(a, DeviceArray) else int(np.asarray(a).size)
+
+    def refuse(msg):
+        raise Dense3dShapeError(msg)
+
+    for axis in range(3):
+        if length(state.velocity[axis]) != (n[0] + (axis == 0)) * (n[1] + (axis == 1)) * (n[2] + (axis == 2)):
+            refuse("face shape")
+    cells = n[0] * n[1] * n[2]
+    if not all(length(a) == cells for a in (state.tracer, state.fraction, state.pressure)):
+        refuse("cell shape")
+    for a, dt in ((state.velocity[0], np.float32), (state.velocity[1], np.float32), (state.velocity[2], np.float32),
+                  (state.tracer, np.float32), (state.fraction, np.float64), (state.pressure, np.float64)):
+        if isinstance(a, DeviceArray) and a.dtype != np.dtype(dt):
+            raise TypeError(f"dense3d field arrays: expected {np.dtype(dt)}, got {a.dtype}")
 
     def dev(a, dtype):
         if isinstance(a, DeviceArray):
@@ -1018,7 +1049,7 @@ def dense3d_frame_device(spec: GridSpec, state: Dense3dState, frame: int, report
     the line's length; with ``out`` None or too small raises BUFFER_LIMIT
     after setting ``needed`` on the error."""
     g = make_grid(spec)
-    view, rep, keep = _dense3d_structs(state, report)
+    view, rep, keep = _dense3d_structs(spec, state, report)
     n = C.c_uint64()
     st = lib().rh_dense3d_frame(C.byref(g), C.byref(view), int(frame), C.byref(rep) if rep is not None else None,
                                 out.ptr if out is not None else None, out.nbytes if out is not None else 0,

@@ -37,7 +37,7 @@ namespace rh {
 namespace {
 
 constexpr int kD3Threads = 256;
-constexpr int kD3PerThread = 8;
+constexpr int kD3PerThread = 4;
 constexpr int kChunk = kD3Threads * kD3PerThread;  // values per workgroup
 constexpr unsigned kBadVelocity = 1u, kBadCell = 2u;
 
@@ -78,7 +78,7 @@ k_d3_count(const T* __restrict__ vals, long long n, unsigned* __restrict__ chunk
   __shared__ unsigned red[kD3Threads / 64];
   const long long base = (long long)blockIdx.x * kChunk;
   unsigned mine = 0;
-#pragma unroll
+#pragma unroll 1
   for (int q = 0; q < kD3PerThread; ++q) {
     const long long i = base + q * kD3Threads + threadIdx.x;
     if (i < n) mine += (unsigned)value_bytes<T, N>(vals, i);
@@ -120,33 +120,47 @@ __global__ void __launch_bounds__(1024) k_d3_scan(const unsigned* __restrict__ i
   }
 }
 
-// Formats chunk blockIdx.x into out + base + chunk offset.
+// Formats chunk blockIdx.x into out + base + chunk offset.  Value q of
+// thread t is chunk value q * kD3Threads + t, so a wave's LDS writes fall on
+// consecutive numbers (spread over the banks); each value's offset comes
+// from an exclusive scan of the chunk's lengths in value order.
 template <typename T, int N>
 __global__ void __launch_bounds__(kD3Threads)
 k_d3_write(const T* __restrict__ vals, long long n, const unsigned long long* __restrict__ chunk_off,
            unsigned long long base, char* __restrict__ out) {
   constexpr int kMaxChars = (N > 10 ? d3::kMaxF64Chars : d3::kMaxF32Chars) + 1;
   __shared__ __align__(16) char buf[kChunk * kMaxChars + 32];
+  __shared__ unsigned long long sD[kChunk];
+  __shared__ short sE[kChunk];
+  __shared__ unsigned voff[kChunk];  // lengths, then offsets within the chunk (bit 31: negative)
   __shared__ unsigned wsum[kD3Threads / 64];
   const long long first = (long long)blockIdx.x * kChunk;
   const unsigned long long start = base + chunk_off[blockIdx.x];
   const int lead = (int)(start & 15u);  // buf offset with start's 16-byte phase
-  // the thread's values are consecutive (q-major within the thread) so the
-  // scan over threads gives each its first byte
-  d3::Sci sc[kD3PerThread];
-  int len[kD3PerThread];
+#pragma unroll 1
+  for (int q = 0; q < kD3PerThread; ++q) {
+    const int v = q * kD3Threads + threadIdx.x;
+    const long long i = first + v;
+    unsigned len = 0;
+    if (i < n) {
+      const d3::Sci sc = d3::decompose<N>(widen(vals[i]));
+      sD[v] = sc.D;
+      sE[v] = (short)sc.E;
+      len = (unsigned)(d3::sci_length(sc, N) + (i > 0 ? 1 : 0)) | (sc.neg ? 0x80000000u : 0u);
+    }
+    voff[v] = len;
+  }
+  __syncthreads();
+  // exclusive scan of the lengths: thread t owns entries t*8 .. t*8+7
   unsigned mine = 0;
+  unsigned loc[kD3PerThread], neg[kD3PerThread];
 #pragma unroll
   for (int q = 0; q < kD3PerThread; ++q) {
-    const long long i = first + (long long)threadIdx.x * kD3PerThread + q;
-    len[q] = 0;
-    if (i < n) {
-      sc[q] = d3::decompose<N>(widen(vals[i]));
-      len[q] = d3::sci_length(sc[q], N) + (i > 0 ? 1 : 0);
-    }
-    mine += (unsigned)len[q];
+    const unsigned w = voff[threadIdx.x * kD3PerThread + q];
+    loc[q] = mine;
+    neg[q] = w & 0x80000000u;
+    mine += w & 0x7fffffffu;
   }
-  // exclusive scan over the block's threads
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   unsigned incl = mine;
   for (int m = 1; m < 64; m <<= 1) {
@@ -159,13 +173,19 @@ k_d3_write(const T* __restrict__ vals, long long n, const unsigned long long* __
   for (int k = 0; k < wave; ++k) off += wsum[k];
   unsigned total = 0;
   for (int k = 0; k < kD3Threads / 64; ++k) total += wsum[k];
-  char* p = buf + lead + off;
 #pragma unroll
+  for (int q = 0; q < kD3PerThread; ++q) voff[threadIdx.x * kD3PerThread + q] = (off + loc[q]) | neg[q];
+  __syncthreads();
+#pragma unroll 1
   for (int q = 0; q < kD3PerThread; ++q) {
-    const long long i = first + (long long)threadIdx.x * kD3PerThread + q;
+    const int v = q * kD3Threads + threadIdx.x;
+    const long long i = first + v;
     if (i >= n) break;
+    const unsigned w = voff[v];
+    char* p = buf + lead + (w & 0x7fffffffu);
     if (i > 0) *p++ = ',';
-    p += d3::sci_write(sc[q], N, p);
+    const d3::Sci sc{sD[v], (int)sE[v], (w & 0x80000000u) != 0};
+    d3::sci_write(sc, N, p);
   }
   __syncthreads();
   // copy [start, start + total) out: 16-byte stores on the aligned body
@@ -381,5 +401,6 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
   }
   RH_HIP(hipGetLastError());
   RH_HIP(hipStreamSynchronize(s));
+  if (prof::enabled()) prof::settle(INT32_MAX);
   return RH_OK;
 }

@@ -16,8 +16,8 @@
 //   * otherwise (tiny or huge magnitudes): the same quotient in multi-word
 //     integers (shift, or long division by 5^c 2^d when N-1-E < 0).
 //
-// E starts from log10 and is corrected until D has exactly N digits; a
-// round-up to 10^N becomes 10^(N-1) with E + 1.
+// E starts from an estimate at most one below and is corrected until D has
+// exactly N digits; a round-up to 10^N becomes 10^(N-1) with E + 1.
 #pragma once
 #include <stdint.h>
 
@@ -147,10 +147,32 @@ RH_HD uint64_t big_divmod(Big& num, const Big& den) {
   return q;
 }
 
+// The multi-word form of scale_digits (magnitudes outside the fast path),
+// kept out of line so its registers do not weigh on the fast path.
+RH_HD uint64_t scale_digits_big(uint64_t m, int e, int s, int* round_up);
+
 // D_trunc = floor(|v| 10^s) and its rounding: 1 when it rounds up, ties to
 // even.  m, e: |v| = m 2^e.
 RH_HD uint64_t scale_digits(uint64_t m, int e, int s, int* round_up) {
   const int g = e + s;
+  if (s >= 0 && s <= 16 && m < (1ull << 26)) {
+    // f32 mantissas at moderate magnitudes: the product fits 64 bits
+    const uint64_t p = m * pow5u(s);
+    if (g >= 0) {
+      *round_up = 0;
+      return p << g;
+    }
+    const int d = -g;
+    if (d >= 64) {
+      *round_up = 0;
+      return 0;
+    }
+    const uint64_t q = p >> d;
+    const uint64_t rem = p & ((1ull << d) - 1);
+    const uint64_t half = d == 0 ? 0 : 1ull << (d - 1);
+    *round_up = d > 0 && (rem > half || (rem == half && (q & 1u)));
+    return q;
+  }
   if (s >= 0 && s <= 27) {
     const unsigned __int128 p = (unsigned __int128)m * pow5u(s);
     if (g >= 0) {
@@ -168,6 +190,11 @@ RH_HD uint64_t scale_digits(uint64_t m, int e, int s, int* round_up) {
     *round_up = d > 0 && (rem > half || (rem == half && (q & 1u)));
     return q;
   }
+  return scale_digits_big(m, e, s, round_up);
+}
+
+RH_HD uint64_t scale_digits_big(uint64_t m, int e, int s, int* round_up) {
+  const int g = e + s;
   Big num;
   num.set(m);
   if (s >= 0) {
@@ -213,7 +240,11 @@ RH_HD Sci decompose(double v) {
   if (bexp == 0 && frac == 0) return r;
   const uint64_t m = bexp ? (frac | (1ull << 52)) : frac;
   const int e = bexp ? bexp - 1075 : -1074;
-  int E = (int)floor(log10(r.neg ? -v : v));
+  // E from below: log10 v = (b + log2(1 + x)) log10(2) with x the mantissa
+  // fraction and log2(1 + x) >= x on [0, 1], so the estimate is E or E - 1
+  // (the loop below adds the one); subnormals take log10.
+  int E = bexp ? (int)floor(((double)(bexp - 1023) + (double)frac * 0x1p-52) * 0.30102999566398120)
+               : (int)floor(log10(r.neg ? -v : v));
   const uint64_t lo = pow10u(n - 1), hi = lo * 10u;
   int up = 0;
   uint64_t D = 0;

@@ -334,3 +334,20 @@ def test_device_frame_256_equals_oracle():
     ref = oracle_frame(n, state, 0, threads=min(16, os.cpu_count() or 1))
     assert len(dev) == len(ref)
     assert hashlib.sha256(dev).digest() == hashlib.sha256(ref).digest()
+
+
+@pytest.mark.parametrize("bad", ["face", "cell"])
+def test_shape_refused_before_the_device(bad):
+    """Arrays shorter than the grid's are refused on the host ("face shape" /
+    "cell shape", src/dense3d_sequence.rs:431-442): the device kernels read
+    every array to the grid's length."""
+    n = (6, 5, 4)
+    vel, tracer, fraction, pressure = make_state(n, 8)
+    if bad == "face":
+        vel[1] = vel[1][:-1]
+    else:
+        pressure = pressure[:-1]
+    spec = GridSpec(n, tuple(1.0 / c for c in n))
+    st = R.Dense3dState(velocity=tuple(vel), tracer=tracer, fraction=fraction, pressure=pressure)
+    with pytest.raises(R.Dense3dShapeError, match=f"{bad} shape"):
+        R.write_dense3d_frame(spec, st, 0)
