@@ -46,43 +46,30 @@ __device__ __forceinline__ double widen(T v) {
   return (double)v;
 }
 
+// The frame's field checks (src/dense3d_sequence.rs:431-450), made on the
+// values the count pass reads anyway: velocity and tracer finite, pressure
+// finite, fraction finite and in [0, 1].  kFraction selects the last.
+template <typename T, int N, bool kFraction>
 __global__ void __launch_bounds__(kD3Threads)
-k_d3_check(const float* __restrict__ u, long long nu, const float* __restrict__ v, long long nv,
-           const float* __restrict__ w, long long nw, const float* __restrict__ tracer,
-           const double* __restrict__ fraction, const double* __restrict__ pressure, long long nc,
+k_d3_count(const T* __restrict__ vals, long long n, unsigned* __restrict__ chunk_bytes, unsigned bad_flag,
            unsigned* flags) {
-  unsigned f = 0;
-  const long long stride = (long long)gridDim.x * kD3Threads;
-  for (long long i = (long long)blockIdx.x * kD3Threads + threadIdx.x; i < nu; i += stride)
-    if (!isfinite(u[i])) f |= kBadVelocity;
-  for (long long i = (long long)blockIdx.x * kD3Threads + threadIdx.x; i < nv; i += stride)
-    if (!isfinite(v[i])) f |= kBadVelocity;
-  for (long long i = (long long)blockIdx.x * kD3Threads + threadIdx.x; i < nw; i += stride)
-    if (!isfinite(w[i])) f |= kBadVelocity;
-  for (long long i = (long long)blockIdx.x * kD3Threads + threadIdx.x; i < nc; i += stride) {
-    const double fr = fraction[i];
-    if (!isfinite(tracer[i]) || !isfinite(pressure[i]) || !(fr >= 0.0 && fr <= 1.0)) f |= kBadCell;
-  }
-  if (f) atomicOr(flags, f);
-}
-
-// Bytes of value i of an array: its number, preceded by ',' unless i = 0.
-template <typename T, int N>
-__device__ __forceinline__ int value_bytes(const T* __restrict__ vals, long long i) {
-  return d3::sci_length(d3::decompose<N>(widen(vals[i])), N) + (i > 0 ? 1 : 0);
-}
-
-template <typename T, int N>
-__global__ void __launch_bounds__(kD3Threads)
-k_d3_count(const T* __restrict__ vals, long long n, unsigned* __restrict__ chunk_bytes) {
   __shared__ unsigned red[kD3Threads / 64];
   const long long base = (long long)blockIdx.x * kChunk;
   unsigned mine = 0;
+  bool bad = false;
 #pragma unroll 1
   for (int q = 0; q < kD3PerThread; ++q) {
     const long long i = base + q * kD3Threads + threadIdx.x;
-    if (i < n) mine += (unsigned)value_bytes<T, N>(vals, i);
+    if (i >= n) continue;
+    const double v = widen(vals[i]);
+    const bool ok = kFraction ? (v >= 0.0 && v <= 1.0) : (bool)isfinite(v);
+    if (!ok) {
+      bad = true;
+      continue;
+    }
+    mine += (unsigned)(d3::sci_length(d3::decompose<N>(v), N) + (i > 0 ? 1 : 0));
   }
+  if (bad) atomicOr(flags, bad_flag);
   for (int m = 32; m >= 1; m >>= 1) mine += __shfl_xor(mine, m, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mine;
   __syncthreads();
@@ -227,6 +214,31 @@ struct Field {
 
 using namespace rh;
 
+// The view's clock and stamps and the report (src/dense3d_sequence.rs:
+// 452-495), after the field checks as in the reference.
+static rh_status header_checks(const rh_dense3d_view* view, uint64_t frame, const rh_dense3d_report* report) {
+  if (!(std::isfinite(view->carrier_time) && view->carrier_time == view->liquid_time &&
+        view->carrier_stamp.version == view->carrier_generation))
+    return invalid_state("state clock/stamp");
+  if (report) {
+    const rh_dense3d_report& r = *report;
+    if (!(r.carrier_time == view->carrier_time && r.liquid_time == r.carrier_time && r.carrier_dt == r.liquid_dt &&
+          r.carrier_dt > 0.0 && r.carrier_generation == view->carrier_generation &&
+          r.liquid_stamp.id == view->liquid_stamp.id && r.liquid_stamp.version == view->liquid_stamp.version &&
+          r.liquid_flow.id == view->carrier_stamp.id && r.liquid_flow.version == view->carrier_stamp.version))
+      return invalid_state("report continuity");
+    const double vals[12] = {r.carrier_dt, r.pressure_true_residual_max, r.carrier_actual_divergence_max,
+                             r.liquid_actual_divergence_max, r.liquid_volume_before, r.liquid_volume_after,
+                             r.liquid_mass_after, r.inward_boundary_volume, r.outward_boundary_volume,
+                             r.source_volume, r.volume_balance_error, r.volume_rounding_budget};
+    for (double v : vals)
+      if (!std::isfinite(v)) return invalid_state("nonfinite report");
+  } else if (!(frame == 0 && view->carrier_time == 0.0 && view->carrier_generation == 0)) {
+    return invalid_state("initial frame");
+  }
+  return RH_OK;
+}
+
 extern "C" int32_t rh_format_sci(double v, int32_t digits, char* out) {
   if (!out || (digits != 10 && digits != 18) || !std::isfinite(v)) return -1;
   const std::string s = sci_host(v, digits);
@@ -259,36 +271,6 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
   if ((st = scr.init())) return st;
   hipStream_t s = scr.stream;
   RH_HIP(hipMemsetAsync(scr.flags, 0, sizeof(unsigned), s));
-  {
-    long long most = std::max(std::max(fl[0], fl[1]), std::max(fl[2], nc));
-    const int blocks = (int)std::min<long long>(4096, (most + kD3Threads - 1) / kD3Threads);
-    k_d3_check<<<blocks, kD3Threads, 0, s>>>(view->velocity[0], fl[0], view->velocity[1], fl[1], view->velocity[2],
-                                              fl[2], view->tracer, view->fraction, view->pressure, nc, scr.flags);
-    RH_HIP(hipGetLastError());
-  }
-  unsigned f = 0;
-  if ((st = scr.read_flags(&f))) return st;
-  if (f & kBadVelocity) return invalid_state("nonfinite velocity");
-  if (f & kBadCell) return invalid_state("nonfinite or invalid cell field");
-  if (!(std::isfinite(view->carrier_time) && view->carrier_time == view->liquid_time &&
-        view->carrier_stamp.version == view->carrier_generation))
-    return invalid_state("state clock/stamp");
-  if (report) {
-    const rh_dense3d_report& r = *report;
-    if (!(r.carrier_time == view->carrier_time && r.liquid_time == r.carrier_time && r.carrier_dt == r.liquid_dt &&
-          r.carrier_dt > 0.0 && r.carrier_generation == view->carrier_generation &&
-          r.liquid_stamp.id == view->liquid_stamp.id && r.liquid_stamp.version == view->liquid_stamp.version &&
-          r.liquid_flow.id == view->carrier_stamp.id && r.liquid_flow.version == view->carrier_stamp.version))
-      return invalid_state("report continuity");
-    const double vals[12] = {r.carrier_dt, r.pressure_true_residual_max, r.carrier_actual_divergence_max,
-                             r.liquid_actual_divergence_max, r.liquid_volume_before, r.liquid_volume_after,
-                             r.liquid_mass_after, r.inward_boundary_volume, r.outward_boundary_volume,
-                             r.source_volume, r.volume_balance_error, r.volume_rounding_budget};
-    for (double v : vals)
-      if (!std::isfinite(v)) return invalid_state("nonfinite report");
-  } else if (!(frame == 0 && view->carrier_time == 0.0 && view->carrier_generation == 0)) {
-    return invalid_state("initial frame");
-  }
   // the text around the field arrays (src/dense3d_sequence.rs:497-563)
   const std::string head = "{\"frame\":" + std::to_string(frame) + ",\"time_s\":" + sci_host(view->carrier_time, 18) +
                            ",\"dt_s\":" + sci_host(report ? report->carrier_dt : 0.0, 18) +
@@ -340,12 +322,16 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
   for (int q = 0; q < 6; ++q) {
     if (!nchunks[q]) continue;
     prof::Scope ps(prof::kDense3dCount, s, -1);
-    if (fields[q].f64)
-      launch(k_d3_count<double, 18>, dim3((unsigned)nchunks[q]), dim3(kD3Threads), 0, s, (const double*)fields[q].data,
-             fields[q].n, counts + coff[q]);
+    const unsigned flag = q < 3 ? kBadVelocity : kBadCell;
+    if (q == 4)
+      launch(k_d3_count<double, 18, true>, dim3((unsigned)nchunks[q]), dim3(kD3Threads), 0, s,
+             (const double*)fields[q].data, fields[q].n, counts + coff[q], flag, scr.flags);
+    else if (fields[q].f64)
+      launch(k_d3_count<double, 18, false>, dim3((unsigned)nchunks[q]), dim3(kD3Threads), 0, s,
+             (const double*)fields[q].data, fields[q].n, counts + coff[q], flag, scr.flags);
     else
-      launch(k_d3_count<float, 10>, dim3((unsigned)nchunks[q]), dim3(kD3Threads), 0, s, (const float*)fields[q].data,
-             fields[q].n, counts + coff[q]);
+      launch(k_d3_count<float, 10, false>, dim3((unsigned)nchunks[q]), dim3(kD3Threads), 0, s,
+             (const float*)fields[q].data, fields[q].n, counts + coff[q], flag, scr.flags);
   }
   for (int q = 0; q < 6; ++q)
     if (nchunks[q]) k_d3_scan<<<1, 1024, 0, s>>>(counts + coff[q], (int)nchunks[q], offs + coff[q] + q);
@@ -356,7 +342,11 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
     if (nchunks[q])
       RH_HIP(hipMemcpyAsync(&tot[q], offs + coff[q] + q + nchunks[q], sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, s));
-  RH_HIP(hipStreamSynchronize(s));
+  unsigned f = 0;
+  if ((st = scr.read_flags(&f))) return st;  // synchronises the stream
+  if (f & kBadVelocity) return invalid_state("nonfinite velocity");
+  if (f & kBadCell) return invalid_state("nonfinite or invalid cell field");
+  if ((st = header_checks(view, frame, report))) return st;
   // layout: head, then per field "name":[ values ] and the separator
   std::vector<std::string> pre(6), post(6);
   for (int q = 0; q < 6; ++q) {
