@@ -889,11 +889,11 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
   if (s) return s;
   RH_HIP(hipEventRecord(sc->ev[4], st));
   RH_HIP(hipGetLastError());
-  if ((s = scr.read_flags(&f))) return s;
-  if (f) return rh::flags_to_status(f);
-  // [0] max|div u|, [1] rate, [2..4] component maxima, [5] sum of squared faces
+  // one read-back: the correction's flags and [0] max|div u|, [1] rate,
+  // [2..4] component maxima, [5] sum of squared faces
   double sv[6];
-  if ((s = scr.read_scalars(sv, 6, 0))) return s;
+  if ((s = scr.read_flags_scalars(&f, sv, 6))) return s;
+  if (f) return rh::flags_to_status(f);
   if (rh::prof::enabled()) rh::prof::settle(INT32_MAX);
   float ms[4] = {0, 0, 0, 0}, total = 0;
   for (int i = 0; i < 4; ++i) RH_HIP(hipEventElapsedTime(&ms[i], sc->ev[i], sc->ev[i + 1]));

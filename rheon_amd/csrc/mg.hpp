@@ -229,7 +229,10 @@ void mg_outer_residual(const DevGrid& g, const MgHierarchy& h, const double* p, 
 template <typename S>
 void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hipStream_t s, bool x0_implicit,
               bool p_zero = false, const MgFold* fold = nullptr);
-void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s);
+// p[gauge] subtracted from the solve's pressure when the solve ended OK or
+// at its iteration limit (solid cells 0); p = 0 when no cycle ran and
+// p_unwritten; decided on the device, so no host round trip precedes it.
+void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s, bool p_unwritten);
 int mg_tail_lds_limit();  // dynamic LDS bytes the tail kernels may use
 
 }  // namespace rh

@@ -715,18 +715,31 @@ k_mg_add(DevGrid g, const S* __restrict__ x, double* __restrict__ p, const PcgCo
 // p <- p - p0 with p0 = p[gauge] captured in ctl->spare[0] beforehand (a
 // split fluid: each region's p at its own gauge, captured per region);
 // solid cells (not unknowns) are set to 0.
-__global__ void k_capture_gauge(const double* __restrict__ p, int gauge, PcgControl* ctl) {
-  if (threadIdx.x == 0 && threadIdx.y == 0) ctl->spare[0] = p[gauge];
+// The gauge shift of the returned pressure, decided on the device from the
+// solve's terminal state: a solve that ended OK or at its iteration limit
+// has p[gauge] subtracted (solid cells set to 0); when no cycle ran and p
+// was never written (p_unwritten: the first cycle would have written all
+// of it), p becomes 0.  Any other outcome leaves p as it is.
+__device__ __forceinline__ int regauge_mode(const PcgControl* ctl, bool p_unwritten) {
+  if (p_unwritten && ctl->iteration == 0) return 2;  // zero-fill
+  return ctl->done && (ctl->status == RH_OK || ctl->status == RH_ITERATION_LIMIT) ? 1 : 0;
 }
-__global__ void k_capture_comp_gauges(DevGrid g, const double* __restrict__ p) {
+__global__ void k_capture_gauge(const double* __restrict__ p, int gauge, PcgControl* ctl, bool p_unwritten) {
+  if (threadIdx.x == 0 && threadIdx.y == 0 && regauge_mode(ctl, p_unwritten) == 1) ctl->spare[0] = p[gauge];
+}
+__global__ void k_capture_comp_gauges(DevGrid g, const double* __restrict__ p, const PcgControl* ctl,
+                                      bool p_unwritten) {
+  if (regauge_mode(ctl, p_unwritten) != 1) return;
   for (int k = threadIdx.x; k < g.ncomp; k += blockDim.x) g.comp_shift[k] = p[g.comp_gauge[k]];
 }
 
 __global__ void __launch_bounds__(kBlock)
-k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl) {
+k_regauge(DevGrid g, double* __restrict__ p, const PcgControl* ctl, bool p_unwritten) {
+  const int mode = regauge_mode(ctl, p_unwritten);
+  if (!mode) return;
   const double p0 = ctl->spare[0];
   FOR_CELLS(g)
-  p[c] = is_fluid(g, c) ? p[c] - (g.ncomp > 1 ? g.comp_shift[g.comp_label[c]] : p0) : 0.0;
+  p[c] = mode == 2 ? 0.0 : (is_fluid(g, c) ? p[c] - (g.ncomp > 1 ? g.comp_shift[g.comp_label[c]] : p0) : 0.0);
   END_CELLS
 }
 
@@ -960,12 +973,12 @@ void mg_cycle(const MgHierarchy& h, double* p, const PcgControl* ctl, int it, hi
   vcycle<S>(h, 0, p, ctl, it, s, x0_implicit, p_zero, fold);
 }
 
-void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s) {
+void mg_regauge(const DevGrid& g, double* p, PcgControl* ctl, hipStream_t s, bool p_unwritten) {
   if (g.ncomp > 1)
-    k_capture_comp_gauges<<<1, 256, 0, s>>>(g, p);
+    k_capture_comp_gauges<<<1, 256, 0, s>>>(g, p, ctl, p_unwritten);
   else
-    k_capture_gauge<<<1, 64, 0, s>>>(p, g.gauge, ctl);
-  k_regauge<<<cells_grid(g), block_shape(), 0, s>>>(g, p, ctl);
+    k_capture_gauge<<<1, 64, 0, s>>>(p, g.gauge, ctl, p_unwritten);
+  k_regauge<<<cells_grid(g), block_shape(), 0, s>>>(g, p, ctl, p_unwritten);
 }
 
 template int mg_outer_blocks<float>(const DevGrid&, const MgHierarchy&, const double*, const double*);

@@ -296,6 +296,17 @@ rh_status Scratch::read_flags(unsigned* out) {
   return RH_OK;
 }
 
+rh_status Scratch::read_flags_scalars(unsigned* f, double* out, int n) {
+  // the flag word lies below result slots 0..n-1 in one allocation
+  char* lo = (char*)flags;
+  const size_t span = (char*)(scal + n) - lo;
+  RH_HIP(hipMemcpyAsync(host, lo, span, hipMemcpyDeviceToHost, stream));
+  RH_HIP(hipStreamSynchronize(stream));
+  std::memcpy(f, host, sizeof(unsigned));
+  std::memcpy(out, (char*)host + ((char*)scal - lo), sizeof(double) * n);
+  return RH_OK;
+}
+
 rh_status Scratch::read_scalars(double* out, int n, int slot) {
   RH_HIP(hipMemcpyAsync(host, scal + slot, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   RH_HIP(hipStreamSynchronize(stream));

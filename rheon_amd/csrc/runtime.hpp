@@ -57,6 +57,8 @@ struct Scratch {
   rh_status clear_flags();
   rh_status read_flags(unsigned* out);
   rh_status read_scalars(double* out, int n, int slot);
+  // The flag word and result slots 0..n-1 in one copy.
+  rh_status read_flags_scalars(unsigned* f, double* out, int n);
 };
 
 Scratch& global_scratch(std::unique_lock<std::mutex>& lock);
@@ -109,7 +111,9 @@ class Solver {
   template <typename Step>
   rh_status drive_paced(int max_iterations, int poll, int ahead, Step step, bool* cancelled,
                         rh_cancel_fn cancel, void* ctx, int* enqueued);
-  rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out);
+  // Reads the control block back (and, with scal3, result slots 0..2 in
+  // the same copy) and fills the report.
+  rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out, double* scal3 = nullptr);
   void release_mask();
   void release_components();
   rh_status upload_components(const ObstacleComponents& comps);

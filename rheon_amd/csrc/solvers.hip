@@ -126,12 +126,24 @@ rh_status Solver::drive_paced(int max_iterations, int poll, int ahead, Step step
   return RH_OK;
 }
 
-rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out) {
+rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out, double* scal3) {
   hipStream_t s = scratch_.stream;
   PcgControl* c = (PcgControl*)scratch_.host + 1;
   RH_HIP(hipEventRecord(scratch_.ev1, s));
-  RH_HIP(hipMemcpyAsync(c, scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
-  RH_HIP(hipStreamSynchronize(s));
+  if (scal3) {
+    // the report sums (result slots 0..2) and the control block, which
+    // follow one another in the scratch allocation, in one copy
+    char* lo = (char*)scratch_.scal;
+    const size_t span = (char*)(scratch_.ctl + 1) - lo;
+    char* h = (char*)scratch_.host + 1024;
+    RH_HIP(hipMemcpyAsync(h, lo, span, hipMemcpyDeviceToHost, s));
+    RH_HIP(hipStreamSynchronize(s));
+    std::memcpy(scal3, h, 3 * sizeof(double));
+    std::memcpy(c, h + ((char*)scratch_.ctl - lo), sizeof(PcgControl));
+  } else {
+    RH_HIP(hipMemcpyAsync(c, scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
+    RH_HIP(hipStreamSynchronize(s));
+  }
   if (prof::enabled()) prof::settle(c->iteration);
   float ms = 0.0f;
   RH_HIP(hipEventElapsedTime(&ms, scratch_.ev0, scratch_.ev1));
@@ -1022,23 +1034,21 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   if (rs) return rs;
   // the last enqueued outer residual's acceptance (a no-op once decided)
   if (fold) launch_finish<1, 2>(scratch_.pdd, scratch_.pmx, nb, FinMgTrue{scratch_.ctl, enqueued, progress}, s);
-  // Fix the gauge p[0] = 0 (the reference's eliminated gauge cell) and
-  // report the true residual of the returned pressure.
-  PcgControl c;
-  rs = finish_report(report, cancelled, &c);
-  if (c.done) last_cycles_ = c.iteration;
-  if (p_zero && c.iteration == 0)  // no cycle ran: the result is p = 0
-    RH_HIP(hipMemsetAsync(d_pressure, 0, sizeof(double) * (size_t)g.nx * g.ny * g.nz, s));
-  if (rs == RH_ITERATION_LIMIT) mg_regauge(g, d_pressure, scratch_.ctl, s);
-  if (rs != RH_OK) return rs;
-  mg_regauge(g, d_pressure, scratch_.ctl, s);
+  // Fix the gauge p[0] = 0 (the reference's eliminated gauge cell), p = 0
+  // when no cycle ran, and report the true residual of the returned
+  // pressure -- all enqueued before the one read-back of the control block
+  // and the report's sums (the gauge shift acts only on an OK or
+  // iteration-limit outcome; the report is used only on OK).
+  mg_regauge(g, d_pressure, scratch_.ctl, s, p_zero);
   if (h_.quad && mg_quad_outer_ok(g, h_.lv[0], d_pressure, d_rhs))
     launch_residual_report_q(g, d_pressure, d_rhs, scratch_.reduce(0), s);
   else
     launch_residual_report(g, d_pressure, d_rhs, scratch_.reduce(0), s);
+  PcgControl c;
   double out[3];
-  rs = scratch_.read_scalars(out, 3, 0);
-  if (rs) return rs;
+  rs = finish_report(report, cancelled, &c, out);
+  if (c.done) last_cycles_ = c.iteration;
+  if (rs != RH_OK) return rs;
   if (report) {
     report->true_residual_l2 = std::sqrt(out[0]);
     report->true_residual_max = out[1];
