@@ -850,7 +850,7 @@ def main() -> int:
     lib.rh_profile_focus(dom.encode())
     lib.rh_profile_enable(1)
     lib.rh_device_synchronize()
-    lat, iters, divs, dts = [], [], [], []
+    lat, iters, divs, dts, solve_s = [], [], [], [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
@@ -860,6 +860,7 @@ def main() -> int:
             print(f"step failed: {R.STATUS_NAMES.get(rep.status)}", file=sys.stderr)
             return 1
         iters.append(rep.pressure.iterations)
+        solve_s.append(rep.pressure.seconds)
         divs.append(rep.actual_divergence_max)
         dts.append((rep.dt, rep.courant))
     lib.rh_device_synchronize()
@@ -928,6 +929,11 @@ def main() -> int:
             "latency_ms_p50": float(np.percentile(np.array(lat) * 1e3, 50)),
             "latency_ms_p99": float(np.percentile(np.array(lat) * 1e3, 99)),
             "algorithmic_GBps": step_bytes / (elapsed / args.steps) / 1e9,
+            # the pressure solve alone: its compulsory bytes over its device time
+            # (the solve report's seconds, HIP events around the solve's launches)
+            "solve_ms_mean": float(np.mean(solve_s)) * 1e3,
+            "solve_algorithmic_GBps": ab["solve"] / float(np.mean(solve_s)) / 1e9,
+            "solve_hbm_frac": ab["solve"] / float(np.mean(solve_s)) / 1e9 / HBM_PEAK_GBS,
             "cell_updates_per_s": spec.cell_len * args.steps / elapsed,
             "kernels_survey_step": {
                 k: {"launches": v[0], "avg_us": v[1] / v[0] * 1e6, "total_us": v[1] * 1e6,

@@ -9,11 +9,11 @@
 // numbers) with the same formatter; the field arrays, which are the frame's
 // bytes, are formatted on the device:
 //
-//   k_d3_check  one pass over every field: non-finite velocity, non-finite
-//               tracer or pressure, fraction outside [0, 1];
 //   k_d3_count  per chunk of kChunk values, the bytes of its numbers and
-//               their separating commas;
-//   k_d3_scan   chunk offsets (exclusive scan, one workgroup);
+//               their separating commas; on the same reads, the frame's
+//               field checks (non-finite velocity, tracer or pressure,
+//               fraction outside [0, 1]);
+//   k_d3_scan   chunk offsets of every field (one workgroup per field);
 //   k_d3_write  per chunk: the numbers formatted into LDS at their offsets,
 //               then copied out with 16-byte stores (head and tail bytes
 //               singly), so the line is written once, in order, unaligned.
@@ -80,27 +80,37 @@ k_d3_count(const T* __restrict__ vals, long long n, unsigned* __restrict__ chunk
   }
 }
 
-// Exclusive scan of nb chunk byte counts into 64-bit offsets; out[nb] = total.
-__global__ void __launch_bounds__(1024) k_d3_scan(const unsigned* __restrict__ in, int nb, unsigned long long* out) {
-  __shared__ unsigned long long part[1024];
-  const int t = threadIdx.x;
+// Exclusive scans of the fields' chunk byte counts into 64-bit offsets,
+// one workgroup per field: counts in[c0 .. c0 + nb) -> out[o0 .. o0 + nb),
+// out[o0 + nb] = totals[q] = the field's total.
+struct ScanJobs {
+  long long c0[6], o0[6];
+  int nb[6], q[6];
+};
+__global__ void __launch_bounds__(1024) k_d3_scan(const unsigned* __restrict__ counts, ScanJobs jobs,
+                                                  unsigned long long* __restrict__ offs,
+                                                  unsigned long long* __restrict__ totals) {
+  __shared__ unsigned long long wsum[16];
+  const int f = blockIdx.x;
+  const unsigned* in = counts + jobs.c0[f];
+  unsigned long long* out = offs + jobs.o0[f];
+  const int nb = jobs.nb[f];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int per = (nb + 1023) / 1024;
-  const int b0 = t * per, b1 = min(nb, b0 + per);
+  const int b0 = min(nb, t * per), b1 = min(nb, b0 + per);
   unsigned long long s = 0;
   for (int b = b0; b < b1; ++b) s += in[b];
-  part[t] = s;
-  __syncthreads();
-  if (t == 0) {
-    unsigned long long run = 0;
-    for (int k = 0; k < 1024; ++k) {
-      const unsigned long long x = part[k];
-      part[k] = run;
-      run += x;
-    }
-    out[nb] = run;
+  // inclusive scan over the workgroup: within each wave, then the wave sums
+  unsigned long long incl = s;
+  for (int m = 1; m < 64; m <<= 1) {
+    const unsigned long long y = __shfl_up(incl, m, 64);
+    if (lane >= m) incl += y;
   }
+  if (lane == 63) wsum[wave] = incl;
   __syncthreads();
-  unsigned long long run = part[t];
+  unsigned long long run = incl - s;
+  for (int k = 0; k < wave; ++k) run += wsum[k];
+  if (t == 1023) out[nb] = totals[jobs.q[f]] = run + s;
   for (int b = b0; b < b1; ++b) {
     out[b] = run;
     run += in[b];
@@ -138,7 +148,8 @@ k_d3_write(const T* __restrict__ vals, long long n, const unsigned long long* __
     voff[v] = len;
   }
   __syncthreads();
-  // exclusive scan of the lengths: thread t owns entries t*8 .. t*8+7
+  // exclusive scan of the lengths: thread t owns entries t*kD3PerThread ..
+  // t*kD3PerThread + kD3PerThread - 1
   unsigned mine = 0;
   unsigned loc[kD3PerThread], neg[kD3PerThread];
 #pragma unroll
@@ -309,16 +320,25 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
     coff[q] = total_chunks;
     total_chunks += nchunks[q];
   }
-  void* work = nullptr;
+  // counts, then (256-aligned) offsets with one total per field; the
+  // allocation is kept for later frames (grown when a larger grid comes),
+  // guarded by the global scratch's lock like the rest of this call
+  static void* work = nullptr;
+  static size_t work_bytes = 0;
   const size_t counts_bytes = sizeof(unsigned) * (size_t)total_chunks;
-  const size_t offs_bytes = sizeof(unsigned long long) * (size_t)(total_chunks + 6);
-  RH_HIP(hipMalloc(&work, counts_bytes + offs_bytes + 256));
-  struct Free {
-    void* p;
-    ~Free() { (void)hipFree(p); }
-  } guard{work};
+  const size_t offs_bytes = sizeof(unsigned long long) * (size_t)(total_chunks + 12);
+  const size_t need = ((counts_bytes + 255) / 256) * 256 + offs_bytes;
+  if (work_bytes < need) {
+    if (work) RH_HIP(hipFree(work));
+    work = nullptr;
+    work_bytes = 0;
+    RH_HIP(hipMalloc(&work, need));
+    work_bytes = need;
+  }
   unsigned* counts = (unsigned*)work;
   unsigned long long* offs = (unsigned long long*)((char*)work + ((counts_bytes + 255) / 256) * 256);
+  unsigned long long* totals = offs + total_chunks + 6;  // [6]: each field's total (0 when empty)
+  RH_HIP(hipMemsetAsync(totals, 0, 6 * sizeof(unsigned long long), s));
   for (int q = 0; q < 6; ++q) {
     if (!nchunks[q]) continue;
     prof::Scope ps(prof::kDense3dCount, s, -1);
@@ -333,17 +353,25 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
       launch(k_d3_count<float, 10, false>, dim3((unsigned)nchunks[q]), dim3(kD3Threads), 0, s,
              (const float*)fields[q].data, fields[q].n, counts + coff[q], flag, scr.flags);
   }
+  ScanJobs jobs{};
+  int njobs = 0;
   for (int q = 0; q < 6; ++q)
-    if (nchunks[q]) k_d3_scan<<<1, 1024, 0, s>>>(counts + coff[q], (int)nchunks[q], offs + coff[q] + q);
+    if (nchunks[q]) {
+      jobs.c0[njobs] = coff[q];
+      jobs.o0[njobs] = coff[q] + q;
+      jobs.nb[njobs] = (int)nchunks[q];
+      jobs.q[njobs] = q;
+      ++njobs;
+    }
+  if (njobs) k_d3_scan<<<njobs, 1024, 0, s>>>(counts, jobs, offs, totals);
   RH_HIP(hipGetLastError());
-  // field totals: offs[coff[q] + q + nchunks[q]]
-  std::vector<unsigned long long> tot(6, 0);
-  for (int q = 0; q < 6; ++q)
-    if (nchunks[q])
-      RH_HIP(hipMemcpyAsync(&tot[q], offs + coff[q] + q + nchunks[q], sizeof(unsigned long long),
-                            hipMemcpyDeviceToHost, s));
+  // the six field totals, gathered by the scan, in one copy to the pinned page
+  unsigned long long* tot_h = (unsigned long long*)((char*)scr.host + 2048);
+  RH_HIP(hipMemcpyAsync(tot_h, totals, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   unsigned f = 0;
   if ((st = scr.read_flags(&f))) return st;  // synchronises the stream
+  unsigned long long tot[6];
+  for (int q = 0; q < 6; ++q) tot[q] = tot_h[q];
   if (f & kBadVelocity) return invalid_state("nonfinite velocity");
   if (f & kBadCell) return invalid_state("nonfinite or invalid cell field");
   if ((st = header_checks(view, frame, report))) return st;
@@ -366,8 +394,8 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
     set_last_error("output buffer smaller than the frame (the length needed is in *bytes)", "rh_dense3d_frame");
     return RH_BUFFER_LIMIT;
   }
-  // text pieces: one host buffer of (offset, text) copied in one transfer each
-  std::string text_blob;
+  // text pieces (offset, text), each copied in one transfer: from the pinned
+  // page when they fit there (asynchronous), else from host memory
   std::vector<std::pair<unsigned long long, std::string>> pieces;
   pieces.push_back({0, head});
   for (int q = 0; q < 6; ++q) {
@@ -375,8 +403,19 @@ extern "C" rh_status rh_dense3d_frame(const rh_grid* grid, const rh_dense3d_view
     pieces.push_back({field_at[q] + tot[q], post[q]});
   }
   pieces.push_back({at - tail.size(), tail});
-  for (auto& pc : pieces)
-    RH_HIP(hipMemcpyAsync(d_out + pc.first, pc.second.data(), pc.second.size(), hipMemcpyHostToDevice, s));
+  size_t text_len = 0;
+  for (auto& pc : pieces) text_len += pc.second.size();
+  char* stage = (char*)scr.host + 2304;  // the page's last 1792 bytes
+  const bool pinned = text_len <= 4096 - 2304;
+  for (auto& pc : pieces) {
+    const char* src = pc.second.data();
+    if (pinned) {
+      std::memcpy(stage, src, pc.second.size());
+      src = stage;
+      stage += pc.second.size();
+    }
+    RH_HIP(hipMemcpyAsync(d_out + pc.first, src, pc.second.size(), hipMemcpyHostToDevice, s));
+  }
   for (int q = 0; q < 6; ++q) {
     if (!nchunks[q]) continue;
     prof::Scope ps(prof::kDense3dWrite, s, -1);
