@@ -455,6 +455,9 @@ typedef struct rh_step_report {
 
 rh_status rh_scene_create(const rh_grid* grid, const rh_scene_config* config, rh_scene** out);
 void rh_scene_destroy(rh_scene* scene);
+/* Device bytes the scene holds: its arrays and solver, plus what its steps
+ * added (the spare faces of the out-of-place correction, the accepted
+ * pressure kept aside by warm steps). */
 uint64_t rh_scene_allocated_bytes(const rh_scene* scene);
 /* Upload/download the accepted faces (host arrays in the scene precision). */
 rh_status rh_scene_set_velocity(rh_scene* scene, const void* u, const void* v, const void* w);

@@ -858,7 +858,10 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
   // aside first so that a failed attempt does not lose it.
   const double* seed = nullptr;
   if (sc->init_policy == RH_INIT_ACCEPTED_PREVIOUS && sc->p_state != 0) {
-    if (!sc->p_acc) RH_HIP(hipMalloc(&sc->p_acc, sc->n * sizeof(double)));
+    if (!sc->p_acc) {
+      RH_HIP(hipMalloc(&sc->p_acc, sc->n * sizeof(double)));
+      sc->bytes += sc->n * sizeof(double);
+    }
     if (sc->p_state == 1) {
       RH_HIP(hipMemcpyAsync(sc->p_acc, sc->p, sc->n * sizeof(double), hipMemcpyDeviceToDevice, st));
       sc->p_state = 2;
@@ -874,19 +877,24 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
                             &rep->pressure, cancel, cancel_ctx);
   if (s) return s;
   RH_HIP(hipEventRecord(sc->ev[3], st));
+  // correct_candidate_in_place and the gates' sums in one pass, out of
+  // place into the spare faces, which then become the candidate
+  if (!sc->spare_base) {
+    const size_t es = prec_bytes(prec);
+    RH_HIP(hipMalloc(&sc->spare_base, es * (sc->flen[0] + sc->flen[1] + sc->flen[2])));
+    sc->bytes += es * (sc->flen[0] + sc->flen[1] + sc->flen[2]);
+    sc->spare[0] = sc->spare_base;
+    sc->spare[1] = (char*)sc->spare_base + es * sc->flen[0];
+    sc->spare[2] = (char*)sc->spare_base + es * (sc->flen[0] + sc->flen[1]);
+  }
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
-    {
-      rh::prof::Scope ps(rh::prof::kCorrect, st, -1);
-      rh::launch_correct<T>(g, sc->p, dt, sc->cand[0], sc->cand[1], sc->cand[2], scr.flags, st);
-    }
-    {
-      rh::prof::Scope ps(rh::prof::kDivMax, st, -1);
-      rh::launch_divergence_rate<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], scr.reduce(0), st);
-    }
+    rh::prof::Scope ps(rh::prof::kCorrect, st, -1);
+    rh::launch_correct_divergence_rate<T>(g, sc->p, dt, sc->cand, sc->spare, scr.reduce(0), st);
     return RH_OK;
   });
   if (s) return s;
+  for (int a = 0; a < 3; ++a) std::swap(sc->cand[a], sc->spare[a]);
   RH_HIP(hipEventRecord(sc->ev[4], st));
   RH_HIP(hipGetLastError());
   // one read-back: the correction's flags and [0] max|div u|, [1] rate,

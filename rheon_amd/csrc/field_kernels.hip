@@ -791,6 +791,117 @@ k_divergence_rate(DevGrid g, const T* __restrict__ u, const T* __restrict__ v, c
   block_publish<1, 4>(dd, m, red.pdd, red.pmx);
 }
 
+// correct_candidate_in_place followed by the divergence gate, the velocity
+// rate, the component maxima and the squared-face sum over the corrected
+// faces, in one pass and out of place (in -> out): cell (i, j, k) corrects
+// its three lower faces (k_correct_cells's expressions and checks; a wall
+// face is copied, a closed obstacle face becomes 0) and writes them, and
+// also evaluates its three upper faces -- by the same expressions from the
+// same inputs, so the same bits the neighbouring cell writes -- to form its
+// divergence without reading corrected faces back.  Upper wall faces belong
+// to no cell: the cell below them copies them.  The loop, the per-thread
+// accumulation order and the partials are k_divergence_rate's, so the
+// results are its results on the corrected faces, bit for bit.
+template <typename T>
+__device__ __forceinline__ double corrected_face(T x, double p_right, double p_left, double dt, double acc,
+                                                 unsigned& bad) {
+  const double difference = p_right - p_left;
+  double value;
+  if (sizeof(T) == 4) {
+    value = (double)x - dt * acc * difference;
+    if (!isfinite(value) || fabs(value) > (double)FLT_MAX) bad |= kFlagArithmetic;
+  } else {
+    const double gradient = acc * difference;
+    value = (double)x - dt * gradient;
+    if (!isfinite(value)) bad |= kFlagArithmetic;
+  }
+  return (double)(T)value;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_correct_divergence_rate(DevGrid g, const double* __restrict__ p, double dt, const T* __restrict__ iu,
+                          const T* __restrict__ iv, const T* __restrict__ iw, T* __restrict__ ou,
+                          T* __restrict__ ov, T* __restrict__ ow, Reduce red) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int sy = nx, sz = nx * ny, last = nx * ny * nz - 1;
+  const RowRange rr = block_rows(ny * nz);
+  const double a0 = g.acc[0], a1 = g.acc[1], a2 = g.acc[2];
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  DD e{0.0, 0.0};
+  unsigned bad = 0;
+  for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
+    const int k = row / ny, j = row - k * ny;
+    for (int i = threadIdx.x; i < nx; i += kLanesX) {
+      const int xu = i + (nx + 1) * row;
+      const int yv = i + nx * (j + (ny + 1) * k);
+      const int c = i + nx * row;
+      // every load first, at in-range addresses (a wall's neighbour index is
+      // clamped; its value is not used), so they are all in flight at once
+      const T fu0 = iu[xu], fu1 = iu[xu + 1], fv0 = iv[yv], fv1 = iv[yv + nx], fw0 = iw[c], fw1 = iw[c + sz];
+      const double pc = p[c];
+      const double pxl = p[max(c - 1, 0)], pyl = p[max(c - sy, 0)], pzl = p[max(c - sz, 0)];
+      const double pxh = p[min(c + 1, last)], pyh = p[min(c + sy, last)], pzh = p[min(c + sz, last)];
+      // open lower faces of this cell and of its +x/+y/+z neighbours
+      bool ox0 = true, oy0 = true, oz0 = true, ox1 = true, oy1 = true, oz1 = true;
+      if (g.mask) {
+        const unsigned mc = g.mask[c];
+        ox0 = mc & 1u;
+        oy0 = (mc >> 2) & 1u;
+        oz0 = (mc >> 4) & 1u;
+        ox1 = g.mask[min(c + 1, last)] & 1u;
+        oy1 = (g.mask[min(c + sy, last)] >> 2) & 1u;
+        oz1 = (g.mask[min(c + sz, last)] >> 4) & 1u;
+      }
+      const bool wx0 = i == 0, wy0 = j == 0, wz0 = k == 0;
+      const bool wx1 = i + 1 == nx, wy1 = j + 1 == ny, wz1 = k + 1 == nz;
+      // each interior open lower face's own check (its owner's, as in
+      // k_correct_cells); an upper face is checked by the cell above it
+      unsigned bu = 0, bv = 0, bw = 0, b = 0;
+      const double cu0 = corrected_face<T>(fu0, pc, pxl, dt, a0, bu);
+      const double cv0 = corrected_face<T>(fv0, pc, pyl, dt, a1, bv);
+      const double cw0 = corrected_face<T>(fw0, pc, pzl, dt, a2, bw);
+      if (!wx0 && ox0) bad |= bu;
+      if (!wy0 && oy0) bad |= bv;
+      if (!wz0 && oz0) bad |= bw;
+      const double u0 = wx0 ? (double)fu0 : (ox0 ? cu0 : 0.0);
+      const double v0 = wy0 ? (double)fv0 : (oy0 ? cv0 : 0.0);
+      const double w0 = wz0 ? (double)fw0 : (oz0 ? cw0 : 0.0);
+      ou[xu] = (T)u0;
+      ov[yv] = (T)v0;
+      ow[c] = (T)w0;
+      const double cu1 = corrected_face<T>(fu1, pxh, pc, dt, a0, b);
+      const double cv1 = corrected_face<T>(fv1, pyh, pc, dt, a1, b);
+      const double cw1 = corrected_face<T>(fw1, pzh, pc, dt, a2, b);
+      (void)b;
+      const double u1 = wx1 ? (double)fu1 : (ox1 ? cu1 : 0.0);
+      const double v1 = wy1 ? (double)fv1 : (oy1 ? cv1 : 0.0);
+      const double w1 = wz1 ? (double)fw1 : (oz1 ? cw1 : 0.0);
+      if (wx1) ou[xu + 1] = fu1;
+      if (wy1) ov[yv + nx] = fv1;
+      if (wz1) ow[c + sz] = fw1;
+      const double value = over_h(g, 0, u1 - u0) + over_h(g, 1, v1 - v0) + over_h(g, 2, w1 - w0);
+      if (!isfinite(value)) bad |= kFlagArithmetic;
+      m[0] = fmax(m[0], fabs(value));
+      m[1] = fmax(m[1], fabs(u0));
+      m[2] = fmax(m[2], fabs(v0));
+      m[3] = fmax(m[3], fabs(w0));
+      if (wx1) m[1] = fmax(m[1], fabs(u1));
+      if (wy1) m[2] = fmax(m[2], fabs(v1));
+      if (wz1) m[3] = fmax(m[3], fabs(w1));
+      dd_acc(e, u0 * u0);
+      dd_acc(e, v0 * v0);
+      dd_acc(e, w0 * w0);
+      if (wx1) dd_acc(e, u1 * u1);
+      if (wy1) dd_acc(e, v1 * v1);
+      if (wz1) dd_acc(e, w1 * w1);
+    }
+  }
+  if (bad) atomicOr(red.flags, bad);
+  DD dd[1] = {e};
+  block_publish<1, 4>(dd, m, red.pdd, red.pmx);
+}
+
 // ------------------------------------------------------------ host launchers
 template <typename T>
 void launch_advect(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
@@ -868,6 +979,15 @@ void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, cons
                             hipStream_t s) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
   launch(k_divergence_rate<T>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, red);
+  launch_finish<1, 4>(red.pdd, red.pmx, grid.x, FinStore{red.out, 3, g.h[0], g.h[1], g.h[2]}, s);
+}
+
+template <typename T>
+void launch_correct_divergence_rate(const DevGrid& g, const double* p, double dt, void* const in[3], void* out[3],
+                                    const Reduce& red, hipStream_t s) {
+  dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
+  launch(k_correct_divergence_rate<T>, grid, block, 0, s, g, p, dt, (const T*)in[0], (const T*)in[1],
+         (const T*)in[2], (T*)out[0], (T*)out[1], (T*)out[2], red);
   launch_finish<1, 4>(red.pdd, red.pmx, grid.x, FinStore{red.out, 3, g.h[0], g.h[1], g.h[2]}, s);
 }
 
@@ -1037,7 +1157,10 @@ void launch_b_velocity_rate(const BatchLaunch& bl, hipStream_t s) {
   template void launch_velocity_rate<T>(const DevGrid&, const void*, const void*, const void*,  \
                                         const Reduce&, hipStream_t);                           \
   template void launch_divergence_rate<T>(const DevGrid&, const void*, const void*, const void*, \
-                                          const Reduce&, hipStream_t);
+                                          const Reduce&, hipStream_t);                         \
+  template void launch_correct_divergence_rate<T>(const DevGrid&, const double*, double,      \
+                                                  void* const[3], void* [3], const Reduce&,   \
+                                                  hipStream_t);
 RH_INSTANTIATE(float)
 RH_INSTANTIATE(double)
 #undef RH_INSTANTIATE

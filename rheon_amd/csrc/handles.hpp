@@ -52,12 +52,19 @@ struct rh_scene {
   int init_policy = RH_INIT_COLD;
   int p_state = 0;
   double* p_acc = nullptr;
+  // Faces the one-scene step corrects into (out of place), then swaps with
+  // cand, so the three buffers of each component circulate among acc, cand
+  // and spare; spare_base (allocated by the first such step) is freed with
+  // the arena.
+  void* spare[3] = {nullptr, nullptr, nullptr};
+  void* spare_base = nullptr;
   ~rh_scene() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (arena) (void)hipFree(arena);
     if (tracer[0]) (void)hipFree(tracer[0]);
     if (p_acc) (void)hipFree(p_acc);
+    if (spare_base) (void)hipFree(spare_base);
   }
 };
 

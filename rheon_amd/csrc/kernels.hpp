@@ -47,6 +47,12 @@ void launch_velocity_rate(const DevGrid& g, const void* u, const void* v, const 
 template <typename T>
 void launch_divergence_rate(const DevGrid& g, const void* u, const void* v, const void* w,
                             const Reduce& red, hipStream_t s);
+// launch_correct then launch_divergence_rate in one pass, out of place: the
+// corrected faces of in[] are written to out[] (every face: walls copied),
+// flags into red.flags, the sums as launch_divergence_rate's, bit for bit.
+template <typename T>
+void launch_correct_divergence_rate(const DevGrid& g, const double* p, double dt, void* const in[3], void* out[3],
+                                    const Reduce& red, hipStream_t s);
 template <typename T>
 void launch_zero_blocked(const DevGrid& g, void* u, void* v, void* w, hipStream_t s);
 // Cells [lo, hi) of a tracer advection; the rest are written +0.0.
