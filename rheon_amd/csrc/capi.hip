@@ -842,10 +842,14 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
   });
   if (s) return s;
   RH_HIP(hipEventRecord(sc->ev[1], st));
+  const bool mg_method = sc->config.solver.method == RH_MIXED_MG;
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
     rh::prof::Scope ps(rh::prof::kRhs, st, -1);
-    rh::launch_rhs<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], dt, sc->rhs, scr.flags, st, true);
+    // a multigrid solve takes the RHS's compatibility partials from this pass
+    const rh::Reduce sums = scr.reduce(0);
+    rh::launch_rhs<T>(g, sc->cand[0], sc->cand[1], sc->cand[2], dt, sc->rhs, scr.flags, st, true,
+                      mg_method ? &sums : nullptr);
     return RH_OK;
   });
   if (s) return s;
@@ -869,6 +873,7 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
     seed = sc->p_acc;
   }
   sc->p_written = true;
+  if (mg_method) sc->solver->impl->rhs_sums_published(sc->rhs);
   if (seed)
     s = rh_solver_solve_rhs_initial(sc->solver.get(), sc->config.density, sc->rhs, seed, dt, &sc->config.pressure,
                                     sc->p, &rep->pressure, cancel, cancel_ctx);

@@ -523,15 +523,19 @@ k_validate_faces(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
 // axis order X, Y, Z, f64 from T faces.  kValidate: also b_validate_faces
 // with the wall check, each face checked by the cell that reads it as its
 // lower face (or, for the last cell along an axis, its upper wall face);
-// same flags, one pass fewer.
-template <typename T, bool kValidate = false>
+// same flags, one pass fewer.  kSums: also the solver's compatibility
+// partials of the values written (b_rhs_sums's sums in its cell order and
+// block slots, so its finisher gives the same bits), published to pdd/pmx.
+template <typename T, bool kValidate = false, bool kSums = false>
 __device__ __forceinline__ void
 b_build_rhs(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __restrict__ v,
             const T* __restrict__ w, double dt, double* __restrict__ out,
-            unsigned* __restrict__ flags) {
+            unsigned* __restrict__ flags, DD* pdd = nullptr, double* pmx = nullptr) {
   const int nx = g.nx, ny = g.ny;
   const RowRange rr = block_rows(g.ny * g.nz, vb);
   unsigned bad = 0;
+  DD sums[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  double nonfinite[1] = {0.0};
   for (int row = rr.begin + threadIdx.y; row < rr.end; row += kRowsPerBlock) {
     const int k = row / ny, j = row - k * ny;
     for (int i = threadIdx.x; i < nx; i += kLanesX) {
@@ -564,16 +568,23 @@ b_build_rhs(VBlock vb, const DevGrid& g, const T* __restrict__ u, const T* __res
       const double value = flux / dt;
       if (!isfinite(value)) bad |= kFlagArithmetic;
       out[i + nx * row] = value;
+      if (kSums) {
+        if (!isfinite(value)) nonfinite[0] = 1.0;
+        dd_acc(sums[0], value);
+        dd_acc(sums[1], fabs(value));
+        dd_acc(sums[2], value * value);
+      }
     }
   }
   if (bad) atomicOr(flags, bad);
+  if (kSums) block_publish<3, 1>(sums, nonfinite, pdd, pmx, vb);
 }
-template <typename T, bool kValidate>
+template <typename T, bool kValidate, bool kSums = false>
 __global__ void __launch_bounds__(kBlock)
 k_build_rhs(DevGrid g, const T* __restrict__ u, const T* __restrict__ v,
             const T* __restrict__ w, double dt, double* __restrict__ out,
-            unsigned* __restrict__ flags) {
-  b_build_rhs<T, kValidate>(hw_block(), g, u, v, w, dt, out, flags);
+            unsigned* __restrict__ flags, DD* pdd, double* pmx) {
+  b_build_rhs<T, kValidate, kSums>(hw_block(), g, u, v, w, dt, out, flags, pdd, pmx);
 }
 
 // correct_candidate_in_place (src/operator.rs:627-668) for f32 storage:
@@ -939,12 +950,17 @@ void launch_validate(const DevGrid& g, const void* u, const void* v, const void*
 
 template <typename T>
 void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
-                double* out, unsigned* flags, hipStream_t s, bool validate) {
+                double* out, unsigned* flags, hipStream_t s, bool validate, const Reduce* sums) {
   dim3 grid(grid_blocks(g.ny * g.nz)), block(kLanesX, kRowsPerBlock);
-  if (validate)
-    launch(k_build_rhs<T, true>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
+  if (sums)
+    launch(k_build_rhs<T, true, true>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags,
+           sums->pdd, sums->pmx);
+  else if (validate)
+    launch(k_build_rhs<T, true>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags,
+           nullptr, nullptr);
   else
-    launch(k_build_rhs<T, false>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags);
+    launch(k_build_rhs<T, false>, grid, block, 0, s, g, (const T*)u, (const T*)v, (const T*)w, dt, out, flags,
+           nullptr, nullptr);
 }
 
 template <typename T>
@@ -1149,7 +1165,7 @@ void launch_b_velocity_rate(const BatchLaunch& bl, hipStream_t s) {
   template void launch_validate<T>(const DevGrid&, const void*, const void*, const void*, int, \
                                    unsigned*, hipStream_t);                                    \
   template void launch_rhs<T>(const DevGrid&, const void*, const void*, const void*, double,   \
-                              double*, unsigned*, hipStream_t, bool);                          \
+                              double*, unsigned*, hipStream_t, bool, const Reduce*);           \
   template void launch_correct<T>(const DevGrid&, const double*, double, void*, void*, void*,  \
                                   unsigned*, hipStream_t);                                     \
   template void launch_divergence_max<T>(const DevGrid&, const void*, const void*, const void*, \

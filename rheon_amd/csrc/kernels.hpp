@@ -30,9 +30,12 @@ template <typename T>
 void launch_validate(const DevGrid& g, const void* u, const void* v, const void* w,
                      int check_walls, unsigned* flags, hipStream_t s);
 // validate: also launch_validate's checks with the wall test, in the same pass.
+// sums (implies validate): also the partials of launch_rhs_sums's kernel,
+// published to sums->pdd / sums->pmx for launch_rhs_sums_finish.
 template <typename T>
 void launch_rhs(const DevGrid& g, const void* u, const void* v, const void* w, double dt,
-                double* out, unsigned* flags, hipStream_t s, bool validate = false);
+                double* out, unsigned* flags, hipStream_t s, bool validate = false,
+                const Reduce* sums = nullptr);
 template <typename T>
 void launch_correct(const DevGrid& g, const double* p, double dt, void* u, void* v, void* w,
                     unsigned* flags, hipStream_t s);
@@ -169,6 +172,10 @@ void launch_comp_check(const DevGrid& g, const double* b, PcgControl* ctl, DD* p
 // p = seed with the seed's checks (non-finite, nonzero gauge) into ctl.
 void launch_seed(const DevGrid& g, const double* seed, double* p, PcgControl* ctl, DD* pdd, double* pmx,
                  hipStream_t s);
+// launch_rhs_sums's finisher alone, on partials already published (by
+// launch_rhs with sums) over the same grid.
+void launch_rhs_sums_finish(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
+                            hipStream_t s);
 void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
                      hipStream_t s);
 

@@ -982,6 +982,11 @@ void launch_residual_report(const DevGrid& g, const double* p, const double* b, 
   launch_finish<1, 2>(red.pdd, red.pmx, grid.x, FinStore{red.out, 2, 0.0, 0.0, 0.0}, s);
 }
 
+void launch_rhs_sums_finish(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
+                            hipStream_t s) {
+  launch_finish<3, 1>(pdd, pmx, cells_grid(g).x, FinRhsSums{ctl}, s);
+  launch_comp_check(g, b, ctl, pdd, pmx, s);
+}
 void launch_rhs_sums(const DevGrid& g, const double* b, PcgControl* ctl, DD* pdd, double* pmx,
                      hipStream_t s) {
   const dim3 grid = cells_grid(g);

@@ -74,6 +74,10 @@ class Solver {
                           rh_pressure_report* report, rh_cancel_fn cancel, void* ctx) = 0;
   virtual uint64_t allocated_bytes() const = 0;
   Scratch& scratch() { return scratch_; }
+  // The RHS at d_rhs was written by launch_rhs with sums into this
+  // solver's scratch partials; the next multigrid solve of that RHS runs
+  // only the sums' finisher (consumed by that solve either way).
+  void rhs_sums_published(const double* d_rhs) { rhs_sums_for_ = d_rhs; }
   const rh_grid& grid() const { return grid_; }
   // Solid obstacles: `solid` holds one flag per cell (nonzero = solid) on
   // the host, or is null for the plain box.  Builds the open-face mask,
@@ -120,6 +124,7 @@ class Solver {
   rh_grid grid_{};
   rh_solver_config config_{};
   Scratch scratch_;
+  const double* rhs_sums_for_ = nullptr;
   double dt_ = 0.0;
   std::vector<uint8_t> hmask_;      // host copy of the obstacle mask (empty = box)
   unsigned char* d_mask_ = nullptr;  // device obstacle mask

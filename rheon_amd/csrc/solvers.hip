@@ -999,7 +999,12 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
   RH_HIP(hipMemcpyAsync(scratch_.ctl, scratch_.host, sizeof(h), hipMemcpyHostToDevice, s));
   RH_HIP(hipEventRecord(scratch_.ev0, s));
   // compatibility + threshold, p = 0, initial true residual (it = 0)
-  launch_rhs_sums(g, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
+  const bool sums_ready = rhs_sums_for_ == d_rhs;
+  rhs_sums_for_ = nullptr;
+  if (sums_ready)
+    launch_rhs_sums_finish(g, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
+  else
+    launch_rhs_sums(g, d_rhs, scratch_.ctl, scratch_.pdd, scratch_.pmx, s);
   const bool x0_implicit = sizeof(S) == 4 && mg_first_from_b(g, h_, d_pressure, d_rhs);
   // Four-cell FP32 path: p = 0 is never read -- the first residual is b
   // and the first cycle's update writes p = 0 + x over every cell -- so p
