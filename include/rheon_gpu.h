@@ -466,7 +466,9 @@ rh_status rh_scene_get_velocity(const rh_scene* scene, void* u, void* v, void* w
  * of the accepted velocity are set to 0 now and after every set_velocity,
  * advection writes 0 on them and the correction leaves them at 0. */
 rh_status rh_scene_set_solid(rh_scene* scene, const uint8_t* solid, size_t len);
-/* Pressure of the last accepted step (N f64, host). */
+/* The scene's pressure (N f64, host): after an accepted step, that step's;
+ * after a failed attempt whose solve ran, that attempt's solve output (as
+ * the reference's, the view is meaningful only after an accepted step). */
 rh_status rh_scene_get_pressure(const rh_scene* scene, double* out, size_t len);
 /* Device pointers of the accepted faces (for zero-copy producers). */
 rh_status rh_scene_device_velocity(rh_scene* scene, void** d_u, void** d_v, void** d_w);

@@ -856,8 +856,17 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
   RH_HIP(hipEventRecord(sc->ev[2], st));
   RH_HIP(hipGetLastError());
   unsigned f = 0;
-  if ((s = scr.read_flags(&f))) return s;
-  if (f) return rh::flags_to_status(f);
+  // Faces this scene validated or committed itself advect to finite
+  // candidates with zero walls, so their check (flags[0]) is read with the
+  // step's final read-back instead of here, where it would hold the solve
+  // back by a host round trip; a failed check still decides the step's
+  // status before anything the solve reports.  Other faces are checked
+  // here, before the solve, as in the reference.
+  const bool defer = sc->acc_checked && !sc->external;
+  if (!defer) {
+    if ((s = scr.read_flags(&f))) return s;
+    if (f) return rh::flags_to_status(f);
+  }
   // AcceptedPreviousV1: seed with the last committed step's pressure, kept
   // aside first so that a failed attempt does not lose it.
   const double* seed = nullptr;
@@ -880,7 +889,13 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
   else
     s = rh_solver_solve_rhs(sc->solver.get(), sc->config.density, sc->rhs, dt, &sc->config.pressure, sc->p,
                             &rep->pressure, cancel, cancel_ctx);
-  if (s) return s;
+  if (s) {
+    if (defer && scr.read_flags(&f) == RH_OK && f) {  // the candidate's check comes first
+      std::memset(&rep->pressure, 0, sizeof(rep->pressure));
+      return rh::flags_to_status(f);
+    }
+    return s;
+  }
   RH_HIP(hipEventRecord(sc->ev[3], st));
   // correct_candidate_in_place and the gates' sums in one pass, out of
   // place into the spare faces, which then become the candidate
@@ -895,18 +910,26 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
   s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
     using T = std::remove_pointer_t<decltype(tag)>;
     rh::prof::Scope ps(rh::prof::kCorrect, st, -1);
-    rh::launch_correct_divergence_rate<T>(g, sc->p, dt, sc->cand, sc->spare, scr.reduce(0), st);
+    rh::Reduce red = scr.reduce(0);
+    red.flags = scr.flags + 1;  // the correction's own word: flags[0] may hold the deferred check
+    rh::launch_correct_divergence_rate<T>(g, sc->p, dt, sc->cand, sc->spare, red, st);
     return RH_OK;
   });
   if (s) return s;
   for (int a = 0; a < 3; ++a) std::swap(sc->cand[a], sc->spare[a]);
   RH_HIP(hipEventRecord(sc->ev[4], st));
   RH_HIP(hipGetLastError());
-  // one read-back: the correction's flags and [0] max|div u|, [1] rate,
-  // [2..4] component maxima, [5] sum of squared faces
+  // one read-back: the candidate's deferred check, the correction's flags
+  // and [0] max|div u|, [1] rate, [2..4] component maxima, [5] sum of
+  // squared faces
+  unsigned fw[2];
   double sv[6];
-  if ((s = scr.read_flags_scalars(&f, sv, 6))) return s;
-  if (f) return rh::flags_to_status(f);
+  if ((s = scr.read_flags_scalars(fw, sv, 6))) return s;
+  if (fw[0]) {
+    std::memset(&rep->pressure, 0, sizeof(rep->pressure));
+    return rh::flags_to_status(fw[0]);
+  }
+  if (fw[1]) return rh::flags_to_status(fw[1]);
   if (rh::prof::enabled()) rh::prof::settle(INT32_MAX);
   float ms[4] = {0, 0, 0, 0}, total = 0;
   for (int i = 0; i < 4; ++i) RH_HIP(hipEventElapsedTime(&ms[i], sc->ev[i], sc->ev[i + 1]));

@@ -285,7 +285,7 @@ Reduce Scratch::reduce(int out_slot) {
 }
 
 rh_status Scratch::clear_flags() {
-  RH_HIP(hipMemsetAsync(flags, 0, sizeof(unsigned), stream));
+  RH_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned), stream));
   return RH_OK;
 }
 
@@ -296,13 +296,13 @@ rh_status Scratch::read_flags(unsigned* out) {
   return RH_OK;
 }
 
-rh_status Scratch::read_flags_scalars(unsigned* f, double* out, int n) {
-  // the flag word lies below result slots 0..n-1 in one allocation
+rh_status Scratch::read_flags_scalars(unsigned f[2], double* out, int n) {
+  // the flag words lie below result slots 0..n-1 in one allocation
   char* lo = (char*)flags;
   const size_t span = (char*)(scal + n) - lo;
   RH_HIP(hipMemcpyAsync(host, lo, span, hipMemcpyDeviceToHost, stream));
   RH_HIP(hipStreamSynchronize(stream));
-  std::memcpy(f, host, sizeof(unsigned));
+  std::memcpy(f, host, 2 * sizeof(unsigned));
   std::memcpy(out, (char*)host + ((char*)scal - lo), sizeof(double) * n);
   return RH_OK;
 }

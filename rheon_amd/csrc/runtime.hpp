@@ -35,7 +35,7 @@ struct Scratch {
   DD* pdd = nullptr;
   double* pmx = nullptr;
   unsigned* counters = nullptr;
-  unsigned* flags = nullptr;
+  unsigned* flags = nullptr;  // two words; clear_flags clears both
   double* scal = nullptr;
   PcgControl* ctl = nullptr;
   void* host = nullptr;
@@ -57,8 +57,8 @@ struct Scratch {
   rh_status clear_flags();
   rh_status read_flags(unsigned* out);
   rh_status read_scalars(double* out, int n, int slot);
-  // The flag word and result slots 0..n-1 in one copy.
-  rh_status read_flags_scalars(unsigned* f, double* out, int n);
+  // Both flag words (flags[0], flags[1]) and result slots 0..n-1 in one copy.
+  rh_status read_flags_scalars(unsigned f[2], double* out, int n);
 };
 
 Scratch& global_scratch(std::unique_lock<std::mutex>& lock);

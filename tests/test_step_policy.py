@@ -171,3 +171,32 @@ def test_tracer_with_sphere_and_dense_support():
     want = O.advect_tracer(ogrid(spec), old, sc.get_velocity(), dt)
     assert bits(sc.get_tracer()) == bits(want)
     sc.close()
+
+
+@pytest.mark.gpu
+def test_rhs_failure_after_a_committed_step():
+    """A candidate whose RHS overflows (src/operator.rs:460-509: a non-finite
+    b is ArithmeticFailure) fails the step with the same status whether the
+    candidate's check is read before the solve (faces handed out through
+    device_velocity) or with the step's last read-back (faces the scene
+    committed itself), and nothing is committed either way."""
+    spec = GridSpec.unit_box(32)
+    vel = [np.ascontiguousarray(v * np.float32(1e30)) for v in taylor_green_noise(spec, seed=12)]
+    st = R.PressureSettings(1e-8, 1e-14, 1e30, 60)
+    statuses = []
+    for external in (False, True):
+        sc = R.Scene(spec, 1.0, R.F32, "mixed-mg-dc", settings=st, actual_divergence_limit=1e30, max_courant=0.0)
+        sc.set_velocity(vel)
+        first = sc.step(1e-3)
+        assert first.status == R.OK
+        before = sc.get_velocity()
+        if external:
+            R._native.lib().rh_scene_device_velocity(sc.handle, *(R._native.C.byref(R._native.C.c_void_p())
+                                                                  for _ in range(3)))
+        rep = sc.step(1e-300)
+        statuses.append(rep.status)
+        assert rep.status != R.OK
+        after = sc.get_velocity()
+        assert all(bits(a) == bits(b) for a, b in zip(after, before))
+        sc.close()
+    assert statuses[0] == statuses[1] == R.ARITHMETIC_FAILURE
