@@ -882,23 +882,11 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
     seed = sc->p_acc;
   }
   sc->p_written = true;
-  if (mg_method) sc->solver->impl->rhs_sums_published(sc->rhs);
-  if (seed)
-    s = rh_solver_solve_rhs_initial(sc->solver.get(), sc->config.density, sc->rhs, seed, dt, &sc->config.pressure,
-                                    sc->p, &rep->pressure, cancel, cancel_ctx);
-  else
-    s = rh_solver_solve_rhs(sc->solver.get(), sc->config.density, sc->rhs, dt, &sc->config.pressure, sc->p,
-                            &rep->pressure, cancel, cancel_ctx);
-  if (s) {
-    if (defer && scr.read_flags(&f) == RH_OK && f) {  // the candidate's check comes first
-      std::memset(&rep->pressure, 0, sizeof(rep->pressure));
-      return rh::flags_to_status(f);
-    }
-    return s;
-  }
-  RH_HIP(hipEventRecord(sc->ev[3], st));
   // correct_candidate_in_place and the gates' sums in one pass, out of
-  // place into the spare faces, which then become the candidate
+  // place into the spare faces (which become the candidate on success),
+  // sums into result slots 8..13 and flags into flags[1]; a multigrid solve
+  // enqueues it behind its own kernels, before its read-back, which then
+  // brings these results back too
   if (!sc->spare_base) {
     const size_t es = prec_bytes(prec);
     RH_HIP(hipMalloc(&sc->spare_base, es * (sc->flen[0] + sc->flen[1] + sc->flen[2])));
@@ -907,24 +895,54 @@ static rh_status scene_step(rh_scene* sc, double dt, rh_step_report* report, rh_
     sc->spare[1] = (char*)sc->spare_base + es * sc->flen[0];
     sc->spare[2] = (char*)sc->spare_base + es * (sc->flen[0] + sc->flen[1]);
   }
-  s = dispatch_precision(prec, [&](auto* tag) -> rh_status {
-    using T = std::remove_pointer_t<decltype(tag)>;
-    rh::prof::Scope ps(rh::prof::kCorrect, st, -1);
-    rh::Reduce red = scr.reduce(0);
-    red.flags = scr.flags + 1;  // the correction's own word: flags[0] may hold the deferred check
-    rh::launch_correct_divergence_rate<T>(g, sc->p, dt, sc->cand, sc->spare, red, st);
-    return RH_OK;
-  });
-  if (s) return s;
-  for (int a = 0; a < 3; ++a) std::swap(sc->cand[a], sc->spare[a]);
-  RH_HIP(hipEventRecord(sc->ev[4], st));
-  RH_HIP(hipGetLastError());
-  // one read-back: the candidate's deferred check, the correction's flags
-  // and [0] max|div u|, [1] rate, [2..4] component maxima, [5] sum of
-  // squared faces
+  constexpr int kSlot = 8;
+  bool corrected = false;
+  auto enqueue_correct = [&]() {
+    (void)hipEventRecord(sc->ev[3], st);
+    (void)dispatch_precision(prec, [&](auto* tag) -> rh_status {
+      using T = std::remove_pointer_t<decltype(tag)>;
+      rh::prof::Scope ps(rh::prof::kCorrect, st, -1);
+      rh::Reduce red = scr.reduce(kSlot);
+      red.flags = scr.flags + 1;  // the correction's own word: flags[0] may hold the deferred check
+      rh::launch_correct_divergence_rate<T>(g, sc->p, dt, sc->cand, sc->spare, red, st);
+      return RH_OK;
+    });
+    (void)hipEventRecord(sc->ev[4], st);
+    corrected = true;
+  };
+  if (mg_method) {
+    sc->solver->impl->rhs_sums_published(sc->rhs);
+    sc->solver->impl->set_solve_tail(enqueue_correct);
+  }
+  if (seed)
+    s = rh_solver_solve_rhs_initial(sc->solver.get(), sc->config.density, sc->rhs, seed, dt, &sc->config.pressure,
+                                    sc->p, &rep->pressure, cancel, cancel_ctx);
+  else
+    s = rh_solver_solve_rhs(sc->solver.get(), sc->config.density, sc->rhs, dt, &sc->config.pressure, sc->p,
+                            &rep->pressure, cancel, cancel_ctx);
+  sc->solver->impl->set_solve_tail(nullptr);  // a solve that returned before taking it leaves it set
+  if (s) {
+    if (defer && scr.read_flags(&f) == RH_OK && f) {  // the candidate's check comes first
+      std::memset(&rep->pressure, 0, sizeof(rep->pressure));
+      return rh::flags_to_status(f);
+    }
+    return s;
+  }
+  // the candidate's deferred check, the correction's flags and [0]
+  // max|div u|, [1] rate, [2..4] component maxima, [5] sum of squared faces:
+  // from the solve's read-back, or one read-back of their own
   unsigned fw[2];
-  double sv[6];
-  if ((s = scr.read_flags_scalars(fw, sv, 6))) return s;
+  double slots[kSlot + 6];
+  const double* sv = slots + kSlot;
+  if (corrected) {
+    RH_HIP(hipGetLastError());
+    scr.last_readback(fw, slots + kSlot, kSlot, 6);
+  } else {
+    enqueue_correct();
+    RH_HIP(hipGetLastError());
+    if ((s = scr.read_flags_scalars(fw, slots, kSlot + 6))) return s;
+  }
+  for (int a = 0; a < 3; ++a) std::swap(sc->cand[a], sc->spare[a]);
   if (fw[0]) {
     std::memset(&rep->pressure, 0, sizeof(rep->pressure));
     return rh::flags_to_status(fw[0]);

@@ -307,6 +307,12 @@ rh_status Scratch::read_flags_scalars(unsigned f[2], double* out, int n) {
   return RH_OK;
 }
 
+void Scratch::last_readback(unsigned f[2], double* out, int slot, int n) const {
+  const char* h = (const char*)host + 1024;  // where Solver::finish_report copies [flags, ctl]
+  std::memcpy(f, h, 2 * sizeof(unsigned));
+  std::memcpy(out, h + ((const char*)(scal + slot) - (const char*)flags), sizeof(double) * n);
+}
+
 rh_status Scratch::read_scalars(double* out, int n, int slot) {
   RH_HIP(hipMemcpyAsync(host, scal + slot, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   RH_HIP(hipStreamSynchronize(stream));

@@ -1,5 +1,6 @@
 // Host-side objects of rheon_amd.
 #pragma once
+#include <functional>
 #include <hip/hip_runtime.h>
 
 #include <memory>
@@ -59,6 +60,9 @@ struct Scratch {
   rh_status read_scalars(double* out, int n, int slot);
   // Both flag words (flags[0], flags[1]) and result slots 0..n-1 in one copy.
   rh_status read_flags_scalars(unsigned f[2], double* out, int n);
+  // The flag words and result slots slot..slot+n-1 as the last multigrid
+  // read-back with a tail brought them back (no device access).
+  void last_readback(unsigned f[2], double* out, int slot, int n) const;
 };
 
 Scratch& global_scratch(std::unique_lock<std::mutex>& lock);
@@ -78,6 +82,11 @@ class Solver {
   // solver's scratch partials; the next multigrid solve of that RHS runs
   // only the sums' finisher (consumed by that solve either way).
   void rhs_sums_published(const double* d_rhs) { rhs_sums_for_ = d_rhs; }
+  // Work the next multigrid solve enqueues on its stream after its own
+  // kernels and before its read-back, which then also brings back the flag
+  // words and every result slot (Scratch::last_readback); dropped unrun when
+  // the solve returns before that point.
+  void set_solve_tail(std::function<void()> f) { tail_ = std::move(f); }
   const rh_grid& grid() const { return grid_; }
   // Solid obstacles: `solid` holds one flag per cell (nonzero = solid) on
   // the host, or is null for the plain box.  Builds the open-face mask,
@@ -117,7 +126,8 @@ class Solver {
                         rh_cancel_fn cancel, void* ctx, int* enqueued);
   // Reads the control block back (and, with scal3, result slots 0..2 in
   // the same copy) and fills the report.
-  rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out, double* scal3 = nullptr);
+  rh_status finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out, double* scal3 = nullptr,
+                          bool ev1_recorded = false);
   void release_mask();
   void release_components();
   rh_status upload_components(const ObstacleComponents& comps);
@@ -125,6 +135,7 @@ class Solver {
   rh_solver_config config_{};
   Scratch scratch_;
   const double* rhs_sums_for_ = nullptr;
+  std::function<void()> tail_;
   double dt_ = 0.0;
   std::vector<uint8_t> hmask_;      // host copy of the obstacle mask (empty = box)
   unsigned char* d_mask_ = nullptr;  // device obstacle mask

@@ -3,6 +3,7 @@
 // scalar decision on the device; the host enqueues iterations in batches and
 // reads the control block back one batch behind, so the device never waits
 // for the host.
+#include <functional>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -126,19 +127,21 @@ rh_status Solver::drive_paced(int max_iterations, int poll, int ahead, Step step
   return RH_OK;
 }
 
-rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out, double* scal3) {
+rh_status Solver::finish_report(rh_pressure_report* report, bool cancelled, PcgControl* out, double* scal3,
+                                 bool ev1_recorded) {
   hipStream_t s = scratch_.stream;
   PcgControl* c = (PcgControl*)scratch_.host + 1;
-  RH_HIP(hipEventRecord(scratch_.ev1, s));
+  if (!ev1_recorded) RH_HIP(hipEventRecord(scratch_.ev1, s));
   if (scal3) {
-    // the report sums (result slots 0..2) and the control block, which
-    // follow one another in the scratch allocation, in one copy
-    char* lo = (char*)scratch_.scal;
+    // the flag words, every result slot (the report sums in 0..2) and the
+    // control block, which follow one another in the scratch allocation,
+    // in one copy (Scratch::last_readback reads the rest of it)
+    char* lo = (char*)scratch_.flags;
     const size_t span = (char*)(scratch_.ctl + 1) - lo;
     char* h = (char*)scratch_.host + 1024;
     RH_HIP(hipMemcpyAsync(h, lo, span, hipMemcpyDeviceToHost, s));
     RH_HIP(hipStreamSynchronize(s));
-    std::memcpy(scal3, h, 3 * sizeof(double));
+    std::memcpy(scal3, h + ((char*)scratch_.scal - lo), 3 * sizeof(double));
     std::memcpy(c, h + ((char*)scratch_.ctl - lo), sizeof(PcgControl));
   } else {
     RH_HIP(hipMemcpyAsync(c, scratch_.ctl, sizeof(PcgControl), hipMemcpyDeviceToHost, s));
@@ -993,6 +996,10 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
                         double* d_pressure, rh_pressure_report* report, rh_cancel_fn cancel,
                         void* ctx) {
   hipStream_t s = scratch_.stream;
+  // the caller's work to enqueue after the solve's kernels (taken now, so
+  // an early return drops it)
+  std::function<void()> tail = std::move(tail_);
+  tail_ = nullptr;
   PcgControl h;
   init_control(h, st, dt_, g.volume);
   std::memcpy(scratch_.host, &h, sizeof(h));
@@ -1049,9 +1056,11 @@ rh_status MgSolver::run(const DevGrid& g, const double* d_rhs, const rh_pressure
     launch_residual_report_q(g, d_pressure, d_rhs, scratch_.reduce(0), s);
   else
     launch_residual_report(g, d_pressure, d_rhs, scratch_.reduce(0), s);
+  RH_HIP(hipEventRecord(scratch_.ev1, s));  // the solve's time ends here, before the caller's tail
+  if (tail) tail();
   PcgControl c;
   double out[3];
-  rs = finish_report(report, cancelled, &c, out);
+  rs = finish_report(report, cancelled, &c, out, true);
   if (c.done) last_cycles_ = c.iteration;
   if (rs != RH_OK) return rs;
   if (report) {
